@@ -1,0 +1,186 @@
+/*
+ * nlf.h -- C ABI of the MI355X-native 2D nonlocal finite-element hot path.
+ *
+ * Every entry point returns an int status (NLF_OK = 0, negative = error) and never throws.  The
+ * message of the last failure on a context is available from nlf_last_error().  All device work
+ * is ordered on the context's HIP stream; "host" pointers are plain host memory owned by the
+ * caller, "dev" pointers are device memory (e.g. from torch or hipMalloc).
+ *
+ * Each function cites the reference interface it replaces (paths relative to the NonLocFEM
+ * repository, VIXER13/nonlocal).
+ */
+#ifndef NLF_H
+#define NLF_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------------------------- */
+#define NLF_OK               0
+#define NLF_ERR_ARGUMENT    -1   /* invalid argument (std::domain_error / logic_error in the reference) */
+#define NLF_ERR_HIP         -2   /* HIP runtime failure */
+#define NLF_ERR_STATE       -3   /* call out of order (e.g. assembly before neighbours) */
+#define NLF_ERR_UNSUPPORTED -4   /* combination not implemented */
+#define NLF_ERR_NUMERIC     -5   /* numerical breakdown (reported, not thrown, like computation_info) */
+
+/* ---- element and model types -------------------------------------------------------------- */
+#define NLF_ELEMENT_Q4 4         /* bilinear quad, Gauss 2x2 (vtk_elements_set.hpp:62-64) */
+#define NLF_ELEMENT_Q8 8         /* serendipity quad p=2/9, Gauss 3x3 (vtk_elements_set.hpp:65-67) */
+
+#define NLF_INFLUENCE_POLYNOMIAL  0   /* influence::polynomial / fast_polynomial (influence_functions_2d.hpp:61-92,127-148) */
+#define NLF_INFLUENCE_CONSTANT    1   /* influence::constant (influence_functions_2d.hpp:39-59) */
+#define NLF_INFLUENCE_EXPONENTIAL 2   /* influence::exponential (influence_functions_2d.hpp:94-125) */
+
+#define NLF_EXPAND_NO   0            /* diameter_expanding_strategy (find_neighbours.hpp:7) */
+#define NLF_EXPAND_MAX  1
+#define NLF_EXPAND_MIN  2
+#define NLF_EXPAND_MEAN 3
+
+#define NLF_PART_LOCAL 0             /* assemble_part (matrix_assembler_2d.hpp:15-19) */
+#define NLF_PART_FULL  2
+
+#define NLF_CONDUCTIVITY_ISOTROPIC   0   /* thermal_parameters_2d.hpp:14-17 */
+#define NLF_CONDUCTIVITY_ORTHOTROPIC 1
+#define NLF_CONDUCTIVITY_ANISOTROPIC 2
+
+#define NLF_BC_NONE        0         /* insulated: no condition */
+#define NLF_BC_FIRST_KIND  1         /* temperature_2d / displacement */
+#define NLF_BC_FLUX        2         /* flux_2d / pressure_2d (second kind; positive = heat entering) */
+#define NLF_BC_CONVECTION  3         /* convection_2d: h, T_ambient (convection_condition_2d.hpp:11-31) */
+
+#define NLF_PRECOND_IDENTITY 0       /* eigen_identity_preconditioner */
+#define NLF_PRECOND_JACOBI   1       /* eigen_diagonal_preconditioner */
+#define NLF_PRECOND_ILLT     2       /* eigen_ILLT_preconditioner: here IC(0), scaled, shift retries */
+#define NLF_PRECOND_LEARNED  3       /* message-passing network, flexible CG only */
+
+#define NLF_MODE_FP64  0             /* plain FP64 CG */
+#define NLF_MODE_MIXED 1             /* FP32 inner CG + FP64 iterative refinement */
+
+/* Nonlocal model of one element group: model_parameters (equation_parameters.hpp:18-24) plus the
+ * influence description read by read_influence_2d (config/read_model.hpp:144-165). */
+typedef struct nlf_model {
+    double  local_weight;      /* p1; LOCAL theory when >= 0.999 (nonlocal_constants.hpp:21-27) */
+    double  radius[2];         /* nonlocal radius of the influence function */
+    double  search_radius[2];  /* neighbour-search radius before expansion; 0 => use radius */
+    int32_t influence;         /* NLF_INFLUENCE_* */
+    int32_t distance_n;        /* Lp exponent of the distance; only 2 is implemented */
+    double  p, q;              /* influence parameters (polynomial: p, q; exponential: p, q) */
+} nlf_model;
+
+typedef struct nlf_thermal_material {
+    int32_t kind;              /* NLF_CONDUCTIVITY_* */
+    int32_t pad_;
+    double  c[3];              /* iso: c[0]; ortho: {l11, l22}; aniso: {l11, l22, l12} */
+} nlf_thermal_material;
+
+typedef struct nlf_elastic_material {
+    int32_t kind;              /* 0 = isotropic plane stress (elastic_parameters.hpp:62-76) */
+    int32_t pad_;
+    double  young, poisson;
+} nlf_elastic_material;
+
+/* One boundary-group condition; for elasticity `value` holds the two components. */
+typedef struct nlf_boundary {
+    int32_t kind[2];           /* NLF_BC_* per degree of freedom (thermal uses kind[0]) */
+    double  value[2];          /* temperature / flux / displacement / traction */
+    double  heat_transfer;     /* convection h */
+    double  ambient;           /* convection T_ambient */
+} nlf_boundary;
+
+typedef struct nlf_solve_stats {
+    int64_t iterations;        /* total Krylov iterations (inner ones in mixed mode) */
+    int64_t outer_iterations;  /* refinement steps (mixed mode) */
+    double  residual;          /* reference criterion sqrt(r'M^-1 r)/||b|| of the last (inner) solve */
+    double  true_residual;     /* FP64 ||b - A x|| / ||b|| */
+    int32_t preconditioner;    /* preconditioner actually used after fallbacks */
+    int32_t fallbacks;         /* number of switches away from the requested preconditioner */
+    double  seconds;           /* wall time of the solve on the stream */
+    double  spmv_ms;           /* average SpMV kernel time (HIP events on the stream, sampled launches) */
+    double  precond_ms;        /* average preconditioner-apply time (sampled) */
+    int64_t spmv_samples;      /* number of sampled SpMV launches */
+    int64_t spmv_bytes;        /* bytes the SpMV kernel moves per launch (values + indices + vectors) */
+} nlf_solve_stats;
+
+typedef struct nlf_ctx_s*     nlf_ctx;
+typedef struct nlf_mesh_s*    nlf_mesh;
+typedef struct nlf_matrix_s*  nlf_matrix;
+typedef struct nlf_precond_s* nlf_precond;
+typedef struct nlf_batch_s*   nlf_batch;
+
+/* ---- context ---------------------------------------------------------------------------------
+ * No reference counterpart (the reference is CPU-only); one context per host thread, `stream` may
+ * be NULL (the library creates a non-blocking stream) or an existing hipStream_t. */
+int         nlf_ctx_create(int device, void* stream, nlf_ctx* out);
+int         nlf_ctx_destroy(nlf_ctx ctx);
+const char* nlf_last_error(nlf_ctx ctx);
+int         nlf_ctx_synchronize(nlf_ctx ctx);
+void*       nlf_ctx_stream(nlf_ctx ctx);
+
+/* ---- mesh: replaces mesh_2d<T,I>(path) + mesh_2d::init (libraries/mesh/mesh_2d/mesh_2d.hpp:83-108).
+ * Node ids 0..N-1, elements E x nn (nn = 4 or 8, reference local order), contiguous element groups
+ * group_begin[G+1], boundary edges n_bedges x nb grouped by bgroup_begin[GB+1]. */
+int nlf_mesh_create(nlf_ctx ctx, int64_t n_nodes, const double* xy, int64_t n_elems, int nn, const int32_t* elems,
+                    int n_groups, const int64_t* group_begin, int64_t n_bedges, int nb, const int32_t* bedges,
+                    int n_bgroups, const int64_t* bgroup_begin, nlf_mesh* out);
+int nlf_mesh_destroy(nlf_mesh m);
+int nlf_mesh_info(nlf_mesh m, int64_t* n_nodes, int64_t* n_elems, int* nn, int* nq);
+/* quad coords [E][nq][2], |det J| [E][nq], det J * grad N [E][nn][nq][2], centres [E][2]
+ * (mesh_2d::quad_coord / jacobian / derivatives, mesh_2d.hpp:141-179; centres mesh_container_2d.hpp:130-139) */
+int nlf_mesh_export_quadrature(nlf_mesh m, double* xq, double* jac, double* D, double* centres);
+
+/* ---- neighbours: replaces find_neighbours (libraries/mesh/mesh_2d/find_neighbours.hpp:119-160) and
+ * mesh_2d::neighbours (mesh_2d.hpp:205-212).  search_radii[2G]; <= 0 skips the group. */
+int nlf_find_neighbours(nlf_mesh m, const double* search_radii, int strategy, double* expanded_out);
+int nlf_neighbours_count(nlf_mesh m, int64_t* total);
+int nlf_neighbours_export(nlf_mesh m, int64_t* offsets, int32_t* ids);
+int nlf_neighbours_device(nlf_mesh m, const int64_t** offsets, const int32_t** ids);
+
+/* ---- assembly: replaces conductivity_matrix_2d::compute (thermal/conductivity_matrix_2d.hpp:166-193)
+ * and stiffness_matrix::compute (mechanical/stiffness_matrix_2d.hpp:171-199); physics 0 / 1,
+ * materials = nlf_thermal_material[G] or nlf_elastic_material[G], is_inner[DoF*N] host mask. */
+int nlf_assemble(nlf_mesh m, int physics, const nlf_model* models, const void* materials, const uint8_t* is_inner,
+                 int symmetric, int part, nlf_matrix* out);
+int nlf_matrix_destroy(nlf_matrix a);
+int nlf_matrix_info(nlf_matrix a, int64_t* rows, int64_t* nnz_upper, int64_t* nnz_full, int64_t* nnz_bound);
+int nlf_matrix_timings(nlf_matrix a, double* pattern_seconds, double* values_seconds);
+/* finite_element_matrix::inner()/bound() export (base/finite_element_matrix.hpp:12-21), int64 indices:
+ * part 0 = inner upper CSR, 1 = bound CSR, 2 = inner full storage */
+int nlf_matrix_export(nlf_matrix a, int part, int64_t* outer, int64_t* inner, double* values);
+int nlf_matrix_device(nlf_matrix a, const int64_t** outer, const int32_t** cols, double** values, const int64_t** diag);
+int nlf_matrix_add_pairs(nlf_matrix a, int64_t n, const int64_t* rc, const double* v);
+/* convection_condition_2d (thermal/convection_condition_2d.hpp:11-31), bcs[GB] */
+int nlf_matrix_add_convection(nlf_matrix a, const nlf_boundary* bcs);
+int nlf_matrix_make_fp32(nlf_matrix a);
+/* inner_nodes (base/solvers_utils.hpp:31-40), boundary_condition_second_kind_2d
+ * (base/boundary_condition_second_kind_2d.hpp:12-33) and the first-kind values (:17-26), host arrays */
+int nlf_boundary_data(nlf_mesh m, int dof, const nlf_boundary* bcs, double* f, double* xD, uint8_t* is_inner);
+/* boundary_condition_first_kind_2d (base/boundary_condition_first_kind_2d.hpp:50-59): f -= K_bound xD; f[D] = xD */
+int nlf_apply_first_kind(nlf_matrix a, const double* xD_dev, double* f_dev);
+
+/* ---- SpMV: replaces independent_symmetric_matrix_vector_product::matrix_vector_product
+ * (slae/independent_symmetric_matrix_vector_product.hpp:55-79); device vectors */
+int nlf_spmv(nlf_matrix a, const double* x_dev, double* y_dev);
+int nlf_spmv_fp32(nlf_matrix a, const float* x_dev, float* y_dev);
+int nlf_spmv_lpr(nlf_matrix a, int lpr, int fp32, const void* x_dev, void* y_dev);
+
+/* ---- preconditioners: replace preconditioner_base (slae/preconditioner_base.hpp:7-15) and
+ * eigen_{identity,diagonal,ILLT}_preconditioner (slae/eigen_preconditioner.hpp:50-57) */
+int nlf_precond_create(nlf_matrix a, int kind, nlf_precond* out);
+int nlf_precond_destroy(nlf_precond p);
+int nlf_precond_info(nlf_precond p, int* ok, double* shift, int* fw_levels, int* bw_levels, double* setup_seconds);
+int nlf_precond_apply(nlf_precond p, const double* r_dev, double* z_dev);
+int nlf_precond_export_factor(nlf_precond p, double* values, double* scale);
+
+/* ---- CG: replaces conjugate_gradient::solve (slae/conjugate_gradient.hpp:48-79); device vectors,
+ * x holds the initial guess */
+int nlf_cg_solve(nlf_matrix a, nlf_precond p, const double* b_dev, double* x_dev, double tol, int64_t max_it, int mode,
+                 nlf_solve_stats* stats);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* NLF_H */

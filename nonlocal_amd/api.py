@@ -1,0 +1,412 @@
+"""Python mirror of NonLocFEM's 2D solver/assembler API over the C ABI (include/nlf.h).
+
+Names follow the reference: mesh_2d (libraries/mesh/mesh_2d/mesh_2d.hpp:18-81), find_neighbours
+(find_neighbours.hpp:119-160), conductivity_matrix_2d (thermal/conductivity_matrix_2d.hpp:11-33),
+stiffness_matrix (mechanical/stiffness_matrix_2d.hpp:10-38), conjugate_gradient
+(slae/conjugate_gradient.hpp:15-42), stationary_heat_equation_solver_2d
+(thermal/stationary_heat_equation_solver_2d.hpp:46-120) and equilibrium_equation
+(mechanical/equilibrium_equation_2d.hpp:20-62).  Device memory and the stream come from torch; all
+compute runs in libnlf.so.  Errors raise NLFError (the reference throws std::domain_error).
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass, field
+
+import numpy as np
+import torch
+
+from . import _abi
+from ._abi import (NLF_EXPAND_MAX, NLF_MODE_FP64, NLF_PART_FULL, NLF_PART_LOCAL, NLF_PRECOND_IDENTITY,
+                   NLF_PRECOND_ILLT, NLF_PRECOND_JACOBI)
+from ._lib import check, load, NLFError  # noqa: F401
+
+_i64p = C.POINTER(C.c_int64)
+_i32p = C.POINTER(C.c_int32)
+_f64p = C.POINTER(C.c_double)
+_u8p = C.POINTER(C.c_uint8)
+
+
+def _ptr(a, t):
+    return a.ctypes.data_as(t)
+
+
+def _dev(t: torch.Tensor):
+    assert t.is_cuda and t.is_contiguous()
+    return C.c_void_p(t.data_ptr())
+
+
+class Context:
+    """One library context bound to torch's current stream on `device`."""
+
+    def __init__(self, device: int = 0):
+        if not torch.cuda.is_available():
+            raise RuntimeError("no GPU available: the nonlocal hot path runs on the GPU only")
+        L = load()
+        self.device = torch.device("cuda", device)
+        torch.cuda.set_device(self.device)
+        # One ordered stream for torch's allocations/copies and the library's kernels: torch's
+        # default stream is the legacy null stream (handle 0), which the C ABI reads as "create a
+        # private stream", so a dedicated default-priority stream is made current for torch too.
+        self.stream = torch.cuda.Stream(self.device)
+        torch.cuda.set_stream(self.stream)
+        h = C.c_void_p()
+        check(L.nlf_ctx_create(device, C.c_void_p(self.stream.cuda_stream), C.byref(h)))
+        self.h = h
+
+    def check(self, code):
+        check(code, self.h)
+
+    def synchronize(self):
+        self.check(load().nlf_ctx_synchronize(self.h))
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            load().nlf_ctx_destroy(self.h)
+            self.h = None
+
+
+_DEFAULT_CTX = None
+
+
+def default_context() -> Context:
+    global _DEFAULT_CTX
+    if _DEFAULT_CTX is None:
+        _DEFAULT_CTX = Context(0)
+    return _DEFAULT_CTX
+
+
+class mesh_2d:
+    """Device-resident mesh with the mesh_2d::init precompute (quad coords, |J|, detJ * grad N)."""
+
+    def __init__(self, mesh, ctx: Context | None = None):
+        self.ctx = ctx or default_context()
+        self.mesh = mesh
+        xy = np.ascontiguousarray(mesh.xy, dtype=np.float64)
+        el = np.ascontiguousarray(mesh.elems, dtype=np.int32)
+        gb = np.ascontiguousarray(mesh.group_begin, dtype=np.int64)
+        be = np.ascontiguousarray(mesh.bedges, dtype=np.int32)
+        bb = np.ascontiguousarray(mesh.bgroup_begin, dtype=np.int64)
+        h = C.c_void_p()
+        self.ctx.check(load().nlf_mesh_create(self.ctx.h, mesh.n_nodes, _ptr(xy, _f64p), mesh.n_elems, mesh.nn,
+                                              _ptr(el, _i32p), len(gb) - 1, _ptr(gb, _i64p), be.shape[0], be.shape[1],
+                                              _ptr(be, _i32p), len(bb) - 1, _ptr(bb, _i64p), C.byref(h)))
+        self.h = h
+        self.influences = None
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            load().nlf_mesh_destroy(self.h)
+            self.h = None
+
+    @property
+    def nodes_count(self):
+        return self.mesh.n_nodes
+
+    @property
+    def elements_count(self):
+        return self.mesh.n_elems
+
+    def quadrature(self):
+        nq = 4 if self.mesh.nn == 4 else 9
+        E, nn = self.mesh.n_elems, self.mesh.nn
+        xq = np.zeros((E, nq, 2))
+        jac = np.zeros((E, nq))
+        D = np.zeros((E, nn, nq, 2))
+        c = np.zeros((E, 2))
+        self.ctx.check(load().nlf_mesh_export_quadrature(self.h, _ptr(xq, _f64p), _ptr(jac, _f64p), _ptr(D, _f64p),
+                                                         _ptr(c, _f64p)))
+        return xq, jac, D, c
+
+    def neighbours(self):
+        """Host copy of the neighbour lists as CSR (offsets int64[E+1], ids int32)."""
+        total = C.c_int64()
+        self.ctx.check(load().nlf_neighbours_count(self.h, C.byref(total)))
+        off = np.zeros(self.mesh.n_elems + 1, dtype=np.int64)
+        ids = np.zeros(max(total.value, 1), dtype=np.int32)
+        self.ctx.check(load().nlf_neighbours_export(self.h, _ptr(off, _i64p), _ptr(ids, _i32p)))
+        return off, ids[:total.value]
+
+
+def find_neighbours(mesh: mesh_2d, search_radii, strategy=NLF_EXPAND_MAX):
+    """find_neighbours + mesh_2d::neighbours: search_radii (G, 2); returns the expanded radii."""
+    r = np.ascontiguousarray(np.asarray(search_radii, dtype=np.float64).reshape(-1, 2))
+    expanded = np.zeros_like(r)
+    mesh.ctx.check(load().nlf_find_neighbours(mesh.h, _ptr(r, _f64p), strategy, _ptr(expanded, _f64p)))
+    mesh.influences = expanded
+    return expanded
+
+
+@dataclass
+class CSRHost:
+    outer: np.ndarray
+    inner: np.ndarray
+    val: np.ndarray
+
+
+class finite_element_matrix:
+    """Device matrix handle: inner (upper view of the full storage) and bound parts."""
+
+    def __init__(self, h, ctx: Context, dof: int):
+        self.h = h
+        self.ctx = ctx
+        self.dof = dof
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            load().nlf_matrix_destroy(self.h)
+            self.h = None
+
+    def info(self):
+        rows, up, full, bound = C.c_int64(), C.c_int64(), C.c_int64(), C.c_int64()
+        self.ctx.check(load().nlf_matrix_info(self.h, C.byref(rows), C.byref(up), C.byref(full), C.byref(bound)))
+        return dict(rows=rows.value, nnz_upper=up.value, nnz_full=full.value, nnz_bound=bound.value)
+
+    def timings(self):
+        a, b = C.c_double(), C.c_double()
+        self.ctx.check(load().nlf_matrix_timings(self.h, C.byref(a), C.byref(b)))
+        return a.value, b.value
+
+    def export(self, part=0) -> CSRHost:
+        """part 0 = inner (upper CSR, the reference's layout), 1 = bound, 2 = inner full storage."""
+        i = self.info()
+        nnz = {0: i["nnz_upper"], 1: i["nnz_bound"], 2: i["nnz_full"]}[part]
+        outer = np.zeros(i["rows"] + 1, dtype=np.int64)
+        inner = np.zeros(max(nnz, 1), dtype=np.int64)
+        val = np.zeros(max(nnz, 1))
+        self.ctx.check(load().nlf_matrix_export(self.h, part, _ptr(outer, _i64p), _ptr(inner, _i64p), _ptr(val, _f64p)))
+        return CSRHost(outer, inner[:nnz], val[:nnz])
+
+    def inner(self):
+        return self.export(0)
+
+    def bound(self):
+        return self.export(1)
+
+    def spmv(self, x: torch.Tensor, y: torch.Tensor | None = None):
+        y = torch.empty_like(x) if y is None else y
+        if x.dtype == torch.float64:
+            self.ctx.check(load().nlf_spmv(self.h, _dev(x), _dev(y)))
+        else:
+            self.ctx.check(load().nlf_spmv_fp32(self.h, _dev(x), _dev(y)))
+        return y
+
+    def make_fp32(self):
+        self.ctx.check(load().nlf_matrix_make_fp32(self.h))
+
+    def add_convection(self, bcs):
+        arr = _abi.array_of(_abi.Boundary, bcs)
+        self.ctx.check(load().nlf_matrix_add_convection(self.h, arr))
+
+    def apply_first_kind(self, xD: torch.Tensor, f: torch.Tensor):
+        self.ctx.check(load().nlf_apply_first_kind(self.h, _dev(xD), _dev(f)))
+
+
+def _assemble(mesh: mesh_2d, physics, models, materials, is_inner, symmetric=True, part=NLF_PART_FULL):
+    marr = _abi.array_of(_abi.Model, models)
+    mat_t = _abi.ThermalMaterial if physics == 0 else _abi.ElasticMaterial
+    matarr = _abi.array_of(mat_t, materials)
+    inner = np.ascontiguousarray(is_inner, dtype=np.uint8)
+    h = C.c_void_p()
+    mesh.ctx.check(load().nlf_assemble(mesh.h, physics, marr, C.cast(matarr, C.c_void_p), _ptr(inner, _u8p),
+                                       int(symmetric), part, C.byref(h)))
+    return finite_element_matrix(h, mesh.ctx, 1 if physics == 0 else 2)
+
+
+class conductivity_matrix_2d:
+    """Thermal conductivity matrix assembler (conductivity_matrix_2d<T, I, J>)."""
+
+    def __init__(self, mesh: mesh_2d):
+        self.mesh = mesh
+        self._matrix = None
+
+    def compute(self, models, materials, is_inner, is_symmetric=True, is_neumann=False, part=NLF_PART_FULL):
+        if is_neumann:
+            raise NLFError(-4, "Neumann (Lagrange-multiplier) problems are not implemented")
+        self._matrix = _assemble(self.mesh, 0, models, materials, is_inner, is_symmetric, part)
+        return self._matrix
+
+    def matrix(self) -> finite_element_matrix:
+        return self._matrix
+
+
+class stiffness_matrix:
+    """Plane-elasticity stiffness assembler (stiffness_matrix<T, I, J>), 2 DoF per node."""
+
+    def __init__(self, mesh: mesh_2d):
+        self.mesh = mesh
+        self._matrix = None
+
+    def compute(self, models, materials, is_inner, is_symmetric=True, part=NLF_PART_FULL):
+        self._matrix = _assemble(self.mesh, 1, models, materials, is_inner, is_symmetric, part)
+        return self._matrix
+
+    def matrix(self) -> finite_element_matrix:
+        return self._matrix
+
+
+class preconditioner:
+    """preconditioner_base implementation on the device: IDENTITY, JACOBI or ILLT (IC(0))."""
+
+    def __init__(self, matrix: finite_element_matrix | None, kind=NLF_PRECOND_IDENTITY, ctx=None):
+        self.ctx = matrix.ctx if matrix is not None else (ctx or default_context())
+        h = C.c_void_p()
+        self.ctx.check(load().nlf_precond_create(matrix.h if matrix is not None else None, kind, C.byref(h)))
+        self.h = h
+        self.kind = kind
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            load().nlf_precond_destroy(self.h)
+            self.h = None
+
+    def info(self):
+        ok, shift, fw, bw, sec = C.c_int(), C.c_double(), C.c_int(), C.c_int(), C.c_double()
+        self.ctx.check(load().nlf_precond_info(self.h, C.byref(ok), C.byref(shift), C.byref(fw), C.byref(bw),
+                                               C.byref(sec)))
+        return dict(ok=bool(ok.value), shift=shift.value, fw_levels=fw.value, bw_levels=bw.value, seconds=sec.value)
+
+    def computation_info(self):
+        return self.info()["ok"]
+
+    def solve(self, r: torch.Tensor):
+        z = torch.empty_like(r)
+        self.ctx.check(load().nlf_precond_apply(self.h, _dev(r), _dev(z)))
+        return z
+
+
+class conjugate_gradient:
+    """conjugate_gradient<T, I>: holds a non-owning reference to the matrix, owns its preconditioner."""
+
+    def __init__(self, matrix: finite_element_matrix):
+        self._matrix = matrix
+        self._precond = None
+        self._tol = 1e-15
+        self._max_it = 10000
+        self.stats = _abi.SolveStats()
+
+    def init_preconditioner(self, kind, matrix: finite_element_matrix | None = None):
+        self._precond = preconditioner(matrix if matrix is not None else
+                                       (self._matrix if kind == NLF_PRECOND_JACOBI else None), kind,
+                                       ctx=self._matrix.ctx)
+        return self._precond
+
+    def preconditioner(self):
+        return self._precond
+
+    def set_tolerance(self, tol):
+        self._tol = float(tol)
+
+    def set_max_interations(self, it):   # reference spelling (iterative_solver_base.hpp:49-51)
+        self._max_it = int(it)
+
+    set_max_iterations = set_max_interations
+
+    def tolerance(self):
+        return self._tol
+
+    def max_iterations(self):
+        return self._max_it
+
+    def solve(self, b: torch.Tensor, x0: torch.Tensor | None = None, mode=NLF_MODE_FP64):
+        x = torch.zeros_like(b) if x0 is None else x0.clone()
+        st = _abi.SolveStats()
+        self._matrix.ctx.check(load().nlf_cg_solve(self._matrix.h, self._precond.h if self._precond else None, _dev(b),
+                                                   _dev(x), self._tol, self._max_it, mode, C.byref(st)))
+        self.stats = st
+        return x
+
+    def iterations(self):
+        return self.stats.iterations
+
+    def residual(self):
+        return self.stats.residual
+
+
+@dataclass
+class Solution:
+    x: torch.Tensor
+    iterations: int
+    residual: float
+    true_residual: float
+    matrix: finite_element_matrix = None
+    local_matrix: finite_element_matrix = None
+    rhs: torch.Tensor = None
+    timings: dict = field(default_factory=dict)
+    stats: object = None
+
+
+def _radii(models):
+    out = []
+    for m in models:
+        if m.local_weight >= 0.999:
+            out.append([0.0, 0.0])
+        else:
+            s0 = m.search_radius[0] or m.radius[0]
+            s1 = m.search_radius[1] or m.radius[1]
+            out.append([s0, s1])
+    return out
+
+
+def _solve_driver(mesh: mesh_2d, physics, models, materials, bcs, precond, tol, max_it, mode, local_for_illt=True):
+    import time
+    ctx = mesh.ctx
+    dof = 1 if physics == 0 else 2
+    tm = {}
+    t0 = time.perf_counter()
+    if any(m.local_weight < 0.999 for m in models):
+        find_neighbours(mesh, _radii(models))
+    tm["neighbours"] = time.perf_counter() - t0
+    rows = dof * mesh.nodes_count
+    f = np.zeros(rows)
+    xD = np.zeros(rows)
+    inner = np.ones(rows, dtype=np.uint8)
+    arr = _abi.array_of(_abi.Boundary, bcs)
+    ctx.check(load().nlf_boundary_data(mesh.h, dof, arr, _ptr(f, _f64p), _ptr(xD, _f64p), _ptr(inner, _u8p)))
+    t1 = time.perf_counter()
+    K = _assemble(mesh, physics, models, materials, inner)
+    if physics == 0:
+        K.add_convection(bcs)
+    fd = torch.from_numpy(f).to(ctx.device)
+    xDd = torch.from_numpy(xD).to(ctx.device)
+    K.apply_first_kind(xDd, fd)
+    ctx.synchronize()
+    tm["assembly"] = time.perf_counter() - t1
+    t2 = time.perf_counter()
+    KL = None
+    cg = conjugate_gradient(K)
+    cg.set_tolerance(tol)
+    cg.set_max_interations(max_it)
+    if precond == NLF_PRECOND_ILLT:
+        KL = _assemble(mesh, physics, models, materials, inner, part=NLF_PART_LOCAL)
+        P = cg.init_preconditioner(NLF_PRECOND_ILLT, KL)
+        if not P.computation_info():          # fall back to identity like the reference
+            cg.init_preconditioner(NLF_PRECOND_IDENTITY)
+    else:
+        cg.init_preconditioner(precond)
+    if mode != NLF_MODE_FP64:
+        K.make_fp32()
+    ctx.synchronize()
+    tm["preconditioner"] = time.perf_counter() - t2
+    t3 = time.perf_counter()
+    x = cg.solve(fd, mode=mode)
+    ctx.synchronize()
+    tm["solve"] = time.perf_counter() - t3
+    st = cg.stats
+    return Solution(x=x, iterations=st.iterations, residual=st.residual, true_residual=st.true_residual, matrix=K,
+                    local_matrix=KL, rhs=fd, timings=tm, stats=st)
+
+
+def stationary_heat_equation_solver_2d(mesh: mesh_2d, models, materials, bcs, precond=NLF_PRECOND_ILLT, tol=1e-12,
+                                       max_it=100000, mode=NLF_MODE_FP64) -> Solution:
+    """K T = f with K = p1 K_L + (1 - p1) K_NL (solved directly; the reference's correction form
+    T = T0 - K^-1 (K T0 - f) with T0 = 0 is the same system)."""
+    return _solve_driver(mesh, 0, models, materials, bcs, precond, tol, max_it, mode)
+
+
+def equilibrium_equation(mesh: mesh_2d, models, materials, bcs, precond=None, tol=1e-12, max_it=100000,
+                         mode=NLF_MODE_FP64) -> Solution:
+    """Plane-stress static elasticity; ILLT of the local stiffness when nonlocal, identity otherwise."""
+    if precond is None:
+        precond = NLF_PRECOND_ILLT if any(m.local_weight < 0.999 for m in models) else NLF_PRECOND_IDENTITY
+    return _solve_driver(mesh, 1, models, materials, bcs, precond, tol, max_it, mode)

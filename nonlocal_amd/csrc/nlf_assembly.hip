@@ -1,0 +1,931 @@
+// CSR sparsity pattern and FP64 element-pair assembly of the nonlocal conductivity / stiffness
+// matrices (matrix_assembler_2d.hpp:37-189, integrator.hpp:36-91, conductivity_matrix_2d.hpp:106-193,
+// stiffness_matrix_2d.hpp:44-199).
+//
+// Storage: the inner part is kept on the device as the FULL (both-triangle) row pattern, columns
+// ascending; the reference's upper-triangular CSR (matrix_separator_base.hpp:50-58) is the tail of
+// each row starting at its diagonal.  The bound part (inner rows x Dirichlet columns, and Dirichlet
+// rows x other Dirichlet columns) is stored as in the reference.
+//
+// Traversal: like mesh_run (mesh_2d_utils.hpp:13-37) every node owns its DoF rows; one workgroup
+// handles one node, visiting its elements eL and their neighbours eNL.  Values are accumulated in
+// LDS in barrier-separated rounds (element slot, neighbour batch, local column index j): within a
+// round two contributions never share a position because e -> node(e, j) is injective, so the sum
+// order is fixed and the result is bitwise reproducible.  No atomics touch the matrix.
+#include "nlf_internal.hpp"
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <limits>
+#include <map>
+
+namespace nlf {
+
+InfluenceParams make_influence(const nlf_model& md) {
+    InfluenceParams f;
+    f.kind = md.influence;
+    f.r[0] = md.radius[0]; f.r[1] = md.radius[1];
+    f.inv_r[0] = 1.0 / f.r[0]; f.inv_r[1] = 1.0 / f.r[1];
+    const double n = 2.0, p = md.p, q = md.q;
+    auto integral = [](double v) { return v >= 0 && std::floor(v) == v && v < 1e9; };
+    if (f.kind == NLF_INFLUENCE_CONSTANT) {              // influence_functions_2d.hpp:44-48
+        f.norm = std::tgamma(1 + 2 / n) / (4 * f.r[0] * f.r[1] * std::pow(std::tgamma(1 + 1 / n), 2));
+    } else if (f.kind == NLF_INFLUENCE_POLYNOMIAL) {
+        if (p == 2.0 && q == 1.0) {                      // fast_polynomial (read_model.hpp:160-161)
+            f.fast = 1;
+            f.norm = 2.0 / (M_PI * f.r[0] * f.r[1]);
+        } else {                                         // polynomial :70-75, exponents :215-216
+            if (integral(p) && long(p) % 2 == 0) { f.p_int = 1; f.p_i = int(p / 2); } else { f.p_int = 0; f.p_d = p / n; }
+            if (integral(q)) { f.q_int = 1; f.q_i = int(q); } else { f.q_int = 0; f.q_d = q; }
+            f.norm = n * p / (4 * f.r[0] * f.r[1] * std::beta(1 / n, 1 / n) * std::beta(2 / p, q + 1));
+        }
+    } else if (f.kind == NLF_INFLUENCE_EXPONENTIAL) {    // :234-251
+        if (integral(p) && long(p) % 2 == 0) { f.p_int = 1; f.p_i = int(p / 2); } else { f.p_int = 0; f.p_d = p / n; }
+        f.exp_q = -q;
+        f.norm = (n * p * std::pow(4.0, 1 / n) * std::pow(q, 2 / p)) /
+                 (8 * f.r[0] * f.r[1] * std::tgamma(2 / p) * std::beta(0.5, 1 / n));
+    } else {
+        throw Error(NLF_ERR_ARGUMENT, "unknown influence function");
+    }
+    return f;
+}
+
+__device__ __forceinline__ double ipow(double x, int e) {     // power.hpp:132-147
+    double r = 1.0;
+    while (e > 0) {
+        if (e & 1) { r *= x; --e; } else { x *= x; e >>= 1; }
+    }
+    return r;
+}
+
+__device__ __forceinline__ double exp_pow(double v, int is_int, int ei, double ed) {
+    if (!is_int) return pow(v, ed);
+    return ei == 1 ? v : ipow(v, ei);
+}
+
+// phi(x, y): normalised influence of the nonlocal radius (influence_functions_2d.hpp:39-148).
+__device__ __forceinline__ double influence(const InfluenceParams& f, double ax, double ay, double bx, double by) {
+    const double dx = ax - bx, dy = ay - by;
+    const double zx = dx * f.inv_r[0], zy = dy * f.inv_r[1];
+    double d = zx * zx + zy * zy;
+    if (fabs(d - 1.0) < 1e-12) {      // near the cutoff: the reference's division form decides
+        const double ex = dx / f.r[0], ey = dy / f.r[1];
+        d = ex * ex + ey * ey;
+    }
+    if (f.fast) return d < 1.0 ? f.norm * (1.0 - d) : 0.0;
+    if (f.kind == NLF_INFLUENCE_POLYNOMIAL)
+        return d < 1.0 ? f.norm * exp_pow(1.0 - exp_pow(d, f.p_int, f.p_i, f.p_d), f.q_int, f.q_i, f.q_d) : 0.0;
+    if (f.kind == NLF_INFLUENCE_CONSTANT) return d <= 1.0 ? f.norm : 0.0;
+    return f.norm * exp(f.exp_q * exp_pow(d, f.p_int, f.p_i, f.p_d));
+}
+
+// ---------------------------------------------------------------------------------------------
+// Pattern
+// ---------------------------------------------------------------------------------------------
+constexpr int PATTERN_THREADS = 256;
+
+struct PatternArgs {
+    i64 N;
+    int dof, nn;
+    const i32* el;
+    const i32* ne_off;
+    const i32* ne_ids;
+    const i32* group_of;
+    const int* group_nonlocal;
+    const i64* nb_off;
+    const i32* nb_ids;
+    const uint8_t* inner;
+    int max_words;
+    const i32* lo_e;
+    const i32* hi_e;
+    // pass 0
+    i64* cnt;          // [rows] inner counts
+    i64* bcnt;         // [rows] bound counts
+    int* overflow;
+    // pass 1
+    const i64* outer;
+    const i64* b_outer;
+    i32* cols;
+    i32* b_cols;
+    i64* diag;
+};
+
+template<class F>
+__device__ __forceinline__ void for_each_column_element(const PatternArgs& a, i64 node, F&& f) {
+    // elements whose nodes are columns of the node's rows: LOCAL eL itself, NONLOCAL all of N(eL)
+    for (i32 k = a.ne_off[node]; k < a.ne_off[node + 1]; ++k) {
+        const i32 eL = a.ne_ids[k];
+        if (a.group_nonlocal[a.group_of[eL]]) {
+            const i64 b = a.nb_off[eL], e = a.nb_off[eL + 1];
+            for (i64 s = b + threadIdx.x; s < e; s += blockDim.x) f(a.nb_ids[s]);
+        } else if (threadIdx.x == 0) {
+            f(eL);
+        }
+    }
+}
+
+// exclusive block scan of up to 4 counters
+__device__ __forceinline__ void block_scan4(i64 v[4], i64 total[4], i64* sh /* 4*(threads+1) */) {
+    const int t = threadIdx.x, T = blockDim.x;
+    for (int k = 0; k < 4; ++k) sh[k * (T + 1) + t] = v[k];
+    __syncthreads();
+    if (t < 4) {
+        i64 run = 0;
+        for (int i = 0; i < T; ++i) {
+            const i64 x = sh[t * (T + 1) + i];
+            sh[t * (T + 1) + i] = run;
+            run += x;
+        }
+        sh[t * (T + 1) + T] = run;
+    }
+    __syncthreads();
+    for (int k = 0; k < 4; ++k) { v[k] = sh[k * (T + 1) + t]; total[k] = sh[k * (T + 1) + T]; }
+    __syncthreads();
+}
+
+// Span [lo, hi] of the column node ids reachable from each element (its own nodes when LOCAL,
+// the nodes of its neighbour list when NONLOCAL); one wave per element.
+__global__ void k_elem_span(PatternArgs a, i64 E, const i32* __restrict__ el_minnode, const i32* __restrict__ el_maxnode,
+                            i32* __restrict__ lo_out, i32* __restrict__ hi_out) {
+    const i64 e = i64(blockIdx.x) * (blockDim.x / 64) + (threadIdx.x >> 6);
+    if (e >= E) return;
+    const int lane = threadIdx.x & 63;
+    i32 lo = el_minnode[e], hi = el_maxnode[e];
+    if (a.group_nonlocal[a.group_of[e]]) {
+        for (i64 s = a.nb_off[e] + lane; s < a.nb_off[e + 1]; s += 64) {
+            const i32 f = a.nb_ids[s];
+            lo = min(lo, el_minnode[f]);
+            hi = max(hi, el_maxnode[f]);
+        }
+        for (int o = 32; o > 0; o >>= 1) { lo = min(lo, __shfl_xor(lo, o)); hi = max(hi, __shfl_xor(hi, o)); }
+    }
+    if (lane == 0) { lo_out[e] = lo; hi_out[e] = hi; }
+}
+
+__global__ void k_node_words(i64 N, const i32* __restrict__ ne_off, const i32* __restrict__ ne_ids,
+                             const i32* __restrict__ lo_e, const i32* __restrict__ hi_e, unsigned int* __restrict__ mx) {
+    const i64 n = i64(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (n >= N) return;
+    i32 lo = 0x7fffffff, hi = -1;
+    for (i32 k = ne_off[n]; k < ne_off[n + 1]; ++k) { lo = min(lo, lo_e[ne_ids[k]]); hi = max(hi, hi_e[ne_ids[k]]); }
+    if (hi >= lo) atomicMax(mx, unsigned((hi - lo) / 32 + 1));
+}
+
+template<int PASS>
+__global__ __launch_bounds__(PATTERN_THREADS) void k_pattern(PatternArgs a) {
+    extern __shared__ unsigned int bitmap[];
+    __shared__ i64 scan[4 * (PATTERN_THREADS + 1)];
+    const i64 node = blockIdx.x;
+    if (node >= a.N) return;
+    i64 lo = a.N, hi = -1;
+    for (i32 k = a.ne_off[node]; k < a.ne_off[node + 1]; ++k) {
+        lo = min(lo, (i64)a.lo_e[a.ne_ids[k]]);
+        hi = max(hi, (i64)a.hi_e[a.ne_ids[k]]);
+    }
+    if (hi < lo) return;                         // isolated node: no element
+    const int words = int((hi - lo) / 32 + 1);
+    if (words > a.max_words) {
+        if (threadIdx.x == 0) atomicOr(a.overflow, 1);
+        return;
+    }
+    for (int w = threadIdx.x; w < words; w += blockDim.x) bitmap[w] = 0u;
+    __syncthreads();
+    for_each_column_element(a, node, [&](i32 e) {
+        for (int j = 0; j < a.nn; ++j) {
+            const i64 v = a.el[i64(e) * a.nn + j] - lo;
+            atomicOr(&bitmap[v >> 5], 1u << (v & 31));
+        }
+    });
+    __syncthreads();
+    // each thread owns a contiguous chunk of words; columns are emitted in ascending order
+    const int per = (words + blockDim.x - 1) / blockDim.x;
+    const int w0 = min(words, int(threadIdx.x) * per), w1 = min(words, w0 + per);
+    const int D = a.dof;
+    i64 rowv[2], rowinner[2];
+    for (int d = 0; d < D; ++d) { rowv[d] = i64(D) * node + d; rowinner[d] = a.inner[rowv[d]]; }
+    i64 cnt[4] = {0, 0, 0, 0};   // inner d0, inner d1, bound d0, bound d1
+    for (int w = w0; w < w1; ++w) {
+        unsigned int bits = bitmap[w];
+        while (bits) {
+            const int b = __ffs(bits) - 1;
+            bits &= bits - 1;
+            const i64 m = lo + i64(w) * 32 + b;
+            for (int dc = 0; dc < D; ++dc) {
+                const i64 c = i64(D) * m + dc;
+                const bool ci = a.inner[c];
+                for (int d = 0; d < D; ++d) {
+                    if (ci) { if (rowinner[d]) cnt[d]++; }
+                    else if (c != rowv[d]) cnt[2 + d]++;
+                }
+            }
+        }
+    }
+    if (PASS == 0) {
+        i64 tot[4];
+        block_scan4(cnt, tot, scan);
+        if (threadIdx.x == 0)
+            for (int d = 0; d < D; ++d) {
+                a.cnt[rowv[d]] = rowinner[d] ? tot[d] : 1;      // Dirichlet rows: the diagonal only
+                a.bcnt[rowv[d]] = tot[2 + d];
+            }
+        return;
+    }
+    i64 tot[4];
+    block_scan4(cnt, tot, scan);
+    i64 pos[4];
+    for (int d = 0; d < D; ++d) { pos[d] = a.outer[rowv[d]] + cnt[d]; pos[2 + d] = a.b_outer[rowv[d]] + cnt[2 + d]; }
+    for (int w = w0; w < w1; ++w) {
+        unsigned int bits = bitmap[w];
+        while (bits) {
+            const int b = __ffs(bits) - 1;
+            bits &= bits - 1;
+            const i64 m = lo + i64(w) * 32 + b;
+            for (int dc = 0; dc < D; ++dc) {
+                const i64 c = i64(D) * m + dc;
+                const bool ci = a.inner[c];
+                for (int d = 0; d < D; ++d) {
+                    if (ci) {
+                        if (rowinner[d]) {
+                            if (c == rowv[d]) a.diag[rowv[d]] = pos[d];
+                            a.cols[pos[d]++] = i32(c);
+                        }
+                    } else if (c != rowv[d]) {
+                        a.b_cols[pos[2 + d]++] = i32(c);
+                    }
+                }
+            }
+        }
+    }
+    if (threadIdx.x == 0)
+        for (int d = 0; d < D; ++d)
+            if (!rowinner[d]) { a.cols[a.outer[rowv[d]]] = i32(rowv[d]); a.diag[rowv[d]] = a.outer[rowv[d]]; }
+}
+
+// ---------------------------------------------------------------------------------------------
+// Values
+// ---------------------------------------------------------------------------------------------
+constexpr int ASM_THREADS = 256;
+
+struct AssemblyArgs {
+    i64 N;
+    const i32* el;
+    const i32* ne_off;
+    const i32* ne_ids;
+    const i32* group_of;
+    const GroupParams* groups;
+    const i64* nb_off;
+    const i32* nb_ids;
+    const i32* el_maxnode;
+    const uint8_t* el_dirichlet;
+    const uint8_t* inner;
+    const double* xq;
+    const double* jac;
+    const double* D;
+    double w[MAX_NQ];
+    const i64* outer;
+    const i32* cols;
+    const i64* diag;
+    const i64* b_outer;
+    const i32* b_cols;
+    double* val;
+    double* b_val;
+    int local_only;      // assemble_part::LOCAL: every group treated as LOCAL
+};
+
+__device__ __forceinline__ int lower_bound_i32(const i32* a, int n, i32 key) {
+    int lo = 0, hi = n;
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (a[mid] < key) lo = mid + 1; else hi = mid;
+    }
+    return lo;
+}
+
+// Local integrand row i of element e: K^L_ij (thermal) or 2x2 blocks (elastic), scaled by p1.
+// conductivity_matrix_2d.hpp:106-130, stiffness_matrix_2d.hpp:44-80.
+template<int DOF, int NN, int NQ>
+__device__ void local_row(const AssemblyArgs& a, const GroupParams& g, i64 e, int i, double (&v)[NN][DOF * DOF]) {
+    for (int j = 0; j < NN; ++j)
+        for (int k = 0; k < DOF * DOF; ++k) v[j][k] = 0.0;
+    for (int q = 0; q < NQ; ++q) {
+        const double f = a.w[q] / a.jac[e * NQ + q];
+        const double ix = a.D[2 * ((e * NN + i) * NQ + q)], iy = a.D[2 * ((e * NN + i) * NQ + q) + 1];
+        for (int j = 0; j < NN; ++j) {
+            const double jx = a.D[2 * ((e * NN + j) * NQ + q)], jy = a.D[2 * ((e * NN + j) * NQ + q) + 1];
+            if (DOF == 1) {
+                double value;
+                if (g.ckind == NLF_CONDUCTIVITY_ISOTROPIC) value = g.c[0] * (ix * jx + iy * jy);
+                else if (g.ckind == NLF_CONDUCTIVITY_ORTHOTROPIC) value = g.c[0] * ix * jx + g.c[1] * iy * jy;
+                else value = ix * (g.c[0] * jx + g.c[2] * jy) + iy * (g.c[2] * jx + g.c[1] * jy);
+                v[j][0] += f * value;
+            } else {
+                const double h11 = g.c[0], h12 = g.c[1], h66 = g.c[2];
+                const double wx = f * ix, wy = f * iy;
+                v[j][0] += h11 * wx * jx + h66 * wy * jy;
+                v[j][1] += h12 * wx * jy + h66 * wy * jx;
+                v[j][2] += h12 * wy * jx + h66 * wx * jy;
+                v[j][DOF * DOF - 1] += h11 * wy * jy + h66 * wx * jx;
+            }
+        }
+    }
+    for (int j = 0; j < NN; ++j)
+        for (int k = 0; k < DOF * DOF; ++k) v[j][k] *= g.p1;
+}
+
+// Nonlocal integrand row iL of pair (eL, eNL):
+//   u(qNL) = sum_qL phi(x_qL, x_qNL) w_qL D_i(qL)            (B_e^T Phi contraction)
+//   v_j    = (1 - p1) sum_qNL w_qNL u(qNL) . C D_j(qNL)       (... x C B_e')
+// conductivity_matrix_2d.hpp:132-164, stiffness_matrix_2d.hpp:82-123 (C taken at the nonlocal point).
+template<int DOF, int NN, int NQ>
+__device__ void nonlocal_row(const AssemblyArgs& a, const GroupParams& g, const double* xl /*2*NQ*/,
+                             const double* hl /*2*NQ: w_qL D_i(qL)*/, i64 eNL, double (&v)[NN][DOF * DOF]) {
+    for (int j = 0; j < NN; ++j)
+        for (int k = 0; k < DOF * DOF; ++k) v[j][k] = 0.0;
+    for (int qn = 0; qn < NQ; ++qn) {
+        const double bx = a.xq[2 * (eNL * NQ + qn)], by = a.xq[2 * (eNL * NQ + qn) + 1];
+        double ux = 0.0, uy = 0.0;
+        for (int ql = 0; ql < NQ; ++ql) {
+            const double phi = influence(g.inf, xl[2 * ql], xl[2 * ql + 1], bx, by);
+            ux += phi * hl[2 * ql];
+            uy += phi * hl[2 * ql + 1];
+        }
+        ux *= a.w[qn];
+        uy *= a.w[qn];
+        for (int j = 0; j < NN; ++j) {
+            const double jx = a.D[2 * ((eNL * NN + j) * NQ + qn)], jy = a.D[2 * ((eNL * NN + j) * NQ + qn) + 1];
+            if (DOF == 1) {
+                if (g.ckind == NLF_CONDUCTIVITY_ISOTROPIC) v[j][0] += g.c[0] * (ux * jx + uy * jy);
+                else if (g.ckind == NLF_CONDUCTIVITY_ORTHOTROPIC) v[j][0] += g.c[0] * ux * jx + g.c[1] * uy * jy;
+                else v[j][0] += ux * (g.c[0] * jx + g.c[2] * jy) + uy * (g.c[2] * jx + g.c[1] * jy);
+            } else {
+                const double h11 = g.c[0], h12 = g.c[1], h66 = g.c[2];
+                v[j][0] += h11 * ux * jx + h66 * uy * jy;
+                v[j][1] += h12 * ux * jy + h66 * uy * jx;
+                v[j][2] += h12 * uy * jx + h66 * ux * jy;
+                v[j][DOF * DOF - 1] += h11 * uy * jy + h66 * ux * jx;
+            }
+        }
+    }
+    const double s = 1.0 - g.p1;
+    for (int j = 0; j < NN; ++j)
+        for (int k = 0; k < DOF * DOF; ++k) v[j][k] *= s;
+}
+
+template<int DOF, int NN, int NQ, bool DETERMINISTIC>
+__global__ __launch_bounds__(ASM_THREADS) void k_assemble(AssemblyArgs a) {
+    extern __shared__ unsigned char smem[];
+    __shared__ double xl[2 * NQ], hl[2 * NQ];
+    __shared__ int row_len[DOF], brow_len[DOF];
+    __shared__ int s_off[2 * DOF + 1];
+    const i64 node = blockIdx.x;
+    if (node >= a.N) return;
+    i64 rows[DOF];
+    for (int d = 0; d < DOF; ++d) rows[d] = i64(DOF) * node + d;
+    if (threadIdx.x == 0) {
+        int off = 0;
+        for (int d = 0; d < DOF; ++d) {
+            // upper part only: [diag, end) of the full row
+            row_len[d] = int(a.outer[rows[d] + 1] - a.diag[rows[d]]);
+            brow_len[d] = int(a.b_outer[rows[d] + 1] - a.b_outer[rows[d]]);
+        }
+        for (int d = 0; d < DOF; ++d) { s_off[d] = off; off += row_len[d]; }
+        for (int d = 0; d < DOF; ++d) { s_off[DOF + d] = off; off += brow_len[d]; }
+        s_off[2 * DOF] = off;
+    }
+    __syncthreads();
+    const int total = s_off[2 * DOF];
+    double* acc = reinterpret_cast<double*>(smem);
+    i32* cl = reinterpret_cast<i32*>(acc + total);
+    for (int d = 0; d < DOF; ++d) {
+        for (int k = threadIdx.x; k < row_len[d]; k += blockDim.x) {
+            cl[s_off[d] + k] = a.cols[a.diag[rows[d]] + k];
+            acc[s_off[d] + k] = 0.0;
+        }
+        for (int k = threadIdx.x; k < brow_len[d]; k += blockDim.x) {
+            cl[s_off[DOF + d] + k] = a.b_cols[a.b_outer[rows[d]] + k];
+            acc[s_off[DOF + d] + k] = 0.0;
+        }
+    }
+    bool rinner[DOF];
+    for (int d = 0; d < DOF; ++d) rinner[d] = a.inner[rows[d]];
+    __syncthreads();
+    for (i32 k = a.ne_off[node]; k < a.ne_off[node + 1]; ++k) {
+        const i32 eL = a.ne_ids[k];
+        int iL = 0;
+        for (int i = 0; i < NN; ++i) if (a.el[i64(eL) * NN + i] == node) iL = i;
+        const GroupParams g = a.groups[a.group_of[eL]];
+        const bool nonlocal = g.nonlocal && !a.local_only;
+        if (threadIdx.x < NQ) {
+            const int q = threadIdx.x;
+            xl[2 * q] = a.xq[2 * (i64(eL) * NQ + q)];
+            xl[2 * q + 1] = a.xq[2 * (i64(eL) * NQ + q) + 1];
+            hl[2 * q] = a.w[q] * a.D[2 * ((i64(eL) * NN + iL) * NQ + q)];
+            hl[2 * q + 1] = a.w[q] * a.D[2 * ((i64(eL) * NN + iL) * NQ + q) + 1];
+        }
+        __syncthreads();
+        const i64 nb_b = nonlocal ? a.nb_off[eL] : 0;
+        const i64 cnt = nonlocal ? a.nb_off[eL + 1] - nb_b : 1;
+        for (i64 base = 0; base < cnt; base += blockDim.x) {
+            const i64 t = base + threadIdx.x;
+            double v[NN][DOF * DOF];
+            int pos[NN][DOF * DOF];
+            bool active = false;
+            i64 eNL = -1;
+            if (t < cnt) {
+                eNL = nonlocal ? a.nb_ids[nb_b + t] : eL;
+                // pairs whose columns all lie left of the node's rows and have no Dirichlet column
+                // contribute nothing to the upper triangle nor to the bound part
+                active = a.el_maxnode[eNL] >= node || a.el_dirichlet[eNL];
+            }
+            if (active) {
+                if (nonlocal) {
+                    nonlocal_row<DOF, NN, NQ>(a, g, xl, hl, eNL, v);
+                    if (eNL == eL) {
+                        double lv[NN][DOF * DOF];
+                        local_row<DOF, NN, NQ>(a, g, eL, iL, lv);
+                        for (int j = 0; j < NN; ++j)
+                            for (int q = 0; q < DOF * DOF; ++q) v[j][q] += lv[j][q];
+                    }
+                } else {
+                    local_row<DOF, NN, NQ>(a, g, eL, iL, v);
+                }
+                for (int j = 0; j < NN; ++j) {
+                    const i64 m = a.el[eNL * NN + j];
+                    for (int d = 0; d < DOF; ++d)
+                        for (int dc = 0; dc < DOF; ++dc) {
+                            const i64 c = i64(DOF) * m + dc;
+                            int p = -1;
+                            if (a.inner[c]) {          // INNER: inner row, c >= row (upper storage)
+                                if (rinner[d] && c >= rows[d]) {
+                                    const int q = lower_bound_i32(cl + s_off[d], row_len[d], i32(c));
+                                    if (q < row_len[d] && cl[s_off[d] + q] == i32(c)) p = s_off[d] + q;
+                                }
+                            } else if (c != rows[d]) { // BOUND
+                                const int q = lower_bound_i32(cl + s_off[DOF + d], brow_len[d], i32(c));
+                                if (q < brow_len[d] && cl[s_off[DOF + d] + q] == i32(c)) p = s_off[DOF + d] + q;
+                            }
+                            pos[j][d * DOF + dc] = p;
+                        }
+                }
+            }
+            for (int j = 0; j < NN; ++j) {
+                if (DETERMINISTIC) __syncthreads();
+                if (active)
+                    for (int q = 0; q < DOF * DOF; ++q) {
+                        const int p = pos[j][q];
+                        if (p >= 0) {
+                            if (DETERMINISTIC) acc[p] += v[j][q];
+                            else atomicAdd(&acc[p], v[j][q]);
+                        }
+                    }
+            }
+        }
+        __syncthreads();
+    }
+    __syncthreads();
+    for (int d = 0; d < DOF; ++d) {
+        for (int k = threadIdx.x; k < row_len[d]; k += blockDim.x)
+            a.val[a.diag[rows[d]] + k] = rinner[d] ? acc[s_off[d] + k] : (k == 0 ? 1.0 : 0.0);
+        for (int k = threadIdx.x; k < brow_len[d]; k += blockDim.x)
+            a.b_val[a.b_outer[rows[d]] + k] = acc[s_off[DOF + d] + k];
+    }
+}
+
+// Lower triangle of the full storage: a(r, c) = a(c, r) for c < r (binary search of r in row c).
+__global__ void k_mirror(i64 rows, const i64* __restrict__ outer, const i32* __restrict__ cols,
+                         const i64* __restrict__ diag, double* __restrict__ val) {
+    const i64 r = i64(blockIdx.x) * (blockDim.x / 64) + (threadIdx.x >> 6);
+    if (r >= rows) return;
+    const int lane = threadIdx.x & 63;
+    for (i64 p = outer[r] + lane; p < diag[r]; p += 64) {
+        const i32 c = cols[p];
+        const i64 b = diag[c], e = outer[c + 1];
+        const int q = lower_bound_i32(cols + b, int(e - b), i32(r));
+        if (q < e - b && cols[b + q] == i32(r)) val[p] = val[b + q];
+    }
+}
+
+__global__ void k_elem_minnode(i64 E, int nn, const i32* __restrict__ el, i32* __restrict__ mn) {
+    const i64 e = i64(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (e >= E) return;
+    i32 m = el[e * nn];
+    for (int i = 1; i < nn; ++i) m = min(m, el[e * nn + i]);
+    mn[e] = m;
+}
+
+__global__ void k_el_dirichlet(i64 E, int nn, int dof, const i32* __restrict__ el, const uint8_t* __restrict__ inner,
+                               uint8_t* __restrict__ out) {
+    const i64 e = i64(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (e >= E) return;
+    uint8_t has = 0;
+    for (int i = 0; i < nn; ++i)
+        for (int d = 0; d < dof; ++d) has |= uint8_t(!inner[i64(dof) * el[e * nn + i] + d]);
+    out[e] = has;
+}
+
+__global__ void k_row_max(i64 rows, const i64* __restrict__ outer, const i64* __restrict__ diag,
+                          const i64* __restrict__ b_outer, int dof, unsigned long long* __restrict__ mx) {
+    const i64 n = i64(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (n * dof >= rows) return;
+    i64 s = 0;
+    for (int d = 0; d < dof; ++d) {
+        const i64 r = n * dof + d;
+        s += (outer[r + 1] - diag[r]) + (b_outer[r + 1] - b_outer[r]);
+    }
+    atomicMax(mx, static_cast<unsigned long long>(s));
+}
+
+__global__ void k_add_pairs(i64 n, const i64* __restrict__ rc, const double* __restrict__ v,
+                            const i64* __restrict__ outer, const i32* __restrict__ cols, double* __restrict__ val) {
+    const i64 k = i64(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const i64 r = rc[2 * k], c = rc[2 * k + 1];
+    const i64 b = outer[r], e = outer[r + 1];
+    const int q = lower_bound_i32(cols + b, int(e - b), i32(c));
+    if (q < e - b && cols[b + q] == c) val[b + q] += v[k];
+    if (r != c) {
+        const i64 b2 = outer[c], e2 = outer[c + 1];
+        const int q2 = lower_bound_i32(cols + b2, int(e2 - b2), i32(r));
+        if (q2 < e2 - b2 && cols[b2 + q2] == r) val[b2 + q2] += v[k];
+    }
+}
+
+__global__ void k_to_f32(i64 n, const double* __restrict__ a, float* __restrict__ b) {
+    for (i64 i = i64(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += i64(gridDim.x) * blockDim.x) b[i] = float(a[i]);
+}
+
+template<int DOF, int NN, int NQ>
+void launch_assemble(const AssemblyArgs& a, bool det, size_t lds, hipStream_t s) {
+    if (det) hipLaunchKernelGGL((k_assemble<DOF, NN, NQ, true>), dim3(unsigned(a.N)), dim3(ASM_THREADS), lds, s, a);
+    else hipLaunchKernelGGL((k_assemble<DOF, NN, NQ, false>), dim3(unsigned(a.N)), dim3(ASM_THREADS), lds, s, a);
+}
+
+}  // namespace nlf
+
+using namespace nlf;
+
+namespace {
+
+constexpr int PATTERN_MAX_WORDS = 12288;   // 48 KiB bitmap: column node ids within a span of 393k
+
+GroupParams make_group(const nlf_model& md, int physics, const void* materials, int g, bool only_local) {
+    GroupParams gp;
+    std::memset(&gp, 0, sizeof(gp));
+    double p1 = md.local_weight;
+    double r0 = md.radius[0], r1 = md.radius[1];
+    if (p1 >= 0.999) { r0 = 0; r1 = 0; }                                             // fix_parameters
+    if (r0 < std::numeric_limits<double>::epsilon() || r1 < std::numeric_limits<double>::epsilon()) p1 = 1.0;
+    gp.p1 = p1;
+    gp.nonlocal = !only_local && p1 < 0.999;                                        // theory_type
+    if (gp.nonlocal) {
+        nlf_model m2 = md;
+        m2.radius[0] = r0; m2.radius[1] = r1;
+        NLF_CHECK(md.distance_n == 2 || md.distance_n == 0, NLF_ERR_UNSUPPORTED, "only the L2 distance is implemented");
+        gp.inf = make_influence(m2);
+    }
+    if (physics == 0) {
+        const nlf_thermal_material& t = static_cast<const nlf_thermal_material*>(materials)[g];
+        gp.ckind = t.kind;
+        gp.c[0] = t.c[0]; gp.c[1] = t.c[1]; gp.c[2] = t.c[2];
+    } else {
+        const nlf_elastic_material& e = static_cast<const nlf_elastic_material*>(materials)[g];
+        NLF_CHECK(e.kind == 0, NLF_ERR_UNSUPPORTED, "only isotropic plane-stress elasticity");
+        const double value = e.young / (1.0 - e.poisson * e.poisson);           // elastic_parameters.hpp:70-75
+        gp.c[0] = value; gp.c[1] = e.poisson * value; gp.c[2] = 0.5 * value * (1.0 - e.poisson);
+    }
+    return gp;
+}
+
+}  // namespace
+
+extern "C" {
+
+// conductivity_matrix_2d::compute / stiffness_matrix::compute (physics 0 / 1).  is_inner[rows] is
+// the host mask of non-Dirichlet rows (solvers_utils.hpp:31-40).  part = NLF_PART_FULL or
+// NLF_PART_LOCAL (every group LOCAL, used for the ILLT preconditioner).  Only symmetric problems
+// are supported (the reference solves the asymmetric case with BiCGSTAB, out of scope here).
+int nlf_assemble(nlf_mesh m, int physics, const nlf_model* models, const void* materials, const uint8_t* is_inner,
+                 int symmetric, int part, nlf_matrix* out) {
+    if (!m) return NLF_ERR_ARGUMENT;
+    return guarded(m->ctx, [&] {
+        NLF_CHECK(out && models && materials && is_inner, NLF_ERR_ARGUMENT, "null argument");
+        NLF_CHECK(physics == 0 || physics == 1, NLF_ERR_ARGUMENT, "physics must be 0 (thermal) or 1 (elastic)");
+        NLF_CHECK(symmetric, NLF_ERR_UNSUPPORTED, "only symmetric assembly is implemented");
+        const auto t0 = std::chrono::steady_clock::now();
+        hipStream_t s = m->ctx->stream;
+        const int dof = physics == 0 ? 1 : 2;
+        const i64 rows = dof * m->N;
+        const bool only_local = part == NLF_PART_LOCAL;
+        std::vector<GroupParams> gps(m->G);
+        std::vector<int> gnl(m->G);
+        bool any_nonlocal = false;
+        for (int g = 0; g < m->G; ++g) {
+            gps[g] = make_group(models[g], physics, materials, g, only_local);
+            gnl[g] = gps[g].nonlocal;
+            any_nonlocal |= gnl[g] != 0;
+        }
+        if (any_nonlocal) {
+            NLF_CHECK(m->has_neighbours, NLF_ERR_STATE, "nonlocal assembly needs nlf_find_neighbours first");
+            for (int g = 0; g < m->G; ++g)
+                NLF_CHECK(!gnl[g] || m->searched[g] || m->gbeg_h[g] == m->gbeg_h[g + 1], NLF_ERR_STATE,
+                          "a nonlocal group was not searched for neighbours");
+        }
+        auto mat = std::make_unique<nlf_matrix_s>();
+        mat->ctx = m->ctx; mat->mesh = m; mat->dof = dof; mat->rows = rows;
+        upload(mat->inner, is_inner, size_t(rows), s);
+        DevBuf<GroupParams> d_groups;
+        upload(d_groups, gps.data(), gps.size(), s);
+        DevBuf<int> d_gnl;
+        upload(d_gnl, gnl.data(), gnl.size(), s);
+        // ---- pattern ----
+        PatternArgs pa{};
+        pa.N = m->N; pa.dof = dof; pa.nn = m->nn; pa.el = m->el.p; pa.ne_off = m->ne_off.p; pa.ne_ids = m->ne_ids.p;
+        pa.group_of = m->group_of.p; pa.group_nonlocal = d_gnl.p;
+        pa.nb_off = m->has_neighbours ? m->nb_off.p : nullptr;
+        pa.nb_ids = m->has_neighbours ? m->nb_ids.p : nullptr;
+        pa.inner = mat->inner.p; pa.max_words = PATTERN_MAX_WORDS;
+        mat->outer.alloc(size_t(rows + 1));
+        mat->b_outer.alloc(size_t(rows + 1));
+        mat->diag.alloc(size_t(rows));
+        NLF_HIP(hipMemsetAsync(mat->outer.p, 0, mat->outer.bytes(), s));
+        NLF_HIP(hipMemsetAsync(mat->b_outer.p, 0, mat->b_outer.bytes(), s));
+        DevBuf<int> overflow{1};
+        NLF_HIP(hipMemsetAsync(overflow.p, 0, sizeof(int), s));
+        pa.cnt = mat->outer.p; pa.bcnt = mat->b_outer.p; pa.overflow = overflow.p;
+        DevBuf<i32> el_min{size_t(m->E)}, lo_e{size_t(m->E)}, hi_e{size_t(m->E)};
+        hipLaunchKernelGGL(k_elem_minnode, dim3(blocks_for(m->E, 256)), dim3(256), 0, s, m->E, m->nn, m->el.p, el_min.p);
+        hipLaunchKernelGGL(k_elem_span, dim3(blocks_for(m->E, 4)), dim3(256), 0, s, pa, m->E, el_min.p,
+                           m->el_maxnode.p, lo_e.p, hi_e.p);
+        pa.lo_e = lo_e.p; pa.hi_e = hi_e.p;
+        DevBuf<unsigned int> d_words{1};
+        NLF_HIP(hipMemsetAsync(d_words.p, 0, sizeof(unsigned int), s));
+        hipLaunchKernelGGL(k_node_words, dim3(blocks_for(m->N, 256)), dim3(256), 0, s, m->N, m->ne_off.p, m->ne_ids.p,
+                           lo_e.p, hi_e.p, d_words.p);
+        NLF_HIP(hipGetLastError());
+        unsigned int words = 0;
+        download(&words, d_words.p, 1, s);
+        NLF_CHECK(int(words) <= PATTERN_MAX_WORDS, NLF_ERR_UNSUPPORTED,
+                  "column node ids of a row span more than 393k ids (renumber the mesh)");
+        const size_t lds_bitmap = std::max<size_t>(words, 1) * sizeof(unsigned int);
+        hipLaunchKernelGGL(k_pattern<0>, dim3(unsigned(m->N)), dim3(PATTERN_THREADS), lds_bitmap, s, pa);
+        NLF_HIP(hipGetLastError());
+        int ovf = 0;
+        download(&ovf, overflow.p, 1, s);
+        NLF_CHECK(!ovf, NLF_ERR_UNSUPPORTED, "column node ids of a row span more than 393k ids (renumber the mesh)");
+        exclusive_scan_i64(mat->outer.p, rows, s);
+        exclusive_scan_i64(mat->b_outer.p, rows, s);
+        download(&mat->nnz_full, mat->outer.p + rows, 1, s);
+        download(&mat->nnz_bound, mat->b_outer.p + rows, 1, s);
+        NLF_CHECK(mat->nnz_full < (i64(1) << 40), NLF_ERR_ARGUMENT, "pattern too large");
+        mat->cols.alloc(size_t(std::max<i64>(mat->nnz_full, 1)));
+        mat->b_cols.alloc(size_t(std::max<i64>(mat->nnz_bound, 1)));
+        pa.outer = mat->outer.p; pa.b_outer = mat->b_outer.p; pa.cols = mat->cols.p; pa.b_cols = mat->b_cols.p;
+        pa.diag = mat->diag.p;
+        hipLaunchKernelGGL(k_pattern<1>, dim3(unsigned(m->N)), dim3(PATTERN_THREADS), lds_bitmap, s, pa);
+        NLF_HIP(hipGetLastError());
+        NLF_HIP(hipStreamSynchronize(s));
+        const auto t1 = std::chrono::steady_clock::now();
+        mat->seconds_pattern = std::chrono::duration<double>(t1 - t0).count();
+        // ---- values ----
+        DevBuf<uint8_t> el_dir{size_t(m->E)};
+        hipLaunchKernelGGL(k_el_dirichlet, dim3(blocks_for(m->E, 256)), dim3(256), 0, s, m->E, m->nn, dof, m->el.p,
+                           mat->inner.p, el_dir.p);
+        DevBuf<unsigned long long> d_max{1};
+        NLF_HIP(hipMemsetAsync(d_max.p, 0, sizeof(unsigned long long), s));
+        hipLaunchKernelGGL(k_row_max, dim3(blocks_for(m->N, 256)), dim3(256), 0, s, rows, mat->outer.p, mat->diag.p,
+                           mat->b_outer.p, dof, d_max.p);
+        unsigned long long maxrow = 0;
+        download(&maxrow, d_max.p, 1, s);
+        const size_t lds = size_t(maxrow) * (sizeof(double) + sizeof(i32)) + 16;
+        NLF_CHECK(lds <= 60 * 1024, NLF_ERR_UNSUPPORTED, "matrix rows too long for the LDS accumulator");
+        mat->val.alloc(size_t(std::max<i64>(mat->nnz_full, 1)));
+        mat->b_val.alloc(size_t(std::max<i64>(mat->nnz_bound, 1)));
+        AssemblyArgs aa{};
+        aa.N = m->N; aa.el = m->el.p; aa.ne_off = m->ne_off.p; aa.ne_ids = m->ne_ids.p; aa.group_of = m->group_of.p;
+        aa.groups = d_groups.p; aa.nb_off = pa.nb_off; aa.nb_ids = pa.nb_ids; aa.el_maxnode = m->el_maxnode.p;
+        aa.el_dirichlet = el_dir.p; aa.inner = mat->inner.p; aa.xq = m->xq.p; aa.jac = m->jac.p; aa.D = m->D.p;
+        for (int q = 0; q < m->fe.nq; ++q) aa.w[q] = m->fe.w[q];
+        aa.outer = mat->outer.p; aa.cols = mat->cols.p; aa.diag = mat->diag.p; aa.b_outer = mat->b_outer.p;
+        aa.b_cols = mat->b_cols.p; aa.val = mat->val.p; aa.b_val = mat->b_val.p; aa.local_only = only_local;
+        const bool det = m->injective;
+        if (physics == 0 && m->nn == 4) launch_assemble<1, 4, 4>(aa, det, lds, s);
+        else if (physics == 0 && m->nn == 8) launch_assemble<1, 8, 9>(aa, det, lds, s);
+        else if (physics == 1 && m->nn == 4) launch_assemble<2, 4, 4>(aa, det, lds, s);
+        else launch_assemble<2, 8, 9>(aa, det, lds, s);
+        NLF_HIP(hipGetLastError());
+        hipLaunchKernelGGL(k_mirror, dim3(blocks_for(rows, 4)), dim3(256), 0, s, rows, mat->outer.p, mat->cols.p,
+                           mat->diag.p, mat->val.p);
+        NLF_HIP(hipGetLastError());
+        NLF_HIP(hipStreamSynchronize(s));
+        mat->nnz_upper = (mat->nnz_full + rows) / 2;
+        mat->seconds_values = std::chrono::duration<double>(std::chrono::steady_clock::now() - t1).count();
+        *out = mat.release();
+    });
+}
+
+int nlf_matrix_destroy(nlf_matrix a) {
+    delete a;
+    return NLF_OK;
+}
+
+// rows, nnz of the reference's upper CSR (inner), nnz of the full storage, nnz of the bound part.
+int nlf_matrix_info(nlf_matrix a, int64_t* rows, int64_t* nnz_upper, int64_t* nnz_full, int64_t* nnz_bound) {
+    if (!a) return NLF_ERR_ARGUMENT;
+    if (rows) *rows = a->rows;
+    if (nnz_upper) *nnz_upper = a->nnz_upper;
+    if (nnz_full) *nnz_full = a->nnz_full;
+    if (nnz_bound) *nnz_bound = a->nnz_bound;
+    return NLF_OK;
+}
+
+int nlf_matrix_timings(nlf_matrix a, double* pattern_seconds, double* values_seconds) {
+    if (!a) return NLF_ERR_ARGUMENT;
+    if (pattern_seconds) *pattern_seconds = a->seconds_pattern;
+    if (values_seconds) *values_seconds = a->seconds_values;
+    return NLF_OK;
+}
+
+// Host export in the reference's layout (finite_element_matrix::inner()/bound(): RowMajor CSR with
+// int64 indices): part 0 = inner upper triangle, 1 = bound, 2 = inner full storage.
+int nlf_matrix_export(nlf_matrix a, int part, int64_t* outer, int64_t* inner, double* values) {
+    if (!a) return NLF_ERR_ARGUMENT;
+    return guarded(a->ctx, [&] {
+        hipStream_t s = a->ctx->stream;
+        const i64 rows = a->rows;
+        if (part == 1) {
+            std::vector<i64> o(rows + 1);
+            download(o.data(), a->b_outer.p, size_t(rows + 1), s);
+            std::vector<i32> c(size_t(a->nnz_bound));
+            download(c.data(), a->b_cols.p, c.size(), s);
+            if (outer) std::copy(o.begin(), o.end(), outer);
+            if (inner) for (size_t k = 0; k < c.size(); ++k) inner[k] = c[k];
+            if (values) download(values, a->b_val.p, size_t(a->nnz_bound), s);
+            return;
+        }
+        std::vector<i64> o(rows + 1), dg(rows);
+        download(o.data(), a->outer.p, size_t(rows + 1), s);
+        download(dg.data(), a->diag.p, size_t(rows), s);
+        std::vector<i32> c(size_t(a->nnz_full));
+        download(c.data(), a->cols.p, c.size(), s);
+        std::vector<double> v(size_t(a->nnz_full));
+        download(v.data(), a->val.p, v.size(), s);
+        if (part == 2) {
+            if (outer) std::copy(o.begin(), o.end(), outer);
+            if (inner) for (size_t k = 0; k < c.size(); ++k) inner[k] = c[k];
+            if (values) std::copy(v.begin(), v.end(), values);
+            return;
+        }
+        i64 pos = 0;
+        if (outer) outer[0] = 0;
+        for (i64 r = 0; r < rows; ++r) {
+            for (i64 p = dg[r]; p < o[r + 1]; ++p, ++pos) {
+                if (inner) inner[pos] = c[p];
+                if (values) values[pos] = v[p];
+            }
+            if (outer) outer[r + 1] = pos;
+        }
+    });
+}
+
+// Device views of the full inner storage: outer i64[rows+1], cols i32, values f64, diag i64[rows].
+int nlf_matrix_device(nlf_matrix a, const int64_t** outer, const int32_t** cols, double** values, const int64_t** diag) {
+    if (!a) return NLF_ERR_ARGUMENT;
+    if (outer) *outer = a->outer.p;
+    if (cols) *cols = a->cols.p;
+    if (values) *values = a->val.p;
+    if (diag) *diag = a->diag.p;
+    return NLF_OK;
+}
+
+// Adds symmetric pair contributions (row <= col, unique pairs) to the inner part on the device.
+int nlf_matrix_add_pairs(nlf_matrix a, int64_t n, const int64_t* rc_host, const double* v_host) {
+    if (!a) return NLF_ERR_ARGUMENT;
+    return guarded(a->ctx, [&] {
+        if (n <= 0) return;
+        hipStream_t s = a->ctx->stream;
+        DevBuf<i64> rc;
+        DevBuf<double> v;
+        upload(rc, rc_host, size_t(2 * n), s);
+        upload(v, v_host, size_t(n), s);
+        hipLaunchKernelGGL(k_add_pairs, dim3(blocks_for(n, 256)), dim3(256), 0, s, i64(n), rc.p, v.p, a->outer.p,
+                           a->cols.p, a->val.p);
+        NLF_HIP(hipGetLastError());
+        NLF_HIP(hipStreamSynchronize(s));
+    });
+}
+
+// convection_condition_2d (convection_condition_2d.hpp:11-31): K(row, col) += h int N_i N_j over the
+// boundary edges of convection groups, for col >= row with both rows inner (Dirichlet rows keep
+// the identity, as after first_kind_matrix_fill_2d).  bcs[GB] per boundary group.
+int nlf_matrix_add_convection(nlf_matrix a, const nlf_boundary* bcs) {
+    if (!a) return NLF_ERR_ARGUMENT;
+    return guarded(a->ctx, [&] {
+        NLF_CHECK(a->dof == 1, NLF_ERR_ARGUMENT, "convection is a thermal condition");
+        const nlf_mesh m = a->mesh;
+        std::vector<uint8_t> inner(size_t(a->rows));
+        download(inner.data(), a->inner.p, inner.size(), a->ctx->stream);
+        std::map<std::pair<i64, i64>, double> acc;
+        const BoundaryTables& t = m->bt;
+        for (int b = 0; b < m->GB; ++b) {
+            if (bcs[b].kind[0] != NLF_BC_CONVECTION) continue;
+            for (i64 be = m->bbeg_h[b]; be < m->bbeg_h[b + 1]; ++be) {
+                double jq[2];
+                for (int q = 0; q < t.nq; ++q) {
+                    double jx = 0, jy = 0;
+                    for (int i = 0; i < t.nb; ++i) {
+                        const i64 v = m->be_h[be * t.nb + i];
+                        jx = jx + m->xy_h[2 * v] * t.qNxi[i * t.nq + q];
+                        jy = jy + m->xy_h[2 * v + 1] * t.qNxi[i * t.nq + q];
+                    }
+                    jq[q] = std::sqrt(jx * jx + jy * jy);
+                }
+                for (int i = 0; i < t.nb; ++i) {
+                    const i64 row = m->be_h[be * t.nb + i];
+                    for (int j = 0; j < t.nb; ++j) {
+                        const i64 col = m->be_h[be * t.nb + j];
+                        if (col < row || !inner[row] || !inner[col]) continue;
+                        double integral = 0.0;
+                        for (int q = 0; q < t.nq; ++q) integral += t.w[q] * t.qN[i * t.nq + q] * t.qN[j * t.nq + q] * jq[q];
+                        acc[{row, col}] += bcs[b].heat_transfer * integral;
+                    }
+                }
+            }
+        }
+        std::vector<i64> rc;
+        std::vector<double> v;
+        for (const auto& [k, x] : acc) { rc.push_back(k.first); rc.push_back(k.second); v.push_back(x); }
+        if (v.empty()) return;
+        hipStream_t s = a->ctx->stream;
+        DevBuf<i64> d_rc;
+        DevBuf<double> d_v;
+        upload(d_rc, rc.data(), rc.size(), s);
+        upload(d_v, v.data(), v.size(), s);
+        hipLaunchKernelGGL(k_add_pairs, dim3(blocks_for(i64(v.size()), 256)), dim3(256), 0, s, i64(v.size()), d_rc.p,
+                           d_v.p, a->outer.p, a->cols.p, a->val.p);
+        NLF_HIP(hipGetLastError());
+        NLF_HIP(hipStreamSynchronize(s));
+    });
+}
+
+// FP32 copy of the values (mixed-precision inner solves).
+int nlf_matrix_make_fp32(nlf_matrix a) {
+    if (!a) return NLF_ERR_ARGUMENT;
+    return guarded(a->ctx, [&] {
+        hipStream_t s = a->ctx->stream;
+        if (!a->val32.p) a->val32.alloc(size_t(std::max<i64>(a->nnz_full, 1)));
+        hipLaunchKernelGGL(k_to_f32, dim3(2048), dim3(256), 0, s, a->nnz_full, a->val.p, a->val32.p);
+        NLF_HIP(hipGetLastError());
+    });
+}
+
+// Boundary data on the host (O(boundary)): second-kind RHS f (boundary_condition_second_kind_2d.hpp:12-33,
+// convection h*T_a), Dirichlet values xD (first wins) and the inner mask (solvers_utils.hpp:31-40).
+int nlf_boundary_data(nlf_mesh m, int dof, const nlf_boundary* bcs, double* f, double* xD, uint8_t* is_inner) {
+    if (!m) return NLF_ERR_ARGUMENT;
+    return guarded(m->ctx, [&] {
+        const i64 rows = dof * m->N;
+        const BoundaryTables& t = m->bt;
+        if (is_inner) std::fill(is_inner, is_inner + rows, uint8_t(1));
+        if (xD) std::fill(xD, xD + rows, 0.0);
+        if (f) std::fill(f, f + rows, 0.0);
+        for (int b = 0; b < m->GB; ++b)
+            for (int d = 0; d < dof; ++d) {
+                if (bcs[b].kind[d] != NLF_BC_FIRST_KIND) continue;
+                for (i64 be = m->bbeg_h[b]; be < m->bbeg_h[b + 1]; ++be)
+                    for (int i = 0; i < t.nb; ++i) {
+                        const i64 v = m->be_h[be * t.nb + i];
+                        if (xD && xD[dof * v + d] == 0.0) xD[dof * v + d] = bcs[b].value[d];
+                        if (is_inner) is_inner[dof * v + d] = 0;
+                    }
+            }
+        if (!f) return;
+        for (int b = 0; b < m->GB; ++b)
+            for (int d = 0; d < dof; ++d) {
+                double g;
+                if (bcs[b].kind[d] == NLF_BC_FLUX) g = bcs[b].value[d];
+                else if (dof == 1 && bcs[b].kind[d] == NLF_BC_CONVECTION) g = bcs[b].heat_transfer * bcs[b].ambient;
+                else continue;
+                for (i64 be = m->bbeg_h[b]; be < m->bbeg_h[b + 1]; ++be) {
+                    double jq[2];
+                    for (int q = 0; q < t.nq; ++q) {
+                        double jx = 0, jy = 0;
+                        for (int i = 0; i < t.nb; ++i) {
+                            const i64 v = m->be_h[be * t.nb + i];
+                            jx = jx + m->xy_h[2 * v] * t.qNxi[i * t.nq + q];
+                            jy = jy + m->xy_h[2 * v + 1] * t.qNxi[i * t.nq + q];
+                        }
+                        jq[q] = std::sqrt(jx * jx + jy * jy);
+                    }
+                    for (int i = 0; i < t.nb; ++i) {
+                        const i64 v = m->be_h[be * t.nb + i];
+                        double integral = 0.0;
+                        for (int q = 0; q < t.nq; ++q) integral += t.w[q] * t.qN[i * t.nq + q] * g * jq[q];
+                        f[dof * v + d] += integral;
+                    }
+                }
+            }
+    });
+}
+
+}  // extern "C"

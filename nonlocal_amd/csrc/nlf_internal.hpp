@@ -1,0 +1,199 @@
+// Internal declarations shared by the HIP translation units of libnlf.so.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstring>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/nlf.h"
+
+namespace nlf {
+
+using i64 = int64_t;
+using i32 = int32_t;
+
+struct Error : std::runtime_error {
+    int code;
+    Error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+#define NLF_HIP(expr)                                                                                   \
+    do {                                                                                                \
+        hipError_t e_ = (expr);                                                                         \
+        if (e_ != hipSuccess)                                                                           \
+            throw ::nlf::Error(NLF_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_));         \
+    } while (0)
+
+#define NLF_CHECK(cond, code, msg)                                                                      \
+    do {                                                                                                \
+        if (!(cond)) throw ::nlf::Error((code), (msg));                                                 \
+    } while (0)
+
+// Owning device buffer (hipMalloc / hipFree).
+template<class T>
+struct DevBuf {
+    T* p = nullptr;
+    size_t n = 0;
+    DevBuf() = default;
+    explicit DevBuf(size_t count) { alloc(count); }
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+    DevBuf(DevBuf&& o) noexcept : p(o.p), n(o.n) { o.p = nullptr; o.n = 0; }
+    DevBuf& operator=(DevBuf&& o) noexcept { release(); p = o.p; n = o.n; o.p = nullptr; o.n = 0; return *this; }
+    ~DevBuf() { release(); }
+    void alloc(size_t count) {
+        release();
+        n = count;
+        if (count) NLF_HIP(hipMalloc(&p, count * sizeof(T)));
+    }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    size_t bytes() const { return n * sizeof(T); }
+};
+
+template<class T>
+void upload(DevBuf<T>& d, const T* h, size_t count, hipStream_t s) {
+    d.alloc(count);
+    if (count) NLF_HIP(hipMemcpyAsync(d.p, h, count * sizeof(T), hipMemcpyHostToDevice, s));
+}
+
+template<class T>
+void download(T* h, const T* d, size_t count, hipStream_t s) {
+    if (count) NLF_HIP(hipMemcpyAsync(h, d, count * sizeof(T), hipMemcpyDeviceToHost, s));
+    NLF_HIP(hipStreamSynchronize(s));
+}
+
+inline unsigned blocks_for(i64 n, int threads) { return unsigned((n + threads - 1) / threads); }
+
+// ---- FE tables (host; the device gets them by value in kernel arguments) ----------------------
+constexpr int MAX_NN = 8;
+constexpr int MAX_NQ = 9;
+
+struct FeTables {
+    int nn = 0, nq = 0;
+    double w[MAX_NQ];
+    double qN[MAX_NN * MAX_NQ], qNxi[MAX_NN * MAX_NQ], qNeta[MAX_NN * MAX_NQ];  // [i*nq + q]
+    double N0[MAX_NN];
+};
+
+struct BoundaryTables {
+    int nb = 0, nq = 0;
+    double w[2], qN[3 * 2], qNxi[3 * 2];
+};
+
+FeTables make_fe_tables(int nn);
+BoundaryTables make_boundary_tables(int nb);
+
+// ---- influence function (device evaluation parameters) ---------------------------------------
+struct InfluenceParams {
+    int kind = NLF_INFLUENCE_POLYNOMIAL;
+    int fast = 0;                        // fast_polynomial: norm*(1-d)
+    int p_int = 1, q_int = 1;            // exponent p/n and q are integers
+    int p_i = 1, q_i = 1;
+    double p_d = 1.0, q_d = 1.0;
+    double exp_q = -0.5;
+    double r[2] = {0, 0};
+    double inv_r[2] = {0, 0};
+    double norm = 1.0;
+};
+
+InfluenceParams make_influence(const nlf_model& m);
+
+struct GroupParams {                     // per element group, copied to device
+    double p1;                           // local weight (1 for LOCAL groups)
+    double c[3];                         // thermal conductivity components or Hooke {h11, h12, h66}
+    int nonlocal;                        // theory NONLOCAL
+    int ckind;                           // conductivity kind
+    InfluenceParams inf;
+};
+
+}  // namespace nlf
+
+// ---- opaque handle definitions -------------------------------------------------------------------
+struct nlf_ctx_s {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    bool own_stream = false;
+    std::string error;
+};
+
+struct nlf_mesh_s {
+    nlf_ctx ctx = nullptr;
+    nlf::i64 N = 0, E = 0;
+    int nn = 0, nb = 0, G = 0, GB = 0;
+    bool injective = true;               // for every local index j, e -> elems[e][j] is injective
+    nlf::FeTables fe;
+    nlf::BoundaryTables bt;
+    // host copies used by boundary integrals and level analysis
+    std::vector<double> xy_h;
+    std::vector<nlf::i32> el_h, be_h;
+    std::vector<nlf::i64> gbeg_h, bbeg_h;
+    // device geometry and precompute
+    nlf::DevBuf<double> xy;              // 2N
+    nlf::DevBuf<nlf::i32> el;            // E*nn
+    nlf::DevBuf<nlf::i32> group_of;      // E
+    nlf::DevBuf<nlf::i32> ne_off, ne_ids; // node -> elements (ascending), int32 offsets (4*E < 2^31)
+    nlf::DevBuf<nlf::i32> el_maxnode;    // max node id of each element
+    nlf::DevBuf<double> xq, jac, D;      // 2*E*nq, E*nq, 2*E*nn*nq ([e][i][q])
+    nlf::DevBuf<double> centres;         // 2E
+    // neighbours
+    bool has_neighbours = false;
+    nlf::DevBuf<nlf::i64> nb_off;        // E+1
+    nlf::DevBuf<nlf::i32> nb_ids;
+    std::vector<double> expanded;        // 2G
+    std::vector<int> searched;           // group searched (radius > 0)
+    double last_neighbour_seconds = 0.0;
+};
+
+struct nlf_matrix_s {
+    nlf_ctx ctx = nullptr;
+    nlf_mesh mesh = nullptr;
+    int dof = 1;
+    nlf::i64 rows = 0;
+    // inner part stored as the full (both-triangle) pattern; the upper triangle of each row is its
+    // tail starting at diag[r] (the reference's upper CSR is a view of it).
+    nlf::i64 nnz_full = 0, nnz_upper = 0, nnz_bound = 0;
+    nlf::DevBuf<nlf::i64> outer;         // rows+1
+    nlf::DevBuf<nlf::i32> cols;
+    nlf::DevBuf<nlf::i64> diag;          // position of the diagonal in each row
+    nlf::DevBuf<nlf::i64> tpos;          // position of the transposed entry (filled on demand)
+    nlf::DevBuf<double> val;             // full values
+    nlf::DevBuf<float> val32;            // FP32 copy (mixed precision), on demand
+    nlf::DevBuf<nlf::i64> b_outer;       // bound part: rows+1
+    nlf::DevBuf<nlf::i32> b_cols;
+    nlf::DevBuf<double> b_val;
+    nlf::DevBuf<uint8_t> inner;          // is_inner per row
+    double seconds_pattern = 0.0, seconds_values = 0.0;
+};
+
+namespace nlf {
+// helpers implemented across translation units
+void exclusive_scan_i64(i64* data, i64 n, hipStream_t s);   // in place over n+1 entries, data[n] = total
+void set_error(nlf_ctx ctx, const std::string& msg);
+
+// Runs f, translating exceptions into status codes and the context's last error.
+template<class F>
+int guarded(nlf_ctx ctx, F&& f) {
+    try {
+        f();
+        return NLF_OK;
+    } catch (const Error& e) {
+        set_error(ctx, e.what());
+        return e.code;
+    } catch (const std::bad_alloc&) {
+        set_error(ctx, "host allocation failed");
+        return NLF_ERR_ARGUMENT;
+    } catch (const std::exception& e) {
+        set_error(ctx, e.what());
+        return NLF_ERR_ARGUMENT;
+    }
+}
+}  // namespace nlf

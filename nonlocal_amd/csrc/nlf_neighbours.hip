@@ -1,0 +1,337 @@
+// Influence-radius element neighbour search (find_neighbours.hpp:49-160), bit-exact.
+//
+// Compiled with -ffp-contract=off.  Output order is the reference's: for each element, the 3x3
+// window of cells is scanned x-cell outer, y-cell inner, each cell's elements in ascending id
+// (split_elements_by_subareas pushes them in id order, find_neighbours.hpp:95-117,152-155).
+//
+// Layout: cells of every searched group are numbered consecutively; (cell << 32 | element) keys are
+// radix-sorted once, which yields every cell's element list in ascending id order.  A wave (64
+// lanes) handles one element and tests 64 candidates per step; ballot + popcount give the ordered
+// write position.  The predicate is evaluated exactly as powered_distance<2>(x, y, r) <= 1
+// (distance.hpp:17-23, norm.hpp:15-22): z_i = (a_i - b_i) / r_i, s = (0 + z0*z0) + z1*z1.  A
+// reciprocal-multiply estimate decides candidates whose value is more than 1e-12 away from 1
+// (the estimate differs from the exact value by < 2e-15 relative), the rest take the division path.
+#include "nlf_internal.hpp"
+
+#include <rocprim/rocprim.hpp>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <limits>
+
+namespace nlf {
+
+void exclusive_scan_i64(i64* data, i64 n, hipStream_t s) {
+    size_t bytes = 0;
+    NLF_HIP(rocprim::exclusive_scan(nullptr, bytes, data, data, i64(0), size_t(n + 1), rocprim::plus<i64>(), s));
+    DevBuf<uint8_t> tmp{bytes + 16};
+    NLF_HIP(rocprim::exclusive_scan(tmp.p, bytes, data, data, i64(0), size_t(n + 1), rocprim::plus<i64>(), s));
+    NLF_HIP(hipStreamSynchronize(s));
+}
+
+struct SearchGroup {
+    double r[2];        // expanded radii
+    double inv_r[2];
+    double R;           // cell side max(r0, r1)
+    double left, down;
+    i64 rows, cols;     // cells along x and y
+    i64 base;           // first global cell id
+    i64 begin, end;     // element range
+    int searched;
+};
+
+__device__ __forceinline__ unsigned long long ordered_bits(double v) {
+    const unsigned long long b = __double_as_longlong(v);
+    return (b & 0x8000000000000000ull) ? ~b : (b | 0x8000000000000000ull);
+}
+
+__host__ __device__ inline double from_ordered_bits(unsigned long long o) {
+    const unsigned long long b = (o & 0x8000000000000000ull) ? (o & ~0x8000000000000000ull) : ~o;
+    double v;
+    memcpy(&v, &b, sizeof(v));
+    return v;
+}
+
+// max (or min) over element-node-adjacent pairs of |c_e - c_a|^2 per group
+// (diameter_between_elements_centers, find_neighbours.hpp:49-72).
+__global__ void k_expand(i64 E, int nn, int strategy, const i32* __restrict__ el, const i32* __restrict__ group_of,
+                         const int* __restrict__ searched, const i32* __restrict__ ne_off, const i32* __restrict__ ne_ids,
+                         const double* __restrict__ c, unsigned long long* __restrict__ out) {
+    const i64 e = i64(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (e >= E) return;
+    const int g = group_of[e];
+    if (!searched[g]) return;
+    double best = strategy == NLF_EXPAND_MIN ? 1.0e308 : 0.0;
+    const double cx = c[2 * e], cy = c[2 * e + 1];
+    for (int i = 0; i < nn; ++i) {
+        const i32 v = el[e * nn + i];
+        for (i32 k = ne_off[v]; k < ne_off[v + 1]; ++k) {
+            const i32 a = ne_ids[k];
+            const double z0 = cx - c[2 * a], z1 = cy - c[2 * a + 1];
+            double s = 0.0;
+            s = s + z0 * z0;
+            s = s + z1 * z1;
+            best = strategy == NLF_EXPAND_MIN ? fmin(best, s) : fmax(best, s);
+        }
+    }
+    const unsigned long long bits = __double_as_longlong(best);   // non-negative: bit order == value order
+    if (strategy == NLF_EXPAND_MIN) atomicMin(&out[g], bits); else atomicMax(&out[g], bits);
+}
+
+// group_corners (find_neighbours.hpp:40-47): min/max over the nodes of the group's elements.
+__global__ void k_bbox(i64 E, int nn, const i32* __restrict__ el, const i32* __restrict__ group_of,
+                       const int* __restrict__ searched, const double* __restrict__ xy,
+                       unsigned long long* __restrict__ box) {
+    const i64 e = i64(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (e >= E) return;
+    const int g = group_of[e];
+    if (!searched[g]) return;
+    double l = 1e308, d = 1e308, r = -1e308, u = -1e308;
+    for (int i = 0; i < nn; ++i) {
+        const i32 v = el[e * nn + i];
+        l = fmin(l, xy[2 * v]); d = fmin(d, xy[2 * v + 1]);
+        r = fmax(r, xy[2 * v]); u = fmax(u, xy[2 * v + 1]);
+    }
+    atomicMin(&box[4 * g + 0], ordered_bits(l));
+    atomicMin(&box[4 * g + 1], ordered_bits(d));
+    atomicMax(&box[4 * g + 2], ordered_bits(r));
+    atomicMax(&box[4 * g + 3], ordered_bits(u));
+}
+
+__device__ __forceinline__ void cell_of(const SearchGroup& sg, double cx, double cy, i64& a, i64& b) {
+    // subarea_id (find_neighbours.hpp:86-93): size_t((c - corner) / R); clamped to the grid so a
+    // centre rounded onto the far corner cannot index past the last cell
+    const double u = (cx - sg.left) / sg.R, v = (cy - sg.down) / sg.R;
+    a = u > 0.0 ? i64((unsigned long long)u) : 0;
+    b = v > 0.0 ? i64((unsigned long long)v) : 0;
+    a = a < sg.rows ? a : sg.rows - 1;
+    b = b < sg.cols ? b : sg.cols - 1;
+}
+
+__global__ void k_cell_keys(i64 E, const i32* __restrict__ group_of, const SearchGroup* __restrict__ groups,
+                            const double* __restrict__ c, unsigned long long* __restrict__ keys,
+                            i64* __restrict__ cell_count, i64* __restrict__ n_keys_unused) {
+    const i64 e = i64(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (e >= E) return;
+    const SearchGroup sg = groups[group_of[e]];
+    unsigned long long key = ~0ull;                               // not searched: sorts last
+    if (sg.searched) {
+        i64 a, b;
+        cell_of(sg, c[2 * e], c[2 * e + 1], a, b);
+        const i64 cell = sg.base + a * sg.cols + b;
+        key = (static_cast<unsigned long long>(cell) << 32) | static_cast<unsigned long long>(e);
+        atomicAdd(reinterpret_cast<unsigned long long*>(&cell_count[cell]), 1ull);
+    }
+    keys[e] = key;
+}
+
+__device__ __forceinline__ bool inside(const SearchGroup& sg, double ax, double ay, double bx, double by) {
+    const double dx = ax - bx, dy = ay - by;
+    const double ex = dx * sg.inv_r[0], ey = dy * sg.inv_r[1];
+    double est = 0.0;
+    est = est + ex * ex;
+    est = est + ey * ey;
+    if (est < 1.0 - 1e-12) return true;
+    if (est > 1.0 + 1e-12) return false;
+    const double z0 = dx / sg.r[0], z1 = dy / sg.r[1];
+    double s = 0.0;
+    s = s + z0 * z0;
+    s = s + z1 * z1;
+    return s <= 1.0;
+}
+
+// One wave per element.  PASS 0 counts, PASS 1 writes ids at offsets.
+template<int PASS>
+__global__ __launch_bounds__(256) void k_neighbours(i64 E, const i32* __restrict__ group_of,
+                                                    const SearchGroup* __restrict__ groups, const double* __restrict__ c,
+                                                    const i64* __restrict__ cell_off, const i32* __restrict__ cell_ids,
+                                                    i64* __restrict__ count_or_off, i32* __restrict__ ids) {
+    const int lane = threadIdx.x & 63;
+    const i64 e = i64(blockIdx.x) * (blockDim.x / 64) + (threadIdx.x >> 6);
+    if (e >= E) return;
+    const SearchGroup sg = groups[group_of[e]];
+    if (!sg.searched) {
+        if (PASS == 0 && lane == 0) count_or_off[e] = 0;
+        return;
+    }
+    const double ax = c[2 * e], ay = c[2 * e + 1];
+    i64 row, col;
+    cell_of(sg, ax, ay, row, col);
+    const i64 r0 = row ? row - 1 : row, r1 = row != sg.rows - 1 ? row + 2 : row + 1;   // subareas_ids :109-117
+    const i64 c0 = col ? col - 1 : col, c1 = col != sg.cols - 1 ? col + 2 : col + 1;
+    i64 pos = PASS == 0 ? 0 : count_or_off[e];
+    for (i64 i = r0; i < r1; ++i)
+        for (i64 j = c0; j < c1; ++j) {
+            const i64 cell = sg.base + i * sg.cols + j;
+            const i64 b = cell_off[cell], end = cell_off[cell + 1];
+            for (i64 k = b; k < end; k += 64) {
+                bool take = false;
+                i32 cand = 0;
+                if (k + lane < end) {
+                    cand = cell_ids[k + lane];
+                    take = inside(sg, ax, ay, c[2 * cand], c[2 * cand + 1]);
+                }
+                const unsigned long long mask = __ballot(take);
+                if (PASS == 1 && take) ids[pos + __popcll(mask & ((1ull << lane) - 1ull))] = cand;
+                pos += __popcll(mask);
+            }
+        }
+    if (PASS == 0 && lane == 0) count_or_off[e] = pos;
+}
+
+__global__ void k_extract_ids(i64 n, const unsigned long long* __restrict__ keys, i32* __restrict__ ids) {
+    const i64 i = i64(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) ids[i] = i32(keys[i] & 0xffffffffull);
+}
+
+}  // namespace nlf
+
+using namespace nlf;
+
+extern "C" {
+
+// find_neighbours (find_neighbours.hpp:119-160) + mesh_2d::neighbours (mesh_2d.hpp:205-212).
+// search_radii[2G]: per-group search radius (search_radius, else nonlocal_radius; read_model.hpp:226-244);
+// a group with a radius <= 0 is not searched and its elements get empty lists.  expanded_out[2G]
+// (optional) receives the expanded radii.  Strategies NO, MAX, MIN are supported.
+int nlf_find_neighbours(nlf_mesh m, const double* search_radii, int strategy, double* expanded_out) {
+    if (!m) return NLF_ERR_ARGUMENT;
+    return guarded(m->ctx, [&] {
+        NLF_CHECK(search_radii, NLF_ERR_ARGUMENT, "null radii");
+        NLF_CHECK(strategy == NLF_EXPAND_NO || strategy == NLF_EXPAND_MAX || strategy == NLF_EXPAND_MIN,
+                  NLF_ERR_UNSUPPORTED, "diameter expanding strategy MEAN is not implemented");
+        const auto t0 = std::chrono::steady_clock::now();
+        hipStream_t s = m->ctx->stream;
+        const int G = m->G;
+        const i64 E = m->E;
+        std::vector<int> searched(G);
+        for (int g = 0; g < G; ++g)
+            searched[g] = search_radii[2 * g] > 0.0 && search_radii[2 * g + 1] > 0.0 && m->gbeg_h[g + 1] > m->gbeg_h[g];
+        DevBuf<int> d_searched;
+        upload(d_searched, searched.data(), size_t(G), s);
+        // expansion
+        std::vector<double> add(G, 0.0);
+        if (strategy != NLF_EXPAND_NO) {
+            std::vector<unsigned long long> init(G, strategy == NLF_EXPAND_MIN ? 0x7fefffffffffffffull : 0ull);
+            DevBuf<unsigned long long> d_exp;
+            upload(d_exp, init.data(), size_t(G), s);
+            hipLaunchKernelGGL(k_expand, dim3(blocks_for(E, 256)), dim3(256), 0, s, E, m->nn, strategy, m->el.p,
+                               m->group_of.p, d_searched.p, m->ne_off.p, m->ne_ids.p, m->centres.p, d_exp.p);
+            NLF_HIP(hipGetLastError());
+            std::vector<unsigned long long> got(G);
+            download(got.data(), d_exp.p, size_t(G), s);
+            for (int g = 0; g < G; ++g) {
+                double v;
+                std::memcpy(&v, &got[g], sizeof(v));
+                add[g] = searched[g] ? std::sqrt(v) : 0.0;
+            }
+        }
+        // corners
+        std::vector<unsigned long long> box_init(4 * G);
+        for (int g = 0; g < G; ++g) {
+            box_init[4 * g + 0] = box_init[4 * g + 1] = ~0ull;
+            box_init[4 * g + 2] = box_init[4 * g + 3] = 0ull;
+        }
+        DevBuf<unsigned long long> d_box;
+        upload(d_box, box_init.data(), box_init.size(), s);
+        hipLaunchKernelGGL(k_bbox, dim3(blocks_for(E, 256)), dim3(256), 0, s, E, m->nn, m->el.p, m->group_of.p,
+                           d_searched.p, m->xy.p, d_box.p);
+        NLF_HIP(hipGetLastError());
+        std::vector<unsigned long long> box(4 * G);
+        download(box.data(), d_box.p, box.size(), s);
+        std::vector<SearchGroup> sg(G);
+        m->expanded.assign(2 * G, 0.0);
+        i64 cells = 0;
+        for (int g = 0; g < G; ++g) {
+            SearchGroup& q = sg[g];
+            std::memset(&q, 0, sizeof(q));
+            q.begin = m->gbeg_h[g]; q.end = m->gbeg_h[g + 1];
+            q.r[0] = search_radii[2 * g] + add[g];
+            q.r[1] = search_radii[2 * g + 1] + add[g];
+            q.searched = searched[g] && q.r[0] > 0.0 && q.r[1] > 0.0;
+            if (!q.searched) continue;
+            m->expanded[2 * g] = q.r[0]; m->expanded[2 * g + 1] = q.r[1];
+            q.inv_r[0] = 1.0 / q.r[0]; q.inv_r[1] = 1.0 / q.r[1];
+            q.R = std::max(q.r[0], q.r[1]);
+            q.left = from_ordered_bits(box[4 * g + 0]); q.down = from_ordered_bits(box[4 * g + 1]);
+            const double right = from_ordered_bits(box[4 * g + 2]), up = from_ordered_bits(box[4 * g + 3]);
+            q.rows = i64(size_t((right - q.left) / q.R)) + 1;          // split_elements_by_subareas :101
+            q.cols = i64(size_t((up - q.down) / q.R)) + 1;
+            q.base = cells;
+            cells += q.rows * q.cols;
+        }
+        NLF_CHECK(cells < (i64(1) << 31), NLF_ERR_UNSUPPORTED, "too many search cells");
+        m->searched = searched;
+        DevBuf<SearchGroup> d_sg;
+        upload(d_sg, sg.data(), size_t(G), s);
+        // bin elements into cells: count + radix sort of (cell, element) keys
+        DevBuf<i64> cell_off{size_t(cells + 1)};
+        NLF_HIP(hipMemsetAsync(cell_off.p, 0, cell_off.bytes(), s));
+        DevBuf<unsigned long long> keys{size_t(E)}, sorted{size_t(E)};
+        hipLaunchKernelGGL(k_cell_keys, dim3(blocks_for(E, 256)), dim3(256), 0, s, E, m->group_of.p, d_sg.p,
+                           m->centres.p, keys.p, cell_off.p, (i64*)nullptr);
+        NLF_HIP(hipGetLastError());
+        exclusive_scan_i64(cell_off.p, cells, s);
+        {
+            int end_bit = 32;
+            while ((i64(1) << (end_bit - 32)) <= cells && end_bit < 64) ++end_bit;
+            end_bit = std::min(64, end_bit + 1);
+            size_t bytes = 0;
+            NLF_HIP(rocprim::radix_sort_keys(nullptr, bytes, keys.p, sorted.p, size_t(E), 0, end_bit, s));
+            DevBuf<uint8_t> tmp{bytes + 16};
+            NLF_HIP(rocprim::radix_sort_keys(tmp.p, bytes, keys.p, sorted.p, size_t(E), 0, end_bit, s));
+        }
+        DevBuf<i32> cell_ids{size_t(E)};
+        hipLaunchKernelGGL(k_extract_ids, dim3(blocks_for(E, 256)), dim3(256), 0, s, E, sorted.p, cell_ids.p);
+        // count, scan, fill
+        m->nb_off.alloc(size_t(E + 1));
+        NLF_HIP(hipMemsetAsync(m->nb_off.p, 0, m->nb_off.bytes(), s));
+        const unsigned wblocks = blocks_for(E, 4);
+        hipLaunchKernelGGL(k_neighbours<0>, dim3(wblocks), dim3(256), 0, s, E, m->group_of.p, d_sg.p, m->centres.p,
+                           cell_off.p, cell_ids.p, m->nb_off.p, (i32*)nullptr);
+        NLF_HIP(hipGetLastError());
+        exclusive_scan_i64(m->nb_off.p, E, s);
+        i64 total = 0;
+        download(&total, m->nb_off.p + E, 1, s);
+        m->nb_ids.alloc(size_t(std::max<i64>(total, 1)));
+        hipLaunchKernelGGL(k_neighbours<1>, dim3(wblocks), dim3(256), 0, s, E, m->group_of.p, d_sg.p, m->centres.p,
+                           cell_off.p, cell_ids.p, m->nb_off.p, m->nb_ids.p);
+        NLF_HIP(hipGetLastError());
+        NLF_HIP(hipStreamSynchronize(s));
+        m->has_neighbours = true;
+        if (expanded_out) std::copy(m->expanded.begin(), m->expanded.end(), expanded_out);
+        m->last_neighbour_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    });
+}
+
+int nlf_neighbours_count(nlf_mesh m, int64_t* total) {
+    if (!m || !total) return NLF_ERR_ARGUMENT;
+    return guarded(m->ctx, [&] {
+        NLF_CHECK(m->has_neighbours, NLF_ERR_STATE, "neighbours not computed");
+        download(total, m->nb_off.p + m->E, 1, m->ctx->stream);
+    });
+}
+
+// Host export of the lists as CSR: offsets[E+1], ids[total] (for bit-exact parity checks).
+int nlf_neighbours_export(nlf_mesh m, int64_t* offsets, int32_t* ids) {
+    if (!m) return NLF_ERR_ARGUMENT;
+    return guarded(m->ctx, [&] {
+        NLF_CHECK(m->has_neighbours, NLF_ERR_STATE, "neighbours not computed");
+        hipStream_t s = m->ctx->stream;
+        i64 total = 0;
+        download(&total, m->nb_off.p + m->E, 1, s);
+        if (offsets) download(offsets, m->nb_off.p, size_t(m->E + 1), s);
+        if (ids && total) download(ids, m->nb_ids.p, size_t(total), s);
+    });
+}
+
+// Device pointers of the lists (offsets i64[E+1], ids i32) for callers that keep them on the GPU.
+int nlf_neighbours_device(nlf_mesh m, const int64_t** offsets, const int32_t** ids) {
+    if (!m || !m->has_neighbours) return NLF_ERR_STATE;
+    if (offsets) *offsets = m->nb_off.p;
+    if (ids) *ids = m->nb_ids.p;
+    return NLF_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,850 @@
+// Krylov layer: CSR SpMV, CG (conjugate_gradient.hpp:48-79), Jacobi and ILLT preconditioners
+// (eigen_preconditioner.hpp:50-57), FP32-inner / FP64-refinement mixed mode.
+//
+// SpMV runs on the full (both-triangle) storage so every output row is a gather (no scatter, no
+// atomics, deterministic); LPR lanes cooperate on one row.  CG keeps its scalars on the device in a
+// state block; each iteration is 5 launches (SpMV+dot, alpha, update+precond+dot, beta, p update)
+// with no host synchronisation; the host reads the state every few iterations.
+//
+// ILLT: IC(0) of the local matrix, diagonal scaling s = 1/sqrt|a_ii| and shift retries (our choice:
+// Eigen::IncompleteCholesky is not vendored, so its factor is not reproduced).  Factorisation and
+// both triangular solves are level-scheduled: rows are grouped by dependency level (computed once on
+// the host from the pattern) and one persistent 1024-thread workgroup sweeps the levels with a
+// barrier between them.
+#include "nlf_internal.hpp"
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+
+namespace nlf {
+
+constexpr int SPMV_THREADS = 256;
+constexpr int LEVEL_THREADS = 1024;
+
+struct CGState {
+    double rz, pq, alpha, beta, bnorm, res, tol;
+    int done, pad;
+    i64 it, max_it;
+    int breakdown, pad2;
+};
+
+template<class T> __device__ __forceinline__ T shfl_down_t(T v, int o) { return __shfl_down(v, o); }
+
+template<class Acc>
+__device__ __forceinline__ Acc block_sum(Acc v, Acc* sh) {
+    // fixed-order tree reduction (deterministic)
+    for (int o = 32; o > 0; o >>= 1) v += shfl_down_t(v, o);
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    if (lane == 0) sh[w] = v;
+    __syncthreads();
+    Acc s = 0;
+    if (threadIdx.x == 0)
+        for (int i = 0; i < int(blockDim.x >> 6); ++i) s += sh[i];
+    __syncthreads();
+    return s;
+}
+
+// y = A x (rows in [0, rows)), optionally partial[b] = sum_r x_r y_r over the block's rows.
+template<int LPR, class V, class X, class Acc, bool DOT>
+__global__ __launch_bounds__(SPMV_THREADS) void k_spmv(i64 rows, const i64* __restrict__ outer, const i32* __restrict__ cols,
+                                                       const V* __restrict__ val, const X* __restrict__ x, X* __restrict__ y,
+                                                       double* __restrict__ partial, const CGState* __restrict__ st) {
+    __shared__ double sh[SPMV_THREADS / 64];
+    if (st && st->done) return;
+    const int sub = threadIdx.x % LPR;
+    const i64 r = i64(blockIdx.x) * (SPMV_THREADS / LPR) + threadIdx.x / LPR;
+    Acc acc = 0;
+    if (r < rows) {
+        const i64 b = outer[r], e = outer[r + 1];
+        for (i64 p = b + sub; p < e; p += LPR) acc += Acc(val[p]) * Acc(x[cols[p]]);
+    }
+    for (int o = LPR / 2; o > 0; o >>= 1) acc += shfl_down_t(acc, o);   // lane sub==0 of each group holds the sum
+    double d = 0.0;
+    if (r < rows && sub == 0) {
+        y[r] = X(acc);
+        if (DOT) d = double(x[r]) * double(X(acc));
+    }
+    if (DOT) {
+        d = block_sum<double>(d, sh);
+        if (threadIdx.x == 0) partial[blockIdx.x] = d;
+    }
+}
+
+__global__ void k_sum_partials(const double* __restrict__ partial, int n, double* __restrict__ out) {
+    __shared__ double sh[1024 / 64];
+    double s = 0.0;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) s += partial[i];
+    s = block_sum<double>(s, sh);
+    if (threadIdx.x == 0) *out = s;
+}
+
+// alpha = rz / (p . A p)
+__global__ void k_alpha(const double* __restrict__ partial, int n, CGState* __restrict__ st) {
+    __shared__ double sh[1024 / 64];
+    if (st->done) return;
+    double s = 0.0;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) s += partial[i];
+    s = block_sum<double>(s, sh);
+    if (threadIdx.x == 0) {
+        st->pq = s;
+        st->alpha = st->rz / s;
+        if (!(s > 0.0) || !isfinite(st->alpha)) { st->breakdown = 1; st->done = 1; }
+    }
+}
+
+// x += alpha p; r -= alpha q; z = M r (identity / Jacobi, or left for a separate apply); partial rz.
+template<class X, bool JACOBI, bool DOT>
+__global__ __launch_bounds__(SPMV_THREADS) void k_update_xr(i64 n, X* __restrict__ x, X* __restrict__ r, const X* __restrict__ p,
+                                                            const X* __restrict__ q, X* __restrict__ z,
+                                                            const X* __restrict__ invd, double* __restrict__ partial,
+                                                            const CGState* __restrict__ st) {
+    __shared__ double sh[SPMV_THREADS / 64];
+    if (st->done) return;
+    const X a = X(st->alpha);
+    double d = 0.0;
+    for (i64 i = i64(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += i64(gridDim.x) * blockDim.x) {
+        x[i] += a * p[i];
+        const X ri = r[i] - a * q[i];
+        r[i] = ri;
+        if (z) {
+            const X zi = JACOBI ? invd[i] * ri : ri;
+            z[i] = zi;
+            if (DOT) d += double(ri) * double(zi);
+        }
+    }
+    if (DOT) {
+        d = block_sum<double>(d, sh);
+        if (threadIdx.x == 0) partial[blockIdx.x] = d;
+    }
+}
+
+template<class X>
+__global__ __launch_bounds__(SPMV_THREADS) void k_dot(i64 n, const X* __restrict__ a, const X* __restrict__ b,
+                                                      double* __restrict__ partial, const CGState* __restrict__ st) {
+    __shared__ double sh[SPMV_THREADS / 64];
+    if (st && st->done) return;
+    double d = 0.0;
+    for (i64 i = i64(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += i64(gridDim.x) * blockDim.x)
+        d += double(a[i]) * double(b[i]);
+    d = block_sum<double>(d, sh);
+    if (threadIdx.x == 0) partial[blockIdx.x] = d;
+}
+
+// mu = rz_new / rz; residual = sqrt(rz)/||b||; stop test of the reference.
+__global__ void k_beta(const double* __restrict__ partial, int n, CGState* __restrict__ st) {
+    __shared__ double sh[1024 / 64];
+    if (st->done) return;
+    double s = 0.0;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) s += partial[i];
+    s = block_sum<double>(s, sh);
+    if (threadIdx.x == 0) {
+        st->beta = s / st->rz;
+        st->rz = s;
+        st->it += 1;
+        st->res = sqrt(fabs(s)) / st->bnorm;
+        if (!isfinite(st->res)) { st->breakdown = 1; st->done = 1; }
+        if (st->res <= st->tol || st->it >= st->max_it) st->done = 1;
+    }
+}
+
+template<class X>
+__global__ void k_update_p(i64 n, X* __restrict__ p, const X* __restrict__ z, const CGState* __restrict__ st) {
+    if (st->done) return;
+    const X b = X(st->beta);
+    for (i64 i = i64(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += i64(gridDim.x) * blockDim.x) p[i] = z[i] + b * p[i];
+}
+
+// CG start: rz = r.z, bnorm from partials, residual, done.
+__global__ void k_init_state(const double* __restrict__ prz, const double* __restrict__ pbb, int n, double tol, i64 max_it,
+                             CGState* __restrict__ st) {
+    __shared__ double sh[1024 / 64];
+    double a = 0.0, b = 0.0;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) { a += prz[i]; b += pbb[i]; }
+    a = block_sum<double>(a, sh);
+    b = block_sum<double>(b, sh);
+    if (threadIdx.x == 0) {
+        st->rz = a;
+        st->bnorm = sqrt(b);
+        st->tol = tol;
+        st->it = 0;
+        st->max_it = max_it;
+        st->breakdown = 0;
+        st->res = st->bnorm > 0.0 ? sqrt(fabs(a)) / st->bnorm : 0.0;
+        st->done = (st->res <= tol || max_it <= 0) ? 1 : 0;
+    }
+}
+
+template<class A, class B>
+__global__ void k_axpby(i64 n, A alpha, const B* __restrict__ x, A beta, B* __restrict__ y) {   // y = alpha x + beta y
+    for (i64 i = i64(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += i64(gridDim.x) * blockDim.x)
+        y[i] = B(alpha * A(x[i]) + beta * A(y[i]));
+}
+
+// y += x (FP32 correction accumulated into the FP64 solution)
+template<class Y>
+__global__ void k_add_f32(i64 n, const float* __restrict__ x, Y* __restrict__ y) {
+    for (i64 i = i64(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += i64(gridDim.x) * blockDim.x) y[i] += Y(x[i]);
+}
+
+template<class A, class B>
+__global__ void k_convert(i64 n, const A* __restrict__ x, B* __restrict__ y) {
+    for (i64 i = i64(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += i64(gridDim.x) * blockDim.x) y[i] = B(x[i]);
+}
+
+template<class X>
+__global__ void k_scale(i64 n, const X* __restrict__ s, const X* __restrict__ x, X* __restrict__ y) {
+    for (i64 i = i64(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += i64(gridDim.x) * blockDim.x) y[i] = s[i] * x[i];
+}
+
+// r = b - A x (full storage, one wave per row) and partial sums of r.r and b.b.
+template<class V, class X>
+__global__ __launch_bounds__(SPMV_THREADS) void k_residual(i64 rows, const i64* __restrict__ outer, const i32* __restrict__ cols,
+                                                           const V* __restrict__ val, const X* __restrict__ x,
+                                                           const X* __restrict__ b, X* __restrict__ r,
+                                                           double* __restrict__ prr, double* __restrict__ pbb) {
+    __shared__ double sh[SPMV_THREADS / 64];
+    const int lane = threadIdx.x & 63;
+    const i64 row = i64(blockIdx.x) * (SPMV_THREADS / 64) + (threadIdx.x >> 6);
+    double acc = 0.0;
+    if (row < rows)
+        for (i64 p = outer[row] + lane; p < outer[row + 1]; p += 64) acc += double(val[p]) * double(x[cols[p]]);
+    for (int o = 32; o > 0; o >>= 1) acc += __shfl_down(acc, o);
+    double rr = 0.0, bb = 0.0;
+    if (row < rows && lane == 0) {
+        const double ri = double(b[row]) - acc;
+        r[row] = X(ri);
+        rr = ri * ri;
+        bb = double(b[row]) * double(b[row]);
+    }
+    rr = block_sum<double>(rr, sh);
+    bb = block_sum<double>(bb, sh);
+    if (threadIdx.x == 0) { prr[blockIdx.x] = rr; pbb[blockIdx.x] = bb; }
+}
+
+// f -= K_bound x_D ; f[D] = x_D   (boundary_condition_first_kind_2d.hpp:50-59)
+__global__ void k_first_kind(i64 rows, const i64* __restrict__ bo, const i32* __restrict__ bc, const double* __restrict__ bv,
+                             const uint8_t* __restrict__ inner, const double* __restrict__ xD, double* __restrict__ f) {
+    const i64 r = i64(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (r >= rows) return;
+    double s = 0.0;
+    for (i64 p = bo[r]; p < bo[r + 1]; ++p) s += bv[p] * xD[bc[p]];
+    f[r] = inner[r] ? f[r] - s : xD[r];
+}
+
+__global__ void k_inv_diag(i64 rows, const i64* __restrict__ diag, const double* __restrict__ val, double* __restrict__ inv,
+                           float* __restrict__ inv32) {
+    const i64 r = i64(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (r >= rows) return;
+    const double d = val[diag[r]];
+    const double v = d != 0.0 ? 1.0 / d : 1.0;      // Eigen::DiagonalPreconditioner: 1 for zero diagonals
+    inv[r] = v;
+    inv32[r] = float(v);
+}
+
+// ---- IC(0) ------------------------------------------------------------------------------------
+struct LevelArgs {
+    i64 rows;
+    int levels;
+    const i64* outer;
+    const i32* cols;
+    const i64* diag;
+    const i64* tpos;       // position of the transposed entry
+    const i32* lrows;      // rows sorted by level
+    const i32* lptr;       // level offsets
+};
+
+__global__ void k_ic_init(i64 rows, const i64* __restrict__ outer, const i32* __restrict__ cols, const i64* __restrict__ diag,
+                          const double* __restrict__ a, const double* __restrict__ s, double shift, double* __restrict__ F) {
+    const i64 r = i64(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (r >= rows) return;
+    for (i64 p = outer[r]; p < outer[r + 1]; ++p) F[p] = s[r] * a[p] * s[cols[p]];
+    F[diag[r]] += shift;
+}
+
+__global__ void k_scale_from_diag(i64 rows, const i64* __restrict__ diag, const double* __restrict__ a, double* __restrict__ s,
+                                  float* __restrict__ s32) {
+    const i64 r = i64(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (r >= rows) return;
+    const double d = fabs(a[diag[r]]);
+    s[r] = d > 0.0 ? 1.0 / sqrt(d) : 1.0;
+    s32[r] = float(s[r]);
+}
+
+// Up-looking IC(0): for row k (all rows i < k it depends on are done and mirrored into row k's
+// lower positions), U_kk = sqrt(a_kk - sum_i U_ik^2), U_kj = (a_kj - sum_{i<k} U_ik U_ij) / U_kk.
+__global__ __launch_bounds__(LEVEL_THREADS) void k_ic_factor(LevelArgs a, double* __restrict__ F, int* __restrict__ fail) {
+    for (int L = 0; L < a.levels; ++L) {
+        for (i32 t = a.lptr[L] + threadIdx.x; t < a.lptr[L + 1]; t += blockDim.x) {
+            const i64 k = a.lrows[t];
+            const i64 b = a.outer[k], dk = a.diag[k], e = a.outer[k + 1];
+            double d = F[dk];
+            for (i64 p = b; p < dk; ++p) d -= F[p] * F[p];
+            if (!(d > 0.0) || !isfinite(d)) { *fail = 1; d = 1.0; }
+            const double ukk = sqrt(d);
+            F[dk] = ukk;
+            for (i64 p = dk + 1; p < e; ++p) {
+                const i32 j = a.cols[p];
+                double v = F[p];
+                // intersection of lower(k) and lower(j) restricted to i < k
+                i64 pj = a.outer[j];
+                const i64 ej = a.diag[j];
+                for (i64 pk = b; pk < dk; ++pk) {
+                    const i32 i = a.cols[pk];
+                    while (pj < ej && a.cols[pj] < i) ++pj;
+                    if (pj < ej && a.cols[pj] == i) v -= F[pk] * F[pj];
+                }
+                v /= ukk;
+                F[p] = v;
+                F[a.tpos[p]] = v;
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// Forward U^T y = s.r then backward U z' = y, z = s.z' (level-scheduled, one workgroup).
+template<class X, class V>
+__global__ __launch_bounds__(LEVEL_THREADS) void k_ic_solve(LevelArgs fw, LevelArgs bw, const V* __restrict__ F,
+                                                            const X* __restrict__ s, const X* __restrict__ r,
+                                                            X* __restrict__ y, X* __restrict__ z,
+                                                            const CGState* __restrict__ st) {
+    if (st && st->done) return;
+    for (int L = 0; L < fw.levels; ++L) {
+        for (i32 t = fw.lptr[L] + threadIdx.x; t < fw.lptr[L + 1]; t += blockDim.x) {
+            const i64 k = fw.lrows[t];
+            X v = s[k] * r[k];
+            for (i64 p = fw.outer[k]; p < fw.diag[k]; ++p) v -= X(F[p]) * y[fw.cols[p]];
+            y[k] = v / X(F[fw.diag[k]]);
+        }
+        __syncthreads();
+    }
+    for (int L = 0; L < bw.levels; ++L) {
+        for (i32 t = bw.lptr[L] + threadIdx.x; t < bw.lptr[L + 1]; t += blockDim.x) {
+            const i64 k = bw.lrows[t];
+            X v = y[k];
+            const i64 dk = bw.diag[k];
+            for (i64 p = dk + 1; p < bw.outer[k + 1]; ++p) v -= X(F[p]) * y[bw.cols[p]];
+            y[k] = v / X(F[dk]);
+        }
+        __syncthreads();
+    }
+    for (i64 i = threadIdx.x; i < fw.rows; i += blockDim.x) z[i] = s[i] * y[i];
+}
+
+__global__ void k_transpose_pos(i64 rows, const i64* __restrict__ outer, const i32* __restrict__ cols, i64* __restrict__ tpos) {
+    const i64 r = i64(blockIdx.x) * (blockDim.x / 64) + (threadIdx.x >> 6);
+    if (r >= rows) return;
+    const int lane = threadIdx.x & 63;
+    for (i64 p = outer[r] + lane; p < outer[r + 1]; p += 64) {
+        const i32 c = cols[p];
+        i64 lo = outer[c], hi = outer[c + 1];
+        while (lo < hi) {
+            const i64 mid = (lo + hi) >> 1;
+            if (cols[mid] < r) lo = mid + 1; else hi = mid;
+        }
+        tpos[p] = (lo < outer[c + 1] && cols[lo] == r) ? lo : p;   // symmetric pattern: always found
+    }
+}
+
+template<class X>
+__global__ void k_zero(i64 n, X* __restrict__ x) {
+    for (i64 i = i64(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += i64(gridDim.x) * blockDim.x) x[i] = X(0);
+}
+
+}  // namespace nlf
+
+// ---- preconditioner object ---------------------------------------------------------------------
+struct nlf_precond_s {
+    nlf_ctx ctx = nullptr;
+    int kind = NLF_PRECOND_IDENTITY;
+    nlf::i64 rows = 0;
+    bool ok = true;
+    double shift = 0.0;
+    double setup_seconds = 0.0;
+    // Jacobi
+    nlf::DevBuf<double> invd;
+    nlf::DevBuf<float> invd32;
+    // ILLT
+    nlf::DevBuf<nlf::i64> outer, diag, tpos;
+    nlf::DevBuf<nlf::i32> cols, fw_rows, fw_ptr, bw_rows, bw_ptr;
+    int fw_levels = 0, bw_levels = 0;
+    nlf::DevBuf<double> F, scale, work;
+    nlf::DevBuf<float> F32, scale32, work32;
+    // learned (set by nlf_learned.hip)
+    void* learned = nullptr;
+    void (*learned_destroy)(void*) = nullptr;
+    ~nlf_precond_s() {
+        if (learned && learned_destroy) learned_destroy(learned);
+    }
+};
+
+namespace nlf {
+
+static LevelArgs fw_args(const nlf_precond_s* P) {
+    return LevelArgs{P->rows, P->fw_levels, P->outer.p, P->cols.p, P->diag.p, P->tpos.p, P->fw_rows.p, P->fw_ptr.p};
+}
+static LevelArgs bw_args(const nlf_precond_s* P) {
+    return LevelArgs{P->rows, P->bw_levels, P->outer.p, P->cols.p, P->diag.p, P->tpos.p, P->bw_rows.p, P->bw_ptr.p};
+}
+
+// Level sets of the lower (forward) and upper (backward) dependency DAGs of a symmetric pattern.
+static void level_sets(const std::vector<i64>& outer, const std::vector<i32>& cols, const std::vector<i64>& diag,
+                       std::vector<i32>& fr, std::vector<i32>& fp, std::vector<i32>& br, std::vector<i32>& bp) {
+    const i64 n = i64(diag.size());
+    std::vector<i32> lev(n, 0);
+    int nl = 0;
+    for (i64 k = 0; k < n; ++k) {
+        int l = 0;
+        for (i64 p = outer[k]; p < diag[k]; ++p) l = std::max(l, lev[cols[p]] + 1);
+        lev[k] = l;
+        nl = std::max(nl, l + 1);
+    }
+    auto bucket = [&](int levels, std::vector<i32>& rowsv, std::vector<i32>& ptr, bool ascending) {
+        ptr.assign(levels + 1, 0);
+        for (i64 k = 0; k < n; ++k) ptr[lev[k] + 1]++;
+        for (int l = 0; l < levels; ++l) ptr[l + 1] += ptr[l];
+        rowsv.assign(n, 0);
+        std::vector<i32> fill(ptr.begin(), ptr.end() - 1);
+        if (ascending) for (i64 k = 0; k < n; ++k) rowsv[fill[lev[k]]++] = i32(k);
+        else for (i64 k = n - 1; k >= 0; --k) rowsv[fill[lev[k]]++] = i32(k);
+    };
+    bucket(nl, fr, fp, true);
+    std::fill(lev.begin(), lev.end(), 0);
+    nl = 0;
+    for (i64 k = n - 1; k >= 0; --k) {
+        int l = 0;
+        for (i64 p = diag[k] + 1; p < outer[k + 1]; ++p) l = std::max(l, lev[cols[p]] + 1);
+        lev[k] = l;
+        nl = std::max(nl, l + 1);
+    }
+    bucket(nl, br, bp, false);
+}
+
+unsigned grid_stride_blocks(i64 n) { return unsigned(std::min<i64>(std::max<i64>((n + 255) / 256, 1), 4096)); }
+
+// Applies the preconditioner z = M^-1 r on the stream (FP64 or FP32 vectors).
+template<class X>
+void precond_apply(const nlf_precond_s* P, i64 n, const X* r, X* z, const CGState* st, hipStream_t s);
+
+template<>
+void precond_apply<double>(const nlf_precond_s* P, i64 n, const double* r, double* z, const CGState* st, hipStream_t s) {
+    if (P && P->kind == NLF_PRECOND_JACOBI)
+        hipLaunchKernelGGL(k_scale<double>, dim3(grid_stride_blocks(n)), dim3(256), 0, s, n, P->invd.p, r, z);
+    else if (P && P->kind == NLF_PRECOND_ILLT)
+        hipLaunchKernelGGL((k_ic_solve<double, double>), dim3(1), dim3(LEVEL_THREADS), 0, s, fw_args(P), bw_args(P), P->F.p,
+                           P->scale.p, r, P->work.p, z, st);
+    else
+        NLF_HIP(hipMemcpyAsync(z, r, size_t(n) * sizeof(double), hipMemcpyDeviceToDevice, s));
+}
+
+template<>
+void precond_apply<float>(const nlf_precond_s* P, i64 n, const float* r, float* z, const CGState* st, hipStream_t s) {
+    if (P && P->kind == NLF_PRECOND_JACOBI)
+        hipLaunchKernelGGL(k_scale<float>, dim3(grid_stride_blocks(n)), dim3(256), 0, s, n, P->invd32.p, r, z);
+    else if (P && P->kind == NLF_PRECOND_ILLT)
+        hipLaunchKernelGGL((k_ic_solve<float, float>), dim3(1), dim3(LEVEL_THREADS), 0, s, fw_args(P), bw_args(P), P->F32.p,
+                           P->scale32.p, r, P->work32.p, z, st);
+    else
+        NLF_HIP(hipMemcpyAsync(z, r, size_t(n) * sizeof(float), hipMemcpyDeviceToDevice, s));
+}
+
+int pick_lpr(double avg) {
+    if (avg > 96) return 64;
+    if (avg > 48) return 32;
+    if (avg > 24) return 16;
+    if (avg > 12) return 8;
+    return 4;
+}
+
+template<class V, class X, class Acc, bool DOT>
+void launch_spmv(int lpr, i64 rows, const i64* outer, const i32* cols, const V* val, const X* x, X* y, double* partial,
+                 const CGState* st, hipStream_t s) {
+    const unsigned blocks = unsigned((rows + (SPMV_THREADS / lpr) - 1) / (SPMV_THREADS / lpr));
+#define NLF_SPMV_CASE(L)                                                                                           \
+    case L:                                                                                                        \
+        hipLaunchKernelGGL((k_spmv<L, V, X, Acc, DOT>), dim3(blocks), dim3(SPMV_THREADS), 0, s, rows, outer, cols, val, \
+                           x, y, partial, st);                                                                     \
+        break;
+    switch (lpr) {
+        NLF_SPMV_CASE(4)
+        NLF_SPMV_CASE(8)
+        NLF_SPMV_CASE(16)
+        NLF_SPMV_CASE(32)
+        default: NLF_SPMV_CASE(64)
+    }
+#undef NLF_SPMV_CASE
+}
+
+unsigned spmv_blocks(int lpr, i64 rows) { return unsigned((rows + (SPMV_THREADS / lpr) - 1) / (SPMV_THREADS / lpr)); }
+
+// Pairs of HIP events around sampled kernel launches on the solver's stream (slot 0: SpMV, 1: preconditioner).
+struct KernelTimer {
+    std::vector<hipEvent_t> ev[2][2];
+    double total_ms[2] = {0.0, 0.0};
+    i64 count[2] = {0, 0};
+    ~KernelTimer() {
+        for (auto& a : ev)
+            for (auto& b : a)
+                for (hipEvent_t e : b) (void)hipEventDestroy(e);
+    }
+    void start(int slot, hipStream_t s) { record(slot, 0, s); }
+    void stop(int slot, hipStream_t s) { record(slot, 1, s); }
+    void record(int slot, int which, hipStream_t s) {
+        hipEvent_t e;
+        NLF_HIP(hipEventCreate(&e));
+        NLF_HIP(hipEventRecord(e, s));
+        ev[slot][which].push_back(e);
+    }
+    void collect() {
+        for (int k = 0; k < 2; ++k)
+            for (size_t i = 0; i < ev[k][1].size(); ++i) {
+                float ms = 0.f;
+                NLF_HIP(hipEventSynchronize(ev[k][1][i]));
+                NLF_HIP(hipEventElapsedTime(&ms, ev[k][0][i], ev[k][1][i]));
+                total_ms[k] += ms;
+                count[k] += 1;
+            }
+        for (auto& a : ev)
+            for (auto& b : a) {
+                for (hipEvent_t e : b) (void)hipEventDestroy(e);
+                b.clear();
+            }
+    }
+    double avg(int k) const { return count[k] ? total_ms[k] / double(count[k]) : 0.0; }
+};
+
+// Preconditioned CG on device vectors (conjugate_gradient::solve).  Vectors of type X, matrix values V.
+template<class V, class X>
+struct CG {
+    hipStream_t s;
+    i64 n;
+    int lpr;
+    const i64* outer;
+    const i32* cols;
+    const V* val;
+    const nlf_precond_s* P;
+    DevBuf<X> r, z, p;
+    DevBuf<double> part1, part2;
+    DevBuf<CGState> st;
+    unsigned sblocks, vblocks;
+    KernelTimer timer;
+
+    CG(hipStream_t s_, i64 n_, i64 nnz, const i64* o, const i32* c, const V* v, const nlf_precond_s* P_)
+        : s(s_), n(n_), outer(o), cols(c), val(v), P(P_) {
+        lpr = pick_lpr(double(nnz) / double(std::max<i64>(n, 1)));
+        r.alloc(size_t(n)); z.alloc(size_t(n)); p.alloc(size_t(n));
+        sblocks = spmv_blocks(lpr, n);
+        vblocks = grid_stride_blocks(n);
+        // k_spmv (sblocks), vector kernels (vblocks) and k_residual (one wave per row) write partials
+        const size_t np = std::max<size_t>(std::max<size_t>(sblocks, vblocks), spmv_blocks(64, n));
+        part1.alloc(np);
+        part2.alloc(np);
+        st.alloc(1);
+    }
+
+    bool jacobi() const { return P && P->kind == NLF_PRECOND_JACOBI; }
+
+    const X* invd() const;
+
+    // x: initial guess in, solution out; b: right-hand side.  Returns iterations, sets residual.
+    i64 solve(const X* b, X* x, double tol, i64 max_it, double* residual, int* breakdown, int check_every = 8) {
+        // r = b - A x ; z = M r ; rz = r.z ; bnorm
+        hipLaunchKernelGGL((k_residual<V, X>), dim3(spmv_blocks(64, n)), dim3(SPMV_THREADS), 0, s, n, outer, cols, val, x, b,
+                           r.p, part1.p, part2.p);
+        const int nb_res = int(spmv_blocks(64, n));
+        DevBuf<double> pbb{size_t(nb_res)};
+        NLF_HIP(hipMemcpyAsync(pbb.p, part2.p, size_t(nb_res) * sizeof(double), hipMemcpyDeviceToDevice, s));
+        precond_apply<X>(P, n, r.p, p.p, nullptr, s);
+        hipLaunchKernelGGL(k_dot<X>, dim3(vblocks), dim3(SPMV_THREADS), 0, s, n, r.p, p.p, part1.p, (const CGState*)nullptr);
+        // pad partial arrays: k_init_state sums vblocks of r.z and nb_res of b.b
+        DevBuf<double> prz{size_t(std::max<unsigned>(vblocks, unsigned(nb_res)))};
+        NLF_HIP(hipMemsetAsync(prz.p, 0, prz.bytes(), s));
+        NLF_HIP(hipMemcpyAsync(prz.p, part1.p, size_t(vblocks) * sizeof(double), hipMemcpyDeviceToDevice, s));
+        DevBuf<double> pb2{size_t(std::max<unsigned>(vblocks, unsigned(nb_res)))};
+        NLF_HIP(hipMemsetAsync(pb2.p, 0, pb2.bytes(), s));
+        NLF_HIP(hipMemcpyAsync(pb2.p, pbb.p, size_t(nb_res) * sizeof(double), hipMemcpyDeviceToDevice, s));
+        hipLaunchKernelGGL(k_init_state, dim3(1), dim3(1024), 0, s, prz.p, pb2.p, int(prz.n), tol, max_it, st.p);
+        CGState hs{};
+        NLF_HIP(hipMemcpyAsync(&hs, st.p, sizeof(CGState), hipMemcpyDeviceToHost, s));
+        NLF_HIP(hipStreamSynchronize(s));
+        while (!hs.done) {
+            // the first iteration of each chunk starts with done == 0, so its kernels run in full:
+            // HIP events around its SpMV and preconditioner give the live kernel timings
+            for (int k = 0; k < check_every; ++k) iteration(x, k == 0);
+            NLF_HIP(hipMemcpyAsync(&hs, st.p, sizeof(CGState), hipMemcpyDeviceToHost, s));
+            NLF_HIP(hipStreamSynchronize(s));
+        }
+        timer.collect();
+        if (residual) *residual = hs.res;
+        if (breakdown) *breakdown = hs.breakdown;
+        return hs.it;
+    }
+
+    void iteration(X* x, bool sample = false) {
+        // z = A p ; alpha = rz / p.z
+        if (sample) timer.start(0, s);
+        launch_spmv<V, X, double, true>(lpr, n, outer, cols, val, p.p, z.p, part1.p, st.p, s);
+        if (sample) timer.stop(0, s);
+        hipLaunchKernelGGL(k_alpha, dim3(1), dim3(1024), 0, s, part1.p, int(sblocks), st.p);
+        if (!P || P->kind == NLF_PRECOND_IDENTITY || jacobi()) {
+            // x += a p ; r -= a z ; z = M r ; partial r.z  (fused)
+            if (jacobi())
+                hipLaunchKernelGGL((k_update_xr<X, true, true>), dim3(vblocks), dim3(SPMV_THREADS), 0, s, n, x, r.p, p.p, z.p,
+                                   z.p, invd(), part2.p, st.p);
+            else
+                hipLaunchKernelGGL((k_update_xr<X, false, true>), dim3(vblocks), dim3(SPMV_THREADS), 0, s, n, x, r.p, p.p,
+                                   z.p, z.p, (const X*)nullptr, part2.p, st.p);
+        } else {
+            hipLaunchKernelGGL((k_update_xr<X, false, false>), dim3(vblocks), dim3(SPMV_THREADS), 0, s, n, x, r.p, p.p, z.p,
+                               (X*)nullptr, (const X*)nullptr, part2.p, st.p);
+            if (sample) timer.start(1, s);
+            precond_apply<X>(P, n, r.p, z.p, st.p, s);
+            if (sample) timer.stop(1, s);
+            hipLaunchKernelGGL(k_dot<X>, dim3(vblocks), dim3(SPMV_THREADS), 0, s, n, r.p, z.p, part2.p, st.p);
+        }
+        hipLaunchKernelGGL(k_beta, dim3(1), dim3(1024), 0, s, part2.p, int(vblocks), st.p);
+        hipLaunchKernelGGL(k_update_p<X>, dim3(vblocks), dim3(256), 0, s, n, p.p, z.p, st.p);
+    }
+};
+
+template<> const double* CG<double, double>::invd() const { return P->invd.p; }
+template<> const float* CG<float, float>::invd() const { return P->invd32.p; }
+
+}  // namespace nlf
+
+using namespace nlf;
+
+extern "C" {
+
+// y = A x on the inner part (the reference's selfadjointView<Upper>() product), device vectors.
+int nlf_spmv(nlf_matrix a, const double* x, double* y) {
+    if (!a) return NLF_ERR_ARGUMENT;
+    return guarded(a->ctx, [&] {
+        const int lpr = pick_lpr(double(a->nnz_full) / double(std::max<i64>(a->rows, 1)));
+        launch_spmv<double, double, double, false>(lpr, a->rows, a->outer.p, a->cols.p, a->val.p, x, y, nullptr, nullptr,
+                                                   a->ctx->stream);
+        NLF_HIP(hipGetLastError());
+    });
+}
+
+// FP32 SpMV on the FP32 copy (nlf_matrix_make_fp32 first).
+int nlf_spmv_fp32(nlf_matrix a, const float* x, float* y) {
+    if (!a) return NLF_ERR_ARGUMENT;
+    return guarded(a->ctx, [&] {
+        NLF_CHECK(a->val32.p, NLF_ERR_STATE, "no FP32 copy");
+        const int lpr = pick_lpr(double(a->nnz_full) / double(std::max<i64>(a->rows, 1)));
+        launch_spmv<float, float, float, false>(lpr, a->rows, a->outer.p, a->cols.p, a->val32.p, x, y, nullptr, nullptr,
+                                                a->ctx->stream);
+        NLF_HIP(hipGetLastError());
+    });
+}
+
+// Same SpMV with an explicit lanes-per-row choice (autotuning / benchmarking).
+int nlf_spmv_lpr(nlf_matrix a, int lpr, int fp32, const void* x, void* y) {
+    if (!a) return NLF_ERR_ARGUMENT;
+    return guarded(a->ctx, [&] {
+        if (fp32) launch_spmv<float, float, float, false>(lpr, a->rows, a->outer.p, a->cols.p, a->val32.p,
+                                                          static_cast<const float*>(x), static_cast<float*>(y), nullptr,
+                                                          nullptr, a->ctx->stream);
+        else launch_spmv<double, double, double, false>(lpr, a->rows, a->outer.p, a->cols.p, a->val.p,
+                                                        static_cast<const double*>(x), static_cast<double*>(y), nullptr,
+                                                        nullptr, a->ctx->stream);
+        NLF_HIP(hipGetLastError());
+    });
+}
+
+// f -= K_bound x_D; f[D] = x_D  (boundary_condition_first_kind_2d), device vectors.
+int nlf_apply_first_kind(nlf_matrix a, const double* xD, double* f) {
+    if (!a) return NLF_ERR_ARGUMENT;
+    return guarded(a->ctx, [&] {
+        hipLaunchKernelGGL(k_first_kind, dim3(blocks_for(a->rows, 256)), dim3(256), 0, a->ctx->stream, a->rows,
+                           a->b_outer.p, a->b_cols.p, a->b_val.p, a->inner.p, xD, f);
+        NLF_HIP(hipGetLastError());
+    });
+}
+
+// init_preconditioner<...> (iterative_solver_base.hpp:203-207): JACOBI from A's diagonal, ILLT from
+// the matrix passed (the reference factors the LOCAL matrix, stationary_heat_equation_solver_2d.hpp:91-96).
+int nlf_precond_create(nlf_matrix a, int kind, nlf_precond* out) {
+    if (!out) return NLF_ERR_ARGUMENT;
+    nlf_ctx ctx = a ? a->ctx : nullptr;
+    return guarded(ctx, [&] {
+        const auto t0 = std::chrono::steady_clock::now();
+        auto P = std::make_unique<nlf_precond_s>();
+        P->ctx = ctx;
+        P->kind = kind;
+        if (kind == NLF_PRECOND_IDENTITY) { P->rows = a ? a->rows : 0; *out = P.release(); return; }
+        NLF_CHECK(a, NLF_ERR_ARGUMENT, "matrix required");
+        hipStream_t s = ctx->stream;
+        P->rows = a->rows;
+        const i64 n = a->rows;
+        if (kind == NLF_PRECOND_JACOBI) {
+            P->invd.alloc(size_t(n));
+            P->invd32.alloc(size_t(n));
+            hipLaunchKernelGGL(k_inv_diag, dim3(blocks_for(n, 256)), dim3(256), 0, s, n, a->diag.p, a->val.p, P->invd.p,
+                               P->invd32.p);
+            NLF_HIP(hipGetLastError());
+        } else if (kind == NLF_PRECOND_ILLT) {
+            // pattern copy + level sets (host analysis of the pattern only)
+            std::vector<i64> outer(n + 1), diag(n);
+            std::vector<i32> cols(size_t(a->nnz_full));
+            download(outer.data(), a->outer.p, outer.size(), s);
+            download(diag.data(), a->diag.p, diag.size(), s);
+            download(cols.data(), a->cols.p, cols.size(), s);
+            std::vector<i32> fr, fp, br, bp;
+            level_sets(outer, cols, diag, fr, fp, br, bp);
+            P->fw_levels = int(fp.size()) - 1;
+            P->bw_levels = int(bp.size()) - 1;
+            upload(P->outer, outer.data(), outer.size(), s);
+            upload(P->diag, diag.data(), diag.size(), s);
+            upload(P->cols, cols.data(), cols.size(), s);
+            upload(P->fw_rows, fr.data(), fr.size(), s);
+            upload(P->fw_ptr, fp.data(), fp.size(), s);
+            upload(P->bw_rows, br.data(), br.size(), s);
+            upload(P->bw_ptr, bp.data(), bp.size(), s);
+            P->tpos.alloc(cols.size());
+            hipLaunchKernelGGL(k_transpose_pos, dim3(blocks_for(n, 4)), dim3(256), 0, s, n, P->outer.p, P->cols.p, P->tpos.p);
+            P->scale.alloc(size_t(n));
+            P->scale32.alloc(size_t(n));
+            P->work.alloc(size_t(n));
+            P->work32.alloc(size_t(n));
+            P->F.alloc(cols.size());
+            hipLaunchKernelGGL(k_scale_from_diag, dim3(blocks_for(n, 256)), dim3(256), 0, s, n, P->diag.p, a->val.p,
+                               P->scale.p, P->scale32.p);
+            DevBuf<int> fail{1};
+            double shift = 0.0;
+            P->ok = false;
+            for (int attempt = 0; attempt < 40; ++attempt) {
+                NLF_HIP(hipMemsetAsync(fail.p, 0, sizeof(int), s));
+                hipLaunchKernelGGL(k_ic_init, dim3(blocks_for(n, 256)), dim3(256), 0, s, n, P->outer.p, P->cols.p, P->diag.p,
+                                   a->val.p, P->scale.p, shift, P->F.p);
+                hipLaunchKernelGGL(k_ic_factor, dim3(1), dim3(LEVEL_THREADS), 0, s, fw_args(P.get()), P->F.p, fail.p);
+                NLF_HIP(hipGetLastError());
+                int f = 0;
+                download(&f, fail.p, 1, s);
+                if (!f) { P->ok = true; P->shift = shift; break; }
+                shift = shift == 0.0 ? 1e-3 : 2 * shift;
+            }
+            P->F32.alloc(cols.size());
+            hipLaunchKernelGGL((k_convert<double, float>), dim3(grid_stride_blocks(i64(cols.size()))), dim3(256), 0, s,
+                               i64(cols.size()), P->F.p, P->F32.p);
+            NLF_HIP(hipGetLastError());
+        } else {
+            throw Error(NLF_ERR_UNSUPPORTED, "use nlf_learned_precond_create for the learned preconditioner");
+        }
+        NLF_HIP(hipStreamSynchronize(s));
+        P->setup_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        *out = P.release();
+    });
+}
+
+int nlf_precond_destroy(nlf_precond p) {
+    delete p;
+    return NLF_OK;
+}
+
+// computation_info(): ok = 1 on success; shift used; number of forward / backward levels.
+int nlf_precond_info(nlf_precond p, int* ok, double* shift, int* fw_levels, int* bw_levels, double* setup_seconds) {
+    if (!p) return NLF_ERR_ARGUMENT;
+    if (ok) *ok = p->ok;
+    if (shift) *shift = p->shift;
+    if (fw_levels) *fw_levels = p->fw_levels;
+    if (bw_levels) *bw_levels = p->bw_levels;
+    if (setup_seconds) *setup_seconds = p->setup_seconds;
+    return NLF_OK;
+}
+
+// preconditioner_base::solve: z = M^-1 r on device vectors.
+int nlf_precond_apply(nlf_precond p, const double* r, double* z) {
+    if (!p) return NLF_ERR_ARGUMENT;
+    return guarded(p->ctx, [&] {
+        NLF_CHECK(p->rows > 0, NLF_ERR_STATE, "preconditioner has no size");
+        precond_apply<double>(p, p->rows, r, z, nullptr, p->ctx->stream);
+        NLF_HIP(hipGetLastError());
+    });
+}
+
+// Host export of the ILLT factor in full storage (upper: U, lower: U^T) and the scaling.
+int nlf_precond_export_factor(nlf_precond p, double* values, double* scale) {
+    if (!p || p->kind != NLF_PRECOND_ILLT) return NLF_ERR_ARGUMENT;
+    return guarded(p->ctx, [&] {
+        if (values) download(values, p->F.p, p->F.n, p->ctx->stream);
+        if (scale) download(scale, p->scale.p, p->scale.n, p->ctx->stream);
+    });
+}
+
+// conjugate_gradient::solve on device vectors.  mode NLF_MODE_FP64: the reference's PCG with its
+// stopping rule sqrt(r'z)/||b|| <= tol.  NLF_MODE_MIXED: FP64 iterative refinement around FP32 PCG
+// (x += A32^-1 (b - A x)) until the FP64 true residual ||b - Ax||/||b|| <= tol.  x holds the initial
+// guess on entry.
+int nlf_cg_solve(nlf_matrix a, nlf_precond p, const double* b, double* x, double tol, int64_t max_it, int mode,
+                 nlf_solve_stats* stats) {
+    if (!a) return NLF_ERR_ARGUMENT;
+    return guarded(a->ctx, [&] {
+        NLF_CHECK(b && x, NLF_ERR_ARGUMENT, "null vector");
+        NLF_CHECK(!p || p->kind != NLF_PRECOND_LEARNED, NLF_ERR_ARGUMENT, "the learned preconditioner needs nlf_fcg_solve");
+        hipStream_t s = a->ctx->stream;
+        const auto t0 = std::chrono::steady_clock::now();
+        const i64 n = a->rows;
+        nlf_solve_stats out{};
+        out.preconditioner = p ? p->kind : NLF_PRECOND_IDENTITY;
+        int breakdown = 0;
+        auto timings = [&](const KernelTimer& t, size_t sv, size_t sx) {
+            out.spmv_ms = t.avg(0);
+            out.precond_ms = t.avg(1);
+            out.spmv_samples = t.count[0];
+            out.spmv_bytes = a->nnz_full * i64(sv + sizeof(i32)) + n * i64(sizeof(i64) + 2 * sx);
+        };
+        if (mode == NLF_MODE_FP64) {
+            CG<double, double> cg(s, n, a->nnz_full, a->outer.p, a->cols.p, a->val.p, p);
+            out.iterations = cg.solve(b, x, tol, max_it, &out.residual, &breakdown);
+            timings(cg.timer, sizeof(double), sizeof(double));
+        } else {
+            NLF_CHECK(a->val32.p, NLF_ERR_STATE, "mixed mode needs nlf_matrix_make_fp32 first");
+            CG<float, float> cg(s, n, a->nnz_full, a->outer.p, a->cols.p, a->val32.p, p);
+            DevBuf<double> r{size_t(n)};
+            DevBuf<float> r32{size_t(n)}, d32{size_t(n)};
+            const unsigned rb = spmv_blocks(64, n);
+            DevBuf<double> prr{rb}, pbb{rb}, sums{2};
+            const double inner_tol = 1e-5;
+            for (i64 outer = 0; outer < 100; ++outer) {
+                hipLaunchKernelGGL((k_residual<double, double>), dim3(rb), dim3(SPMV_THREADS), 0, s, n, a->outer.p, a->cols.p,
+                                   a->val.p, x, b, r.p, prr.p, pbb.p);
+                hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(1024), 0, s, prr.p, int(rb), sums.p);
+                hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(1024), 0, s, pbb.p, int(rb), sums.p + 1);
+                double h[2];
+                download(h, sums.p, 2, s);
+                out.true_residual = std::sqrt(h[0]) / std::sqrt(h[1]);
+                if (out.true_residual <= tol || out.iterations >= max_it) break;
+                out.outer_iterations++;
+                hipLaunchKernelGGL((k_convert<double, float>), dim3(grid_stride_blocks(n)), dim3(256), 0, s, n, r.p, r32.p);
+                hipLaunchKernelGGL(k_zero<float>, dim3(grid_stride_blocks(n)), dim3(256), 0, s, n, d32.p);
+                double res = 0.0;
+                out.iterations += cg.solve(r32.p, d32.p, inner_tol, max_it - out.iterations, &res, &breakdown);
+                out.residual = res;
+                // x += d (FP64 accumulation)
+                hipLaunchKernelGGL((k_add_f32<double>), dim3(grid_stride_blocks(n)), dim3(256), 0, s, n, d32.p, x);
+                NLF_HIP(hipGetLastError());
+            }
+            timings(cg.timer, sizeof(float), sizeof(float));
+        }
+        // FP64 true residual ||b - A x|| / ||b||
+        {
+            const unsigned rb = spmv_blocks(64, n);
+            DevBuf<double> r{size_t(n)}, prr{rb}, pbb{rb}, sums{2};
+            hipLaunchKernelGGL((k_residual<double, double>), dim3(rb), dim3(SPMV_THREADS), 0, s, n, a->outer.p, a->cols.p,
+                               a->val.p, x, b, r.p, prr.p, pbb.p);
+            hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(1024), 0, s, prr.p, int(rb), sums.p);
+            hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(1024), 0, s, pbb.p, int(rb), sums.p + 1);
+            double h[2];
+            download(h, sums.p, 2, s);
+            out.true_residual = h[1] > 0 ? std::sqrt(h[0]) / std::sqrt(h[1]) : 0.0;
+        }
+        out.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        if (stats) *stats = out;
+        NLF_CHECK(!breakdown, NLF_ERR_NUMERIC, "CG breakdown (non-positive curvature or non-finite residual)");
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,1115 @@
+// ORACLE -- test infrastructure only.
+//
+// FP64 OpenMP C++ restatement of NonLocFEM's 2D nonlocal hot path (VIXER13/nonlocal), used by
+// tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg as the checker and the CPU
+// baseline ("reference algorithm, restated").  It is never linked into or called by the product
+// library.  Each function cites the reference file:line it follows.
+//
+// Parity status of each part (what pins it) is listed in DESIGN.md; in short:
+//   * element tables, mesh precompute, centres, neighbour search, pattern: pinned by closed forms,
+//     brute force and invariants (tests/test_oracle_*.py);
+//   * assembly: pinned by the textbook Q4 element matrix (p1 = 1), the constant-influence rank-2
+//     closed form, K*1 = 0, symmetry and dense brute force;
+//   * IC(0): our own choice of incomplete Cholesky (Eigen::IncompleteCholesky is not vendored) --
+//     its factor is parity unpinned; only the solution it leads to is checked.
+//
+// Compile with -ffp-contract=off: the neighbour predicate must be evaluated op-for-op like the
+// reference (search_function.hpp:28-63 -> distance.hpp:17-23 -> norm.hpp:15-22).
+
+#include "../include/nlf.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <limits>
+#include <numeric>
+#include <set>
+#include <stdexcept>
+#include <string>
+#include <vector>
+#include <omp.h>
+
+namespace {
+
+using i64 = int64_t;
+using i32 = int32_t;
+
+// ---------------------------------------------------------------------------------------------
+// Finite-element tables.  Quadrature points follow element_2d_integrate::set_quadrature
+// (element_2d_integrate.hpp:69-106): q = ix*ny + iy, x = -1 + (g + 1)*1, w = wx*1*wy*1.
+// Gauss rules: gaussian_quadrature.hpp:22-40.  Bases: serendipity_2d.hpp:36-114 (p = 2/9).
+// ---------------------------------------------------------------------------------------------
+struct Tables2D {
+    int nn = 0, nq = 0;
+    std::vector<double> w, qN, qNxi, qNeta;   // [i*nq + q]
+    std::vector<double> N0;                   // N_i(0, 0) (centre weights)
+};
+
+void gauss_rule(int k, std::vector<double>& x, std::vector<double>& w) {
+    if (k == 1) { x = {0.0}; w = {2.0}; }
+    else if (k == 2) { const double s = std::sqrt(3.0); x = {-1.0 / s, 1.0 / s}; w = {1.0, 1.0}; }
+    else if (k == 3) { const double s = std::sqrt(3.0 / 5.0); x = {-s, 0.0, s}; w = {5.0 / 9.0, 8.0 / 9.0, 5.0 / 9.0}; }
+    else throw std::domain_error("unsupported Gauss rule");
+}
+
+// Q4 bilinear basis (serendipity<T,1>).
+void q4_basis(double x, double y, double* N, double* Nx, double* Ny) {
+    N[0] = (1 - x) * (1 - y) / 4; N[1] = (1 + x) * (1 - y) / 4; N[2] = (1 + x) * (1 + y) / 4; N[3] = (1 - x) * (1 + y) / 4;
+    Nx[0] = -(1 - y) / 4; Nx[1] = (1 - y) / 4; Nx[2] = (1 + y) / 4; Nx[3] = -(1 + y) / 4;
+    Ny[0] = -(1 - x) / 4; Ny[1] = -(1 + x) / 4; Ny[2] = (1 + x) / 4; Ny[3] = (1 - x) / 4;
+}
+
+// Q8 serendipity basis with parameter p (serendipity<T,2>, p = 2/9): closed-form derivatives.
+void q8_basis(double x, double y, double* N, double* Nx, double* Ny) {
+    const double p = 2.0 / 9.0;
+    const double a = 9 * p - 1, b = 9 * p + 3, c = 9 * p - 5;
+    // corners: s = (1 -/+ x)(1 -/+ y)/16 * (a*(1 + sx*x + sy*y) + sx*sy*b*x*y) with signs of the corner
+    const int sxs[4] = {-1, 1, 1, -1}, sys[4] = {-1, -1, 1, 1};
+    const int cid[4] = {0, 2, 4, 6};
+    for (int k = 0; k < 4; ++k) {
+        // corner (sx, sy): (1 + sx x)(1 + sy y)(a (1 - sx x - sy y) + b sx sy x y)/16, e.g. corner
+        // (-1,-1) is (1-x)(1-y)((9p-1)(1+x+y) + (9p+3)xy)/16 (serendipity_2d.hpp:102)
+        const double sx = sxs[k], sy = sys[k];
+        const double X = 1 + sx * x, Y = 1 + sy * y;
+        const double gg = a * (1 - sx * x - sy * y) + b * sx * sy * x * y;
+        N[cid[k]] = X * Y * gg / 16;
+        const double dgx = -a * sx + b * sx * sy * y, dgy = -a * sy + b * sx * sy * x;
+        Nx[cid[k]] = (sx * Y * gg + X * Y * dgx) / 16;
+        Ny[cid[k]] = (X * sy * gg + X * Y * dgy) / 16;
+    }
+    // midsides on y = -1 (m01) and y = +1 (m23): -(1-x^2)(1 -/+ y)(c +/- b*y)/16
+    {   // m01: -(1-x^2)(1-y)(c + b y)/16
+        const double f = -(1 - x * x), Y = 1 - y, g = c + b * y;
+        N[1] = f * Y * g / 16; Nx[1] = (2 * x) * Y * g / 16; Ny[1] = f * (-g + Y * b) / 16;
+    }
+    {   // m23: -(1-x^2)(1+y)(c - b y)/16
+        const double f = -(1 - x * x), Y = 1 + y, g = c - b * y;
+        N[5] = f * Y * g / 16; Nx[5] = (2 * x) * Y * g / 16; Ny[5] = f * (g - Y * b) / 16;
+    }
+    {   // m12: -(1-y^2)(1+x)(c - b x)/16
+        const double f = -(1 - y * y), X = 1 + x, g = c - b * x;
+        N[3] = f * X * g / 16; Ny[3] = (2 * y) * X * g / 16; Nx[3] = f * (g - X * b) / 16;
+    }
+    {   // m30: -(1-y^2)(1-x)(c + b x)/16
+        const double f = -(1 - y * y), X = 1 - x, g = c + b * x;
+        N[7] = f * X * g / 16; Ny[7] = (2 * y) * X * g / 16; Nx[7] = f * (-g + X * b) / 16;
+    }
+}
+
+void basis(int nn, double x, double y, double* N, double* Nx, double* Ny) {
+    if (nn == 4) q4_basis(x, y, N, Nx, Ny); else q8_basis(x, y, N, Nx, Ny);
+}
+
+Tables2D make_tables(int nn) {
+    Tables2D t;
+    t.nn = nn;
+    std::vector<double> g, gw;
+    gauss_rule(nn == 4 ? 2 : 3, g, gw);
+    const int k = int(g.size());
+    t.nq = k * k;
+    t.w.resize(t.nq); t.qN.resize(nn * t.nq); t.qNxi.resize(nn * t.nq); t.qNeta.resize(nn * t.nq);
+    std::vector<double> px(k), py(k);
+    for (int i = 0; i < k; ++i) { px[i] = -1.0 + (g[i] - (-1.0)) * 1.0; py[i] = px[i]; }
+    for (int i = 0; i < k; ++i)
+        for (int j = 0; j < k; ++j) t.w[i * k + j] = gw[i] * 1.0 * gw[j] * 1.0;
+    double N[8], Nx[8], Ny[8];
+    for (int i = 0; i < k; ++i)
+        for (int j = 0; j < k; ++j) {
+            basis(nn, px[i], py[j], N, Nx, Ny);
+            for (int a = 0; a < nn; ++a) {
+                t.qN[a * t.nq + i * k + j] = N[a];
+                t.qNxi[a * t.nq + i * k + j] = Nx[a];
+                t.qNeta[a * t.nq + i * k + j] = Ny[a];
+            }
+        }
+    basis(nn, 0.0, 0.0, N, Nx, Ny);
+    t.N0.assign(N, N + nn);
+    if (nn == 8) {   // exact reference weights (9p-1)/16 and -(9p-5)/16 at xi0 = 0
+        const double p = 2.0 / 9.0;
+        for (int a = 0; a < 8; a += 2) { t.N0[a] = (9 * p - 1) / 16; t.N0[a + 1] = -(9 * p - 5) / 16; }
+    }
+    return t;
+}
+
+// 1D boundary elements: linear with Gauss 1, quadratic (nodes -1, 0, 1) with Gauss 2
+// (vtk_elements_set.hpp:43-52; lagrangian_elements_1d.hpp).
+struct Tables1D { int nb = 0, nq = 0; std::vector<double> w, qN, qNxi; };
+
+Tables1D make_tables_1d(int nb) {
+    Tables1D t;
+    t.nb = nb;
+    std::vector<double> g, gw;
+    gauss_rule(nb == 2 ? 1 : 2, g, gw);
+    t.nq = int(g.size());
+    t.w = gw;
+    t.qN.resize(nb * t.nq); t.qNxi.resize(nb * t.nq);
+    for (int q = 0; q < t.nq; ++q) {
+        const double x = g[q];
+        if (nb == 2) {
+            t.qN[0 * t.nq + q] = (1 - x) / 2; t.qN[1 * t.nq + q] = (1 + x) / 2;
+            t.qNxi[0 * t.nq + q] = -0.5; t.qNxi[1 * t.nq + q] = 0.5;
+        } else {
+            t.qN[0 * t.nq + q] = x * (x - 1) / 2; t.qN[1 * t.nq + q] = 1 - x * x; t.qN[2 * t.nq + q] = x * (x + 1) / 2;
+            t.qNxi[0 * t.nq + q] = x - 0.5; t.qNxi[1 * t.nq + q] = -2 * x; t.qNxi[2 * t.nq + q] = x + 0.5;
+        }
+    }
+    return t;
+}
+
+// ---------------------------------------------------------------------------------------------
+// Mesh + precompute (mesh_2d::init, mesh_2d.hpp:96-108; mesh_container_2d_utils.hpp:17-132)
+// ---------------------------------------------------------------------------------------------
+struct Mesh {
+    i64 N = 0, E = 0;
+    int nn = 0, nb = 0;
+    std::vector<double> xy;              // 2N
+    std::vector<i32> el;                 // E*nn
+    std::vector<i64> gbeg;               // G+1
+    std::vector<i32> be;                 // B*nb
+    std::vector<i64> bbeg;               // GB+1
+    Tables2D t;
+    Tables1D t1;
+    std::vector<i64> ne_off;             // node -> elements (ascending)
+    std::vector<i32> ne_ids;
+    std::vector<double> xq;              // 2*E*nq
+    std::vector<double> jac;             // E*nq
+    std::vector<double> D;               // 2*E*nn*nq, layout [e][i][q] (mesh_2d.hpp:161-179)
+    std::vector<double> centres;         // 2E
+    // neighbours
+    std::vector<i64> nb_off;
+    std::vector<i32> nb_ids;
+    std::vector<double> expanded;        // 2G
+    std::vector<int> group_of;           // E
+
+    int G() const { return int(gbeg.size()) - 1; }
+    const i32* nodes(i64 e) const { return &el[size_t(e) * nn]; }
+    int local(i64 e, i64 node) const {
+        for (int i = 0; i < nn; ++i) if (el[size_t(e) * nn + i] == node) return i;
+        return -1;
+    }
+    double Dx(i64 e, int i, int q) const { return D[2 * ((size_t(e) * nn + i) * t.nq + q)]; }
+    double Dy(i64 e, int i, int q) const { return D[2 * ((size_t(e) * nn + i) * t.nq + q) + 1]; }
+    const double* xqp(i64 e, int q) const { return &xq[2 * (size_t(e) * t.nq + q)]; }
+};
+
+void mesh_init(Mesh& m) {
+    m.t = make_tables(m.nn);
+    m.t1 = make_tables_1d(m.nb);
+    const int nq = m.t.nq, nn = m.nn;
+    // node_elements_2d (mesh_container_2d_utils.hpp:17-26)
+    m.ne_off.assign(m.N + 1, 0);
+    for (i64 e = 0; e < m.E; ++e) for (int i = 0; i < nn; ++i) m.ne_off[m.el[e * nn + i] + 1]++;
+    for (i64 n = 0; n < m.N; ++n) m.ne_off[n + 1] += m.ne_off[n];
+    m.ne_ids.resize(m.ne_off[m.N]);
+    std::vector<i64> fill(m.ne_off.begin(), m.ne_off.end() - 1);
+    for (i64 e = 0; e < m.E; ++e) for (int i = 0; i < nn; ++i) m.ne_ids[fill[m.el[e * nn + i]]++] = i32(e);
+    m.group_of.resize(m.E);
+    for (int g = 0; g < m.G(); ++g) for (i64 e = m.gbeg[g]; e < m.gbeg[g + 1]; ++e) m.group_of[e] = g;
+    // quad coords, Jacobi matrices, |det J|, derivatives (mesh_container_2d_utils.hpp:75-123;
+    // element_data_2d::quad_coord / jacobi_matrix, mesh_container_2d.hpp:141-160)
+    m.xq.assign(2 * m.E * nq, 0.0);
+    m.jac.assign(m.E * nq, 0.0);
+    m.D.assign(2 * m.E * nn * nq, 0.0);
+    m.centres.assign(2 * m.E, 0.0);
+#pragma omp parallel for schedule(static)
+    for (i64 e = 0; e < m.E; ++e) {
+        const i32* v = m.nodes(e);
+        for (int q = 0; q < nq; ++q) {
+            double cx = 0, cy = 0, J00 = 0, J01 = 0, J10 = 0, J11 = 0;
+            for (int i = 0; i < nn; ++i) {
+                const double x = m.xy[2 * v[i]], y = m.xy[2 * v[i] + 1];
+                const double N = m.t.qN[i * nq + q], Nx = m.t.qNxi[i * nq + q], Ny = m.t.qNeta[i * nq + q];
+                cx = cx + x * N; cy = cy + y * N;
+                J00 = J00 + x * Nx; J01 = J01 + x * Ny;
+                J10 = J10 + y * Nx; J11 = J11 + y * Ny;
+            }
+            m.xq[2 * (e * nq + q)] = cx; m.xq[2 * (e * nq + q) + 1] = cy;
+            m.jac[e * nq + q] = std::abs(J00 * J11 - J01 * J10);
+            for (int i = 0; i < nn; ++i) {
+                const double Nx = m.t.qNxi[i * nq + q], Ny = m.t.qNeta[i * nq + q];
+                m.D[2 * ((e * nn + i) * nq + q)] = Nx * J11 - Ny * J10;
+                m.D[2 * ((e * nn + i) * nq + q) + 1] = -Nx * J01 + Ny * J00;
+            }
+        }
+        // centre (mesh_container_2d.hpp:130-139): xi0 = 0 for every quad
+        double cx = 0, cy = 0;
+        for (int i = 0; i < nn; ++i) {
+            cx = cx + m.xy[2 * v[i]] * m.t.N0[i];
+            cy = cy + m.xy[2 * v[i] + 1] * m.t.N0[i];
+        }
+        m.centres[2 * e] = cx; m.centres[2 * e + 1] = cy;
+    }
+}
+
+// ---------------------------------------------------------------------------------------------
+// Neighbour search (find_neighbours.hpp:49-160)
+// ---------------------------------------------------------------------------------------------
+double powered_distance2(const double* a, const double* b) {          // distance.hpp:7-10, norm.hpp:15-22
+    const double z0 = a[0] - b[0], z1 = a[1] - b[1];
+    double s = 0.0;
+    s = s + z0 * z0;
+    s = s + z1 * z1;
+    return s;
+}
+
+double powered_distance2r(const double* a, const double* b, const double* r) {   // distance.hpp:17-23
+    const double z0 = (a[0] - b[0]) / r[0];
+    const double z1 = (a[1] - b[1]) / r[1];
+    double s = 0.0;
+    s = s + z0 * z0;
+    s = s + z1 * z1;
+    return s;
+}
+
+double expansion(const Mesh& m, int g, int strategy) {                  // find_neighbours.hpp:49-72
+    if (strategy == NLF_EXPAND_NO) return 0.0;
+    double diameter = strategy == NLF_EXPAND_MIN ? std::numeric_limits<double>::max() : 0.0;
+    std::set<std::pair<i64, i64>> pairs;
+    for (i64 e = m.gbeg[g]; e < m.gbeg[g + 1]; ++e)
+        for (int i = 0; i < m.nn; ++i) {
+            const i64 node = m.el[e * m.nn + i];
+            for (i64 k = m.ne_off[node]; k < m.ne_off[node + 1]; ++k) {
+                const i64 a = m.ne_ids[k];
+                const double d = powered_distance2(&m.centres[2 * e], &m.centres[2 * a]);
+                if (strategy == NLF_EXPAND_MAX) diameter = std::max(diameter, d);
+                else if (strategy == NLF_EXPAND_MIN) diameter = std::min(diameter, d);
+                else if (!pairs.count({e, a}) && !pairs.count({a, e})) { pairs.insert({e, a}); diameter += std::sqrt(d); }
+            }
+        }
+    return strategy == NLF_EXPAND_MEAN ? diameter / double(pairs.size()) : std::sqrt(diameter);
+}
+
+i64 find_neighbours(Mesh& m, const double* radii, int strategy, double* expanded_out) {
+    m.nb_off.assign(m.E + 1, 0);
+    std::vector<std::vector<i32>> lists(m.E);
+    m.expanded.assign(2 * m.G(), 0.0);
+    for (int g = 0; g < m.G(); ++g) {
+        const double add = expansion(m, g, strategy);                   // increase_influence :74-84
+        const double r[2] = {radii[2 * g] + add, radii[2 * g + 1] + add};
+        m.expanded[2 * g] = r[0]; m.expanded[2 * g + 1] = r[1];
+        if (radii[2 * g] <= 0.0 || radii[2 * g + 1] <= 0.0) {
+            // A group whose influence radius is 0 is local: the reference's influences map only holds
+            // nonlocal groups after fix_parameters (read_model.hpp:186-192), so it gets no lists.
+            m.expanded[2 * g] = 0.0; m.expanded[2 * g + 1] = 0.0;
+            continue;
+        }
+        if (r[0] <= 0.0 || r[1] <= 0.0) continue;                       // :131-134
+        // group_corners :40-47
+        double left = std::numeric_limits<double>::max(), down = left;
+        double right = std::numeric_limits<double>::lowest(), up = right;
+        for (i64 e = m.gbeg[g]; e < m.gbeg[g + 1]; ++e)
+            for (int i = 0; i < m.nn; ++i) {
+                const i64 node = m.el[e * m.nn + i];
+                left = std::min(left, m.xy[2 * node]); down = std::min(down, m.xy[2 * node + 1]);
+                right = std::max(right, m.xy[2 * node]); up = std::max(up, m.xy[2 * node + 1]);
+            }
+        const double R = std::max(r[0], r[1]);
+        const size_t rows = size_t((right - left) / R) + 1, cols = size_t((up - down) / R) + 1;
+        std::vector<std::vector<i32>> cells(rows * cols);               // split_elements_by_subareas :95-107
+        auto cell_of = [&](i64 e) {
+            const size_t a = size_t((m.centres[2 * e] - left) / R);
+            const size_t b = size_t((m.centres[2 * e + 1] - down) / R);
+            return std::pair<size_t, size_t>{a, b};
+        };
+        for (i64 e = m.gbeg[g]; e < m.gbeg[g + 1]; ++e) {
+            const auto [a, b] = cell_of(e);
+            cells[a * cols + b].push_back(i32(e));
+        }
+        for (i64 eL = m.gbeg[g]; eL < m.gbeg[g + 1]; ++eL) {             // :140-157
+            const auto [row, col] = cell_of(eL);
+            const size_t r0 = row ? row - 1 : row, r1 = row != rows - 1 ? row + 2 : row + 1;
+            const size_t c0 = col ? col - 1 : col, c1 = col != cols - 1 ? col + 2 : col + 1;
+            auto& out = lists[eL];
+            for (size_t i = r0; i < r1; ++i)
+                for (size_t j = c0; j < c1; ++j)
+                    for (const i32 eNL : cells[i * cols + j])
+                        if (powered_distance2r(&m.centres[2 * eL], &m.centres[2 * eNL], r) <= 1.0)
+                            out.push_back(eNL);
+        }
+    }
+    for (i64 e = 0; e < m.E; ++e) m.nb_off[e + 1] = m.nb_off[e] + i64(lists[e].size());
+    m.nb_ids.resize(m.nb_off[m.E]);
+    for (i64 e = 0; e < m.E; ++e) std::copy(lists[e].begin(), lists[e].end(), m.nb_ids.begin() + m.nb_off[e]);
+    if (expanded_out) std::copy(m.expanded.begin(), m.expanded.end(), expanded_out);
+    return m.nb_off[m.E];
+}
+
+// ---------------------------------------------------------------------------------------------
+// Influence functions (influence_functions_2d.hpp:39-148), selected like read_influence_2d
+// (read_model.hpp:144-165): polynomial with n = p = 2, q = 1 -> fast_polynomial.
+// ---------------------------------------------------------------------------------------------
+struct Influence {
+    int kind = NLF_INFLUENCE_POLYNOMIAL;
+    bool fast = false;
+    double r[2] = {0, 0};
+    double norm = 1.0;
+    // polynomial exponent p/n and q: integer if exactly integral (metamath size_t_or)
+    bool p_int = true, q_int = true;
+    i64 p_i = 1, q_i = 1;
+    double p_d = 1.0, q_d = 1.0;
+    double exp_q = 0.5;
+};
+
+bool is_integral(double v) { return v >= 0 && std::floor(v) == v && v < 1e9; }
+
+double ipow(double x, i64 e) {                       // power.hpp:132-147 (integral exponent)
+    if (e < 0) return 1.0 / ipow(x, -e);
+    double result = 1.0;
+    while (e > 0) {
+        if (e & 1) { result *= x; --e; }
+        else { x *= x; e >>= 1; }
+    }
+    return result;
+}
+
+double exp_pow(double v, bool is_int, i64 ei, double ed) {   // _influence_function_2d::power :151-158
+    if (!is_int) return std::pow(v, ed);
+    if (ei == 1) return v;
+    return ipow(v, ei);
+}
+
+Influence make_influence(const nlf_model& md) {
+    Influence f;
+    f.kind = md.influence;
+    f.r[0] = md.radius[0]; f.r[1] = md.radius[1];
+    const double n = 2.0;
+    const double p = md.p, q = md.q;
+    if (f.kind == NLF_INFLUENCE_CONSTANT) {
+        f.norm = std::tgamma(1 + 2 / n) / (4 * f.r[0] * f.r[1] * std::pow(std::tgamma(1 + 1 / n), 2));
+    } else if (f.kind == NLF_INFLUENCE_POLYNOMIAL) {
+        if (p == 2.0 && q == 1.0) {
+            f.fast = true;
+            f.norm = 2.0 / (M_PI * f.r[0] * f.r[1]);
+        } else {
+            // _p = division(p, n): integral when p, n integral and n | p
+            if (is_integral(p) && i64(p) % 2 == 0) { f.p_int = true; f.p_i = i64(p) / 2; }
+            else { f.p_int = false; f.p_d = p / n; }
+            if (is_integral(q)) { f.q_int = true; f.q_i = i64(q); } else { f.q_int = false; f.q_d = q; }
+            f.norm = n * p / (4 * f.r[0] * f.r[1] * std::beta(1 / n, 1 / n) * std::beta(2 / p, q + 1));
+        }
+    } else if (f.kind == NLF_INFLUENCE_EXPONENTIAL) {
+        if (is_integral(p) && i64(p) % 2 == 0) { f.p_int = true; f.p_i = i64(p) / 2; }
+        else { f.p_int = false; f.p_d = p / n; }
+        f.exp_q = -q;
+        f.norm = (n * p * std::pow(4.0, 1 / n) * std::pow(q, 2 / p)) /
+                 (8 * f.r[0] * f.r[1] * std::tgamma(2 / p) * std::beta(0.5, 1 / n));
+    } else {
+        throw std::domain_error("unknown influence");
+    }
+    return f;
+}
+
+double influence_eval(const Influence& f, const double* x, const double* y) {
+    const double dist = powered_distance2r(x, y, f.r);
+    if (f.kind == NLF_INFLUENCE_CONSTANT) return dist <= 1.0 ? f.norm : 0.0;
+    if (f.kind == NLF_INFLUENCE_EXPONENTIAL) return f.norm * std::exp(f.exp_q * exp_pow(dist, f.p_int, f.p_i, f.p_d));
+    if (f.fast) return dist < 1.0 ? f.norm * (1.0 - dist) : 0.0;
+    return dist < 1.0 ? f.norm * exp_pow(1.0 - exp_pow(dist, f.p_int, f.p_i, f.p_d), f.q_int, f.q_i, f.q_d) : 0.0;
+}
+
+// ---------------------------------------------------------------------------------------------
+// CSR (Eigen::SparseMatrix<T, RowMajor, int64> layout: outer offsets + inner indices + values)
+// ---------------------------------------------------------------------------------------------
+struct CSR {
+    i64 rows = 0, cols = 0;
+    std::vector<i64> outer;
+    std::vector<i64> inner;
+    std::vector<double> val;
+    double& coeffRef(i64 r, i64 c) {     // binary search within the row (Eigen compressed coeffRef)
+        auto b = inner.begin() + outer[r], e = inner.begin() + outer[r + 1];
+        auto it = std::lower_bound(b, e, c);
+        if (it == e || *it != c) throw std::logic_error("coeffRef outside the pattern");
+        return val[size_t(it - inner.begin())];
+    }
+    double coeff(i64 r, i64 c) const {
+        auto b = inner.begin() + outer[r], e = inner.begin() + outer[r + 1];
+        auto it = std::lower_bound(b, e, c);
+        return (it == e || *it != c) ? 0.0 : val[size_t(it - inner.begin())];
+    }
+};
+
+enum Part { INNER = 0, BOUND = 1, NO = 2 };
+
+struct Separator {                                   // matrix_separator_base::part :50-58
+    const std::vector<uint8_t>* inner;
+    bool symmetric;
+    Part part(i64 row, i64 col) const {
+        if ((*inner)[col]) {
+            if ((!symmetric || row <= col) && (*inner)[row]) return INNER;
+        } else if (row != col) return BOUND;
+        return NO;
+    }
+};
+
+struct Group {
+    bool nonlocal = false;
+    double p1 = 1.0;
+    Influence inf;
+    nlf_thermal_material th{};
+    double hooke[3] = {0, 0, 0};   // {h11, h12, h66} isotropic plane stress
+};
+
+struct Assembly {
+    int dof = 1;
+    bool symmetric = true;
+    CSR A[2];                                        // inner, bound
+};
+
+// mesh_run traversal (mesh_2d_utils.hpp:13-37): omp dynamic over nodes, elements of the node in
+// ascending order, then LOCAL (jL) or NONLOCAL (eNL in neighbours, jNL).
+template<class Runner>
+void mesh_run(const Mesh& m, const std::vector<Group>& groups, Runner proto) {
+#pragma omp parallel firstprivate(proto)
+    {
+        Runner runner = proto;
+        runner.init();
+#pragma omp for schedule(dynamic)
+        for (i64 node = 0; node < m.N; ++node) {
+            runner.reset(node);
+            for (i64 k = m.ne_off[node]; k < m.ne_off[node + 1]; ++k) {
+                const i64 eL = m.ne_ids[k];
+                const int iL = m.local(eL, node);
+                const int g = m.group_of[eL];
+                if (!groups[g].nonlocal) {
+                    for (int jL = 0; jL < m.nn; ++jL) runner(g, eL, eL, iL, jL, true);
+                } else {
+                    for (i64 s = m.nb_off[eL]; s < m.nb_off[eL + 1]; ++s) {
+                        const i64 eNL = m.nb_ids[s];
+                        for (int jNL = 0; jNL < m.nn; ++jNL) runner(g, eL, eNL, iL, jNL, false);
+                    }
+                }
+            }
+        }
+    }
+}
+
+// matrix_indexator_base flags (matrix_indexator_base.hpp:34-61).  `faithful_reset` clears the
+// whole tail of the flag vectors per node like the reference (O(rows) per node); otherwise only the
+// flags that were set are cleared -- same result, used by the tests for speed.
+struct Flags {
+    i64 rows = 0;
+    int dof = 1;
+    bool symmetric = true, faithful = false;
+    std::vector<std::vector<uint8_t>> f;             // [part*dof + d][col]
+    std::vector<std::vector<i64>> touched;
+    void init_flags() {
+        f.assign(2 * dof, std::vector<uint8_t>(rows, 0));
+        touched.assign(2 * dof, {});
+    }
+    void reset(i64 node) {
+        for (int d = 0; d < dof; ++d) {
+            if (faithful) {
+                std::fill(f[BOUND * dof + d].begin(), f[BOUND * dof + d].end(), 0);
+                std::fill(f[INNER * dof + d].begin() + (symmetric ? dof * node : 0), f[INNER * dof + d].end(), 0);
+            } else {
+                for (int p = 0; p < 2; ++p) {
+                    for (i64 c : touched[p * dof + d]) f[p * dof + d][c] = 0;
+                    touched[p * dof + d].clear();
+                }
+            }
+        }
+    }
+    bool check(Part part, int d, i64 col) {
+        if (part == NO) return false;
+        auto& v = f[part * dof + d];
+        if (v[col]) return false;
+        v[col] = 1;
+        if (!faithful) touched[part * dof + d].push_back(col);
+        return true;
+    }
+};
+
+struct ShiftRunner {                                 // shift_initializer.hpp:36-57
+    const Mesh* m; Assembly* a; Separator sep; Flags fl;
+    void init() { fl.init_flags(); }
+    void reset(i64 node) { fl.reset(node); }
+    void operator()(int, i64 eL, i64 eNL, int iL, int jNL, bool) {
+        const int D = a->dof;
+        const i64 rg = i64(D) * m->el[eL * m->nn + iL], cg = i64(D) * m->el[eNL * m->nn + jNL];
+        for (int rl = 0; rl < D; ++rl)
+            for (int cl = 0; cl < D; ++cl) {
+                const i64 row = rg + rl, col = cg + cl;
+                const Part p = sep.part(row, col);
+                if (fl.check(p, rl, col)) {
+#pragma omp atomic
+                    a->A[p].outer[row + 1]++;
+                }
+            }
+    }
+};
+
+struct IndexRunner {                                 // matrix_index_initializer.hpp:95-124
+    const Mesh* m; Assembly* a; Separator sep; Flags fl;
+    i64 cur[2][2] = {{0, 0}, {0, 0}};
+    void init() { fl.init_flags(); }
+    void reset(i64 node) {
+        for (int p = 0; p < 2; ++p)
+            for (int d = 0; d < a->dof; ++d) cur[p][d] = a->A[p].outer[a->dof * node + d];
+        fl.reset(node);
+    }
+    void operator()(int, i64 eL, i64 eNL, int iL, int jNL, bool) {
+        const int D = a->dof;
+        const i64 rg = i64(D) * m->el[eL * m->nn + iL], cg = i64(D) * m->el[eNL * m->nn + jNL];
+        for (int rl = 0; rl < D; ++rl)
+            for (int cl = 0; cl < D; ++cl) {
+                const i64 row = rg + rl, col = cg + cl;
+                const Part p = sep.part(row, col);
+                if (fl.check(p, rl, col)) a->A[p].inner[cur[p][rl]++] = col;
+            }
+    }
+};
+
+// Local and nonlocal integrands.
+// conductivity_matrix_2d.hpp:106-164, stiffness_matrix_2d.hpp:44-123.
+void thermal_local(const Mesh& m, const Group& g, i64 e, int i, int j, double* out) {
+    double integral = 0.0;
+    for (int q = 0; q < m.t.nq; ++q) {
+        const double ix = m.Dx(e, i, q), iy = m.Dy(e, i, q), jx = m.Dx(e, j, q), jy = m.Dy(e, j, q);
+        double value;
+        if (g.th.kind == NLF_CONDUCTIVITY_ISOTROPIC) value = g.th.c[0] * (ix * jx + iy * jy);
+        else if (g.th.kind == NLF_CONDUCTIVITY_ORTHOTROPIC) value = g.th.c[0] * ix * jx + g.th.c[1] * iy * jy;
+        else value = ix * (g.th.c[0] * jx + g.th.c[2] * jy) + iy * (g.th.c[2] * jx + g.th.c[1] * jy);
+        integral += m.t.w[q] * value / m.jac[e * m.t.nq + q];
+    }
+    out[0] = g.p1 * integral;
+}
+
+void thermal_nonlocal(const Mesh& m, const Group& g, i64 eL, i64 eNL, int iL, int jNL, double* out) {
+    double integral = 0.0;
+    const int nq = m.t.nq;
+    for (int qL = 0; qL < nq; ++qL) {
+        double in0 = 0.0, in1 = 0.0;
+        const double* xl = m.xqp(eL, qL);
+        const double ix = m.Dx(eL, iL, qL), iy = m.Dy(eL, iL, qL);
+        for (int qN = 0; qN < nq; ++qN) {
+            const double jx = m.Dx(eNL, jNL, qN), jy = m.Dy(eNL, jNL, qN);
+            const double iw = m.t.w[qN] * influence_eval(g.inf, xl, m.xqp(eNL, qN));
+            if (g.th.kind == NLF_CONDUCTIVITY_ISOTROPIC) {
+                const double s = iw * g.th.c[0];
+                in0 += s * jx; in1 += s * jy;
+            } else if (g.th.kind == NLF_CONDUCTIVITY_ORTHOTROPIC) {
+                in0 += iw * g.th.c[0] * jx; in1 += iw * g.th.c[1] * jy;
+            } else {
+                in0 += iw * (g.th.c[0] * jx + g.th.c[2] * jy);
+                in1 += iw * (g.th.c[2] * jx + g.th.c[1] * jy);
+            }
+        }
+        integral += m.t.w[qL] * (ix * in0 + iy * in1);
+    }
+    out[0] = (1.0 - g.p1) * integral;
+}
+
+void elastic_local(const Mesh& m, const Group& g, i64 e, int i, int j, double* out) {
+    double b[4] = {0, 0, 0, 0};
+    const double h11 = g.hooke[0], h12 = g.hooke[1], h66 = g.hooke[2];
+    for (int q = 0; q < m.t.nq; ++q) {
+        const double f = m.t.w[q] / m.jac[e * m.t.nq + q];
+        const double wx = f * m.Dx(e, i, q), wy = f * m.Dy(e, i, q);
+        const double jx = m.Dx(e, j, q), jy = m.Dy(e, j, q);
+        const double s1 = h66 * wy, s2 = h66 * wx;
+        b[0] += h11 * wx * jx + s1 * jy;
+        b[1] += h12 * wx * jy + s1 * jx;
+        b[2] += h12 * wy * jx + s2 * jy;
+        b[3] += h11 * wy * jy + s2 * jx;
+    }
+    for (int k = 0; k < 4; ++k) out[k] = g.p1 * b[k];
+}
+
+void elastic_nonlocal(const Mesh& m, const Group& g, i64 eL, i64 eNL, int iL, int jNL, double* out) {
+    double b[4] = {0, 0, 0, 0};
+    const double h11 = g.hooke[0], h12 = g.hooke[1], h66 = g.hooke[2];
+    const int nq = m.t.nq;
+    for (int qL = 0; qL < nq; ++qL) {
+        const double* xl = m.xqp(eL, qL);
+        const double ix = m.t.w[qL] * m.Dx(eL, iL, qL), iy = m.t.w[qL] * m.Dy(eL, iL, qL);
+        for (int qN = 0; qN < nq; ++qN) {
+            const double wgt = m.t.w[qN] * influence_eval(g.inf, xl, m.xqp(eNL, qN));
+            const double jx = wgt * m.Dx(eNL, jNL, qN), jy = wgt * m.Dy(eNL, jNL, qN);
+            b[0] += h11 * ix * jx + h66 * iy * jy;
+            b[1] += h12 * ix * jy + h66 * iy * jx;
+            b[2] += h12 * iy * jx + h66 * ix * jy;
+            b[3] += h11 * iy * jy + h66 * ix * jx;
+        }
+    }
+    for (int k = 0; k < 4; ++k) out[k] = (1.0 - g.p1) * b[k];
+}
+
+struct ValueRunner {                                 // integrator.hpp:36-91
+    const Mesh* m; Assembly* a; Separator sep; const std::vector<Group>* groups; int physics;
+    void init() {}
+    void reset(i64) {}
+    void operator()(int gi, i64 eL, i64 eNL, int iL, int jNL, bool local_theory) {
+        const int D = a->dof;
+        const i64 rg = i64(D) * m->el[eL * m->nn + iL], cg = i64(D) * m->el[eNL * m->nn + jNL];
+        const Group& g = (*groups)[gi];
+        double block[4] = {0, 0, 0, 0};
+        bool have = false;
+        for (int rl = 0; rl < D; ++rl) {
+            double* val = nullptr;
+            for (int cl = 0; cl < D; ++cl) {
+                const i64 row = rg + rl, col = cg + cl;
+                const Part p = sep.part(row, col);
+                if (p == NO) continue;
+                if (!have) {
+                    have = true;
+                    if (local_theory) {
+                        if (physics == 0) thermal_local(*m, g, eL, iL, jNL, block); else elastic_local(*m, g, eL, iL, jNL, block);
+                    } else {
+                        if (physics == 0) thermal_nonlocal(*m, g, eL, eNL, iL, jNL, block); else elastic_nonlocal(*m, g, eL, eNL, iL, jNL, block);
+                        if (eL == eNL) {
+                            double lb[4];
+                            if (physics == 0) thermal_local(*m, g, eL, iL, jNL, lb); else elastic_local(*m, g, eL, iL, jNL, lb);
+                            for (int k = 0; k < D * D; ++k) block[k] += lb[k];
+                        }
+                    }
+                }
+                if (p == BOUND) a->A[BOUND].coeffRef(row, col) += block[rl * D + cl];
+                else {
+                    if (!val) val = &a->A[INNER].coeffRef(row, col);
+                    *val += block[rl * D + cl];
+                    ++val;
+                }
+            }
+        }
+    }
+};
+
+Group make_group(const nlf_model& md, int physics, const nlf_thermal_material* th, const nlf_elastic_material* el, bool only_local) {
+    Group g;
+    double p1 = md.local_weight;
+    double r0 = md.radius[0], r1 = md.radius[1];
+    // fix_parameters (read_model.hpp:186-192)
+    if (p1 >= 0.999) { r0 = 0; r1 = 0; }
+    if (r0 < std::numeric_limits<double>::epsilon() || r1 < std::numeric_limits<double>::epsilon()) p1 = 1.0;
+    g.p1 = p1;
+    g.nonlocal = !only_local && p1 < 0.999;
+    if (g.nonlocal) {
+        nlf_model m2 = md; m2.radius[0] = r0; m2.radius[1] = r1;
+        g.inf = make_influence(m2);
+    }
+    if (physics == 0) g.th = *th;
+    else {
+        const double E = el->young, nu = el->poisson;
+        const double value = E / (1.0 - nu * nu);
+        g.hooke[0] = value; g.hooke[1] = nu * value; g.hooke[2] = 0.5 * value * (1.0 - nu);
+    }
+    return g;
+}
+
+Assembly* assemble(const Mesh& m, int physics, const std::vector<Group>& groups, const std::vector<uint8_t>& is_inner,
+                   bool symmetric, bool faithful_reset) {
+    auto* a = new Assembly();
+    a->dof = physics == 0 ? 1 : 2;
+    a->symmetric = symmetric;
+    const i64 rows = i64(a->dof) * m.N;
+    for (int p = 0; p < 2; ++p) { a->A[p].rows = rows; a->A[p].cols = rows; a->A[p].outer.assign(rows + 1, 0); }
+    Separator sep{&is_inner, symmetric};
+    Flags fl; fl.rows = rows; fl.dof = a->dof; fl.symmetric = symmetric; fl.faithful = faithful_reset;
+    // create_matrix_portrait (conductivity_matrix_2d.hpp:39-72) -> init_shifts (matrix_assembler_2d.hpp:144-156)
+    mesh_run(m, groups, ShiftRunner{&m, a, sep, fl});
+    for (i64 r = 0; r < rows; ++r) if (!is_inner[r]) a->A[INNER].outer[r + 1]++;   // first_kind_filler
+    for (int p = 0; p < 2; ++p) for (i64 r = 0; r < rows; ++r) a->A[p].outer[r + 1] += a->A[p].outer[r];
+    for (int p = 0; p < 2; ++p) {                    // allocate_matrix (solvers/base/utils.hpp:32-38)
+        a->A[p].inner.assign(a->A[p].outer[rows], 0);
+        a->A[p].val.assign(a->A[p].outer[rows], 0.0);
+    }
+    mesh_run(m, groups, IndexRunner{&m, a, sep, fl});                // init_indices :158-175
+    for (i64 r = 0; r < rows; ++r) if (!is_inner[r]) a->A[INNER].inner[a->A[INNER].outer[r]] = r;
+    for (int p = 0; p < 2; ++p) {                    // sort_indices (solvers/base/utils.hpp:40-45)
+        CSR& A = a->A[p];
+#pragma omp parallel for schedule(dynamic)
+        for (i64 r = 0; r < rows; ++r) std::sort(A.inner.begin() + A.outer[r], A.inner.begin() + A.outer[r + 1]);
+    }
+    mesh_run(m, groups, ValueRunner{&m, a, sep, &groups, physics});  // calc_coeffs :177-189
+    for (i64 r = 0; r < rows; ++r) if (!is_inner[r]) a->A[INNER].val[a->A[INNER].outer[r]] = 1.0;
+    return a;
+}
+
+// ---------------------------------------------------------------------------------------------
+// Boundary conditions and right-hand side
+// ---------------------------------------------------------------------------------------------
+// 1D element integration data for boundary element be, quad point q: (|J|, N_i)
+double boundary_jac(const Mesh& m, i64 be, int q) {  // element_data_1d::jacobi_matrix + mesh::jacobian
+    double jx = 0, jy = 0;
+    for (int i = 0; i < m.nb; ++i) {
+        const i64 node = m.be[be * m.nb + i];
+        jx = jx + m.xy[2 * node] * m.t1.qNxi[i * m.t1.nq + q];
+        jy = jy + m.xy[2 * node + 1] * m.t1.qNxi[i * m.t1.nq + q];
+    }
+    return std::sqrt(jx * jx + jy * jy);
+}
+
+// boundary_condition_second_kind_2d (boundary_condition_second_kind_2d.hpp:12-33) with constant values;
+// thermal convection also adds h*T_a (thermal_boundary_conditions_2d.hpp:54-56).
+void second_kind(const Mesh& m, int dof, const nlf_boundary* bc, double* f) {
+    for (size_t b = 0; b + 1 < m.bbeg.size(); ++b)
+        for (int d = 0; d < dof; ++d) {
+            double g;
+            if (bc[b].kind[d] == NLF_BC_FLUX) g = bc[b].value[d];
+            else if (dof == 1 && bc[b].kind[d] == NLF_BC_CONVECTION) g = bc[b].heat_transfer * bc[b].ambient;
+            else continue;
+            for (i64 be = m.bbeg[b]; be < m.bbeg[b + 1]; ++be)
+                for (int i = 0; i < m.nb; ++i) {
+                    const i64 node = m.be[be * m.nb + i];
+                    double integral = 0.0;
+                    for (int q = 0; q < m.t1.nq; ++q)
+                        integral += m.t1.w[q] * m.t1.qN[i * m.t1.nq + q] * g * boundary_jac(m, be, q);
+                    f[dof * node + d] += integral;
+                }
+        }
+}
+
+// convection_condition_2d (convection_condition_2d.hpp:11-31): K(row, col) += h*int N_i N_j for
+// col >= row.  Entries whose row or column is Dirichlet are left out (the reference zeroes them in
+// Dirichlet rows through first_kind_matrix_fill_2d).
+void convection(const Mesh& m, const nlf_boundary* bc, const std::vector<uint8_t>& inner, CSR& K) {
+    for (size_t b = 0; b + 1 < m.bbeg.size(); ++b) {
+        if (bc[b].kind[0] != NLF_BC_CONVECTION) continue;
+        for (i64 be = m.bbeg[b]; be < m.bbeg[b + 1]; ++be)
+            for (int i = 0; i < m.nb; ++i) {
+                const i64 row = m.be[be * m.nb + i];
+                for (int j = 0; j < m.nb; ++j) {
+                    const i64 col = m.be[be * m.nb + j];
+                    if (col < row || !inner[row] || !inner[col]) continue;
+                    double integral = 0.0;
+                    for (int q = 0; q < m.t1.nq; ++q)
+                        integral += m.t1.w[q] * m.t1.qN[i * m.t1.nq + q] * m.t1.qN[j * m.t1.nq + q] * boundary_jac(m, be, q);
+                    K.coeffRef(row, col) += bc[b].heat_transfer * integral;
+                }
+            }
+    }
+}
+
+// Dirichlet values x_D (calc_vector :17-26) and inner mask (solvers_utils.hpp:31-40)
+void first_kind_vector(const Mesh& m, int dof, const nlf_boundary* bc, double* x, uint8_t* inner) {
+    for (size_t b = 0; b + 1 < m.bbeg.size(); ++b)
+        for (int d = 0; d < dof; ++d) {
+            if (bc[b].kind[d] != NLF_BC_FIRST_KIND) continue;
+            for (i64 be = m.bbeg[b]; be < m.bbeg[b + 1]; ++be)
+                for (int i = 0; i < m.nb; ++i) {
+                    const i64 node = m.be[be * m.nb + i];
+                    if (x && x[dof * node + d] == 0.0) x[dof * node + d] = bc[b].value[d];
+                    if (inner) inner[dof * node + d] = 0;
+                }
+        }
+}
+
+void upper_spmv(const CSR& A, const double* x, double* y);
+
+// boundary_condition_first_kind_2d (:50-59): f -= K_bound * x_D; f[D] = x_D
+void first_kind_rhs(const CSR& bound, const std::vector<uint8_t>& inner, const double* xD, double* f) {
+    for (i64 r = 0; r < bound.rows; ++r) {
+        double s = 0.0;
+        for (i64 k = bound.outer[r]; k < bound.outer[r + 1]; ++k) s += bound.val[k] * xD[bound.inner[k]];
+        f[r] -= s;
+    }
+    for (i64 r = 0; r < bound.rows; ++r) if (!inner[r]) f[r] = xD[r];
+}
+
+// ---------------------------------------------------------------------------------------------
+// SpMV on upper symmetric CSR (independent_symmetric_matrix_vector_product.hpp:41-91)
+// ---------------------------------------------------------------------------------------------
+void upper_spmv(const CSR& A, const double* x, double* y) {
+    const int T = omp_get_max_threads();
+    std::vector<double> acc(size_t(A.rows) * T, 0.0);
+    // uniform_ranges by nnz (uniform_ranges.hpp:14-40)
+    std::vector<i64> cut(T + 1, A.rows);
+    cut[0] = 0;
+    const i64 nnz = A.outer[A.rows];
+    for (int t = 1; t < T; ++t) {
+        const i64 target = nnz * t / T;
+        cut[t] = std::lower_bound(A.outer.begin(), A.outer.end(), target) - A.outer.begin();
+        cut[t] = std::min(std::max(cut[t], cut[t - 1]), A.rows);
+    }
+#pragma omp parallel num_threads(T)
+    {
+        const int t = omp_get_thread_num();
+        double* a = &acc[size_t(t) * A.rows];
+        for (i64 row = cut[t]; row < cut[t + 1]; ++row) {
+            const i64 ind = A.outer[row];
+            if (ind == A.outer[row + 1]) continue;
+            a[row] += A.val[ind] * x[A.inner[ind]];
+            for (i64 i = ind + 1; i < A.outer[row + 1]; ++i) {
+                a[row] += A.val[i] * x[A.inner[i]];
+                a[A.inner[i]] += A.val[i] * x[row];
+            }
+        }
+    }
+    for (i64 r = 0; r < A.rows; ++r) {
+        double s = 0.0;
+        for (int t = 0; t < T; ++t) s += acc[size_t(t) * A.rows + r];
+        y[r] = s;
+    }
+}
+
+// ---------------------------------------------------------------------------------------------
+// Preconditioners: identity, Jacobi (Eigen::DiagonalPreconditioner), IC(0) with diagonal scaling
+// and shift retries (our incomplete Cholesky; Eigen::IncompleteCholesky is not vendored).
+// ---------------------------------------------------------------------------------------------
+struct Precond {
+    int kind = NLF_PRECOND_IDENTITY;
+    std::vector<double> invdiag;
+    CSR U;                   // upper factor of the scaled matrix, diagonal first
+    std::vector<double> scale;
+    double shift = 0.0;
+    bool ok = true;
+    void apply(const double* r, double* z, i64 n) const {
+        if (kind == NLF_PRECOND_IDENTITY) { std::copy(r, r + n, z); return; }
+        if (kind == NLF_PRECOND_JACOBI) { for (i64 i = 0; i < n; ++i) z[i] = invdiag[i] * r[i]; return; }
+        // z = S U^{-1} U^{-T} S r
+        std::vector<double> y(n);
+        for (i64 i = 0; i < n; ++i) y[i] = scale[i] * r[i];
+        for (i64 k = 0; k < n; ++k) {                // forward U^T y = b (column sweep)
+            const i64 b = U.outer[k];
+            y[k] /= U.val[b];
+            for (i64 p = b + 1; p < U.outer[k + 1]; ++p) y[U.inner[p]] -= U.val[p] * y[k];
+        }
+        for (i64 k = n - 1; k >= 0; --k) {           // backward U z = y
+            const i64 b = U.outer[k];
+            double s = y[k];
+            for (i64 p = b + 1; p < U.outer[k + 1]; ++p) s -= U.val[p] * y[U.inner[p]];
+            y[k] = s / U.val[b];
+        }
+        for (i64 i = 0; i < n; ++i) z[i] = scale[i] * y[i];
+    }
+};
+
+bool ic0_factor(const CSR& A, const std::vector<double>& s, double shift, CSR& U) {
+    U = A;
+    const i64 n = A.rows;
+    for (i64 r = 0; r < n; ++r)
+        for (i64 k = U.outer[r]; k < U.outer[r + 1]; ++k) U.val[k] = s[r] * A.val[k] * s[U.inner[k]];
+    for (i64 r = 0; r < n; ++r) U.val[U.outer[r]] += shift;
+    for (i64 k = 0; k < n; ++k) {
+        const i64 b = U.outer[k], e = U.outer[k + 1];
+        const double d = U.val[b];
+        if (!(d > 0.0) || !std::isfinite(d)) return false;
+        const double ukk = std::sqrt(d);
+        U.val[b] = ukk;
+        for (i64 p = b + 1; p < e; ++p) U.val[p] /= ukk;
+        for (i64 p = b + 1; p < e; ++p) {
+            const i64 j = U.inner[p];
+            const double ukj = U.val[p];
+            // update row j at columns l >= j present in row k
+            i64 qj = U.outer[j];
+            const i64 ej = U.outer[j + 1];
+            for (i64 q = p; q < e; ++q) {
+                const i64 l = U.inner[q];
+                while (qj < ej && U.inner[qj] < l) ++qj;
+                if (qj < ej && U.inner[qj] == l) U.val[qj] -= ukj * U.val[q];
+            }
+        }
+    }
+    return true;
+}
+
+Precond* make_precond(int kind, const CSR* A) {
+    auto* P = new Precond();
+    P->kind = kind;
+    if (kind == NLF_PRECOND_JACOBI) {
+        P->invdiag.resize(A->rows);
+        for (i64 r = 0; r < A->rows; ++r) {
+            const double d = A->coeff(r, r);
+            P->invdiag[r] = d != 0.0 ? 1.0 / d : 1.0;
+        }
+    } else if (kind == NLF_PRECOND_ILLT) {
+        P->scale.resize(A->rows);
+        for (i64 r = 0; r < A->rows; ++r) {
+            const double d = std::abs(A->coeff(r, r));
+            P->scale[r] = d > 0.0 ? 1.0 / std::sqrt(d) : 1.0;
+        }
+        double shift = 0.0;
+        P->ok = false;
+        for (int attempt = 0; attempt < 40; ++attempt) {
+            if (ic0_factor(*A, P->scale, shift, P->U)) { P->ok = true; P->shift = shift; break; }
+            shift = shift == 0.0 ? 1e-3 : 2 * shift;
+        }
+    }
+    return P;
+}
+
+// conjugate_gradient::solve (conjugate_gradient.hpp:48-79): stops on sqrt(r'z)/||b|| <= tol.
+struct CGResult { i64 iterations; double residual; };
+
+CGResult cg(const CSR& A, const Precond& M, const double* b, double* x, double tol, i64 max_it) {
+    const i64 n = A.rows;
+    std::vector<double> r(n), z(n), p(n);
+    upper_spmv(A, x, r.data());
+    for (i64 i = 0; i < n; ++i) r[i] = b[i] - r[i];
+    M.apply(r.data(), p.data(), n);
+    double rz = 0.0, bn = 0.0;
+    for (i64 i = 0; i < n; ++i) { rz += r[i] * p[i]; bn += b[i] * b[i]; }
+    bn = std::sqrt(bn);
+    i64 it = 0;
+    double res = std::sqrt(rz) / bn;
+    while (it < max_it && res > tol) {
+        upper_spmv(A, p.data(), z.data());
+        double pz = 0.0;
+        for (i64 i = 0; i < n; ++i) pz += p[i] * z[i];
+        const double nu = rz / pz;
+        for (i64 i = 0; i < n; ++i) { x[i] += nu * p[i]; r[i] -= nu * z[i]; }
+        M.apply(r.data(), z.data(), n);
+        const double prev = rz;
+        rz = 0.0;
+        for (i64 i = 0; i < n; ++i) rz += r[i] * z[i];
+        const double mu = rz / prev;
+        for (i64 i = 0; i < n; ++i) p[i] = z[i] + mu * p[i];
+        ++it;
+        res = std::sqrt(rz) / bn;
+    }
+    return {it, res};
+}
+
+thread_local std::string g_error;
+
+}  // namespace
+
+// =============================================================================================
+// C ABI of the oracle (loaded by tests through ctypes)
+// =============================================================================================
+extern "C" {
+
+const char* orc_last_error() { return g_error.c_str(); }
+
+int orc_set_threads(int n) { if (n > 0) omp_set_num_threads(n); return omp_get_max_threads(); }
+
+void* orc_mesh_create(i64 n_nodes, const double* xy, i64 n_elems, int nn, const i32* elems, int n_groups,
+                      const i64* group_begin, i64 n_bedges, int nb, const i32* bedges, int n_bgroups,
+                      const i64* bgroup_begin) {
+    try {
+        auto* m = new Mesh();
+        m->N = n_nodes; m->E = n_elems; m->nn = nn; m->nb = nb;
+        m->xy.assign(xy, xy + 2 * n_nodes);
+        m->el.assign(elems, elems + n_elems * nn);
+        m->gbeg.assign(group_begin, group_begin + n_groups + 1);
+        m->be.assign(bedges, bedges + n_bedges * nb);
+        m->bbeg.assign(bgroup_begin, bgroup_begin + n_bgroups + 1);
+        mesh_init(*m);
+        return m;
+    } catch (const std::exception& e) { g_error = e.what(); return nullptr; }
+}
+
+void orc_mesh_destroy(void* h) { delete static_cast<Mesh*>(h); }
+
+void orc_centres(void* h, double* out) {
+    const Mesh& m = *static_cast<Mesh*>(h);
+    std::copy(m.centres.begin(), m.centres.end(), out);
+}
+
+void orc_quadrature(void* h, double* xq, double* jac, double* D) {
+    const Mesh& m = *static_cast<Mesh*>(h);
+    if (xq) std::copy(m.xq.begin(), m.xq.end(), xq);
+    if (jac) std::copy(m.jac.begin(), m.jac.end(), jac);
+    if (D) std::copy(m.D.begin(), m.D.end(), D);
+}
+
+void orc_tables(int nn, double* w, double* qN, double* qNxi, double* qNeta, double* N0) {
+    const Tables2D t = make_tables(nn);
+    std::copy(t.w.begin(), t.w.end(), w);
+    std::copy(t.qN.begin(), t.qN.end(), qN);
+    std::copy(t.qNxi.begin(), t.qNxi.end(), qNxi);
+    std::copy(t.qNeta.begin(), t.qNeta.end(), qNeta);
+    std::copy(t.N0.begin(), t.N0.end(), N0);
+}
+
+i64 orc_find_neighbours(void* h, const double* radii, int strategy, double* expanded) {
+    try { return find_neighbours(*static_cast<Mesh*>(h), radii, strategy, expanded); }
+    catch (const std::exception& e) { g_error = e.what(); return -1; }
+}
+
+void orc_neighbours_export(void* h, i64* offsets, i32* ids) {
+    const Mesh& m = *static_cast<Mesh*>(h);
+    std::copy(m.nb_off.begin(), m.nb_off.end(), offsets);
+    std::copy(m.nb_ids.begin(), m.nb_ids.end(), ids);
+}
+
+double orc_influence(const nlf_model* md, const double* x, const double* y) {
+    return influence_eval(make_influence(*md), x, y);
+}
+
+double orc_influence_norm(const nlf_model* md) { return make_influence(*md).norm; }
+
+// physics 0 = thermal (materials: nlf_thermal_material[G]), 1 = elastic (nlf_elastic_material[G]).
+void* orc_assemble(void* h, int physics, const nlf_model* models, const void* materials, const uint8_t* is_inner,
+                   int symmetric, int part, int faithful_reset) {
+    try {
+        const Mesh& m = *static_cast<Mesh*>(h);
+        std::vector<Group> groups;
+        for (int g = 0; g < m.G(); ++g)
+            groups.push_back(make_group(models[g], physics,
+                                        physics == 0 ? &static_cast<const nlf_thermal_material*>(materials)[g] : nullptr,
+                                        physics == 1 ? &static_cast<const nlf_elastic_material*>(materials)[g] : nullptr,
+                                        part == NLF_PART_LOCAL));
+        const i64 rows = (physics == 0 ? 1 : 2) * m.N;
+        std::vector<uint8_t> inner(is_inner, is_inner + rows);
+        return assemble(m, physics, groups, inner, symmetric != 0, faithful_reset != 0);
+    } catch (const std::exception& e) { g_error = e.what(); return nullptr; }
+}
+
+void orc_matrix_destroy(void* a) { delete static_cast<Assembly*>(a); }
+
+void orc_matrix_info(void* a, int part, i64* rows, i64* nnz) {
+    const CSR& A = static_cast<Assembly*>(a)->A[part];
+    *rows = A.rows;
+    *nnz = A.outer[A.rows];
+}
+
+void orc_matrix_export(void* a, int part, i64* outer, i64* inner, double* val) {
+    const CSR& A = static_cast<Assembly*>(a)->A[part];
+    if (outer) std::copy(A.outer.begin(), A.outer.end(), outer);
+    if (inner) std::copy(A.inner.begin(), A.inner.end(), inner);
+    if (val) std::copy(A.val.begin(), A.val.end(), val);
+}
+
+void orc_matrix_set_values(void* a, int part, const double* val) {
+    CSR& A = static_cast<Assembly*>(a)->A[part];
+    std::copy(val, val + A.outer[A.rows], A.val.begin());
+}
+
+// Thermal boundary data: f (second kind + convection RHS), x_D, inner mask; and the convection matrix term.
+void orc_boundary(void* h, int dof, const nlf_boundary* bc, double* f, double* xD, uint8_t* inner) {
+    const Mesh& m = *static_cast<Mesh*>(h);
+    const i64 rows = dof * m.N;
+    if (inner) std::fill(inner, inner + rows, uint8_t(1));
+    if (xD) std::fill(xD, xD + rows, 0.0);
+    first_kind_vector(m, dof, bc, xD, inner);
+    if (f) { std::fill(f, f + rows, 0.0); second_kind(m, dof, bc, f); }
+}
+
+void orc_convection(void* h, void* a, const nlf_boundary* bc, const uint8_t* is_inner) {
+    const Mesh& m = *static_cast<Mesh*>(h);
+    Assembly* A = static_cast<Assembly*>(a);
+    std::vector<uint8_t> inner(is_inner, is_inner + A->A[0].rows);
+    convection(m, bc, inner, A->A[INNER]);
+}
+
+void orc_first_kind_rhs(void* a, const uint8_t* is_inner, const double* xD, double* f) {
+    Assembly* A = static_cast<Assembly*>(a);
+    std::vector<uint8_t> inner(is_inner, is_inner + A->A[0].rows);
+    first_kind_rhs(A->A[BOUND], inner, xD, f);
+}
+
+void orc_spmv(void* a, const double* x, double* y) { upper_spmv(static_cast<Assembly*>(a)->A[INNER], x, y); }
+
+void* orc_precond_create(int kind, void* a) {
+    return make_precond(kind, a ? &static_cast<Assembly*>(a)->A[INNER] : nullptr);
+}
+void orc_precond_destroy(void* p) { delete static_cast<Precond*>(p); }
+int orc_precond_ok(void* p, double* shift) { auto* P = static_cast<Precond*>(p); if (shift) *shift = P->shift; return P->ok; }
+void orc_precond_apply(void* p, const double* r, double* z, i64 n) { static_cast<Precond*>(p)->apply(r, z, n); }
+void orc_precond_factor(void* p, i64* outer, i64* inner, double* val, double* scale) {
+    const Precond& P = *static_cast<Precond*>(p);
+    if (outer) std::copy(P.U.outer.begin(), P.U.outer.end(), outer);
+    if (inner) std::copy(P.U.inner.begin(), P.U.inner.end(), inner);
+    if (val) std::copy(P.U.val.begin(), P.U.val.end(), val);
+    if (scale) std::copy(P.scale.begin(), P.scale.end(), scale);
+}
+
+i64 orc_cg(void* a, void* p, const double* b, double* x, double tol, i64 max_it, double* residual) {
+    const auto r = cg(static_cast<Assembly*>(a)->A[INNER], *static_cast<Precond*>(p), b, x, tol, max_it);
+    if (residual) *residual = r.residual;
+    return r.iterations;
+}
+
+}  // extern "C"

@@ -1,0 +1,309 @@
+"""ORACLE -- test infrastructure only (tests/, __graft_entry__.smoke(), bench.py cpu_baseline).
+
+ctypes front end of oracle/liboracle.so, the FP64 OpenMP C++ restatement of NonLocFEM's 2D
+nonlocal hot path.  The product package never imports this module.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass
+
+import numpy as np
+
+from nonlocal_amd import _abi
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+
+_i64 = C.c_int64
+_i32p = C.POINTER(C.c_int32)
+_i64p = C.POINTER(C.c_int64)
+_f64p = C.POINTER(C.c_double)
+_u8p = C.POINTER(C.c_uint8)
+
+
+def _ptr(a, t):
+    return a.ctypes.data_as(t)
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        path = os.path.join(_HERE, "liboracle.so")
+        if not os.path.exists(path):
+            raise RuntimeError(f"oracle not built: {path} (run __graft_entry__.build())")
+        L = C.CDLL(path)
+        L.orc_mesh_create.restype = C.c_void_p
+        L.orc_mesh_create.argtypes = [_i64, _f64p, _i64, C.c_int, _i32p, C.c_int, _i64p, _i64, C.c_int, _i32p,
+                                      C.c_int, _i64p]
+        L.orc_mesh_destroy.argtypes = [C.c_void_p]
+        L.orc_centres.argtypes = [C.c_void_p, _f64p]
+        L.orc_quadrature.argtypes = [C.c_void_p, _f64p, _f64p, _f64p]
+        L.orc_tables.argtypes = [C.c_int, _f64p, _f64p, _f64p, _f64p, _f64p]
+        L.orc_find_neighbours.restype = _i64
+        L.orc_find_neighbours.argtypes = [C.c_void_p, _f64p, C.c_int, _f64p]
+        L.orc_neighbours_export.argtypes = [C.c_void_p, _i64p, _i32p]
+        L.orc_influence.restype = C.c_double
+        L.orc_influence.argtypes = [C.POINTER(_abi.Model), _f64p, _f64p]
+        L.orc_influence_norm.restype = C.c_double
+        L.orc_influence_norm.argtypes = [C.POINTER(_abi.Model)]
+        L.orc_assemble.restype = C.c_void_p
+        L.orc_assemble.argtypes = [C.c_void_p, C.c_int, C.POINTER(_abi.Model), C.c_void_p, _u8p, C.c_int, C.c_int,
+                                   C.c_int]
+        L.orc_matrix_destroy.argtypes = [C.c_void_p]
+        L.orc_matrix_info.argtypes = [C.c_void_p, C.c_int, _i64p, _i64p]
+        L.orc_matrix_export.argtypes = [C.c_void_p, C.c_int, _i64p, _i64p, _f64p]
+        L.orc_matrix_set_values.argtypes = [C.c_void_p, C.c_int, _f64p]
+        L.orc_boundary.argtypes = [C.c_void_p, C.c_int, C.POINTER(_abi.Boundary), _f64p, _f64p, _u8p]
+        L.orc_convection.argtypes = [C.c_void_p, C.c_void_p, C.POINTER(_abi.Boundary), _u8p]
+        L.orc_first_kind_rhs.argtypes = [C.c_void_p, _u8p, _f64p, _f64p]
+        L.orc_spmv.argtypes = [C.c_void_p, _f64p, _f64p]
+        L.orc_precond_create.restype = C.c_void_p
+        L.orc_precond_create.argtypes = [C.c_int, C.c_void_p]
+        L.orc_precond_destroy.argtypes = [C.c_void_p]
+        L.orc_precond_ok.argtypes = [C.c_void_p, _f64p]
+        L.orc_precond_apply.argtypes = [C.c_void_p, _f64p, _f64p, _i64]
+        L.orc_precond_factor.argtypes = [C.c_void_p, _i64p, _i64p, _f64p, _f64p]
+        L.orc_cg.restype = _i64
+        L.orc_cg.argtypes = [C.c_void_p, C.c_void_p, _f64p, _f64p, C.c_double, _i64, _f64p]
+        L.orc_set_threads.argtypes = [C.c_int]
+        L.orc_last_error.restype = C.c_char_p
+        _LIB = L
+    return _LIB
+
+
+def set_threads(n: int) -> int:
+    return lib().orc_set_threads(int(n))
+
+
+def tables(nn: int):
+    nq = 4 if nn == 4 else 9
+    w = np.zeros(nq)
+    qN = np.zeros(nn * nq)
+    qx = np.zeros(nn * nq)
+    qy = np.zeros(nn * nq)
+    N0 = np.zeros(nn)
+    lib().orc_tables(nn, _ptr(w, _f64p), _ptr(qN, _f64p), _ptr(qx, _f64p), _ptr(qy, _f64p), _ptr(N0, _f64p))
+    return w, qN.reshape(nn, nq), qx.reshape(nn, nq), qy.reshape(nn, nq), N0
+
+
+@dataclass
+class CSR:
+    outer: np.ndarray
+    inner: np.ndarray
+    val: np.ndarray
+
+    @property
+    def rows(self):
+        return self.outer.size - 1
+
+    def dense(self, symmetric_upper=False):
+        n = self.rows
+        A = np.zeros((n, n))
+        for r in range(n):
+            for k in range(self.outer[r], self.outer[r + 1]):
+                A[r, self.inner[k]] += self.val[k]
+        if symmetric_upper:
+            A = A + np.triu(A, 1).T
+        return A
+
+    def to_scipy(self, n_cols=None):
+        import scipy.sparse as sp
+        n = self.rows
+        return sp.csr_matrix((self.val, self.inner, self.outer), shape=(n, n_cols or n))
+
+
+class OracleMatrix:
+    def __init__(self, handle, rows):
+        self.h = handle
+        self.rows = rows
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().orc_matrix_destroy(self.h)
+            self.h = None
+
+    def export(self, part=0) -> CSR:
+        rows = _i64()
+        nnz = _i64()
+        lib().orc_matrix_info(self.h, part, C.byref(rows), C.byref(nnz))
+        outer = np.zeros(rows.value + 1, dtype=np.int64)
+        inner = np.zeros(nnz.value, dtype=np.int64)
+        val = np.zeros(nnz.value)
+        lib().orc_matrix_export(self.h, part, _ptr(outer, _i64p), _ptr(inner, _i64p), _ptr(val, _f64p))
+        return CSR(outer, inner, val)
+
+    def set_values(self, val, part=0):
+        val = np.ascontiguousarray(val, dtype=np.float64)
+        lib().orc_matrix_set_values(self.h, part, _ptr(val, _f64p))
+
+    def spmv(self, x):
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        y = np.zeros_like(x)
+        lib().orc_spmv(self.h, _ptr(x, _f64p), _ptr(y, _f64p))
+        return y
+
+
+class OracleMesh:
+    """Oracle mesh: precompute, centres and neighbour lists (mesh_2d + find_neighbours)."""
+
+    def __init__(self, mesh):
+        self.mesh = mesh
+        xy = np.ascontiguousarray(mesh.xy, dtype=np.float64)
+        el = np.ascontiguousarray(mesh.elems, dtype=np.int32)
+        gb = np.ascontiguousarray(mesh.group_begin, dtype=np.int64)
+        be = np.ascontiguousarray(mesh.bedges, dtype=np.int32)
+        bb = np.ascontiguousarray(mesh.bgroup_begin, dtype=np.int64)
+        self._keep = (xy, el, gb, be, bb)
+        self.h = lib().orc_mesh_create(mesh.n_nodes, _ptr(xy, _f64p), mesh.n_elems, mesh.nn, _ptr(el, _i32p),
+                                       len(gb) - 1, _ptr(gb, _i64p), be.shape[0], be.shape[1], _ptr(be, _i32p),
+                                       len(bb) - 1, _ptr(bb, _i64p))
+        if not self.h:
+            raise RuntimeError(lib().orc_last_error().decode())
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().orc_mesh_destroy(self.h)
+            self.h = None
+
+    def centres(self):
+        c = np.zeros((self.mesh.n_elems, 2))
+        lib().orc_centres(self.h, _ptr(c, _f64p))
+        return c
+
+    def quadrature(self):
+        nq = 4 if self.mesh.nn == 4 else 9
+        E, nn = self.mesh.n_elems, self.mesh.nn
+        xq = np.zeros((E, nq, 2))
+        jac = np.zeros((E, nq))
+        D = np.zeros((E, nn, nq, 2))
+        lib().orc_quadrature(self.h, _ptr(xq, _f64p), _ptr(jac, _f64p), _ptr(D, _f64p))
+        return xq, jac, D
+
+    def find_neighbours(self, radii, strategy=_abi.NLF_EXPAND_MAX):
+        """radii: (G, 2) search radii per group.  Returns (offsets, ids, expanded_radii)."""
+        radii = np.ascontiguousarray(np.asarray(radii, dtype=np.float64).reshape(-1, 2))
+        expanded = np.zeros_like(radii)
+        total = lib().orc_find_neighbours(self.h, _ptr(radii, _f64p), strategy, _ptr(expanded, _f64p))
+        if total < 0:
+            raise RuntimeError(lib().orc_last_error().decode())
+        off = np.zeros(self.mesh.n_elems + 1, dtype=np.int64)
+        ids = np.zeros(total, dtype=np.int32)
+        lib().orc_neighbours_export(self.h, _ptr(off, _i64p), _ptr(ids, _i32p))
+        return off, ids, expanded
+
+    def boundary(self, bcs, dof=1):
+        """bcs: list of nlf_boundary per boundary group.  Returns (f, xD, is_inner)."""
+        rows = dof * self.mesh.n_nodes
+        arr = _abi.array_of(_abi.Boundary, bcs)
+        f = np.zeros(rows)
+        xD = np.zeros(rows)
+        inner = np.ones(rows, dtype=np.uint8)
+        lib().orc_boundary(self.h, dof, arr, _ptr(f, _f64p), _ptr(xD, _f64p), _ptr(inner, _u8p))
+        return f, xD, inner
+
+    def assemble(self, physics, models, materials, is_inner, symmetric=True, part=_abi.NLF_PART_FULL,
+                 faithful_reset=False) -> OracleMatrix:
+        marr = _abi.array_of(_abi.Model, models)
+        mat_t = _abi.ThermalMaterial if physics == 0 else _abi.ElasticMaterial
+        matarr = _abi.array_of(mat_t, materials)
+        inner = np.ascontiguousarray(is_inner, dtype=np.uint8)
+        h = lib().orc_assemble(self.h, physics, marr, C.cast(matarr, C.c_void_p), _ptr(inner, _u8p),
+                               int(symmetric), part, int(faithful_reset))
+        if not h:
+            raise RuntimeError(lib().orc_last_error().decode())
+        return OracleMatrix(h, inner.size)
+
+    def convection(self, matrix: OracleMatrix, bcs, is_inner):
+        arr = _abi.array_of(_abi.Boundary, bcs)
+        inner = np.ascontiguousarray(is_inner, dtype=np.uint8)
+        lib().orc_convection(self.h, matrix.h, arr, _ptr(inner, _u8p))
+
+
+def first_kind_rhs(matrix: OracleMatrix, is_inner, xD, f):
+    inner = np.ascontiguousarray(is_inner, dtype=np.uint8)
+    xD = np.ascontiguousarray(xD, dtype=np.float64)
+    f = np.array(f, dtype=np.float64)
+    lib().orc_first_kind_rhs(matrix.h, _ptr(inner, _u8p), _ptr(xD, _f64p), _ptr(f, _f64p))
+    return f
+
+
+class OraclePrecond:
+    def __init__(self, kind, matrix: OracleMatrix | None):
+        self.h = lib().orc_precond_create(kind, matrix.h if matrix is not None else None)
+        self.kind = kind
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().orc_precond_destroy(self.h)
+            self.h = None
+
+    def ok(self):
+        s = C.c_double()
+        return bool(lib().orc_precond_ok(self.h, C.byref(s))), s.value
+
+    def apply(self, r):
+        r = np.ascontiguousarray(r, dtype=np.float64)
+        z = np.zeros_like(r)
+        lib().orc_precond_apply(self.h, _ptr(r, _f64p), _ptr(z, _f64p), r.size)
+        return z
+
+
+def cg(matrix: OracleMatrix, precond: OraclePrecond, b, x0=None, tol=1e-15, max_it=10000):
+    """conjugate_gradient::solve restated; returns (x, iterations, residual)."""
+    b = np.ascontiguousarray(b, dtype=np.float64)
+    x = np.zeros_like(b) if x0 is None else np.array(x0, dtype=np.float64)
+    res = C.c_double()
+    it = lib().orc_cg(matrix.h, precond.h, _ptr(b, _f64p), _ptr(x, _f64p), tol, max_it, C.byref(res))
+    return x, it, res.value
+
+
+# ---------------------------------------------------------------------------------------------
+# Problem drivers (stationary_heat_equation_solver_2d.hpp:46-120, equilibrium_equation_2d.hpp:20-62)
+# ---------------------------------------------------------------------------------------------
+def thermal_problem(mesh, models, materials, bcs, precond=_abi.NLF_PRECOND_ILLT, tol=1e-12, max_it=100000,
+                    radii=None, faithful_reset=False):
+    """Solve the stationary thermal problem; returns a dict with neighbours, matrices, f and T."""
+    om = OracleMesh(mesh)
+    if radii is None:
+        radii = [[m.search_radius[0] or m.radius[0], m.search_radius[1] or m.radius[1]]
+                 if m.local_weight < 0.999 else [0.0, 0.0] for m in models]
+    off, ids, expanded = om.find_neighbours(radii)
+    f, xD, inner = om.boundary(bcs, dof=1)
+    K = om.assemble(0, models, materials, inner, symmetric=True, faithful_reset=faithful_reset)
+    om.convection(K, bcs, inner)
+    f = first_kind_rhs(K, inner, xD, f)
+    if precond == _abi.NLF_PRECOND_ILLT:
+        KL = om.assemble(0, models, materials, inner, symmetric=True, part=_abi.NLF_PART_LOCAL)
+        P = OraclePrecond(precond, KL)
+        if not P.ok()[0]:
+            P = OraclePrecond(_abi.NLF_PRECOND_IDENTITY, None)
+    else:
+        KL = None
+        P = OraclePrecond(precond, K if precond == _abi.NLF_PRECOND_JACOBI else None)
+    T, it, res = cg(K, P, f, tol=tol, max_it=max_it)
+    return dict(mesh=om, neighbours=(off, ids, expanded), K=K, KL=KL, f=f, xD=xD, inner=inner, T=T,
+                iterations=it, residual=res)
+
+
+def elastic_problem(mesh, models, materials, bcs, tol=1e-12, max_it=100000, radii=None):
+    om = OracleMesh(mesh)
+    if radii is None:
+        radii = [[m.radius[0], m.radius[1]] if m.local_weight < 0.999 else [0.0, 0.0] for m in models]
+    off, ids, expanded = om.find_neighbours(radii)
+    f, xD, inner = om.boundary(bcs, dof=2)
+    K = om.assemble(1, models, materials, inner, symmetric=True)
+    f = first_kind_rhs(K, inner, xD, f)
+    nonlocal_ = any(m.local_weight < 0.999 for m in models)
+    if nonlocal_:
+        KL = om.assemble(1, models, materials, inner, symmetric=True, part=_abi.NLF_PART_LOCAL)
+        P = OraclePrecond(_abi.NLF_PRECOND_ILLT, KL)
+        if not P.ok()[0]:
+            P = OraclePrecond(_abi.NLF_PRECOND_IDENTITY, None)
+    else:
+        KL = None
+        P = OraclePrecond(_abi.NLF_PRECOND_IDENTITY, None)
+    u, it, res = cg(K, P, f, tol=tol, max_it=max_it)
+    return dict(mesh=om, neighbours=(off, ids, expanded), K=K, KL=KL, f=f, xD=xD, inner=inner, u=u,
+                iterations=it, residual=res)
