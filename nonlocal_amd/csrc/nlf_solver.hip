@@ -303,33 +303,86 @@ __global__ __launch_bounds__(LEVEL_THREADS) void k_ic_factor(LevelArgs a, double
     }
 }
 
-// Forward U^T y = s.r then backward U z' = y, z = s.z' (level-scheduled, one workgroup).
-template<class X, class V>
-__global__ __launch_bounds__(LEVEL_THREADS) void k_ic_solve(LevelArgs fw, LevelArgs bw, const V* __restrict__ F,
-                                                            const X* __restrict__ s, const X* __restrict__ r,
+// One triangular sweep in level order.  Records (one per row) are stored contiguously in level
+// order so a level's loads are coalesced; a dependency on a row solved at most K levels earlier is
+// read from an LDS window of K x W slots (slot = (level mod K) * W + position in level), otherwise
+// (dep < 0) from global memory.
+template<class X>
+struct PackedSweep {
+    int levels, K, W;
+    const i32* lptr;      // [levels+1] record offsets
+    const i32* row;       // row of each record
+    const i32* off;       // [records+1] entry offsets
+    const i32* dep;       // window slot, or -(row+1)
+    const X* val;         // factor entries
+    const X* dinv;        // 1 / U_kk
+    const X* scale;       // s_k
+};
+
+// z = S U^-1 U^-T S r : forward U^T y = S r, backward U z' = y, z = S z'.  One workgroup.
+template<class X>
+__global__ __launch_bounds__(LEVEL_THREADS) void k_ic_solve(PackedSweep<X> fw, PackedSweep<X> bw, const X* __restrict__ r,
                                                             X* __restrict__ y, X* __restrict__ z,
                                                             const CGState* __restrict__ st) {
+    extern __shared__ unsigned char smem_raw[];
+    X* win = reinterpret_cast<X*>(smem_raw);
     if (st && st->done) return;
     for (int L = 0; L < fw.levels; ++L) {
-        for (i32 t = fw.lptr[L] + threadIdx.x; t < fw.lptr[L + 1]; t += blockDim.x) {
-            const i64 k = fw.lrows[t];
-            X v = s[k] * r[k];
-            for (i64 p = fw.outer[k]; p < fw.diag[k]; ++p) v -= X(F[p]) * y[fw.cols[p]];
-            y[k] = v / X(F[fw.diag[k]]);
+        const i32 b = fw.lptr[L], cnt = fw.lptr[L + 1] - b;
+        for (i32 t = threadIdx.x; t < cnt; t += blockDim.x) {
+            const i32 rec = b + t;
+            const i32 k = fw.row[rec];
+            X v = fw.scale[rec] * r[k];
+            for (i32 e = fw.off[rec]; e < fw.off[rec + 1]; ++e) {
+                const i32 d = fw.dep[e];
+                v -= fw.val[e] * (d >= 0 ? win[d] : y[-d - 1]);
+            }
+            v *= fw.dinv[rec];
+            if (fw.K) win[(L % fw.K) * fw.W + t] = v;
+            y[k] = v;
         }
         __syncthreads();
     }
     for (int L = 0; L < bw.levels; ++L) {
-        for (i32 t = bw.lptr[L] + threadIdx.x; t < bw.lptr[L + 1]; t += blockDim.x) {
-            const i64 k = bw.lrows[t];
+        const i32 b = bw.lptr[L], cnt = bw.lptr[L + 1] - b;
+        for (i32 t = threadIdx.x; t < cnt; t += blockDim.x) {
+            const i32 rec = b + t;
+            const i32 k = bw.row[rec];
             X v = y[k];
-            const i64 dk = bw.diag[k];
-            for (i64 p = dk + 1; p < bw.outer[k + 1]; ++p) v -= X(F[p]) * y[bw.cols[p]];
-            y[k] = v / X(F[dk]);
+            for (i32 e = bw.off[rec]; e < bw.off[rec + 1]; ++e) {
+                const i32 d = bw.dep[e];
+                v -= bw.val[e] * (d >= 0 ? win[d] : y[-d - 1]);
+            }
+            v *= bw.dinv[rec];
+            if (bw.K) win[(L % bw.K) * bw.W + t] = v;
+            y[k] = v;                  // z' (row k of the backward sweep reads only its own y[k])
+            z[k] = bw.scale[rec] * v;
         }
         __syncthreads();
     }
-    for (i64 i = threadIdx.x; i < fw.rows; i += blockDim.x) z[i] = s[i] * y[i];
+}
+
+// Gathers the factor into the packed sweep layout (values by source position, per-record 1/U_kk, s_k).
+__global__ void k_pack_entries(i64 n, const i64* __restrict__ src, const double* __restrict__ F, double* __restrict__ v64,
+                               float* __restrict__ v32) {
+    for (i64 e = i64(blockIdx.x) * blockDim.x + threadIdx.x; e < n; e += i64(gridDim.x) * blockDim.x) {
+        const double v = F[src[e]];
+        v64[e] = v;
+        v32[e] = float(v);
+    }
+}
+
+__global__ void k_pack_rows(i64 n, const i32* __restrict__ row, const i64* __restrict__ diag, const double* __restrict__ F,
+                            const double* __restrict__ s, double* __restrict__ dinv64, float* __restrict__ dinv32,
+                            double* __restrict__ s64, float* __restrict__ s32) {
+    for (i64 t = i64(blockIdx.x) * blockDim.x + threadIdx.x; t < n; t += i64(gridDim.x) * blockDim.x) {
+        const i32 k = row[t];
+        const double d = 1.0 / F[diag[k]];
+        dinv64[t] = d;
+        dinv32[t] = float(d);
+        s64[t] = s[k];
+        s32[t] = float(s[k]);
+    }
 }
 
 __global__ void k_transpose_pos(i64 rows, const i64* __restrict__ outer, const i32* __restrict__ cols, i64* __restrict__ tpos) {
@@ -370,7 +423,20 @@ struct nlf_precond_s {
     nlf::DevBuf<nlf::i32> cols, fw_rows, fw_ptr, bw_rows, bw_ptr;
     int fw_levels = 0, bw_levels = 0;
     nlf::DevBuf<double> F, scale, work;
-    nlf::DevBuf<float> F32, scale32, work32;
+    nlf::DevBuf<float> work32;
+    struct Sweep {
+        int levels = 0, K = 0, W = 0;
+        nlf::DevBuf<nlf::i32> lptr, row, off, dep;
+        nlf::DevBuf<double> val, dinv, scale;
+        nlf::DevBuf<float> val32, dinv32, scale32;
+        nlf::PackedSweep<double> f64() const {
+            return {levels, K, W, lptr.p, row.p, off.p, dep.p, val.p, dinv.p, scale.p};
+        }
+        nlf::PackedSweep<float> f32() const {
+            return {levels, K, W, lptr.p, row.p, off.p, dep.p, val32.p, dinv32.p, scale32.p};
+        }
+    } fw, bw;
+    int solve_threads = 64;
     // learned (set by nlf_learned.hip)
     void* learned = nullptr;
     void (*learned_destroy)(void*) = nullptr;
@@ -423,6 +489,67 @@ static void level_sets(const std::vector<i64>& outer, const std::vector<i32>& co
 
 unsigned grid_stride_blocks(i64 n) { return unsigned(std::min<i64>(std::max<i64>((n + 255) / 256, 1), 4096)); }
 
+constexpr size_t IC_WINDOW_BYTES = 64 * 1024;
+
+// Builds one packed sweep: records in level order, entries (lower part for the forward sweep,
+// strict upper part for the backward one) with their dependency slots; values are gathered on the
+// device from the factor by source position.
+static void build_sweep(const std::vector<i64>& outer, const std::vector<i32>& cols, const std::vector<i64>& diag,
+                        const std::vector<i32>& lrows, const std::vector<i32>& lptr, bool forward, const double* F,
+                        const double* scale, const i64* d_diag, nlf_precond_s::Sweep& sw, hipStream_t s) {
+    const i64 n = i64(diag.size());
+    const int levels = int(lptr.size()) - 1;
+    std::vector<i32> level_of(n), pos_of(n);
+    int W = 1;
+    for (int L = 0; L < levels; ++L) {
+        W = std::max(W, lptr[L + 1] - lptr[L]);
+        for (i32 t = lptr[L]; t < lptr[L + 1]; ++t) { level_of[lrows[t]] = L; pos_of[lrows[t]] = t - lptr[L]; }
+    }
+    int K = 1;
+    std::vector<i32> off(n + 1, 0);
+    for (i64 t = 0; t < n; ++t) {
+        const i32 k = lrows[t];
+        const i64 b = forward ? outer[k] : diag[k] + 1, e = forward ? diag[k] : outer[k + 1];
+        off[t + 1] = off[t] + i32(e - b);
+        for (i64 p = b; p < e; ++p) K = std::max(K, level_of[k] - level_of[cols[p]]);
+    }
+    // K + 1 slot blocks: the level being written must not share its block with a dependency K
+    // levels back that other threads of the same level are still reading
+    K += 1;
+    const bool window = size_t(K) * size_t(W) * sizeof(double) <= IC_WINDOW_BYTES;
+    std::vector<i32> dep(off[n]);
+    std::vector<i64> src(off[n]);
+    for (i64 t = 0; t < n; ++t) {
+        const i32 k = lrows[t];
+        const i64 b = forward ? outer[k] : diag[k] + 1, e = forward ? diag[k] : outer[k + 1];
+        i32 q = off[t];
+        for (i64 p = b; p < e; ++p, ++q) {
+            const i32 i = cols[p];
+            dep[q] = window ? (level_of[i] % K) * W + pos_of[i] : -(i + 1);
+            src[q] = p;
+        }
+    }
+    sw.levels = levels;
+    sw.K = window ? K : 0;
+    sw.W = W;
+    upload(sw.lptr, lptr.data(), lptr.size(), s);
+    upload(sw.row, lrows.data(), lrows.size(), s);
+    upload(sw.off, off.data(), off.size(), s);
+    upload(sw.dep, dep.data(), std::max<size_t>(dep.size(), 1), s);
+    DevBuf<i64> d_src;
+    upload(d_src, src.data(), std::max<size_t>(src.size(), 1), s);
+    const size_t ne = std::max<size_t>(src.size(), 1);
+    sw.val.alloc(ne); sw.val32.alloc(ne);
+    sw.dinv.alloc(size_t(n)); sw.dinv32.alloc(size_t(n)); sw.scale.alloc(size_t(n)); sw.scale32.alloc(size_t(n));
+    if (!src.empty())
+        hipLaunchKernelGGL(k_pack_entries, dim3(grid_stride_blocks(i64(src.size()))), dim3(256), 0, s, i64(src.size()),
+                           d_src.p, F, sw.val.p, sw.val32.p);
+    hipLaunchKernelGGL(k_pack_rows, dim3(grid_stride_blocks(n)), dim3(256), 0, s, n, sw.row.p, d_diag, F, scale,
+                       sw.dinv.p, sw.dinv32.p, sw.scale.p, sw.scale32.p);
+    NLF_HIP(hipGetLastError());
+    NLF_HIP(hipStreamSynchronize(s));
+}
+
 // Applies the preconditioner z = M^-1 r on the stream (FP64 or FP32 vectors).
 template<class X>
 void precond_apply(const nlf_precond_s* P, i64 n, const X* r, X* z, const CGState* st, hipStream_t s);
@@ -431,10 +558,11 @@ template<>
 void precond_apply<double>(const nlf_precond_s* P, i64 n, const double* r, double* z, const CGState* st, hipStream_t s) {
     if (P && P->kind == NLF_PRECOND_JACOBI)
         hipLaunchKernelGGL(k_scale<double>, dim3(grid_stride_blocks(n)), dim3(256), 0, s, n, P->invd.p, r, z);
-    else if (P && P->kind == NLF_PRECOND_ILLT)
-        hipLaunchKernelGGL((k_ic_solve<double, double>), dim3(1), dim3(LEVEL_THREADS), 0, s, fw_args(P), bw_args(P), P->F.p,
-                           P->scale.p, r, P->work.p, z, st);
-    else
+    else if (P && P->kind == NLF_PRECOND_ILLT) {
+        const size_t lds = size_t(std::max(P->fw.K * P->fw.W, P->bw.K * P->bw.W)) * sizeof(double);
+        hipLaunchKernelGGL((k_ic_solve<double>), dim3(1), dim3(P->solve_threads), lds, s, P->fw.f64(), P->bw.f64(), r,
+                           P->work.p, z, st);
+    } else
         NLF_HIP(hipMemcpyAsync(z, r, size_t(n) * sizeof(double), hipMemcpyDeviceToDevice, s));
 }
 
@@ -442,10 +570,11 @@ template<>
 void precond_apply<float>(const nlf_precond_s* P, i64 n, const float* r, float* z, const CGState* st, hipStream_t s) {
     if (P && P->kind == NLF_PRECOND_JACOBI)
         hipLaunchKernelGGL(k_scale<float>, dim3(grid_stride_blocks(n)), dim3(256), 0, s, n, P->invd32.p, r, z);
-    else if (P && P->kind == NLF_PRECOND_ILLT)
-        hipLaunchKernelGGL((k_ic_solve<float, float>), dim3(1), dim3(LEVEL_THREADS), 0, s, fw_args(P), bw_args(P), P->F32.p,
-                           P->scale32.p, r, P->work32.p, z, st);
-    else
+    else if (P && P->kind == NLF_PRECOND_ILLT) {
+        const size_t lds = size_t(std::max(P->fw.K * P->fw.W, P->bw.K * P->bw.W)) * sizeof(float);
+        hipLaunchKernelGGL((k_ic_solve<float>), dim3(1), dim3(P->solve_threads), lds, s, P->fw.f32(), P->bw.f32(), r,
+                           P->work32.p, z, st);
+    } else
         NLF_HIP(hipMemcpyAsync(z, r, size_t(n) * sizeof(float), hipMemcpyDeviceToDevice, s));
 }
 
@@ -706,12 +835,12 @@ int nlf_precond_create(nlf_matrix a, int kind, nlf_precond* out) {
             P->tpos.alloc(cols.size());
             hipLaunchKernelGGL(k_transpose_pos, dim3(blocks_for(n, 4)), dim3(256), 0, s, n, P->outer.p, P->cols.p, P->tpos.p);
             P->scale.alloc(size_t(n));
-            P->scale32.alloc(size_t(n));
             P->work.alloc(size_t(n));
             P->work32.alloc(size_t(n));
             P->F.alloc(cols.size());
+            DevBuf<float> scale32{size_t(n)};
             hipLaunchKernelGGL(k_scale_from_diag, dim3(blocks_for(n, 256)), dim3(256), 0, s, n, P->diag.p, a->val.p,
-                               P->scale.p, P->scale32.p);
+                               P->scale.p, scale32.p);
             DevBuf<int> fail{1};
             double shift = 0.0;
             P->ok = false;
@@ -726,10 +855,10 @@ int nlf_precond_create(nlf_matrix a, int kind, nlf_precond* out) {
                 if (!f) { P->ok = true; P->shift = shift; break; }
                 shift = shift == 0.0 ? 1e-3 : 2 * shift;
             }
-            P->F32.alloc(cols.size());
-            hipLaunchKernelGGL((k_convert<double, float>), dim3(grid_stride_blocks(i64(cols.size()))), dim3(256), 0, s,
-                               i64(cols.size()), P->F.p, P->F32.p);
-            NLF_HIP(hipGetLastError());
+            build_sweep(outer, cols, diag, fr, fp, true, P->F.p, P->scale.p, P->diag.p, P->fw, s);
+            build_sweep(outer, cols, diag, br, bp, false, P->F.p, P->scale.p, P->diag.p, P->bw, s);
+            const int widest = std::max(P->fw.W, P->bw.W);
+            P->solve_threads = std::min(LEVEL_THREADS, std::max(64, (widest + 63) / 64 * 64));
         } else {
             throw Error(NLF_ERR_UNSUPPORTED, "use nlf_learned_precond_create for the learned preconditioner");
         }

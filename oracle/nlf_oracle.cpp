@@ -1004,6 +1004,8 @@ void orc_quadrature(void* h, double* xq, double* jac, double* D) {
     if (D) std::copy(m.D.begin(), m.D.end(), D);
 }
 
+void orc_basis(int nn, double x, double y, double* N, double* Nx, double* Ny) { basis(nn, x, y, N, Nx, Ny); }
+
 void orc_tables(int nn, double* w, double* qN, double* qNxi, double* qNeta, double* N0) {
     const Tables2D t = make_tables(nn);
     std::copy(t.w.begin(), t.w.end(), w);

@@ -41,6 +41,7 @@ def lib():
         L.orc_centres.argtypes = [C.c_void_p, _f64p]
         L.orc_quadrature.argtypes = [C.c_void_p, _f64p, _f64p, _f64p]
         L.orc_tables.argtypes = [C.c_int, _f64p, _f64p, _f64p, _f64p, _f64p]
+        L.orc_basis.argtypes = [C.c_int, C.c_double, C.c_double, _f64p, _f64p, _f64p]
         L.orc_find_neighbours.restype = _i64
         L.orc_find_neighbours.argtypes = [C.c_void_p, _f64p, C.c_int, _f64p]
         L.orc_neighbours_export.argtypes = [C.c_void_p, _i64p, _i32p]
@@ -86,6 +87,15 @@ def tables(nn: int):
     N0 = np.zeros(nn)
     lib().orc_tables(nn, _ptr(w, _f64p), _ptr(qN, _f64p), _ptr(qx, _f64p), _ptr(qy, _f64p), _ptr(N0, _f64p))
     return w, qN.reshape(nn, nq), qx.reshape(nn, nq), qy.reshape(nn, nq), N0
+
+
+def basis(nn: int, x: float, y: float):
+    """Shape functions and their xi/eta derivatives at (x, y) of the reference element."""
+    N = np.zeros(nn)
+    Nx = np.zeros(nn)
+    Ny = np.zeros(nn)
+    lib().orc_basis(nn, x, y, _ptr(N, _f64p), _ptr(Nx, _f64p), _ptr(Ny, _f64p))
+    return N, Nx, Ny
 
 
 @dataclass

@@ -167,3 +167,20 @@ def test_solution(gpu_ctx, problem, precond, mode, bound):
     err = H.rel_l2(sol.x.cpu().numpy(), x_ref)
     print(f"  iterations {sol.iterations} true residual {sol.true_residual:.2e} rel err {err:.2e}", flush=True)
     assert err <= bound
+
+
+def test_cfg0_against_golden_fixture(gpu_ctx):
+    """The GPU path on the 40 x 40 workload against the committed oracle fixture."""
+    import hashlib
+    import os
+    G = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "golden.npz"))
+    problem = configs.cfg0()
+    g = _gpu_mesh(problem, gpu_ctx)
+    sol = api.stationary_heat_equation_solver_2d(g, problem.models, problem.materials, problem.bcs,
+                                                 precond=A.NLF_PRECOND_ILLT, tol=1e-14)
+    off, ids = g.neighbours()
+    h = hashlib.sha256()
+    h.update(np.ascontiguousarray(off).tobytes())
+    h.update(np.ascontiguousarray(ids).tobytes())
+    assert h.hexdigest() == str(G["cfg0_nb_sha"])
+    assert H.rel_l2(sol.x.cpu().numpy(), G["cfg0_solution"]) <= 1e-10
