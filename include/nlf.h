@@ -109,6 +109,7 @@ typedef struct nlf_mesh_s*    nlf_mesh;
 typedef struct nlf_matrix_s*  nlf_matrix;
 typedef struct nlf_precond_s* nlf_precond;
 typedef struct nlf_batch_s*   nlf_batch;
+typedef struct nlf_trainer_s* nlf_trainer;
 
 /* ---- context ---------------------------------------------------------------------------------
  * No reference counterpart (the reference is CPU-only); one context per host thread, `stream` may
@@ -178,6 +179,29 @@ int nlf_precond_export_factor(nlf_precond p, double* values, double* scale);
  * x holds the initial guess */
 int nlf_cg_solve(nlf_matrix a, nlf_precond p, const double* b_dev, double* x_dev, double tol, int64_t max_it, int mode,
                  nlf_solve_stats* stats);
+
+/* ---- learned preconditioner (no reference counterpart: an additional preconditioner_base
+ * implementation, used first with ILLT / Jacobi as the fallback).  A message-passing network (3
+ * layers, width 32) over the graph of the local matrix; weights are nlf_learned_num_params() floats. */
+int nlf_learned_num_params(void);
+int nlf_learned_init_weights(uint64_t seed, float* w_host);
+int nlf_learned_create(nlf_matrix a, nlf_matrix a_local, const nlf_model* models, nlf_precond* out);
+int nlf_learned_set_weights(nlf_precond p, const float* w_host);
+int nlf_learned_get_weights(nlf_precond p, float* w_host);
+int nlf_learned_export_graph(nlf_precond p, int64_t* outer, int32_t* cols, float* feat, float* s);
+/* loss ||M(r) - e||_A^2 / ||e||_A^2 for a pair with A e = r, and its weight gradient (host) */
+int nlf_learned_loss_grad(nlf_precond p, nlf_matrix a, const double* r_dev, const double* e_dev, double* loss,
+                          float* grad_host);
+/* training on the GPU: Adam over (residual, error) pairs of any number of problems */
+int nlf_trainer_create(nlf_ctx ctx, const float* w_host, nlf_trainer* out);
+int nlf_trainer_destroy(nlf_trainer t);
+int nlf_trainer_step(nlf_trainer t, nlf_precond graph, nlf_matrix a, const double* r_dev, const double* e_dev, float lr,
+                     double* loss);
+int nlf_trainer_get_weights(nlf_trainer t, float* w_host);
+/* flexible CG (conjugate_gradient.hpp:48-79 generalised to a variable preconditioner) with the
+ * learned preconditioner; on stagnation continues with PCG and `fallback` */
+int nlf_fcg_solve(nlf_matrix a, nlf_precond learned, nlf_precond fallback, const double* b_dev, double* x_dev, double tol,
+                  int64_t max_it, int mode, nlf_solve_stats* stats);
 
 #ifdef __cplusplus
 }

@@ -20,6 +20,7 @@ _dbl = C.c_double
 _i64p = C.POINTER(C.c_int64)
 _i32p = C.POINTER(C.c_int32)
 _f64p = C.POINTER(C.c_double)
+_f32p = C.POINTER(C.c_float)
 _u8p = C.POINTER(C.c_uint8)
 _pp = C.POINTER(C.c_void_p)
 
@@ -58,6 +59,18 @@ SIGNATURES = {
     "nlf_precond_apply": (_int, [_p, _p, _p]),
     "nlf_precond_export_factor": (_int, [_p, _f64p, _f64p]),
     "nlf_cg_solve": (_int, [_p, _p, _p, _p, _dbl, _i64, _int, C.POINTER(_abi.SolveStats)]),
+    "nlf_learned_num_params": (_int, []),
+    "nlf_learned_init_weights": (_int, [C.c_uint64, _f32p]),
+    "nlf_learned_create": (_int, [_p, _p, C.POINTER(_abi.Model), _pp]),
+    "nlf_learned_set_weights": (_int, [_p, _f32p]),
+    "nlf_learned_get_weights": (_int, [_p, _f32p]),
+    "nlf_learned_export_graph": (_int, [_p, _i64p, _i32p, _f32p, _f32p]),
+    "nlf_learned_loss_grad": (_int, [_p, _p, _p, _p, _f64p, _f32p]),
+    "nlf_trainer_create": (_int, [_p, _f32p, _pp]),
+    "nlf_trainer_destroy": (_int, [_p]),
+    "nlf_trainer_step": (_int, [_p, _p, _p, _p, _p, C.c_float, _f64p]),
+    "nlf_trainer_get_weights": (_int, [_p, _f32p]),
+    "nlf_fcg_solve": (_int, [_p, _p, _p, _p, _p, _dbl, _i64, _int, C.POINTER(_abi.SolveStats)]),
 }
 
 _LIB = None

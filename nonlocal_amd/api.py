@@ -31,6 +31,14 @@ def _ptr(a, t):
     return a.ctypes.data_as(t)
 
 
+def _destroy(name, h):
+    """Release a library handle; a no-op while the interpreter is shutting down."""
+    try:
+        getattr(load(), name)(h)
+    except Exception:       # module globals are already torn down at interpreter exit
+        pass
+
+
 def _dev(t: torch.Tensor):
     assert t.is_cuda and t.is_contiguous()
     return C.c_void_p(t.data_ptr())
@@ -62,7 +70,7 @@ class Context:
 
     def __del__(self):
         if getattr(self, "h", None):
-            load().nlf_ctx_destroy(self.h)
+            _destroy("nlf_ctx_destroy", self.h)
             self.h = None
 
 
@@ -96,7 +104,7 @@ class mesh_2d:
 
     def __del__(self):
         if getattr(self, "h", None):
-            load().nlf_mesh_destroy(self.h)
+            _destroy("nlf_mesh_destroy", self.h)
             self.h = None
 
     @property
@@ -154,7 +162,7 @@ class finite_element_matrix:
 
     def __del__(self):
         if getattr(self, "h", None):
-            load().nlf_matrix_destroy(self.h)
+            _destroy("nlf_matrix_destroy", self.h)
             self.h = None
 
     def info(self):
@@ -257,7 +265,7 @@ class preconditioner:
 
     def __del__(self):
         if getattr(self, "h", None):
-            load().nlf_precond_destroy(self.h)
+            _destroy("nlf_precond_destroy", self.h)
             self.h = None
 
     def info(self):

@@ -22,6 +22,26 @@
 
 namespace nlf {
 
+// Model part of a group's parameters: fix_parameters (read_model.hpp:186-192) zeroes the radii of
+// LOCAL groups and sets p1 = 1 when a radius vanishes; theory_type (nonlocal_constants.hpp:21-27).
+GroupParams model_group_params(const nlf_model& md, bool only_local) {
+    GroupParams gp;
+    std::memset(&gp, 0, sizeof(gp));
+    double p1 = md.local_weight;
+    double r0 = md.radius[0], r1 = md.radius[1];
+    if (p1 >= 0.999) { r0 = 0; r1 = 0; }
+    if (r0 < std::numeric_limits<double>::epsilon() || r1 < std::numeric_limits<double>::epsilon()) p1 = 1.0;
+    gp.p1 = p1;
+    gp.nonlocal = !only_local && p1 < 0.999;
+    if (gp.nonlocal) {
+        nlf_model m2 = md;
+        m2.radius[0] = r0; m2.radius[1] = r1;
+        NLF_CHECK(md.distance_n == 2 || md.distance_n == 0, NLF_ERR_UNSUPPORTED, "only the L2 distance is implemented");
+        gp.inf = make_influence(m2);
+    }
+    return gp;
+}
+
 InfluenceParams make_influence(const nlf_model& md) {
     InfluenceParams f;
     f.kind = md.influence;
@@ -51,34 +71,6 @@ InfluenceParams make_influence(const nlf_model& md) {
     return f;
 }
 
-__device__ __forceinline__ double ipow(double x, int e) {     // power.hpp:132-147
-    double r = 1.0;
-    while (e > 0) {
-        if (e & 1) { r *= x; --e; } else { x *= x; e >>= 1; }
-    }
-    return r;
-}
-
-__device__ __forceinline__ double exp_pow(double v, int is_int, int ei, double ed) {
-    if (!is_int) return pow(v, ed);
-    return ei == 1 ? v : ipow(v, ei);
-}
-
-// phi(x, y): normalised influence of the nonlocal radius (influence_functions_2d.hpp:39-148).
-__device__ __forceinline__ double influence(const InfluenceParams& f, double ax, double ay, double bx, double by) {
-    const double dx = ax - bx, dy = ay - by;
-    const double zx = dx * f.inv_r[0], zy = dy * f.inv_r[1];
-    double d = zx * zx + zy * zy;
-    if (fabs(d - 1.0) < 1e-12) {      // near the cutoff: the reference's division form decides
-        const double ex = dx / f.r[0], ey = dy / f.r[1];
-        d = ex * ex + ey * ey;
-    }
-    if (f.fast) return d < 1.0 ? f.norm * (1.0 - d) : 0.0;
-    if (f.kind == NLF_INFLUENCE_POLYNOMIAL)
-        return d < 1.0 ? f.norm * exp_pow(1.0 - exp_pow(d, f.p_int, f.p_i, f.p_d), f.q_int, f.q_i, f.q_d) : 0.0;
-    if (f.kind == NLF_INFLUENCE_CONSTANT) return d <= 1.0 ? f.norm : 0.0;
-    return f.norm * exp(f.exp_q * exp_pow(d, f.p_int, f.p_i, f.p_d));
-}
 
 // ---------------------------------------------------------------------------------------------
 // Pattern
@@ -570,20 +562,7 @@ namespace {
 constexpr int PATTERN_MAX_WORDS = 12288;   // 48 KiB bitmap: column node ids within a span of 393k
 
 GroupParams make_group(const nlf_model& md, int physics, const void* materials, int g, bool only_local) {
-    GroupParams gp;
-    std::memset(&gp, 0, sizeof(gp));
-    double p1 = md.local_weight;
-    double r0 = md.radius[0], r1 = md.radius[1];
-    if (p1 >= 0.999) { r0 = 0; r1 = 0; }                                             // fix_parameters
-    if (r0 < std::numeric_limits<double>::epsilon() || r1 < std::numeric_limits<double>::epsilon()) p1 = 1.0;
-    gp.p1 = p1;
-    gp.nonlocal = !only_local && p1 < 0.999;                                        // theory_type
-    if (gp.nonlocal) {
-        nlf_model m2 = md;
-        m2.radius[0] = r0; m2.radius[1] = r1;
-        NLF_CHECK(md.distance_n == 2 || md.distance_n == 0, NLF_ERR_UNSUPPORTED, "only the L2 distance is implemented");
-        gp.inf = make_influence(m2);
-    }
+    GroupParams gp = model_group_params(md, only_local);
     if (physics == 0) {
         const nlf_thermal_material& t = static_cast<const nlf_thermal_material*>(materials)[g];
         gp.ckind = t.kind;

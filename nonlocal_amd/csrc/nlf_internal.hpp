@@ -107,6 +107,35 @@ struct InfluenceParams {
 
 InfluenceParams make_influence(const nlf_model& m);
 
+__device__ __forceinline__ double ipow(double x, int e) {     // power.hpp:132-147 (integral exponent)
+    double r = 1.0;
+    while (e > 0) {
+        if (e & 1) { r *= x; --e; } else { x *= x; e >>= 1; }
+    }
+    return r;
+}
+
+__device__ __forceinline__ double exp_pow(double v, int is_int, int ei, double ed) {
+    if (!is_int) return pow(v, ed);
+    return ei == 1 ? v : ipow(v, ei);
+}
+
+// phi(x, y): normalised influence of the nonlocal radius (influence_functions_2d.hpp:39-148).
+__device__ __forceinline__ double influence(const InfluenceParams& f, double ax, double ay, double bx, double by) {
+    const double dx = ax - bx, dy = ay - by;
+    const double zx = dx * f.inv_r[0], zy = dy * f.inv_r[1];
+    double d = zx * zx + zy * zy;
+    if (fabs(d - 1.0) < 1e-12) {      // near the cutoff: the reference's division form decides
+        const double ex = dx / f.r[0], ey = dy / f.r[1];
+        d = ex * ex + ey * ey;
+    }
+    if (f.fast) return d < 1.0 ? f.norm * (1.0 - d) : 0.0;
+    if (f.kind == NLF_INFLUENCE_POLYNOMIAL)
+        return d < 1.0 ? f.norm * exp_pow(1.0 - exp_pow(d, f.p_int, f.p_i, f.p_d), f.q_int, f.q_i, f.q_d) : 0.0;
+    if (f.kind == NLF_INFLUENCE_CONSTANT) return d <= 1.0 ? f.norm : 0.0;
+    return f.norm * exp(f.exp_q * exp_pow(d, f.p_int, f.p_i, f.p_d));
+}
+
 struct GroupParams {                     // per element group, copied to device
     double p1;                           // local weight (1 for LOCAL groups)
     double c[3];                         // thermal conductivity components or Hooke {h11, h12, h66}
@@ -114,6 +143,8 @@ struct GroupParams {                     // per element group, copied to device
     int ckind;                           // conductivity kind
     InfluenceParams inf;
 };
+
+GroupParams model_group_params(const nlf_model& md, bool only_local);
 
 }  // namespace nlf
 

@@ -182,9 +182,30 @@ __global__ void k_axpby(i64 n, A alpha, const B* __restrict__ x, A beta, B* __re
 }
 
 // y += x (FP32 correction accumulated into the FP64 solution)
-template<class Y>
-__global__ void k_add_f32(i64 n, const float* __restrict__ x, Y* __restrict__ y) {
+template<class S, class Y>
+__global__ void k_add_to(i64 n, const S* __restrict__ x, Y* __restrict__ y) {
     for (i64 i = i64(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += i64(gridDim.x) * blockDim.x) y[i] += Y(x[i]);
+}
+
+// rz from partial sums; `first` also initialises the state of a reliable-update solve (no
+// convergence test inside: the host decides on the FP64 true residual).
+__global__ void k_set_rz(const double* __restrict__ partial, int n, int first, i64 max_it, CGState* __restrict__ st) {
+    __shared__ double sh[1024 / 64];
+    double s = 0.0;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) s += partial[i];
+    s = block_sum<double>(s, sh);
+    if (threadIdx.x == 0) {
+        st->rz = s;
+        st->done = 0;
+        if (first) {
+            st->it = 0;
+            st->max_it = max_it;
+            st->tol = -1.0;
+            st->bnorm = 1.0;
+            st->breakdown = 0;
+        }
+        st->res = sqrt(fabs(s));
+    }
 }
 
 template<class A, class B>
@@ -362,6 +383,104 @@ __global__ __launch_bounds__(LEVEL_THREADS) void k_ic_solve(PackedSweep<X> fw, P
     }
 }
 
+// ELL form of one sweep for the fast path: every level is a block of `cnt` records with exactly WE
+// entry slots each, stored slot-major inside the level (entry j of record t of level L at
+// WE * lptr[L] + j * cnt + t), so all loads of a level are coalesced and independent of each other;
+// padding slots carry value 0 and a dependency on slot 0 of the (zero-initialised) window.
+template<class X>
+struct EllSweep {
+    int levels, K, W;
+    const i32* lptr;
+    const i32* row;
+    const i32* dep;
+    const X* val;
+    const X* dinv;
+};
+
+// rhs_lvl[rec] = scale[rec] * r[row[rec]] (forward) or y[row[rec]] (backward, scale == nullptr);
+// also clears `clear` (the sweep's output, read through global dependencies).
+template<class X>
+__global__ void k_sweep_gather(i64 n, const i32* __restrict__ row, const X* __restrict__ scale, const X* __restrict__ src,
+                               X* __restrict__ rhs_lvl, X* __restrict__ clear, const CGState* __restrict__ st) {
+    if (st && st->done) return;
+    for (i64 t = i64(blockIdx.x) * blockDim.x + threadIdx.x; t < n; t += i64(gridDim.x) * blockDim.x) {
+        const i32 k = row[t];
+        rhs_lvl[t] = scale ? scale[t] * src[k] : src[k];
+        if (clear) clear[t] = X(0);
+    }
+}
+
+// One level-scheduled triangular sweep by one workgroup (blockDim >= widest level).  The loads of
+// level L+1 are issued before level L is solved, so only the LDS reads of the dependencies and the
+// barrier remain on the critical path.  out[row] = v; out2[row] = scale2[rec] * v when given.
+template<class X, int WE>
+__global__ __launch_bounds__(LEVEL_THREADS) void k_ic_sweep(EllSweep<X> s, const X* __restrict__ rhs, X* __restrict__ out,
+                                                            X* __restrict__ out2, const X* __restrict__ scale2,
+                                                            const CGState* __restrict__ st) {
+    extern __shared__ unsigned char smem_raw[];
+    X* win = reinterpret_cast<X*>(smem_raw);
+    if (st && st->done) return;
+    const int t = threadIdx.x;
+    if (s.K)
+        for (int i = t; i < s.K * s.W; i += blockDim.x) win[i] = X(0);
+    X c_rhs = 0, c_dinv = 0, c_val[WE];
+    i32 c_dep[WE], c_row = 0;
+    auto load = [&](int L, X& rv, X& dv, X* vv, i32* dd, i32& rw) {
+        const i32 b = s.lptr[L], cnt = s.lptr[L + 1] - b;
+        if (t < cnt) {
+            const i32 rec = b + t;
+            rv = rhs[rec];
+            dv = s.dinv[rec];
+            rw = s.row[rec];
+            const i64 e0 = i64(WE) * b + t;
+#pragma unroll
+            for (int j = 0; j < WE; ++j) {
+                vv[j] = s.val[e0 + i64(j) * cnt];
+                dd[j] = s.dep[e0 + i64(j) * cnt];
+            }
+        }
+    };
+    if (s.levels > 0) load(0, c_rhs, c_dinv, c_val, c_dep, c_row);
+    __syncthreads();
+    for (int L = 0; L < s.levels; ++L) {
+        X n_rhs = 0, n_dinv = 0, n_val[WE];
+        i32 n_dep[WE], n_row = 0;
+        if (L + 1 < s.levels) load(L + 1, n_rhs, n_dinv, n_val, n_dep, n_row);
+        const i32 cnt = s.lptr[L + 1] - s.lptr[L];
+        if (t < cnt) {
+            X v = c_rhs;
+#pragma unroll
+            for (int j = 0; j < WE; ++j) {
+                const i32 d = c_dep[j];
+                v -= c_val[j] * (d >= 0 ? win[d] : out[-d - 1]);
+            }
+            v *= c_dinv;
+            if (s.K) win[(L % s.K) * s.W + t] = v;
+            out[c_row] = v;
+            if (out2) out2[c_row] = scale2[s.lptr[L] + t] * v;
+        }
+        if (s.K) {
+            // every dependency lives in the LDS window: wait for LDS traffic only, so the global
+            // loads prefetched for the next level stay in flight across the barrier
+            asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
+        } else {
+            __syncthreads();
+        }
+        c_rhs = n_rhs; c_dinv = n_dinv; c_row = n_row;
+#pragma unroll
+        for (int j = 0; j < WE; ++j) { c_val[j] = n_val[j]; c_dep[j] = n_dep[j]; }
+    }
+}
+
+__global__ void k_pack_ell(i64 n, const i64* __restrict__ src, const double* __restrict__ F, double* __restrict__ v64,
+                           float* __restrict__ v32) {
+    for (i64 e = i64(blockIdx.x) * blockDim.x + threadIdx.x; e < n; e += i64(gridDim.x) * blockDim.x) {
+        const double v = src[e] >= 0 ? F[src[e]] : 0.0;
+        v64[e] = v;
+        v32[e] = float(v);
+    }
+}
+
 // Gathers the factor into the packed sweep layout (values by source position, per-record 1/U_kk, s_k).
 __global__ void k_pack_entries(i64 n, const i64* __restrict__ src, const double* __restrict__ F, double* __restrict__ v64,
                                float* __restrict__ v32) {
@@ -437,6 +556,16 @@ struct nlf_precond_s {
         }
     } fw, bw;
     int solve_threads = 64;
+    struct Ell {
+        int levels = 0, K = 0, W = 0, WE = 0, threads = 64;
+        nlf::DevBuf<nlf::i32> lptr, row, dep;
+        nlf::DevBuf<double> val, dinv, scale;
+        nlf::DevBuf<float> val32, dinv32, scale32;
+        template<class X> nlf::EllSweep<X> get() const;
+    } efw, ebw;
+    bool ell = false;
+    nlf::DevBuf<double> rhs_lvl, y1, y2;
+    nlf::DevBuf<float> rhs_lvl32, y1_32, y2_32;
     // learned (set by nlf_learned.hip)
     void* learned = nullptr;
     void (*learned_destroy)(void*) = nullptr;
@@ -444,6 +573,13 @@ struct nlf_precond_s {
         if (learned && learned_destroy) learned_destroy(learned);
     }
 };
+
+template<> nlf::EllSweep<double> nlf_precond_s::Ell::get<double>() const {
+    return {levels, K, W, lptr.p, row.p, dep.p, val.p, dinv.p};
+}
+template<> nlf::EllSweep<float> nlf_precond_s::Ell::get<float>() const {
+    return {levels, K, W, lptr.p, row.p, dep.p, val32.p, dinv32.p};
+}
 
 namespace nlf {
 
@@ -550,6 +686,99 @@ static void build_sweep(const std::vector<i64>& outer, const std::vector<i32>& c
     NLF_HIP(hipStreamSynchronize(s));
 }
 
+// ELL sweep for the fast kernel; false when a level is wider than one workgroup or rows have too
+// many entries (the generic packed sweep is used then).
+static bool build_ell(const std::vector<i64>& outer, const std::vector<i32>& cols, const std::vector<i64>& diag,
+                      const std::vector<i32>& lrows, const std::vector<i32>& lptr, bool forward, const double* F,
+                      const double* scale, const i64* d_diag, nlf_precond_s::Ell& el, hipStream_t s) {
+    const i64 n = i64(diag.size());
+    const int levels = int(lptr.size()) - 1;
+    std::vector<i32> level_of(n), pos_of(n);
+    int W = 1, WEmax = 0;
+    for (int L = 0; L < levels; ++L) {
+        W = std::max(W, lptr[L + 1] - lptr[L]);
+        for (i32 t = lptr[L]; t < lptr[L + 1]; ++t) { level_of[lrows[t]] = L; pos_of[lrows[t]] = t - lptr[L]; }
+    }
+    int K = 1;
+    for (i64 t = 0; t < n; ++t) {
+        const i32 k = lrows[t];
+        const i64 b = forward ? outer[k] : diag[k] + 1, e = forward ? diag[k] : outer[k + 1];
+        WEmax = std::max(WEmax, int(e - b));
+        for (i64 p = b; p < e; ++p) K = std::max(K, level_of[k] - level_of[cols[p]]);
+    }
+    K += 1;
+    int WE = 4;
+    while (WE < WEmax) WE *= 2;
+    if (W > LEVEL_THREADS || WE > 32) return false;
+    const bool window = size_t(K) * size_t(W) * sizeof(double) <= IC_WINDOW_BYTES;
+    std::vector<i32> dep(size_t(WE) * n);
+    std::vector<i64> src(size_t(WE) * n, -1);
+    for (int L = 0; L < levels; ++L) {
+        const i32 b = lptr[L], cnt = lptr[L + 1] - b;
+        for (i32 t = 0; t < cnt; ++t) {
+            const i32 k = lrows[b + t];
+            const i64 pb = forward ? outer[k] : diag[k] + 1, pe = forward ? diag[k] : outer[k + 1];
+            for (int j = 0; j < WE; ++j) {
+                const size_t e = size_t(WE) * b + size_t(j) * cnt + t;
+                if (pb + j < pe) {
+                    const i32 i = cols[pb + j];
+                    dep[e] = window ? (level_of[i] % K) * W + pos_of[i] : -(i + 1);
+                    src[e] = pb + j;
+                } else {
+                    dep[e] = window ? 0 : -1;       // value 0 times a finite slot
+                }
+            }
+        }
+    }
+    el.levels = levels;
+    el.K = window ? K : 0;
+    el.W = W;
+    el.WE = WE;
+    el.threads = std::min(LEVEL_THREADS, std::max(64, (W + 63) / 64 * 64));
+    upload(el.lptr, lptr.data(), lptr.size(), s);
+    upload(el.row, lrows.data(), lrows.size(), s);
+    upload(el.dep, dep.data(), dep.size(), s);
+    DevBuf<i64> d_src;
+    upload(d_src, src.data(), src.size(), s);
+    el.val.alloc(src.size()); el.val32.alloc(src.size());
+    hipLaunchKernelGGL(k_pack_ell, dim3(grid_stride_blocks(i64(src.size()))), dim3(256), 0, s, i64(src.size()), d_src.p,
+                       F, el.val.p, el.val32.p);
+    el.dinv.alloc(size_t(n)); el.dinv32.alloc(size_t(n)); el.scale.alloc(size_t(n)); el.scale32.alloc(size_t(n));
+    hipLaunchKernelGGL(k_pack_rows, dim3(grid_stride_blocks(n)), dim3(256), 0, s, n, el.row.p, d_diag, F, scale, el.dinv.p,
+                       el.dinv32.p, el.scale.p, el.scale32.p);
+    NLF_HIP(hipGetLastError());
+    NLF_HIP(hipStreamSynchronize(s));
+    return true;
+}
+
+template<class X>
+const X* ell_scale(const nlf_precond_s::Ell& e);
+template<> const double* ell_scale<double>(const nlf_precond_s::Ell& e) { return e.scale.p; }
+template<> const float* ell_scale<float>(const nlf_precond_s::Ell& e) { return e.scale32.p; }
+
+template<class X>
+void ell_sweep(const nlf_precond_s::Ell& e, const X* rhs, X* out, X* out2, const X* scale2, const CGState* st,
+               hipStream_t s) {
+    const size_t lds = size_t(std::max(e.K * e.W, 1)) * sizeof(X);
+    const EllSweep<X> a = e.template get<X>();
+    switch (e.WE) {
+        case 4: hipLaunchKernelGGL((k_ic_sweep<X, 4>), dim3(1), dim3(e.threads), lds, s, a, rhs, out, out2, scale2, st); break;
+        case 8: hipLaunchKernelGGL((k_ic_sweep<X, 8>), dim3(1), dim3(e.threads), lds, s, a, rhs, out, out2, scale2, st); break;
+        case 16: hipLaunchKernelGGL((k_ic_sweep<X, 16>), dim3(1), dim3(e.threads), lds, s, a, rhs, out, out2, scale2, st); break;
+        default: hipLaunchKernelGGL((k_ic_sweep<X, 32>), dim3(1), dim3(e.threads), lds, s, a, rhs, out, out2, scale2, st); break;
+    }
+}
+
+// z = S U^-1 U^-T S r through the ELL sweeps: gather, forward sweep, gather, backward sweep.
+template<class X>
+void ell_apply(const nlf_precond_s* P, i64 n, const X* r, X* z, X* rhs, X* y1, X* y2, const CGState* st, hipStream_t s) {
+    const unsigned g = grid_stride_blocks(n);
+    hipLaunchKernelGGL(k_sweep_gather<X>, dim3(g), dim3(256), 0, s, n, P->efw.row.p, ell_scale<X>(P->efw), r, rhs, y1, st);
+    ell_sweep<X>(P->efw, rhs, y1, (X*)nullptr, (const X*)nullptr, st, s);
+    hipLaunchKernelGGL(k_sweep_gather<X>, dim3(g), dim3(256), 0, s, n, P->ebw.row.p, (const X*)nullptr, y1, rhs, y2, st);
+    ell_sweep<X>(P->ebw, rhs, y2, z, ell_scale<X>(P->ebw), st, s);
+}
+
 // Applies the preconditioner z = M^-1 r on the stream (FP64 or FP32 vectors).
 template<class X>
 void precond_apply(const nlf_precond_s* P, i64 n, const X* r, X* z, const CGState* st, hipStream_t s);
@@ -558,6 +787,8 @@ template<>
 void precond_apply<double>(const nlf_precond_s* P, i64 n, const double* r, double* z, const CGState* st, hipStream_t s) {
     if (P && P->kind == NLF_PRECOND_JACOBI)
         hipLaunchKernelGGL(k_scale<double>, dim3(grid_stride_blocks(n)), dim3(256), 0, s, n, P->invd.p, r, z);
+    else if (P && P->kind == NLF_PRECOND_ILLT && P->ell)
+        ell_apply<double>(P, n, r, z, P->rhs_lvl.p, P->y1.p, P->y2.p, st, s);
     else if (P && P->kind == NLF_PRECOND_ILLT) {
         const size_t lds = size_t(std::max(P->fw.K * P->fw.W, P->bw.K * P->bw.W)) * sizeof(double);
         hipLaunchKernelGGL((k_ic_solve<double>), dim3(1), dim3(P->solve_threads), lds, s, P->fw.f64(), P->bw.f64(), r,
@@ -570,6 +801,8 @@ template<>
 void precond_apply<float>(const nlf_precond_s* P, i64 n, const float* r, float* z, const CGState* st, hipStream_t s) {
     if (P && P->kind == NLF_PRECOND_JACOBI)
         hipLaunchKernelGGL(k_scale<float>, dim3(grid_stride_blocks(n)), dim3(256), 0, s, n, P->invd32.p, r, z);
+    else if (P && P->kind == NLF_PRECOND_ILLT && P->ell)
+        ell_apply<float>(P, n, r, z, P->rhs_lvl32.p, P->y1_32.p, P->y2_32.p, st, s);
     else if (P && P->kind == NLF_PRECOND_ILLT) {
         const size_t lds = size_t(std::max(P->fw.K * P->fw.W, P->bw.K * P->bw.W)) * sizeof(float);
         hipLaunchKernelGGL((k_ic_solve<float>), dim3(1), dim3(P->solve_threads), lds, s, P->fw.f32(), P->bw.f32(), r,
@@ -710,6 +943,68 @@ struct CG {
         return hs.it;
     }
 
+    // Mixed precision with reliable updates: the CG recurrences run in X (FP32) on the X matrix;
+    // whenever the recursive residual has dropped by `delta` since the last update, the FP32
+    // solution increment is folded into the FP64 solution, the residual is recomputed in FP64 with
+    // the FP64 matrix and replaces the recursive one; the search direction is kept, so the Krylov
+    // process is not restarted.  Stops on the FP64 true residual ||b - A x|| / ||b|| <= tol.
+    i64 solve_reliable(const i64* o64, const i32* c64, const double* v64, const double* b, double* x, double tol,
+                       i64 max_it, double delta, double* true_res, i64* updates, int* breakdown) {
+        const unsigned rb = spmv_blocks(64, n);
+        DevBuf<double> r64{size_t(n)}, sums{2};
+        DevBuf<X> xh{size_t(n)};
+        auto true_residual = [&]() {
+            hipLaunchKernelGGL((k_residual<double, double>), dim3(rb), dim3(SPMV_THREADS), 0, s, n, o64, c64, v64, x, b,
+                               r64.p, part1.p, part2.p);
+            hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(1024), 0, s, part1.p, int(rb), sums.p);
+            hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(1024), 0, s, part2.p, int(rb), sums.p + 1);
+            double h[2];
+            download(h, sums.p, 2, s);
+            return h[1] > 0.0 ? std::sqrt(h[0] / h[1]) : 0.0;
+        };
+        // r = M^-1-preconditioned restart data from the FP64 residual: r32 = r64, z = M r32, rz = r32.z
+        auto load_residual = [&](bool first) {
+            hipLaunchKernelGGL((k_convert<double, X>), dim3(vblocks), dim3(256), 0, s, n, r64.p, r.p);
+            precond_apply<X>(P, n, r.p, z.p, nullptr, s);
+            if (first) NLF_HIP(hipMemcpyAsync(p.p, z.p, size_t(n) * sizeof(X), hipMemcpyDeviceToDevice, s));
+            hipLaunchKernelGGL(k_dot<X>, dim3(vblocks), dim3(SPMV_THREADS), 0, s, n, r.p, z.p, part1.p,
+                               (const CGState*)nullptr);
+            hipLaunchKernelGGL(k_set_rz, dim3(1), dim3(1024), 0, s, part1.p, int(vblocks), first ? 1 : 0, max_it, st.p);
+        };
+        i64 nupd = 0;
+        double rel = true_residual();
+        hipLaunchKernelGGL(k_zero<X>, dim3(vblocks), dim3(256), 0, s, n, xh.p);
+        CGState hs{};
+        if (rel > tol && max_it > 0) {
+            load_residual(true);
+            NLF_HIP(hipMemcpyAsync(&hs, st.p, sizeof(CGState), hipMemcpyDeviceToHost, s));
+            NLF_HIP(hipStreamSynchronize(s));
+            double last = std::sqrt(std::fabs(hs.rz));
+            for (;;) {
+                for (int k = 0; k < 8; ++k) iteration(xh.p, k == 0);
+                NLF_HIP(hipMemcpyAsync(&hs, st.p, sizeof(CGState), hipMemcpyDeviceToHost, s));
+                NLF_HIP(hipStreamSynchronize(s));
+                const double est = std::sqrt(std::fabs(hs.rz));
+                const bool stop = hs.it >= max_it || hs.breakdown;
+                if (!(est < delta * last) && !stop) continue;
+                hipLaunchKernelGGL((k_add_to<X, double>), dim3(vblocks), dim3(256), 0, s, n, xh.p, x);
+                hipLaunchKernelGGL(k_zero<X>, dim3(vblocks), dim3(256), 0, s, n, xh.p);
+                ++nupd;
+                rel = true_residual();
+                if (rel <= tol || stop) break;
+                load_residual(false);
+                NLF_HIP(hipMemcpyAsync(&hs, st.p, sizeof(CGState), hipMemcpyDeviceToHost, s));
+                NLF_HIP(hipStreamSynchronize(s));
+                last = std::sqrt(std::fabs(hs.rz));
+            }
+        }
+        timer.collect();
+        if (true_res) *true_res = rel;
+        if (updates) *updates = nupd;
+        if (breakdown) *breakdown = hs.breakdown;
+        return hs.it;
+    }
+
     void iteration(X* x, bool sample = false) {
         // z = A p ; alpha = rz / p.z
         if (sample) timer.start(0, s);
@@ -739,8 +1034,11 @@ struct CG {
 
 template<> const double* CG<double, double>::invd() const { return P->invd.p; }
 template<> const float* CG<float, float>::invd() const { return P->invd32.p; }
+template<> const double* CG<float, double>::invd() const { return P->invd.p; }
 
 }  // namespace nlf
+
+#include "nlf_learned.inl"
 
 using namespace nlf;
 
@@ -859,6 +1157,12 @@ int nlf_precond_create(nlf_matrix a, int kind, nlf_precond* out) {
             build_sweep(outer, cols, diag, br, bp, false, P->F.p, P->scale.p, P->diag.p, P->bw, s);
             const int widest = std::max(P->fw.W, P->bw.W);
             P->solve_threads = std::min(LEVEL_THREADS, std::max(64, (widest + 63) / 64 * 64));
+            P->ell = build_ell(outer, cols, diag, fr, fp, true, P->F.p, P->scale.p, P->diag.p, P->efw, s) &&
+                     build_ell(outer, cols, diag, br, bp, false, P->F.p, P->scale.p, P->diag.p, P->ebw, s);
+            if (P->ell) {
+                P->rhs_lvl.alloc(size_t(n)); P->y1.alloc(size_t(n)); P->y2.alloc(size_t(n));
+                P->rhs_lvl32.alloc(size_t(n)); P->y1_32.alloc(size_t(n)); P->y2_32.alloc(size_t(n));
+            }
         } else {
             throw Error(NLF_ERR_UNSUPPORTED, "use nlf_learned_precond_create for the learned preconditioner");
         }
@@ -889,7 +1193,12 @@ int nlf_precond_apply(nlf_precond p, const double* r, double* z) {
     if (!p) return NLF_ERR_ARGUMENT;
     return guarded(p->ctx, [&] {
         NLF_CHECK(p->rows > 0, NLF_ERR_STATE, "preconditioner has no size");
-        precond_apply<double>(p, p->rows, r, z, nullptr, p->ctx->stream);
+        if (p->kind == NLF_PRECOND_LEARNED) {
+            LearnedGraph& g = *learned_of(p);
+            learned_forward<double, double>(g, g.w.p, r, z, false, p->ctx->stream);
+        } else {
+            precond_apply<double>(p, p->rows, r, z, nullptr, p->ctx->stream);
+        }
         NLF_HIP(hipGetLastError());
     });
 }
@@ -931,32 +1240,15 @@ int nlf_cg_solve(nlf_matrix a, nlf_precond p, const double* b, double* x, double
             timings(cg.timer, sizeof(double), sizeof(double));
         } else {
             NLF_CHECK(a->val32.p, NLF_ERR_STATE, "mixed mode needs nlf_matrix_make_fp32 first");
-            CG<float, float> cg(s, n, a->nnz_full, a->outer.p, a->cols.p, a->val32.p, p);
-            DevBuf<double> r{size_t(n)};
-            DevBuf<float> r32{size_t(n)}, d32{size_t(n)};
-            const unsigned rb = spmv_blocks(64, n);
-            DevBuf<double> prr{rb}, pbb{rb}, sums{2};
-            const double inner_tol = 1e-5;
-            for (i64 outer = 0; outer < 100; ++outer) {
-                hipLaunchKernelGGL((k_residual<double, double>), dim3(rb), dim3(SPMV_THREADS), 0, s, n, a->outer.p, a->cols.p,
-                                   a->val.p, x, b, r.p, prr.p, pbb.p);
-                hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(1024), 0, s, prr.p, int(rb), sums.p);
-                hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(1024), 0, s, pbb.p, int(rb), sums.p + 1);
-                double h[2];
-                download(h, sums.p, 2, s);
-                out.true_residual = std::sqrt(h[0]) / std::sqrt(h[1]);
-                if (out.true_residual <= tol || out.iterations >= max_it) break;
-                out.outer_iterations++;
-                hipLaunchKernelGGL((k_convert<double, float>), dim3(grid_stride_blocks(n)), dim3(256), 0, s, n, r.p, r32.p);
-                hipLaunchKernelGGL(k_zero<float>, dim3(grid_stride_blocks(n)), dim3(256), 0, s, n, d32.p);
-                double res = 0.0;
-                out.iterations += cg.solve(r32.p, d32.p, inner_tol, max_it - out.iterations, &res, &breakdown);
-                out.residual = res;
-                // x += d (FP64 accumulation)
-                hipLaunchKernelGGL((k_add_f32<double>), dim3(grid_stride_blocks(n)), dim3(256), 0, s, n, d32.p, x);
-                NLF_HIP(hipGetLastError());
-            }
-            timings(cg.timer, sizeof(float), sizeof(float));
+            // FP32 matrix values (the bytes that bound the SpMV) with FP64 vectors and recurrences: FP32
+            // CG recurrences lose orthogonality at the conditioning of these operators (kappa ~ 1e6)
+            // and need several times the iterations, while FP32 storage costs none; the FP64 true
+            // residual with the FP64 matrix replaces the recursive one at every reliable update.
+            CG<float, double> cg(s, n, a->nnz_full, a->outer.p, a->cols.p, a->val32.p, p);
+            out.iterations = cg.solve_reliable(a->outer.p, a->cols.p, a->val.p, b, x, tol, max_it, 0.1,
+                                               &out.true_residual, &out.outer_iterations, &breakdown);
+            out.residual = out.true_residual;
+            timings(cg.timer, sizeof(float), sizeof(double));
         }
         // FP64 true residual ||b - A x|| / ||b||
         {

@@ -317,3 +317,10 @@ def elastic_problem(mesh, models, materials, bcs, tol=1e-12, max_it=100000, radi
     u, it, res = cg(K, P, f, tol=tol, max_it=max_it)
     return dict(mesh=om, neighbours=(off, ids, expanded), K=K, KL=KL, f=f, xD=xD, inner=inner, u=u,
                 iterations=it, residual=res)
+
+
+def influence(model, x, y) -> float:
+    """phi(x, y) of one model (influence_functions_2d.hpp:39-148), as the assembly evaluates it."""
+    xa = np.ascontiguousarray(x, dtype=np.float64)
+    ya = np.ascontiguousarray(y, dtype=np.float64)
+    return lib().orc_influence(C.byref(model), _ptr(xa, _f64p), _ptr(ya, _f64p))

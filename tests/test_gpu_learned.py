@@ -1,0 +1,194 @@
+"""GPU tests of the learned preconditioner: features, forward, gradients against the FP64 oracle,
+training, flexible CG, fallback, and the gain of the trained network over the method without it."""
+import os
+
+import numpy as np
+import pytest
+import scipy.sparse as sp
+import scipy.sparse.linalg as spla
+import torch
+
+from nonlocal_amd import _abi as A
+from nonlocal_amd import api, configs, learned
+from oracle import learned_oracle as LO
+from oracle import oracle as O
+from tests.helpers import oracle_assembly, rel_l2, upper_to_full
+
+pytestmark = pytest.mark.gpu
+
+
+def _full(csr, n):
+    return sp.csr_matrix((csr.val, csr.inner, csr.outer), shape=(n, n))
+
+
+@pytest.fixture(scope="module")
+def small(gpu_ctx):
+    """16 x 16 composite with three inclusions (r = 0.3): GPU problem and its exported graph."""
+    print("[learned] small composite 16x16", flush=True)
+    tp = learned.TrainingProblem(16, seed=1, ctx=gpu_ctx, count=3, r=0.3)
+    n = tp.problem.mesh.n_nodes
+    Kf = _full(tp.K.export(2), n)
+    return tp, Kf
+
+
+def test_graph_and_features_match_oracle(small):
+    tp, Kf = small
+    p = tp.problem
+    n = p.mesh.n_nodes
+    outer, cols, feat, s = tp.learned.graph()
+    _, K, _, _, _ = oracle_assembly(p)
+    _, KL, _, _, _ = oracle_assembly(p, part=A.NLF_PART_LOCAL, convection=False)
+    Ko = upper_to_full(K.export(0), n)
+    KLo = upper_to_full(KL.export(0), n)
+    np.testing.assert_array_equal(outer, KLo.indptr)
+    np.testing.assert_array_equal(cols, KLo.indices)
+    groups = LO.node_groups(p.mesh.elems, p.mesh.group_begin, n)
+    fo, so = LO.features(KLo.indptr.astype(np.int64), KLo.indices.astype(np.int64), KLo.data, Ko, p.mesh.xy, groups,
+                         p.models, O.influence)
+    np.testing.assert_allclose(s, so, rtol=2e-7)
+    for k in range(4):
+        scale = max(1.0, np.abs(fo[:, k]).max())
+        assert np.abs(feat[:, k] - fo[:, k]).max() <= 1e-6 * scale, k
+
+
+def _gpu_weights(seed, out_scale=0.2):
+    rng = np.random.default_rng(seed)
+    w = learned.init_weights(seed).astype(np.float64)
+    w[LO.P_WOUT:LO.P_WOUT + LO.H] = rng.uniform(-out_scale, out_scale, LO.H)
+    w[LO.P_BOUT] = 0.05
+    return w.astype(np.float32)
+
+
+def test_forward_matches_oracle(small, gpu_ctx):
+    tp, Kf = small
+    outer, cols, feat, s = tp.learned.graph()
+    w = _gpu_weights(3)
+    tp.learned.set_weights(w)
+    np.testing.assert_array_equal(tp.learned.get_weights(), w)
+    r = np.random.default_rng(4).standard_normal(len(s))
+    z = tp.learned.solve(torch.from_numpy(r).to(gpu_ctx.device)).cpu().numpy()
+    zo = LO.forward(w.astype(np.float64), outer, cols, feat.astype(np.float64), s.astype(np.float64), r)
+    assert rel_l2(z, zo) < 1e-5
+
+
+def test_zero_output_is_jacobi(small, gpu_ctx):
+    tp, Kf = small
+    w = _gpu_weights(5)
+    w[LO.P_WOUT:] = 0.0
+    tp.learned.set_weights(w)
+    r = np.random.default_rng(6).standard_normal(Kf.shape[0])
+    z = tp.learned.solve(torch.from_numpy(r).to(gpu_ctx.device)).cpu().numpy()
+    assert rel_l2(z, r / Kf.diagonal()) < 1e-6
+
+
+@pytest.mark.parametrize("seed", [0, 1])
+def test_gradient_matches_oracle(small, gpu_ctx, seed):
+    tp, Kf = small
+    outer, cols, feat, s = tp.learned.graph()
+    w = _gpu_weights(20 + seed)
+    tp.learned.set_weights(w)
+    rng = np.random.default_rng(30 + seed)
+    r = rng.standard_normal(Kf.shape[0])
+    e = spla.spsolve(Kf.tocsc(), r)
+    dev = gpu_ctx.device
+    loss, g = tp.learned.loss_grad(torch.from_numpy(r).to(dev), torch.from_numpy(e).to(dev))
+    lo, go = LO.loss_grad(w.astype(np.float64), outer, cols, feat.astype(np.float64), s.astype(np.float64), Kf, r, e)
+    print(f"[learned] loss gpu {loss:.8e} oracle {lo:.8e}; grad rel {rel_l2(g, go):.2e}", flush=True)
+    assert abs(loss - lo) <= 1e-5 * abs(lo)
+    assert rel_l2(g, go) < 1e-4
+    # per block too, so a wrong small block cannot hide behind a large one
+    for b, size in [(LO.P_AIN, 2 * LO.H), (LO.P_LAYER0, LO.P_LAYER_SIZE), (LO.P_WOUT, LO.H + 1)]:
+        assert rel_l2(g[b:b + size], go[b:b + size]) < 1e-3, b
+
+
+def test_gradient_deterministic(small, gpu_ctx):
+    tp, Kf = small
+    tp.learned.set_weights(_gpu_weights(40))
+    r = torch.from_numpy(np.random.default_rng(41).standard_normal(Kf.shape[0])).to(gpu_ctx.device)
+    e = torch.from_numpy(spla.spsolve(Kf.tocsc(), r.cpu().numpy())).to(gpu_ctx.device)
+    l1, g1 = tp.learned.loss_grad(r, e)
+    l2, g2 = tp.learned.loss_grad(r, e)
+    assert l1 == l2
+    np.testing.assert_array_equal(g1, g2)
+
+
+@pytest.fixture(scope="module")
+def trained_small(gpu_ctx):
+    """A 32 x 32 composite, its (r, e) pairs and 150 Adam steps from the Jacobi start."""
+    print("[learned] training problem 32x32", flush=True)
+    tp = learned.TrainingProblem(32, seed=2, ctx=gpu_ctx, count=4)
+    tp.make_pairs(np.random.default_rng(0), n_rhs=2, snapshots=(0, 10, 40))
+    w, hist = learned.train([tp], steps=150, lr=2e-3, seed=0, log=None)
+    return tp, w, hist
+
+
+def test_training_reduces_loss(trained_small):
+    tp, w, hist = trained_small
+    first, last = np.mean(hist[:10]), np.mean(hist[-20:])
+    print(f"[learned] loss {first:.4f} -> {last:.4f}", flush=True)
+    assert np.all(np.isfinite(hist))
+    assert last < 0.9 * first
+
+
+def test_fcg_converges_to_direct_solution(trained_small):
+    tp, w, _ = trained_small
+    tp.learned.set_weights(w)
+    x, st = learned.fcg_solve(tp.K, tp.learned, tp.b, tol=1e-11, max_it=5000)
+    n = tp.b.numel()
+    Kf = _full(tp.K.export(2), n)
+    xd = spla.spsolve(Kf.tocsc(), tp.b.cpu().numpy())
+    print(f"[learned] fcg it {st.iterations} fallbacks {st.fallbacks} true res {st.true_residual:.2e}", flush=True)
+    assert st.fallbacks == 0
+    assert st.true_residual <= 1e-11
+    assert rel_l2(x.cpu().numpy(), xd) < 1e-9
+
+
+def test_fcg_mixed_values(trained_small):
+    tp, w, _ = trained_small
+    tp.learned.set_weights(w)
+    tp.K.make_fp32()
+    x, st = learned.fcg_solve(tp.K, tp.learned, tp.b, tol=1e-10, max_it=20000, mode=A.NLF_MODE_MIXED)
+    xd, st2 = learned.fcg_solve(tp.K, tp.learned, tp.b, tol=1e-12, max_it=20000)
+    assert st.true_residual <= 1e-10
+    assert rel_l2(x.cpu().numpy(), xd.cpu().numpy()) < 1e-8
+
+
+def test_poisoned_weights_fall_back(trained_small):
+    """Non-finite network output: flexible CG stops and the Jacobi-PCG fallback finishes the solve."""
+    tp, w, _ = trained_small
+    bad = w.copy()
+    bad[LO.P_BOUT] = np.nan
+    tp.learned.set_weights(bad)
+    jac = api.preconditioner(tp.K, A.NLF_PRECOND_JACOBI)
+    x, st = learned.fcg_solve(tp.K, tp.learned, tp.b, fallback=jac, tol=1e-10, max_it=20000)
+    tp.learned.set_weights(w)
+    assert st.fallbacks == 1
+    assert st.preconditioner == A.NLF_PRECOND_JACOBI
+    assert st.true_residual <= 1e-10
+    assert torch.isfinite(x).all()
+
+
+def test_trained_network_gain(gpu_ctx):
+    """Committed trained weights on a held-out composite (seed not used in training): flexible CG with
+    the network needs fewer iterations than the same FCG with the network switched off (zero output
+    weights, i.e. Jacobi) and than Jacobi-PCG."""
+    if not os.path.exists(learned.DEFAULT_WEIGHTS):
+        pytest.skip("no trained weights committed")
+    w, meta = learned.load_weights()
+    tp = learned.TrainingProblem(96, seed=1234, ctx=gpu_ctx)
+    tp.learned.set_weights(w)
+    _, st = learned.fcg_solve(tp.K, tp.learned, tp.b, tol=1e-10, max_it=20000)
+    off = w.copy()
+    off[LO.P_WOUT:] = 0.0
+    tp.learned.set_weights(off)
+    _, st0 = learned.fcg_solve(tp.K, tp.learned, tp.b, tol=1e-10, max_it=20000)
+    cg = api.conjugate_gradient(tp.K)
+    cg.init_preconditioner(A.NLF_PRECOND_JACOBI)
+    cg.set_tolerance(1e-10)
+    cg.set_max_interations(20000)
+    cg.solve(tp.b)
+    print(f"[learned] held-out 96x96: trained {st.iterations} it, network off {st0.iterations} it, "
+          f"Jacobi-PCG {cg.iterations()} it", flush=True)
+    assert st.fallbacks == 0 and st.true_residual <= 1e-10
+    assert st.iterations < 0.8 * st0.iterations
+    assert st.iterations < 0.8 * cg.iterations()
