@@ -4,14 +4,20 @@
 // Graph: the nodes of the thermal problem; edges = the pattern of the LOCAL matrix (element
 // adjacency, self loops included).  Edge features (symmetric in u, v):
 //   f0 = K_L(u,v) s_u s_v, f1 = K(u,v) s_u s_v, f2 = (1-p1) phi(x_u, x_v) / phi(0), f3 = [u == v],
-// with s = 1/sqrt(diag K).  Network (H = 32, three layers, FP32):
+// with s = 1/sqrt(diag K).  Network N (H = 32 channels, three message-passing layers, FP32):
 //   x_u = s_u r_u / sigma,  sigma = rms(S r)
-//   h0  = relu(a_in x + b_in)
-//   m_u = sum_{v in N(u)} (W_e f_uv) (.) h_v ;  h' = relu(W_s h_u + W_m m_u + b)      (x3)
-//   y_u = w_out . h3_u + b_out ;  z_u = s_u sigma (x_u + y_u)
-// so z = D^-1 r (Jacobi) when the output weights are zero; the network learns the correction.
-// Training minimises ||z(r) - e||_A^2 / ||e||_A^2 over (residual, error) pairs (A e = r).
+//   h0  = a_in x_u
+//   m_u = sum_{v in N(u)} (W_e f_uv) (.) h_v ;  h' = W_s h_u + W_m m_u           (x3)
+//   y_u = w_out . h3_u ;  N(r)_u = s_u sigma (x_u + y_u) = s_u^2 r_u + s_u sigma y_u  (first term in FP64)
+// The message path is linear in the residual (edge gates W_e f carry the mesh, the influence
+// function and the coefficients), so N = S^2 + S Y S is a fixed linear operator with a three-hop
+// stencil and flexible CG keeps the convergence of PCG.  The preconditioner applied in the Krylov
+// loop is its symmetric part  M r = S^2 r + S (Y + Y^T)/2 S r : Y^T runs the layers backwards
+// (transposed weights, same gates over the symmetric graph).  With zero output weights M is Jacobi.
+// Training minimises ||N(r) - e||_A^2 / ||e||_A^2 over (residual, error) pairs (A e = r).
 // Gradients are reduced per block in a fixed order (deterministic) and applied with Adam.
+// Parameter layout keeps slots for biases (P_BIN, P_B, P_BOUT); they are held at zero so that the
+// operator stays linear.
 
 namespace nlf {
 
@@ -33,7 +39,13 @@ struct LearnedGraph {
     DevBuf<float> s;                            // 1/sqrt(K_uu)
     DevBuf<float> w;                            // NPARAM
     DevBuf<float> x;                            // scaled residual
-    DevBuf<float> h[MP_L + 1], a[MP_L + 1], m[MP_L];
+    DevBuf<float> y;                            // forward output y = w_out . h3
+    DevBuf<float> h[MP_L + 1], m[MP_L];         // training: all activations; inference: h[0], h[1]
+    DevBuf<float> d;                            // transposed pass: own-node part (in place)
+    int ell_d = 0;                              // inference graph: ELL, k-major, D = longest row
+    DevBuf<i32> ell_col;
+    DevBuf<float4> ell_f;
+    DevBuf<float> v;                            // 64 w_out-contracted weights of the last layer
     DevBuf<float> g0, g1, gdir, gm;             // backward work
     DevBuf<double> part, sigma, zbuf, az;
     DevBuf<float> gpart, grad;
@@ -115,65 +127,299 @@ __global__ void k_lf_sigma(const double* __restrict__ partial, int nb, i64 n, do
 template<class R>
 __global__ __launch_bounds__(MP_THREADS) void k_lf_input(i64 n, const R* __restrict__ r, const float* __restrict__ s,
                                                          const double* __restrict__ sigma, const float* __restrict__ w,
-                                                         float* __restrict__ x, float* __restrict__ h0, float* __restrict__ a0) {
+                                                         float* __restrict__ x, float* __restrict__ h0) {
     const int c = threadIdx.x % MP_H;
     const float inv = float(1.0 / *sigma);
+    const float a = w[P_AIN + c];
     for (i64 u = i64(blockIdx.x) * MP_NODES + threadIdx.x / MP_H; u < n; u += i64(gridDim.x) * MP_NODES) {
         const float xu = float(double(s[u]) * double(r[u])) * inv;
-        const float av = w[P_AIN + c] * xu + w[P_BIN + c];
-        h0[u * MP_H + c] = fmaxf(av, 0.0f);
-        if (a0) a0[u * MP_H + c] = av;
+        h0[u * MP_H + c] = a * xu;
         if (c == 0) x[u] = xu;
     }
 }
 
-// One message-passing layer; a (pre-activation) and m (messages) are stored when given (training).
-__global__ __launch_bounds__(MP_THREADS) void k_lf_layer(i64 n, const i64* __restrict__ outer, const i32* __restrict__ cols,
-                                                         const float4* __restrict__ feat, const float* __restrict__ wl,
-                                                         const float* __restrict__ hin, float* __restrict__ hout,
-                                                         float* __restrict__ aout, float* __restrict__ mout) {
-    __shared__ float Ws[MP_H][MP_H + 1], Wm[MP_H][MP_H + 1], We[MP_H][MP_F], bb[MP_H];
+// ---- message-passing layers ----------------------------------------------------------------------
+// Block = 256 threads = 8 slots x 32 channels; each slot carries MP_NPL nodes per iteration, so
+// every lane keeps MP_NPL independent gather streams in flight (the gathers of one node form a
+// dependent chain: row offsets -> columns -> neighbour vectors).  Edge columns of the 8*MP_NPL
+// nodes are staged in LDS (local pattern rows are at most MP_MAXDEG long); edge features are read
+// directly (one 16-byte broadcast per edge and node).  The 32 x 32 products read the weights and
+// the node vectors from LDS as float4, each weight float4 serving MP_NPL nodes.
+constexpr int MP_MAXDEG = 32;
+constexpr int MP_NPL = 4;                       // nodes per lane and iteration
+constexpr int MP_TILE = MP_NODES * MP_NPL;      // nodes per block iteration
+constexpr int WPAD = MP_H + 4;                  // LDS row stride of a weight matrix (float4 rows)
+
+struct __align__(16) LayerLds {
+    float A[MP_H][WPAD], B[MP_H][WPAD];         // two 32 x 32 matrices (as stored or transposed)
+    float va[MP_TILE][MP_H], vb[MP_TILE][MP_H]; // node vectors for the products
+    int ecol[MP_TILE][MP_MAXDEG];               // staged edge columns
+};
+
+// W row-major [out][in] from global into LDS, as stored (dst[o][i]) or transposed (dst[i][o]).
+template<bool T>
+__device__ __forceinline__ void lds_load_w(float (*dst)[WPAD], const float* __restrict__ src) {
     for (int i = threadIdx.x; i < MP_H * MP_H; i += blockDim.x) {
-        Ws[i / MP_H][i % MP_H] = wl[P_WS + i];
-        Wm[i / MP_H][i % MP_H] = wl[P_WM + i];
+        const int o = i / MP_H, k = i % MP_H;
+        if (T) dst[k][o] = src[i]; else dst[o][k] = src[i];
     }
-    for (int i = threadIdx.x; i < MP_H * MP_F; i += blockDim.x) We[i / MP_F][i % MP_F] = wl[P_WE + i];
-    if (threadIdx.x < MP_H) bb[threadIdx.x] = wl[P_B + threadIdx.x];
-    __syncthreads();
-    const int c = threadIdx.x % MP_H;
-    const float e0 = We[c][0], e1 = We[c][1], e2 = We[c][2], e3 = We[c][3];
-    // all lanes of a wave stay in the loop (both 32-lane halves) so the width-32 shuffles are uniform
-    const i64 nn = (n + MP_NODES - 1) / MP_NODES * MP_NODES;
-    for (i64 u0 = i64(blockIdx.x) * MP_NODES; u0 < nn; u0 += i64(gridDim.x) * MP_NODES) {
-        const i64 u = u0 + threadIdx.x / MP_H;
-        const bool live = u < n;
-        float hu = 0.0f, m = 0.0f;
-        if (live) {
-            hu = hin[u * MP_H + c];
-            for (i64 p = outer[u]; p < outer[u + 1]; ++p) {
-                const float4 f = feat[p];
-                const float g = e0 * f.x + e1 * f.y + e2 * f.z + e3 * f.w;
-                m += g * hin[i64(cols[p]) * MP_H + c];
-            }
-        }
-        float acc = bb[c];
+}
+
+// sum_k W[c][k] v[k] (v broadcast from LDS)
+__device__ __forceinline__ float lds_dot(const float (*W)[WPAD], const float* v, int c) {
+    float acc = 0.0f;
 #pragma unroll
-        for (int k = 0; k < MP_H; ++k) {
-            acc += Ws[c][k] * __shfl(hu, k, MP_H);
-            acc += Wm[c][k] * __shfl(m, k, MP_H);
-        }
-        if (live) {
-            hout[u * MP_H + c] = fmaxf(acc, 0.0f);
-            if (aout) aout[u * MP_H + c] = acc;
-            if (mout) mout[u * MP_H + c] = m;
+    for (int k = 0; k < MP_H; k += 4) {
+        const float4 w = *reinterpret_cast<const float4*>(&W[c][k]);
+        const float4 x = *reinterpret_cast<const float4*>(&v[k]);
+        acc += w.x * x.x + w.y * x.y + w.z * x.z + w.w * x.w;
+    }
+    return acc;
+}
+
+// out[j] = sum_k W[c][k] v[slot + 8 j][k] for the MP_NPL nodes of a slot; accumulates into out
+__device__ __forceinline__ void lds_dot_tile(const float (*W)[WPAD], const float (*v)[MP_H], int slot, int c,
+                                             float (&out)[MP_NPL]) {
+#pragma unroll 2
+    for (int k = 0; k < MP_H; k += 4) {
+        const float4 w = *reinterpret_cast<const float4*>(&W[c][k]);
+#pragma unroll
+        for (int j = 0; j < MP_NPL; ++j) {
+            const float4 x = *reinterpret_cast<const float4*>(&v[slot + MP_NODES * j][k]);
+            out[j] += w.x * x.x + w.y * x.y + w.z * x.z + w.w * x.w;
         }
     }
 }
 
-// z_u = s_u sigma (x_u + w_out . h_u + b_out)
+// Row offsets of the slot's nodes and their columns into LDS (lane c stages edge c of each node).
+__device__ __forceinline__ void stage_tile(LayerLds& L, i64 u0, int slot, int c, i64 n, const i64* __restrict__ outer,
+                                           const i32* __restrict__ cols, i64 (&p0)[MP_NPL], int (&deg)[MP_NPL]) {
+#pragma unroll
+    for (int j = 0; j < MP_NPL; ++j) {
+        const i64 u = u0 + slot + MP_NODES * j;
+        p0[j] = 0;
+        deg[j] = 0;
+        if (u < n) {
+            p0[j] = outer[u];
+            deg[j] = int(outer[u + 1] - p0[j]);
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < MP_NPL; ++j)
+        if (c < deg[j]) L.ecol[slot + MP_NODES * j][c] = cols[p0[j] + c];
+}
+
+// m[j] = sum over node j's edges of (W_e f)_c * src[col * stride + c * cstep]: a vector gather
+// (stride 32, cstep 1) or a scalar one (stride 1, cstep 0).  The edge loop runs to the slot's
+// longest row so the MP_NPL streams advance together.
+__device__ __forceinline__ void gather_tile(const LayerLds& L, int slot, int c, const i64 (&p0)[MP_NPL],
+                                            const int (&deg)[MP_NPL], const float4* __restrict__ feat, float e0, float e1,
+                                            float e2, float e3, const float* __restrict__ src, int stride, int cstep,
+                                            float (&m)[MP_NPL]) {
+    int kmax = 0;
+#pragma unroll
+    for (int j = 0; j < MP_NPL; ++j) { m[j] = 0.0f; kmax = max(kmax, deg[j]); }
+#pragma unroll 1
+    for (int k = 0; k < kmax; ++k) {
+#pragma unroll
+        for (int j = 0; j < MP_NPL; ++j) {
+            if (k < deg[j]) {
+                const float4 f = feat[p0[j] + k];
+                const i64 v = L.ecol[slot + MP_NODES * j][k];
+                m[j] += (e0 * f.x + e1 * f.y + e2 * f.z + e3 * f.w) * src[v * stride + c * cstep];
+            }
+        }
+    }
+}
+
+// One message-passing layer h' = W_s h + W_m m, m_u = sum_p (W_e f_p) (.) h_{cols p}; the messages
+// are stored when mout is given (training).  With wout given, y_u = w_out . h'_u is written instead
+// of h' (last layer of inference).
+__global__ __launch_bounds__(MP_THREADS) void k_lf_layer(i64 n, const i64* __restrict__ outer, const i32* __restrict__ cols,
+                                                         const float4* __restrict__ feat, const float* __restrict__ wl,
+                                                         const float* __restrict__ hin, float* __restrict__ hout,
+                                                         float* __restrict__ mout, const float* __restrict__ wout,
+                                                         float* __restrict__ y) {
+    __shared__ LayerLds L;
+    lds_load_w<false>(L.A, wl + P_WS);
+    lds_load_w<false>(L.B, wl + P_WM);
+    const int c = threadIdx.x % MP_H, slot = threadIdx.x / MP_H;
+    const float e0 = wl[P_WE + c * MP_F + 0], e1 = wl[P_WE + c * MP_F + 1], e2 = wl[P_WE + c * MP_F + 2],
+                e3 = wl[P_WE + c * MP_F + 3];
+    const float wo = wout ? wout[c] : 0.0f;
+    // every thread runs the same number of iterations (barriers; uniform width-32 shuffles)
+    const i64 nn = (n + MP_TILE - 1) / MP_TILE * MP_TILE;
+    for (i64 u0 = i64(blockIdx.x) * MP_TILE; u0 < nn; u0 += i64(gridDim.x) * MP_TILE) {
+        i64 p0[MP_NPL];
+        int deg[MP_NPL];
+        float hu[MP_NPL], m[MP_NPL], acc[MP_NPL];
+        stage_tile(L, u0, slot, c, n, outer, cols, p0, deg);
+#pragma unroll
+        for (int j = 0; j < MP_NPL; ++j) {
+            const i64 u = u0 + slot + MP_NODES * j;
+            hu[j] = u < n ? hin[u * MP_H + c] : 0.0f;
+        }
+        __syncthreads();
+        gather_tile(L, slot, c, p0, deg, feat, e0, e1, e2, e3, hin, MP_H, 1, m);
+#pragma unroll
+        for (int j = 0; j < MP_NPL; ++j) {
+            L.va[slot + MP_NODES * j][c] = hu[j];
+            L.vb[slot + MP_NODES * j][c] = m[j];
+            acc[j] = 0.0f;
+        }
+        __syncthreads();
+        lds_dot_tile(L.A, L.va, slot, c, acc);
+        lds_dot_tile(L.B, L.vb, slot, c, acc);
+#pragma unroll
+        for (int j = 0; j < MP_NPL; ++j) {
+            const i64 u = u0 + slot + MP_NODES * j;
+            const bool live = u < n;
+            if (wout) {
+                float t = wo * acc[j];
+                for (int o = MP_H / 2; o > 0; o >>= 1) t += __shfl_down(t, o, MP_H);
+                if (live && c == 0) y[u] = t;
+            } else if (live) {
+                hout[u * MP_H + c] = acc[j];
+                if (mout) mout[u * MP_H + c] = m[j];
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// ---- transposed pass Y^T x (layers backwards) and the symmetric output ----------------------------
+// Last layer, transposed, from the scalar input: g_v = x_v W_s^T w_out + sum_p G_p (.) x_{cols p} W_m^T w_out;
+// then d = W_s'^T g, q = W_m'^T g with the previous layer's weights (wprev).
+__global__ __launch_bounds__(MP_THREADS) void k_lft_first(i64 n, const i64* __restrict__ outer, const i32* __restrict__ cols,
+                                                          const float4* __restrict__ feat, const float* __restrict__ wl,
+                                                          const float* __restrict__ wout, const float* __restrict__ wprev,
+                                                          const float* __restrict__ x, float* __restrict__ d,
+                                                          float* __restrict__ q) {
+    __shared__ LayerLds L;
+    __shared__ float vd[MP_H], vq[MP_H];
+    lds_load_w<true>(L.A, wl + P_WS);
+    lds_load_w<true>(L.B, wl + P_WM);
+    if (threadIdx.x < MP_H) L.va[0][threadIdx.x] = wout[threadIdx.x];
+    __syncthreads();
+    if (threadIdx.x < MP_H) {
+        vd[threadIdx.x] = lds_dot(L.A, L.va[0], threadIdx.x);      // (W_s^T w_out)_c
+        vq[threadIdx.x] = lds_dot(L.B, L.va[0], threadIdx.x);      // (W_m^T w_out)_c
+    }
+    __syncthreads();
+    lds_load_w<true>(L.A, wprev + P_WS);
+    lds_load_w<true>(L.B, wprev + P_WM);
+    const int c = threadIdx.x % MP_H, slot = threadIdx.x / MP_H;
+    const float e0 = wl[P_WE + c * MP_F + 0], e1 = wl[P_WE + c * MP_F + 1], e2 = wl[P_WE + c * MP_F + 2],
+                e3 = wl[P_WE + c * MP_F + 3];
+    const float vdc = vd[c], vqc = vq[c];
+    const i64 nn = (n + MP_TILE - 1) / MP_TILE * MP_TILE;
+    for (i64 u0 = i64(blockIdx.x) * MP_TILE; u0 < nn; u0 += i64(gridDim.x) * MP_TILE) {
+        i64 p0[MP_NPL];
+        int deg[MP_NPL];
+        float xv[MP_NPL], m[MP_NPL], dd[MP_NPL], qq[MP_NPL];
+        stage_tile(L, u0, slot, c, n, outer, cols, p0, deg);
+#pragma unroll
+        for (int j = 0; j < MP_NPL; ++j) {
+            const i64 v = u0 + slot + MP_NODES * j;
+            xv[j] = v < n ? x[v] : 0.0f;
+        }
+        __syncthreads();
+        gather_tile(L, slot, c, p0, deg, feat, e0, e1, e2, e3, x, 1, 0, m);
+#pragma unroll
+        for (int j = 0; j < MP_NPL; ++j) {
+            L.va[slot + MP_NODES * j][c] = xv[j] * vdc + m[j] * vqc;
+            dd[j] = 0.0f;
+            qq[j] = 0.0f;
+        }
+        __syncthreads();
+        lds_dot_tile(L.A, L.va, slot, c, dd);
+        lds_dot_tile(L.B, L.va, slot, c, qq);
+#pragma unroll
+        for (int j = 0; j < MP_NPL; ++j) {
+            const i64 v = u0 + slot + MP_NODES * j;
+            if (v < n) {
+                d[v * MP_H + c] = dd[j];
+                q[v * MP_H + c] = qq[j];
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// A middle layer, transposed: g_v = d_v + sum_p G_p (.) q_{cols p}; then d' = W_s'^T g (in place),
+// q' = W_m'^T g (new buffer).  For the first layer (wprev null) the output is
+// M r = s^2 r + s sigma (y + a_in . g) / 2.
+template<class Z>
+__global__ __launch_bounds__(MP_THREADS) void k_lft_layer(i64 n, const i64* __restrict__ outer, const i32* __restrict__ cols,
+                                                          const float4* __restrict__ feat, const float* __restrict__ wl,
+                                                          const float* __restrict__ wprev, float* __restrict__ d,
+                                                          const float* __restrict__ q, float* __restrict__ qout,
+                                                          const float* __restrict__ ain, const float* __restrict__ x,
+                                                          const float* __restrict__ y, const float* __restrict__ s,
+                                                          const double* __restrict__ sigma, const double* __restrict__ r,
+                                                          Z* __restrict__ z) {
+    __shared__ LayerLds L;
+    if (wprev) {
+        lds_load_w<true>(L.A, wprev + P_WS);
+        lds_load_w<true>(L.B, wprev + P_WM);
+    }
+    const int c = threadIdx.x % MP_H, slot = threadIdx.x / MP_H;
+    const float e0 = wl[P_WE + c * MP_F + 0], e1 = wl[P_WE + c * MP_F + 1], e2 = wl[P_WE + c * MP_F + 2],
+                e3 = wl[P_WE + c * MP_F + 3];
+    const float a = ain ? ain[c] : 0.0f;
+    const double sg = sigma ? *sigma : 1.0;
+    const i64 nn = (n + MP_TILE - 1) / MP_TILE * MP_TILE;
+    for (i64 u0 = i64(blockIdx.x) * MP_TILE; u0 < nn; u0 += i64(gridDim.x) * MP_TILE) {
+        i64 p0[MP_NPL];
+        int deg[MP_NPL];
+        float g[MP_NPL], m[MP_NPL], dd[MP_NPL], qq[MP_NPL];
+        stage_tile(L, u0, slot, c, n, outer, cols, p0, deg);
+#pragma unroll
+        for (int j = 0; j < MP_NPL; ++j) {
+            const i64 v = u0 + slot + MP_NODES * j;
+            g[j] = v < n ? d[v * MP_H + c] : 0.0f;
+        }
+        __syncthreads();
+        gather_tile(L, slot, c, p0, deg, feat, e0, e1, e2, e3, q, MP_H, 1, m);
+        if (wprev) {
+#pragma unroll
+            for (int j = 0; j < MP_NPL; ++j) {
+                L.va[slot + MP_NODES * j][c] = g[j] + m[j];
+                dd[j] = 0.0f;
+                qq[j] = 0.0f;
+            }
+            __syncthreads();
+            lds_dot_tile(L.A, L.va, slot, c, dd);
+            lds_dot_tile(L.B, L.va, slot, c, qq);
+#pragma unroll
+            for (int j = 0; j < MP_NPL; ++j) {
+                const i64 v = u0 + slot + MP_NODES * j;
+                if (v < n) {
+                    d[v * MP_H + c] = dd[j];
+                    qout[v * MP_H + c] = qq[j];
+                }
+            }
+        } else {
+#pragma unroll
+            for (int j = 0; j < MP_NPL; ++j) {
+                const i64 v = u0 + slot + MP_NODES * j;
+                float t = a * (g[j] + m[j]);
+                for (int o = MP_H / 2; o > 0; o >>= 1) t += __shfl_down(t, o, MP_H);
+                if (v < n && c == 0) {
+                    // the Jacobi term s^2 r in FP64 (exactly linear); the network correction in FP32
+                    const double sv = double(s[v]);
+                    z[v] = Z(sv * sv * r[v] + sv * sg * 0.5 * (double(y[v]) + double(t)));
+                }
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// N(r) (training forward, not symmetrised): z_u = s_u^2 r_u + s_u sigma w_out . h_u
 template<class Z>
 __global__ __launch_bounds__(MP_THREADS) void k_lf_output(i64 n, const float* __restrict__ h, const float* __restrict__ w,
-                                                          const float* __restrict__ x, const float* __restrict__ s,
+                                                          const double* __restrict__ r, const float* __restrict__ s,
                                                           const double* __restrict__ sigma, Z* __restrict__ z) {
     const int c = threadIdx.x % MP_H;
     const double sg = *sigma;
@@ -182,12 +428,15 @@ __global__ __launch_bounds__(MP_THREADS) void k_lf_output(i64 n, const float* __
         const i64 u = u0 + threadIdx.x / MP_H;
         float y = u < n ? w[P_WOUT + c] * h[u * MP_H + c] : 0.0f;
         for (int o = MP_H / 2; o > 0; o >>= 1) y += __shfl_down(y, o, MP_H);
-        if (c == 0 && u < n) z[u] = Z(double(s[u]) * sg * (double(x[u]) + double(y) + double(w[P_BOUT])));
+        if (c == 0 && u < n) {
+            const double su = double(s[u]);
+            z[u] = Z(su * su * r[u] + su * sg * double(y));
+        }
     }
 }
 
 // ---- backward ------------------------------------------------------------------------------------
-// dL/dz (double) -> dL/dh3; gradients of w_out, b_out into the block partial row
+// dL/dz (double) -> dL/dh3; gradient of w_out into the block partial row (the b_out slot stays 0)
 __global__ __launch_bounds__(MP_THREADS) void k_lf_out_back(i64 n, const double* __restrict__ gz, const float* __restrict__ s,
                                                             const double* __restrict__ sigma, const float* __restrict__ h,
                                                             const float* __restrict__ w, float* __restrict__ gh,
@@ -195,12 +444,11 @@ __global__ __launch_bounds__(MP_THREADS) void k_lf_out_back(i64 n, const double*
     __shared__ float red[MP_NODES][MP_H + 1];
     const int c = threadIdx.x % MP_H, slot = threadIdx.x / MP_H;
     const double sg = *sigma;
-    float gw = 0.0f, gb = 0.0f;
+    float gw = 0.0f;
     for (i64 u = i64(blockIdx.x) * MP_NODES + slot; u < n; u += i64(gridDim.x) * MP_NODES) {
         const float dy = float(double(s[u]) * sg * gz[u]);
         gh[u * MP_H + c] = dy * w[P_WOUT + c];
         gw += dy * h[u * MP_H + c];
-        gb += c == 0 ? dy : 0.0f;
     }
     float* row = gpart + i64(blockIdx.x) * NPARAM;
     red[slot][c] = gw;
@@ -210,21 +458,13 @@ __global__ __launch_bounds__(MP_THREADS) void k_lf_out_back(i64 n, const double*
         for (int k = 0; k < MP_NODES; ++k) t += red[k][c];
         row[P_WOUT + c] = t;
     }
-    __syncthreads();
-    red[slot][c] = gb;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        float t = 0.0f;
-        for (int k = 0; k < MP_NODES; ++k) t += red[k][0];
-        row[P_BOUT] = t;
-    }
 }
 
-// da = gh (.) [a > 0]; gradients of W_s, W_m, b; dL/dh_in (direct part) = W_s^T da, dL/dm = W_m^T da.
-__global__ __launch_bounds__(MP_THREADS) void k_lf_layer_back1(i64 n, const float* __restrict__ gh, const float* __restrict__ a,
-                                                               const float* __restrict__ hin, const float* __restrict__ m,
-                                                               const float* __restrict__ wl, float* __restrict__ gdir,
-                                                               float* __restrict__ gm, float* __restrict__ gpart_layer) {
+// gradients of W_s, W_m; dL/dh_in (direct part) = W_s^T gh, dL/dm = W_m^T gh.
+__global__ __launch_bounds__(MP_THREADS) void k_lf_layer_back1(i64 n, const float* __restrict__ gh, const float* __restrict__ hin,
+                                                               const float* __restrict__ m, const float* __restrict__ wl,
+                                                               float* __restrict__ gdir, float* __restrict__ gm,
+                                                               float* __restrict__ gpart_layer) {
     __shared__ float Ws[MP_H][MP_H + 1], Wm[MP_H][MP_H + 1];
     __shared__ float red[MP_NODES][MP_H + 1];
     for (int i = threadIdx.x; i < MP_H * MP_H; i += blockDim.x) {
@@ -233,7 +473,7 @@ __global__ __launch_bounds__(MP_THREADS) void k_lf_layer_back1(i64 n, const floa
     }
     __syncthreads();
     const int c = threadIdx.x % MP_H, slot = threadIdx.x / MP_H;
-    float gws[MP_H], gwm[MP_H], gbias = 0.0f;
+    float gws[MP_H], gwm[MP_H];
 #pragma unroll
     for (int k = 0; k < MP_H; ++k) { gws[k] = 0.0f; gwm[k] = 0.0f; }
     const i64 nn = (n + MP_NODES - 1) / MP_NODES * MP_NODES;
@@ -242,7 +482,7 @@ __global__ __launch_bounds__(MP_THREADS) void k_lf_layer_back1(i64 n, const floa
         const bool live = u < n;
         float da = 0.0f, hu = 0.0f, mu = 0.0f;
         if (live) {
-            da = a[u * MP_H + c] > 0.0f ? gh[u * MP_H + c] : 0.0f;
+            da = gh[u * MP_H + c];
             hu = hin[u * MP_H + c];
             mu = m[u * MP_H + c];
         }
@@ -255,7 +495,6 @@ __global__ __launch_bounds__(MP_THREADS) void k_lf_layer_back1(i64 n, const floa
             dh += Ws[k][c] * dk;                // (W_s^T da)_c
             dm += Wm[k][c] * dk;                // (W_m^T da)_c
         }
-        gbias += da;
         if (live) {
             gdir[u * MP_H + c] = dh;
             gm[u * MP_H + c] = dm;
@@ -279,13 +518,6 @@ __global__ __launch_bounds__(MP_THREADS) void k_lf_layer_back1(i64 n, const floa
             row[P_WM + c * MP_H + k] = t;
         }
         __syncthreads();
-    }
-    red[slot][c] = gbias;
-    __syncthreads();
-    if (slot == 0) {
-        float t = 0.0f;
-        for (int q = 0; q < MP_NODES; ++q) t += red[q][c];
-        row[P_B + c] = t;
     }
 }
 
@@ -325,16 +557,12 @@ __global__ __launch_bounds__(MP_THREADS) void k_lf_layer_back2(i64 n, const i64*
     }
 }
 
-__global__ __launch_bounds__(MP_THREADS) void k_lf_input_back(i64 n, const float* __restrict__ gh, const float* __restrict__ a0,
-                                                              const float* __restrict__ x, float* __restrict__ gpart) {
+__global__ __launch_bounds__(MP_THREADS) void k_lf_input_back(i64 n, const float* __restrict__ gh, const float* __restrict__ x,
+                                                              float* __restrict__ gpart) {
     __shared__ float red[MP_NODES][MP_H + 1];
     const int c = threadIdx.x % MP_H, slot = threadIdx.x / MP_H;
-    float ga = 0.0f, gb = 0.0f;
-    for (i64 u = i64(blockIdx.x) * MP_NODES + slot; u < n; u += i64(gridDim.x) * MP_NODES) {
-        const float da = a0[u * MP_H + c] > 0.0f ? gh[u * MP_H + c] : 0.0f;
-        ga += da * x[u];
-        gb += da;
-    }
+    float ga = 0.0f;
+    for (i64 u = i64(blockIdx.x) * MP_NODES + slot; u < n; u += i64(gridDim.x) * MP_NODES) ga += gh[u * MP_H + c] * x[u];
     float* row = gpart + i64(blockIdx.x) * NPARAM;
     red[slot][c] = ga;
     __syncthreads();
@@ -342,14 +570,6 @@ __global__ __launch_bounds__(MP_THREADS) void k_lf_input_back(i64 n, const float
         float t = 0.0f;
         for (int q = 0; q < MP_NODES; ++q) t += red[q][c];
         row[P_AIN + c] = t;
-    }
-    __syncthreads();
-    red[slot][c] = gb;
-    __syncthreads();
-    if (slot == 0) {
-        float t = 0.0f;
-        for (int q = 0; q < MP_NODES; ++q) t += red[q][c];
-        row[P_BIN + c] = t;
     }
 }
 
@@ -400,12 +620,15 @@ __global__ void k_fill_f32(i64 n, float* __restrict__ x, float v) {
     for (i64 i = i64(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += i64(gridDim.x) * blockDim.x) x[i] = v;
 }
 
+// blocks of the node-loop kernels; the layer kernels take MP_TILE nodes per iteration, the others
+// MP_NODES, and all of them stride over the grid
 unsigned mp_blocks(i64 n) { return unsigned(std::min<i64>(std::max<i64>((n + MP_NODES - 1) / MP_NODES, 1), 8192)); }
+unsigned mp_tile_blocks(i64 n) { return unsigned(std::min<i64>(std::max<i64>((n + MP_TILE - 1) / MP_TILE, 1), 4096)); }
 
 void learned_alloc_train(LearnedGraph& g) {
     if (g.train_ready) return;
     const size_t nh = size_t(g.n) * MP_H;
-    for (int l = 0; l <= MP_L; ++l) { g.h[l].alloc(nh); g.a[l].alloc(nh); }
+    for (int l = 2; l <= MP_L; ++l) g.h[l].alloc(nh);        // h[0], h[1] exist for inference
     for (int l = 0; l < MP_L; ++l) g.m[l].alloc(nh);
     g.g0.alloc(nh); g.g1.alloc(nh); g.gdir.alloc(nh); g.gm.alloc(nh);
     g.gpart.alloc(size_t(MP_GRAD_BLOCKS) * NPARAM);
@@ -415,32 +638,312 @@ void learned_alloc_train(LearnedGraph& g) {
     g.train_ready = true;
 }
 
-// z = M(r): inference (two ping-pong activation buffers) or training (all activations kept).
-template<class R, class Z>
-void learned_forward(LearnedGraph& g, const float* w, const R* r, Z* z, bool train, hipStream_t s) {
+void learned_scaled_input(LearnedGraph& g, const float* w, const double* r, hipStream_t s) {
     const i64 n = g.n;
     const unsigned vb = grid_stride_blocks(n), mb = mp_blocks(n);
-    hipLaunchKernelGGL(k_lf_rnorm<R>, dim3(vb), dim3(256), 0, s, n, r, g.s.p, g.part.p);
+    hipLaunchKernelGGL(k_lf_rnorm<double>, dim3(vb), dim3(256), 0, s, n, r, g.s.p, g.part.p);
     hipLaunchKernelGGL(k_lf_sigma, dim3(1), dim3(1024), 0, s, g.part.p, int(vb), n, g.sigma.p);
-    hipLaunchKernelGGL(k_lf_input<R>, dim3(mb), dim3(MP_THREADS), 0, s, n, r, g.s.p, g.sigma.p, w, g.x.p, g.h[0].p,
-                       train ? g.a[0].p : (float*)nullptr);
-    const float4* feat = reinterpret_cast<const float4*>(g.feat.p);
-    for (int l = 0; l < MP_L; ++l) {
-        const float* hin = train ? g.h[l].p : g.h[l % 2].p;
-        float* hout = train ? g.h[l + 1].p : g.h[(l + 1) % 2].p;
-        hipLaunchKernelGGL(k_lf_layer, dim3(mb), dim3(MP_THREADS), 0, s, n, g.outer.p, g.cols.p, feat,
-                           w + P_LAYER0 + l * P_LAYER_SIZE, hin, hout, train ? g.a[l + 1].p : (float*)nullptr,
-                           train ? g.m[l].p : (float*)nullptr);
-    }
-    const float* hl = train ? g.h[MP_L].p : g.h[MP_L % 2].p;
-    hipLaunchKernelGGL(k_lf_output<Z>, dim3(mb), dim3(MP_THREADS), 0, s, n, hl, w, g.x.p, g.s.p, g.sigma.p, z);
+    hipLaunchKernelGGL(k_lf_input<double>, dim3(mb), dim3(MP_THREADS), 0, s, n, r, g.s.p, g.sigma.p, w, g.x.p, g.h[0].p);
 }
 
-// Loss ||z - e||_A^2 / ||e||_A^2 and its gradient w.r.t. the weights (into g.grad).
+// Training forward: z = N(r) with every activation and message kept (h[0..3], m[0..2]).
+void learned_forward_train(LearnedGraph& g, const float* w, const double* r, double* z, hipStream_t s) {
+    const i64 n = g.n;
+    const unsigned mb = mp_blocks(n), tb = mp_tile_blocks(n);
+    learned_scaled_input(g, w, r, s);
+    const float4* feat = reinterpret_cast<const float4*>(g.feat.p);
+    for (int l = 0; l < MP_L; ++l)
+        hipLaunchKernelGGL(k_lf_layer, dim3(tb), dim3(MP_THREADS), 0, s, n, g.outer.p, g.cols.p, feat,
+                           w + P_LAYER0 + l * P_LAYER_SIZE, g.h[l].p, g.h[l + 1].p, g.m[l].p, (const float*)nullptr,
+                           (float*)nullptr);
+    hipLaunchKernelGGL(k_lf_output<double>, dim3(mb), dim3(MP_THREADS), 0, s, n, g.h[MP_L].p, w, r, g.s.p, g.sigma.p, z);
+}
+
+// ---- inference: one node per lane, channel-major activations --------------------------------------
+// The applied operator runs with lane = node: activations are stored channel-blocked (float4 of
+// channels 4b..4b+3 at a4[b * n + u]),
+// the graph as ELL with DM slots per node (k-major; rows of the local pattern padded with the node
+// itself and zero features), and the weights are wave-uniform (scalar loads).  A lane first loads
+// its DM columns and features, then gathers four channels at a time: the DM float4 loads of one
+// block are independent and coalesced across neighbouring nodes.  Offsets are 32-bit (32 n < 2^31).  The last
+// forward layer and the first transposed layer only need w_out-contracted weights:
+// v_s = W_s2^T w_out, v_m = W_m2^T w_out (64 floats, computed per application by k_soa_prep).
+constexpr int SOA_THREADS = 256;
+
+__global__ void k_soa_ell(i64 n, int D, const i64* __restrict__ outer, const i32* __restrict__ cols,
+                          const float4* __restrict__ feat, i32* __restrict__ ecol, float4* __restrict__ ef) {
+    const i64 u = i64(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (u >= n) return;
+    const i64 p0 = outer[u];
+    const int deg = int(outer[u + 1] - p0);
+    for (int k = 0; k < D; ++k) {
+        const bool e = k < deg;
+        ecol[i64(k) * n + u] = e ? cols[p0 + k] : i32(u);
+        ef[i64(k) * n + u] = e ? feat[p0 + k] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    }
+}
+
+// v[0:32] = W_s^T w_out, v[32:64] = W_m^T w_out for the last layer
+__global__ void k_soa_prep(const float* __restrict__ wl, const float* __restrict__ wout, float* __restrict__ v) {
+    const int c = threadIdx.x % MP_H, which = threadIdx.x / MP_H;
+    if (which > 1) return;
+    const float* W = wl + (which ? P_WM : P_WS);
+    float acc = 0.0f;
+    for (int o = 0; o < MP_H; ++o) acc += W[o * MP_H + c] * wout[o];
+    v[which * MP_H + c] = acc;
+}
+
+template<int DM>
+struct EllRow {
+    uint32_t col[DM];
+    float4 f[DM];
+};
+
+template<int DM>
+__device__ __forceinline__ void ell_row(uint32_t n, uint32_t u, const i32* __restrict__ ecol, const float4* __restrict__ ef,
+                                        EllRow<DM>& e) {
+#pragma unroll
+    for (int k = 0; k < DM; ++k) {
+        e.col[k] = uint32_t(ecol[k * n + u]);
+        e.f[k] = ef[k * n + u];
+    }
+}
+
+// Activations are channel-blocked float4: a4[b * n + u] holds channels 4b..4b+3 of node u.
+__device__ __forceinline__ float comp(const float4& v, int j) { return j == 0 ? v.x : j == 1 ? v.y : j == 2 ? v.z : v.w; }
+
+// m[c] = sum_k (W_e f_k)_c * x[col_k]  (scalar input)
+template<int DM>
+__device__ __forceinline__ void soa_gather_x(const EllRow<DM>& e, const float* __restrict__ wl, const float* __restrict__ x,
+                                             float (&m)[MP_H]) {
+    float v[DM];
+#pragma unroll
+    for (int k = 0; k < DM; ++k) v[k] = x[e.col[k]];
+#pragma unroll
+    for (int c = 0; c < MP_H; ++c) {
+        const float w0 = wl[P_WE + c * MP_F + 0], w1 = wl[P_WE + c * MP_F + 1], w2 = wl[P_WE + c * MP_F + 2],
+                    w3 = wl[P_WE + c * MP_F + 3];
+        float acc = 0.0f;
+#pragma unroll
+        for (int k = 0; k < DM; ++k) acc += (w0 * e.f[k].x + w1 * e.f[k].y + w2 * e.f[k].z + w3 * e.f[k].w) * v[k];
+        m[c] = acc;
+    }
+}
+
+// m[c] = sum_k (W_e f_k)_c * a_{col_k}[c]  (channel-blocked vector input: DM float4 loads per block)
+template<int DM>
+__device__ __forceinline__ void soa_gather_v(uint32_t n, const EllRow<DM>& e, const float* __restrict__ wl,
+                                             const float4* __restrict__ a4, float (&m)[MP_H]) {
+#pragma unroll
+    for (int b = 0; b < MP_H / 4; ++b) {
+        float4 v[DM];
+#pragma unroll
+        for (int k = 0; k < DM; ++k) v[k] = a4[uint32_t(b) * n + e.col[k]];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int c = 4 * b + j;
+            const float w0 = wl[P_WE + c * MP_F + 0], w1 = wl[P_WE + c * MP_F + 1], w2 = wl[P_WE + c * MP_F + 2],
+                        w3 = wl[P_WE + c * MP_F + 3];
+            float acc = 0.0f;
+#pragma unroll
+            for (int k = 0; k < DM; ++k)
+                acc += (w0 * e.f[k].x + w1 * e.f[k].y + w2 * e.f[k].z + w3 * e.f[k].w) * comp(v[k], j);
+            m[c] = acc;
+        }
+    }
+}
+
+__device__ __forceinline__ void load_node(uint32_t n, uint32_t u, const float4* __restrict__ a4, float (&h)[MP_H]) {
+#pragma unroll
+    for (int b = 0; b < MP_H / 4; ++b) {
+        const float4 v = a4[uint32_t(b) * n + u];
+        h[4 * b] = v.x; h[4 * b + 1] = v.y; h[4 * b + 2] = v.z; h[4 * b + 3] = v.w;
+    }
+}
+
+// out = A a (+ init) for a row-major 32 x 32 A, or A^T a when T; stored channel-blocked
+template<bool T>
+__device__ __forceinline__ void product_store(uint32_t n, uint32_t u, const float* __restrict__ A, const float (&a)[MP_H],
+                                              const float* init, float4* __restrict__ out4) {
+    for (int b = 0; b < MP_H / 4; ++b) {
+        float o4[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int o = 4 * b + j;
+            float t = init ? init[o] : 0.0f;
+#pragma unroll
+            for (int k = 0; k < MP_H; ++k) t += A[T ? k * MP_H + o : o * MP_H + k] * a[k];
+            o4[j] = t;
+        }
+        out4[uint32_t(b) * n + u] = make_float4(o4[0], o4[1], o4[2], o4[3]);
+    }
+}
+
+// Forward layers 0 and 1: h' = W_s h + W_m m.  Layer 0 reads the scalar input (h0 = a_in x).
+template<int DM, bool FIRST>
+__global__ __launch_bounds__(SOA_THREADS) void k_soa_layer(uint32_t n, const i32* __restrict__ ecol,
+                                                           const float4* __restrict__ ef, const float* __restrict__ wl,
+                                                           const float* __restrict__ ain, const float* __restrict__ x,
+                                                           const float4* __restrict__ hin, float4* __restrict__ hout) {
+    const uint32_t u = blockIdx.x * blockDim.x + threadIdx.x;
+    if (u >= n) return;
+    // the own-node part W_s h first, so that only 32 partial outputs stay live through the gather
+    float acc[MP_H];
+    {
+        float h[MP_H];
+        if (FIRST) {
+            const float xu = x[u];
+#pragma unroll
+            for (int c = 0; c < MP_H; ++c) h[c] = ain[c] * xu;
+        } else {
+            load_node(n, u, hin, h);
+        }
+        for (int o = 0; o < MP_H; ++o) {
+            float t = 0.0f;
+#pragma unroll
+            for (int k = 0; k < MP_H; ++k) t += wl[P_WS + o * MP_H + k] * h[k];
+            acc[o] = t;
+        }
+    }
+    EllRow<DM> e;
+    ell_row<DM>(n, u, ecol, ef, e);
+    float m[MP_H];
+    if (FIRST) {
+        soa_gather_x<DM>(e, wl, x, m);
+#pragma unroll
+        for (int c = 0; c < MP_H; ++c) m[c] *= ain[c];
+    } else {
+        soa_gather_v<DM>(n, e, wl, hin, m);
+    }
+    product_store<false>(n, u, wl + P_WM, m, acc, hout);
+}
+
+// Forward layer 2 contracted with w_out: y_u = v_s . h_u + v_m . m_u
+template<int DM>
+__global__ __launch_bounds__(SOA_THREADS) void k_soa_last(uint32_t n, const i32* __restrict__ ecol, const float4* __restrict__ ef,
+                                                          const float* __restrict__ wl, const float* __restrict__ v,
+                                                          const float4* __restrict__ hin, float* __restrict__ y) {
+    const uint32_t u = blockIdx.x * blockDim.x + threadIdx.x;
+    if (u >= n) return;
+    EllRow<DM> e;
+    ell_row<DM>(n, u, ecol, ef, e);
+    float m[MP_H], h[MP_H];
+    soa_gather_v<DM>(n, e, wl, hin, m);
+    load_node(n, u, hin, h);
+    float acc = 0.0f;
+#pragma unroll
+    for (int c = 0; c < MP_H; ++c) acc += v[c] * h[c] + v[MP_H + c] * m[c];
+    y[u] = acc;
+}
+
+// Transposed layer 2 from the scalar input: g = x_u v_s + v_m (.) sum_k G_k x_{col k};
+// d = W_s1^T g, q = W_m1^T g.
+template<int DM>
+__global__ __launch_bounds__(SOA_THREADS) void k_soa_tfirst(uint32_t n, const i32* __restrict__ ecol,
+                                                            const float4* __restrict__ ef, const float* __restrict__ wl,
+                                                            const float* __restrict__ v, const float* __restrict__ wprev,
+                                                            const float* __restrict__ x, float4* __restrict__ d,
+                                                            float4* __restrict__ q) {
+    const uint32_t u = blockIdx.x * blockDim.x + threadIdx.x;
+    if (u >= n) return;
+    EllRow<DM> e;
+    ell_row<DM>(n, u, ecol, ef, e);
+    float m[MP_H];
+    soa_gather_x<DM>(e, wl, x, m);
+    const float xu = x[u];
+#pragma unroll
+    for (int c = 0; c < MP_H; ++c) m[c] = xu * v[c] + v[MP_H + c] * m[c];
+    product_store<true>(n, u, wprev + P_WS, m, nullptr, d);
+    product_store<true>(n, u, wprev + P_WM, m, nullptr, q);
+}
+
+// Transposed middle layer: g = d_u + sum_k G_k (.) q_{col k}; d = W_s'^T g (in place), q' = W_m'^T g.
+template<int DM>
+__global__ __launch_bounds__(SOA_THREADS) void k_soa_tmid(uint32_t n, const i32* __restrict__ ecol, const float4* __restrict__ ef,
+                                                          const float* __restrict__ wl, const float* __restrict__ wprev,
+                                                          float4* __restrict__ d, const float4* __restrict__ q,
+                                                          float4* __restrict__ qout) {
+    const uint32_t u = blockIdx.x * blockDim.x + threadIdx.x;
+    if (u >= n) return;
+    EllRow<DM> e;
+    ell_row<DM>(n, u, ecol, ef, e);
+    float g[MP_H], dv[MP_H];
+    soa_gather_v<DM>(n, e, wl, q, g);
+    load_node(n, u, d, dv);
+#pragma unroll
+    for (int c = 0; c < MP_H; ++c) g[c] += dv[c];
+    product_store<true>(n, u, wprev + P_WS, g, nullptr, d);
+    product_store<true>(n, u, wprev + P_WM, g, nullptr, qout);
+}
+
+// Transposed layer 0 and the output: t = a_in . (d_u + sum_k G_k (.) q_{col k});
+// M r = s^2 r (FP64) + s sigma (y + t) / 2.
+template<int DM, class Z>
+__global__ __launch_bounds__(SOA_THREADS) void k_soa_tlast(uint32_t n, const i32* __restrict__ ecol, const float4* __restrict__ ef,
+                                                           const float* __restrict__ wl, const float* __restrict__ ain,
+                                                           const float4* __restrict__ d, const float4* __restrict__ q,
+                                                           const float* __restrict__ y, const float* __restrict__ s,
+                                                           const double* __restrict__ sigma, const double* __restrict__ r,
+                                                           Z* __restrict__ z) {
+    const uint32_t u = blockIdx.x * blockDim.x + threadIdx.x;
+    if (u >= n) return;
+    EllRow<DM> e;
+    ell_row<DM>(n, u, ecol, ef, e);
+    float g[MP_H], dv[MP_H];
+    soa_gather_v<DM>(n, e, wl, q, g);
+    load_node(n, u, d, dv);
+    float t = 0.0f;
+#pragma unroll
+    for (int c = 0; c < MP_H; ++c) t += ain[c] * (g[c] + dv[c]);
+    const double su = double(s[u]);
+    z[u] = Z(su * su * r[u] + su * (*sigma) * 0.5 * (double(y[u]) + double(t)));
+}
+
+// ELL width for a longest row D: the smallest instantiated DM >= D
+int ell_width(i64 D) { return D <= 9 ? 9 : D <= 16 ? 16 : D <= 24 ? 24 : 32; }
+
+template<int DM, class Z>
+void learned_apply_ell(LearnedGraph& g, const float* w, const double* r, Z* z, hipStream_t s) {
+    const uint32_t n = uint32_t(g.n);
+    const unsigned nb = blocks_for(g.n, SOA_THREADS);
+    auto layer = [&](int l) { return w + P_LAYER0 + l * P_LAYER_SIZE; };
+    static_assert(MP_L == 3, "the inference schedule below is written for three layers");
+    float4* A = reinterpret_cast<float4*>(g.h[0].p);
+    float4* B = reinterpret_cast<float4*>(g.h[1].p);
+    float4* Dv = reinterpret_cast<float4*>(g.d.p);
+    const i32* ec = g.ell_col.p;
+    const float4* ef = g.ell_f.p;
+    hipLaunchKernelGGL(k_soa_prep, dim3(1), dim3(64), 0, s, layer(2), w + P_WOUT, g.v.p);
+    hipLaunchKernelGGL((k_soa_layer<DM, true>), dim3(nb), dim3(SOA_THREADS), 0, s, n, ec, ef, layer(0), w + P_AIN, g.x.p,
+                       (const float4*)nullptr, A);
+    hipLaunchKernelGGL((k_soa_layer<DM, false>), dim3(nb), dim3(SOA_THREADS), 0, s, n, ec, ef, layer(1), w + P_AIN, g.x.p,
+                       A, B);
+    hipLaunchKernelGGL(k_soa_last<DM>, dim3(nb), dim3(SOA_THREADS), 0, s, n, ec, ef, layer(2), g.v.p, B, g.y.p);
+    hipLaunchKernelGGL(k_soa_tfirst<DM>, dim3(nb), dim3(SOA_THREADS), 0, s, n, ec, ef, layer(2), g.v.p, layer(1), g.x.p,
+                       Dv, A);
+    hipLaunchKernelGGL(k_soa_tmid<DM>, dim3(nb), dim3(SOA_THREADS), 0, s, n, ec, ef, layer(1), layer(0), Dv, A, B);
+    hipLaunchKernelGGL((k_soa_tlast<DM, Z>), dim3(nb), dim3(SOA_THREADS), 0, s, n, ec, ef, layer(0), w + P_AIN, Dv, B,
+                       g.y.p, g.s.p, g.sigma.p, r, z);
+}
+
+// z = M r, the symmetric part of N, on the lane-per-node kernels: forward layers (A, B channel-major
+// ping-pong), y; transposed layers (d in place, q through A then B) ending in the output.
+template<class Z>
+void learned_apply(LearnedGraph& g, const float* w, const double* r, Z* z, hipStream_t s) {
+    learned_scaled_input(g, w, r, s);
+    switch (g.ell_d) {
+        case 9: learned_apply_ell<9, Z>(g, w, r, z, s); break;
+        case 16: learned_apply_ell<16, Z>(g, w, r, z, s); break;
+        case 24: learned_apply_ell<24, Z>(g, w, r, z, s); break;
+        default: learned_apply_ell<32, Z>(g, w, r, z, s); break;
+    }
+}
+
+// Loss ||N(r) - e||_A^2 / ||e||_A^2 and its gradient w.r.t. the weights (into g.grad).
 double learned_loss_grad(LearnedGraph& g, const float* w, nlf_matrix A, const double* r, const double* e, hipStream_t s) {
     learned_alloc_train(g);
     const i64 n = g.n;
-    learned_forward<double, double>(g, w, r, g.zbuf.p, true, s);
+    learned_forward_train(g, w, r, g.zbuf.p, s);
     const int lpr = pick_lpr(double(A->nnz_full) / double(std::max<i64>(A->rows, 1)));
     launch_spmv<double, double, double, false>(lpr, n, A->outer.p, A->cols.p, A->val.p, g.zbuf.p, g.az.p, nullptr, nullptr, s);
     const unsigned vb = grid_stride_blocks(n);
@@ -449,7 +952,7 @@ double learned_loss_grad(LearnedGraph& g, const float* w, nlf_matrix A, const do
     hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(1024), 0, s, p1.p, int(vb), sums.p);
     hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(1024), 0, s, p2.p, int(vb), sums.p + 1);
     hipLaunchKernelGGL(k_scale_vec, dim3(vb), dim3(256), 0, s, n, gz.p, sums.p);
-    // backward
+    // backward (bias slots stay zero: the partial rows are cleared and never written there)
     hipLaunchKernelGGL(k_fill_f32, dim3(256), dim3(256), 0, s, i64(MP_GRAD_BLOCKS) * NPARAM, g.gpart.p, 0.0f);
     const unsigned gb = MP_GRAD_BLOCKS;
     hipLaunchKernelGGL(k_lf_out_back, dim3(gb), dim3(MP_THREADS), 0, s, n, gz.p, g.s.p, g.sigma.p, g.h[MP_L].p, w, g.g0.p,
@@ -460,13 +963,13 @@ double learned_loss_grad(LearnedGraph& g, const float* w, nlf_matrix A, const do
     for (int l = MP_L - 1; l >= 0; --l) {
         const float* wl = w + P_LAYER0 + l * P_LAYER_SIZE;
         float* gp = g.gpart.p + P_LAYER0 + l * P_LAYER_SIZE;
-        hipLaunchKernelGGL(k_lf_layer_back1, dim3(gb), dim3(MP_THREADS), 0, s, n, gcur, g.a[l + 1].p, g.h[l].p, g.m[l].p,
-                           wl, g.gdir.p, g.gm.p, gp);
+        hipLaunchKernelGGL(k_lf_layer_back1, dim3(gb), dim3(MP_THREADS), 0, s, n, gcur, g.h[l].p, g.m[l].p, wl, g.gdir.p,
+                           g.gm.p, gp);
         hipLaunchKernelGGL(k_lf_layer_back2, dim3(gb), dim3(MP_THREADS), 0, s, n, g.outer.p, g.cols.p, feat, wl, g.gm.p,
                            g.h[l].p, g.gdir.p, gnext, gp);
         std::swap(gcur, gnext);
     }
-    hipLaunchKernelGGL(k_lf_input_back, dim3(gb), dim3(MP_THREADS), 0, s, n, gcur, g.a[0].p, g.x.p, g.gpart.p);
+    hipLaunchKernelGGL(k_lf_input_back, dim3(gb), dim3(MP_THREADS), 0, s, n, gcur, g.x.p, g.gpart.p);
     hipLaunchKernelGGL(k_lf_reduce, dim3((NPARAM + 255) / 256), dim3(256), 0, s, g.gpart.p, int(gb), g.grad.p);
     NLF_HIP(hipGetLastError());
     double h[2];
@@ -495,6 +998,10 @@ LearnedGraph* learned_of(nlf_precond p) {
 // recomputed at every tenfold drop of the recursive residual (reliable update) and is the only
 // stopping test.  Stagnation (no 10% decrease of the residual over `window` iterations, or a
 // non-finite value) returns -1 so the caller can fall back.
+// Stagnation window: the 2-norm of a CG residual is not monotone, so a plateau must be long before
+// it counts as stagnation.
+constexpr int FCG_WINDOW = 1000;
+
 struct FCGState {
     double pr, pq, zq, rr, alpha, beta;
 };
@@ -576,7 +1083,7 @@ i64 fcg(hipStream_t s, nlf_matrix A, const V* val, nlf_precond M, const double* 
     };
     auto apply_m = [&](bool sample) {
         if (sample) timer.start(1, s);
-        learned_forward<double, double>(g, g.w.p, r.p, z.p, false, s);
+        learned_apply<double>(g, g.w.p, r.p, z.p, s);
         if (sample) timer.stop(1, s);
     };
     auto [rn, bn] = true_residual();
@@ -648,13 +1155,14 @@ extern "C" {
 
 int nlf_learned_num_params(void) { return NPARAM; }
 
-// Deterministic initial weights; the output layer starts at zero, so the initial network is Jacobi.
+// Deterministic initial weights (splitmix64); output weights and the unused bias slots start at zero,
+// so the initial preconditioner is Jacobi.
 int nlf_learned_init_weights(uint64_t seed, float* w) {
     if (!w) return NLF_ERR_ARGUMENT;
     uint64_t st = seed * 0x2545f4914f6cdd1dull + 1;
     for (int i = 0; i < NPARAM; ++i) w[i] = 0.0f;
-    for (int c = 0; c < MP_H; ++c) { w[P_AIN + c] = uniform(st, 1.0f); w[P_BIN + c] = uniform(st, 0.1f); }
-    const float a = std::sqrt(6.0f / float(2 * MP_H));
+    for (int c = 0; c < MP_H; ++c) w[P_AIN + c] = uniform(st, 1.0f);
+    const float a = 0.5f * std::sqrt(6.0f / float(2 * MP_H));   // half the Glorot bound: linear layers compound
     for (int l = 0; l < MP_L; ++l) {
         float* L = w + P_LAYER0 + l * P_LAYER_SIZE;
         for (int i = 0; i < MP_H * MP_F; ++i) L[P_WE + i] = uniform(st, 0.5f);
@@ -683,6 +1191,9 @@ int nlf_learned_create(nlf_matrix A, nlf_matrix A_local, const nlf_model* models
         g->nnz = A_local->nnz_full;
         std::vector<i64> lo(size_t(g->n + 1));
         download(lo.data(), A_local->outer.p, lo.size(), s);
+        i64 maxdeg = 0;
+        for (i64 u = 0; u < g->n; ++u) maxdeg = std::max(maxdeg, lo[u + 1] - lo[u]);
+        NLF_CHECK(maxdeg <= MP_MAXDEG, NLF_ERR_UNSUPPORTED, "local pattern has a row longer than the staged edge list");
         upload(g->outer, lo.data(), lo.size(), s);
         std::vector<i32> lc(size_t(g->nnz));
         download(lc.data(), A_local->cols.p, lc.size(), s);
@@ -691,8 +1202,10 @@ int nlf_learned_create(nlf_matrix A, nlf_matrix A_local, const nlf_model* models
         g->feat.alloc(size_t(4 * g->nnz));
         g->w.alloc(NPARAM);
         g->x.alloc(size_t(g->n));
+        g->y.alloc(size_t(g->n));
         g->h[0].alloc(size_t(g->n) * MP_H);
         g->h[1].alloc(size_t(g->n) * MP_H);
+        g->d.alloc(size_t(g->n) * MP_H);
         g->part.alloc(grid_stride_blocks(g->n));
         g->sigma.alloc(1);
         std::vector<GroupParams> gps(m->G);
@@ -705,6 +1218,13 @@ int nlf_learned_create(nlf_matrix A, nlf_matrix A_local, const nlf_model* models
                            A_local->val.p, A->outer.p, A->cols.p, A->val.p, g->s.p, m->xy.p, m->ne_off.p, m->ne_ids.p,
                            m->group_of.p, d_groups.p, g->feat.p);
         hipLaunchKernelGGL(k_fill_f32, dim3(64), dim3(256), 0, s, i64(NPARAM), g->w.p, 0.0f);
+        NLF_CHECK(n * MP_H < (i64(1) << 31), NLF_ERR_UNSUPPORTED, "too many nodes for 32-bit activation offsets");
+        g->ell_d = ell_width(maxdeg);
+        g->ell_col.alloc(size_t(g->ell_d) * size_t(n));
+        g->ell_f.alloc(size_t(g->ell_d) * size_t(n));
+        g->v.alloc(2 * MP_H);
+        hipLaunchKernelGGL(k_soa_ell, dim3(blocks_for(n, 256)), dim3(256), 0, s, n, g->ell_d, g->outer.p, g->cols.p,
+                           reinterpret_cast<const float4*>(g->feat.p), g->ell_col.p, g->ell_f.p);
         NLF_HIP(hipGetLastError());
         NLF_HIP(hipStreamSynchronize(s));
         P->learned = g.release();
@@ -809,9 +1329,9 @@ int nlf_fcg_solve(nlf_matrix A, nlf_precond M, nlf_precond fallback, const doubl
         i64 it = 0;
         if (mode == NLF_MODE_MIXED) {
             NLF_CHECK(A->val32.p, NLF_ERR_STATE, "mixed mode needs nlf_matrix_make_fp32 first");
-            it = fcg<float>(s, A, A->val32.p, M, b, x, tol, max_it, &out.true_residual, &out.outer_iterations, 400, timer);
+            it = fcg<float>(s, A, A->val32.p, M, b, x, tol, max_it, &out.true_residual, &out.outer_iterations, FCG_WINDOW, timer);
         } else {
-            it = fcg<double>(s, A, A->val.p, M, b, x, tol, max_it, &out.true_residual, &out.outer_iterations, 400, timer);
+            it = fcg<double>(s, A, A->val.p, M, b, x, tol, max_it, &out.true_residual, &out.outer_iterations, FCG_WINDOW, timer);
         }
         out.spmv_ms = timer.avg(0);
         out.precond_ms = timer.avg(1);

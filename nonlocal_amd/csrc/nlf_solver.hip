@@ -1195,7 +1195,7 @@ int nlf_precond_apply(nlf_precond p, const double* r, double* z) {
         NLF_CHECK(p->rows > 0, NLF_ERR_STATE, "preconditioner has no size");
         if (p->kind == NLF_PRECOND_LEARNED) {
             LearnedGraph& g = *learned_of(p);
-            learned_forward<double, double>(g, g.w.p, r, z, false, p->ctx->stream);
+            learned_apply<double>(g, g.w.p, r, z, p->ctx->stream);
         } else {
             precond_apply<double>(p, p->rows, r, z, nullptr, p->ctx->stream);
         }

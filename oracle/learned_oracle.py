@@ -1,15 +1,18 @@
 """ORACLE -- test infrastructure only (tests/, __graft_entry__.smoke(), bench.py cpu_baseline).
 
 FP64 numpy restatement of the learned preconditioner (message-passing network over mesh nodes):
-edge features, forward pass, loss ||z - e||_A^2 / ||e||_A^2 and its gradient by reverse-mode
-differentiation written out by hand.  The reference has no learned component; what pins this file
+edge features, the network N (linear in the residual), its transpose and the symmetric operator
+M = (N + N^T)/2 applied in flexible CG, the loss ||N(r) - e||_A^2 / ||e||_A^2 and its gradient by
+reverse-mode differentiation written out by hand.  The reference has no learned component; what pins this file
 is mathematics, not the reference:
   * the backward pass is checked against central finite differences of the forward pass;
+  * the transpose is checked by the adjoint identity <v, N u> = <N^T v, u> and the symmetric
+    operator against the explicit matrix of N built column by column;
   * with zero output weights the network is exactly Jacobi, z = r / diag(K);
   * the loss is zero when z = e and equals 1 when z = 0;
   * edge features: f0/f1 are the symmetrically scaled K_L / K entries, f2 uses the influence function
     of the node's group (oracle/nlf_oracle.cpp, influence_functions_2d.hpp:39-148).
-The parameter layout is the one of nlf_learned_num_params() floats.
+The parameter layout is the one of nlf_learned_num_params() floats; the bias slots are unused.
 """
 from __future__ import annotations
 
@@ -79,31 +82,57 @@ def features(outer, cols, kl_val, k_full, xy, groups_of_nodes, models, influence
     return feat, s
 
 
-def forward(w, outer, cols, feat, s, r, keep=False):
-    """z = M(r).  Returns z (and the activations when keep)."""
-    a_in, b_in, layers, w_out, b_out = unpack(w)
+def _scaled(s, r):
     s = np.asarray(s, dtype=np.float64)
-    r = np.asarray(r, dtype=np.float64)
-    sr = s * r
+    sr = s * np.asarray(r, dtype=np.float64)
     sigma = np.sqrt(np.mean(sr * sr))
     if not (sigma > 0 and np.isfinite(sigma)):
         sigma = 1.0
-    x = sr / sigma
-    a = [np.outer(x, a_in) + b_in]
-    h = [np.maximum(a[0], 0.0)]
+    return s, sr / sigma, sigma
+
+
+def forward(w, outer, cols, feat, s, r, keep=False):
+    """N(r): x = S r / sigma, h0 = a_in x, h' = W_s h + W_m sum_p (W_e f_p) (.) h_{cols p}, y = w_out . h3,
+    N(r) = s sigma (x + y).  Returns N(r) (and the activations when keep)."""
+    a_in, _, layers, w_out, _ = unpack(w)
+    s, x, sigma = _scaled(s, r)
+    h = [np.outer(x, a_in)]
     m = []
     for lay in layers:
         G = feat @ lay["We"].T                      # nnz x H edge gates
         mm = _segsum(G * h[-1][cols], outer)
-        al = h[-1] @ lay["Ws"].T + mm @ lay["Wm"].T + lay["b"]
         m.append(mm)
-        a.append(al)
-        h.append(np.maximum(al, 0.0))
-    y = h[-1] @ w_out + b_out
+        h.append(h[-1] @ lay["Ws"].T + mm @ lay["Wm"].T)
+    y = h[-1] @ w_out
     z = s * sigma * (x + y)
     if keep:
-        return z, dict(x=x, sigma=sigma, a=a, h=h, m=m)
+        return z, dict(x=x, sigma=sigma, h=h, m=m, y=y)
     return z
+
+
+def transpose_y(w, outer, cols, feat, x):
+    """Y^T x where y = Y x is the message path of the network (layers run backwards)."""
+    a_in, _, layers, w_out, _ = unpack(w)
+    g = np.outer(x, w_out)
+    for lay in reversed(layers):
+        G = feat @ lay["We"].T
+        q = g @ lay["Wm"]                            # W_m^T g per node
+        gin = g @ lay["Ws"]
+        gin += _segsum(G * q[cols], outer)           # symmetric graph and gates
+        g = gin
+    return g @ a_in
+
+
+def transpose(w, outer, cols, feat, s, v):
+    """N^T v = S^2 v + S Y^T S v (sigma cancels: the operator is linear)."""
+    s = np.asarray(s, dtype=np.float64)
+    v = np.asarray(v, dtype=np.float64)
+    return s * s * v + s * transpose_y(w, outer, cols, feat, s * v)
+
+
+def apply_sym(w, outer, cols, feat, s, r):
+    """M r = (N r + N^T r) / 2, the operator the flexible CG applies."""
+    return 0.5 * (forward(w, outer, cols, feat, s, r) + transpose(w, outer, cols, feat, s, r))
 
 
 def loss(z, e, A, r):
@@ -112,8 +141,8 @@ def loss(z, e, A, r):
 
 
 def loss_grad(w, outer, cols, feat, s, A, r, e):
-    """Loss and dLoss/dw (FP64) by hand-written reverse mode."""
-    a_in, b_in, layers, w_out, b_out = unpack(w)
+    """Loss of N and dLoss/dw (FP64) by hand-written reverse mode; bias slots get zero gradient."""
+    a_in, _, layers, w_out, _ = unpack(w)
     s = np.asarray(s, dtype=np.float64)
     z, act = forward(w, outer, cols, feat, s, r, keep=True)
     Az = A @ z
@@ -122,27 +151,22 @@ def loss_grad(w, outer, cols, feat, s, A, r, e):
     gz = 2.0 * (Az - r) / er
     g = np.zeros(NPARAM)
     dy = s * act["sigma"] * gz
-    h, a, m, x = act["h"], act["a"], act["m"], act["x"]
+    h, m, x = act["h"], act["m"], act["x"]
     g[P_WOUT:P_WOUT + H] = dy @ h[L]
-    g[P_BOUT] = dy.sum()
     gh = np.outer(dy, w_out)
     rows = _rows(outer)
     for l in range(L - 1, -1, -1):
         lay = layers[l]
         o = P_LAYER0 + l * P_LAYER_SIZE
-        da = gh * (a[l + 1] > 0)
-        g[o + P_WS:o + P_WM] = (da.T @ h[l]).ravel()
-        g[o + P_WM:o + P_B] = (da.T @ m[l]).ravel()
-        g[o + P_B:o + P_B + H] = da.sum(axis=0)
-        gm = da @ lay["Wm"]                          # dL/dm
+        g[o + P_WS:o + P_WM] = (gh.T @ h[l]).ravel()
+        g[o + P_WM:o + P_B] = (gh.T @ m[l]).ravel()
+        gm = gh @ lay["Wm"]                          # dL/dm
         G = feat @ lay["We"].T
         # m_u = sum_{p in row u} G_p * h_{cols p}: dh_v += G_p * gm_u ; dWe[c,f] += gm_u[c] h_v[c] f_p[f]
-        ghin = da @ lay["Ws"]
+        ghin = gh @ lay["Ws"]
         np.add.at(ghin, cols, G * gm[rows])
         t = gm[rows] * h[l][cols]                    # nnz x H
         g[o + P_WE:o + P_WS] = (t.T @ feat).ravel()
         gh = ghin
-    da0 = gh * (a[0] > 0)
-    g[P_AIN:P_AIN + H] = da0.T @ x
-    g[P_BIN:P_BIN + H] = da0.sum(axis=0)
+    g[P_AIN:P_AIN + H] = gh.T @ x
     return val, g

@@ -59,7 +59,7 @@ def _gpu_weights(seed, out_scale=0.2):
     return w.astype(np.float32)
 
 
-def test_forward_matches_oracle(small, gpu_ctx):
+def test_symmetric_apply_matches_oracle(small, gpu_ctx):
     tp, Kf = small
     outer, cols, feat, s = tp.learned.graph()
     w = _gpu_weights(3)
@@ -67,8 +67,14 @@ def test_forward_matches_oracle(small, gpu_ctx):
     np.testing.assert_array_equal(tp.learned.get_weights(), w)
     r = np.random.default_rng(4).standard_normal(len(s))
     z = tp.learned.solve(torch.from_numpy(r).to(gpu_ctx.device)).cpu().numpy()
-    zo = LO.forward(w.astype(np.float64), outer, cols, feat.astype(np.float64), s.astype(np.float64), r)
+    zo = LO.apply_sym(w.astype(np.float64), outer, cols, feat.astype(np.float64), s.astype(np.float64), r)
     assert rel_l2(z, zo) < 1e-5
+    # the applied operator is symmetric: <v, M u> = <M v, u>
+    u, v = np.random.default_rng(7).standard_normal((2, len(s)))
+    dev = gpu_ctx.device
+    Mu = tp.learned.solve(torch.from_numpy(u).to(dev)).cpu().numpy()
+    Mv = tp.learned.solve(torch.from_numpy(v).to(dev)).cpu().numpy()
+    assert abs(v @ Mu - u @ Mv) <= 1e-5 * abs(v @ Mu)
 
 
 def test_zero_output_is_jacobi(small, gpu_ctx):
@@ -117,7 +123,7 @@ def trained_small(gpu_ctx):
     """A 32 x 32 composite, its (r, e) pairs and 150 Adam steps from the Jacobi start."""
     print("[learned] training problem 32x32", flush=True)
     tp = learned.TrainingProblem(32, seed=2, ctx=gpu_ctx, count=4)
-    tp.make_pairs(np.random.default_rng(0), n_rhs=2, snapshots=(0, 10, 40))
+    tp.make_pairs(np.random.default_rng(0), count=6)
     w, hist = learned.train([tp], steps=150, lr=2e-3, seed=0, log=None)
     return tp, w, hist
 
@@ -157,7 +163,7 @@ def test_poisoned_weights_fall_back(trained_small):
     """Non-finite network output: flexible CG stops and the Jacobi-PCG fallback finishes the solve."""
     tp, w, _ = trained_small
     bad = w.copy()
-    bad[LO.P_BOUT] = np.nan
+    bad[LO.P_WOUT] = np.nan
     tp.learned.set_weights(bad)
     jac = api.preconditioner(tp.K, A.NLF_PRECOND_JACOBI)
     x, st = learned.fcg_solve(tp.K, tp.learned, tp.b, fallback=jac, tol=1e-10, max_it=20000)

@@ -77,6 +77,33 @@ def test_zero_output_weights_is_jacobi(graph):
     np.testing.assert_allclose(z, r / Kf.diagonal(), rtol=1e-13)
 
 
+def test_transpose_adjoint_identity(graph):
+    p, Kf, outer, cols, feat, s, f = graph
+    w = _random_weights(8)
+    rng = np.random.default_rng(9)
+    u, v = rng.standard_normal((2, len(s)))
+    lhs = v @ LO.forward(w, outer, cols, feat, s, u)
+    rhs = LO.transpose(w, outer, cols, feat, s, v) @ u
+    assert abs(lhs - rhs) <= 1e-12 * abs(lhs)
+
+
+def test_symmetric_operator_against_explicit_matrix(graph):
+    """Build N column by column; M = (N + N^T)/2 must match apply_sym, be symmetric, and N must
+    be linear with a three-hop stencil."""
+    p, Kf, outer, cols, feat, s, f = graph
+    w = _random_weights(11)
+    n = len(s)
+    N = np.stack([LO.forward(w, outer, cols, feat, s, np.eye(n)[j]) for j in range(n)], axis=1)
+    r = np.random.default_rng(12).standard_normal(n)
+    np.testing.assert_allclose(N @ r, LO.forward(w, outer, cols, feat, s, r), rtol=1e-11, atol=1e-13 * np.abs(N).max())
+    Msym = 0.5 * (N + N.T)
+    np.testing.assert_allclose(LO.apply_sym(w, outer, cols, feat, s, r), Msym @ r, rtol=1e-11,
+                               atol=1e-13 * np.abs(N).max())
+    KL = sp.csr_matrix((np.ones(len(cols)), cols, outer), shape=(n, n))
+    hop3 = (KL @ KL @ KL).toarray() != 0
+    assert np.all(N[~hop3] == 0.0)
+
+
 def test_loss_closed_values(graph):
     p, Kf, outer, cols, feat, s, f = graph
     r = np.random.default_rng(3).standard_normal(len(s))
