@@ -6,8 +6,9 @@ x = 0 and unit inward flux on x = 1.  A "step" is the whole hot path on that inp
 resident in HBM: neighbour search, CSR pattern, FP64 element-pair assembly, boundary conditions,
 preconditioner set-up and the solve to <= 1e-10 FP64 true residual.
 
-value = assembly+solve seconds of the fastest accurate GPU method (mixed-precision CG; the plain
-FP64 ILLT-CG time is reported beside it).  roofline = the dominant kernel (CSR SpMV inside the
+value = assembly+solve seconds of the fastest accurate GPU method (normally flexible CG with the
+learned preconditioner: FP32 network, FP64 Krylov vectors, SpMV and true residual); the plain FP64
+ILLT-CG time and the other methods are reported beside it.  roofline = the dominant kernel (CSR SpMV inside the
 solve) against 8 TB/s HBM, timed live with HIP events on the solver stream.  cpu_baseline = the
 oracle (the reference algorithm restated in C++/OpenMP) timed on this host on a smaller instance of
 the same workload family and scaled to the full size (see the `sample` field).
@@ -39,7 +40,7 @@ def parse():
     ap.add_argument("--tol", type=float, default=1e-10)
     ap.add_argument("--cpu-sample-n", type=int, default=160)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--methods", default="mixed_jacobi,mixed_illt")
+    ap.add_argument("--methods", default="fp64_jacobi,mixed_jacobi,learned_fcg")
     return ap.parse_args()
 
 
@@ -106,6 +107,7 @@ def main():
         "mixed_illt": (A.NLF_PRECOND_ILLT, A.NLF_MODE_MIXED),
         "mixed_jacobi": (A.NLF_PRECOND_JACOBI, A.NLF_MODE_MIXED),
         "fp64_jacobi": (A.NLF_PRECOND_JACOBI, A.NLF_MODE_FP64),
+        "learned_fcg": (A.NLF_PRECOND_LEARNED, A.NLF_MODE_FP64),
     }
     # plain FP64 ILLT-CG: the baseline the metric compares against; also the FP64 reference solution
     log("[bench] plain FP64 ILLT-CG")
@@ -167,7 +169,7 @@ def main():
         "higher_is_better": False,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64+f32" if mode == A.NLF_MODE_MIXED else "f64",
+        "dtype": "f64+f32" if (mode == A.NLF_MODE_MIXED or pc == A.NLF_PRECOND_LEARNED) else "f64",
         "data": "synthetic",
         "config": {"workload": f"composite 2D stationary heat, {args.n}x{args.n} Q4, 16 inclusions, p1=0.5, "
                                f"bell r=0.01, solve to {args.tol:g} true residual",
@@ -180,6 +182,7 @@ def main():
         "iterations": int(last.iterations),
         "methods": results,
         "phase_seconds": {k: round(v, 4) for k, v in last.timings.items()},
+        "precond_apply_ms": round(st.precond_ms, 4),
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
                      "kernel": f"k_spmv ({'fp32' if sv == 4 else 'fp64'}, full CSR)",

@@ -356,7 +356,7 @@ def _radii(models):
     return out
 
 
-def _solve_driver(mesh: mesh_2d, physics, models, materials, bcs, precond, tol, max_it, mode, local_for_illt=True):
+def _solve_driver(mesh: mesh_2d, physics, models, materials, bcs, precond, tol, max_it, mode, weights=None):
     import time
     ctx = mesh.ctx
     dof = 1 if physics == 0 else 2
@@ -385,11 +385,19 @@ def _solve_driver(mesh: mesh_2d, physics, models, materials, bcs, precond, tol, 
     cg = conjugate_gradient(K)
     cg.set_tolerance(tol)
     cg.set_max_interations(max_it)
+    learned_pc = None
     if precond == NLF_PRECOND_ILLT:
         KL = _assemble(mesh, physics, models, materials, inner, part=NLF_PART_LOCAL)
         P = cg.init_preconditioner(NLF_PRECOND_ILLT, KL)
         if not P.computation_info():          # fall back to identity like the reference
             cg.init_preconditioner(NLF_PRECOND_IDENTITY)
+    elif precond == _abi.NLF_PRECOND_LEARNED:
+        # the learned preconditioner is taken first; Jacobi is the fallback of the flexible CG
+        from . import learned
+        KL = _assemble(mesh, physics, models, materials, inner, part=NLF_PART_LOCAL)
+        w, _ = learned.load_weights(weights or learned.DEFAULT_WEIGHTS)
+        learned_pc = learned.LearnedPreconditioner(K, KL, models, w)
+        fallback = preconditioner(K, NLF_PRECOND_JACOBI)
     else:
         cg.init_preconditioner(precond)
     if mode != NLF_MODE_FP64:
@@ -397,7 +405,11 @@ def _solve_driver(mesh: mesh_2d, physics, models, materials, bcs, precond, tol, 
     ctx.synchronize()
     tm["preconditioner"] = time.perf_counter() - t2
     t3 = time.perf_counter()
-    x = cg.solve(fd, mode=mode)
+    if learned_pc is not None:
+        x, st = learned.fcg_solve(K, learned_pc, fd, fallback=fallback, tol=tol, max_it=max_it, mode=mode)
+        cg.stats = st
+    else:
+        x = cg.solve(fd, mode=mode)
     ctx.synchronize()
     tm["solve"] = time.perf_counter() - t3
     st = cg.stats
@@ -406,10 +418,12 @@ def _solve_driver(mesh: mesh_2d, physics, models, materials, bcs, precond, tol, 
 
 
 def stationary_heat_equation_solver_2d(mesh: mesh_2d, models, materials, bcs, precond=NLF_PRECOND_ILLT, tol=1e-12,
-                                       max_it=100000, mode=NLF_MODE_FP64) -> Solution:
+                                       max_it=100000, mode=NLF_MODE_FP64, weights=None) -> Solution:
     """K T = f with K = p1 K_L + (1 - p1) K_NL (solved directly; the reference's correction form
-    T = T0 - K^-1 (K T0 - f) with T0 = 0 is the same system)."""
-    return _solve_driver(mesh, 0, models, materials, bcs, precond, tol, max_it, mode)
+    T = T0 - K^-1 (K T0 - f) with T0 = 0 is the same system).  precond NLF_PRECOND_LEARNED runs
+    flexible CG with the learned preconditioner (weights file `weights`, default the packaged one)
+    and Jacobi-PCG as its fallback; tol is then on the FP64 true residual."""
+    return _solve_driver(mesh, 0, models, materials, bcs, precond, tol, max_it, mode, weights)
 
 
 def equilibrium_equation(mesh: mesh_2d, models, materials, bcs, precond=None, tol=1e-12, max_it=100000,

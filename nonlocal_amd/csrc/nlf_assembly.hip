@@ -498,6 +498,21 @@ __global__ void k_mirror(i64 rows, const i64* __restrict__ outer, const i32* __r
     }
 }
 
+// Largest |col - row| of a sorted-row pattern (rows are ascending: first and last entries decide).
+__global__ void k_bandwidth(i64 rows, const i64* __restrict__ outer, const i32* __restrict__ cols, unsigned* __restrict__ mx) {
+    const i64 r = i64(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (r >= rows || outer[r] == outer[r + 1]) return;
+    const i64 lo = r - i64(cols[outer[r]]), hi = i64(cols[outer[r + 1] - 1]) - r;
+    atomicMax(mx, unsigned(std::min<i64>(std::max(lo, hi), i64(0x7fffffff))));
+}
+
+// 16-bit column offsets col - row for the SpMV (only when the bandwidth fits).
+__global__ void k_to_off16(i64 rows, const i64* __restrict__ outer, const i32* __restrict__ cols, int16_t* __restrict__ off) {
+    const i64 r = i64(blockIdx.x) * (blockDim.x / 64) + (threadIdx.x >> 6);
+    if (r >= rows) return;
+    for (i64 p = outer[r] + (threadIdx.x & 63); p < outer[r + 1]; p += 64) off[p] = int16_t(i64(cols[p]) - r);
+}
+
 __global__ void k_elem_minnode(i64 E, int nn, const i32* __restrict__ el, i32* __restrict__ mn) {
     const i64 e = i64(blockIdx.x) * blockDim.x + threadIdx.x;
     if (e >= E) return;
@@ -696,6 +711,20 @@ int nlf_assemble(nlf_mesh m, int physics, const nlf_model* models, const void* m
         hipLaunchKernelGGL(k_mirror, dim3(blocks_for(rows, 4)), dim3(256), 0, s, rows, mat->outer.p, mat->cols.p,
                            mat->diag.p, mat->val.p);
         NLF_HIP(hipGetLastError());
+        {   // 16-bit column offsets for the SpMV when every |col - row| < 2^15
+            DevBuf<unsigned> bw{1};
+            NLF_HIP(hipMemsetAsync(bw.p, 0, sizeof(unsigned), s));
+            hipLaunchKernelGGL(k_bandwidth, dim3(blocks_for(rows, 256)), dim3(256), 0, s, rows, mat->outer.p, mat->cols.p,
+                               bw.p);
+            unsigned hb = 0;
+            download(&hb, bw.p, 1, s);
+            if (hb < 32768u && mat->nnz_full > 0) {
+                mat->off16.alloc(size_t(mat->nnz_full));
+                hipLaunchKernelGGL(k_to_off16, dim3(blocks_for(rows, 4)), dim3(256), 0, s, rows, mat->outer.p, mat->cols.p,
+                                   mat->off16.p);
+                NLF_HIP(hipGetLastError());
+            }
+        }
         NLF_HIP(hipStreamSynchronize(s));
         mat->nnz_upper = (mat->nnz_full + rows) / 2;
         mat->seconds_values = std::chrono::duration<double>(std::chrono::steady_clock::now() - t1).count();

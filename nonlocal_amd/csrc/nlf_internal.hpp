@@ -198,6 +198,7 @@ struct nlf_matrix_s {
     nlf::DevBuf<nlf::i64> tpos;          // position of the transposed entry (filled on demand)
     nlf::DevBuf<double> val;             // full values
     nlf::DevBuf<float> val32;            // FP32 copy (mixed precision), on demand
+    nlf::DevBuf<int16_t> off16;          // col - row per nonzero when every |col - row| < 2^15 (SpMV)
     nlf::DevBuf<nlf::i64> b_outer;       // bound part: rows+1
     nlf::DevBuf<nlf::i32> b_cols;
     nlf::DevBuf<double> b_val;

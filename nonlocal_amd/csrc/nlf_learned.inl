@@ -945,7 +945,8 @@ double learned_loss_grad(LearnedGraph& g, const float* w, nlf_matrix A, const do
     const i64 n = g.n;
     learned_forward_train(g, w, r, g.zbuf.p, s);
     const int lpr = pick_lpr(double(A->nnz_full) / double(std::max<i64>(A->rows, 1)));
-    launch_spmv<double, double, double, false>(lpr, n, A->outer.p, A->cols.p, A->val.p, g.zbuf.p, g.az.p, nullptr, nullptr, s);
+    launch_spmv<double, double, double, false>(lpr, n, A->outer.p, A->cols.p, A->off16.p, A->val.p, g.zbuf.p, g.az.p, nullptr,
+                                               nullptr, s);
     const unsigned vb = grid_stride_blocks(n);
     DevBuf<double> p1{vb}, p2{vb}, sums{2}, gz{size_t(n)};
     hipLaunchKernelGGL(k_lf_loss, dim3(vb), dim3(256), 0, s, n, g.zbuf.p, e, g.az.p, r, gz.p, p1.p, p2.p);
@@ -1077,7 +1078,7 @@ i64 fcg(hipStream_t s, nlf_matrix A, const V* val, nlf_precond M, const double* 
     };
     auto spmv_pq = [&](bool sample) {             // q = A p, partial p.q into pb; partial p.r into pa
         if (sample) timer.start(0, s);
-        launch_spmv<V, double, double, true>(lpr, n, A->outer.p, A->cols.p, val, p.p, q.p, pb.p, nullptr, s);
+        launch_spmv<V, double, double, true>(lpr, n, A->outer.p, A->cols.p, A->off16.p, val, p.p, q.p, pb.p, nullptr, s);
         if (sample) timer.stop(0, s);
         hipLaunchKernelGGL(k_dot<double>, dim3(sb), dim3(SPMV_THREADS), 0, s, n, p.p, r.p, pa.p, (const CGState*)nullptr);
     };
@@ -1336,7 +1337,7 @@ int nlf_fcg_solve(nlf_matrix A, nlf_precond M, nlf_precond fallback, const doubl
         out.spmv_ms = timer.avg(0);
         out.precond_ms = timer.avg(1);
         out.spmv_samples = timer.count[0];
-        out.spmv_bytes = A->nnz_full * i64((mode == NLF_MODE_MIXED ? 4 : 8) + 4) + A->rows * i64(8 + 16);
+        out.spmv_bytes = A->nnz_full * i64((mode == NLF_MODE_MIXED ? 4 : 8) + (A->off16.p ? 2 : 4)) + A->rows * i64(8 + 16);
         if (it < 0) {
             // stagnation: fall back to PCG with the fallback preconditioner from the current iterate
             out.fallbacks = 1;
@@ -1345,6 +1346,7 @@ int nlf_fcg_solve(nlf_matrix A, nlf_precond M, nlf_precond fallback, const doubl
             // the PCG tolerance is tightened tenfold (at most three times) until the true residual meets tol
             int breakdown = 0;
             CG<double, double> cg(s, A->rows, A->nnz_full, A->outer.p, A->cols.p, A->val.p, fallback);
+            cg.off16 = A->off16.p;
             const unsigned rb = spmv_blocks(64, A->rows);
             DevBuf<double> r{size_t(A->rows)}, prr{rb}, pbb{rb}, sums{2};
             i64 left = max_it + it, total = -it;

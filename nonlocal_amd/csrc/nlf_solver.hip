@@ -45,25 +45,52 @@ __device__ __forceinline__ Acc block_sum(Acc v, Acc* sh) {
     return s;
 }
 
-// y = A x (rows in [0, rows)), optionally partial[b] = sum_r x_r y_r over the block's rows.
+// y = A x (rows in [0, rows)), optionally partial[b] = sum_r x_r y_r over the block's rows.  LPR
+// lanes per row, grid-stride over row groups (the grid is capped, so DOT partials stay few).
+// Columns are absolute (cols) or, when off16 is given, 16-bit offsets from the row (col = r + off):
+// 10 instead of 12 bytes per FP64 nonzero.  Each lane keeps four (value, column) pairs and then
+// four x gathers in flight.
 template<int LPR, class V, class X, class Acc, bool DOT>
 __global__ __launch_bounds__(SPMV_THREADS) void k_spmv(i64 rows, const i64* __restrict__ outer, const i32* __restrict__ cols,
-                                                       const V* __restrict__ val, const X* __restrict__ x, X* __restrict__ y,
+                                                       const int16_t* __restrict__ off16, const V* __restrict__ val,
+                                                       const X* __restrict__ x, X* __restrict__ y,
                                                        double* __restrict__ partial, const CGState* __restrict__ st) {
     __shared__ double sh[SPMV_THREADS / 64];
     if (st && st->done) return;
+    constexpr int RPB = SPMV_THREADS / LPR;
     const int sub = threadIdx.x % LPR;
-    const i64 r = i64(blockIdx.x) * (SPMV_THREADS / LPR) + threadIdx.x / LPR;
-    Acc acc = 0;
-    if (r < rows) {
-        const i64 b = outer[r], e = outer[r + 1];
-        for (i64 p = b + sub; p < e; p += LPR) acc += Acc(val[p]) * Acc(x[cols[p]]);
-    }
-    for (int o = LPR / 2; o > 0; o >>= 1) acc += shfl_down_t(acc, o);   // lane sub==0 of each group holds the sum
     double d = 0.0;
-    if (r < rows && sub == 0) {
-        y[r] = X(acc);
-        if (DOT) d = double(x[r]) * double(X(acc));
+    const i64 rows_r = (rows + RPB - 1) / RPB * RPB;
+    for (i64 r0 = i64(blockIdx.x) * RPB; r0 < rows_r; r0 += i64(gridDim.x) * RPB) {
+        const i64 r = r0 + threadIdx.x / LPR;
+        Acc acc = 0;
+        if (r < rows) {
+            const i64 b = outer[r], e = outer[r + 1];
+            i64 p = b + sub;
+            for (; p + 3 * LPR < e; p += 4 * LPR) {
+                i64 c[4];
+                V v[4];
+#pragma unroll
+                for (int t = 0; t < 4; ++t) {
+                    c[t] = off16 ? r + i64(off16[p + t * LPR]) : i64(cols[p + t * LPR]);
+                    v[t] = val[p + t * LPR];
+                }
+                X xv[4];
+#pragma unroll
+                for (int t = 0; t < 4; ++t) xv[t] = x[c[t]];
+#pragma unroll
+                for (int t = 0; t < 4; ++t) acc += Acc(v[t]) * Acc(xv[t]);
+            }
+            for (; p < e; p += LPR) {
+                const i64 c = off16 ? r + i64(off16[p]) : i64(cols[p]);
+                acc += Acc(val[p]) * Acc(x[c]);
+            }
+        }
+        for (int o = LPR / 2; o > 0; o >>= 1) acc += shfl_down_t(acc, o);   // lane sub==0 of each group holds the sum
+        if (r < rows && sub == 0) {
+            y[r] = X(acc);
+            if (DOT) d += double(x[r]) * double(X(acc));
+        }
     }
     if (DOT) {
         d = block_sum<double>(d, sh);
@@ -226,17 +253,21 @@ __global__ __launch_bounds__(SPMV_THREADS) void k_residual(i64 rows, const i64* 
                                                            double* __restrict__ prr, double* __restrict__ pbb) {
     __shared__ double sh[SPMV_THREADS / 64];
     const int lane = threadIdx.x & 63;
-    const i64 row = i64(blockIdx.x) * (SPMV_THREADS / 64) + (threadIdx.x >> 6);
-    double acc = 0.0;
-    if (row < rows)
-        for (i64 p = outer[row] + lane; p < outer[row + 1]; p += 64) acc += double(val[p]) * double(x[cols[p]]);
-    for (int o = 32; o > 0; o >>= 1) acc += __shfl_down(acc, o);
+    constexpr int RPB = SPMV_THREADS / 64;
     double rr = 0.0, bb = 0.0;
-    if (row < rows && lane == 0) {
-        const double ri = double(b[row]) - acc;
-        r[row] = X(ri);
-        rr = ri * ri;
-        bb = double(b[row]) * double(b[row]);
+    const i64 rows_r = (rows + RPB - 1) / RPB * RPB;
+    for (i64 r0 = i64(blockIdx.x) * RPB; r0 < rows_r; r0 += i64(gridDim.x) * RPB) {     // one wave per row
+        const i64 row = r0 + (threadIdx.x >> 6);
+        double acc = 0.0;
+        if (row < rows)
+            for (i64 p = outer[row] + lane; p < outer[row + 1]; p += 64) acc += double(val[p]) * double(x[cols[p]]);
+        for (int o = 32; o > 0; o >>= 1) acc += __shfl_down(acc, o);
+        if (row < rows && lane == 0) {
+            const double ri = double(b[row]) - acc;
+            r[row] = X(ri);
+            rr += ri * ri;
+            bb += double(b[row]) * double(b[row]);
+        }
     }
     rr = block_sum<double>(rr, sh);
     bb = block_sum<double>(bb, sh);
@@ -819,14 +850,20 @@ int pick_lpr(double avg) {
     return 4;
 }
 
+// SpMV grid: one row group per block iteration, capped so that DOT partials stay <= SPMV_MAX_BLOCKS
+constexpr i64 SPMV_MAX_BLOCKS = 4096;
+unsigned spmv_blocks(int lpr, i64 rows) {
+    return unsigned(std::min<i64>(std::max<i64>((rows + (SPMV_THREADS / lpr) - 1) / (SPMV_THREADS / lpr), 1), SPMV_MAX_BLOCKS));
+}
+
 template<class V, class X, class Acc, bool DOT>
-void launch_spmv(int lpr, i64 rows, const i64* outer, const i32* cols, const V* val, const X* x, X* y, double* partial,
-                 const CGState* st, hipStream_t s) {
-    const unsigned blocks = unsigned((rows + (SPMV_THREADS / lpr) - 1) / (SPMV_THREADS / lpr));
+void launch_spmv(int lpr, i64 rows, const i64* outer, const i32* cols, const int16_t* off16, const V* val, const X* x, X* y,
+                 double* partial, const CGState* st, hipStream_t s) {
+    const unsigned blocks = spmv_blocks(lpr, rows);
 #define NLF_SPMV_CASE(L)                                                                                           \
     case L:                                                                                                        \
-        hipLaunchKernelGGL((k_spmv<L, V, X, Acc, DOT>), dim3(blocks), dim3(SPMV_THREADS), 0, s, rows, outer, cols, val, \
-                           x, y, partial, st);                                                                     \
+        hipLaunchKernelGGL((k_spmv<L, V, X, Acc, DOT>), dim3(blocks), dim3(SPMV_THREADS), 0, s, rows, outer, cols, off16, \
+                           val, x, y, partial, st);                                                                \
         break;
     switch (lpr) {
         NLF_SPMV_CASE(4)
@@ -838,7 +875,6 @@ void launch_spmv(int lpr, i64 rows, const i64* outer, const i32* cols, const V* 
 #undef NLF_SPMV_CASE
 }
 
-unsigned spmv_blocks(int lpr, i64 rows) { return unsigned((rows + (SPMV_THREADS / lpr) - 1) / (SPMV_THREADS / lpr)); }
 
 // Pairs of HIP events around sampled kernel launches on the solver's stream (slot 0: SpMV, 1: preconditioner).
 struct KernelTimer {
@@ -884,6 +920,7 @@ struct CG {
     int lpr;
     const i64* outer;
     const i32* cols;
+    const int16_t* off16 = nullptr;     // 16-bit column offsets of the matrix, when it has them
     const V* val;
     const nlf_precond_s* P;
     DevBuf<X> r, z, p;
@@ -1008,7 +1045,7 @@ struct CG {
     void iteration(X* x, bool sample = false) {
         // z = A p ; alpha = rz / p.z
         if (sample) timer.start(0, s);
-        launch_spmv<V, X, double, true>(lpr, n, outer, cols, val, p.p, z.p, part1.p, st.p, s);
+        launch_spmv<V, X, double, true>(lpr, n, outer, cols, off16, val, p.p, z.p, part1.p, st.p, s);
         if (sample) timer.stop(0, s);
         hipLaunchKernelGGL(k_alpha, dim3(1), dim3(1024), 0, s, part1.p, int(sblocks), st.p);
         if (!P || P->kind == NLF_PRECOND_IDENTITY || jacobi()) {
@@ -1049,7 +1086,8 @@ int nlf_spmv(nlf_matrix a, const double* x, double* y) {
     if (!a) return NLF_ERR_ARGUMENT;
     return guarded(a->ctx, [&] {
         const int lpr = pick_lpr(double(a->nnz_full) / double(std::max<i64>(a->rows, 1)));
-        launch_spmv<double, double, double, false>(lpr, a->rows, a->outer.p, a->cols.p, a->val.p, x, y, nullptr, nullptr,
+        launch_spmv<double, double, double, false>(lpr, a->rows, a->outer.p, a->cols.p, a->off16.p, a->val.p, x, y, nullptr,
+                                                   nullptr,
                                                    a->ctx->stream);
         NLF_HIP(hipGetLastError());
     });
@@ -1061,7 +1099,8 @@ int nlf_spmv_fp32(nlf_matrix a, const float* x, float* y) {
     return guarded(a->ctx, [&] {
         NLF_CHECK(a->val32.p, NLF_ERR_STATE, "no FP32 copy");
         const int lpr = pick_lpr(double(a->nnz_full) / double(std::max<i64>(a->rows, 1)));
-        launch_spmv<float, float, float, false>(lpr, a->rows, a->outer.p, a->cols.p, a->val32.p, x, y, nullptr, nullptr,
+        launch_spmv<float, float, float, false>(lpr, a->rows, a->outer.p, a->cols.p, a->off16.p, a->val32.p, x, y, nullptr,
+                                                nullptr,
                                                 a->ctx->stream);
         NLF_HIP(hipGetLastError());
     });
@@ -1071,10 +1110,10 @@ int nlf_spmv_fp32(nlf_matrix a, const float* x, float* y) {
 int nlf_spmv_lpr(nlf_matrix a, int lpr, int fp32, const void* x, void* y) {
     if (!a) return NLF_ERR_ARGUMENT;
     return guarded(a->ctx, [&] {
-        if (fp32) launch_spmv<float, float, float, false>(lpr, a->rows, a->outer.p, a->cols.p, a->val32.p,
+        if (fp32) launch_spmv<float, float, float, false>(lpr, a->rows, a->outer.p, a->cols.p, a->off16.p, a->val32.p,
                                                           static_cast<const float*>(x), static_cast<float*>(y), nullptr,
                                                           nullptr, a->ctx->stream);
-        else launch_spmv<double, double, double, false>(lpr, a->rows, a->outer.p, a->cols.p, a->val.p,
+        else launch_spmv<double, double, double, false>(lpr, a->rows, a->outer.p, a->cols.p, a->off16.p, a->val.p,
                                                         static_cast<const double*>(x), static_cast<double*>(y), nullptr,
                                                         nullptr, a->ctx->stream);
         NLF_HIP(hipGetLastError());
@@ -1232,10 +1271,11 @@ int nlf_cg_solve(nlf_matrix a, nlf_precond p, const double* b, double* x, double
             out.spmv_ms = t.avg(0);
             out.precond_ms = t.avg(1);
             out.spmv_samples = t.count[0];
-            out.spmv_bytes = a->nnz_full * i64(sv + sizeof(i32)) + n * i64(sizeof(i64) + 2 * sx);
+            out.spmv_bytes = a->nnz_full * i64(sv + (a->off16.p ? sizeof(int16_t) : sizeof(i32))) + n * i64(sizeof(i64) + 2 * sx);
         };
         if (mode == NLF_MODE_FP64) {
             CG<double, double> cg(s, n, a->nnz_full, a->outer.p, a->cols.p, a->val.p, p);
+            cg.off16 = a->off16.p;
             out.iterations = cg.solve(b, x, tol, max_it, &out.residual, &breakdown);
             timings(cg.timer, sizeof(double), sizeof(double));
         } else {
@@ -1245,6 +1285,7 @@ int nlf_cg_solve(nlf_matrix a, nlf_precond p, const double* b, double* x, double
             // and need several times the iterations, while FP32 storage costs none; the FP64 true
             // residual with the FP64 matrix replaces the recursive one at every reliable update.
             CG<float, double> cg(s, n, a->nnz_full, a->outer.p, a->cols.p, a->val32.p, p);
+            cg.off16 = a->off16.p;
             out.iterations = cg.solve_reliable(a->outer.p, a->cols.p, a->val.p, b, x, tol, max_it, 0.1,
                                                &out.true_residual, &out.outer_iterations, &breakdown);
             out.residual = out.true_residual;
