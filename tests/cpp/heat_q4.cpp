@@ -1,0 +1,55 @@
+// Stationary nonlocal heat conduction on an n x n Q4 mesh through the C++ mirror of the reference API
+// (mesh_2d, find_neighbours, conductivity_matrix_2d, conjugate_gradient with ILLT), written the way
+// the reference's own tests drive the solver (isotropic_solid_ring.cpp-style set-up).
+//
+//   heat_q4 <n> <p1> <r> <out.bin>
+//
+// T = 0 on x = 0, unit inward flux on x = 1, insulated elsewhere, lambda = 1.  Writes the
+// temperature (n+1)^2 doubles to out.bin and prints iterations and residuals.
+#include <nonlocal/nonlocal_amd.hpp>
+
+#include <cstdio>
+#include <cstdlib>
+#include <fstream>
+
+int main(int argc, char** argv) {
+    if (argc < 5) {
+        std::fprintf(stderr, "usage: %s n p1 r out.bin\n", argv[0]);
+        return 2;
+    }
+    const int n = std::atoi(argv[1]);
+    const double p1 = std::atof(argv[2]), r = std::atof(argv[3]);
+    using namespace nonlocal;
+    try {
+        mesh::mesh_2d_data d;
+        auto v = [n](int i, int j) { return int32_t(j * (n + 1) + i); };
+        for (int j = 0; j <= n; ++j)
+            for (int i = 0; i <= n; ++i) d.nodes.push_back({double(i) / n, double(j) / n});
+        for (int j = 0; j < n; ++j)
+            for (int i = 0; i < n; ++i)
+                for (int32_t id : {v(i, j), v(i + 1, j), v(i + 1, j + 1), v(i, j + 1)}) d.elements.push_back(id);
+        d.group_names = {"material"};
+        d.group_begin = {0, int64_t(n) * n};
+        d.boundary_names = {"left", "right", "bottom", "top"};
+        for (int j = 0; j < n; ++j) { d.boundary_edges.push_back(v(0, j)); d.boundary_edges.push_back(v(0, j + 1)); }
+        for (int j = 0; j < n; ++j) { d.boundary_edges.push_back(v(n, j)); d.boundary_edges.push_back(v(n, j + 1)); }
+        for (int i = 0; i < n; ++i) { d.boundary_edges.push_back(v(i, 0)); d.boundary_edges.push_back(v(i + 1, 0)); }
+        for (int i = 0; i < n; ++i) { d.boundary_edges.push_back(v(i, n)); d.boundary_edges.push_back(v(i + 1, n)); }
+        d.boundary_begin = {0, n, 2 * int64_t(n), 3 * int64_t(n), 4 * int64_t(n)};
+        auto m = std::make_shared<mesh::mesh_2d<double, int64_t>>(d);
+
+        thermal::parameters_2d parameters;
+        parameters["material"] = thermal::parameter_2d{model_parameters::bell(p1, r), {NLF_CONDUCTIVITY_ISOTROPIC, 0, {1.0, 0, 0}}};
+        thermal::boundaries_conditions_2d conditions;
+        conditions["left"] = thermal::boundary_condition::temperature(0.0);
+        conditions["right"] = thermal::boundary_condition::flux(1.0);
+        const auto sol = thermal::stationary_heat_equation_solver_2d(m, parameters, conditions, 1e-13, 100000);
+        std::ofstream(argv[4], std::ios::binary)
+            .write(reinterpret_cast<const char*>(sol.temperature.data()), std::streamsize(sol.temperature.size() * sizeof(double)));
+        std::printf("iterations %ju residual %.3e true_residual %.3e\n", sol.iterations, sol.residual, sol.true_residual);
+        return 0;
+    } catch (const std::exception& e) {
+        std::fprintf(stderr, "error: %s\n", e.what());
+        return 1;
+    }
+}

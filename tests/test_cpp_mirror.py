@@ -1,0 +1,44 @@
+"""The C++ mirror of the reference API (nonlocal_amd/include/nonlocal/nonlocal_amd.hpp): it compiles
+against include/nlf.h and links libnlf.so (CPU), and its heat solver driver reproduces the oracle's
+solution (GPU)."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from nonlocal_amd import configs
+from tests.helpers import direct_solution, oracle_assembly, rel_l2
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+CPP = os.path.join(ROOT, "tests", "cpp")
+
+
+def _compile(out):
+    cmd = ["g++", "-O1", "-std=c++17", "-Wall", "-Wextra", "-Werror", f"-I{ROOT}/include",
+           f"-I{ROOT}/nonlocal_amd/include", "-isystem", "/opt/rocm/include", "-D__HIP_PLATFORM_AMD__",
+           os.path.join(CPP, "heat_q4.cpp"), "-o", out, f"-L{ROOT}/nonlocal_amd", "-lnlf", "-L/opt/rocm/lib",
+           "-lamdhip64", f"-Wl,-rpath,{ROOT}/nonlocal_amd", "-Wl,-rpath,/opt/rocm/lib"]
+    return subprocess.run(cmd, capture_output=True, text=True)
+
+
+def test_mirror_compiles_and_links(tmp_path):
+    res = _compile(str(tmp_path / "heat_q4"))
+    assert res.returncode == 0, res.stderr
+
+
+@pytest.mark.gpu
+def test_cpp_heat_solver_matches_oracle(tmp_path):
+    exe = os.path.join(CPP, "heat_q4")
+    if not os.path.exists(exe):
+        res = _compile(exe)
+        assert res.returncode == 0, res.stderr
+    out = str(tmp_path / "T.bin")
+    print("[cpp] heat_q4 40 0.5 0.1", flush=True)
+    res = subprocess.run([exe, "40", "0.5", "0.1", out], capture_output=True, text=True, timeout=120)
+    assert res.returncode == 0, res.stderr
+    print(res.stdout.strip(), flush=True)
+    T = np.fromfile(out, dtype=np.float64)
+    p = configs.cfg0()
+    om, K, f, xD, inner = oracle_assembly(p)
+    assert rel_l2(T, direct_solution(K, f)) < 1e-10
