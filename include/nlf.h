@@ -203,6 +203,17 @@ int nlf_trainer_get_weights(nlf_trainer t, float* w_host);
 int nlf_fcg_solve(nlf_matrix a, nlf_precond learned, nlf_precond fallback, const double* b_dev, double* x_dev, double tol,
                   int64_t max_it, int mode, nlf_solve_stats* stats);
 
+/* ---- batched solves (no reference counterpart: the reference solves one problem per call).  Many
+ * independent instances assembled into one matrix as diagonal blocks (instance i owns rows
+ * [row_begin[i], row_begin[i+1])) and solved by lockstep Jacobi-PCG with per-instance scalars,
+ * G iterations per captured hipGraph replay; each instance stops on its FP64 true residual. */
+int nlf_batch_create(nlf_matrix a, int count, const int64_t* row_begin, nlf_batch* out);
+int nlf_batch_destroy(nlf_batch b);
+/* picks each instance's SpMV tile (lanes per row, rows per block) by timing; cache_path may be NULL */
+int nlf_batch_autotune(nlf_batch b, const char* cache_path, double* chosen);
+int nlf_batch_solve(nlf_batch b, const double* b_dev, double* x_dev, double tol, int64_t max_it, int mode, int use_graph,
+                    int64_t* iterations, double* true_residual, nlf_solve_stats* stats);
+
 #ifdef __cplusplus
 }
 #endif

@@ -71,6 +71,10 @@ SIGNATURES = {
     "nlf_trainer_step": (_int, [_p, _p, _p, _p, _p, C.c_float, _f64p]),
     "nlf_trainer_get_weights": (_int, [_p, _f32p]),
     "nlf_fcg_solve": (_int, [_p, _p, _p, _p, _p, _dbl, _i64, _int, C.POINTER(_abi.SolveStats)]),
+    "nlf_batch_create": (_int, [_p, _int, _i64p, _pp]),
+    "nlf_batch_destroy": (_int, [_p]),
+    "nlf_batch_autotune": (_int, [_p, C.c_char_p, _f64p]),
+    "nlf_batch_solve": (_int, [_p, _p, _p, _dbl, _i64, _int, _int, _i64p, _f64p, C.POINTER(_abi.SolveStats)]),
 }
 
 _LIB = None

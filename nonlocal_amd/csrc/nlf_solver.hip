@@ -14,7 +14,11 @@
 #include "nlf_internal.hpp"
 
 #include <algorithm>
+#include <array>
 #include <chrono>
+#include <cstdio>
+#include <cstring>
+#include <map>
 #include <cmath>
 
 namespace nlf {
@@ -1076,6 +1080,7 @@ template<> const double* CG<float, double>::invd() const { return P->invd.p; }
 }  // namespace nlf
 
 #include "nlf_learned.inl"
+#include "nlf_batch.inl"
 
 using namespace nlf;
 
