@@ -41,6 +41,9 @@ def parse():
     ap.add_argument("--cpu-sample-n", type=int, default=160)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--methods", default="fp64_jacobi,mixed_jacobi,learned_fcg")
+    ap.add_argument("--workload", default="composite1000", choices=["composite1000", "batch64"],
+                    help="composite1000: the 1000x1000 composite (the metric's workload); batch64: 64 uneven "
+                         "instances solved together vs one after another")
     return ap.parse_args()
 
 
@@ -85,12 +88,75 @@ def cpu_baseline(n_sample, full_nodes, full_iterations, tol):
                 measured_seconds=round(t2 - t0, 3))
 
 
+def bench_batch(args):
+    """64 uneven thermal instances (Q4 sides 32-256 log-uniform, r in [0.02, 0.1], p1 in [0.2, 1],
+    lambda in [0.5, 2], seed 3): one batched assembly + lockstep mixed-precision solve under a captured
+    hipGraph, against 64 sequential plain FP64 ILLT-CG solves of the same instances."""
+    import numpy as np
+    import torch
+    from nonlocal_amd import _abi as A
+    from nonlocal_amd import api, batch, configs
+    problems = configs.cfg3_instances(64, seed=3)
+    ctx = api.default_context()
+    log(f"[bench] batch64: {sum(p.mesh.n_nodes for p in problems)} nodes in {len(problems)} instances")
+    # sequential plain FP64 ILLT-CG, one instance after the other
+    t0 = time.perf_counter()
+    seq = []
+    for p in problems:
+        mesh = api.mesh_2d(p.mesh, ctx)
+        sol = api.stationary_heat_equation_solver_2d(mesh, p.models, p.materials, p.bcs, precond=A.NLF_PRECOND_ILLT,
+                                                     tol=args.tol, max_it=100000)
+        seq.append((sol.x.clone(), int(sol.iterations)))
+        del mesh, sol
+    torch.cuda.synchronize()
+    t_seq = time.perf_counter() - t0
+    log(f"[bench] sequential FP64 ILLT-CG: {t_seq:.3f} s")
+    cache = os.path.join(ROOT, "gpurun_out", "batch_autotune.txt") if os.path.isdir(os.path.join(ROOT, "gpurun_out")) \
+        else None
+    mode = A.NLF_MODE_MIXED
+    times = []
+    for k in range(args.warmup + args.steps):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        bs = batch.BatchSolver(problems, ctx)
+        t_asm = time.perf_counter() - t0
+        target = bs.autotune(cache)
+        t1 = time.perf_counter()
+        x, its, res, st = bs.solve(tol=args.tol, max_it=100000, mode=mode, use_graph=True)
+        torch.cuda.synchronize()
+        t2 = time.perf_counter()
+        log(f"[bench] batch step {k}: assembly {t_asm:.3f} s, autotune {t1 - t0 - t_asm:.3f} s, solve {t2 - t1:.3f} s, "
+            f"max iterations {int(its.max())}, graph replays {st.outer_iterations}")
+        if k >= args.warmup:
+            times.append(t2 - t0)
+    errs = [float(torch.linalg.norm(xi - xs) / torch.linalg.norm(xs)) for xi, (xs, _) in zip(bs.split(x), seq)]
+    value = sum(times) / len(times)
+    line = {
+        "metric": "batched assembly+solve s, 64 uneven instances, mixed vs 64 sequential plain FP64",
+        "value": round(value, 4), "unit": "s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(value * 1e3, 2), "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64+f32", "data": "synthetic",
+        "config": {"workload": "64 thermal Q4 instances, sides 32-256 log-uniform, r U(0.02,0.1), p1 U(0.2,1), "
+                               "lambda U(0.5,2), seed 3, solve to %g true residual" % args.tol,
+                   "nodes": int(sum(p.mesh.n_nodes for p in problems)),
+                   "nnz_full": int(bs.K.info()["nnz_full"]), "autotune_target_nnz_per_lane": target},
+        "sequential_fp64_illt_s": round(t_seq, 4), "speedup_vs_sequential": round(t_seq / value, 3),
+        "max_rel_err_vs_fp64": max(errs), "max_true_residual": float(res.max()),
+        "iterations": {"batch_max": int(its.max()), "batch_sum": int(its.sum()),
+                       "sequential_illt_sum": int(sum(i for _, i in seq))},
+    }
+    print(json.dumps(line), flush=True)
+
+
 def main():
     args = parse()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (got --gpus {args.gpus}); multi-GPU runs are not part of this "
               f"build", file=sys.stderr)
         sys.exit(2)
+    if args.workload == "batch64":
+        bench_batch(args)
+        return
     import torch
     from nonlocal_amd import _abi as A
     from nonlocal_amd import api, configs
