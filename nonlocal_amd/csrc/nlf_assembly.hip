@@ -16,6 +16,8 @@
 
 #include <algorithm>
 #include <chrono>
+#include <cstdlib>
+#include <cstring>
 #include <cmath>
 #include <limits>
 #include <map>
@@ -283,6 +285,8 @@ struct AssemblyArgs {
     double* val;
     double* b_val;
     int local_only;      // assemble_part::LOCAL: every group treated as LOCAL
+    const double* pairs; // thermal Q4: nonlocal pair blocks from k_pairs_mfma (null = compute in place)
+    i64 plane;           // pairs[i * plane + p * 4 + j], p = neighbour-list position
 };
 
 __device__ __forceinline__ int lower_bound_i32(const i32* a, int n, i32 key) {
@@ -364,6 +368,96 @@ __device__ void nonlocal_row(const AssemblyArgs& a, const GroupParams& g, const 
         for (int k = 0; k < DOF * DOF; ++k) v[j][k] *= s;
 }
 
+// ---- MFMA element-pair contractions (thermal Q4) -------------------------------------------------
+// For a pair (eL, eNL) of one group the nonlocal block is
+//   K_ij = (1 - p1) sum_k sum_qn sum_ql Phi[ql][qn] HL_k[i][ql] HN_k[j][qn]
+// with HL_k[i][q] = w_q D_i(q)_k and HN_k[j][q] = w_q (C D_j(q))_k (conductivity_matrix_2d.hpp:132-164,
+// C taken at the nonlocal point).  V_FMA_F64_4X4X4F64 computes four independent 4x4x4 products per
+// wave; lane l = 16 h + 4 g + c belongs to block g, and (measured on gfx950) holds
+//   A_g[c][h], B_g[h][c], D_g[h][c].
+// One wave handles one element eL and four neighbours per step (block g = neighbour slot):
+//   V_k = Phi^T HL_k^T           A = Phi^T  (lane: Phi[ql = h][qn = c], one influence value per lane)
+//                                B = HL_k^T (lane: HL_k[i = c][ql = h], fixed per eL)
+//   K  += V_k^T-as-A . HN_k^T    D of the first product is exactly the A operand of the second
+//                                (A[i = c][qn = h] = V[qn = h][i = c]); B lane: HN_k[j = c][qn = h]
+// so a pair costs four MFMA instructions per four pairs and no lane permutes.  Blocks are written
+// to pairs[i * plane + p * 4 + j]; the node-centric assembly reads row i of them.
+struct PairArgs {
+    i64 E;
+    const i64* nb_off;
+    const i32* nb_ids;
+    const i32* group_of;
+    const GroupParams* groups;
+    const double* xq;          // [e][q][2]
+    const double* HL;          // [e][k][i][q], 32 per element
+    const double* HN;
+    const i32* el_min;
+    const i32* el_max;
+    const uint8_t* el_dir;
+    double* pairs;
+    i64 plane;
+};
+
+struct Weights4 {
+    double w[4];
+};
+
+__global__ void k_pair_prep(i64 E, const double* __restrict__ D, Weights4 w4, const i32* __restrict__ group_of,
+                            const GroupParams* __restrict__ groups, double* __restrict__ HL, double* __restrict__ HN) {
+    const i64 t = i64(blockIdx.x) * blockDim.x + threadIdx.x;      // (e, i, q)
+    if (t >= E * 16) return;
+    const i64 e = t / 16;
+    const int i = int(t % 16) / 4, q = int(t % 4);
+    const GroupParams g = groups[group_of[e]];
+    const double w = w4.w[q];
+    const double dx = D[2 * ((e * 4 + i) * 4 + q)], dy = D[2 * ((e * 4 + i) * 4 + q) + 1];
+    double cx, cy;
+    if (g.ckind == NLF_CONDUCTIVITY_ISOTROPIC) { cx = g.c[0] * dx; cy = g.c[0] * dy; }
+    else if (g.ckind == NLF_CONDUCTIVITY_ORTHOTROPIC) { cx = g.c[0] * dx; cy = g.c[1] * dy; }
+    else { cx = g.c[0] * dx + g.c[2] * dy; cy = g.c[2] * dx + g.c[1] * dy; }
+    HL[e * 32 + i * 4 + q] = w * dx;
+    HL[e * 32 + 16 + i * 4 + q] = w * dy;
+    HN[e * 32 + i * 4 + q] = w * cx;
+    HN[e * 32 + 16 + i * 4 + q] = w * cy;
+}
+
+__global__ __launch_bounds__(256) void k_pairs_mfma(PairArgs a) {
+    const int l = threadIdx.x & 63;
+    const int g = (l >> 2) & 3, c = l & 3, h = l >> 4;
+    const i64 waves = i64(gridDim.x) * (blockDim.x >> 6);
+    for (i64 eL = i64(blockIdx.x) * (blockDim.x >> 6) + (threadIdx.x >> 6); eL < a.E; eL += waves) {
+        const GroupParams gp = a.groups[a.group_of[eL]];          // one group per wave
+        if (!gp.nonlocal) continue;
+        const double hl0 = a.HL[eL * 32 + c * 4 + h];             // HL_0[i = c][ql = h]
+        const double hl1 = a.HL[eL * 32 + 16 + c * 4 + h];
+        const double xlx = a.xq[2 * (eL * 4 + h)], xly = a.xq[2 * (eL * 4 + h) + 1];
+        const i64 nb_b = a.nb_off[eL], cnt = a.nb_off[eL + 1] - nb_b;
+        const i32 emin = a.el_min[eL];
+        const double s = 1.0 - gp.p1;
+        for (i64 base = 0; base < cnt; base += 4) {
+            const i64 t = base + g;
+            bool need = false;
+            i64 eNL = eL;
+            if (t < cnt) {
+                eNL = a.nb_ids[nb_b + t];
+                // the pair reaches the upper triangle of some row of eL, or a Dirichlet column
+                need = a.el_max[eNL] >= emin || a.el_dir[eNL];
+            }
+            double phi = 0.0, hn0 = 0.0, hn1 = 0.0;
+            if (need) {
+                phi = influence(gp.inf, xlx, xly, a.xq[2 * (eNL * 4 + c)], a.xq[2 * (eNL * 4 + c) + 1]);
+                hn0 = a.HN[eNL * 32 + c * 4 + h];                  // HN_0[j = c][qn = h]
+                hn1 = a.HN[eNL * 32 + 16 + c * 4 + h];
+            }
+            const double v0 = __builtin_amdgcn_mfma_f64_4x4x4f64(phi, hl0, 0.0, 0, 0, 0);
+            const double v1 = __builtin_amdgcn_mfma_f64_4x4x4f64(phi, hl1, 0.0, 0, 0, 0);
+            double k = __builtin_amdgcn_mfma_f64_4x4x4f64(v0, hn0, 0.0, 0, 0, 0);
+            k = __builtin_amdgcn_mfma_f64_4x4x4f64(v1, hn1, k, 0, 0, 0);
+            if (need) a.pairs[i64(h) * a.plane + (nb_b + t) * 4 + c] = s * k;     // K[i = h][j = c]
+        }
+    }
+}
+
 template<int DOF, int NN, int NQ, bool DETERMINISTIC>
 __global__ __launch_bounds__(ASM_THREADS) void k_assemble(AssemblyArgs a) {
     extern __shared__ unsigned char smem[];
@@ -432,7 +526,16 @@ __global__ __launch_bounds__(ASM_THREADS) void k_assemble(AssemblyArgs a) {
             }
             if (active) {
                 if (nonlocal) {
-                    nonlocal_row<DOF, NN, NQ>(a, g, xl, hl, eNL, v);
+                    if constexpr (DOF == 1 && NN == 4 && NQ == 4) {
+                        if (a.pairs) {
+                            const double* pr = a.pairs + i64(iL) * a.plane + (nb_b + t) * 4;
+                            for (int j = 0; j < NN; ++j) v[j][0] = pr[j];
+                        } else {
+                            nonlocal_row<DOF, NN, NQ>(a, g, xl, hl, eNL, v);
+                        }
+                    } else {
+                        nonlocal_row<DOF, NN, NQ>(a, g, xl, hl, eNL, v);
+                    }
                     if (eNL == eL) {
                         double lv[NN][DOF * DOF];
                         local_row<DOF, NN, NQ>(a, g, eL, iL, lv);
@@ -703,6 +806,37 @@ int nlf_assemble(nlf_mesh m, int physics, const nlf_model* models, const void* m
         aa.outer = mat->outer.p; aa.cols = mat->cols.p; aa.diag = mat->diag.p; aa.b_outer = mat->b_outer.p;
         aa.b_cols = mat->b_cols.p; aa.val = mat->val.p; aa.b_val = mat->b_val.p; aa.local_only = only_local;
         const bool det = m->injective;
+        // thermal Q4 nonlocal blocks through the MFMA pair kernel when the pair buffer (128 B per
+        // neighbour pair) fits in half of the free device memory; NLF_ASSEMBLY=node forces the
+        // node-centric contraction
+        DevBuf<double> pairs, HL, HN;
+        const char* mode_env = std::getenv("NLF_ASSEMBLY");
+        const bool node_only = mode_env && std::strcmp(mode_env, "node") == 0;
+        if (physics == 0 && m->nn == 4 && any_nonlocal && !only_local && !node_only) {
+            i64 total_pairs = 0;
+            download(&total_pairs, m->nb_off.p + m->E, 1, s);
+            size_t free_b = 0, total_b = 0;
+            NLF_HIP(hipMemGetInfo(&free_b, &total_b));
+            const size_t need_b = size_t(total_pairs) * 16 * sizeof(double) + size_t(m->E) * 64 * sizeof(double);
+            if (total_pairs > 0 && need_b < free_b / 2) {
+                pairs.alloc(size_t(total_pairs) * 16);
+                HL.alloc(size_t(m->E) * 32);
+                HN.alloc(size_t(m->E) * 32);
+                Weights4 w4{};
+                for (int q = 0; q < 4; ++q) w4.w[q] = m->fe.w[q];
+                hipLaunchKernelGGL(k_pair_prep, dim3(blocks_for(m->E * 16, 256)), dim3(256), 0, s, m->E, m->D.p, w4,
+                                   m->group_of.p, d_groups.p, HL.p, HN.p);
+                PairArgs pr{};
+                pr.E = m->E; pr.nb_off = m->nb_off.p; pr.nb_ids = m->nb_ids.p; pr.group_of = m->group_of.p;
+                pr.groups = d_groups.p; pr.xq = m->xq.p; pr.HL = HL.p; pr.HN = HN.p; pr.el_min = el_min.p;
+                pr.el_max = m->el_maxnode.p; pr.el_dir = el_dir.p; pr.pairs = pairs.p; pr.plane = total_pairs * 4;
+                const unsigned pb = unsigned(std::min<i64>((m->E + 3) / 4, 16384));
+                hipLaunchKernelGGL(k_pairs_mfma, dim3(pb), dim3(256), 0, s, pr);
+                NLF_HIP(hipGetLastError());
+                aa.pairs = pairs.p;
+                aa.plane = total_pairs * 4;
+            }
+        }
         if (physics == 0 && m->nn == 4) launch_assemble<1, 4, 4>(aa, det, lds, s);
         else if (physics == 0 && m->nn == 8) launch_assemble<1, 8, 9>(aa, det, lds, s);
         else if (physics == 1 && m->nn == 4) launch_assemble<2, 4, 4>(aa, det, lds, s);
