@@ -54,7 +54,10 @@ __device__ __forceinline__ Acc block_sum(Acc v, Acc* sh) {
 // Columns are absolute (cols) or, when off16 is given, 16-bit offsets from the row (col = r + off):
 // 10 instead of 12 bytes per FP64 nonzero.  Each lane keeps four (value, column) pairs and then
 // four x gathers in flight.
-template<int LPR, class V, class X, class Acc, bool DOT>
+template<class T>
+__device__ __forceinline__ T stream_load(const T* p, bool nt) { return nt ? __builtin_nontemporal_load(p) : *p; }
+
+template<int LPR, class V, class X, class Acc, bool DOT, bool XCD = false, int UNROLL = 4, bool NT = false>
 __global__ __launch_bounds__(SPMV_THREADS) void k_spmv(i64 rows, const i64* __restrict__ outer, const i32* __restrict__ cols,
                                                        const int16_t* __restrict__ off16, const V* __restrict__ val,
                                                        const X* __restrict__ x, X* __restrict__ y,
@@ -64,30 +67,34 @@ __global__ __launch_bounds__(SPMV_THREADS) void k_spmv(i64 rows, const i64* __re
     constexpr int RPB = SPMV_THREADS / LPR;
     const int sub = threadIdx.x % LPR;
     double d = 0.0;
+    // XCD: workgroups are dealt round-robin to the 8 XCDs; renumbering them so that each XCD sweeps
+    // a contiguous range of row groups keeps the x window of neighbouring rows in one L2
+    i64 lb = blockIdx.x;
+    if (XCD && gridDim.x % 8 == 0) lb = i64(blockIdx.x % 8) * (gridDim.x / 8) + blockIdx.x / 8;
     const i64 rows_r = (rows + RPB - 1) / RPB * RPB;
-    for (i64 r0 = i64(blockIdx.x) * RPB; r0 < rows_r; r0 += i64(gridDim.x) * RPB) {
+    for (i64 r0 = lb * RPB; r0 < rows_r; r0 += i64(gridDim.x) * RPB) {
         const i64 r = r0 + threadIdx.x / LPR;
         Acc acc = 0;
         if (r < rows) {
             const i64 b = outer[r], e = outer[r + 1];
             i64 p = b + sub;
-            for (; p + 3 * LPR < e; p += 4 * LPR) {
-                i64 c[4];
-                V v[4];
+            for (; p + (UNROLL - 1) * LPR < e; p += UNROLL * LPR) {
+                i64 c[UNROLL];
+                V v[UNROLL];
 #pragma unroll
-                for (int t = 0; t < 4; ++t) {
-                    c[t] = off16 ? r + i64(off16[p + t * LPR]) : i64(cols[p + t * LPR]);
-                    v[t] = val[p + t * LPR];
+                for (int t = 0; t < UNROLL; ++t) {
+                    c[t] = off16 ? r + i64(stream_load(off16 + p + t * LPR, NT)) : i64(stream_load(cols + p + t * LPR, NT));
+                    v[t] = stream_load(val + p + t * LPR, NT);
                 }
-                X xv[4];
+                X xv[UNROLL];
 #pragma unroll
-                for (int t = 0; t < 4; ++t) xv[t] = x[c[t]];
+                for (int t = 0; t < UNROLL; ++t) xv[t] = x[c[t]];
 #pragma unroll
-                for (int t = 0; t < 4; ++t) acc += Acc(v[t]) * Acc(xv[t]);
+                for (int t = 0; t < UNROLL; ++t) acc += Acc(v[t]) * Acc(xv[t]);
             }
             for (; p < e; p += LPR) {
-                const i64 c = off16 ? r + i64(off16[p]) : i64(cols[p]);
-                acc += Acc(val[p]) * Acc(x[c]);
+                const i64 c = off16 ? r + i64(stream_load(off16 + p, NT)) : i64(stream_load(cols + p, NT));
+                acc += Acc(stream_load(val + p, NT)) * Acc(x[c]);
             }
         }
         for (int o = LPR / 2; o > 0; o >>= 1) acc += shfl_down_t(acc, o);   // lane sub==0 of each group holds the sum
@@ -846,12 +853,12 @@ void precond_apply<float>(const nlf_precond_s* P, i64 n, const float* r, float* 
         NLF_HIP(hipMemcpyAsync(z, r, size_t(n) * sizeof(float), hipMemcpyDeviceToDevice, s));
 }
 
+// Lanes per row: the smallest power of two giving each lane at most 16 nonzeros (4..64).  Measured on
+// the 1000 x 1000 composite (500 nonzeros per row): 32 lanes 1.19 ms, 64 lanes 1.29 ms, 16 lanes 1.31 ms.
 int pick_lpr(double avg) {
-    if (avg > 96) return 64;
-    if (avg > 48) return 32;
-    if (avg > 24) return 16;
-    if (avg > 12) return 8;
-    return 4;
+    int l = 4;
+    while (l < 64 && double(l) * 16.0 < avg) l *= 2;
+    return l;
 }
 
 // SpMV grid: one row group per block iteration, capped so that DOT partials stay <= SPMV_MAX_BLOCKS
@@ -862,8 +869,23 @@ unsigned spmv_blocks(int lpr, i64 rows) {
 
 template<class V, class X, class Acc, bool DOT>
 void launch_spmv(int lpr, i64 rows, const i64* outer, const i32* cols, const int16_t* off16, const V* val, const X* x, X* y,
-                 double* partial, const CGState* st, hipStream_t s) {
+                 double* partial, const CGState* st, hipStream_t s, bool xcd = false) {
     const unsigned blocks = spmv_blocks(lpr, rows);
+    if (xcd) {
+#define NLF_SPMV_XCASE(L)                                                                                          \
+    case L:                                                                                                        \
+        hipLaunchKernelGGL((k_spmv<L, V, X, Acc, DOT, true>), dim3(blocks), dim3(SPMV_THREADS), 0, s, rows, outer, cols, \
+                           off16, val, x, y, partial, st);                                                         \
+        return;
+        switch (lpr) {
+            NLF_SPMV_XCASE(4)
+            NLF_SPMV_XCASE(8)
+            NLF_SPMV_XCASE(16)
+            NLF_SPMV_XCASE(32)
+            default: NLF_SPMV_XCASE(64)
+        }
+#undef NLF_SPMV_XCASE
+    }
 #define NLF_SPMV_CASE(L)                                                                                           \
     case L:                                                                                                        \
         hipLaunchKernelGGL((k_spmv<L, V, X, Acc, DOT>), dim3(blocks), dim3(SPMV_THREADS), 0, s, rows, outer, cols, off16, \
@@ -1084,6 +1106,17 @@ template<> const double* CG<float, double>::invd() const { return P->invd.p; }
 
 using namespace nlf;
 
+namespace nlf {
+
+template<int LPR, int UNROLL, bool NT>
+void launch_spmv_variant(nlf_matrix a, const double* x, double* y) {
+    hipLaunchKernelGGL((k_spmv<LPR, double, double, double, false, false, UNROLL, NT>), dim3(spmv_blocks(LPR, a->rows)),
+                       dim3(SPMV_THREADS), 0, a->ctx->stream, a->rows, a->outer.p, a->cols.p, a->off16.p, a->val.p, x, y,
+                       nullptr, nullptr);
+}
+
+}  // namespace nlf
+
 extern "C" {
 
 // y = A x on the inner part (the reference's selfadjointView<Upper>() product), device vectors.
@@ -1111,16 +1144,34 @@ int nlf_spmv_fp32(nlf_matrix a, const float* x, float* y) {
     });
 }
 
-// Same SpMV with an explicit lanes-per-row choice (autotuning / benchmarking).
+
+// Same SpMV with an explicit lanes-per-row choice (autotuning / benchmarking).  lpr bit 8 selects
+// the XCD-contiguous workgroup numbering, bit 9 the 32-bit column indices instead of the offsets;
+// bits 10 / 11 (FP64 only, LPR 32 or 64) select 8 nonzeros in flight per lane / non-temporal
+// loads of the matrix stream.
 int nlf_spmv_lpr(nlf_matrix a, int lpr, int fp32, const void* x, void* y) {
     if (!a) return NLF_ERR_ARGUMENT;
     return guarded(a->ctx, [&] {
-        if (fp32) launch_spmv<float, float, float, false>(lpr, a->rows, a->outer.p, a->cols.p, a->off16.p, a->val32.p,
+        const int variant = (lpr >> 10) & 3;
+        if (variant && !fp32 && ((lpr & 0xff) == 32 || (lpr & 0xff) == 64)) {
+            const double* xd = static_cast<const double*>(x);
+            double* yd = static_cast<double*>(y);
+            const bool l32 = (lpr & 0xff) == 32;
+            if (variant == 1) l32 ? launch_spmv_variant<32, 8, false>(a, xd, yd) : launch_spmv_variant<64, 8, false>(a, xd, yd);
+            else if (variant == 2) l32 ? launch_spmv_variant<32, 4, true>(a, xd, yd) : launch_spmv_variant<64, 4, true>(a, xd, yd);
+            else l32 ? launch_spmv_variant<32, 8, true>(a, xd, yd) : launch_spmv_variant<64, 8, true>(a, xd, yd);
+            NLF_HIP(hipGetLastError());
+            return;
+        }
+        const bool xcd = (lpr & 0x100) != 0;
+        const int16_t* off = (lpr & 0x200) ? nullptr : a->off16.p;
+        lpr &= 0xff;
+        if (fp32) launch_spmv<float, float, float, false>(lpr, a->rows, a->outer.p, a->cols.p, off, a->val32.p,
                                                           static_cast<const float*>(x), static_cast<float*>(y), nullptr,
-                                                          nullptr, a->ctx->stream);
-        else launch_spmv<double, double, double, false>(lpr, a->rows, a->outer.p, a->cols.p, a->off16.p, a->val.p,
+                                                          nullptr, a->ctx->stream, xcd);
+        else launch_spmv<double, double, double, false>(lpr, a->rows, a->outer.p, a->cols.p, off, a->val.p,
                                                         static_cast<const double*>(x), static_cast<double*>(y), nullptr,
-                                                        nullptr, a->ctx->stream);
+                                                        nullptr, a->ctx->stream, xcd);
         NLF_HIP(hipGetLastError());
     });
 }
