@@ -118,6 +118,8 @@ int         nlf_ctx_create(int device, void* stream, nlf_ctx* out);
 int         nlf_ctx_destroy(nlf_ctx ctx);
 const char* nlf_last_error(nlf_ctx ctx);
 int         nlf_ctx_synchronize(nlf_ctx ctx);
+#define NLF_OPT_ASSEMBLY_PAIRS 1     /* 1: MFMA element-pair contractions (thermal Q4), 0: node-centric */
+int         nlf_ctx_set_option(nlf_ctx ctx, int option, int value);
 void*       nlf_ctx_stream(nlf_ctx ctx);
 
 /* ---- mesh: replaces mesh_2d<T,I>(path) + mesh_2d::init (libraries/mesh/mesh_2d/mesh_2d.hpp:83-108).
@@ -188,6 +190,12 @@ int nlf_learned_init_weights(uint64_t seed, float* w_host);
 int nlf_learned_create(nlf_matrix a, nlf_matrix a_local, const nlf_model* models, nlf_precond* out);
 int nlf_learned_set_weights(nlf_precond p, const float* w_host);
 int nlf_learned_get_weights(nlf_precond p, float* w_host);
+/* the network is linear in r: set_weights compiles its symmetric part into a sparse matrix on the
+ * three-hop pattern (applied as one SpMV); compiled = 0 when a row's node sets exceed the kernel's
+ * capacities, the network kernels are then used */
+int nlf_learned_info(nlf_precond p, int* compiled, int64_t* nnz, double* compile_seconds);
+/* 0 = apply the compiled operator when available (default), 1 = always run the network kernels */
+int nlf_learned_set_apply(nlf_precond p, int network);
 int nlf_learned_export_graph(nlf_precond p, int64_t* outer, int32_t* cols, float* feat, float* s);
 /* loss ||M(r) - e||_A^2 / ||e||_A^2 for a pair with A e = r, and its weight gradient (host) */
 int nlf_learned_loss_grad(nlf_precond p, nlf_matrix a, const double* r_dev, const double* e_dev, double* loss,

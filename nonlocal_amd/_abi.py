@@ -13,6 +13,7 @@ NLF_CONDUCTIVITY_ISOTROPIC, NLF_CONDUCTIVITY_ORTHOTROPIC, NLF_CONDUCTIVITY_ANISO
 NLF_BC_NONE, NLF_BC_FIRST_KIND, NLF_BC_FLUX, NLF_BC_CONVECTION = 0, 1, 2, 3
 NLF_PRECOND_IDENTITY, NLF_PRECOND_JACOBI, NLF_PRECOND_ILLT, NLF_PRECOND_LEARNED = 0, 1, 2, 3
 NLF_MODE_FP64, NLF_MODE_MIXED = 0, 1
+NLF_OPT_ASSEMBLY_PAIRS = 1
 
 
 class Model(C.Structure):

@@ -30,6 +30,7 @@ SIGNATURES = {
     "nlf_ctx_destroy": (_int, [_p]),
     "nlf_last_error": (C.c_char_p, [_p]),
     "nlf_ctx_synchronize": (_int, [_p]),
+    "nlf_ctx_set_option": (_int, [_p, _int, _int]),
     "nlf_ctx_stream": (_p, [_p]),
     "nlf_mesh_create": (_int, [_p, _i64, _f64p, _i64, _int, _i32p, _int, _i64p, _i64, _int, _i32p, _int, _i64p, _pp]),
     "nlf_mesh_destroy": (_int, [_p]),
@@ -64,6 +65,8 @@ SIGNATURES = {
     "nlf_learned_create": (_int, [_p, _p, C.POINTER(_abi.Model), _pp]),
     "nlf_learned_set_weights": (_int, [_p, _f32p]),
     "nlf_learned_get_weights": (_int, [_p, _f32p]),
+    "nlf_learned_info": (_int, [_p, C.POINTER(_int), _i64p, _f64p]),
+    "nlf_learned_set_apply": (_int, [_p, _int]),
     "nlf_learned_export_graph": (_int, [_p, _i64p, _i32p, _f32p, _f32p]),
     "nlf_learned_loss_grad": (_int, [_p, _p, _p, _p, _f64p, _f32p]),
     "nlf_trainer_create": (_int, [_p, _f32p, _pp]),

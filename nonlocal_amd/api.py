@@ -68,6 +68,9 @@ class Context:
     def synchronize(self):
         self.check(load().nlf_ctx_synchronize(self.h))
 
+    def set_option(self, option, value):
+        self.check(load().nlf_ctx_set_option(self.h, option, int(value)))
+
     def __del__(self):
         if getattr(self, "h", None):
             _destroy("nlf_ctx_destroy", self.h)

@@ -807,11 +807,10 @@ int nlf_assemble(nlf_mesh m, int physics, const nlf_model* models, const void* m
         aa.b_cols = mat->b_cols.p; aa.val = mat->val.p; aa.b_val = mat->b_val.p; aa.local_only = only_local;
         const bool det = m->injective;
         // thermal Q4 nonlocal blocks through the MFMA pair kernel when the pair buffer (128 B per
-        // neighbour pair) fits in half of the free device memory; NLF_ASSEMBLY=node forces the
-        // node-centric contraction
+        // neighbour pair) fits in half of the free device memory (option NLF_OPT_ASSEMBLY_PAIRS = 0
+        // forces the node-centric contraction)
         DevBuf<double> pairs, HL, HN;
-        const char* mode_env = std::getenv("NLF_ASSEMBLY");
-        const bool node_only = mode_env && std::strcmp(mode_env, "node") == 0;
+        const bool node_only = m->ctx->assembly_pairs == 0;
         if (physics == 0 && m->nn == 4 && any_nonlocal && !only_local && !node_only) {
             i64 total_pairs = 0;
             download(&total_pairs, m->nb_off.p + m->E, 1, s);

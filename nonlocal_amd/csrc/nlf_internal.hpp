@@ -154,6 +154,7 @@ struct nlf_ctx_s {
     hipStream_t stream = nullptr;
     bool own_stream = false;
     std::string error;
+    int assembly_pairs = 1;              // NLF_OPT_ASSEMBLY_PAIRS
 };
 
 struct nlf_mesh_s {

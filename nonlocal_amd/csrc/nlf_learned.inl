@@ -46,6 +46,15 @@ struct LearnedGraph {
     DevBuf<i32> ell_col;
     DevBuf<float4> ell_f;
     DevBuf<float> v;                            // 64 w_out-contracted weights of the last layer
+    i64 band = 0;                               // max |col - row| of the local pattern
+    bool m_ready = false;                       // compiled operator M (three-hop pattern, FP32 values)
+    i64 m_nnz = 0;
+    DevBuf<i64> m_outer;
+    DevBuf<i32> m_cols;
+    DevBuf<int16_t> m_off16;
+    DevBuf<float> m_val;
+    double compile_seconds = 0.0;
+    bool network_apply = false;                 // run the network kernels even when M is compiled
     DevBuf<float> g0, g1, gdir, gm;             // backward work
     DevBuf<double> part, sigma, zbuf, az;
     DevBuf<float> gpart, grad;
@@ -899,6 +908,279 @@ __global__ __launch_bounds__(SOA_THREADS) void k_soa_tlast(uint32_t n, const i32
     z[u] = Z(su * su * r[u] + su * (*sigma) * 0.5 * (double(y[u]) + double(t)));
 }
 
+// ---- the learned operator as an explicit sparse matrix --------------------------------------------
+// N is linear, so M = S^2 + S (Y + Y^T)/2 S is a fixed matrix on the three-hop pattern P3 of the local
+// graph.  Row u of Y is the transposed network applied to a unit impulse at u (Y^T e_u), and the
+// layer structure collapses it to small sums (all vectors have 32 channels):
+//   g2(v) = [v = u] v_s + sum_f f_uv[f] nu_f,          nu_f = W_e2[:, f] (.) v_m,  v_s/m = W_s2/m2^T w_out
+//   d1(v) = W_s1^T g2(v),  t1(v) = W_m1^T g2(v)        (linear in the 5 coefficients: 10 basis vectors)
+//   g1(w) = [w in P1] d1(w) + sum_{v in row(w), v in P1} (W_e1 f_wv) (.) t1(v)            (w in P2)
+//   Y_ux  = [x in P2] alpha . g1(x) + sum_{w in row(x), w in P2} f_xw . (Lambda^T g1(w))   (x in P3)
+// with alpha = W_s0 a_in and Lambda[c][f] = sum_k W_m0[c][k] a_in[k] W_e0[k][f].  Per node this is a
+// few thousand flops instead of the network's six gather passes per application; the apply becomes
+// one SpMV (FP32 values, FP64 vectors).  Node sets are built in LDS bitmaps over the id window
+// [u - 3b, u + 3b] (b = bandwidth of the local pattern); a row that does not fit leaves the
+// operator uncompiled and the network kernels are used instead.
+constexpr int CP_THREADS = 64;
+constexpr int CP_P1 = 32, CP_P2 = 160, CP_P3 = 512;     // set capacities
+constexpr int CP_WORDS = 2048;                          // bitmap words: id window of 65536
+
+struct CompileBasis {
+    float d1[5][MP_H], t1[5][MP_H];     // basis of d1 / t1: index 0 = the impulse term, 1 + f = feature f
+    float alpha[MP_H];
+    float lam[MP_H][MP_F];
+};
+
+// one block of 32 threads
+__global__ void k_lf_basis(const float* __restrict__ w, CompileBasis* __restrict__ B) {
+    const int c = threadIdx.x;
+    const float* L0 = w + P_LAYER0;
+    const float* L1 = w + P_LAYER0 + P_LAYER_SIZE;
+    const float* L2 = w + P_LAYER0 + 2 * P_LAYER_SIZE;
+    const float* wout = w + P_WOUT;
+    const float* ain = w + P_AIN;
+    __shared__ float vs[MP_H], vm[MP_H], nu[MP_F][MP_H];
+    float a = 0.0f, b = 0.0f;
+    for (int o = 0; o < MP_H; ++o) { a += L2[P_WS + o * MP_H + c] * wout[o]; b += L2[P_WM + o * MP_H + c] * wout[o]; }
+    vs[c] = a;
+    vm[c] = b;
+    __syncthreads();
+    for (int f = 0; f < MP_F; ++f) nu[f][c] = L2[P_WE + c * MP_F + f] * vm[c];
+    __syncthreads();
+    for (int m = 0; m < 5; ++m) {
+        const float* g = m == 0 ? vs : nu[m - 1];
+        float d = 0.0f, t = 0.0f;
+        for (int k = 0; k < MP_H; ++k) { d += L1[P_WS + k * MP_H + c] * g[k]; t += L1[P_WM + k * MP_H + c] * g[k]; }
+        B->d1[m][c] = d;
+        B->t1[m][c] = t;
+    }
+    float al = 0.0f;
+    for (int k = 0; k < MP_H; ++k) al += L0[P_WS + c * MP_H + k] * ain[k];
+    B->alpha[c] = al;
+    for (int f = 0; f < MP_F; ++f) {
+        float l = 0.0f;
+        for (int k = 0; k < MP_H; ++k) l += L0[P_WM + c * MP_H + k] * ain[k] * L0[P_WE + k * MP_F + f];
+        B->lam[c][f] = l;
+    }
+}
+
+__device__ __forceinline__ int find_sorted(const i32* a, int n, i32 key) {
+    int lo = 0, hi = n;
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (a[mid] < key) lo = mid + 1; else hi = mid;
+    }
+    return (lo < n && a[lo] == key) ? lo : -1;
+}
+
+// Union of the rows of `src` (ns nodes) into a sorted list `dst` through the bitmap of `words` words
+// (a multiple of CP_THREADS) starting at id `base`; returns its size, or -1 when it exceeds cap or
+// the id window.
+__device__ int union_rows(const i64* __restrict__ outer, const i32* __restrict__ cols, const i32* src, int ns, i64 base,
+                          unsigned* bm, int words, i32* dst, int cap) {
+    for (int i = threadIdx.x; i < words; i += blockDim.x) bm[i] = 0u;
+    __shared__ int bad;
+    if (threadIdx.x == 0) bad = 0;
+    __syncthreads();
+    for (int k = 0; k < ns; ++k) {
+        const i64 v = src[k];
+        for (i64 p = outer[v] + threadIdx.x; p < outer[v + 1]; p += blockDim.x) {
+            const i64 off = i64(cols[p]) - base;
+            if (off < 0 || off >= i64(words) * 32) bad = 1;
+            else atomicOr(&bm[off >> 5], 1u << (off & 31));
+        }
+    }
+    __syncthreads();
+    if (bad) return -1;
+    // ordered compaction: each thread owns a contiguous run of words
+    const int PER = words / CP_THREADS;          // words is a multiple of CP_THREADS
+    __shared__ int cnt[CP_THREADS];
+    int c = 0;
+    for (int i = 0; i < PER; ++i) c += __popc(bm[threadIdx.x * PER + i]);
+    cnt[threadIdx.x] = c;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int acc = 0;
+        for (int t = 0; t < CP_THREADS; ++t) { const int v = cnt[t]; cnt[t] = acc; acc += v; }
+        bad = acc;
+    }
+    __syncthreads();
+    const int total = bad;
+    if (total > cap) return -1;
+    int pos = cnt[threadIdx.x];
+    for (int i = 0; i < PER; ++i) {
+        unsigned word = bm[threadIdx.x * PER + i];
+        while (word) {
+            const int b = __ffs(word) - 1;
+            word &= word - 1;
+            dst[pos++] = i32(base + i64(threadIdx.x * PER + i) * 32 + b);
+        }
+    }
+    __syncthreads();
+    return total;
+}
+
+// PASS 0: |P3(u)| into cnt[u] (-1 on overflow).  PASS 1: columns and Y row values of u.
+template<int PASS>
+__global__ __launch_bounds__(CP_THREADS) void k_lf_compile(i64 n, const i64* __restrict__ outer, const i32* __restrict__ cols,
+                                                           const float4* __restrict__ feat, i64 band, int words,
+                                                           const CompileBasis* __restrict__ B,
+                                                           const float* __restrict__ w, i64* __restrict__ cnt,
+                                                           const i64* __restrict__ m_outer, i32* __restrict__ m_cols,
+                                                           float* __restrict__ yrow, int* __restrict__ overflow) {
+    __shared__ unsigned bm[CP_WORDS];
+    __shared__ i32 p1[CP_P1], p2[CP_P2], p3[CP_P3];
+    __shared__ float cf[CP_P1][5];
+    __shared__ float aw[CP_P2], kw[CP_P2][MP_F];
+    const i64 u = blockIdx.x;
+    if (u >= n) return;
+    const i64 r0 = outer[u];
+    const int n1 = int(outer[u + 1] - r0);
+    if (n1 > CP_P1) {
+        if (threadIdx.x == 0) { if (PASS == 0) cnt[u] = 0; *overflow = 1; }
+        return;
+    }
+    for (int k = threadIdx.x; k < n1; k += blockDim.x) {
+        p1[k] = cols[r0 + k];
+        const float4 f = feat[r0 + k];
+        cf[k][0] = cols[r0 + k] == i32(u) ? 1.0f : 0.0f;
+        cf[k][1] = f.x; cf[k][2] = f.y; cf[k][3] = f.z; cf[k][4] = f.w;
+    }
+    __syncthreads();
+    const i64 base2 = std::max<i64>(u - 2 * band, 0), base3 = std::max<i64>(u - 3 * band, 0);
+    const int n2 = union_rows(outer, cols, p1, n1, base2, bm, words, p2, CP_P2);
+    if (n2 < 0) {
+        if (threadIdx.x == 0) { if (PASS == 0) cnt[u] = 0; *overflow = 1; }
+        return;
+    }
+    const int n3 = union_rows(outer, cols, p2, n2, base3, bm, words, p3, CP_P3);
+    if (n3 < 0) {
+        if (threadIdx.x == 0) { if (PASS == 0) cnt[u] = 0; *overflow = 1; }
+        return;
+    }
+    if (PASS == 0) {
+        if (threadIdx.x == 0) cnt[u] = n3;
+        return;
+    }
+    const float* L1 = w + P_LAYER0 + P_LAYER_SIZE;
+    // g1(w) for w in P2: lanes 0..31 channels of one w, lanes 32..63 the next w
+    const int c = threadIdx.x & 31, half = threadIdx.x >> 5;
+    for (int i0 = 0; i0 < n2; i0 += 2) {
+        const int i = i0 + half;
+        float g = 0.0f;
+        if (i < n2) {
+            const i32 wn = p2[i];
+            const int j = find_sorted(p1, n1, wn);
+            if (j >= 0)
+                for (int m = 0; m < 5; ++m) g += cf[j][m] * B->d1[m][c];
+            const float e0 = L1[P_WE + c * MP_F + 0], e1 = L1[P_WE + c * MP_F + 1], e2 = L1[P_WE + c * MP_F + 2],
+                        e3 = L1[P_WE + c * MP_F + 3];
+            for (i64 p = outer[wn]; p < outer[wn + 1]; ++p) {
+                const int jv = find_sorted(p1, n1, cols[p]);
+                if (jv < 0) continue;
+                const float4 f = feat[p];
+                float t = 0.0f;
+                for (int m = 0; m < 5; ++m) t += cf[jv][m] * B->t1[m][c];
+                g += (e0 * f.x + e1 * f.y + e2 * f.z + e3 * f.w) * t;
+            }
+        }
+        // alpha . g and Lambda^T g over the 32 channels of each half
+        float a = B->alpha[c] * g;
+        float kf[MP_F];
+        for (int f = 0; f < MP_F; ++f) kf[f] = B->lam[c][f] * g;
+        for (int o = 16; o > 0; o >>= 1) {
+            a += __shfl_down(a, o, 32);
+            for (int f = 0; f < MP_F; ++f) kf[f] += __shfl_down(kf[f], o, 32);
+        }
+        if (c == 0 && i < n2) {
+            aw[i] = a;
+            for (int f = 0; f < MP_F; ++f) kw[i][f] = kf[f];
+        }
+    }
+    __syncthreads();
+    const i64 o = m_outer[u];
+    for (int k = threadIdx.x; k < n3; k += blockDim.x) {
+        const i32 x = p3[k];
+        const int ix = find_sorted(p2, n2, x);
+        float y = ix >= 0 ? aw[ix] : 0.0f;
+        for (i64 p = outer[x]; p < outer[x + 1]; ++p) {
+            const int iw = find_sorted(p2, n2, cols[p]);
+            if (iw < 0) continue;
+            const float4 f = feat[p];
+            y += f.x * kw[iw][0] + f.y * kw[iw][1] + f.z * kw[iw][2] + f.w * kw[iw][3];
+        }
+        m_cols[o + k] = x;
+        yrow[o + k] = y;
+    }
+}
+
+// M_ux = [u = x] s_u^2 + s_u s_x (Y_ux + Y_xu) / 2 (Y_xu found in row x by binary search)
+__global__ void k_lf_symmetrize(i64 n, const i64* __restrict__ m_outer, const i32* __restrict__ m_cols,
+                                const float* __restrict__ yrow, const float* __restrict__ s, float* __restrict__ mval) {
+    const i64 u = i64(blockIdx.x) * (blockDim.x / 64) + (threadIdx.x >> 6);
+    if (u >= n) return;
+    const double su = s[u];
+    for (i64 p = m_outer[u] + (threadIdx.x & 63); p < m_outer[u + 1]; p += 64) {
+        const i32 x = m_cols[p];
+        const i64 b = m_outer[x], e = m_outer[x + 1];
+        const int q = find_sorted(m_cols + b, int(e - b), i32(u));
+        const double yxu = q >= 0 ? double(yrow[b + q]) : 0.0;
+        double v = su * double(s[x]) * 0.5 * (double(yrow[p]) + yxu);
+        if (x == i32(u)) v += su * su;
+        mval[p] = float(v);
+    }
+}
+
+__global__ void k_lf_off16(i64 n, const i64* __restrict__ outer, const i32* __restrict__ cols, int16_t* __restrict__ off) {
+    const i64 r = i64(blockIdx.x) * (blockDim.x / 64) + (threadIdx.x >> 6);
+    if (r >= n) return;
+    for (i64 p = outer[r] + (threadIdx.x & 63); p < outer[r + 1]; p += 64) off[p] = int16_t(i64(cols[p]) - r);
+}
+
+// Builds M for the weights w; leaves m_ready false when a node's sets exceed the LDS capacities.
+void learned_compile(LearnedGraph& g, const float* w, hipStream_t s) {
+    const auto t0 = std::chrono::steady_clock::now();
+    g.m_ready = false;
+    const i64 n = g.n;
+    const i64 need_words = (6 * g.band + 1 + 31) / 32 + 1;
+    if (need_words > CP_WORDS) return;
+    const int words = int((need_words + CP_THREADS - 1) / CP_THREADS * CP_THREADS);
+    DevBuf<CompileBasis> basis{1};
+    DevBuf<i64> cnt{size_t(n + 1)};
+    DevBuf<int> overflow{1};
+    hipLaunchKernelGGL(k_lf_basis, dim3(1), dim3(MP_H), 0, s, w, basis.p);
+    NLF_HIP(hipMemsetAsync(overflow.p, 0, sizeof(int), s));
+    NLF_HIP(hipMemsetAsync(cnt.p, 0, size_t(n + 1) * sizeof(i64), s));
+    const float4* feat = reinterpret_cast<const float4*>(g.feat.p);
+    hipLaunchKernelGGL(k_lf_compile<0>, dim3(unsigned(n)), dim3(CP_THREADS), 0, s, n, g.outer.p, g.cols.p, feat, g.band,
+                       words, basis.p, w, cnt.p, (const i64*)nullptr, (i32*)nullptr, (float*)nullptr, overflow.p);
+    NLF_HIP(hipGetLastError());
+    int ovf = 0;
+    download(&ovf, overflow.p, 1, s);
+    if (ovf) return;
+    exclusive_scan_i64(cnt.p, n, s);
+    i64 nnz = 0;
+    download(&nnz, cnt.p + n, 1, s);
+    g.m_outer = std::move(cnt);
+    g.m_nnz = nnz;
+    g.m_cols.alloc(size_t(std::max<i64>(nnz, 1)));
+    g.m_val.alloc(size_t(std::max<i64>(nnz, 1)));
+    DevBuf<float> yrow{size_t(std::max<i64>(nnz, 1))};
+    hipLaunchKernelGGL(k_lf_compile<1>, dim3(unsigned(n)), dim3(CP_THREADS), 0, s, n, g.outer.p, g.cols.p, feat, g.band,
+                       words, basis.p, w, (i64*)nullptr, g.m_outer.p, g.m_cols.p, yrow.p, overflow.p);
+    hipLaunchKernelGGL(k_lf_symmetrize, dim3(blocks_for(n, 4)), dim3(256), 0, s, n, g.m_outer.p, g.m_cols.p, yrow.p, g.s.p,
+                       g.m_val.p);
+    g.m_off16.release();
+    if (3 * g.band < 32768) {
+        g.m_off16.alloc(size_t(std::max<i64>(nnz, 1)));
+        hipLaunchKernelGGL(k_lf_off16, dim3(blocks_for(n, 4)), dim3(256), 0, s, n, g.m_outer.p, g.m_cols.p, g.m_off16.p);
+    }
+    NLF_HIP(hipGetLastError());
+    NLF_HIP(hipStreamSynchronize(s));
+    g.m_ready = true;
+    g.compile_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+}
+
 // ELL width for a longest row D: the smallest instantiated DM >= D
 int ell_width(i64 D) { return D <= 9 ? 9 : D <= 16 ? 16 : D <= 24 ? 24 : 32; }
 
@@ -930,6 +1212,12 @@ void learned_apply_ell(LearnedGraph& g, const float* w, const double* r, Z* z, h
 // ping-pong), y; transposed layers (d in place, q through A then B) ending in the output.
 template<class Z>
 void learned_apply(LearnedGraph& g, const float* w, const double* r, Z* z, hipStream_t s) {
+    if (g.m_ready && !g.network_apply) {
+        const int lpr = pick_lpr(double(g.m_nnz) / double(std::max<i64>(g.n, 1)));
+        launch_spmv<float, double, double, false>(lpr, g.n, g.m_outer.p, g.m_cols.p, g.m_off16.p, g.m_val.p, r, z, nullptr,
+                                                  nullptr, s);
+        return;
+    }
     learned_scaled_input(g, w, r, s);
     switch (g.ell_d) {
         case 9: learned_apply_ell<9, Z>(g, w, r, z, s); break;
@@ -1199,6 +1487,9 @@ int nlf_learned_create(nlf_matrix A, nlf_matrix A_local, const nlf_model* models
         std::vector<i32> lc(size_t(g->nnz));
         download(lc.data(), A_local->cols.p, lc.size(), s);
         upload(g->cols, lc.data(), lc.size(), s);
+        for (i64 u = 0; u < g->n; ++u)
+            if (lo[u + 1] > lo[u])
+                g->band = std::max(g->band, std::max(u - i64(lc[size_t(lo[u])]), i64(lc[size_t(lo[u + 1] - 1)]) - u));
         g->s.alloc(size_t(g->n));
         g->feat.alloc(size_t(4 * g->nnz));
         g->w.alloc(NPARAM);
@@ -1239,8 +1530,26 @@ int nlf_learned_set_weights(nlf_precond p, const float* w) {
     return guarded(p->ctx, [&] {
         LearnedGraph& g = *learned_of(p);
         NLF_HIP(hipMemcpyAsync(g.w.p, w, NPARAM * sizeof(float), hipMemcpyHostToDevice, p->ctx->stream));
+        learned_compile(g, g.w.p, p->ctx->stream);
         NLF_HIP(hipStreamSynchronize(p->ctx->stream));
     });
+}
+
+// Whether the operator is compiled to its explicit sparse form, its nonzeros and compile time.
+int nlf_learned_info(nlf_precond p, int* compiled, int64_t* nnz, double* compile_seconds) {
+    if (!p) return NLF_ERR_ARGUMENT;
+    return guarded(p->ctx, [&] {
+        LearnedGraph& g = *learned_of(p);
+        if (compiled) *compiled = g.m_ready ? 1 : 0;
+        if (nnz) *nnz = g.m_ready ? g.m_nnz : 0;
+        if (compile_seconds) *compile_seconds = g.compile_seconds;
+    });
+}
+
+// Application path: 0 = the compiled sparse operator when available (default), 1 = the network kernels.
+int nlf_learned_set_apply(nlf_precond p, int network) {
+    if (!p) return NLF_ERR_ARGUMENT;
+    return guarded(p->ctx, [&] { learned_of(p)->network_apply = network != 0; });
 }
 
 int nlf_learned_get_weights(nlf_precond p, float* w) {

@@ -204,6 +204,15 @@ int nlf_ctx_destroy(nlf_ctx ctx) {
 
 const char* nlf_last_error(nlf_ctx ctx) { return ctx ? ctx->error.c_str() : "null context"; }
 
+// Context options (no reference counterpart): NLF_OPT_ASSEMBLY_PAIRS 1 = MFMA element-pair
+// contractions where implemented (thermal Q4, default), 0 = node-centric FP64 contraction.
+int nlf_ctx_set_option(nlf_ctx ctx, int option, int value) {
+    if (!ctx) return NLF_ERR_ARGUMENT;
+    if (option != NLF_OPT_ASSEMBLY_PAIRS) return NLF_ERR_ARGUMENT;
+    ctx->assembly_pairs = value != 0;
+    return NLF_OK;
+}
+
 int nlf_ctx_synchronize(nlf_ctx ctx) {
     return guarded(ctx, [&] { NLF_HIP(hipStreamSynchronize(ctx->stream)); });
 }

@@ -88,6 +88,16 @@ class LearnedPreconditioner:
         w = np.ascontiguousarray(w, dtype=np.float32)
         self.ctx.check(load().nlf_learned_set_weights(self.h, w.ctypes.data_as(_f32p)))
 
+    def info(self):
+        """(compiled to an explicit sparse operator, its nonzeros, compile seconds)."""
+        c, nnz, sec = C.c_int(), C.c_int64(), C.c_double()
+        self.ctx.check(load().nlf_learned_info(self.h, C.byref(c), C.byref(nnz), C.byref(sec)))
+        return bool(c.value), nnz.value, sec.value
+
+    def set_apply(self, network: bool):
+        """Apply through the network kernels (True) or the compiled sparse operator (False, default)."""
+        self.ctx.check(load().nlf_learned_set_apply(self.h, int(network)))
+
     def get_weights(self):
         w = np.zeros(num_params(), dtype=np.float32)
         self.ctx.check(load().nlf_learned_get_weights(self.h, w.ctypes.data_as(_f32p)))

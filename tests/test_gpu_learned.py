@@ -77,6 +77,23 @@ def test_symmetric_apply_matches_oracle(small, gpu_ctx):
     assert abs(v @ Mu - u @ Mv) <= 1e-5 * abs(v @ Mu)
 
 
+def test_compiled_operator_matches_network_kernels(small, gpu_ctx):
+    """The compiled sparse M (three-hop pattern) and the six network passes apply the same operator."""
+    tp, Kf = small
+    tp.learned.set_weights(_gpu_weights(9))
+    compiled, nnz, sec = tp.learned.info()
+    print(f"[learned] compiled {compiled}, nnz {nnz}, {sec * 1e3:.2f} ms", flush=True)
+    assert compiled and nnz > 0
+    r = torch.from_numpy(np.random.default_rng(10).standard_normal(Kf.shape[0])).to(gpu_ctx.device)
+    zm = tp.learned.solve(r).cpu().numpy()
+    tp.learned.set_apply(True)
+    zn = tp.learned.solve(r).cpu().numpy()
+    tp.learned.set_apply(False)
+    print(f"[learned] compiled vs network: {rel_l2(zm, zn):.2e}", flush=True)
+    assert rel_l2(zm, zn) < 1e-5
+    assert not np.array_equal(zm, zn), "the two paths round differently; equal outputs mean one path ran twice"
+
+
 def test_zero_output_is_jacobi(small, gpu_ctx):
     tp, Kf = small
     w = _gpu_weights(5)

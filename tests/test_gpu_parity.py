@@ -103,21 +103,24 @@ def test_pattern_and_values(gpu_ctx, problem, part):
 
 
 @pytest.mark.parametrize("problem", [PROBLEMS[0], PROBLEMS[3]], ids=[IDS[0], IDS[3]])
-def test_mfma_pair_path_matches_node_path(gpu_ctx, problem, monkeypatch):
+def test_mfma_pair_path_matches_node_path(gpu_ctx, problem):
     """Thermal Q4: the MFMA element-pair contraction (default) against the node-centric FP64 one."""
     print(f"[mfma] {problem.name}", flush=True)
     g = _gpu_mesh(problem, gpu_ctx)
     api.find_neighbours(g, H.radii(problem.models))
     inner = O.OracleMesh(problem.mesh).boundary(problem.bcs, problem.dof)[2]
     a = api._assemble(g, 0, problem.models, problem.materials, inner).export(2)
-    monkeypatch.setenv("NLF_ASSEMBLY", "node")
-    b = api._assemble(g, 0, problem.models, problem.materials, inner).export(2)
-    monkeypatch.delenv("NLF_ASSEMBLY")
+    gpu_ctx.set_option(A.NLF_OPT_ASSEMBLY_PAIRS, 0)
+    try:
+        b = api._assemble(g, 0, problem.models, problem.materials, inner).export(2)
+    finally:
+        gpu_ctx.set_option(A.NLF_OPT_ASSEMBLY_PAIRS, 1)
     c = api._assemble(g, 0, problem.models, problem.materials, inner).export(2)
     assert np.array_equal(a.inner, b.inner)
     err = np.abs(a.val - b.val).max() / np.abs(b.val).max()
     print(f"  max rel difference {err:.3e}", flush=True)
     assert err <= 1e-13
+    assert not np.array_equal(a.val, b.val), "the two contractions round differently; equal values mean one ran twice"
     assert np.array_equal(a.val, c.val), "the MFMA path must be bitwise reproducible"
 
 
