@@ -21,6 +21,8 @@ import os
 import sys
 import time
 
+import numpy as np
+
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
@@ -38,12 +40,15 @@ def parse():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--n", type=int, default=1000, help="mesh side (1000 = the benchmark workload)")
     ap.add_argument("--tol", type=float, default=1e-10)
-    ap.add_argument("--cpu-sample-n", type=int, default=160)
+    ap.add_argument("--cpu-sample-n", type=int, default=480)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--methods", default="fp64_jacobi,mixed_jacobi,learned_fcg")
-    ap.add_argument("--workload", default="composite1000", choices=["composite1000", "batch64"],
+    ap.add_argument("--methods", default="fp64_jacobi,mixed_jacobi,learned_fcg,learned_fcg_split")
+    ap.add_argument("--workload", default="composite1000", choices=["composite1000", "batch64", "precision", "spmv"],
                     help="composite1000: the 1000x1000 composite (the metric's workload); batch64: 64 uneven "
-                         "instances solved together vs one after another")
+                         "instances solved together vs one after another; precision: FP32 assembly and "
+                         "mixed-precision errors against the FP64 path on every workload; spmv: the FP64 "
+                         "SpMV of the composite alone, plus a device copy of known size (HBM counter "
+                         "calibration)")
     return ap.parse_args()
 
 
@@ -148,6 +153,143 @@ def bench_batch(args):
     print(json.dumps(line), flush=True)
 
 
+def _fp32_entry_report(K, t_fp32):
+    d = K.compare_fp32()
+    return dict(fp32_assembly_s=round(t_fp32, 4), entry_max_err_rel_to_max_entry=d["max_abs"] / d["max_entry"],
+                entry_frobenius_rel_err=d["l2"] / d["norm"])
+
+
+def bench_precision(args):
+    """FP32 assembly (FP32 element-pair contractions and accumulation) and mixed-precision solves
+    against the FP64 GPU path on the full-size workloads: 40x40 Q4, 200x200 jittered Q8,
+    300x300 elastic Q4, 64 batched instances and the 1000x1000 composite.  The FP64 GPU path is the
+    reference here (it matches the oracle to 1e-12 in the entries and 1e-10 in the solution on the
+    sizes the oracle solves directly)."""
+    import torch
+    from nonlocal_amd import _abi as A
+    from nonlocal_amd import api, batch, configs
+    ctx = api.default_context()
+    out = {}
+
+    def solve(problem, mesh, precond, mode, tol, assembly32=False):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        sol = api._solve_driver(mesh, problem.physics, problem.models, problem.materials, problem.bcs, precond, tol,
+                                100000, mode, assembly32=assembly32)
+        torch.cuda.synchronize()
+        return sol, time.perf_counter() - t0
+
+    cases = [("thermal_q4_40", configs.cfg0(), A.NLF_PRECOND_ILLT),
+             ("thermal_q8_200_jittered", configs.cfg1(), A.NLF_PRECOND_ILLT),
+             ("elastic_q4_300", configs.cfg2(), A.NLF_PRECOND_ILLT),
+             ("composite_q4_%d" % args.n, configs.cfg4(args.n), A.NLF_PRECOND_LEARNED)]
+    for name, problem, precond in cases:
+        log(f"[precision] {name}: {problem.mesh.n_nodes} nodes")
+        mesh = api.mesh_2d(problem.mesh, ctx)
+        tol = args.tol
+        ref, t64 = solve(problem, mesh, A.NLF_PRECOND_ILLT, A.NLF_MODE_FP64, tol)
+        x64 = ref.x.clone()
+        K = ref.matrix
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        K.assemble_fp32()
+        torch.cuda.synchronize()
+        rec = _fp32_entry_report(K, time.perf_counter() - t0)
+        rec.update(nodes=int(problem.mesh.n_nodes), nnz_full=int(K.info()["nnz_full"]),
+                   fp64_values_s=round(K.timings()[1], 4), fp64_illt_s=round(t64, 4),
+                   fp64_illt_iterations=int(ref.iterations))
+        del ref, K
+        runs = [("mixed_fp32_assembly", precond, A.NLF_MODE_MIXED, True)]
+        if precond == A.NLF_PRECOND_LEARNED:
+            runs.append(("learned_fcg_fp64_spmv", precond, A.NLF_MODE_FP64, False))
+        for label, pc, mode, a32 in runs:
+            sol, t = solve(problem, mesh, pc, mode, tol, assembly32=a32)
+            err = float(torch.linalg.norm(sol.x - x64) / torch.linalg.norm(x64))
+            rec[label] = dict(seconds=round(t, 4), iterations=int(sol.iterations), true_residual=sol.true_residual,
+                              rel_err_vs_fp64=err, precond=int(pc), phases={k: round(v, 4) for k, v in sol.timings.items()})
+            log(f"[precision] {name} {label}: {t:.3f} s, it {sol.iterations}, rel err {err:.2e}")
+            del sol
+        log(f"[precision] {name}: {rec}")
+        out[name] = rec
+        del mesh
+    # 64 batched instances: one FP32 assembly of the block-diagonal batch, lockstep mixed solve
+    problems = configs.cfg3_instances(64, seed=3)
+    log("[precision] batch64")
+    seq = []
+    for p in problems:
+        mesh = api.mesh_2d(p.mesh, ctx)
+        sol = api.stationary_heat_equation_solver_2d(mesh, p.models, p.materials, p.bcs, precond=A.NLF_PRECOND_ILLT,
+                                                     tol=args.tol, max_it=100000)
+        seq.append(sol.x.clone())
+        del mesh, sol
+    bs = batch.BatchSolver(problems, ctx)
+    bs.autotune(None)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    bs.K.assemble_fp32()
+    torch.cuda.synchronize()
+    rec = _fp32_entry_report(bs.K, time.perf_counter() - t0)
+    x, its, res, st = bs.solve(tol=args.tol, max_it=100000, mode=A.NLF_MODE_MIXED, use_graph=True, assembly32=True)
+    errs = [float(torch.linalg.norm(xi - xs) / torch.linalg.norm(xs)) for xi, xs in zip(bs.split(x), seq)]
+    rec.update(nodes=int(sum(p.mesh.n_nodes for p in problems)), nnz_full=int(bs.K.info()["nnz_full"]),
+               mixed_fp32_assembly=dict(max_iterations=int(its.max()), max_true_residual=float(res.max()),
+                                        max_rel_err_vs_fp64=max(errs)))
+    out["batch64"] = rec
+    log(f"[precision] batch64: {rec}")
+    line = {"metric": "FP32 assembly entry error and mixed-precision solution error vs the FP64 path",
+            "unit": "relative", "n_gpus": 1, "dtype": "f32 assembly / f32 inner, f64 refinement",
+            "data": "synthetic", "tol": args.tol, "configs": out}
+    print(json.dumps(line), flush=True)
+
+
+def bench_spmv(args):
+    """The composite's FP64 SpMV alone (--steps launches after --warmup), then one device-to-device
+    copy of 2^28 doubles (2 GiB read, 2 GiB written): a kernel of known traffic against which the
+    HBM counters (FETCH_SIZE, WRITE_SIZE) of the SpMV are scaled."""
+    import torch
+    from nonlocal_amd import _abi as A
+    from nonlocal_amd import api, configs
+    ctx = api.default_context()
+    problem = configs.cfg4(args.n)
+    mesh = api.mesh_2d(problem.mesh, ctx)
+    api.find_neighbours(mesh, api._radii(problem.models))
+    inner = np.ones(problem.mesh.n_nodes, dtype=np.uint8)
+    f = np.zeros(problem.mesh.n_nodes)
+    xD = np.zeros(problem.mesh.n_nodes)
+    arr = A.array_of(A.Boundary, problem.bcs)
+    ctx.check(api.load().nlf_boundary_data(mesh.h, 1, arr, api._ptr(f, api._f64p), api._ptr(xD, api._f64p),
+                                           api._ptr(inner, api._u8p)))
+    K = api._assemble(mesh, 0, problem.models, problem.materials, inner)
+    info = K.info()
+    x = torch.randn(info["rows"], dtype=torch.float64, device=ctx.device)
+    y = torch.empty_like(x)
+    for _ in range(args.warmup):
+        K.spmv(x, y)
+    ctx.synchronize()
+    log(f"[spmv] {args.steps} launches, nnz_full {info['nnz_full']}")
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        K.spmv(x, y)
+    ctx.synchronize()
+    ms = (time.perf_counter() - t0) / args.steps * 1e3
+    a = torch.empty(1 << 28, dtype=torch.float64, device=ctx.device)
+    b = torch.empty_like(a)
+    a.fill_(1.0)
+    torch.cuda.synchronize()
+    b.copy_(a)
+    torch.cuda.synchronize()
+    rows = info["rows"]
+    algo = info["nnz_upper"] * 12 + rows * 24
+    line = {"metric": "FP64 SpMV of the composite, ms per launch (host clock over back-to-back launches)",
+            "value": round(ms, 4), "unit": "ms", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+            "higher_is_better": False, "dtype": "f64", "data": "synthetic",
+            "config": {"workload": f"composite {args.n}x{args.n} Q4, full CSR SpMV", "rows": rows,
+                       "nnz_full": int(info["nnz_full"]), "nnz_upper": int(info["nnz_upper"])},
+            "algorithmic_bytes": int(algo), "algorithmic_GBps": round(algo / ms * 1e-6, 1),
+            "calibration_copy_bytes": {"read": int(a.numel() * 8), "written": int(b.numel() * 8)}}
+    print(json.dumps(line), flush=True)
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -156,6 +298,12 @@ def main():
         sys.exit(2)
     if args.workload == "batch64":
         bench_batch(args)
+        return
+    if args.workload == "precision":
+        bench_precision(args)
+        return
+    if args.workload == "spmv":
+        bench_spmv(args)
         return
     import torch
     from nonlocal_amd import _abi as A
@@ -174,6 +322,8 @@ def main():
         "mixed_jacobi": (A.NLF_PRECOND_JACOBI, A.NLF_MODE_MIXED),
         "fp64_jacobi": (A.NLF_PRECOND_JACOBI, A.NLF_MODE_FP64),
         "learned_fcg": (A.NLF_PRECOND_LEARNED, A.NLF_MODE_FP64),
+        "learned_fcg_split": (A.NLF_PRECOND_LEARNED, A.NLF_MODE_SPLIT),
+        "split_jacobi": (A.NLF_PRECOND_JACOBI, A.NLF_MODE_SPLIT),
     }
     # plain FP64 ILLT-CG: the baseline the metric compares against; also the FP64 reference solution
     log("[bench] plain FP64 ILLT-CG")
@@ -214,7 +364,7 @@ def main():
     info = last.matrix.info()
     st = last.stats
     rows = info["rows"]
-    sv = 4 if mode == A.NLF_MODE_MIXED else 8      # matrix value bytes (vectors stay FP64)
+    sv = {A.NLF_MODE_MIXED: 4, A.NLF_MODE_SPLIT: 6}.get(mode, 8)      # matrix value bytes (vectors stay FP64)
     sx = 8
     algo_bytes = info["nnz_upper"] * (sv + 4) + rows * (8 + 2 * sx)     # upper-symmetric CSR formula
     achieved = algo_bytes / (st.spmv_ms * 1e-3) / 1e9 if st.spmv_ms > 0 else 0.0
@@ -235,7 +385,7 @@ def main():
         "higher_is_better": False,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64+f32" if (mode == A.NLF_MODE_MIXED or pc == A.NLF_PRECOND_LEARNED) else "f64",
+        "dtype": "f64+f32" if (mode != A.NLF_MODE_FP64 or pc == A.NLF_PRECOND_LEARNED) else "f64",
         "data": "synthetic",
         "config": {"workload": f"composite 2D stationary heat, {args.n}x{args.n} Q4, 16 inclusions, p1=0.5, "
                                f"bell r=0.01, solve to {args.tol:g} true residual",
@@ -251,7 +401,7 @@ def main():
         "precond_apply_ms": round(st.precond_ms, 4),
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                     "kernel": f"k_spmv ({'fp32' if sv == 4 else 'fp64'}, full CSR)",
+                     "kernel": f"k_spmv ({ {4: 'fp32', 6: 'fp32+bf16 split'}.get(sv, 'fp64')} values, full CSR)",
                      "avg_launch_ms": round(st.spmv_ms, 4), "samples": int(st.spmv_samples),
                      "algorithmic_bytes_per_launch": int(algo_bytes),
                      "moved_bytes_per_launch": int(st.spmv_bytes), "moved_GBps": round(moved_gbs, 1)},

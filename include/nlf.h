@@ -58,6 +58,8 @@ extern "C" {
 
 #define NLF_MODE_FP64  0             /* plain FP64 CG */
 #define NLF_MODE_MIXED 1             /* FP32 inner CG + FP64 iterative refinement */
+#define NLF_MODE_SPLIT 2             /* values read as FP32 + 16-bit remainder (40 significand bits),
+                                        FP64 vectors, FP64 true residual with reliable updates */
 
 /* Nonlocal model of one element group: model_parameters (equation_parameters.hpp:18-24) plus the
  * influence description read by read_influence_2d (config/read_model.hpp:144-165). */
@@ -119,6 +121,7 @@ int         nlf_ctx_destroy(nlf_ctx ctx);
 const char* nlf_last_error(nlf_ctx ctx);
 int         nlf_ctx_synchronize(nlf_ctx ctx);
 #define NLF_OPT_ASSEMBLY_PAIRS 1     /* 1: MFMA element-pair contractions (thermal Q4), 0: node-centric */
+#define NLF_OPT_ROW_PATTERNS   2     /* 1: SpMV column offsets from a dictionary of distinct rows, 0: per nonzero */
 int         nlf_ctx_set_option(nlf_ctx ctx, int option, int value);
 void*       nlf_ctx_stream(nlf_ctx ctx);
 
@@ -150,13 +153,30 @@ int nlf_matrix_destroy(nlf_matrix a);
 int nlf_matrix_info(nlf_matrix a, int64_t* rows, int64_t* nnz_upper, int64_t* nnz_full, int64_t* nnz_bound);
 int nlf_matrix_timings(nlf_matrix a, double* pattern_seconds, double* values_seconds);
 /* finite_element_matrix::inner()/bound() export (base/finite_element_matrix.hpp:12-21), int64 indices:
- * part 0 = inner upper CSR, 1 = bound CSR, 2 = inner full storage */
+ * part 0 = inner upper CSR, 1 = bound CSR, 2 = inner full storage; 3 and 4 = parts 0 and 2 of the
+ * FP32 values (nlf_matrix_make_fp32 / nlf_matrix_assemble_fp32), widened to double */
 int nlf_matrix_export(nlf_matrix a, int part, int64_t* outer, int64_t* inner, double* values);
 int nlf_matrix_device(nlf_matrix a, const int64_t** outer, const int32_t** cols, double** values, const int64_t** diag);
+/* No reference counterpart.  Row-pattern dictionary of the SpMV (NLF_OPT_ROW_PATTERNS): number of
+ * distinct offset rows and their total length (both 0 when the per-nonzero offsets are used). */
+int nlf_matrix_pattern_info(nlf_matrix a, int64_t* groups, int64_t* entries);
 int nlf_matrix_add_pairs(nlf_matrix a, int64_t n, const int64_t* rc, const double* v);
 /* convection_condition_2d (thermal/convection_condition_2d.hpp:11-31), bcs[GB] */
 int nlf_matrix_add_convection(nlf_matrix a, const nlf_boundary* bcs);
+/* No reference counterpart (the reference assembles in T = double only).  FP32 values for the
+ * mixed-precision inner solves: make_fp32 rounds the FP64 values; assemble_fp32 repeats the value
+ * pass of nlf_assemble in FP32 (FP32 element-pair contractions, MFMA for thermal Q4, and FP32
+ * accumulation; Jacobian determinants, coordinates and the cutoff decisions stay FP64) and then
+ * the boundary additions made since.  The FP64 values are kept for the true residual. */
 int nlf_matrix_make_fp32(nlf_matrix a);
+int nlf_matrix_assemble_fp32(nlf_matrix a);
+/* No reference counterpart.  Values for NLF_MODE_SPLIT: val32 = FP32 rounding of the FP64 values
+ * and the remainder as a 16-bit integer in units of 2^-16 ulp(val32), so that the pair carries 40
+ * significand bits (relative error <= 2^-40) in 6 bytes per nonzero instead of 8. */
+int nlf_matrix_make_split(nlf_matrix a);
+/* No reference counterpart.  Distance of the FP32 values from the FP64 ones, computed on the
+ * device over the full storage: out[4] = {max |v32 - v64|, max |v64|, ||v32 - v64||_2, ||v64||_2}. */
+int nlf_matrix_compare_fp32(nlf_matrix a, double* out);
 /* inner_nodes (base/solvers_utils.hpp:31-40), boundary_condition_second_kind_2d
  * (base/boundary_condition_second_kind_2d.hpp:12-33) and the first-kind values (:17-26), host arrays */
 int nlf_boundary_data(nlf_mesh m, int dof, const nlf_boundary* bcs, double* f, double* xD, uint8_t* is_inner);

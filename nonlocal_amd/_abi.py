@@ -12,8 +12,9 @@ NLF_PART_LOCAL, NLF_PART_FULL = 0, 2
 NLF_CONDUCTIVITY_ISOTROPIC, NLF_CONDUCTIVITY_ORTHOTROPIC, NLF_CONDUCTIVITY_ANISOTROPIC = 0, 1, 2
 NLF_BC_NONE, NLF_BC_FIRST_KIND, NLF_BC_FLUX, NLF_BC_CONVECTION = 0, 1, 2, 3
 NLF_PRECOND_IDENTITY, NLF_PRECOND_JACOBI, NLF_PRECOND_ILLT, NLF_PRECOND_LEARNED = 0, 1, 2, 3
-NLF_MODE_FP64, NLF_MODE_MIXED = 0, 1
+NLF_MODE_FP64, NLF_MODE_MIXED, NLF_MODE_SPLIT = 0, 1, 2
 NLF_OPT_ASSEMBLY_PAIRS = 1
+NLF_OPT_ROW_PATTERNS = 2
 
 
 class Model(C.Structure):

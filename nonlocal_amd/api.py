@@ -73,7 +73,10 @@ class Context:
 
     def __del__(self):
         if getattr(self, "h", None):
-            _destroy("nlf_ctx_destroy", self.h)
+            try:
+                _destroy("nlf_ctx_destroy", self.h)
+            except TypeError:      # module globals already cleared at interpreter exit
+                pass
             self.h = None
 
 
@@ -173,15 +176,23 @@ class finite_element_matrix:
         self.ctx.check(load().nlf_matrix_info(self.h, C.byref(rows), C.byref(up), C.byref(full), C.byref(bound)))
         return dict(rows=rows.value, nnz_upper=up.value, nnz_full=full.value, nnz_bound=bound.value)
 
+    def pattern_info(self):
+        """(distinct offset rows, their total length) of the SpMV's row-pattern dictionary; (0, 0)
+        when the SpMV reads per-nonzero offsets."""
+        g, e = C.c_int64(), C.c_int64()
+        self.ctx.check(load().nlf_matrix_pattern_info(self.h, C.byref(g), C.byref(e)))
+        return g.value, e.value
+
     def timings(self):
         a, b = C.c_double(), C.c_double()
         self.ctx.check(load().nlf_matrix_timings(self.h, C.byref(a), C.byref(b)))
         return a.value, b.value
 
     def export(self, part=0) -> CSRHost:
-        """part 0 = inner (upper CSR, the reference's layout), 1 = bound, 2 = inner full storage."""
+        """part 0 = inner (upper CSR, the reference's layout), 1 = bound, 2 = inner full storage;
+        3 and 4 = parts 0 and 2 of the FP32 values (make_fp32 / assemble_fp32)."""
         i = self.info()
-        nnz = {0: i["nnz_upper"], 1: i["nnz_bound"], 2: i["nnz_full"]}[part]
+        nnz = {0: i["nnz_upper"], 1: i["nnz_bound"], 2: i["nnz_full"], 3: i["nnz_upper"], 4: i["nnz_full"]}[part]
         outer = np.zeros(i["rows"] + 1, dtype=np.int64)
         inner = np.zeros(max(nnz, 1), dtype=np.int64)
         val = np.zeros(max(nnz, 1))
@@ -203,7 +214,24 @@ class finite_element_matrix:
         return y
 
     def make_fp32(self):
+        """FP32 values for mixed-precision inner solves: the FP64 values rounded."""
         self.ctx.check(load().nlf_matrix_make_fp32(self.h))
+
+    def assemble_fp32(self):
+        """FP32 values for mixed-precision inner solves: the value pass repeated in FP32 (FP32
+        element-pair contractions and accumulation), plus the boundary additions made since."""
+        self.ctx.check(load().nlf_matrix_assemble_fp32(self.h))
+
+    def make_split(self):
+        """Values for NLF_MODE_SPLIT: FP32 rounding plus a 16-bit remainder (40 significand bits)."""
+        self.ctx.check(load().nlf_matrix_make_split(self.h))
+
+    def compare_fp32(self):
+        """Distance of the FP32 values from the FP64 ones (device reduction over the full storage):
+        {"max_abs", "max_entry", "l2", "norm"}."""
+        out = np.zeros(4)
+        self.ctx.check(load().nlf_matrix_compare_fp32(self.h, _ptr(out, _f64p)))
+        return dict(max_abs=out[0], max_entry=out[1], l2=out[2], norm=out[3])
 
     def add_convection(self, bcs):
         arr = _abi.array_of(_abi.Boundary, bcs)
@@ -359,7 +387,8 @@ def _radii(models):
     return out
 
 
-def _solve_driver(mesh: mesh_2d, physics, models, materials, bcs, precond, tol, max_it, mode, weights=None):
+def _solve_driver(mesh: mesh_2d, physics, models, materials, bcs, precond, tol, max_it, mode, weights=None,
+                  assembly32=False):
     import time
     ctx = mesh.ctx
     dof = 1 if physics == 0 else 2
@@ -403,10 +432,18 @@ def _solve_driver(mesh: mesh_2d, physics, models, materials, bcs, precond, tol, 
         fallback = preconditioner(K, NLF_PRECOND_JACOBI)
     else:
         cg.init_preconditioner(precond)
-    if mode != NLF_MODE_FP64:
-        K.make_fp32()
     ctx.synchronize()
     tm["preconditioner"] = time.perf_counter() - t2
+    if mode != NLF_MODE_FP64:
+        t4 = time.perf_counter()
+        if mode == _abi.NLF_MODE_SPLIT:
+            K.make_split()
+        elif assembly32:
+            K.assemble_fp32()
+        else:
+            K.make_fp32()
+        ctx.synchronize()
+        tm["fp32_values"] = time.perf_counter() - t4
     t3 = time.perf_counter()
     if learned_pc is not None:
         x, st = learned.fcg_solve(K, learned_pc, fd, fallback=fallback, tol=tol, max_it=max_it, mode=mode)
@@ -421,17 +458,18 @@ def _solve_driver(mesh: mesh_2d, physics, models, materials, bcs, precond, tol, 
 
 
 def stationary_heat_equation_solver_2d(mesh: mesh_2d, models, materials, bcs, precond=NLF_PRECOND_ILLT, tol=1e-12,
-                                       max_it=100000, mode=NLF_MODE_FP64, weights=None) -> Solution:
+                                       max_it=100000, mode=NLF_MODE_FP64, weights=None, assembly32=False) -> Solution:
     """K T = f with K = p1 K_L + (1 - p1) K_NL (solved directly; the reference's correction form
     T = T0 - K^-1 (K T0 - f) with T0 = 0 is the same system).  precond NLF_PRECOND_LEARNED runs
     flexible CG with the learned preconditioner (weights file `weights`, default the packaged one)
-    and Jacobi-PCG as its fallback; tol is then on the FP64 true residual."""
-    return _solve_driver(mesh, 0, models, materials, bcs, precond, tol, max_it, mode, weights)
+    and Jacobi-PCG as its fallback; tol is then on the FP64 true residual.  In the mixed mode the
+    inner FP32 operator is the rounded FP64 matrix, or with assembly32 an FP32 assembly."""
+    return _solve_driver(mesh, 0, models, materials, bcs, precond, tol, max_it, mode, weights, assembly32)
 
 
 def equilibrium_equation(mesh: mesh_2d, models, materials, bcs, precond=None, tol=1e-12, max_it=100000,
-                         mode=NLF_MODE_FP64) -> Solution:
+                         mode=NLF_MODE_FP64, assembly32=False) -> Solution:
     """Plane-stress static elasticity; ILLT of the local stiffness when nonlocal, identity otherwise."""
     if precond is None:
         precond = NLF_PRECOND_ILLT if any(m.local_weight < 0.999 for m in models) else NLF_PRECOND_IDENTITY
-    return _solve_driver(mesh, 1, models, materials, bcs, precond, tol, max_it, mode)
+    return _solve_driver(mesh, 1, models, materials, bcs, precond, tol, max_it, mode, assembly32=assembly32)

@@ -97,9 +97,14 @@ class BatchSolver:
         self.target = t.value
         return t.value
 
-    def solve(self, tol=1e-10, max_it=100000, mode=_abi.NLF_MODE_FP64, use_graph=True, x0=None):
+    def solve(self, tol=1e-10, max_it=100000, mode=_abi.NLF_MODE_FP64, use_graph=True, x0=None, assembly32=False):
+        """mode NLF_MODE_MIXED reads FP32 values: the rounded FP64 ones, or with assembly32 an FP32
+        assembly of the whole batch."""
         if mode != _abi.NLF_MODE_FP64:
-            self.K.make_fp32()
+            if assembly32:
+                self.K.assemble_fp32()
+            else:
+                self.K.make_fp32()
         x = torch.zeros_like(self.b) if x0 is None else x0.clone()
         count = len(self.problems)
         its = np.zeros(count, dtype=np.int64)

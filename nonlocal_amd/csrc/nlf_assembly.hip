@@ -284,10 +284,18 @@ struct AssemblyArgs {
     const i32* b_cols;
     double* val;
     double* b_val;
+    float* val32;        // FP32 value pass: inner values only (the bound part stays FP64)
     int local_only;      // assemble_part::LOCAL: every group treated as LOCAL
     const double* pairs; // thermal Q4: nonlocal pair blocks from k_pairs_mfma (null = compute in place)
+    const float* pairs32;
     i64 plane;           // pairs[i * plane + p * 4 + j], p = neighbour-list position
 };
+
+template<class T>
+__device__ __forceinline__ T phi_of(const InfluenceParams& f, double ax, double ay, double bx, double by) {
+    if constexpr (sizeof(T) == sizeof(double)) return influence(f, ax, ay, bx, by);
+    else return influence32(f, ax, ay, bx, by);
+}
 
 __device__ __forceinline__ int lower_bound_i32(const i32* a, int n, i32 key) {
     int lo = 0, hi = n;
@@ -299,25 +307,27 @@ __device__ __forceinline__ int lower_bound_i32(const i32* a, int n, i32 key) {
 }
 
 // Local integrand row i of element e: K^L_ij (thermal) or 2x2 blocks (elastic), scaled by p1.
-// conductivity_matrix_2d.hpp:106-130, stiffness_matrix_2d.hpp:44-80.
-template<int DOF, int NN, int NQ>
-__device__ void local_row(const AssemblyArgs& a, const GroupParams& g, i64 e, int i, double (&v)[NN][DOF * DOF]) {
+// conductivity_matrix_2d.hpp:106-130, stiffness_matrix_2d.hpp:44-80.  T = float is the FP32 value
+// pass: w_q / |det J| is formed in FP64, the products and sums in T.
+template<int DOF, int NN, int NQ, class T>
+__device__ void local_row(const AssemblyArgs& a, const GroupParams& g, i64 e, int i, T (&v)[NN][DOF * DOF]) {
     for (int j = 0; j < NN; ++j)
-        for (int k = 0; k < DOF * DOF; ++k) v[j][k] = 0.0;
+        for (int k = 0; k < DOF * DOF; ++k) v[j][k] = T(0);
+    const T c0 = T(g.c[0]), c1 = T(g.c[1]), c2 = T(g.c[2]);
     for (int q = 0; q < NQ; ++q) {
-        const double f = a.w[q] / a.jac[e * NQ + q];
-        const double ix = a.D[2 * ((e * NN + i) * NQ + q)], iy = a.D[2 * ((e * NN + i) * NQ + q) + 1];
+        const T f = T(a.w[q] / a.jac[e * NQ + q]);
+        const T ix = T(a.D[2 * ((e * NN + i) * NQ + q)]), iy = T(a.D[2 * ((e * NN + i) * NQ + q) + 1]);
         for (int j = 0; j < NN; ++j) {
-            const double jx = a.D[2 * ((e * NN + j) * NQ + q)], jy = a.D[2 * ((e * NN + j) * NQ + q) + 1];
+            const T jx = T(a.D[2 * ((e * NN + j) * NQ + q)]), jy = T(a.D[2 * ((e * NN + j) * NQ + q) + 1]);
             if (DOF == 1) {
-                double value;
-                if (g.ckind == NLF_CONDUCTIVITY_ISOTROPIC) value = g.c[0] * (ix * jx + iy * jy);
-                else if (g.ckind == NLF_CONDUCTIVITY_ORTHOTROPIC) value = g.c[0] * ix * jx + g.c[1] * iy * jy;
-                else value = ix * (g.c[0] * jx + g.c[2] * jy) + iy * (g.c[2] * jx + g.c[1] * jy);
+                T value;
+                if (g.ckind == NLF_CONDUCTIVITY_ISOTROPIC) value = c0 * (ix * jx + iy * jy);
+                else if (g.ckind == NLF_CONDUCTIVITY_ORTHOTROPIC) value = c0 * ix * jx + c1 * iy * jy;
+                else value = ix * (c0 * jx + c2 * jy) + iy * (c2 * jx + c1 * jy);
                 v[j][0] += f * value;
             } else {
-                const double h11 = g.c[0], h12 = g.c[1], h66 = g.c[2];
-                const double wx = f * ix, wy = f * iy;
+                const T h11 = c0, h12 = c1, h66 = c2;
+                const T wx = f * ix, wy = f * iy;
                 v[j][0] += h11 * wx * jx + h66 * wy * jy;
                 v[j][1] += h12 * wx * jy + h66 * wy * jx;
                 v[j][2] += h12 * wy * jx + h66 * wx * jy;
@@ -325,37 +335,41 @@ __device__ void local_row(const AssemblyArgs& a, const GroupParams& g, i64 e, in
             }
         }
     }
+    const T p1 = T(g.p1);
     for (int j = 0; j < NN; ++j)
-        for (int k = 0; k < DOF * DOF; ++k) v[j][k] *= g.p1;
+        for (int k = 0; k < DOF * DOF; ++k) v[j][k] *= p1;
 }
 
 // Nonlocal integrand row iL of pair (eL, eNL):
 //   u(qNL) = sum_qL phi(x_qL, x_qNL) w_qL D_i(qL)            (B_e^T Phi contraction)
 //   v_j    = (1 - p1) sum_qNL w_qNL u(qNL) . C D_j(qNL)       (... x C B_e')
 // conductivity_matrix_2d.hpp:132-164, stiffness_matrix_2d.hpp:82-123 (C taken at the nonlocal point).
-template<int DOF, int NN, int NQ>
+// T = float: coordinates stay FP64 (their differences are rounded once), phi and the contractions
+// run in FP32.
+template<int DOF, int NN, int NQ, class T>
 __device__ void nonlocal_row(const AssemblyArgs& a, const GroupParams& g, const double* xl /*2*NQ*/,
-                             const double* hl /*2*NQ: w_qL D_i(qL)*/, i64 eNL, double (&v)[NN][DOF * DOF]) {
+                             const T* hl /*2*NQ: w_qL D_i(qL)*/, i64 eNL, T (&v)[NN][DOF * DOF]) {
     for (int j = 0; j < NN; ++j)
-        for (int k = 0; k < DOF * DOF; ++k) v[j][k] = 0.0;
+        for (int k = 0; k < DOF * DOF; ++k) v[j][k] = T(0);
+    const T c0 = T(g.c[0]), c1 = T(g.c[1]), c2 = T(g.c[2]);
     for (int qn = 0; qn < NQ; ++qn) {
         const double bx = a.xq[2 * (eNL * NQ + qn)], by = a.xq[2 * (eNL * NQ + qn) + 1];
-        double ux = 0.0, uy = 0.0;
+        T ux = T(0), uy = T(0);
         for (int ql = 0; ql < NQ; ++ql) {
-            const double phi = influence(g.inf, xl[2 * ql], xl[2 * ql + 1], bx, by);
+            const T phi = phi_of<T>(g.inf, xl[2 * ql], xl[2 * ql + 1], bx, by);
             ux += phi * hl[2 * ql];
             uy += phi * hl[2 * ql + 1];
         }
-        ux *= a.w[qn];
-        uy *= a.w[qn];
+        ux *= T(a.w[qn]);
+        uy *= T(a.w[qn]);
         for (int j = 0; j < NN; ++j) {
-            const double jx = a.D[2 * ((eNL * NN + j) * NQ + qn)], jy = a.D[2 * ((eNL * NN + j) * NQ + qn) + 1];
+            const T jx = T(a.D[2 * ((eNL * NN + j) * NQ + qn)]), jy = T(a.D[2 * ((eNL * NN + j) * NQ + qn) + 1]);
             if (DOF == 1) {
-                if (g.ckind == NLF_CONDUCTIVITY_ISOTROPIC) v[j][0] += g.c[0] * (ux * jx + uy * jy);
-                else if (g.ckind == NLF_CONDUCTIVITY_ORTHOTROPIC) v[j][0] += g.c[0] * ux * jx + g.c[1] * uy * jy;
-                else v[j][0] += ux * (g.c[0] * jx + g.c[2] * jy) + uy * (g.c[2] * jx + g.c[1] * jy);
+                if (g.ckind == NLF_CONDUCTIVITY_ISOTROPIC) v[j][0] += c0 * (ux * jx + uy * jy);
+                else if (g.ckind == NLF_CONDUCTIVITY_ORTHOTROPIC) v[j][0] += c0 * ux * jx + c1 * uy * jy;
+                else v[j][0] += ux * (c0 * jx + c2 * jy) + uy * (c2 * jx + c1 * jy);
             } else {
-                const double h11 = g.c[0], h12 = g.c[1], h66 = g.c[2];
+                const T h11 = c0, h12 = c1, h66 = c2;
                 v[j][0] += h11 * ux * jx + h66 * uy * jy;
                 v[j][1] += h12 * ux * jy + h66 * uy * jx;
                 v[j][2] += h12 * uy * jx + h66 * ux * jy;
@@ -363,7 +377,7 @@ __device__ void nonlocal_row(const AssemblyArgs& a, const GroupParams& g, const 
             }
         }
     }
-    const double s = 1.0 - g.p1;
+    const T s = T(1.0 - g.p1);
     for (int j = 0; j < NN; ++j)
         for (int k = 0; k < DOF * DOF; ++k) v[j][k] *= s;
 }
@@ -382,6 +396,7 @@ __device__ void nonlocal_row(const AssemblyArgs& a, const GroupParams& g, const 
 //                                (A[i = c][qn = h] = V[qn = h][i = c]); B lane: HN_k[j = c][qn = h]
 // so a pair costs four MFMA instructions per four pairs and no lane permutes.  Blocks are written
 // to pairs[i * plane + p * 4 + j]; the node-centric assembly reads row i of them.
+template<class T>
 struct PairArgs {
     i64 E;
     const i64* nb_off;
@@ -389,12 +404,12 @@ struct PairArgs {
     const i32* group_of;
     const GroupParams* groups;
     const double* xq;          // [e][q][2]
-    const double* HL;          // [e][k][i][q], 32 per element
-    const double* HN;
+    const T* HL;               // [e][k][i][q], 32 per element
+    const T* HN;
     const i32* el_min;
     const i32* el_max;
     const uint8_t* el_dir;
-    double* pairs;
+    T* pairs;
     i64 plane;
 };
 
@@ -402,8 +417,9 @@ struct Weights4 {
     double w[4];
 };
 
+template<class T>
 __global__ void k_pair_prep(i64 E, const double* __restrict__ D, Weights4 w4, const i32* __restrict__ group_of,
-                            const GroupParams* __restrict__ groups, double* __restrict__ HL, double* __restrict__ HN) {
+                            const GroupParams* __restrict__ groups, T* __restrict__ HL, T* __restrict__ HN) {
     const i64 t = i64(blockIdx.x) * blockDim.x + threadIdx.x;      // (e, i, q)
     if (t >= E * 16) return;
     const i64 e = t / 16;
@@ -415,13 +431,13 @@ __global__ void k_pair_prep(i64 E, const double* __restrict__ D, Weights4 w4, co
     if (g.ckind == NLF_CONDUCTIVITY_ISOTROPIC) { cx = g.c[0] * dx; cy = g.c[0] * dy; }
     else if (g.ckind == NLF_CONDUCTIVITY_ORTHOTROPIC) { cx = g.c[0] * dx; cy = g.c[1] * dy; }
     else { cx = g.c[0] * dx + g.c[2] * dy; cy = g.c[2] * dx + g.c[1] * dy; }
-    HL[e * 32 + i * 4 + q] = w * dx;
-    HL[e * 32 + 16 + i * 4 + q] = w * dy;
-    HN[e * 32 + i * 4 + q] = w * cx;
-    HN[e * 32 + 16 + i * 4 + q] = w * cy;
+    HL[e * 32 + i * 4 + q] = T(w * dx);
+    HL[e * 32 + 16 + i * 4 + q] = T(w * dy);
+    HN[e * 32 + i * 4 + q] = T(w * cx);
+    HN[e * 32 + 16 + i * 4 + q] = T(w * cy);
 }
 
-__global__ __launch_bounds__(256) void k_pairs_mfma(PairArgs a) {
+__global__ __launch_bounds__(256) void k_pairs_mfma(PairArgs<double> a) {
     const int l = threadIdx.x & 63;
     const int g = (l >> 2) & 3, c = l & 3, h = l >> 4;
     const i64 waves = i64(gridDim.x) * (blockDim.x >> 6);
@@ -458,10 +474,71 @@ __global__ __launch_bounds__(256) void k_pairs_mfma(PairArgs a) {
     }
 }
 
-template<int DOF, int NN, int NQ, bool DETERMINISTIC>
+// FP32 pair blocks with V_MFMA_F32_4X4X1_16B_F32: sixteen independent rank-1 4x4 updates per
+// wave; lane l = 4 b + c belongs to block b and (measured on gfx950) holds A_b[c][0], B_b[0][c]
+// and D_b[i][c] in accumulator register i.  One wave handles one element eL and sixteen
+// neighbours per step (block b = neighbour slot):
+//   V_k[qn][i] = sum_ql Phi[ql][qn] HL_k[i][ql]   four rank-1 steps over ql:
+//                A lane: Phi[ql][qn = c], B lane: HL_k[i = c][ql]; D: lane 4b + i, register qn
+//   K[i][j]   += sum_k sum_qn V_k[qn][i] HN_k[j][qn]   eight rank-1 steps over (k, qn):
+//                A lane 4b + i: register qn of V_k (already in place), B lane: HN_k[j = c][qn]
+// Sixteen instructions per sixteen pairs, every product of the blocks used; FP32 accumulation.
+__global__ __launch_bounds__(256) void k_pairs_mfma32(PairArgs<float> a) {
+    typedef float f32x4 __attribute__((ext_vector_type(4)));
+    const int l = threadIdx.x & 63;
+    const int b = l >> 2, c = l & 3;
+    const i64 waves = i64(gridDim.x) * (blockDim.x >> 6);
+    for (i64 eL = i64(blockIdx.x) * (blockDim.x >> 6) + (threadIdx.x >> 6); eL < a.E; eL += waves) {
+        const GroupParams gp = a.groups[a.group_of[eL]];          // one group per wave
+        if (!gp.nonlocal) continue;
+        const f32x4 hl0 = *reinterpret_cast<const f32x4*>(a.HL + eL * 32 + c * 4);       // HL_0[i = c][ql]
+        const f32x4 hl1 = *reinterpret_cast<const f32x4*>(a.HL + eL * 32 + 16 + c * 4);
+        double xl[8];
+        for (int q = 0; q < 8; ++q) xl[q] = a.xq[eL * 8 + q];
+        const i64 nb_b = a.nb_off[eL], cnt = a.nb_off[eL + 1] - nb_b;
+        const i32 emin = a.el_min[eL];
+        const float s = float(1.0 - gp.p1);
+        for (i64 base = 0; base < cnt; base += 16) {
+            const i64 t = base + b;
+            bool need = false;
+            i64 eNL = eL;
+            if (t < cnt) {
+                eNL = a.nb_ids[nb_b + t];
+                need = a.el_max[eNL] >= emin || a.el_dir[eNL];
+            }
+            float phi[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+            f32x4 hn0 = {0.0f, 0.0f, 0.0f, 0.0f}, hn1 = {0.0f, 0.0f, 0.0f, 0.0f};
+            if (need) {
+                const double bx = a.xq[2 * (eNL * 4 + c)], by = a.xq[2 * (eNL * 4 + c) + 1];
+                for (int ql = 0; ql < 4; ++ql) phi[ql] = influence32(gp.inf, xl[2 * ql], xl[2 * ql + 1], bx, by);
+                hn0 = *reinterpret_cast<const f32x4*>(a.HN + eNL * 32 + c * 4);            // HN_0[j = c][qn]
+                hn1 = *reinterpret_cast<const f32x4*>(a.HN + eNL * 32 + 16 + c * 4);
+            }
+            const f32x4 zero = {0.0f, 0.0f, 0.0f, 0.0f};
+            f32x4 v0 = zero, v1 = zero;
+            for (int ql = 0; ql < 4; ++ql) {
+                v0 = __builtin_amdgcn_mfma_f32_4x4x1f32(phi[ql], hl0[ql], v0, 0, 0, 0);
+                v1 = __builtin_amdgcn_mfma_f32_4x4x1f32(phi[ql], hl1[ql], v1, 0, 0, 0);
+            }
+            f32x4 k = zero;
+            for (int qn = 0; qn < 4; ++qn) {
+                k = __builtin_amdgcn_mfma_f32_4x4x1f32(v0[qn], hn0[qn], k, 0, 0, 0);
+                k = __builtin_amdgcn_mfma_f32_4x4x1f32(v1[qn], hn1[qn], k, 0, 0, 0);
+            }
+            if (need)
+                for (int i = 0; i < 4; ++i) a.pairs[i64(i) * a.plane + (nb_b + t) * 4 + c] = s * k[i];  // K[i][j = c]
+        }
+    }
+}
+
+// T = double: the FP64 value pass (inner and bound parts).  T = float: the FP32 value pass with
+// FP32 accumulation into val32 (inner part only).
+template<int DOF, int NN, int NQ, bool DETERMINISTIC, class T>
 __global__ __launch_bounds__(ASM_THREADS) void k_assemble(AssemblyArgs a) {
+    constexpr bool F64 = sizeof(T) == sizeof(double);
     extern __shared__ unsigned char smem[];
-    __shared__ double xl[2 * NQ], hl[2 * NQ];
+    __shared__ double xl[2 * NQ];
+    __shared__ T hl[2 * NQ];
     __shared__ int row_len[DOF], brow_len[DOF];
     __shared__ int s_off[2 * DOF + 1];
     const i64 node = blockIdx.x;
@@ -481,16 +558,16 @@ __global__ __launch_bounds__(ASM_THREADS) void k_assemble(AssemblyArgs a) {
     }
     __syncthreads();
     const int total = s_off[2 * DOF];
-    double* acc = reinterpret_cast<double*>(smem);
+    T* acc = reinterpret_cast<T*>(smem);
     i32* cl = reinterpret_cast<i32*>(acc + total);
     for (int d = 0; d < DOF; ++d) {
         for (int k = threadIdx.x; k < row_len[d]; k += blockDim.x) {
             cl[s_off[d] + k] = a.cols[a.diag[rows[d]] + k];
-            acc[s_off[d] + k] = 0.0;
+            acc[s_off[d] + k] = T(0);
         }
         for (int k = threadIdx.x; k < brow_len[d]; k += blockDim.x) {
             cl[s_off[DOF + d] + k] = a.b_cols[a.b_outer[rows[d]] + k];
-            acc[s_off[DOF + d] + k] = 0.0;
+            acc[s_off[DOF + d] + k] = T(0);
         }
     }
     bool rinner[DOF];
@@ -506,15 +583,15 @@ __global__ __launch_bounds__(ASM_THREADS) void k_assemble(AssemblyArgs a) {
             const int q = threadIdx.x;
             xl[2 * q] = a.xq[2 * (i64(eL) * NQ + q)];
             xl[2 * q + 1] = a.xq[2 * (i64(eL) * NQ + q) + 1];
-            hl[2 * q] = a.w[q] * a.D[2 * ((i64(eL) * NN + iL) * NQ + q)];
-            hl[2 * q + 1] = a.w[q] * a.D[2 * ((i64(eL) * NN + iL) * NQ + q) + 1];
+            hl[2 * q] = T(a.w[q] * a.D[2 * ((i64(eL) * NN + iL) * NQ + q)]);
+            hl[2 * q + 1] = T(a.w[q] * a.D[2 * ((i64(eL) * NN + iL) * NQ + q) + 1]);
         }
         __syncthreads();
         const i64 nb_b = nonlocal ? a.nb_off[eL] : 0;
         const i64 cnt = nonlocal ? a.nb_off[eL + 1] - nb_b : 1;
         for (i64 base = 0; base < cnt; base += blockDim.x) {
             const i64 t = base + threadIdx.x;
-            double v[NN][DOF * DOF];
+            T v[NN][DOF * DOF];
             int pos[NN][DOF * DOF];
             bool active = false;
             i64 eNL = -1;
@@ -526,24 +603,26 @@ __global__ __launch_bounds__(ASM_THREADS) void k_assemble(AssemblyArgs a) {
             }
             if (active) {
                 if (nonlocal) {
+                    const T* pairs = nullptr;
+                    if constexpr (F64) pairs = a.pairs; else pairs = a.pairs32;
                     if constexpr (DOF == 1 && NN == 4 && NQ == 4) {
-                        if (a.pairs) {
-                            const double* pr = a.pairs + i64(iL) * a.plane + (nb_b + t) * 4;
+                        if (pairs) {
+                            const T* pr = pairs + i64(iL) * a.plane + (nb_b + t) * 4;
                             for (int j = 0; j < NN; ++j) v[j][0] = pr[j];
                         } else {
-                            nonlocal_row<DOF, NN, NQ>(a, g, xl, hl, eNL, v);
+                            nonlocal_row<DOF, NN, NQ, T>(a, g, xl, hl, eNL, v);
                         }
                     } else {
-                        nonlocal_row<DOF, NN, NQ>(a, g, xl, hl, eNL, v);
+                        nonlocal_row<DOF, NN, NQ, T>(a, g, xl, hl, eNL, v);
                     }
                     if (eNL == eL) {
-                        double lv[NN][DOF * DOF];
-                        local_row<DOF, NN, NQ>(a, g, eL, iL, lv);
+                        T lv[NN][DOF * DOF];
+                        local_row<DOF, NN, NQ, T>(a, g, eL, iL, lv);
                         for (int j = 0; j < NN; ++j)
                             for (int q = 0; q < DOF * DOF; ++q) v[j][q] += lv[j][q];
                     }
                 } else {
-                    local_row<DOF, NN, NQ>(a, g, eL, iL, v);
+                    local_row<DOF, NN, NQ, T>(a, g, eL, iL, v);
                 }
                 for (int j = 0; j < NN; ++j) {
                     const i64 m = a.el[eNL * NN + j];
@@ -580,16 +659,21 @@ __global__ __launch_bounds__(ASM_THREADS) void k_assemble(AssemblyArgs a) {
     }
     __syncthreads();
     for (int d = 0; d < DOF; ++d) {
-        for (int k = threadIdx.x; k < row_len[d]; k += blockDim.x)
-            a.val[a.diag[rows[d]] + k] = rinner[d] ? acc[s_off[d] + k] : (k == 0 ? 1.0 : 0.0);
-        for (int k = threadIdx.x; k < brow_len[d]; k += blockDim.x)
-            a.b_val[a.b_outer[rows[d]] + k] = acc[s_off[DOF + d] + k];
+        for (int k = threadIdx.x; k < row_len[d]; k += blockDim.x) {
+            const T value = rinner[d] ? acc[s_off[d] + k] : (k == 0 ? T(1) : T(0));
+            if constexpr (F64) a.val[a.diag[rows[d]] + k] = value;
+            else a.val32[a.diag[rows[d]] + k] = value;
+        }
+        if constexpr (F64)
+            for (int k = threadIdx.x; k < brow_len[d]; k += blockDim.x)
+                a.b_val[a.b_outer[rows[d]] + k] = acc[s_off[DOF + d] + k];
     }
 }
 
 // Lower triangle of the full storage: a(r, c) = a(c, r) for c < r (binary search of r in row c).
+template<class T>
 __global__ void k_mirror(i64 rows, const i64* __restrict__ outer, const i32* __restrict__ cols,
-                         const i64* __restrict__ diag, double* __restrict__ val) {
+                         const i64* __restrict__ diag, T* __restrict__ val) {
     const i64 r = i64(blockIdx.x) * (blockDim.x / 64) + (threadIdx.x >> 6);
     if (r >= rows) return;
     const int lane = threadIdx.x & 63;
@@ -646,18 +730,19 @@ __global__ void k_row_max(i64 rows, const i64* __restrict__ outer, const i64* __
     atomicMax(mx, static_cast<unsigned long long>(s));
 }
 
+template<class T>
 __global__ void k_add_pairs(i64 n, const i64* __restrict__ rc, const double* __restrict__ v,
-                            const i64* __restrict__ outer, const i32* __restrict__ cols, double* __restrict__ val) {
+                            const i64* __restrict__ outer, const i32* __restrict__ cols, T* __restrict__ val) {
     const i64 k = i64(blockIdx.x) * blockDim.x + threadIdx.x;
     if (k >= n) return;
     const i64 r = rc[2 * k], c = rc[2 * k + 1];
     const i64 b = outer[r], e = outer[r + 1];
     const int q = lower_bound_i32(cols + b, int(e - b), i32(c));
-    if (q < e - b && cols[b + q] == c) val[b + q] += v[k];
+    if (q < e - b && cols[b + q] == c) val[b + q] += T(v[k]);
     if (r != c) {
         const i64 b2 = outer[c], e2 = outer[c + 1];
         const int q2 = lower_bound_i32(cols + b2, int(e2 - b2), i32(r));
-        if (q2 < e2 - b2 && cols[b2 + q2] == r) val[b2 + q2] += v[k];
+        if (q2 < e2 - b2 && cols[b2 + q2] == r) val[b2 + q2] += T(v[k]);
     }
 }
 
@@ -665,10 +750,54 @@ __global__ void k_to_f32(i64 n, const double* __restrict__ a, float* __restrict_
     for (i64 i = i64(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += i64(gridDim.x) * blockDim.x) b[i] = float(a[i]);
 }
 
-template<int DOF, int NN, int NQ>
+// hi = fp32(v); lo = the remainder v - hi in units of 2^-16 ulp(hi), rounded to the nearest
+// integer.  |v - hi| <= ulp(hi) / 2, so lo fits 16 bits and hi + lo carries 40 significand bits:
+// |hi + lo * 2^-16 ulp(hi) - v| <= 2^-17 ulp(hi) <= 2^-40 |v|.  ulp(hi) = 2^(e - 150) for the
+// biased exponent e of hi (e = 1 for subnormals), so the scale is 2^(e - 166).
+__device__ __forceinline__ double split_scale(float h) {
+    unsigned e = (__float_as_uint(h) >> 23) & 0xffu;
+    e = e ? e : 1u;
+    return __longlong_as_double(static_cast<long long>(e + 857u) << 52);      // 2^(e - 166)
+}
+
+__global__ void k_to_split(i64 n, const double* __restrict__ a, float* __restrict__ hi, int16_t* __restrict__ lo) {
+    for (i64 i = i64(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += i64(gridDim.x) * blockDim.x) {
+        const double v = a[i];
+        const float h = float(v);
+        const double t = rint((v - double(h)) / split_scale(h));                 // exact power-of-two scaling
+        hi[i] = h;
+        lo[i] = int16_t(fmin(fmax(t, -32768.0), 32767.0));
+    }
+}
+
+// Per-block partials of max|b - a|, max|a|, sum (b - a)^2, sum a^2 (fixed order: deterministic).
+__global__ __launch_bounds__(256) void k_compare32(i64 n, const double* __restrict__ a, const float* __restrict__ b,
+                                                   double* __restrict__ part) {
+    __shared__ double red[4][256];
+    double m0 = 0.0, m1 = 0.0, s0 = 0.0, s1 = 0.0;
+    for (i64 i = i64(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += i64(gridDim.x) * blockDim.x) {
+        const double x = a[i], d = double(b[i]) - x;
+        m0 = fmax(m0, fabs(d)); m1 = fmax(m1, fabs(x));
+        s0 += d * d; s1 += x * x;
+    }
+    red[0][threadIdx.x] = m0; red[1][threadIdx.x] = m1; red[2][threadIdx.x] = s0; red[3][threadIdx.x] = s1;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (int(threadIdx.x) < w) {
+            red[0][threadIdx.x] = fmax(red[0][threadIdx.x], red[0][threadIdx.x + w]);
+            red[1][threadIdx.x] = fmax(red[1][threadIdx.x], red[1][threadIdx.x + w]);
+            red[2][threadIdx.x] += red[2][threadIdx.x + w];
+            red[3][threadIdx.x] += red[3][threadIdx.x + w];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x < 4) part[blockIdx.x * 4 + threadIdx.x] = red[threadIdx.x][0];
+}
+
+template<int DOF, int NN, int NQ, class T>
 void launch_assemble(const AssemblyArgs& a, bool det, size_t lds, hipStream_t s) {
-    if (det) hipLaunchKernelGGL((k_assemble<DOF, NN, NQ, true>), dim3(unsigned(a.N)), dim3(ASM_THREADS), lds, s, a);
-    else hipLaunchKernelGGL((k_assemble<DOF, NN, NQ, false>), dim3(unsigned(a.N)), dim3(ASM_THREADS), lds, s, a);
+    if (det) hipLaunchKernelGGL((k_assemble<DOF, NN, NQ, true, T>), dim3(unsigned(a.N)), dim3(ASM_THREADS), lds, s, a);
+    else hipLaunchKernelGGL((k_assemble<DOF, NN, NQ, false, T>), dim3(unsigned(a.N)), dim3(ASM_THREADS), lds, s, a);
 }
 
 }  // namespace nlf
@@ -692,6 +821,135 @@ GroupParams make_group(const nlf_model& md, int physics, const void* materials, 
         gp.c[0] = value; gp.c[1] = e.poisson * value; gp.c[2] = 0.5 * value * (1.0 - e.poisson);
     }
     return gp;
+}
+
+// Value pass on the matrix's pattern.  T = double fills val and b_val (and the 16-bit column
+// offsets); T = float fills val32 with FP32 contractions and FP32 accumulation.  Thermal Q4
+// nonlocal blocks go through the MFMA pair kernel when its pair buffer (16 values per neighbour
+// pair) fits in half of the free device memory; option NLF_OPT_ASSEMBLY_PAIRS = 0 forces the
+// node-centric contraction.
+template<class T>
+void assemble_values(nlf_matrix_s* mat, hipStream_t s) {
+    constexpr bool F64 = sizeof(T) == sizeof(double);
+    const nlf_mesh m = mat->mesh;
+    const int dof = mat->dof;
+    const i64 rows = mat->rows;
+    const bool only_local = mat->only_local;
+    bool any_nonlocal = false;
+    for (const GroupParams& g : mat->groups) any_nonlocal |= g.nonlocal != 0;
+    DevBuf<GroupParams> d_groups;
+    upload(d_groups, mat->groups.data(), mat->groups.size(), s);
+    DevBuf<i32> el_min{size_t(m->E)};
+    hipLaunchKernelGGL(k_elem_minnode, dim3(blocks_for(m->E, 256)), dim3(256), 0, s, m->E, m->nn, m->el.p, el_min.p);
+    DevBuf<uint8_t> el_dir{size_t(m->E)};
+    hipLaunchKernelGGL(k_el_dirichlet, dim3(blocks_for(m->E, 256)), dim3(256), 0, s, m->E, m->nn, dof, m->el.p,
+                       mat->inner.p, el_dir.p);
+    DevBuf<unsigned long long> d_max{1};
+    NLF_HIP(hipMemsetAsync(d_max.p, 0, sizeof(unsigned long long), s));
+    hipLaunchKernelGGL(k_row_max, dim3(blocks_for(m->N, 256)), dim3(256), 0, s, rows, mat->outer.p, mat->diag.p,
+                       mat->b_outer.p, dof, d_max.p);
+    unsigned long long maxrow = 0;
+    download(&maxrow, d_max.p, 1, s);
+    const size_t lds = size_t(maxrow) * (sizeof(T) + sizeof(i32)) + 16;
+    NLF_CHECK(lds <= 60 * 1024, NLF_ERR_UNSUPPORTED, "matrix rows too long for the LDS accumulator");
+    if constexpr (F64) {
+        mat->val.alloc(size_t(std::max<i64>(mat->nnz_full, 1)));
+        mat->b_val.alloc(size_t(std::max<i64>(mat->nnz_bound, 1)));
+    } else {
+        if (!mat->val32.p) mat->val32.alloc(size_t(std::max<i64>(mat->nnz_full, 1)));
+    }
+    AssemblyArgs aa{};
+    aa.N = m->N; aa.el = m->el.p; aa.ne_off = m->ne_off.p; aa.ne_ids = m->ne_ids.p; aa.group_of = m->group_of.p;
+    aa.groups = d_groups.p;
+    aa.nb_off = m->has_neighbours ? m->nb_off.p : nullptr;
+    aa.nb_ids = m->has_neighbours ? m->nb_ids.p : nullptr;
+    aa.el_maxnode = m->el_maxnode.p;
+    aa.el_dirichlet = el_dir.p; aa.inner = mat->inner.p; aa.xq = m->xq.p; aa.jac = m->jac.p; aa.D = m->D.p;
+    for (int q = 0; q < m->fe.nq; ++q) aa.w[q] = m->fe.w[q];
+    aa.outer = mat->outer.p; aa.cols = mat->cols.p; aa.diag = mat->diag.p; aa.b_outer = mat->b_outer.p;
+    aa.b_cols = mat->b_cols.p; aa.val = mat->val.p; aa.b_val = mat->b_val.p; aa.val32 = mat->val32.p;
+    aa.local_only = only_local;
+    const bool det = m->injective;
+    DevBuf<T> pairs, HL, HN;
+    const bool node_only = m->ctx->assembly_pairs == 0;
+    if (mat->physics == 0 && m->nn == 4 && any_nonlocal && !only_local && !node_only) {
+        i64 total_pairs = 0;
+        download(&total_pairs, m->nb_off.p + m->E, 1, s);
+        size_t free_b = 0, total_b = 0;
+        NLF_HIP(hipMemGetInfo(&free_b, &total_b));
+        const size_t need_b = size_t(total_pairs) * 16 * sizeof(T) + size_t(m->E) * 64 * sizeof(T);
+        if (total_pairs > 0 && need_b < free_b / 2) {
+            pairs.alloc(size_t(total_pairs) * 16);
+            HL.alloc(size_t(m->E) * 32);
+            HN.alloc(size_t(m->E) * 32);
+            Weights4 w4{};
+            for (int q = 0; q < 4; ++q) w4.w[q] = m->fe.w[q];
+            hipLaunchKernelGGL(k_pair_prep<T>, dim3(blocks_for(m->E * 16, 256)), dim3(256), 0, s, m->E, m->D.p, w4,
+                               m->group_of.p, d_groups.p, HL.p, HN.p);
+            PairArgs<T> pr{};
+            pr.E = m->E; pr.nb_off = m->nb_off.p; pr.nb_ids = m->nb_ids.p; pr.group_of = m->group_of.p;
+            pr.groups = d_groups.p; pr.xq = m->xq.p; pr.HL = HL.p; pr.HN = HN.p; pr.el_min = el_min.p;
+            pr.el_max = m->el_maxnode.p; pr.el_dir = el_dir.p; pr.pairs = pairs.p; pr.plane = total_pairs * 4;
+            const unsigned pb = unsigned(std::min<i64>((m->E + 3) / 4, 16384));
+            if constexpr (F64) {
+                hipLaunchKernelGGL(k_pairs_mfma, dim3(pb), dim3(256), 0, s, pr);
+                aa.pairs = pairs.p;
+            } else {
+                hipLaunchKernelGGL(k_pairs_mfma32, dim3(pb), dim3(256), 0, s, pr);
+                aa.pairs32 = pairs.p;
+            }
+            NLF_HIP(hipGetLastError());
+            aa.plane = total_pairs * 4;
+        }
+    }
+    if (mat->physics == 0 && m->nn == 4) launch_assemble<1, 4, 4, T>(aa, det, lds, s);
+    else if (mat->physics == 0 && m->nn == 8) launch_assemble<1, 8, 9, T>(aa, det, lds, s);
+    else if (mat->physics == 1 && m->nn == 4) launch_assemble<2, 4, 4, T>(aa, det, lds, s);
+    else launch_assemble<2, 8, 9, T>(aa, det, lds, s);
+    NLF_HIP(hipGetLastError());
+    T* values = nullptr;
+    if constexpr (F64) values = mat->val.p; else values = mat->val32.p;
+    hipLaunchKernelGGL(k_mirror<T>, dim3(blocks_for(rows, 4)), dim3(256), 0, s, rows, mat->outer.p, mat->cols.p,
+                       mat->diag.p, values);
+    NLF_HIP(hipGetLastError());
+    if constexpr (F64) {   // 16-bit column offsets for the SpMV when every |col - row| < 2^15
+        DevBuf<unsigned> bw{1};
+        NLF_HIP(hipMemsetAsync(bw.p, 0, sizeof(unsigned), s));
+        hipLaunchKernelGGL(k_bandwidth, dim3(blocks_for(rows, 256)), dim3(256), 0, s, rows, mat->outer.p, mat->cols.p,
+                           bw.p);
+        unsigned hb = 0;
+        download(&hb, bw.p, 1, s);
+        if (hb < 32768u && mat->nnz_full > 0) {
+            mat->off16.alloc(size_t(mat->nnz_full));
+            hipLaunchKernelGGL(k_to_off16, dim3(blocks_for(rows, 4)), dim3(256), 0, s, rows, mat->outer.p, mat->cols.p,
+                               mat->off16.p);
+            NLF_HIP(hipGetLastError());
+            if (m->ctx->row_patterns) build_row_patterns(mat, s);
+        }
+    }
+    NLF_HIP(hipStreamSynchronize(s));
+}
+
+// Adds symmetric (row <= col) pair contributions to the FP64 values, and to the FP32 values when
+// they exist; remembers them so that a later FP32 value pass repeats them.
+void add_pairs_device(nlf_matrix_s* a, const std::vector<i64>& rc, const std::vector<double>& v) {
+    if (v.empty()) return;
+    hipStream_t s = a->ctx->stream;
+    DevBuf<i64> d_rc;
+    DevBuf<double> d_v;
+    upload(d_rc, rc.data(), rc.size(), s);
+    upload(d_v, v.data(), v.size(), s);
+    const i64 n = i64(v.size());
+    hipLaunchKernelGGL(k_add_pairs<double>, dim3(blocks_for(n, 256)), dim3(256), 0, s, n, d_rc.p, d_v.p, a->outer.p,
+                       a->cols.p, a->val.p);
+    if (a->val32.p)
+        hipLaunchKernelGGL(k_add_pairs<float>, dim3(blocks_for(n, 256)), dim3(256), 0, s, n, d_rc.p, d_v.p,
+                           a->outer.p, a->cols.p, a->val32.p);
+    NLF_HIP(hipGetLastError());
+    NLF_HIP(hipStreamSynchronize(s));
+    a->split_ready = false;                           // the split remainders no longer match
+    a->extra_rc.insert(a->extra_rc.end(), rc.begin(), rc.end());
+    a->extra_v.insert(a->extra_v.end(), v.begin(), v.end());
 }
 
 }  // namespace
@@ -731,8 +989,6 @@ int nlf_assemble(nlf_mesh m, int physics, const nlf_model* models, const void* m
         auto mat = std::make_unique<nlf_matrix_s>();
         mat->ctx = m->ctx; mat->mesh = m; mat->dof = dof; mat->rows = rows;
         upload(mat->inner, is_inner, size_t(rows), s);
-        DevBuf<GroupParams> d_groups;
-        upload(d_groups, gps.data(), gps.size(), s);
         DevBuf<int> d_gnl;
         upload(d_gnl, gnl.data(), gnl.size(), s);
         // ---- pattern ----
@@ -785,80 +1041,10 @@ int nlf_assemble(nlf_mesh m, int physics, const nlf_model* models, const void* m
         const auto t1 = std::chrono::steady_clock::now();
         mat->seconds_pattern = std::chrono::duration<double>(t1 - t0).count();
         // ---- values ----
-        DevBuf<uint8_t> el_dir{size_t(m->E)};
-        hipLaunchKernelGGL(k_el_dirichlet, dim3(blocks_for(m->E, 256)), dim3(256), 0, s, m->E, m->nn, dof, m->el.p,
-                           mat->inner.p, el_dir.p);
-        DevBuf<unsigned long long> d_max{1};
-        NLF_HIP(hipMemsetAsync(d_max.p, 0, sizeof(unsigned long long), s));
-        hipLaunchKernelGGL(k_row_max, dim3(blocks_for(m->N, 256)), dim3(256), 0, s, rows, mat->outer.p, mat->diag.p,
-                           mat->b_outer.p, dof, d_max.p);
-        unsigned long long maxrow = 0;
-        download(&maxrow, d_max.p, 1, s);
-        const size_t lds = size_t(maxrow) * (sizeof(double) + sizeof(i32)) + 16;
-        NLF_CHECK(lds <= 60 * 1024, NLF_ERR_UNSUPPORTED, "matrix rows too long for the LDS accumulator");
-        mat->val.alloc(size_t(std::max<i64>(mat->nnz_full, 1)));
-        mat->b_val.alloc(size_t(std::max<i64>(mat->nnz_bound, 1)));
-        AssemblyArgs aa{};
-        aa.N = m->N; aa.el = m->el.p; aa.ne_off = m->ne_off.p; aa.ne_ids = m->ne_ids.p; aa.group_of = m->group_of.p;
-        aa.groups = d_groups.p; aa.nb_off = pa.nb_off; aa.nb_ids = pa.nb_ids; aa.el_maxnode = m->el_maxnode.p;
-        aa.el_dirichlet = el_dir.p; aa.inner = mat->inner.p; aa.xq = m->xq.p; aa.jac = m->jac.p; aa.D = m->D.p;
-        for (int q = 0; q < m->fe.nq; ++q) aa.w[q] = m->fe.w[q];
-        aa.outer = mat->outer.p; aa.cols = mat->cols.p; aa.diag = mat->diag.p; aa.b_outer = mat->b_outer.p;
-        aa.b_cols = mat->b_cols.p; aa.val = mat->val.p; aa.b_val = mat->b_val.p; aa.local_only = only_local;
-        const bool det = m->injective;
-        // thermal Q4 nonlocal blocks through the MFMA pair kernel when the pair buffer (128 B per
-        // neighbour pair) fits in half of the free device memory (option NLF_OPT_ASSEMBLY_PAIRS = 0
-        // forces the node-centric contraction)
-        DevBuf<double> pairs, HL, HN;
-        const bool node_only = m->ctx->assembly_pairs == 0;
-        if (physics == 0 && m->nn == 4 && any_nonlocal && !only_local && !node_only) {
-            i64 total_pairs = 0;
-            download(&total_pairs, m->nb_off.p + m->E, 1, s);
-            size_t free_b = 0, total_b = 0;
-            NLF_HIP(hipMemGetInfo(&free_b, &total_b));
-            const size_t need_b = size_t(total_pairs) * 16 * sizeof(double) + size_t(m->E) * 64 * sizeof(double);
-            if (total_pairs > 0 && need_b < free_b / 2) {
-                pairs.alloc(size_t(total_pairs) * 16);
-                HL.alloc(size_t(m->E) * 32);
-                HN.alloc(size_t(m->E) * 32);
-                Weights4 w4{};
-                for (int q = 0; q < 4; ++q) w4.w[q] = m->fe.w[q];
-                hipLaunchKernelGGL(k_pair_prep, dim3(blocks_for(m->E * 16, 256)), dim3(256), 0, s, m->E, m->D.p, w4,
-                                   m->group_of.p, d_groups.p, HL.p, HN.p);
-                PairArgs pr{};
-                pr.E = m->E; pr.nb_off = m->nb_off.p; pr.nb_ids = m->nb_ids.p; pr.group_of = m->group_of.p;
-                pr.groups = d_groups.p; pr.xq = m->xq.p; pr.HL = HL.p; pr.HN = HN.p; pr.el_min = el_min.p;
-                pr.el_max = m->el_maxnode.p; pr.el_dir = el_dir.p; pr.pairs = pairs.p; pr.plane = total_pairs * 4;
-                const unsigned pb = unsigned(std::min<i64>((m->E + 3) / 4, 16384));
-                hipLaunchKernelGGL(k_pairs_mfma, dim3(pb), dim3(256), 0, s, pr);
-                NLF_HIP(hipGetLastError());
-                aa.pairs = pairs.p;
-                aa.plane = total_pairs * 4;
-            }
-        }
-        if (physics == 0 && m->nn == 4) launch_assemble<1, 4, 4>(aa, det, lds, s);
-        else if (physics == 0 && m->nn == 8) launch_assemble<1, 8, 9>(aa, det, lds, s);
-        else if (physics == 1 && m->nn == 4) launch_assemble<2, 4, 4>(aa, det, lds, s);
-        else launch_assemble<2, 8, 9>(aa, det, lds, s);
-        NLF_HIP(hipGetLastError());
-        hipLaunchKernelGGL(k_mirror, dim3(blocks_for(rows, 4)), dim3(256), 0, s, rows, mat->outer.p, mat->cols.p,
-                           mat->diag.p, mat->val.p);
-        NLF_HIP(hipGetLastError());
-        {   // 16-bit column offsets for the SpMV when every |col - row| < 2^15
-            DevBuf<unsigned> bw{1};
-            NLF_HIP(hipMemsetAsync(bw.p, 0, sizeof(unsigned), s));
-            hipLaunchKernelGGL(k_bandwidth, dim3(blocks_for(rows, 256)), dim3(256), 0, s, rows, mat->outer.p, mat->cols.p,
-                               bw.p);
-            unsigned hb = 0;
-            download(&hb, bw.p, 1, s);
-            if (hb < 32768u && mat->nnz_full > 0) {
-                mat->off16.alloc(size_t(mat->nnz_full));
-                hipLaunchKernelGGL(k_to_off16, dim3(blocks_for(rows, 4)), dim3(256), 0, s, rows, mat->outer.p, mat->cols.p,
-                                   mat->off16.p);
-                NLF_HIP(hipGetLastError());
-            }
-        }
-        NLF_HIP(hipStreamSynchronize(s));
+        mat->physics = physics;
+        mat->only_local = only_local;
+        mat->groups = gps;
+        assemble_values<double>(mat.get(), s);
         mat->nnz_upper = (mat->nnz_full + rows) / 2;
         mat->seconds_values = std::chrono::duration<double>(std::chrono::steady_clock::now() - t1).count();
         *out = mat.release();
@@ -888,7 +1074,8 @@ int nlf_matrix_timings(nlf_matrix a, double* pattern_seconds, double* values_sec
 }
 
 // Host export in the reference's layout (finite_element_matrix::inner()/bound(): RowMajor CSR with
-// int64 indices): part 0 = inner upper triangle, 1 = bound, 2 = inner full storage.
+// int64 indices): part 0 = inner upper triangle, 1 = bound, 2 = inner full storage; parts 3 and 4
+// are parts 0 and 2 of the FP32 values, widened.
 int nlf_matrix_export(nlf_matrix a, int part, int64_t* outer, int64_t* inner, double* values) {
     if (!a) return NLF_ERR_ARGUMENT;
     return guarded(a->ctx, [&] {
@@ -909,8 +1096,18 @@ int nlf_matrix_export(nlf_matrix a, int part, int64_t* outer, int64_t* inner, do
         download(dg.data(), a->diag.p, size_t(rows), s);
         std::vector<i32> c(size_t(a->nnz_full));
         download(c.data(), a->cols.p, c.size(), s);
+        NLF_CHECK(part >= 0 && part <= 4, NLF_ERR_ARGUMENT, "part must be 0..4");
         std::vector<double> v(size_t(a->nnz_full));
-        download(v.data(), a->val.p, v.size(), s);
+        if (part >= 3) {
+            NLF_CHECK(a->val32.p, NLF_ERR_STATE, "the matrix has no FP32 values");
+            std::vector<float> v32(v.size());
+            download(v32.data(), a->val32.p, v32.size(), s);
+            std::copy(v32.begin(), v32.end(), v.begin());
+            part -= 3;
+            if (part == 1) part = 2;
+        } else {
+            download(v.data(), a->val.p, v.size(), s);
+        }
         if (part == 2) {
             if (outer) std::copy(o.begin(), o.end(), outer);
             if (inner) for (size_t k = 0; k < c.size(); ++k) inner[k] = c[k];
@@ -929,6 +1126,13 @@ int nlf_matrix_export(nlf_matrix a, int part, int64_t* outer, int64_t* inner, do
     });
 }
 
+int nlf_matrix_pattern_info(nlf_matrix a, int64_t* groups, int64_t* entries) {
+    if (!a) return NLF_ERR_ARGUMENT;
+    if (groups) *groups = a->pstart.p ? a->pat_groups : 0;
+    if (entries) *entries = a->pstart.p ? a->pat_size : 0;
+    return NLF_OK;
+}
+
 // Device views of the full inner storage: outer i64[rows+1], cols i32, values f64, diag i64[rows].
 int nlf_matrix_device(nlf_matrix a, const int64_t** outer, const int32_t** cols, double** values, const int64_t** diag) {
     if (!a) return NLF_ERR_ARGUMENT;
@@ -944,15 +1148,8 @@ int nlf_matrix_add_pairs(nlf_matrix a, int64_t n, const int64_t* rc_host, const 
     if (!a) return NLF_ERR_ARGUMENT;
     return guarded(a->ctx, [&] {
         if (n <= 0) return;
-        hipStream_t s = a->ctx->stream;
-        DevBuf<i64> rc;
-        DevBuf<double> v;
-        upload(rc, rc_host, size_t(2 * n), s);
-        upload(v, v_host, size_t(n), s);
-        hipLaunchKernelGGL(k_add_pairs, dim3(blocks_for(n, 256)), dim3(256), 0, s, i64(n), rc.p, v.p, a->outer.p,
-                           a->cols.p, a->val.p);
-        NLF_HIP(hipGetLastError());
-        NLF_HIP(hipStreamSynchronize(s));
+        NLF_CHECK(rc_host && v_host, NLF_ERR_ARGUMENT, "null argument");
+        add_pairs_device(a, std::vector<i64>(rc_host, rc_host + 2 * n), std::vector<double>(v_host, v_host + n));
     });
 }
 
@@ -996,16 +1193,7 @@ int nlf_matrix_add_convection(nlf_matrix a, const nlf_boundary* bcs) {
         std::vector<i64> rc;
         std::vector<double> v;
         for (const auto& [k, x] : acc) { rc.push_back(k.first); rc.push_back(k.second); v.push_back(x); }
-        if (v.empty()) return;
-        hipStream_t s = a->ctx->stream;
-        DevBuf<i64> d_rc;
-        DevBuf<double> d_v;
-        upload(d_rc, rc.data(), rc.size(), s);
-        upload(d_v, v.data(), v.size(), s);
-        hipLaunchKernelGGL(k_add_pairs, dim3(blocks_for(i64(v.size()), 256)), dim3(256), 0, s, i64(v.size()), d_rc.p,
-                           d_v.p, a->outer.p, a->cols.p, a->val.p);
-        NLF_HIP(hipGetLastError());
-        NLF_HIP(hipStreamSynchronize(s));
+        add_pairs_device(a, rc, v);
     });
 }
 
@@ -1017,6 +1205,69 @@ int nlf_matrix_make_fp32(nlf_matrix a) {
         if (!a->val32.p) a->val32.alloc(size_t(std::max<i64>(a->nnz_full, 1)));
         hipLaunchKernelGGL(k_to_f32, dim3(2048), dim3(256), 0, s, a->nnz_full, a->val.p, a->val32.p);
         NLF_HIP(hipGetLastError());
+        a->val32_assembled = false;
+        a->split_ready = false;
+    });
+}
+
+int nlf_matrix_make_split(nlf_matrix a) {
+    if (!a) return NLF_ERR_ARGUMENT;
+    return guarded(a->ctx, [&] {
+        hipStream_t s = a->ctx->stream;
+        if (!a->val32.p) a->val32.alloc(size_t(std::max<i64>(a->nnz_full, 1)));
+        if (!a->val_lo.p) a->val_lo.alloc(size_t(std::max<i64>(a->nnz_full, 1)));
+        hipLaunchKernelGGL(k_to_split, dim3(2048), dim3(256), 0, s, a->nnz_full, a->val.p, a->val32.p, a->val_lo.p);
+        NLF_HIP(hipGetLastError());
+        a->val32_assembled = false;
+        a->split_ready = true;
+    });
+}
+
+// Distance of the FP32 values from the FP64 ones over the full storage: out[0] = max |v32 - v64|,
+// out[1] = max |v64|, out[2] = ||v32 - v64||_2, out[3] = ||v64||_2.
+int nlf_matrix_compare_fp32(nlf_matrix a, double* out) {
+    if (!a) return NLF_ERR_ARGUMENT;
+    return guarded(a->ctx, [&] {
+        NLF_CHECK(out, NLF_ERR_ARGUMENT, "null argument");
+        NLF_CHECK(a->val32.p, NLF_ERR_STATE, "the matrix has no FP32 values");
+        hipStream_t s = a->ctx->stream;
+        constexpr int B = 1024;
+        DevBuf<double> part{size_t(4 * B)};
+        hipLaunchKernelGGL(k_compare32, dim3(B), dim3(256), 0, s, a->nnz_full, a->val.p, a->val32.p, part.p);
+        NLF_HIP(hipGetLastError());
+        std::vector<double> h(4 * B);
+        download(h.data(), part.p, h.size(), s);
+        double m0 = 0.0, m1 = 0.0, s0 = 0.0, s1 = 0.0;
+        for (int b = 0; b < B; ++b) {
+            m0 = std::max(m0, h[4 * b]); m1 = std::max(m1, h[4 * b + 1]);
+            s0 += h[4 * b + 2]; s1 += h[4 * b + 3];
+        }
+        out[0] = m0; out[1] = m1; out[2] = std::sqrt(s0); out[3] = std::sqrt(s1);
+    });
+}
+
+// FP32 values of the same operator assembled in FP32: the element-pair contractions (MFMA for the
+// thermal Q4 blocks) and the accumulation run in FP32; Jacobian determinants, coordinates and the
+// cutoff decisions of phi stay FP64.  Later boundary additions are repeated.
+int nlf_matrix_assemble_fp32(nlf_matrix a) {
+    if (!a) return NLF_ERR_ARGUMENT;
+    return guarded(a->ctx, [&] {
+        NLF_CHECK(a->val.p && !a->groups.empty(), NLF_ERR_STATE, "the matrix has no FP64 assembly to repeat");
+        hipStream_t s = a->ctx->stream;
+        assemble_values<float>(a, s);
+        if (!a->extra_v.empty()) {
+            DevBuf<i64> d_rc;
+            DevBuf<double> d_v;
+            upload(d_rc, a->extra_rc.data(), a->extra_rc.size(), s);
+            upload(d_v, a->extra_v.data(), a->extra_v.size(), s);
+            const i64 n = i64(a->extra_v.size());
+            hipLaunchKernelGGL(k_add_pairs<float>, dim3(blocks_for(n, 256)), dim3(256), 0, s, n, d_rc.p, d_v.p,
+                               a->outer.p, a->cols.p, a->val32.p);
+            NLF_HIP(hipGetLastError());
+            NLF_HIP(hipStreamSynchronize(s));
+        }
+        a->val32_assembled = true;
+        a->split_ready = false;
     });
 }
 

@@ -58,7 +58,8 @@ template<int LPR, class V, bool DOT>
 __global__ __launch_bounds__(SPMV_THREADS) void k_bspmv(const i32* __restrict__ binst, const i64* __restrict__ brow0,
                                                         const i32* __restrict__ bslot, const i64* __restrict__ row_begin,
                                                         const i64* __restrict__ outer, const i32* __restrict__ cols,
-                                                        const int16_t* __restrict__ off16, const V* __restrict__ val,
+                                                        const int16_t* __restrict__ off16, const i64* __restrict__ pstart,
+                                                        const V* __restrict__ val,
                                                         const double* __restrict__ x, double* __restrict__ y,
                                                         double* __restrict__ partial, const BatchState* __restrict__ st) {
     __shared__ double sh[SPMV_THREADS / 64];
@@ -69,8 +70,9 @@ __global__ __launch_bounds__(SPMV_THREADS) void k_bspmv(const i32* __restrict__ 
     double acc = 0.0;
     if (!skip && r < row_begin[inst + 1]) {
         const i64 b = outer[r], e = outer[r + 1];
+        const i64 ob = pstart ? pstart[r] : 0;        // row-pattern dictionary (shared offsets)
         for (i64 p = b + sub; p < e; p += LPR) {
-            const i64 c = off16 ? r + i64(off16[p]) : i64(cols[p]);
+            const i64 c = off16 ? r + i64(off16[ob + p]) : i64(cols[p]);
             acc += double(val[p]) * x[c];
         }
     }
@@ -345,12 +347,13 @@ void batch_build_plan(BatchPlan& P, nlf_matrix A, double target, hipStream_t s) 
 template<class V, bool DOT>
 void batch_spmv(const BatchPlan& P, nlf_matrix A, const V* val, const double* x, double* y, double* partial,
                 const BatchState* st, hipStream_t s) {
+    const SpmvCols cs = spmv_cols(A);
     for (const auto& c : P.classes) {
         if (!c.blocks) continue;
 #define NLF_BSPMV_CASE(L)                                                                                            \
     case L:                                                                                                          \
         hipLaunchKernelGGL((k_bspmv<L, V, DOT>), dim3(c.blocks), dim3(SPMV_THREADS), 0, s, c.inst.p, c.row0.p, c.slot.p, \
-                           P.row_begin.p, A->outer.p, A->cols.p, A->off16.p, val, x, y, partial, st);                  \
+                           P.row_begin.p, A->outer.p, cs.cols, cs.off, cs.pstart, val, x, y, partial, st);            \
         break;
         switch (c.lpr) {
             NLF_BSPMV_CASE(4)
@@ -560,6 +563,8 @@ int nlf_batch_solve(nlf_batch bt, const double* b, double* x, double tol, int64_
     if (!bt || !b || !x) return NLF_ERR_ARGUMENT;
     return guarded(bt->ctx, [&] {
         const auto t0 = std::chrono::steady_clock::now();
+        NLF_CHECK(mode == NLF_MODE_FP64 || mode == NLF_MODE_MIXED, NLF_ERR_UNSUPPORTED,
+                  "batched solves run in NLF_MODE_FP64 or NLF_MODE_MIXED");
         std::vector<BatchState> h;
         i64 replays = 0;
         if (mode == NLF_MODE_MIXED) {

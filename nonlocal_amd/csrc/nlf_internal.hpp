@@ -136,6 +136,30 @@ __device__ __forceinline__ double influence(const InfluenceParams& f, double ax,
     return f.norm * exp(f.exp_q * exp_pow(d, f.p_int, f.p_i, f.p_d));
 }
 
+// FP32 phi for the FP32 assembly: the distance and the polynomial are evaluated in FP32; within
+// 4e-6 of the cutoff the FP64 evaluation decides, so that the support (and the discontinuity of
+// the constant influence) is the FP64 one.
+__device__ __forceinline__ float exp_pow32(float v, int is_int, int ei, double ed) {
+    if (!is_int) return powf(v, float(ed));
+    float r = 1.0f;
+    while (ei > 0) {
+        if (ei & 1) { r *= v; --ei; } else { v *= v; ei >>= 1; }
+    }
+    return r;
+}
+
+__device__ __forceinline__ float influence32(const InfluenceParams& f, double ax, double ay, double bx, double by) {
+    const float zx = float(ax - bx) * float(f.inv_r[0]), zy = float(ay - by) * float(f.inv_r[1]);
+    const float d = zx * zx + zy * zy;
+    if (fabsf(d - 1.0f) < 4e-6f) return float(influence(f, ax, ay, bx, by));
+    const float norm = float(f.norm);
+    if (f.fast) return d < 1.0f ? norm * (1.0f - d) : 0.0f;
+    if (f.kind == NLF_INFLUENCE_POLYNOMIAL)
+        return d < 1.0f ? norm * exp_pow32(1.0f - exp_pow32(d, f.p_int, f.p_i, f.p_d), f.q_int, f.q_i, f.q_d) : 0.0f;
+    if (f.kind == NLF_INFLUENCE_CONSTANT) return d <= 1.0f ? norm : 0.0f;
+    return norm * expf(float(f.exp_q) * exp_pow32(d, f.p_int, f.p_i, f.p_d));
+}
+
 struct GroupParams {                     // per element group, copied to device
     double p1;                           // local weight (1 for LOCAL groups)
     double c[3];                         // thermal conductivity components or Hooke {h11, h12, h66}
@@ -155,6 +179,7 @@ struct nlf_ctx_s {
     bool own_stream = false;
     std::string error;
     int assembly_pairs = 1;              // NLF_OPT_ASSEMBLY_PAIRS
+    int row_patterns = 1;                // NLF_OPT_ROW_PATTERNS
 };
 
 struct nlf_mesh_s {
@@ -199,16 +224,44 @@ struct nlf_matrix_s {
     nlf::DevBuf<nlf::i64> tpos;          // position of the transposed entry (filled on demand)
     nlf::DevBuf<double> val;             // full values
     nlf::DevBuf<float> val32;            // FP32 copy (mixed precision), on demand
+    nlf::DevBuf<int16_t> val_lo;         // val - val32 in units of 2^-16 ulp(val32) (split mode), on demand
     nlf::DevBuf<int16_t> off16;          // col - row per nonzero when every |col - row| < 2^15 (SpMV)
+    nlf::DevBuf<int16_t> pat;            // dictionary of the distinct off16 rows (SpMV), when it pays
+    nlf::DevBuf<nlf::i64> pstart;        // row r's offsets are pat[pstart[r] + p], p in [outer[r], outer[r+1])
+    nlf::i64 pat_size = 0, pat_groups = 0;
     nlf::DevBuf<nlf::i64> b_outer;       // bound part: rows+1
     nlf::DevBuf<nlf::i32> b_cols;
     nlf::DevBuf<double> b_val;
     nlf::DevBuf<uint8_t> inner;          // is_inner per row
     double seconds_pattern = 0.0, seconds_values = 0.0;
+    // what nlf_matrix_assemble_fp32 needs to repeat the value pass in FP32
+    int physics = 0;
+    bool only_local = false;
+    std::vector<nlf::GroupParams> groups;
+    std::vector<nlf::i64> extra_rc;      // pairs added after assembly (convection, nlf_matrix_add_pairs)
+    std::vector<double> extra_v;
+    bool val32_assembled = false;        // val32 holds FP32-assembled values (not the rounded copy)
+    bool split_ready = false;            // val32 + val_lo is the split of the current val
 };
 
 namespace nlf {
 // helpers implemented across translation units
+void build_row_patterns(nlf_matrix_s* a, hipStream_t s);    // fills a->pat / a->pstart when they pay
+// Same for any CSR with 16-bit offsets; returns whether the dictionary was kept.
+bool build_row_patterns(i64 rows, i64 nnz, const i64* outer, const int16_t* off16, DevBuf<int16_t>& pat,
+                        DevBuf<i64>& pstart, i64& entries, i64& groups, hipStream_t s);
+
+// Column source of the SpMV: the row-pattern dictionary, else the 16-bit offsets, else int32 columns.
+struct SpmvCols {
+    const i32* cols;
+    const int16_t* off;       // offsets (col - row); with pstart: the dictionary
+    const i64* pstart;
+};
+i64 spmv_stream_bytes(const nlf_matrix_s* a, size_t sv, size_t sx);   // HBM bytes of one SpMV
+inline SpmvCols spmv_cols(const nlf_matrix_s* a) {
+    if (a->pstart.p) return {a->cols.p, a->pat.p, a->pstart.p};
+    return {a->cols.p, a->off16.p, nullptr};
+}
 void exclusive_scan_i64(i64* data, i64 n, hipStream_t s);   // in place over n+1 entries, data[n] = total
 void set_error(nlf_ctx ctx, const std::string& msg);
 

@@ -52,6 +52,9 @@ struct LearnedGraph {
     DevBuf<i64> m_outer;
     DevBuf<i32> m_cols;
     DevBuf<int16_t> m_off16;
+    DevBuf<int16_t> m_pat;                      // row-pattern dictionary of M (when it pays)
+    DevBuf<i64> m_pstart;
+    i64 m_pat_size = 0, m_pat_groups = 0;
     DevBuf<float> m_val;
     double compile_seconds = 0.0;
     bool network_apply = false;                 // run the network kernels even when M is compiled
@@ -1138,7 +1141,7 @@ __global__ void k_lf_off16(i64 n, const i64* __restrict__ outer, const i32* __re
 }
 
 // Builds M for the weights w; leaves m_ready false when a node's sets exceed the LDS capacities.
-void learned_compile(LearnedGraph& g, const float* w, hipStream_t s) {
+void learned_compile(LearnedGraph& g, const float* w, hipStream_t s, bool ctx_row_patterns) {
     const auto t0 = std::chrono::steady_clock::now();
     g.m_ready = false;
     const i64 n = g.n;
@@ -1174,6 +1177,9 @@ void learned_compile(LearnedGraph& g, const float* w, hipStream_t s) {
     if (3 * g.band < 32768) {
         g.m_off16.alloc(size_t(std::max<i64>(nnz, 1)));
         hipLaunchKernelGGL(k_lf_off16, dim3(blocks_for(n, 4)), dim3(256), 0, s, n, g.m_outer.p, g.m_cols.p, g.m_off16.p);
+        NLF_HIP(hipGetLastError());
+        if (ctx_row_patterns) build_row_patterns(n, nnz, g.m_outer.p, g.m_off16.p, g.m_pat, g.m_pstart, g.m_pat_size,
+                                                 g.m_pat_groups, s);
     }
     NLF_HIP(hipGetLastError());
     NLF_HIP(hipStreamSynchronize(s));
@@ -1213,8 +1219,10 @@ void learned_apply_ell(LearnedGraph& g, const float* w, const double* r, Z* z, h
 template<class Z>
 void learned_apply(LearnedGraph& g, const float* w, const double* r, Z* z, hipStream_t s) {
     if (g.m_ready && !g.network_apply) {
-        const int lpr = pick_lpr(double(g.m_nnz) / double(std::max<i64>(g.n, 1)));
-        launch_spmv<float, double, double, false>(lpr, g.n, g.m_outer.p, g.m_cols.p, g.m_off16.p, g.m_val.p, r, z, nullptr,
+        const int lpr = pick_lpr(double(g.m_nnz) / double(std::max<i64>(g.n, 1)), g.m_off16.p != nullptr);
+        const SpmvCols cs = g.m_pstart.p ? SpmvCols{g.m_cols.p, g.m_pat.p, g.m_pstart.p}
+                                         : SpmvCols{g.m_cols.p, g.m_off16.p, nullptr};
+        launch_spmv<float, double, double, false>(lpr, g.n, g.m_outer.p, cs, g.m_val.p, r, z, nullptr,
                                                   nullptr, s);
         return;
     }
@@ -1232,8 +1240,8 @@ double learned_loss_grad(LearnedGraph& g, const float* w, nlf_matrix A, const do
     learned_alloc_train(g);
     const i64 n = g.n;
     learned_forward_train(g, w, r, g.zbuf.p, s);
-    const int lpr = pick_lpr(double(A->nnz_full) / double(std::max<i64>(A->rows, 1)));
-    launch_spmv<double, double, double, false>(lpr, n, A->outer.p, A->cols.p, A->off16.p, A->val.p, g.zbuf.p, g.az.p, nullptr,
+    const int lpr = pick_lpr(A);
+    launch_spmv<double, double, double, false>(lpr, n, A->outer.p, spmv_cols(A), A->val.p, g.zbuf.p, g.az.p, nullptr,
                                                nullptr, s);
     const unsigned vb = grid_stride_blocks(n);
     DevBuf<double> p1{vb}, p2{vb}, sums{2}, gz{size_t(n)};
@@ -1347,10 +1355,10 @@ __global__ void k_fcg_p(i64 n, double* __restrict__ p, const double* __restrict_
 
 template<class V>
 i64 fcg(hipStream_t s, nlf_matrix A, const V* val, nlf_precond M, const double* b, double* x, double tol, i64 max_it,
-        double* true_res, i64* updates, int window, KernelTimer& timer) {
+        double* true_res, i64* updates, int window, KernelTimer& timer, const int16_t* lo = nullptr) {
     const i64 n = A->rows;
     LearnedGraph& g = *learned_of(M);
-    const int lpr = pick_lpr(double(A->nnz_full) / double(std::max<i64>(n, 1)));
+    const int lpr = pick_lpr(A);
     const unsigned sb = spmv_blocks(lpr, n), vb = grid_stride_blocks(n), rb = spmv_blocks(64, n);
     const size_t np = std::max<size_t>(std::max<size_t>(sb, vb), rb);
     DevBuf<double> r{size_t(n)}, z{size_t(n)}, p{size_t(n)}, q{size_t(n)}, pa{np}, pb{np}, sums{2};
@@ -1366,7 +1374,7 @@ i64 fcg(hipStream_t s, nlf_matrix A, const V* val, nlf_precond M, const double* 
     };
     auto spmv_pq = [&](bool sample) {             // q = A p, partial p.q into pb; partial p.r into pa
         if (sample) timer.start(0, s);
-        launch_spmv<V, double, double, true>(lpr, n, A->outer.p, A->cols.p, A->off16.p, val, p.p, q.p, pb.p, nullptr, s);
+        launch_spmv<V, double, double, true>(lpr, n, A->outer.p, spmv_cols(A), val, p.p, q.p, pb.p, nullptr, s, false, lo);
         if (sample) timer.stop(0, s);
         hipLaunchKernelGGL(k_dot<double>, dim3(sb), dim3(SPMV_THREADS), 0, s, n, p.p, r.p, pa.p, (const CGState*)nullptr);
     };
@@ -1530,7 +1538,7 @@ int nlf_learned_set_weights(nlf_precond p, const float* w) {
     return guarded(p->ctx, [&] {
         LearnedGraph& g = *learned_of(p);
         NLF_HIP(hipMemcpyAsync(g.w.p, w, NPARAM * sizeof(float), hipMemcpyHostToDevice, p->ctx->stream));
-        learned_compile(g, g.w.p, p->ctx->stream);
+        learned_compile(g, g.w.p, p->ctx->stream, p->ctx->row_patterns != 0);
         NLF_HIP(hipStreamSynchronize(p->ctx->stream));
     });
 }
@@ -1625,7 +1633,8 @@ int nlf_trainer_get_weights(nlf_trainer t, float* w) {
 // Flexible CG with the learned preconditioner M (taken first).  If FCG stagnates or produces a
 // non-finite value, the solve continues from the current iterate with standard PCG using
 // `fallback` (ILLT or Jacobi; NULL = Jacobi-free identity).  mode NLF_MODE_MIXED reads FP32
-// matrix values in the Krylov SpMV (nlf_matrix_make_fp32 first).
+// matrix values in the Krylov SpMV (nlf_matrix_make_fp32 first), NLF_MODE_SPLIT FP32 + 16-bit
+// remainders (nlf_matrix_make_split first); the FP64 true residual decides in every mode.
 int nlf_fcg_solve(nlf_matrix A, nlf_precond M, nlf_precond fallback, const double* b, double* x, double tol,
                   int64_t max_it, int mode, nlf_solve_stats* stats) {
     if (!A || !M) return NLF_ERR_ARGUMENT;
@@ -1640,13 +1649,18 @@ int nlf_fcg_solve(nlf_matrix A, nlf_precond M, nlf_precond fallback, const doubl
         if (mode == NLF_MODE_MIXED) {
             NLF_CHECK(A->val32.p, NLF_ERR_STATE, "mixed mode needs nlf_matrix_make_fp32 first");
             it = fcg<float>(s, A, A->val32.p, M, b, x, tol, max_it, &out.true_residual, &out.outer_iterations, FCG_WINDOW, timer);
+        } else if (mode == NLF_MODE_SPLIT) {
+            NLF_CHECK(A->split_ready, NLF_ERR_STATE, "split mode needs nlf_matrix_make_split first");
+            it = fcg<float>(s, A, A->val32.p, M, b, x, tol, max_it, &out.true_residual, &out.outer_iterations, FCG_WINDOW, timer,
+                            A->val_lo.p);
         } else {
+            NLF_CHECK(mode == NLF_MODE_FP64, NLF_ERR_ARGUMENT, "unknown mode");
             it = fcg<double>(s, A, A->val.p, M, b, x, tol, max_it, &out.true_residual, &out.outer_iterations, FCG_WINDOW, timer);
         }
         out.spmv_ms = timer.avg(0);
         out.precond_ms = timer.avg(1);
         out.spmv_samples = timer.count[0];
-        out.spmv_bytes = A->nnz_full * i64((mode == NLF_MODE_MIXED ? 4 : 8) + (A->off16.p ? 2 : 4)) + A->rows * i64(8 + 16);
+        out.spmv_bytes = spmv_stream_bytes(A, mode == NLF_MODE_MIXED ? 4 : mode == NLF_MODE_SPLIT ? 6 : 8, 8);
         if (it < 0) {
             // stagnation: fall back to PCG with the fallback preconditioner from the current iterate
             out.fallbacks = 1;
@@ -1655,7 +1669,7 @@ int nlf_fcg_solve(nlf_matrix A, nlf_precond M, nlf_precond fallback, const doubl
             // the PCG tolerance is tightened tenfold (at most three times) until the true residual meets tol
             int breakdown = 0;
             CG<double, double> cg(s, A->rows, A->nnz_full, A->outer.p, A->cols.p, A->val.p, fallback);
-            cg.off16 = A->off16.p;
+            cg.set_cols(A);
             const unsigned rb = spmv_blocks(64, A->rows);
             DevBuf<double> r{size_t(A->rows)}, prr{rb}, pbb{rb}, sums{2};
             i64 left = max_it + it, total = -it;

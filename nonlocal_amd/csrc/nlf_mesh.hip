@@ -205,11 +205,14 @@ int nlf_ctx_destroy(nlf_ctx ctx) {
 const char* nlf_last_error(nlf_ctx ctx) { return ctx ? ctx->error.c_str() : "null context"; }
 
 // Context options (no reference counterpart): NLF_OPT_ASSEMBLY_PAIRS 1 = MFMA element-pair
-// contractions where implemented (thermal Q4, default), 0 = node-centric FP64 contraction.
+// contractions where implemented (thermal Q4, default), 0 = node-centric contraction;
+// NLF_OPT_ROW_PATTERNS 1 = shared row-pattern dictionary for the SpMV of matrices assembled
+// afterwards (default), 0 = per-nonzero 16-bit offsets.
 int nlf_ctx_set_option(nlf_ctx ctx, int option, int value) {
     if (!ctx) return NLF_ERR_ARGUMENT;
-    if (option != NLF_OPT_ASSEMBLY_PAIRS) return NLF_ERR_ARGUMENT;
-    ctx->assembly_pairs = value != 0;
+    if (option == NLF_OPT_ASSEMBLY_PAIRS) ctx->assembly_pairs = value != 0;
+    else if (option == NLF_OPT_ROW_PATTERNS) ctx->row_patterns = value != 0;
+    else return NLF_ERR_ARGUMENT;
     return NLF_OK;
 }
 

@@ -55,28 +55,64 @@ __host__ __device__ inline double from_ordered_bits(unsigned long long o) {
 
 // max (or min) over element-node-adjacent pairs of |c_e - c_a|^2 per group
 // (diameter_between_elements_centers, find_neighbours.hpp:49-72).
+// Wave-level pre-reduction for the per-group atomics below: when every active lane of a wave
+// works on the same group (the usual case: groups are contiguous element ranges), the wave
+// reduces first and one lane issues the atomic, instead of 64 atomics on one address.
+__device__ __forceinline__ bool wave_same_group(int g, bool act, int& lead) {
+    const unsigned long long m = __ballot(act);
+    if (!m) { lead = -1; return true; }
+    lead = __ffsll(static_cast<long long>(m)) - 1;
+    const int gl = __shfl(g, lead);
+    return __ballot(act && g != gl) == 0;
+}
+
+template<bool MIN>
+__device__ __forceinline__ unsigned long long wave_reduce_u64(unsigned long long v) {
+    for (int o = 32; o > 0; o >>= 1) {
+        const unsigned long long w = __shfl_xor(v, o);
+        v = MIN ? (w < v ? w : v) : (w > v ? w : v);
+    }
+    return v;
+}
+
 __global__ void k_expand(i64 E, int nn, int strategy, const i32* __restrict__ el, const i32* __restrict__ group_of,
                          const int* __restrict__ searched, const i32* __restrict__ ne_off, const i32* __restrict__ ne_ids,
                          const double* __restrict__ c, unsigned long long* __restrict__ out) {
     const i64 e = i64(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (e >= E) return;
-    const int g = group_of[e];
-    if (!searched[g]) return;
-    double best = strategy == NLF_EXPAND_MIN ? 1.0e308 : 0.0;
-    const double cx = c[2 * e], cy = c[2 * e + 1];
-    for (int i = 0; i < nn; ++i) {
-        const i32 v = el[e * nn + i];
-        for (i32 k = ne_off[v]; k < ne_off[v + 1]; ++k) {
-            const i32 a = ne_ids[k];
-            const double z0 = cx - c[2 * a], z1 = cy - c[2 * a + 1];
-            double s = 0.0;
-            s = s + z0 * z0;
-            s = s + z1 * z1;
-            best = strategy == NLF_EXPAND_MIN ? fmin(best, s) : fmax(best, s);
-        }
+    const bool is_min = strategy == NLF_EXPAND_MIN;
+    int g = -1;
+    bool act = false;
+    if (e < E) {
+        g = group_of[e];
+        act = searched[g] != 0;
     }
-    const unsigned long long bits = __double_as_longlong(best);   // non-negative: bit order == value order
-    if (strategy == NLF_EXPAND_MIN) atomicMin(&out[g], bits); else atomicMax(&out[g], bits);
+    // non-negative doubles: bit order == value order; identities for the wave reduction
+    unsigned long long bits = is_min ? ~0ull : 0ull;
+    if (act) {
+        double best = is_min ? 1.0e308 : 0.0;
+        const double cx = c[2 * e], cy = c[2 * e + 1];
+        for (int i = 0; i < nn; ++i) {
+            const i32 v = el[e * nn + i];
+            for (i32 k = ne_off[v]; k < ne_off[v + 1]; ++k) {
+                const i32 a = ne_ids[k];
+                const double z0 = cx - c[2 * a], z1 = cy - c[2 * a + 1];
+                double s = 0.0;
+                s = s + z0 * z0;
+                s = s + z1 * z1;
+                best = is_min ? fmin(best, s) : fmax(best, s);
+            }
+        }
+        bits = __double_as_longlong(best);
+    }
+    int lead;
+    if (wave_same_group(g, act, lead)) {
+        if (lead < 0) return;
+        bits = is_min ? wave_reduce_u64<true>(bits) : wave_reduce_u64<false>(bits);
+        if (int(threadIdx.x & 63) != lead) return;
+    } else if (!act) {
+        return;
+    }
+    if (is_min) atomicMin(&out[g], bits); else atomicMax(&out[g], bits);
 }
 
 // group_corners (find_neighbours.hpp:40-47): min/max over the nodes of the group's elements.
@@ -84,19 +120,37 @@ __global__ void k_bbox(i64 E, int nn, const i32* __restrict__ el, const i32* __r
                        const int* __restrict__ searched, const double* __restrict__ xy,
                        unsigned long long* __restrict__ box) {
     const i64 e = i64(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (e >= E) return;
-    const int g = group_of[e];
-    if (!searched[g]) return;
-    double l = 1e308, d = 1e308, r = -1e308, u = -1e308;
-    for (int i = 0; i < nn; ++i) {
-        const i32 v = el[e * nn + i];
-        l = fmin(l, xy[2 * v]); d = fmin(d, xy[2 * v + 1]);
-        r = fmax(r, xy[2 * v]); u = fmax(u, xy[2 * v + 1]);
+    int g = -1;
+    bool act = false;
+    if (e < E) {
+        g = group_of[e];
+        act = searched[g] != 0;
     }
-    atomicMin(&box[4 * g + 0], ordered_bits(l));
-    atomicMin(&box[4 * g + 1], ordered_bits(d));
-    atomicMax(&box[4 * g + 2], ordered_bits(r));
-    atomicMax(&box[4 * g + 3], ordered_bits(u));
+    unsigned long long lo_x = ~0ull, lo_y = ~0ull, hi_x = 0ull, hi_y = 0ull;   // ordered bits
+    if (act) {
+        double l = 1e308, d = 1e308, r = -1e308, u = -1e308;
+        for (int i = 0; i < nn; ++i) {
+            const i32 v = el[e * nn + i];
+            l = fmin(l, xy[2 * v]); d = fmin(d, xy[2 * v + 1]);
+            r = fmax(r, xy[2 * v]); u = fmax(u, xy[2 * v + 1]);
+        }
+        lo_x = ordered_bits(l); lo_y = ordered_bits(d); hi_x = ordered_bits(r); hi_y = ordered_bits(u);
+    }
+    int lead;
+    if (wave_same_group(g, act, lead)) {
+        if (lead < 0) return;
+        lo_x = wave_reduce_u64<true>(lo_x);
+        lo_y = wave_reduce_u64<true>(lo_y);
+        hi_x = wave_reduce_u64<false>(hi_x);
+        hi_y = wave_reduce_u64<false>(hi_y);
+        if (int(threadIdx.x & 63) != lead) return;
+    } else if (!act) {
+        return;
+    }
+    atomicMin(&box[4 * g + 0], lo_x);
+    atomicMin(&box[4 * g + 1], lo_y);
+    atomicMax(&box[4 * g + 2], hi_x);
+    atomicMax(&box[4 * g + 3], hi_y);
 }
 
 __device__ __forceinline__ void cell_of(const SearchGroup& sg, double cx, double cy, i64& a, i64& b) {

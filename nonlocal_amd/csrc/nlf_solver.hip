@@ -58,9 +58,19 @@ __device__ __forceinline__ Acc block_sum(Acc v, Acc* sh) {
 template<class T>
 __device__ __forceinline__ T stream_load(const T* p, bool nt) { return nt ? __builtin_nontemporal_load(p) : *p; }
 
-template<int LPR, class V, class X, class Acc, bool DOT, bool XCD = false, int UNROLL = 4, bool NT = false>
+// Split values: hi (FP32) + lo * 2^-16 ulp(hi), lo a 16-bit integer (40 significand bits).
+__device__ __forceinline__ double split_value(float h, int16_t l) {
+    unsigned e = (__float_as_uint(h) >> 23) & 0xffu;
+    e = e ? e : 1u;
+    return double(h) + double(l) * __longlong_as_double(static_cast<long long>(e + 857u) << 52);
+}
+
+// SPLIT (V = float): each value is split_value(val[p], lo[p]), summed in Acc.
+template<int LPR, class V, class X, class Acc, bool DOT, bool XCD = false, int UNROLL = 4, bool NT = false,
+         bool SPLIT = false>
 __global__ __launch_bounds__(SPMV_THREADS) void k_spmv(i64 rows, const i64* __restrict__ outer, const i32* __restrict__ cols,
-                                                       const int16_t* __restrict__ off16, const V* __restrict__ val,
+                                                       const int16_t* __restrict__ off16, const i64* __restrict__ pstart,
+                                                       const V* __restrict__ val, const int16_t* __restrict__ lo,
                                                        const X* __restrict__ x, X* __restrict__ y,
                                                        double* __restrict__ partial, const CGState* __restrict__ st) {
     __shared__ double sh[SPMV_THREADS / 64];
@@ -78,27 +88,112 @@ __global__ __launch_bounds__(SPMV_THREADS) void k_spmv(i64 rows, const i64* __re
         Acc acc = 0;
         if (r < rows) {
             const i64 b = outer[r], e = outer[r + 1];
+            // offsets of row r: off16[ob + p] (ob = 0 for per-nonzero offsets; with the row-pattern
+            // dictionary ob moves p into the row's shared pattern, which stays in L2)
+            const i64 ob = pstart ? pstart[r] : 0;
+            const bool nt_off = NT && !pstart;
             i64 p = b + sub;
             for (; p + (UNROLL - 1) * LPR < e; p += UNROLL * LPR) {
                 i64 c[UNROLL];
                 V v[UNROLL];
+                int16_t vl[UNROLL];
 #pragma unroll
                 for (int t = 0; t < UNROLL; ++t) {
-                    c[t] = off16 ? r + i64(stream_load(off16 + p + t * LPR, NT)) : i64(stream_load(cols + p + t * LPR, NT));
+                    c[t] = off16 ? r + i64(stream_load(off16 + ob + p + t * LPR, nt_off))
+                                 : i64(stream_load(cols + p + t * LPR, NT));
                     v[t] = stream_load(val + p + t * LPR, NT);
+                    if constexpr (SPLIT) vl[t] = lo[p + t * LPR];
                 }
                 X xv[UNROLL];
 #pragma unroll
                 for (int t = 0; t < UNROLL; ++t) xv[t] = x[c[t]];
 #pragma unroll
-                for (int t = 0; t < UNROLL; ++t) acc += Acc(v[t]) * Acc(xv[t]);
+                for (int t = 0; t < UNROLL; ++t) {
+                    if constexpr (SPLIT) acc += Acc(split_value(v[t], vl[t])) * Acc(xv[t]);
+                    else acc += Acc(v[t]) * Acc(xv[t]);
+                }
             }
             for (; p < e; p += LPR) {
-                const i64 c = off16 ? r + i64(stream_load(off16 + p, NT)) : i64(stream_load(cols + p, NT));
-                acc += Acc(stream_load(val + p, NT)) * Acc(x[c]);
+                const i64 c = off16 ? r + i64(stream_load(off16 + ob + p, nt_off)) : i64(stream_load(cols + p, NT));
+                if constexpr (SPLIT) acc += Acc(split_value(stream_load(val + p, NT), lo[p])) * Acc(x[c]);
+                else acc += Acc(stream_load(val + p, NT)) * Acc(x[c]);
             }
         }
         for (int o = LPR / 2; o > 0; o >>= 1) acc += shfl_down_t(acc, o);   // lane sub==0 of each group holds the sum
+        if (r < rows && sub == 0) {
+            y[r] = X(acc);
+            if (DOT) d += double(x[r]) * double(X(acc));
+        }
+    }
+    if (DOT) {
+        d = block_sum<double>(d, sh);
+        if (threadIdx.x == 0) partial[blockIdx.x] = d;
+    }
+}
+
+// Pair SpMV over 16-bit offsets (per nonzero, or the row-pattern dictionary whose row starts are
+// even, see nlf_rowpat.hip): each lane takes two consecutive nonzeros per load, so one 2-value load
+// and one 4-byte offset load replace two of each; two pairs stay in flight per lane.  A row that
+// starts at an odd index handles its first nonzero apart, and an odd last one at the end, so
+// every pair load is aligned.  The products are summed in the same order by every launch of a
+// given LPR (results are reproducible, not bitwise equal to k_spmv's).
+template<class V> struct Pair;
+template<> struct Pair<double> { typedef double T __attribute__((ext_vector_type(2))); };
+template<> struct Pair<float> { typedef float T __attribute__((ext_vector_type(2))); };
+
+template<int LPR, class V, class X, class Acc, bool DOT, bool SPLIT = false>
+__global__ __launch_bounds__(SPMV_THREADS) void k_spmv2(i64 rows, const i64* __restrict__ outer,
+                                                        const int16_t* __restrict__ off, const i64* __restrict__ pstart,
+                                                        const V* __restrict__ val, const int16_t* __restrict__ lo,
+                                                        const X* __restrict__ x, X* __restrict__ y,
+                                                        double* __restrict__ partial, const CGState* __restrict__ st) {
+    typedef typename Pair<V>::T V2;
+    __shared__ double sh[SPMV_THREADS / 64];
+    if (st && st->done) return;
+    constexpr int RPB = SPMV_THREADS / LPR;
+    const int sub = threadIdx.x % LPR;
+    double d = 0.0;
+    auto value = [&](V v, int16_t l) -> Acc {
+        if constexpr (SPLIT) return Acc(split_value(v, l));
+        else return Acc(v);
+    };
+    auto lo_pair = [&](i64 p) -> int {
+        if constexpr (SPLIT) return *reinterpret_cast<const int*>(lo + p);
+        else return 0;
+    };
+    const i64 rows_r = (rows + RPB - 1) / RPB * RPB;
+    for (i64 r0 = i64(blockIdx.x) * RPB; r0 < rows_r; r0 += i64(gridDim.x) * RPB) {
+        const i64 r = r0 + threadIdx.x / LPR;
+        Acc acc = 0;
+        if (r < rows) {
+            const i64 b = outer[r], e = outer[r + 1];
+            const int16_t* o = off + (pstart ? pstart[r] : 0);
+            if ((b & 1) && sub == 0 && b < e)
+                acc += value(val[b], SPLIT ? lo[b] : int16_t(0)) * Acc(x[r + o[b]]);
+            i64 p = ((b + 1) & ~i64(1)) + 2 * sub;
+            for (; p + 2 * LPR + 1 < e; p += 4 * LPR) {
+                const V2 v0 = *reinterpret_cast<const V2*>(val + p);
+                const V2 v1 = *reinterpret_cast<const V2*>(val + p + 2 * LPR);
+                const int c0 = *reinterpret_cast<const int*>(o + p);
+                const int c1 = *reinterpret_cast<const int*>(o + p + 2 * LPR);
+                const int l0 = lo_pair(p), l1 = lo_pair(p + 2 * LPR);
+                const X x0 = x[r + int16_t(c0 & 0xffff)], x1 = x[r + int16_t(c0 >> 16)];
+                const X x2 = x[r + int16_t(c1 & 0xffff)], x3 = x[r + int16_t(c1 >> 16)];
+                acc += value(v0.x, int16_t(l0 & 0xffff)) * Acc(x0);
+                acc += value(v0.y, int16_t(l0 >> 16)) * Acc(x1);
+                acc += value(v1.x, int16_t(l1 & 0xffff)) * Acc(x2);
+                acc += value(v1.y, int16_t(l1 >> 16)) * Acc(x3);
+            }
+            for (; p + 1 < e; p += 2 * LPR) {
+                const V2 v0 = *reinterpret_cast<const V2*>(val + p);
+                const int c0 = *reinterpret_cast<const int*>(o + p);
+                const int l0 = lo_pair(p);
+                acc += value(v0.x, int16_t(l0 & 0xffff)) * Acc(x[r + int16_t(c0 & 0xffff)]);
+                acc += value(v0.y, int16_t(l0 >> 16)) * Acc(x[r + int16_t(c0 >> 16)]);
+            }
+            if (p < e) acc += value(val[p], SPLIT ? lo[p] : int16_t(0)) * Acc(x[r + o[p]]);
+        }
+        for (int k = LPR / 2; k > 0; k >>= 1) acc += shfl_down_t(acc, k);
         if (r < rows && sub == 0) {
             y[r] = X(acc);
             if (DOT) d += double(x[r]) * double(X(acc));
@@ -854,12 +949,20 @@ void precond_apply<float>(const nlf_precond_s* P, i64 n, const float* r, float* 
         NLF_HIP(hipMemcpyAsync(z, r, size_t(n) * sizeof(float), hipMemcpyDeviceToDevice, s));
 }
 
-// Lanes per row: the smallest power of two giving each lane at most 16 nonzeros (4..64).  Measured on
-// the 1000 x 1000 composite (500 nonzeros per row): 32 lanes 1.19 ms, 64 lanes 1.29 ms, 16 lanes 1.31 ms.
-int pick_lpr(double avg) {
+// Lanes per row: the smallest power of two (4..64) giving each lane at most 16 nonzeros for the
+// one-nonzero-per-load kernel, 64 for the pair kernel (16-bit offsets).  Measured on the 1000 x 1000
+// composite (500 nonzeros per row, FP64): one-per-load 32 lanes 1.19 ms, 64 lanes 1.29 ms, 16 lanes
+// 1.31 ms; pair kernel with the row-pattern dictionary 8 lanes 0.905 ms, 16 lanes 0.93 ms, 32 lanes
+// 1.00 ms, 64 lanes 1.12 ms.
+int pick_lpr(double avg, bool pairs = false) {
+    const double per_lane = pairs ? 64.0 : 16.0;
     int l = 4;
-    while (l < 64 && double(l) * 16.0 < avg) l *= 2;
+    while (l < 64 && double(l) * per_lane < avg) l *= 2;
     return l;
+}
+
+int pick_lpr(const nlf_matrix_s* a) {
+    return pick_lpr(double(a->nnz_full) / double(std::max<i64>(a->rows, 1)), a->off16.p != nullptr);
 }
 
 // SpMV grid: one row group per block iteration, capped so that DOT partials stay <= SPMV_MAX_BLOCKS
@@ -868,15 +971,73 @@ unsigned spmv_blocks(int lpr, i64 rows) {
     return unsigned(std::min<i64>(std::max<i64>((rows + (SPMV_THREADS / lpr) - 1) / (SPMV_THREADS / lpr), 1), SPMV_MAX_BLOCKS));
 }
 
+// cs: column source (spmv_cols of the matrix, or plain offsets / columns with cs.pstart = null).
+// lo: 16-bit remainders of split values (V = float, NLF_MODE_SPLIT), else null.  With 16-bit
+// offsets (per nonzero or the dictionary) the pair kernel k_spmv2 runs unless pairs = false or an
+// XCD-contiguous numbering is asked for.
 template<class V, class X, class Acc, bool DOT>
-void launch_spmv(int lpr, i64 rows, const i64* outer, const i32* cols, const int16_t* off16, const V* val, const X* x, X* y,
-                 double* partial, const CGState* st, hipStream_t s, bool xcd = false) {
+void launch_spmv(int lpr, i64 rows, const i64* outer, SpmvCols cs, const V* val, const X* x, X* y,
+                 double* partial, const CGState* st, hipStream_t s, bool xcd = false, const int16_t* lo = nullptr,
+                 bool pairs = true) {
+    const i32* cols = cs.cols;
+    const int16_t* off16 = cs.off;
+    const i64* pstart = cs.pstart;
     const unsigned blocks = spmv_blocks(lpr, rows);
+    if (off16 && pairs && !xcd) {
+        if (lo) {
+            if constexpr (sizeof(V) == sizeof(float)) {
+#define NLF_SPMV2_SCASE(L)                                                                                         \
+    case L:                                                                                                        \
+        hipLaunchKernelGGL((k_spmv2<L, V, X, Acc, DOT, true>), dim3(blocks), dim3(SPMV_THREADS), 0, s, rows, outer, off16, \
+                           pstart, val, lo, x, y, partial, st);                                                    \
+        return;
+                switch (lpr) {
+                    NLF_SPMV2_SCASE(4)
+                    NLF_SPMV2_SCASE(8)
+                    NLF_SPMV2_SCASE(16)
+                    NLF_SPMV2_SCASE(32)
+                    default: NLF_SPMV2_SCASE(64)
+                }
+#undef NLF_SPMV2_SCASE
+            }
+        } else {
+#define NLF_SPMV2_CASE(L)                                                                                          \
+    case L:                                                                                                        \
+        hipLaunchKernelGGL((k_spmv2<L, V, X, Acc, DOT>), dim3(blocks), dim3(SPMV_THREADS), 0, s, rows, outer, off16,       \
+                           pstart, val, static_cast<const int16_t*>(nullptr), x, y, partial, st);                 \
+        return;
+            switch (lpr) {
+                NLF_SPMV2_CASE(4)
+                NLF_SPMV2_CASE(8)
+                NLF_SPMV2_CASE(16)
+                NLF_SPMV2_CASE(32)
+                default: NLF_SPMV2_CASE(64)
+            }
+#undef NLF_SPMV2_CASE
+        }
+    }
+    if constexpr (sizeof(V) == sizeof(float)) {
+        if (lo) {
+#define NLF_SPMV_SCASE(L)                                                                                          \
+    case L:                                                                                                        \
+        hipLaunchKernelGGL((k_spmv<L, V, X, Acc, DOT, false, 4, false, true>), dim3(blocks), dim3(SPMV_THREADS), 0, s,  \
+                           rows, outer, cols, off16, pstart, val, lo, x, y, partial, st);                          \
+        return;
+            switch (lpr) {
+                NLF_SPMV_SCASE(4)
+                NLF_SPMV_SCASE(8)
+                NLF_SPMV_SCASE(16)
+                NLF_SPMV_SCASE(32)
+                default: NLF_SPMV_SCASE(64)
+            }
+#undef NLF_SPMV_SCASE
+        }
+    }
     if (xcd) {
 #define NLF_SPMV_XCASE(L)                                                                                          \
     case L:                                                                                                        \
         hipLaunchKernelGGL((k_spmv<L, V, X, Acc, DOT, true>), dim3(blocks), dim3(SPMV_THREADS), 0, s, rows, outer, cols, \
-                           off16, val, x, y, partial, st);                                                         \
+                           off16, pstart, val, static_cast<const int16_t*>(nullptr), x, y, partial, st);          \
         return;
         switch (lpr) {
             NLF_SPMV_XCASE(4)
@@ -890,7 +1051,7 @@ void launch_spmv(int lpr, i64 rows, const i64* outer, const i32* cols, const int
 #define NLF_SPMV_CASE(L)                                                                                           \
     case L:                                                                                                        \
         hipLaunchKernelGGL((k_spmv<L, V, X, Acc, DOT>), dim3(blocks), dim3(SPMV_THREADS), 0, s, rows, outer, cols, off16, \
-                           val, x, y, partial, st);                                                                \
+                           pstart, val, static_cast<const int16_t*>(nullptr), x, y, partial, st);                 \
         break;
     switch (lpr) {
         NLF_SPMV_CASE(4)
@@ -947,8 +1108,9 @@ struct CG {
     int lpr;
     const i64* outer;
     const i32* cols;
-    const int16_t* off16 = nullptr;     // 16-bit column offsets of the matrix, when it has them
+    SpmvCols cs{};                      // column source of the SpMV (spmv_cols of the matrix)
     const V* val;
+    const int16_t* lo = nullptr;        // split values: 16-bit remainders (NLF_MODE_SPLIT)
     const nlf_precond_s* P;
     DevBuf<X> r, z, p;
     DevBuf<double> part1, part2;
@@ -958,6 +1120,7 @@ struct CG {
 
     CG(hipStream_t s_, i64 n_, i64 nnz, const i64* o, const i32* c, const V* v, const nlf_precond_s* P_)
         : s(s_), n(n_), outer(o), cols(c), val(v), P(P_) {
+        cs.cols = c;
         lpr = pick_lpr(double(nnz) / double(std::max<i64>(n, 1)));
         r.alloc(size_t(n)); z.alloc(size_t(n)); p.alloc(size_t(n));
         sblocks = spmv_blocks(lpr, n);
@@ -970,6 +1133,13 @@ struct CG {
     }
 
     bool jacobi() const { return P && P->kind == NLF_PRECOND_JACOBI; }
+
+    // Column source of the matrix; the lane count follows the kernel it selects.
+    void set_cols(const nlf_matrix_s* a) {
+        cs = spmv_cols(a);
+        lpr = pick_lpr(a);
+        sblocks = spmv_blocks(lpr, n);
+    }
 
     const X* invd() const;
 
@@ -1044,13 +1214,17 @@ struct CG {
             NLF_HIP(hipMemcpyAsync(&hs, st.p, sizeof(CGState), hipMemcpyDeviceToHost, s));
             NLF_HIP(hipStreamSynchronize(s));
             double last = std::sqrt(std::fabs(hs.rz));
+            // the recursive estimate sqrt(r'z) is preconditioned; q maps it to the relative true
+            // residual as measured at the last update, so that the true residual is also checked
+            // once it is predicted to meet tol (not only at the next tenfold drop)
+            double q = last > 0.0 ? rel / last : 0.0;
             for (;;) {
                 for (int k = 0; k < 8; ++k) iteration(xh.p, k == 0);
                 NLF_HIP(hipMemcpyAsync(&hs, st.p, sizeof(CGState), hipMemcpyDeviceToHost, s));
                 NLF_HIP(hipStreamSynchronize(s));
                 const double est = std::sqrt(std::fabs(hs.rz));
                 const bool stop = hs.it >= max_it || hs.breakdown;
-                if (!(est < delta * last) && !stop) continue;
+                if (!(est < delta * last) && !(q * est <= tol) && !stop) continue;
                 hipLaunchKernelGGL((k_add_to<X, double>), dim3(vblocks), dim3(256), 0, s, n, xh.p, x);
                 hipLaunchKernelGGL(k_zero<X>, dim3(vblocks), dim3(256), 0, s, n, xh.p);
                 ++nupd;
@@ -1060,6 +1234,7 @@ struct CG {
                 NLF_HIP(hipMemcpyAsync(&hs, st.p, sizeof(CGState), hipMemcpyDeviceToHost, s));
                 NLF_HIP(hipStreamSynchronize(s));
                 last = std::sqrt(std::fabs(hs.rz));
+                q = last > 0.0 ? rel / last : 0.0;
             }
         }
         timer.collect();
@@ -1072,7 +1247,7 @@ struct CG {
     void iteration(X* x, bool sample = false) {
         // z = A p ; alpha = rz / p.z
         if (sample) timer.start(0, s);
-        launch_spmv<V, X, double, true>(lpr, n, outer, cols, off16, val, p.p, z.p, part1.p, st.p, s);
+        launch_spmv<V, X, double, true>(lpr, n, outer, cs, val, p.p, z.p, part1.p, st.p, s, false, lo);
         if (sample) timer.stop(0, s);
         hipLaunchKernelGGL(k_alpha, dim3(1), dim3(1024), 0, s, part1.p, int(sblocks), st.p);
         if (!P || P->kind == NLF_PRECOND_IDENTITY || jacobi()) {
@@ -1109,11 +1284,20 @@ using namespace nlf;
 
 namespace nlf {
 
+// HBM bytes one SpMV streams: values, column source, row pointers, x once and y.
+i64 spmv_stream_bytes(const nlf_matrix_s* a, size_t sv, size_t sx) {
+    const i64 n = a->rows;
+    i64 cols = a->nnz_full * i64(a->off16.p ? sizeof(int16_t) : sizeof(i32));
+    if (a->pstart.p) cols = a->pat_size * i64(sizeof(int16_t)) + n * i64(sizeof(i64));
+    return a->nnz_full * i64(sv) + cols + n * i64(sizeof(i64) + 2 * sx);
+}
+
 template<int LPR, int UNROLL, bool NT>
 void launch_spmv_variant(nlf_matrix a, const double* x, double* y) {
     hipLaunchKernelGGL((k_spmv<LPR, double, double, double, false, false, UNROLL, NT>), dim3(spmv_blocks(LPR, a->rows)),
-                       dim3(SPMV_THREADS), 0, a->ctx->stream, a->rows, a->outer.p, a->cols.p, a->off16.p, a->val.p, x, y,
-                       nullptr, nullptr);
+                       dim3(SPMV_THREADS), 0, a->ctx->stream, a->rows, a->outer.p, a->cols.p, a->off16.p,
+                       static_cast<const i64*>(nullptr), a->val.p, static_cast<const int16_t*>(nullptr), x, y, nullptr,
+                       nullptr);
 }
 
 }  // namespace nlf
@@ -1124,8 +1308,8 @@ extern "C" {
 int nlf_spmv(nlf_matrix a, const double* x, double* y) {
     if (!a) return NLF_ERR_ARGUMENT;
     return guarded(a->ctx, [&] {
-        const int lpr = pick_lpr(double(a->nnz_full) / double(std::max<i64>(a->rows, 1)));
-        launch_spmv<double, double, double, false>(lpr, a->rows, a->outer.p, a->cols.p, a->off16.p, a->val.p, x, y, nullptr,
+        const int lpr = pick_lpr(a);
+        launch_spmv<double, double, double, false>(lpr, a->rows, a->outer.p, spmv_cols(a), a->val.p, x, y, nullptr,
                                                    nullptr,
                                                    a->ctx->stream);
         NLF_HIP(hipGetLastError());
@@ -1137,8 +1321,8 @@ int nlf_spmv_fp32(nlf_matrix a, const float* x, float* y) {
     if (!a) return NLF_ERR_ARGUMENT;
     return guarded(a->ctx, [&] {
         NLF_CHECK(a->val32.p, NLF_ERR_STATE, "no FP32 copy");
-        const int lpr = pick_lpr(double(a->nnz_full) / double(std::max<i64>(a->rows, 1)));
-        launch_spmv<float, float, float, false>(lpr, a->rows, a->outer.p, a->cols.p, a->off16.p, a->val32.p, x, y, nullptr,
+        const int lpr = pick_lpr(a);
+        launch_spmv<float, float, float, false>(lpr, a->rows, a->outer.p, spmv_cols(a), a->val32.p, x, y, nullptr,
                                                 nullptr,
                                                 a->ctx->stream);
         NLF_HIP(hipGetLastError());
@@ -1147,9 +1331,11 @@ int nlf_spmv_fp32(nlf_matrix a, const float* x, float* y) {
 
 
 // Same SpMV with an explicit lanes-per-row choice (autotuning / benchmarking).  lpr bit 8 selects
-// the XCD-contiguous workgroup numbering, bit 9 the 32-bit column indices instead of the offsets;
-// bits 10 / 11 (FP64 only, LPR 32 or 64) select 8 nonzeros in flight per lane / non-temporal
-// loads of the matrix stream.
+// the XCD-contiguous workgroup numbering, bit 9 the 32-bit column indices instead of the offsets,
+// bit 12 the per-nonzero 16-bit offsets instead of the row-pattern dictionary, bit 13 the split
+// values (FP32 + 16-bit remainder, FP64 vectors; fp32 = 0), bit 14 the one-nonzero-per-load kernel
+// instead of the pair kernel; bits 10 / 11 (FP64 only, LPR 32 or 64, per-nonzero offsets) select 8
+// nonzeros in flight per lane / non-temporal loads of the matrix stream.
 int nlf_spmv_lpr(nlf_matrix a, int lpr, int fp32, const void* x, void* y) {
     if (!a) return NLF_ERR_ARGUMENT;
     return guarded(a->ctx, [&] {
@@ -1165,14 +1351,27 @@ int nlf_spmv_lpr(nlf_matrix a, int lpr, int fp32, const void* x, void* y) {
             return;
         }
         const bool xcd = (lpr & 0x100) != 0;
-        const int16_t* off = (lpr & 0x200) ? nullptr : a->off16.p;
+        const bool split = (lpr & 0x2000) != 0;
+        const bool pairs = (lpr & 0x4000) == 0;
+        SpmvCols cs = spmv_cols(a);
+        if (lpr & 0x1000) cs = SpmvCols{a->cols.p, a->off16.p, nullptr};
+        if (lpr & 0x200) cs = SpmvCols{a->cols.p, nullptr, nullptr};
         lpr &= 0xff;
-        if (fp32) launch_spmv<float, float, float, false>(lpr, a->rows, a->outer.p, a->cols.p, off, a->val32.p,
-                                                          static_cast<const float*>(x), static_cast<float*>(y), nullptr,
-                                                          nullptr, a->ctx->stream, xcd);
-        else launch_spmv<double, double, double, false>(lpr, a->rows, a->outer.p, a->cols.p, off, a->val.p,
-                                                        static_cast<const double*>(x), static_cast<double*>(y), nullptr,
-                                                        nullptr, a->ctx->stream, xcd);
+        if (split) {
+            NLF_CHECK(a->split_ready, NLF_ERR_STATE, "no split values (nlf_matrix_make_split)");
+            launch_spmv<float, double, double, false>(lpr, a->rows, a->outer.p, cs, a->val32.p,
+                                                      static_cast<const double*>(x), static_cast<double*>(y), nullptr,
+                                                      nullptr, a->ctx->stream, false, a->val_lo.p, pairs);
+        } else if (fp32) {
+            NLF_CHECK(a->val32.p, NLF_ERR_STATE, "no FP32 values");
+            launch_spmv<float, float, float, false>(lpr, a->rows, a->outer.p, cs, a->val32.p, static_cast<const float*>(x),
+                                                    static_cast<float*>(y), nullptr, nullptr, a->ctx->stream, xcd, nullptr,
+                                                    pairs);
+        } else {
+            launch_spmv<double, double, double, false>(lpr, a->rows, a->outer.p, cs, a->val.p, static_cast<const double*>(x),
+                                                       static_cast<double*>(y), nullptr, nullptr, a->ctx->stream, xcd, nullptr,
+                                                       pairs);
+        }
         NLF_HIP(hipGetLastError());
     });
 }
@@ -1310,8 +1509,8 @@ int nlf_precond_export_factor(nlf_precond p, double* values, double* scale) {
 
 // conjugate_gradient::solve on device vectors.  mode NLF_MODE_FP64: the reference's PCG with its
 // stopping rule sqrt(r'z)/||b|| <= tol.  NLF_MODE_MIXED: FP64 iterative refinement around FP32 PCG
-// (x += A32^-1 (b - A x)) until the FP64 true residual ||b - Ax||/||b|| <= tol.  x holds the initial
-// guess on entry.
+// (x += A32^-1 (b - A x)) until the FP64 true residual ||b - Ax||/||b|| <= tol.  NLF_MODE_SPLIT: the
+// same with the values read as FP32 + 16-bit remainder.  x holds the initial guess on entry.
 int nlf_cg_solve(nlf_matrix a, nlf_precond p, const double* b, double* x, double tol, int64_t max_it, int mode,
                  nlf_solve_stats* stats) {
     if (!a) return NLF_ERR_ARGUMENT;
@@ -1328,25 +1527,28 @@ int nlf_cg_solve(nlf_matrix a, nlf_precond p, const double* b, double* x, double
             out.spmv_ms = t.avg(0);
             out.precond_ms = t.avg(1);
             out.spmv_samples = t.count[0];
-            out.spmv_bytes = a->nnz_full * i64(sv + (a->off16.p ? sizeof(int16_t) : sizeof(i32))) + n * i64(sizeof(i64) + 2 * sx);
+            out.spmv_bytes = spmv_stream_bytes(a, sv, sx);
         };
         if (mode == NLF_MODE_FP64) {
             CG<double, double> cg(s, n, a->nnz_full, a->outer.p, a->cols.p, a->val.p, p);
-            cg.off16 = a->off16.p;
+            cg.set_cols(a);
             out.iterations = cg.solve(b, x, tol, max_it, &out.residual, &breakdown);
             timings(cg.timer, sizeof(double), sizeof(double));
         } else {
+            NLF_CHECK(mode == NLF_MODE_MIXED || mode == NLF_MODE_SPLIT, NLF_ERR_ARGUMENT, "unknown mode");
             NLF_CHECK(a->val32.p, NLF_ERR_STATE, "mixed mode needs nlf_matrix_make_fp32 first");
+            NLF_CHECK(mode != NLF_MODE_SPLIT || a->split_ready, NLF_ERR_STATE, "split mode needs nlf_matrix_make_split first");
             // FP32 matrix values (the bytes that bound the SpMV) with FP64 vectors and recurrences: FP32
             // CG recurrences lose orthogonality at the conditioning of these operators (kappa ~ 1e6)
             // and need several times the iterations, while FP32 storage costs none; the FP64 true
             // residual with the FP64 matrix replaces the recursive one at every reliable update.
             CG<float, double> cg(s, n, a->nnz_full, a->outer.p, a->cols.p, a->val32.p, p);
-            cg.off16 = a->off16.p;
+            cg.set_cols(a);
+            if (mode == NLF_MODE_SPLIT) cg.lo = a->val_lo.p;
             out.iterations = cg.solve_reliable(a->outer.p, a->cols.p, a->val.p, b, x, tol, max_it, 0.1,
                                                &out.true_residual, &out.outer_iterations, &breakdown);
             out.residual = out.true_residual;
-            timings(cg.timer, sizeof(float), sizeof(double));
+            timings(cg.timer, mode == NLF_MODE_SPLIT ? sizeof(float) + sizeof(int16_t) : sizeof(float), sizeof(double));
         }
         // FP64 true residual ||b - A x|| / ||b||
         {

@@ -189,7 +189,7 @@ class finite_element_matrix {
     csr_matrix part(int which) const {
         int64_t rows, up, full, bound;
         nlf_check(nlf_matrix_info(h_, &rows, &up, &full, &bound), ctx_);
-        const int64_t nnz = which == 0 ? up : which == 1 ? bound : full;
+        const int64_t nnz = (which == 0 || which == 3) ? up : which == 1 ? bound : full;
         csr_matrix m;
         m.outer.resize(size_t(rows + 1));
         m.inner.resize(size_t(std::max<int64_t>(nnz, 1)));
@@ -222,6 +222,10 @@ public:
     csr_matrix inner() const { return part(0); }
     csr_matrix bound() const { return part(1); }
     csr_matrix inner_full() const { return part(2); }
+    // FP32 values for solve(..., NLF_MODE_MIXED): the FP64 values rounded, or an FP32 assembly
+    csr_matrix inner_fp32() const { return part(3); }
+    void make_fp32() { nlf_check(nlf_matrix_make_fp32(h_), ctx_); }
+    void assemble_fp32() { nlf_check(nlf_matrix_assemble_fp32(h_), ctx_); }
 };
 
 // Parameters of one element group: model_parameters (equation_parameters.hpp:18-24) with the

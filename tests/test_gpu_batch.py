@@ -52,10 +52,12 @@ def test_block_diagonal_assembly_matches_instances(solver):
         np.testing.assert_allclose(csr.val[csr.outer[r0]:csr.outer[r1]], Ko.val, rtol=1e-12, atol=1e-12 * np.abs(Ko.val).max())
 
 
-@pytest.mark.parametrize("mode,bound", [(A.NLF_MODE_FP64, 1e-10), (A.NLF_MODE_MIXED, 1e-8)])
-def test_batch_solutions_match_oracle(solver, references, mode, bound):
-    x, its, res, st = solver.solve(tol=1e-12, mode=mode)
-    print(f"[batch] mode {mode}: iterations {its.tolist()}, max true residual {res.max():.2e}", flush=True)
+@pytest.mark.parametrize("mode,bound,assembly32", [(A.NLF_MODE_FP64, 1e-10, False), (A.NLF_MODE_MIXED, 1e-8, False),
+                                                   (A.NLF_MODE_MIXED, 1e-8, True)])
+def test_batch_solutions_match_oracle(solver, references, mode, bound, assembly32):
+    x, its, res, st = solver.solve(tol=1e-12, mode=mode, assembly32=assembly32)
+    print(f"[batch] mode {mode} fp32 assembly {assembly32}: iterations {its.tolist()}, "
+          f"max true residual {res.max():.2e}", flush=True)
     assert np.all(res <= 1e-12)
     for xi, ref in zip(solver.split(x), references):
         assert rel_l2(xi.cpu().numpy(), ref) < bound

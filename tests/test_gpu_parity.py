@@ -206,3 +206,31 @@ def test_cfg0_against_golden_fixture(gpu_ctx):
     h.update(np.ascontiguousarray(ids).tobytes())
     assert h.hexdigest() == str(G["cfg0_nb_sha"])
     assert H.rel_l2(sol.x.cpu().numpy(), G["cfg0_solution"]) <= 1e-10
+
+
+@pytest.mark.parametrize("problem", PROBLEMS, ids=IDS)
+def test_row_pattern_dictionary(gpu_ctx, problem):
+    """The SpMV through the shared row-pattern dictionary equals the per-nonzero-offset SpMV bit for
+    bit (same products in the same order); the dictionary is in use on the uniform meshes."""
+    from nonlocal_amd._lib import load
+    print(f"[row patterns] {problem.name}", flush=True)
+    om, K, f, xD, inner, g, Kg = _assemble_both(problem, gpu_ctx)
+    groups, entries = Kg.pattern_info()
+    nnz = Kg.info()["nnz_full"]
+    print(f"  {groups} distinct rows, {entries} entries for {nnz} nonzeros", flush=True)
+    if problem.name.startswith("cfg0"):
+        assert 0 < entries * 4 <= nnz, "a uniform one-material Q4 mesh must use the dictionary"
+    x = torch.from_numpy(np.random.default_rng(2).standard_normal(f.size)).cuda()
+    y_dict = torch.empty_like(x)
+    y_off = torch.empty_like(x)
+    for lpr in (4, 16, 32):
+        gpu_ctx.check(load().nlf_spmv_lpr(Kg.h, lpr, 0, api._dev(x), api._dev(y_dict)))
+        gpu_ctx.check(load().nlf_spmv_lpr(Kg.h, lpr | 0x1000, 0, api._dev(x), api._dev(y_off)))
+        assert torch.equal(y_dict, y_off)
+    assert H.rel_l2(y_dict.cpu().numpy(), K.spmv(x.cpu().numpy())) <= 1e-13
+    gpu_ctx.set_option(A.NLF_OPT_ROW_PATTERNS, 0)
+    try:
+        Kn = api._assemble(g, problem.physics, problem.models, problem.materials, inner)
+    finally:
+        gpu_ctx.set_option(A.NLF_OPT_ROW_PATTERNS, 1)
+    assert Kn.pattern_info() == (0, 0)
