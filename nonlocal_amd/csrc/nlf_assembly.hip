@@ -670,6 +670,65 @@ __global__ __launch_bounds__(ASM_THREADS) void k_assemble(AssemblyArgs a) {
     }
 }
 
+// Local-only assembly (assemble_part::LOCAL, or no nonlocal group): one wave per node instead of
+// one workgroup.  A node's rows have at most 64 upper and 64 bound entries here (checked by the
+// host), so lane k owns entry k of each; every lane evaluates the element's local row and adds the
+// blocks whose column it owns, in the node kernel's order (elements ascending, then j), so no
+// atomics and no barriers are needed.
+template<int DOF, int NN, int NQ, class T>
+__global__ __launch_bounds__(256) void k_assemble_local(AssemblyArgs a) {
+    constexpr bool F64 = sizeof(T) == sizeof(double);
+    const i64 node = i64(blockIdx.x) * 4 + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (node >= a.N) return;
+    i64 rows[DOF], ub[DOF], bb[DOF];
+    int ulen[DOF], blen[DOF];
+    i32 cu[DOF], cb[DOF];
+    T au[DOF], ab[DOF];
+    bool rinner[DOF];
+    for (int d = 0; d < DOF; ++d) {
+        rows[d] = i64(DOF) * node + d;
+        ub[d] = a.diag[rows[d]];
+        ulen[d] = int(a.outer[rows[d] + 1] - ub[d]);
+        bb[d] = a.b_outer[rows[d]];
+        blen[d] = int(a.b_outer[rows[d] + 1] - bb[d]);
+        cu[d] = lane < ulen[d] ? a.cols[ub[d] + lane] : -1;
+        cb[d] = lane < blen[d] ? a.b_cols[bb[d] + lane] : -1;
+        au[d] = T(0);
+        ab[d] = T(0);
+        rinner[d] = a.inner[rows[d]];
+    }
+    for (i32 k = a.ne_off[node]; k < a.ne_off[node + 1]; ++k) {
+        const i32 eL = a.ne_ids[k];
+        int iL = 0;
+        for (int i = 0; i < NN; ++i) if (a.el[i64(eL) * NN + i] == node) iL = i;
+        const GroupParams g = a.groups[a.group_of[eL]];
+        T v[NN][DOF * DOF];
+        local_row<DOF, NN, NQ, T>(a, g, eL, iL, v);
+        for (int j = 0; j < NN; ++j) {
+            const i64 m = a.el[i64(eL) * NN + j];
+            for (int d = 0; d < DOF; ++d)
+                for (int dc = 0; dc < DOF; ++dc) {
+                    const i64 c = i64(DOF) * m + dc;
+                    if (a.inner[c]) {                  // INNER: inner row, c >= row (upper storage)
+                        if (rinner[d] && c >= rows[d] && cu[d] == i32(c)) au[d] += v[j][d * DOF + dc];
+                    } else if (c != rows[d]) {         // BOUND
+                        if (cb[d] == i32(c)) ab[d] += v[j][d * DOF + dc];
+                    }
+                }
+        }
+    }
+    for (int d = 0; d < DOF; ++d) {
+        if (lane < ulen[d]) {
+            const T value = rinner[d] ? au[d] : (lane == 0 ? T(1) : T(0));
+            if constexpr (F64) a.val[ub[d] + lane] = value;
+            else a.val32[ub[d] + lane] = value;
+        }
+        if constexpr (F64)
+            if (lane < blen[d]) a.b_val[bb[d] + lane] = ab[d];
+    }
+}
+
 // Lower triangle of the full storage: a(r, c) = a(c, r) for c < r (binary search of r in row c).
 template<class T>
 __global__ void k_mirror(i64 rows, const i64* __restrict__ outer, const i32* __restrict__ cols,
@@ -728,6 +787,15 @@ __global__ void k_row_max(i64 rows, const i64* __restrict__ outer, const i64* __
         s += (outer[r + 1] - diag[r]) + (b_outer[r + 1] - b_outer[r]);
     }
     atomicMax(mx, static_cast<unsigned long long>(s));
+}
+
+// Longest upper part and longest bound part over all rows.
+__global__ void k_row_lengths(i64 rows, const i64* __restrict__ outer, const i64* __restrict__ diag,
+                              const i64* __restrict__ b_outer, unsigned long long* __restrict__ mx) {
+    const i64 r = i64(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (r >= rows) return;
+    atomicMax(&mx[0], static_cast<unsigned long long>(outer[r + 1] - diag[r]));
+    atomicMax(&mx[1], static_cast<unsigned long long>(b_outer[r + 1] - b_outer[r]));
 }
 
 template<class T>
@@ -902,7 +970,25 @@ void assemble_values(nlf_matrix_s* mat, hipStream_t s) {
             aa.plane = total_pairs * 4;
         }
     }
-    if (mat->physics == 0 && m->nn == 4) launch_assemble<1, 4, 4, T>(aa, det, lds, s);
+    // local-only matrices whose rows fit a wave: one wave per node
+    unsigned long long max_upper = 0, max_bound = 0;
+    if (!any_nonlocal || only_local) {
+        DevBuf<unsigned long long> d_len{2};
+        NLF_HIP(hipMemsetAsync(d_len.p, 0, d_len.bytes(), s));
+        hipLaunchKernelGGL(k_row_lengths, dim3(blocks_for(rows, 256)), dim3(256), 0, s, rows, mat->outer.p, mat->diag.p,
+                           mat->b_outer.p, d_len.p);
+        unsigned long long h_len[2];
+        download(h_len, d_len.p, 2, s);
+        max_upper = h_len[0];
+        max_bound = h_len[1];
+    }
+    if ((!any_nonlocal || only_local) && max_upper <= 64 && max_bound <= 64) {
+        const unsigned nb = unsigned((m->N + 3) / 4);
+        if (mat->physics == 0 && m->nn == 4) hipLaunchKernelGGL((k_assemble_local<1, 4, 4, T>), dim3(nb), dim3(256), 0, s, aa);
+        else if (mat->physics == 0 && m->nn == 8) hipLaunchKernelGGL((k_assemble_local<1, 8, 9, T>), dim3(nb), dim3(256), 0, s, aa);
+        else if (mat->physics == 1 && m->nn == 4) hipLaunchKernelGGL((k_assemble_local<2, 4, 4, T>), dim3(nb), dim3(256), 0, s, aa);
+        else hipLaunchKernelGGL((k_assemble_local<2, 8, 9, T>), dim3(nb), dim3(256), 0, s, aa);
+    } else if (mat->physics == 0 && m->nn == 4) launch_assemble<1, 4, 4, T>(aa, det, lds, s);
     else if (mat->physics == 0 && m->nn == 8) launch_assemble<1, 8, 9, T>(aa, det, lds, s);
     else if (mat->physics == 1 && m->nn == 4) launch_assemble<2, 4, 4, T>(aa, det, lds, s);
     else launch_assemble<2, 8, 9, T>(aa, det, lds, s);

@@ -141,7 +141,10 @@ template<class V> struct Pair;
 template<> struct Pair<double> { typedef double T __attribute__((ext_vector_type(2))); };
 template<> struct Pair<float> { typedef float T __attribute__((ext_vector_type(2))); };
 
-template<int LPR, class V, class X, class Acc, bool DOT, bool SPLIT = false>
+// XCD: each block takes a contiguous chunk of row groups, and the blocks that the hardware deals
+// round-robin to one XCD take adjacent chunks, so each XCD's L2 holds the x window of its own
+// eighth of the rows (without it, concurrently active rows span the whole matrix on every XCD).
+template<int LPR, class V, class X, class Acc, bool DOT, bool SPLIT = false, bool XCD = false>
 __global__ __launch_bounds__(SPMV_THREADS) void k_spmv2(i64 rows, const i64* __restrict__ outer,
                                                         const int16_t* __restrict__ off, const i64* __restrict__ pstart,
                                                         const V* __restrict__ val, const int16_t* __restrict__ lo,
@@ -161,9 +164,17 @@ __global__ __launch_bounds__(SPMV_THREADS) void k_spmv2(i64 rows, const i64* __r
         if constexpr (SPLIT) return *reinterpret_cast<const int*>(lo + p);
         else return 0;
     };
-    const i64 rows_r = (rows + RPB - 1) / RPB * RPB;
-    for (i64 r0 = i64(blockIdx.x) * RPB; r0 < rows_r; r0 += i64(gridDim.x) * RPB) {
-        const i64 r = r0 + threadIdx.x / LPR;
+    const i64 groups = (rows + RPB - 1) / RPB;
+    i64 g_begin = blockIdx.x, g_end = groups, g_step = gridDim.x;
+    if (XCD) {
+        const i64 lb = i64(blockIdx.x % 8) * (gridDim.x / 8) + blockIdx.x / 8;
+        const i64 per = (groups + gridDim.x - 1) / gridDim.x;
+        g_begin = lb * per;
+        g_end = std::min<i64>(groups, g_begin + per);
+        g_step = 1;
+    }
+    for (i64 grp = g_begin; grp < g_end; grp += g_step) {
+        const i64 r = grp * RPB + threadIdx.x / LPR;
         Acc acc = 0;
         if (r < rows) {
             const i64 b = outer[r], e = outer[r + 1];
@@ -973,8 +984,9 @@ unsigned spmv_blocks(int lpr, i64 rows) {
 
 // cs: column source (spmv_cols of the matrix, or plain offsets / columns with cs.pstart = null).
 // lo: 16-bit remainders of split values (V = float, NLF_MODE_SPLIT), else null.  With 16-bit
-// offsets (per nonzero or the dictionary) the pair kernel k_spmv2 runs unless pairs = false or an
-// XCD-contiguous numbering is asked for.
+// offsets (per nonzero or the dictionary) the pair kernel k_spmv2 runs unless pairs = false; xcd
+// selects its XCD-contiguous row chunks (the one-nonzero-per-load kernel's XCD numbering when
+// pairs = false).
 template<class V, class X, class Acc, bool DOT>
 void launch_spmv(int lpr, i64 rows, const i64* outer, SpmvCols cs, const V* val, const X* x, X* y,
                  double* partial, const CGState* st, hipStream_t s, bool xcd = false, const int16_t* lo = nullptr,
@@ -983,6 +995,21 @@ void launch_spmv(int lpr, i64 rows, const i64* outer, SpmvCols cs, const V* val,
     const int16_t* off16 = cs.off;
     const i64* pstart = cs.pstart;
     const unsigned blocks = spmv_blocks(lpr, rows);
+    if (off16 && pairs && xcd && blocks % 8 == 0 && !lo) {
+#define NLF_SPMV2_XCASE(L)                                                                                         \
+    case L:                                                                                                        \
+        hipLaunchKernelGGL((k_spmv2<L, V, X, Acc, DOT, false, true>), dim3(blocks), dim3(SPMV_THREADS), 0, s, rows, outer, \
+                           off16, pstart, val, static_cast<const int16_t*>(nullptr), x, y, partial, st);          \
+        return;
+        switch (lpr) {
+            NLF_SPMV2_XCASE(4)
+            NLF_SPMV2_XCASE(8)
+            NLF_SPMV2_XCASE(16)
+            NLF_SPMV2_XCASE(32)
+            default: NLF_SPMV2_XCASE(64)
+        }
+#undef NLF_SPMV2_XCASE
+    }
     if (off16 && pairs && !xcd) {
         if (lo) {
             if constexpr (sizeof(V) == sizeof(float)) {
