@@ -33,6 +33,20 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "spmv_composite1000_hbm_counters.txt")
+
+
+def counter_traffic(n, mode_fp64):
+    """HBM bytes per SpMV launch from the committed FETCH_SIZE / WRITE_SIZE summary of the FP64
+    SpMV on the 1000x1000 composite (None for other workloads or value types)."""
+    import re
+    if n != 1000 or not mode_fp64 or not os.path.exists(TRAFFIC_PROFILE):
+        return None
+    with open(TRAFFIC_PROFILE) as f:
+        m = re.search(r"SpMV traffic per launch:.*=\s*([0-9.]+e[+0-9]+)\s*B", f.read())
+    return float(m.group(1)) if m else None
+
+
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -400,7 +414,8 @@ def main():
         "phase_seconds": {k: round(v, 4) for k, v in last.timings.items()},
         "precond_apply_ms": round(st.precond_ms, 4),
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": counter_traffic(args.n, sv == 8),
+                     "traffic_source": os.path.relpath(TRAFFIC_PROFILE, ROOT),
                      "kernel": f"k_spmv ({ {4: 'fp32', 6: 'fp32+bf16 split'}.get(sv, 'fp64')} values, full CSR)",
                      "avg_launch_ms": round(st.spmv_ms, 4), "samples": int(st.spmv_samples),
                      "algorithmic_bytes_per_launch": int(algo_bytes),
