@@ -56,7 +56,7 @@ def parse():
     ap.add_argument("--tol", type=float, default=1e-10)
     ap.add_argument("--cpu-sample-n", type=int, default=480)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--methods", default="fp64_jacobi,mixed_jacobi,learned_fcg,learned_fcg_split")
+    ap.add_argument("--methods", default="fp64_jacobi,mixed_jacobi,learned_fcg_onelevel,learned_fcg")
     ap.add_argument("--workload", default="composite1000", choices=["composite1000", "batch64", "precision", "spmv"],
                     help="composite1000: the 1000x1000 composite (the metric's workload); batch64: 64 uneven "
                          "instances solved together vs one after another; precision: FP32 assembly and "
@@ -66,12 +66,12 @@ def parse():
     return ap.parse_args()
 
 
-def run_step(api, A, mesh, problem, precond, mode, tol):
+def run_step(api, A, mesh, problem, precond, mode, tol, coarse=None):
     """One pass of the hot path; returns (solution, timings)."""
     torch = api.torch
     t0 = time.perf_counter()
     sol = api._solve_driver(mesh, problem.physics, problem.models, problem.materials, problem.bcs, precond, tol,
-                            30000, mode)
+                            30000, mode, coarse=coarse)
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     return sol, wall
@@ -338,7 +338,9 @@ def main():
         "learned_fcg": (A.NLF_PRECOND_LEARNED, A.NLF_MODE_FP64),
         "learned_fcg_split": (A.NLF_PRECOND_LEARNED, A.NLF_MODE_SPLIT),
         "split_jacobi": (A.NLF_PRECOND_JACOBI, A.NLF_MODE_SPLIT),
+        "learned_fcg_onelevel": (A.NLF_PRECOND_LEARNED, A.NLF_MODE_FP64),
     }
+    coarse_of = {"learned_fcg_onelevel": 0}
     # plain FP64 ILLT-CG: the baseline the metric compares against; also the FP64 reference solution
     log("[bench] plain FP64 ILLT-CG")
     ref, t_plain = run_step(api, A, mesh, problem, *methods["plain_fp64_illt"], args.tol)
@@ -350,7 +352,7 @@ def main():
     best = None
     for name in [m for m in args.methods.split(",") if m]:
         pc, mode = methods[name]
-        sol, t = run_step(api, A, mesh, problem, pc, mode, args.tol)
+        sol, t = run_step(api, A, mesh, problem, pc, mode, args.tol, coarse_of.get(name))
         err = float(torch.linalg.norm(sol.x - x_ref) / torch.linalg.norm(x_ref))
         log(f"[bench] {name}: {t:.3f} s, it {sol.iterations}, outer {sol.stats.outer_iterations}, "
             f"true res {sol.true_residual:.2e}, rel err vs FP64 {err:.2e}, timings {sol.timings}")
@@ -363,12 +365,12 @@ def main():
     pc, mode = methods[best]
     log(f"[bench] flagship method: {best}; warmup {args.warmup}, steps {args.steps}")
     for _ in range(args.warmup):
-        run_step(api, A, mesh, problem, pc, mode, args.tol)
+        run_step(api, A, mesh, problem, pc, mode, args.tol, coarse_of.get(best))
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     sols = []
     for _ in range(args.steps):
-        sol, _t = run_step(api, A, mesh, problem, pc, mode, args.tol)
+        sol, _t = run_step(api, A, mesh, problem, pc, mode, args.tol, coarse_of.get(best))
         sols.append(sol)
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
@@ -416,7 +418,7 @@ def main():
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": counter_traffic(args.n, sv == 8),
                      "traffic_source": os.path.relpath(TRAFFIC_PROFILE, ROOT),
-                     "kernel": f"k_spmv ({ {4: 'fp32', 6: 'fp32+bf16 split'}.get(sv, 'fp64')} values, full CSR)",
+                     "kernel": f"k_spmv2 ({ {4: 'fp32', 6: 'fp32 + 16-bit remainder'}.get(sv, 'fp64')} values, full CSR)",
                      "avg_launch_ms": round(st.spmv_ms, 4), "samples": int(st.spmv_samples),
                      "algorithmic_bytes_per_launch": int(algo_bytes),
                      "moved_bytes_per_launch": int(st.spmv_bytes), "moved_GBps": round(moved_gbs, 1)},

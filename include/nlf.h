@@ -216,6 +216,12 @@ int nlf_learned_get_weights(nlf_precond p, float* w_host);
 int nlf_learned_info(nlf_precond p, int* compiled, int64_t* nnz, double* compile_seconds);
 /* 0 = apply the compiled operator when available (default), 1 = always run the network kernels */
 int nlf_learned_set_apply(nlf_precond p, int network);
+/* No reference counterpart.  Two-level form of the learned preconditioner: B r = M r + P A_c^-1 P^T r
+ * with P the 0/1 map of rows to about `aggregates` aggregates (a grid of cells over the node
+ * coordinates, at most 4096 coarse rows and at least 16 nodes per aggregate on average) and
+ * A_c = P^T a P inverted once on the device.  aggregates = 0 removes it. */
+int nlf_learned_set_coarse(nlf_precond p, nlf_matrix a, int64_t aggregates);
+int nlf_learned_coarse_info(nlf_precond p, int64_t* coarse_rows, double* setup_seconds);
 int nlf_learned_export_graph(nlf_precond p, int64_t* outer, int32_t* cols, float* feat, float* s);
 /* loss ||M(r) - e||_A^2 / ||e||_A^2 for a pair with A e = r, and its weight gradient (host) */
 int nlf_learned_loss_grad(nlf_precond p, nlf_matrix a, const double* r_dev, const double* e_dev, double* loss,

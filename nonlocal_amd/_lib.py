@@ -53,6 +53,8 @@ SIGNATURES = {
     "nlf_matrix_make_split": (_int, [_p]),
     "nlf_matrix_compare_fp32": (_int, [_p, _f64p]),
     "nlf_matrix_pattern_info": (_int, [_p, _i64p, _i64p]),
+    "nlf_learned_set_coarse": (_int, [_p, _p, _i64]),
+    "nlf_learned_coarse_info": (_int, [_p, _i64p, _f64p]),
     "nlf_boundary_data": (_int, [_p, _int, C.POINTER(_abi.Boundary), _f64p, _f64p, _u8p]),
     "nlf_apply_first_kind": (_int, [_p, _p, _p]),
     "nlf_spmv": (_int, [_p, _p, _p]),

@@ -388,7 +388,7 @@ def _radii(models):
 
 
 def _solve_driver(mesh: mesh_2d, physics, models, materials, bcs, precond, tol, max_it, mode, weights=None,
-                  assembly32=False):
+                  assembly32=False, coarse=None):
     import time
     ctx = mesh.ctx
     dof = 1 if physics == 0 else 2
@@ -429,6 +429,7 @@ def _solve_driver(mesh: mesh_2d, physics, models, materials, bcs, precond, tol, 
         KL = _assemble(mesh, physics, models, materials, inner, part=NLF_PART_LOCAL)
         w, _ = learned.load_weights(weights or learned.DEFAULT_WEIGHTS)
         learned_pc = learned.LearnedPreconditioner(K, KL, models, w)
+        learned_pc.set_coarse(learned.DEFAULT_AGGREGATES if coarse is None else coarse)
         fallback = preconditioner(K, NLF_PRECOND_JACOBI)
     else:
         cg.init_preconditioner(precond)
@@ -458,13 +459,16 @@ def _solve_driver(mesh: mesh_2d, physics, models, materials, bcs, precond, tol, 
 
 
 def stationary_heat_equation_solver_2d(mesh: mesh_2d, models, materials, bcs, precond=NLF_PRECOND_ILLT, tol=1e-12,
-                                       max_it=100000, mode=NLF_MODE_FP64, weights=None, assembly32=False) -> Solution:
+                                       max_it=100000, mode=NLF_MODE_FP64, weights=None, assembly32=False,
+                                       coarse=None) -> Solution:
     """K T = f with K = p1 K_L + (1 - p1) K_NL (solved directly; the reference's correction form
     T = T0 - K^-1 (K T0 - f) with T0 = 0 is the same system).  precond NLF_PRECOND_LEARNED runs
     flexible CG with the learned preconditioner (weights file `weights`, default the packaged one)
     and Jacobi-PCG as its fallback; tol is then on the FP64 true residual.  In the mixed mode the
-    inner FP32 operator is the rounded FP64 matrix, or with assembly32 an FP32 assembly."""
-    return _solve_driver(mesh, 0, models, materials, bcs, precond, tol, max_it, mode, weights, assembly32)
+    inner FP32 operator is the rounded FP64 matrix, or with assembly32 an FP32 assembly.  coarse sets
+    the learned preconditioner's aggregation coarse correction (about that many aggregates; 0 = one
+    level; default 4096)."""
+    return _solve_driver(mesh, 0, models, materials, bcs, precond, tol, max_it, mode, weights, assembly32, coarse)
 
 
 def equilibrium_equation(mesh: mesh_2d, models, materials, bcs, precond=None, tol=1e-12, max_it=100000,

@@ -288,7 +288,9 @@ struct AssemblyArgs {
     int local_only;      // assemble_part::LOCAL: every group treated as LOCAL
     const double* pairs; // thermal Q4: nonlocal pair blocks from k_pairs_mfma (null = compute in place)
     const float* pairs32;
-    i64 plane;           // pairs[i * plane + p * 4 + j], p = neighbour-list position
+    i64 plane;           // pairs[i * plane + (eoff[eL] + t) * 4 + j], t = position in eL's neighbour list
+    const i64* eoff;
+    i64 node0;           // nodes [node0, node0 + grid) of this launch
 };
 
 template<class T>
@@ -398,7 +400,9 @@ __device__ void nonlocal_row(const AssemblyArgs& a, const GroupParams& g, const 
 // to pairs[i * plane + p * 4 + j]; the node-centric assembly reads row i of them.
 template<class T>
 struct PairArgs {
-    i64 E;
+    i64 count;                 // elements of this chunk: list[0 .. count)
+    const i32* list;
+    const i64* eoff;           // chunk slot of each listed element's first neighbour pair
     const i64* nb_off;
     const i32* nb_ids;
     const i32* group_of;
@@ -441,7 +445,9 @@ __global__ __launch_bounds__(256) void k_pairs_mfma(PairArgs<double> a) {
     const int l = threadIdx.x & 63;
     const int g = (l >> 2) & 3, c = l & 3, h = l >> 4;
     const i64 waves = i64(gridDim.x) * (blockDim.x >> 6);
-    for (i64 eL = i64(blockIdx.x) * (blockDim.x >> 6) + (threadIdx.x >> 6); eL < a.E; eL += waves) {
+    for (i64 li = i64(blockIdx.x) * (blockDim.x >> 6) + (threadIdx.x >> 6); li < a.count; li += waves) {
+        const i64 eL = a.list[li];
+        const i64 slot = a.eoff[eL];
         const GroupParams gp = a.groups[a.group_of[eL]];          // one group per wave
         if (!gp.nonlocal) continue;
         const double hl0 = a.HL[eL * 32 + c * 4 + h];             // HL_0[i = c][ql = h]
@@ -469,7 +475,7 @@ __global__ __launch_bounds__(256) void k_pairs_mfma(PairArgs<double> a) {
             const double v1 = __builtin_amdgcn_mfma_f64_4x4x4f64(phi, hl1, 0.0, 0, 0, 0);
             double k = __builtin_amdgcn_mfma_f64_4x4x4f64(v0, hn0, 0.0, 0, 0, 0);
             k = __builtin_amdgcn_mfma_f64_4x4x4f64(v1, hn1, k, 0, 0, 0);
-            if (need) a.pairs[i64(h) * a.plane + (nb_b + t) * 4 + c] = s * k;     // K[i = h][j = c]
+            if (need) a.pairs[i64(h) * a.plane + (slot + t) * 4 + c] = s * k;     // K[i = h][j = c]
         }
     }
 }
@@ -488,7 +494,9 @@ __global__ __launch_bounds__(256) void k_pairs_mfma32(PairArgs<float> a) {
     const int l = threadIdx.x & 63;
     const int b = l >> 2, c = l & 3;
     const i64 waves = i64(gridDim.x) * (blockDim.x >> 6);
-    for (i64 eL = i64(blockIdx.x) * (blockDim.x >> 6) + (threadIdx.x >> 6); eL < a.E; eL += waves) {
+    for (i64 li = i64(blockIdx.x) * (blockDim.x >> 6) + (threadIdx.x >> 6); li < a.count; li += waves) {
+        const i64 eL = a.list[li];
+        const i64 slot = a.eoff[eL];
         const GroupParams gp = a.groups[a.group_of[eL]];          // one group per wave
         if (!gp.nonlocal) continue;
         const f32x4 hl0 = *reinterpret_cast<const f32x4*>(a.HL + eL * 32 + c * 4);       // HL_0[i = c][ql]
@@ -526,7 +534,7 @@ __global__ __launch_bounds__(256) void k_pairs_mfma32(PairArgs<float> a) {
                 k = __builtin_amdgcn_mfma_f32_4x4x1f32(v1[qn], hn1[qn], k, 0, 0, 0);
             }
             if (need)
-                for (int i = 0; i < 4; ++i) a.pairs[i64(i) * a.plane + (nb_b + t) * 4 + c] = s * k[i];  // K[i][j = c]
+                for (int i = 0; i < 4; ++i) a.pairs[i64(i) * a.plane + (slot + t) * 4 + c] = s * k[i];  // K[i][j]
         }
     }
 }
@@ -541,7 +549,7 @@ __global__ __launch_bounds__(ASM_THREADS) void k_assemble(AssemblyArgs a) {
     __shared__ T hl[2 * NQ];
     __shared__ int row_len[DOF], brow_len[DOF];
     __shared__ int s_off[2 * DOF + 1];
-    const i64 node = blockIdx.x;
+    const i64 node = a.node0 + blockIdx.x;
     if (node >= a.N) return;
     i64 rows[DOF];
     for (int d = 0; d < DOF; ++d) rows[d] = i64(DOF) * node + d;
@@ -607,7 +615,7 @@ __global__ __launch_bounds__(ASM_THREADS) void k_assemble(AssemblyArgs a) {
                     if constexpr (F64) pairs = a.pairs; else pairs = a.pairs32;
                     if constexpr (DOF == 1 && NN == 4 && NQ == 4) {
                         if (pairs) {
-                            const T* pr = pairs + i64(iL) * a.plane + (nb_b + t) * 4;
+                            const T* pr = pairs + i64(iL) * a.plane + (a.eoff[eL] + t) * 4;
                             for (int j = 0; j < NN; ++j) v[j][0] = pr[j];
                         } else {
                             nonlocal_row<DOF, NN, NQ, T>(a, g, xl, hl, eNL, v);
@@ -789,6 +797,12 @@ __global__ void k_row_max(i64 rows, const i64* __restrict__ outer, const i64* __
     atomicMax(mx, static_cast<unsigned long long>(s));
 }
 
+// eoff[list[i]] = ofs[i]: chunk slots of the chunk's elements.
+__global__ void k_scatter_offsets(i64 n, const i32* __restrict__ list, const i64* __restrict__ ofs, i64* __restrict__ eoff) {
+    const i64 i = i64(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) eoff[list[i]] = ofs[i];
+}
+
 // Longest upper part and longest bound part over all rows.
 __global__ void k_row_lengths(i64 rows, const i64* __restrict__ outer, const i64* __restrict__ diag,
                               const i64* __restrict__ b_outer, unsigned long long* __restrict__ mx) {
@@ -862,10 +876,12 @@ __global__ __launch_bounds__(256) void k_compare32(i64 n, const double* __restri
     if (threadIdx.x < 4) part[blockIdx.x * 4 + threadIdx.x] = red[threadIdx.x][0];
 }
 
+// nodes [a.node0, a.node0 + count)
 template<int DOF, int NN, int NQ, class T>
-void launch_assemble(const AssemblyArgs& a, bool det, size_t lds, hipStream_t s) {
-    if (det) hipLaunchKernelGGL((k_assemble<DOF, NN, NQ, true, T>), dim3(unsigned(a.N)), dim3(ASM_THREADS), lds, s, a);
-    else hipLaunchKernelGGL((k_assemble<DOF, NN, NQ, false, T>), dim3(unsigned(a.N)), dim3(ASM_THREADS), lds, s, a);
+void launch_assemble(const AssemblyArgs& a, bool det, size_t lds, i64 count, hipStream_t s) {
+    if (count <= 0) return;
+    if (det) hipLaunchKernelGGL((k_assemble<DOF, NN, NQ, true, T>), dim3(unsigned(count)), dim3(ASM_THREADS), lds, s, a);
+    else hipLaunchKernelGGL((k_assemble<DOF, NN, NQ, false, T>), dim3(unsigned(count)), dim3(ASM_THREADS), lds, s, a);
 }
 
 }  // namespace nlf
@@ -938,38 +954,8 @@ void assemble_values(nlf_matrix_s* mat, hipStream_t s) {
     aa.b_cols = mat->b_cols.p; aa.val = mat->val.p; aa.b_val = mat->b_val.p; aa.val32 = mat->val32.p;
     aa.local_only = only_local;
     const bool det = m->injective;
-    DevBuf<T> pairs, HL, HN;
     const bool node_only = m->ctx->assembly_pairs == 0;
-    if (mat->physics == 0 && m->nn == 4 && any_nonlocal && !only_local && !node_only) {
-        i64 total_pairs = 0;
-        download(&total_pairs, m->nb_off.p + m->E, 1, s);
-        size_t free_b = 0, total_b = 0;
-        NLF_HIP(hipMemGetInfo(&free_b, &total_b));
-        const size_t need_b = size_t(total_pairs) * 16 * sizeof(T) + size_t(m->E) * 64 * sizeof(T);
-        if (total_pairs > 0 && need_b < free_b / 2) {
-            pairs.alloc(size_t(total_pairs) * 16);
-            HL.alloc(size_t(m->E) * 32);
-            HN.alloc(size_t(m->E) * 32);
-            Weights4 w4{};
-            for (int q = 0; q < 4; ++q) w4.w[q] = m->fe.w[q];
-            hipLaunchKernelGGL(k_pair_prep<T>, dim3(blocks_for(m->E * 16, 256)), dim3(256), 0, s, m->E, m->D.p, w4,
-                               m->group_of.p, d_groups.p, HL.p, HN.p);
-            PairArgs<T> pr{};
-            pr.E = m->E; pr.nb_off = m->nb_off.p; pr.nb_ids = m->nb_ids.p; pr.group_of = m->group_of.p;
-            pr.groups = d_groups.p; pr.xq = m->xq.p; pr.HL = HL.p; pr.HN = HN.p; pr.el_min = el_min.p;
-            pr.el_max = m->el_maxnode.p; pr.el_dir = el_dir.p; pr.pairs = pairs.p; pr.plane = total_pairs * 4;
-            const unsigned pb = unsigned(std::min<i64>((m->E + 3) / 4, 16384));
-            if constexpr (F64) {
-                hipLaunchKernelGGL(k_pairs_mfma, dim3(pb), dim3(256), 0, s, pr);
-                aa.pairs = pairs.p;
-            } else {
-                hipLaunchKernelGGL(k_pairs_mfma32, dim3(pb), dim3(256), 0, s, pr);
-                aa.pairs32 = pairs.p;
-            }
-            NLF_HIP(hipGetLastError());
-            aa.plane = total_pairs * 4;
-        }
-    }
+    const bool use_pairs = mat->physics == 0 && m->nn == 4 && any_nonlocal && !only_local && !node_only;
     // local-only matrices whose rows fit a wave: one wave per node
     unsigned long long max_upper = 0, max_bound = 0;
     if (!any_nonlocal || only_local) {
@@ -988,10 +974,88 @@ void assemble_values(nlf_matrix_s* mat, hipStream_t s) {
         else if (mat->physics == 0 && m->nn == 8) hipLaunchKernelGGL((k_assemble_local<1, 8, 9, T>), dim3(nb), dim3(256), 0, s, aa);
         else if (mat->physics == 1 && m->nn == 4) hipLaunchKernelGGL((k_assemble_local<2, 4, 4, T>), dim3(nb), dim3(256), 0, s, aa);
         else hipLaunchKernelGGL((k_assemble_local<2, 8, 9, T>), dim3(nb), dim3(256), 0, s, aa);
-    } else if (mat->physics == 0 && m->nn == 4) launch_assemble<1, 4, 4, T>(aa, det, lds, s);
-    else if (mat->physics == 0 && m->nn == 8) launch_assemble<1, 8, 9, T>(aa, det, lds, s);
-    else if (mat->physics == 1 && m->nn == 4) launch_assemble<2, 4, 4, T>(aa, det, lds, s);
-    else launch_assemble<2, 8, 9, T>(aa, det, lds, s);
+    } else if (use_pairs) {
+        // thermal Q4 nonlocal blocks through the MFMA pair kernel, node chunk by node chunk: a chunk
+        // is a run of nodes whose elements' pair blocks (16 values per neighbour pair) take at most
+        // PAIR_BUDGET bytes, so the buffer is bounded whatever the mesh size.  Elements shared by
+        // two chunks are contracted once in each.
+        constexpr size_t PAIR_BUDGET = size_t(8) << 30;
+        const i64 E = m->E, N = m->N, nn = m->nn;
+        std::vector<i64> nb_off(size_t(E + 1));
+        download(nb_off.data(), m->nb_off.p, nb_off.size(), s);
+        std::vector<i32> noff(size_t(N + 1), 0), nel(size_t(E * nn));     // node -> elements (ascending)
+        for (i64 e = 0; e < E * nn; ++e) ++noff[size_t(m->el_h[size_t(e)]) + 1];
+        for (i64 v = 0; v < N; ++v) noff[size_t(v + 1)] += noff[size_t(v)];
+        {
+            std::vector<i32> fill(noff.begin(), noff.end() - 1);
+            for (i64 e = 0; e < E; ++e)
+                for (i64 k = 0; k < nn; ++k) nel[size_t(fill[size_t(m->el_h[size_t(e * nn + k)])]++)] = i32(e);
+        }
+        DevBuf<T> HL{size_t(E) * 32}, HN{size_t(E) * 32}, pairs;
+        Weights4 w4{};
+        for (int q = 0; q < 4; ++q) w4.w[q] = m->fe.w[q];
+        hipLaunchKernelGGL(k_pair_prep<T>, dim3(blocks_for(E * 16, 256)), dim3(256), 0, s, E, m->D.p, w4, m->group_of.p,
+                           d_groups.p, HL.p, HN.p);
+        DevBuf<i64> d_eoff{size_t(E)}, d_cofs;
+        DevBuf<i32> d_list;
+        PairArgs<T> pr{};
+        pr.nb_off = m->nb_off.p; pr.nb_ids = m->nb_ids.p; pr.group_of = m->group_of.p;
+        pr.groups = d_groups.p; pr.xq = m->xq.p; pr.HL = HL.p; pr.HN = HN.p; pr.el_min = el_min.p;
+        pr.el_max = m->el_maxnode.p; pr.el_dir = el_dir.p; pr.eoff = d_eoff.p;
+        const i64 per_pair = 16 * i64(sizeof(T));
+        std::vector<i32> stamp(size_t(E), -1), list;
+        std::vector<i64> cofs;
+        int chunk = 0;
+        for (i64 n0 = 0; n0 < N; ++chunk) {
+            list.clear();
+            cofs.clear();
+            i64 total = 0, n1 = n0;
+            for (; n1 < N; ++n1) {
+                i64 add = 0;
+                for (i32 k = noff[size_t(n1)]; k < noff[size_t(n1 + 1)]; ++k) {
+                    const i32 e = nel[size_t(k)];
+                    if (stamp[size_t(e)] != chunk) add += nb_off[size_t(e + 1)] - nb_off[size_t(e)];
+                }
+                if (n1 > n0 && (total + add) * per_pair > i64(PAIR_BUDGET)) break;
+                for (i32 k = noff[size_t(n1)]; k < noff[size_t(n1 + 1)]; ++k) {
+                    const i32 e = nel[size_t(k)];
+                    if (stamp[size_t(e)] == chunk) continue;
+                    stamp[size_t(e)] = chunk;
+                    list.push_back(e);
+                    cofs.push_back(total);
+                    total += nb_off[size_t(e + 1)] - nb_off[size_t(e)];
+                }
+            }
+            AssemblyArgs ca = aa;
+            ca.node0 = n0;
+            if (!list.empty() && total > 0) {
+                upload(d_list, list.data(), list.size(), s);
+                upload(d_cofs, cofs.data(), cofs.size(), s);
+                hipLaunchKernelGGL(k_scatter_offsets, dim3(blocks_for(i64(list.size()), 256)), dim3(256), 0, s,
+                                   i64(list.size()), d_list.p, d_cofs.p, d_eoff.p);
+                if (pairs.n < size_t(total) * 16) pairs.alloc(size_t(total) * 16);
+                pr.count = i64(list.size()); pr.list = d_list.p; pr.pairs = pairs.p; pr.plane = total * 4;
+                const unsigned pb = unsigned(std::min<i64>((pr.count + 3) / 4, 16384));
+                if constexpr (F64) {
+                    hipLaunchKernelGGL(k_pairs_mfma, dim3(pb), dim3(256), 0, s, pr);
+                    ca.pairs = pairs.p;
+                } else {
+                    hipLaunchKernelGGL(k_pairs_mfma32, dim3(pb), dim3(256), 0, s, pr);
+                    ca.pairs32 = pairs.p;
+                }
+                NLF_HIP(hipGetLastError());
+                ca.plane = total * 4;
+                ca.eoff = d_eoff.p;
+            }
+            launch_assemble<1, 4, 4, T>(ca, det, lds, n1 - n0, s);
+            NLF_HIP(hipGetLastError());
+            NLF_HIP(hipStreamSynchronize(s));          // the next chunk rewrites the list and the offsets
+            n0 = n1;
+        }
+    } else if (mat->physics == 0 && m->nn == 4) launch_assemble<1, 4, 4, T>(aa, det, lds, m->N, s);
+    else if (mat->physics == 0 && m->nn == 8) launch_assemble<1, 8, 9, T>(aa, det, lds, m->N, s);
+    else if (mat->physics == 1 && m->nn == 4) launch_assemble<2, 4, 4, T>(aa, det, lds, m->N, s);
+    else launch_assemble<2, 8, 9, T>(aa, det, lds, m->N, s);
     NLF_HIP(hipGetLastError());
     T* values = nullptr;
     if constexpr (F64) values = mat->val.p; else values = mat->val32.p;

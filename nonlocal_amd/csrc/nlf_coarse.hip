@@ -1,0 +1,298 @@
+// Aggregation coarse space for the two-level learned preconditioner (no reference counterpart:
+// the reference's preconditioners are Eigen's diagonal and incomplete Cholesky).
+//
+//   B r = M r + P A_c^-1 P^T r
+//
+// P maps every fine row to one aggregate: the nodes are binned into a grid of square cells over
+// their bounding box (one aggregate per non-empty cell and DoF), so P^T r sums a cell's rows and
+// P y copies a cell's value back.  A_c = P^T A P is assembled densely by one workgroup per coarse
+// row; its entries are accumulated as 64-bit fixed-point integers in LDS, which makes the sum
+// independent of the order of the atomic additions (the result is reproducible bit for bit).
+// A_c^-1 is formed once by blocked Gauss-Jordan elimination on 64 x 64 tiles (A_c is symmetric
+// positive definite, so no pivoting), and each application is three small kernels: restriction,
+// a dense matrix-vector product with A_c^-1, and prolongation.
+#include "nlf_internal.hpp"
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+
+namespace nlf {
+
+namespace {
+
+constexpr int TB = 64;                     // tile of the blocked Gauss-Jordan elimination
+
+// Per-block maxima over rows of sum_j |a_ij|: one wave per row, waves striding over the rows.
+__global__ __launch_bounds__(256) void k_row_abs_max(i64 rows, const i64* __restrict__ outer,
+                                                     const double* __restrict__ val, double* __restrict__ part) {
+    __shared__ double wmax[4];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    double best = 0.0;
+    for (i64 r = i64(blockIdx.x) * 4 + wv; r < rows; r += i64(gridDim.x) * 4) {
+        double s = 0.0;
+        for (i64 p = outer[r] + lane; p < outer[r + 1]; p += 64) s += fabs(val[p]);
+        for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
+        best = fmax(best, s);
+    }
+    if (lane == 0) wmax[wv] = best;
+    __syncthreads();
+    if (threadIdx.x == 0) part[blockIdx.x] = fmax(fmax(wmax[0], wmax[1]), fmax(wmax[2], wmax[3]));
+}
+
+// Row I of A_c = P^T A P: the nonzeros of I's fine rows, added in fixed point into an LDS row.
+__global__ __launch_bounds__(256) void k_coarse_galerkin(i64 ncp, const i64* __restrict__ mem_off,
+                                                         const i32* __restrict__ mem, const i64* __restrict__ outer,
+                                                         const i32* __restrict__ cols, const double* __restrict__ val,
+                                                         const i32* __restrict__ agg, double scale,
+                                                         double* __restrict__ ac) {
+    extern __shared__ unsigned long long acc[];                // ncp entries
+    const i64 I = blockIdx.x;
+    for (i64 j = threadIdx.x; j < ncp; j += blockDim.x) acc[j] = 0ull;
+    __syncthreads();
+    for (i64 m = mem_off[I]; m < mem_off[I + 1]; ++m) {
+        const i32 r = mem[m];
+        for (i64 p = outer[r] + threadIdx.x; p < outer[r + 1]; p += blockDim.x) {
+            const long long q = llrint(val[p] * scale);
+            atomicAdd(&acc[agg[cols[p]]], static_cast<unsigned long long>(q));
+        }
+    }
+    __syncthreads();
+    const double inv_scale = 1.0 / scale;
+    for (i64 j = threadIdx.x; j < ncp; j += blockDim.x)
+        ac[I * ncp + j] = double(static_cast<long long>(acc[j])) * inv_scale;
+}
+
+__global__ void k_unit_diag(i64 from, i64 ncp, double* __restrict__ a) {
+    const i64 i = from + i64(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < ncp) a[i * ncp + i] = 1.0;
+}
+
+// In-place Gauss-Jordan inverse of the 64 x 64 diagonal tile (k, k) into dinv (no pivoting).
+__global__ __launch_bounds__(256) void k_gj_diag(const double* __restrict__ a, i64 ncp, i64 k, double* __restrict__ dinv) {
+    __shared__ double d[TB][TB + 1];
+    __shared__ double colk[TB];
+    const int t = threadIdx.x;
+    for (int e = t; e < TB * TB; e += 256) d[e / TB][e % TB] = a[(k * TB + e / TB) * ncp + k * TB + e % TB];
+    __syncthreads();
+    for (int p = 0; p < TB; ++p) {
+        const double piv = d[p][p];
+        __syncthreads();
+        if (t < TB) {
+            colk[t] = d[t][p];
+            d[p][t] = (t == p ? 1.0 : d[p][t]) / piv;
+        }
+        __syncthreads();
+        for (int e = t; e < TB * TB; e += 256) {
+            const int i = e / TB, j = e % TB;
+            if (i != p) d[i][j] = (j == p ? 0.0 : d[i][j]) - colk[i] * d[p][j];
+        }
+        __syncthreads();
+    }
+    for (int e = t; e < TB * TB; e += 256) dinv[e] = d[e / TB][e % TB];
+}
+
+// acc += X (64 x 64, leading dimension ldx) * Y (64 x 64, ldy); thread (tx, ty) owns rows 4 ty .. 4 ty + 3
+// and columns 4 tx .. 4 tx + 3.  The inner dimension is staged through LDS in two halves.
+__device__ __forceinline__ void tile_mm(const double* __restrict__ X, i64 ldx, const double* __restrict__ Y, i64 ldy,
+                                        double (&acc)[4][4], double (*sx)[33], double (*sy)[65]) {
+    const int t = threadIdx.x, tx = t % 16, ty = t / 16;
+    for (int h = 0; h < 2; ++h) {
+        __syncthreads();
+        for (int e = t; e < 64 * 32; e += 256) {
+            const int i = e / 32, j = e % 32;
+            sx[i][j] = X[i64(i) * ldx + h * 32 + j];
+        }
+        for (int e = t; e < 32 * 64; e += 256) {
+            const int i = e / 64, j = e % 64;
+            sy[i][j] = Y[i64(h * 32 + i) * ldy + j];
+        }
+        __syncthreads();
+        for (int q = 0; q < 32; ++q) {
+            double xa[4], yb[4];
+            for (int u = 0; u < 4; ++u) xa[u] = sx[ty * 4 + u][q];
+            for (int v = 0; v < 4; ++v) yb[v] = sy[q][tx * 4 + v];
+            for (int u = 0; u < 4; ++u)
+                for (int v = 0; v < 4; ++v) acc[u][v] += xa[u] * yb[v];
+        }
+    }
+}
+
+// Row panel: A(k, j) = D^-1 A(k, j) for every tile column j != k.
+__global__ __launch_bounds__(256) void k_gj_rowpanel(double* __restrict__ a, i64 ncp, i64 k, const double* __restrict__ dinv) {
+    __shared__ double sx[64][33], sy[32][65];
+    const i64 nb = ncp / TB;
+    const i64 j = blockIdx.x < k ? blockIdx.x : blockIdx.x + 1;
+    if (j >= nb) return;
+    double acc[4][4] = {};
+    double* blk = a + (k * TB) * ncp + j * TB;
+    tile_mm(dinv, TB, blk, ncp, acc, sx, sy);
+    __syncthreads();                                              // every read of the tile is done
+    const int t = threadIdx.x, tx = t % 16, ty = t / 16;
+    for (int u = 0; u < 4; ++u)
+        for (int v = 0; v < 4; ++v) blk[i64(ty * 4 + u) * ncp + tx * 4 + v] = acc[u][v];
+}
+
+// Trailing update: A(i, j) -= A(i, k) A(k, j) for tile rows i != k and tile columns j != k.
+__global__ __launch_bounds__(256) void k_gj_update(double* __restrict__ a, i64 ncp, i64 k) {
+    __shared__ double sx[64][33], sy[32][65];
+    const i64 bi = blockIdx.y < k ? blockIdx.y : blockIdx.y + 1;
+    const i64 bj = blockIdx.x < k ? blockIdx.x : blockIdx.x + 1;
+    const i64 nb = ncp / TB;
+    if (bi >= nb || bj >= nb) return;
+    double acc[4][4] = {};
+    tile_mm(a + (bi * TB) * ncp + k * TB, ncp, a + (k * TB) * ncp + bj * TB, ncp, acc, sx, sy);
+    const int t = threadIdx.x, tx = t % 16, ty = t / 16;
+    double* blk = a + (bi * TB) * ncp + bj * TB;
+    for (int u = 0; u < 4; ++u)
+        for (int v = 0; v < 4; ++v) blk[i64(ty * 4 + u) * ncp + tx * 4 + v] -= acc[u][v];
+}
+
+// Column panel: A(i, k) = -A(i, k) D^-1 for tile rows i != k.
+__global__ __launch_bounds__(256) void k_gj_colpanel(double* __restrict__ a, i64 ncp, i64 k, const double* __restrict__ dinv) {
+    __shared__ double sx[64][33], sy[32][65];
+    const i64 nb = ncp / TB;
+    const i64 i = blockIdx.x < k ? blockIdx.x : blockIdx.x + 1;
+    if (i >= nb) return;
+    double acc[4][4] = {};
+    double* blk = a + (i * TB) * ncp + k * TB;
+    tile_mm(blk, ncp, dinv, TB, acc, sx, sy);
+    __syncthreads();
+    const int t = threadIdx.x, tx = t % 16, ty = t / 16;
+    for (int u = 0; u < 4; ++u)
+        for (int v = 0; v < 4; ++v) blk[i64(ty * 4 + u) * ncp + tx * 4 + v] = -acc[u][v];
+}
+
+__global__ void k_gj_setdiag(double* __restrict__ a, i64 ncp, i64 k, const double* __restrict__ dinv) {
+    for (int e = threadIdx.x; e < TB * TB; e += blockDim.x) a[(k * TB + e / TB) * ncp + k * TB + e % TB] = dinv[e];
+}
+
+// rc[I] = sum of r over the fine rows of I (one wave per coarse row, fixed order).
+__global__ __launch_bounds__(256) void k_coarse_restrict(i64 nc, const i64* __restrict__ mem_off,
+                                                         const i32* __restrict__ mem, const double* __restrict__ r,
+                                                         double* __restrict__ rc) {
+    const i64 I = i64(blockIdx.x) * 4 + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (I >= nc) return;
+    double s = 0.0;
+    for (i64 m = mem_off[I] + lane; m < mem_off[I + 1]; m += 64) s += r[mem[m]];
+    for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
+    if (lane == 0) rc[I] = s;
+}
+
+// yc = A_c^-1 rc (one wave per row, fixed order).
+__global__ __launch_bounds__(256) void k_coarse_gemv(i64 ncp, const double* __restrict__ inv,
+                                                     const double* __restrict__ rc, double* __restrict__ yc) {
+    const i64 I = i64(blockIdx.x) * 4 + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (I >= ncp) return;
+    const double* row = inv + I * ncp;
+    double s = 0.0;
+    for (i64 j = lane; j < ncp; j += 64) s += row[j] * rc[j];
+    for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
+    if (lane == 0) yc[I] = s;
+}
+
+__global__ void k_coarse_prolong(i64 n, const i32* __restrict__ agg, const double* __restrict__ yc, double* __restrict__ z) {
+    for (i64 i = i64(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += i64(gridDim.x) * blockDim.x) z[i] += yc[agg[i]];
+}
+
+}  // namespace
+
+std::unique_ptr<CoarseSpace> build_coarse(const nlf_matrix_s* a, i64 target, hipStream_t s) {
+    const auto t0 = std::chrono::steady_clock::now();
+    const nlf_mesh m = a->mesh;
+    const int dof = a->dof;
+    const i64 N = m->N, n = a->rows;
+    NLF_CHECK(target > 0, NLF_ERR_ARGUMENT, "coarse target must be positive");
+    // at least 16 nodes per aggregate on average, at most 4096 coarse rows (dense inverse in LDS-tiled
+    // Gauss-Jordan, one 32 KiB LDS row per coarse row in the Galerkin product)
+    target = std::min<i64>(target, std::max<i64>(1, N / 16));
+    target = std::min<i64>(target, 4096 / dof);
+    double xmin = 1e308, xmax = -1e308, ymin = 1e308, ymax = -1e308;
+    for (i64 v = 0; v < N; ++v) {
+        xmin = std::min(xmin, m->xy_h[2 * v]); xmax = std::max(xmax, m->xy_h[2 * v]);
+        ymin = std::min(ymin, m->xy_h[2 * v + 1]); ymax = std::max(ymax, m->xy_h[2 * v + 1]);
+    }
+    const double w = std::max(xmax - xmin, 1e-300), h = std::max(ymax - ymin, 1e-300);
+    const i64 cx = std::min<i64>(target, std::max<i64>(1, i64(std::lround(std::sqrt(double(target) * w / h)))));
+    const i64 cy = std::max<i64>(1, target / cx);
+    const double hx = w / double(cx), hy = h / double(cy);
+    // cell of every node, then the non-empty cells numbered in cell order
+    std::vector<i64> cell(static_cast<size_t>(N));
+    std::vector<i32> used(static_cast<size_t>(cx * cy), 0);
+    for (i64 v = 0; v < N; ++v) {
+        const i64 ix = std::min<i64>(cx - 1, i64((m->xy_h[2 * v] - xmin) / hx));
+        const i64 iy = std::min<i64>(cy - 1, i64((m->xy_h[2 * v + 1] - ymin) / hy));
+        cell[size_t(v)] = ix * cy + iy;
+        used[size_t(cell[size_t(v)])] = 1;
+    }
+    std::vector<i64> cid(static_cast<size_t>(cx * cy), -1);
+    i64 nagg = 0;
+    for (i64 c = 0; c < cx * cy; ++c)
+        if (used[size_t(c)]) cid[size_t(c)] = nagg++;
+    auto cs = std::make_unique<CoarseSpace>();
+    cs->n = n;
+    cs->nc = nagg * dof;
+    cs->ncp = (cs->nc + TB - 1) / TB * TB;
+    std::vector<i32> agg(static_cast<size_t>(n));
+    std::vector<i64> cnt(static_cast<size_t>(cs->nc + 1), 0);
+    for (i64 r = 0; r < n; ++r) {
+        agg[size_t(r)] = i32(cid[size_t(cell[size_t(r / dof)])] * dof + r % dof);
+        ++cnt[size_t(agg[size_t(r)]) + 1];
+    }
+    for (i64 I = 0; I < cs->nc; ++I) cnt[size_t(I + 1)] += cnt[size_t(I)];
+    std::vector<i32> mem(static_cast<size_t>(n));
+    std::vector<i64> fill(cnt.begin(), cnt.end() - 1);
+    for (i64 r = 0; r < n; ++r) mem[size_t(fill[size_t(agg[size_t(r)])]++)] = i32(r);   // rows ascending per aggregate
+    upload(cs->agg, agg.data(), agg.size(), s);
+    upload(cs->mem_off, cnt.data(), cnt.size(), s);
+    upload(cs->mem, mem.data(), mem.size(), s);
+    // fixed-point scale: every partial sum of a coarse entry is bounded by (rows per aggregate) x
+    // (largest absolute row sum); keep it below 2^61
+    i64 maxmem = 0;
+    for (i64 I = 0; I < cs->nc; ++I) maxmem = std::max(maxmem, cnt[size_t(I + 1)] - cnt[size_t(I)]);
+    constexpr int ABS_BLOCKS = 2048;
+    DevBuf<double> d_abs{ABS_BLOCKS};
+    hipLaunchKernelGGL(k_row_abs_max, dim3(ABS_BLOCKS), dim3(256), 0, s, n, a->outer.p, a->val.p, d_abs.p);
+    std::vector<double> h_abs(ABS_BLOCKS);
+    download(h_abs.data(), d_abs.p, h_abs.size(), s);
+    const double rowabs = *std::max_element(h_abs.begin(), h_abs.end());
+    const double bound = std::max(1e-300, rowabs * double(std::max<i64>(maxmem, 1)));
+    const double scale = std::ldexp(1.0, std::clamp(61 - int(std::ceil(std::log2(bound))), -1000, 1000));
+    cs->inv.alloc(size_t(cs->ncp * cs->ncp));
+    NLF_HIP(hipMemsetAsync(cs->inv.p, 0, cs->inv.bytes(), s));
+    hipLaunchKernelGGL(k_coarse_galerkin, dim3(unsigned(cs->nc)), dim3(256), size_t(cs->ncp) * sizeof(unsigned long long), s,
+                       cs->ncp, cs->mem_off.p, cs->mem.p, a->outer.p, a->cols.p, a->val.p, cs->agg.p, scale, cs->inv.p);
+    if (cs->ncp > cs->nc)
+        hipLaunchKernelGGL(k_unit_diag, dim3(1), dim3(64), 0, s, cs->nc, cs->ncp, cs->inv.p);
+    NLF_HIP(hipGetLastError());
+    // blocked Gauss-Jordan inversion in place
+    const i64 nb = cs->ncp / TB;
+    DevBuf<double> dinv{size_t(TB * TB)};
+    for (i64 k = 0; k < nb; ++k) {
+        hipLaunchKernelGGL(k_gj_diag, dim3(1), dim3(256), 0, s, cs->inv.p, cs->ncp, k, dinv.p);
+        if (nb > 1) {
+            hipLaunchKernelGGL(k_gj_rowpanel, dim3(unsigned(nb - 1)), dim3(256), 0, s, cs->inv.p, cs->ncp, k, dinv.p);
+            hipLaunchKernelGGL(k_gj_update, dim3(unsigned(nb - 1), unsigned(nb - 1)), dim3(256), 0, s, cs->inv.p, cs->ncp, k);
+            hipLaunchKernelGGL(k_gj_colpanel, dim3(unsigned(nb - 1)), dim3(256), 0, s, cs->inv.p, cs->ncp, k, dinv.p);
+        }
+        hipLaunchKernelGGL(k_gj_setdiag, dim3(1), dim3(256), 0, s, cs->inv.p, cs->ncp, k, dinv.p);
+    }
+    NLF_HIP(hipGetLastError());
+    cs->rc.alloc(size_t(cs->ncp));
+    cs->yc.alloc(size_t(cs->ncp));
+    NLF_HIP(hipMemsetAsync(cs->rc.p, 0, cs->rc.bytes(), s));
+    NLF_HIP(hipStreamSynchronize(s));
+    cs->setup_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    return cs;
+}
+
+void coarse_apply_add(const CoarseSpace& c, const double* r, double* z, hipStream_t s) {
+    hipLaunchKernelGGL(k_coarse_restrict, dim3(blocks_for(c.nc, 4)), dim3(256), 0, s, c.nc, c.mem_off.p, c.mem.p, r, c.rc.p);
+    hipLaunchKernelGGL(k_coarse_gemv, dim3(blocks_for(c.ncp, 4)), dim3(256), 0, s, c.ncp, c.inv.p, c.rc.p, c.yc.p);
+    hipLaunchKernelGGL(k_coarse_prolong, dim3(unsigned(std::min<i64>(blocks_for(c.n, 256), 4096))), dim3(256), 0, s, c.n,
+                       c.agg.p, c.yc.p, z);
+}
+
+}  // namespace nlf

@@ -258,6 +258,23 @@ struct SpmvCols {
     const i64* pstart;
 };
 i64 spmv_stream_bytes(const nlf_matrix_s* a, size_t sv, size_t sx);   // HBM bytes of one SpMV
+
+// Aggregation coarse space for the learned preconditioner's two-level form:
+// B r = M r + P A_c^-1 P^T r, P the 0/1 map of rows to aggregates (a grid of cells over the node
+// coordinates, one coarse row per aggregate and DoF), A_c = P^T A P kept as its dense inverse.
+struct CoarseSpace {
+    i64 n = 0;                 // fine rows
+    i64 nc = 0;                // coarse rows
+    i64 ncp = 0;               // coarse rows padded to a multiple of 64 (identity padding)
+    DevBuf<i32> agg;           // [n] coarse row of each fine row
+    DevBuf<i64> mem_off;       // [nc + 1] fine rows of each coarse row (ascending)
+    DevBuf<i32> mem;           // [n]
+    DevBuf<double> inv;        // [ncp * ncp] A_c^-1, row-major
+    DevBuf<double> rc, yc;     // [ncp] restricted residual, coarse correction
+    double setup_seconds = 0.0;
+};
+std::unique_ptr<CoarseSpace> build_coarse(const nlf_matrix_s* a, i64 target, hipStream_t s);
+void coarse_apply_add(const CoarseSpace& c, const double* r, double* z, hipStream_t s);   // z += P A_c^-1 P^T r
 inline SpmvCols spmv_cols(const nlf_matrix_s* a) {
     if (a->pstart.p) return {a->cols.p, a->pat.p, a->pstart.p};
     return {a->cols.p, a->off16.p, nullptr};

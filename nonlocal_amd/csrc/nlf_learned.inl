@@ -58,6 +58,7 @@ struct LearnedGraph {
     DevBuf<float> m_val;
     double compile_seconds = 0.0;
     bool network_apply = false;                 // run the network kernels even when M is compiled
+    std::unique_ptr<CoarseSpace> coarse;        // two-level form: + P A_c^-1 P^T r (nlf_learned_set_coarse)
     DevBuf<float> g0, g1, gdir, gm;             // backward work
     DevBuf<double> part, sigma, zbuf, az;
     DevBuf<float> gpart, grad;
@@ -1217,7 +1218,7 @@ void learned_apply_ell(LearnedGraph& g, const float* w, const double* r, Z* z, h
 // z = M r, the symmetric part of N, on the lane-per-node kernels: forward layers (A, B channel-major
 // ping-pong), y; transposed layers (d in place, q through A then B) ending in the output.
 template<class Z>
-void learned_apply(LearnedGraph& g, const float* w, const double* r, Z* z, hipStream_t s) {
+void learned_apply_one(LearnedGraph& g, const float* w, const double* r, Z* z, hipStream_t s) {
     if (g.m_ready && !g.network_apply) {
         const int lpr = pick_lpr(double(g.m_nnz) / double(std::max<i64>(g.n, 1)), g.m_off16.p != nullptr);
         const SpmvCols cs = g.m_pstart.p ? SpmvCols{g.m_cols.p, g.m_pat.p, g.m_pstart.p}
@@ -1233,6 +1234,14 @@ void learned_apply(LearnedGraph& g, const float* w, const double* r, Z* z, hipSt
         case 24: learned_apply_ell<24, Z>(g, w, r, z, s); break;
         default: learned_apply_ell<32, Z>(g, w, r, z, s); break;
     }
+}
+
+// z = B r: M r, plus the coarse correction P A_c^-1 P^T r when the two-level form is on.
+template<class Z>
+void learned_apply(LearnedGraph& g, const float* w, const double* r, Z* z, hipStream_t s) {
+    learned_apply_one<Z>(g, w, r, z, s);
+    if constexpr (sizeof(Z) == sizeof(double))
+        if (g.coarse) coarse_apply_add(*g.coarse, r, z, s);
 }
 
 // Loss ||N(r) - e||_A^2 / ||e||_A^2 and its gradient w.r.t. the weights (into g.grad).
@@ -1551,6 +1560,28 @@ int nlf_learned_info(nlf_precond p, int* compiled, int64_t* nnz, double* compile
         if (compiled) *compiled = g.m_ready ? 1 : 0;
         if (nnz) *nnz = g.m_ready ? g.m_nnz : 0;
         if (compile_seconds) *compile_seconds = g.compile_seconds;
+    });
+}
+
+// Two-level form: aggregates > 0 builds the coarse correction P A_c^-1 P^T from the system matrix
+// a (about that many aggregates: a grid of cells over the nodes), 0 removes it.
+int nlf_learned_set_coarse(nlf_precond p, nlf_matrix a, int64_t aggregates) {
+    if (!p) return NLF_ERR_ARGUMENT;
+    return guarded(p->ctx, [&] {
+        LearnedGraph& g = *learned_of(p);
+        g.coarse.reset();
+        if (aggregates <= 0) return;
+        NLF_CHECK(a && a->rows == g.n, NLF_ERR_ARGUMENT, "the coarse space needs the system matrix of the preconditioner");
+        g.coarse = build_coarse(a, aggregates, p->ctx->stream);
+    });
+}
+
+int nlf_learned_coarse_info(nlf_precond p, int64_t* coarse_rows, double* setup_seconds) {
+    if (!p) return NLF_ERR_ARGUMENT;
+    return guarded(p->ctx, [&] {
+        LearnedGraph& g = *learned_of(p);
+        if (coarse_rows) *coarse_rows = g.coarse ? g.coarse->nc : 0;
+        if (setup_seconds) *setup_seconds = g.coarse ? g.coarse->setup_seconds : 0.0;
     });
 }
 

@@ -64,6 +64,9 @@ def load_weights(path=DEFAULT_WEIGHTS):
     return w, dict(seed=seed, config_sha256=data[20:52].hex())
 
 
+DEFAULT_AGGREGATES = 4096
+
+
 class LearnedPreconditioner:
     """preconditioner_base implementation backed by the message-passing network."""
 
@@ -97,6 +100,18 @@ class LearnedPreconditioner:
     def set_apply(self, network: bool):
         """Apply through the network kernels (True) or the compiled sparse operator (False, default)."""
         self.ctx.check(load().nlf_learned_set_apply(self.h, int(network)))
+
+    def set_coarse(self, aggregates: int = DEFAULT_AGGREGATES, matrix: api.finite_element_matrix | None = None):
+        """Two-level form B r = M r + P A_c^-1 P^T r with about `aggregates` aggregates of the system
+        matrix (default: the one the preconditioner was built for); 0 removes the coarse part."""
+        m = matrix if matrix is not None else self.matrix
+        self.ctx.check(load().nlf_learned_set_coarse(self.h, m.h, int(aggregates)))
+
+    def coarse_info(self):
+        """(coarse rows, set-up seconds); (0, 0.0) without the coarse part."""
+        rows, sec = C.c_int64(), C.c_double()
+        self.ctx.check(load().nlf_learned_coarse_info(self.h, C.byref(rows), C.byref(sec)))
+        return rows.value, sec.value
 
     def get_weights(self):
         w = np.zeros(num_params(), dtype=np.float32)
