@@ -15,6 +15,8 @@ NLF_PRECOND_IDENTITY, NLF_PRECOND_JACOBI, NLF_PRECOND_ILLT, NLF_PRECOND_LEARNED 
 NLF_MODE_FP64, NLF_MODE_MIXED, NLF_MODE_SPLIT = 0, 1, 2
 NLF_OPT_ASSEMBLY_PAIRS = 1
 NLF_OPT_ROW_PATTERNS = 2
+NLF_OPT_ASSEMBLY_THREADS = 3
+NLF_OPT_PATTERN_THREADS = 4
 
 
 class Model(C.Structure):

@@ -120,21 +120,31 @@ __device__ __forceinline__ void for_each_column_element(const PatternArgs& a, i6
 }
 
 // exclusive block scan of up to 4 counters
-__device__ __forceinline__ void block_scan4(i64 v[4], i64 total[4], i64* sh /* 4*(threads+1) */) {
-    const int t = threadIdx.x, T = blockDim.x;
-    for (int k = 0; k < 4; ++k) sh[k * (T + 1) + t] = v[k];
-    __syncthreads();
-    if (t < 4) {
-        i64 run = 0;
-        for (int i = 0; i < T; ++i) {
-            const i64 x = sh[t * (T + 1) + i];
-            sh[t * (T + 1) + i] = run;
-            run += x;
+// Exclusive block scan of four counters per thread (v in, exclusive prefix out) and their block
+// totals: wave prefix sums by shuffles, then the wave totals (at most 16 waves) through LDS.
+__device__ __forceinline__ void block_scan4(i64 v[4], i64 total[4], i64* sh /* >= 4 * waves */) {
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6, W = int(blockDim.x + 63) >> 6;
+    i64 excl[4];
+    for (int k = 0; k < 4; ++k) {
+        i64 x = v[k];
+        for (int o = 1; o < 64; o <<= 1) {
+            const i64 y = __shfl_up(x, o);
+            if (lane >= o) x += y;
         }
-        sh[t * (T + 1) + T] = run;
+        excl[k] = x - v[k];
+        if (lane == 63) sh[k * W + w] = x;
     }
     __syncthreads();
-    for (int k = 0; k < 4; ++k) { v[k] = sh[k * (T + 1) + t]; total[k] = sh[k * (T + 1) + T]; }
+    for (int k = 0; k < 4; ++k) {
+        i64 off = 0, tot = 0;
+        for (int i = 0; i < W; ++i) {
+            const i64 s = sh[k * W + i];
+            if (i < w) off += s;
+            tot += s;
+        }
+        v[k] = excl[k] + off;
+        total[k] = tot;
+    }
     __syncthreads();
 }
 
@@ -541,8 +551,8 @@ __global__ __launch_bounds__(256) void k_pairs_mfma32(PairArgs<float> a) {
 
 // T = double: the FP64 value pass (inner and bound parts).  T = float: the FP32 value pass with
 // FP32 accumulation into val32 (inner part only).
-template<int DOF, int NN, int NQ, bool DETERMINISTIC, class T>
-__global__ __launch_bounds__(ASM_THREADS) void k_assemble(AssemblyArgs a) {
+template<int DOF, int NN, int NQ, bool DETERMINISTIC, class T, int THREADS = ASM_THREADS>
+__global__ __launch_bounds__(THREADS) void k_assemble(AssemblyArgs a) {
     constexpr bool F64 = sizeof(T) == sizeof(double);
     extern __shared__ unsigned char smem[];
     __shared__ double xl[2 * NQ];
@@ -876,12 +886,24 @@ __global__ __launch_bounds__(256) void k_compare32(i64 n, const double* __restri
     if (threadIdx.x < 4) part[blockIdx.x * 4 + threadIdx.x] = red[threadIdx.x][0];
 }
 
-// nodes [a.node0, a.node0 + count)
+// nodes [a.node0, a.node0 + count); threads per workgroup 64, 128, 256 or 512
 template<int DOF, int NN, int NQ, class T>
-void launch_assemble(const AssemblyArgs& a, bool det, size_t lds, i64 count, hipStream_t s) {
+void launch_assemble(const AssemblyArgs& a, bool det, size_t lds, i64 count, hipStream_t s, int threads = ASM_THREADS) {
     if (count <= 0) return;
-    if (det) hipLaunchKernelGGL((k_assemble<DOF, NN, NQ, true, T>), dim3(unsigned(count)), dim3(ASM_THREADS), lds, s, a);
-    else hipLaunchKernelGGL((k_assemble<DOF, NN, NQ, false, T>), dim3(unsigned(count)), dim3(ASM_THREADS), lds, s, a);
+    const dim3 g(static_cast<unsigned>(count));
+    if (threads == 512) {
+        if (det) hipLaunchKernelGGL((k_assemble<DOF, NN, NQ, true, T, 512>), g, dim3(512), lds, s, a);
+        else hipLaunchKernelGGL((k_assemble<DOF, NN, NQ, false, T, 512>), g, dim3(512), lds, s, a);
+    } else if (threads == 128) {
+        if (det) hipLaunchKernelGGL((k_assemble<DOF, NN, NQ, true, T, 128>), g, dim3(128), lds, s, a);
+        else hipLaunchKernelGGL((k_assemble<DOF, NN, NQ, false, T, 128>), g, dim3(128), lds, s, a);
+    } else if (threads == 64) {
+        if (det) hipLaunchKernelGGL((k_assemble<DOF, NN, NQ, true, T, 64>), g, dim3(64), lds, s, a);
+        else hipLaunchKernelGGL((k_assemble<DOF, NN, NQ, false, T, 64>), g, dim3(64), lds, s, a);
+    } else {
+        if (det) hipLaunchKernelGGL((k_assemble<DOF, NN, NQ, true, T>), g, dim3(ASM_THREADS), lds, s, a);
+        else hipLaunchKernelGGL((k_assemble<DOF, NN, NQ, false, T>), g, dim3(ASM_THREADS), lds, s, a);
+    }
 }
 
 }  // namespace nlf
@@ -1047,15 +1069,15 @@ void assemble_values(nlf_matrix_s* mat, hipStream_t s) {
                 ca.plane = total * 4;
                 ca.eoff = d_eoff.p;
             }
-            launch_assemble<1, 4, 4, T>(ca, det, lds, n1 - n0, s);
+            launch_assemble<1, 4, 4, T>(ca, det, lds, n1 - n0, s, m->ctx->assembly_threads);
             NLF_HIP(hipGetLastError());
             NLF_HIP(hipStreamSynchronize(s));          // the next chunk rewrites the list and the offsets
             n0 = n1;
         }
-    } else if (mat->physics == 0 && m->nn == 4) launch_assemble<1, 4, 4, T>(aa, det, lds, m->N, s);
-    else if (mat->physics == 0 && m->nn == 8) launch_assemble<1, 8, 9, T>(aa, det, lds, m->N, s);
-    else if (mat->physics == 1 && m->nn == 4) launch_assemble<2, 4, 4, T>(aa, det, lds, m->N, s);
-    else launch_assemble<2, 8, 9, T>(aa, det, lds, m->N, s);
+    } else if (mat->physics == 0 && m->nn == 4) launch_assemble<1, 4, 4, T>(aa, det, lds, m->N, s, m->ctx->assembly_threads);
+    else if (mat->physics == 0 && m->nn == 8) launch_assemble<1, 8, 9, T>(aa, det, lds, m->N, s, m->ctx->assembly_threads);
+    else if (mat->physics == 1 && m->nn == 4) launch_assemble<2, 4, 4, T>(aa, det, lds, m->N, s, m->ctx->assembly_threads);
+    else launch_assemble<2, 8, 9, T>(aa, det, lds, m->N, s, m->ctx->assembly_threads);
     NLF_HIP(hipGetLastError());
     T* values = nullptr;
     if constexpr (F64) values = mat->val.p; else values = mat->val32.p;
@@ -1171,7 +1193,8 @@ int nlf_assemble(nlf_mesh m, int physics, const nlf_model* models, const void* m
         NLF_CHECK(int(words) <= PATTERN_MAX_WORDS, NLF_ERR_UNSUPPORTED,
                   "column node ids of a row span more than 393k ids (renumber the mesh)");
         const size_t lds_bitmap = std::max<size_t>(words, 1) * sizeof(unsigned int);
-        hipLaunchKernelGGL(k_pattern<0>, dim3(unsigned(m->N)), dim3(PATTERN_THREADS), lds_bitmap, s, pa);
+        const int pth = m->ctx->pattern_threads;
+        hipLaunchKernelGGL(k_pattern<0>, dim3(unsigned(m->N)), dim3(pth), lds_bitmap, s, pa);
         NLF_HIP(hipGetLastError());
         int ovf = 0;
         download(&ovf, overflow.p, 1, s);
@@ -1185,7 +1208,7 @@ int nlf_assemble(nlf_mesh m, int physics, const nlf_model* models, const void* m
         mat->b_cols.alloc(size_t(std::max<i64>(mat->nnz_bound, 1)));
         pa.outer = mat->outer.p; pa.b_outer = mat->b_outer.p; pa.cols = mat->cols.p; pa.b_cols = mat->b_cols.p;
         pa.diag = mat->diag.p;
-        hipLaunchKernelGGL(k_pattern<1>, dim3(unsigned(m->N)), dim3(PATTERN_THREADS), lds_bitmap, s, pa);
+        hipLaunchKernelGGL(k_pattern<1>, dim3(unsigned(m->N)), dim3(pth), lds_bitmap, s, pa);
         NLF_HIP(hipGetLastError());
         NLF_HIP(hipStreamSynchronize(s));
         const auto t1 = std::chrono::steady_clock::now();

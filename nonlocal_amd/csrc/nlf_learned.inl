@@ -1362,6 +1362,29 @@ __global__ void k_fcg_p(i64 n, double* __restrict__ p, const double* __restrict_
     for (i64 i = i64(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += i64(gridDim.x) * blockDim.x) p[i] = z[i] + b * p[i];
 }
 
+// Per-block longest row and bandwidth max |col - row| of a CSR pattern (partials: 2 per block).
+__global__ __launch_bounds__(256) void k_graph_stats(i64 n, const i64* __restrict__ outer, const i32* __restrict__ cols,
+                                                     i64* __restrict__ part) {
+    __shared__ i64 sd[256], sb[256];
+    i64 deg = 0, band = 0;
+    for (i64 u = i64(blockIdx.x) * blockDim.x + threadIdx.x; u < n; u += i64(gridDim.x) * blockDim.x) {
+        const i64 b = outer[u], e = outer[u + 1];
+        deg = std::max(deg, e - b);
+        if (e > b) band = std::max(band, std::max(u - i64(cols[b]), i64(cols[e - 1]) - u));
+    }
+    sd[threadIdx.x] = deg;
+    sb[threadIdx.x] = band;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (int(threadIdx.x) < w) {
+            sd[threadIdx.x] = std::max(sd[threadIdx.x], sd[threadIdx.x + w]);
+            sb[threadIdx.x] = std::max(sb[threadIdx.x], sb[threadIdx.x + w]);
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) { part[2 * blockIdx.x] = sd[0]; part[2 * blockIdx.x + 1] = sb[0]; }
+}
+
 template<class V>
 i64 fcg(hipStream_t s, nlf_matrix A, const V* val, nlf_precond M, const double* b, double* x, double tol, i64 max_it,
         double* true_res, i64* updates, int window, KernelTimer& timer, const int16_t* lo = nullptr) {
@@ -1373,8 +1396,7 @@ i64 fcg(hipStream_t s, nlf_matrix A, const V* val, nlf_precond M, const double* 
     DevBuf<double> r{size_t(n)}, z{size_t(n)}, p{size_t(n)}, q{size_t(n)}, pa{np}, pb{np}, sums{2};
     DevBuf<FCGState> st{1};
     auto true_residual = [&]() {
-        hipLaunchKernelGGL((k_residual<double, double>), dim3(rb), dim3(SPMV_THREADS), 0, s, n, A->outer.p, A->cols.p, A->val.p,
-                           x, b, r.p, pa.p, pb.p);
+        residual64(n, A->outer.p, spmv_cols(A), A->val.p, pick_lpr(A), x, b, r.p, pa.p, pb.p, rb, s);
         hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(1024), 0, s, pa.p, int(rb), sums.p);
         hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(1024), 0, s, pb.p, int(rb), sums.p + 1);
         double h[2];
@@ -1495,18 +1517,23 @@ int nlf_learned_create(nlf_matrix A, nlf_matrix A_local, const nlf_model* models
         auto g = std::make_unique<LearnedGraph>();
         g->n = A->rows;
         g->nnz = A_local->nnz_full;
-        std::vector<i64> lo(size_t(g->n + 1));
-        download(lo.data(), A_local->outer.p, lo.size(), s);
+        // the graph is the local pattern, copied on the device; its longest row and bandwidth are
+        // reduced on the device
+        g->outer.alloc(size_t(g->n + 1));
+        g->cols.alloc(size_t(std::max<i64>(g->nnz, 1)));
+        NLF_HIP(hipMemcpyAsync(g->outer.p, A_local->outer.p, g->outer.bytes(), hipMemcpyDeviceToDevice, s));
+        NLF_HIP(hipMemcpyAsync(g->cols.p, A_local->cols.p, size_t(g->nnz) * sizeof(i32), hipMemcpyDeviceToDevice, s));
+        constexpr int GS_BLOCKS = 1024;
+        DevBuf<i64> d_stats{size_t(2 * GS_BLOCKS)};
+        hipLaunchKernelGGL(k_graph_stats, dim3(GS_BLOCKS), dim3(256), 0, s, g->n, g->outer.p, g->cols.p, d_stats.p);
+        std::vector<i64> h_stats(size_t(2 * GS_BLOCKS));
+        download(h_stats.data(), d_stats.p, h_stats.size(), s);
         i64 maxdeg = 0;
-        for (i64 u = 0; u < g->n; ++u) maxdeg = std::max(maxdeg, lo[u + 1] - lo[u]);
+        for (int b = 0; b < GS_BLOCKS; ++b) {
+            maxdeg = std::max(maxdeg, h_stats[size_t(2 * b)]);
+            g->band = std::max(g->band, h_stats[size_t(2 * b + 1)]);
+        }
         NLF_CHECK(maxdeg <= MP_MAXDEG, NLF_ERR_UNSUPPORTED, "local pattern has a row longer than the staged edge list");
-        upload(g->outer, lo.data(), lo.size(), s);
-        std::vector<i32> lc(size_t(g->nnz));
-        download(lc.data(), A_local->cols.p, lc.size(), s);
-        upload(g->cols, lc.data(), lc.size(), s);
-        for (i64 u = 0; u < g->n; ++u)
-            if (lo[u + 1] > lo[u])
-                g->band = std::max(g->band, std::max(u - i64(lc[size_t(lo[u])]), i64(lc[size_t(lo[u + 1] - 1)]) - u));
         g->s.alloc(size_t(g->n));
         g->feat.alloc(size_t(4 * g->nnz));
         g->w.alloc(NPARAM);
@@ -1709,8 +1736,7 @@ int nlf_fcg_solve(nlf_matrix A, nlf_precond M, nlf_precond fallback, const doubl
                 const i64 more = cg.solve(b, x, pcg_tol, left, &out.residual, &breakdown);
                 total += more;
                 left -= more;
-                hipLaunchKernelGGL((k_residual<double, double>), dim3(rb), dim3(SPMV_THREADS), 0, s, A->rows, A->outer.p,
-                                   A->cols.p, A->val.p, x, b, r.p, prr.p, pbb.p);
+                residual64(A->rows, A->outer.p, spmv_cols(A), A->val.p, pick_lpr(A), x, b, r.p, prr.p, pbb.p, rb, s);
                 hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(1024), 0, s, prr.p, int(rb), sums.p);
                 hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(1024), 0, s, pbb.p, int(rb), sums.p + 1);
                 double h[2];

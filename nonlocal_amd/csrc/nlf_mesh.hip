@@ -207,11 +207,20 @@ const char* nlf_last_error(nlf_ctx ctx) { return ctx ? ctx->error.c_str() : "nul
 // Context options (no reference counterpart): NLF_OPT_ASSEMBLY_PAIRS 1 = MFMA element-pair
 // contractions where implemented (thermal Q4, default), 0 = node-centric contraction;
 // NLF_OPT_ROW_PATTERNS 1 = shared row-pattern dictionary for the SpMV of matrices assembled
-// afterwards (default), 0 = per-nonzero 16-bit offsets.
+// afterwards (default), 0 = per-nonzero 16-bit offsets; NLF_OPT_ASSEMBLY_THREADS = threads per
+// node workgroup of the node-centric value pass (64 default, 128, 256, 512); NLF_OPT_PATTERN_THREADS
+// = threads per node workgroup of the pattern build (64, 128, 256 default).
 int nlf_ctx_set_option(nlf_ctx ctx, int option, int value) {
     if (!ctx) return NLF_ERR_ARGUMENT;
     if (option == NLF_OPT_ASSEMBLY_PAIRS) ctx->assembly_pairs = value != 0;
     else if (option == NLF_OPT_ROW_PATTERNS) ctx->row_patterns = value != 0;
+    else if (option == NLF_OPT_ASSEMBLY_THREADS) {
+        if (value != 64 && value != 128 && value != 256 && value != 512) return NLF_ERR_ARGUMENT;
+        ctx->assembly_threads = value;
+    } else if (option == NLF_OPT_PATTERN_THREADS) {
+        if (value != 64 && value != 128 && value != 256) return NLF_ERR_ARGUMENT;
+        ctx->pattern_threads = value;
+    }
     else return NLF_ERR_ARGUMENT;
     return NLF_OK;
 }

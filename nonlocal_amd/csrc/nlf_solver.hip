@@ -1127,6 +1127,31 @@ struct KernelTimer {
     double avg(int k) const { return count[k] ? total_ms[k] / double(count[k]) : 0.0; }
 };
 
+// r = b - r (r holds A x on entry); per-block partials of r.r and b.b.
+__global__ __launch_bounds__(SPMV_THREADS) void k_resid_finish(i64 n, const double* __restrict__ b, double* __restrict__ r,
+                                                               double* __restrict__ prr, double* __restrict__ pbb) {
+    __shared__ double sh[SPMV_THREADS / 64];
+    double rr = 0.0, bb = 0.0;
+    for (i64 i = i64(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += i64(gridDim.x) * blockDim.x) {
+        const double bi = b[i], ri = bi - r[i];
+        r[i] = ri;
+        rr += ri * ri;
+        bb += bi * bi;
+    }
+    rr = block_sum<double>(rr, sh);
+    __syncthreads();
+    bb = block_sum<double>(bb, sh);
+    if (threadIdx.x == 0) { prr[blockIdx.x] = rr; pbb[blockIdx.x] = bb; }
+}
+
+// FP64 true residual r = b - A x through the SpMV kernels, with nb partials of r.r and b.b
+// (nb <= SPMV_MAX_BLOCKS; k_sum_partials reduces them).
+void residual64(i64 n, const i64* outer, SpmvCols cs, const double* val, int lpr, const double* x, const double* b,
+                double* r, double* prr, double* pbb, unsigned nb, hipStream_t s) {
+    launch_spmv<double, double, double, false>(lpr, n, outer, cs, val, x, r, nullptr, nullptr, s);
+    hipLaunchKernelGGL(k_resid_finish, dim3(nb), dim3(SPMV_THREADS), 0, s, n, b, r, prr, pbb);
+}
+
 // Preconditioned CG on device vectors (conjugate_gradient::solve).  Vectors of type X, matrix values V.
 template<class V, class X>
 struct CG {
@@ -1215,8 +1240,7 @@ struct CG {
         DevBuf<double> r64{size_t(n)}, sums{2};
         DevBuf<X> xh{size_t(n)};
         auto true_residual = [&]() {
-            hipLaunchKernelGGL((k_residual<double, double>), dim3(rb), dim3(SPMV_THREADS), 0, s, n, o64, c64, v64, x, b,
-                               r64.p, part1.p, part2.p);
+            residual64(n, o64, cs, v64, lpr, x, b, r64.p, part1.p, part2.p, rb, s);
             hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(1024), 0, s, part1.p, int(rb), sums.p);
             hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(1024), 0, s, part2.p, int(rb), sums.p + 1);
             double h[2];
@@ -1581,8 +1605,7 @@ int nlf_cg_solve(nlf_matrix a, nlf_precond p, const double* b, double* x, double
         {
             const unsigned rb = spmv_blocks(64, n);
             DevBuf<double> r{size_t(n)}, prr{rb}, pbb{rb}, sums{2};
-            hipLaunchKernelGGL((k_residual<double, double>), dim3(rb), dim3(SPMV_THREADS), 0, s, n, a->outer.p, a->cols.p,
-                               a->val.p, x, b, r.p, prr.p, pbb.p);
+            residual64(n, a->outer.p, spmv_cols(a), a->val.p, pick_lpr(a), x, b, r.p, prr.p, pbb.p, rb, s);
             hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(1024), 0, s, prr.p, int(rb), sums.p);
             hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(1024), 0, s, pbb.p, int(rb), sums.p + 1);
             double h[2];
