@@ -91,6 +91,8 @@ struct PatternArgs {
     const i32* nb_ids;
     const uint8_t* inner;
     int max_words;
+    unsigned int* saved;   // pass 0 stores each node's bitmap here (stride words), pass 1 reads it back
+    int stride;
     const i32* lo_e;
     const i32* hi_e;
     // pass 0
@@ -193,14 +195,21 @@ __global__ __launch_bounds__(PATTERN_THREADS) void k_pattern(PatternArgs a) {
         if (threadIdx.x == 0) atomicOr(a.overflow, 1);
         return;
     }
-    for (int w = threadIdx.x; w < words; w += blockDim.x) bitmap[w] = 0u;
-    __syncthreads();
-    for_each_column_element(a, node, [&](i32 e) {
-        for (int j = 0; j < a.nn; ++j) {
-            const i64 v = a.el[i64(e) * a.nn + j] - lo;
-            atomicOr(&bitmap[v >> 5], 1u << (v & 31));
-        }
-    });
+    if (PASS == 1 && a.saved) {
+        for (int w = threadIdx.x; w < words; w += blockDim.x) bitmap[w] = a.saved[node * a.stride + w];
+    } else {
+        for (int w = threadIdx.x; w < words; w += blockDim.x) bitmap[w] = 0u;
+        __syncthreads();
+        for_each_column_element(a, node, [&](i32 e) {
+            for (int j = 0; j < a.nn; ++j) {
+                const i64 v = a.el[i64(e) * a.nn + j] - lo;
+                atomicOr(&bitmap[v >> 5], 1u << (v & 31));
+            }
+        });
+        __syncthreads();
+        if (PASS == 0 && a.saved)
+            for (int w = threadIdx.x; w < words; w += blockDim.x) a.saved[node * a.stride + w] = bitmap[w];
+    }
     __syncthreads();
     // each thread owns a contiguous chunk of words; columns are emitted in ascending order
     const int per = (words + blockDim.x - 1) / blockDim.x;
@@ -1194,6 +1203,14 @@ int nlf_assemble(nlf_mesh m, int physics, const nlf_model* models, const void* m
                   "column node ids of a row span more than 393k ids (renumber the mesh)");
         const size_t lds_bitmap = std::max<size_t>(words, 1) * sizeof(unsigned int);
         const int pth = m->ctx->pattern_threads;
+        // pass 0 keeps every node's bitmap (words per node at most the widest span) when that takes
+        // at most 8 GB, so that pass 1 emits the columns without rebuilding it
+        DevBuf<unsigned int> saved;
+        if (size_t(m->N) * std::max<size_t>(words, 1) * sizeof(unsigned int) <= (size_t(8) << 30)) {
+            saved.alloc(size_t(m->N) * std::max<size_t>(words, 1));
+            pa.saved = saved.p;
+            pa.stride = int(std::max<unsigned>(words, 1));
+        }
         hipLaunchKernelGGL(k_pattern<0>, dim3(unsigned(m->N)), dim3(pth), lds_bitmap, s, pa);
         NLF_HIP(hipGetLastError());
         int ovf = 0;

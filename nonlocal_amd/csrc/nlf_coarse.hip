@@ -68,12 +68,13 @@ __global__ void k_unit_diag(i64 from, i64 ncp, double* __restrict__ a) {
     if (i < ncp) a[i * ncp + i] = 1.0;
 }
 
-// In-place Gauss-Jordan inverse of the 64 x 64 diagonal tile (k, k) into dinv (no pivoting).
-__global__ __launch_bounds__(256) void k_gj_diag(const double* __restrict__ a, i64 ncp, i64 k, double* __restrict__ dinv) {
+// In-place Gauss-Jordan inverse of the 64 x 64 diagonal tile (k, k) into dinv (no pivoting); 1024
+// threads, four entries each per pivot step.
+__global__ __launch_bounds__(1024) void k_gj_diag(const double* __restrict__ a, i64 ncp, i64 k, double* __restrict__ dinv) {
     __shared__ double d[TB][TB + 1];
     __shared__ double colk[TB];
-    const int t = threadIdx.x;
-    for (int e = t; e < TB * TB; e += 256) d[e / TB][e % TB] = a[(k * TB + e / TB) * ncp + k * TB + e % TB];
+    const int t = threadIdx.x, T = blockDim.x;
+    for (int e = t; e < TB * TB; e += T) d[e / TB][e % TB] = a[(k * TB + e / TB) * ncp + k * TB + e % TB];
     __syncthreads();
     for (int p = 0; p < TB; ++p) {
         const double piv = d[p][p];
@@ -83,13 +84,13 @@ __global__ __launch_bounds__(256) void k_gj_diag(const double* __restrict__ a, i
             d[p][t] = (t == p ? 1.0 : d[p][t]) / piv;
         }
         __syncthreads();
-        for (int e = t; e < TB * TB; e += 256) {
+        for (int e = t; e < TB * TB; e += T) {
             const int i = e / TB, j = e % TB;
             if (i != p) d[i][j] = (j == p ? 0.0 : d[i][j]) - colk[i] * d[p][j];
         }
         __syncthreads();
     }
-    for (int e = t; e < TB * TB; e += 256) dinv[e] = d[e / TB][e % TB];
+    for (int e = t; e < TB * TB; e += T) dinv[e] = d[e / TB][e % TB];
 }
 
 // acc += X (64 x 64, leading dimension ldx) * Y (64 x 64, ldy); thread (tx, ty) owns rows 4 ty .. 4 ty + 3
@@ -271,7 +272,7 @@ std::unique_ptr<CoarseSpace> build_coarse(const nlf_matrix_s* a, i64 target, hip
     const i64 nb = cs->ncp / TB;
     DevBuf<double> dinv{size_t(TB * TB)};
     for (i64 k = 0; k < nb; ++k) {
-        hipLaunchKernelGGL(k_gj_diag, dim3(1), dim3(256), 0, s, cs->inv.p, cs->ncp, k, dinv.p);
+        hipLaunchKernelGGL(k_gj_diag, dim3(1), dim3(1024), 0, s, cs->inv.p, cs->ncp, k, dinv.p);
         if (nb > 1) {
             hipLaunchKernelGGL(k_gj_rowpanel, dim3(unsigned(nb - 1)), dim3(256), 0, s, cs->inv.p, cs->ncp, k, dinv.p);
             hipLaunchKernelGGL(k_gj_update, dim3(unsigned(nb - 1), unsigned(nb - 1)), dim3(256), 0, s, cs->inv.p, cs->ncp, k);
