@@ -57,12 +57,14 @@ def parse():
     ap.add_argument("--cpu-sample-n", type=int, default=480)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--methods", default="fp64_jacobi,mixed_jacobi,learned_fcg_onelevel,learned_fcg")
-    ap.add_argument("--workload", default="composite1000", choices=["composite1000", "batch64", "precision", "spmv"],
+    ap.add_argument("--workload", default="composite1000",
+                    choices=["composite1000", "batch64", "precision", "spmv", "flux"],
                     help="composite1000: the 1000x1000 composite (the metric's workload); batch64: 64 uneven "
                          "instances solved together vs one after another; precision: FP32 assembly and "
                          "mixed-precision errors against the FP64 path on every workload; spmv: the FP64 "
                          "SpMV of the composite alone, plus a device copy of known size (HBM counter "
-                         "calibration)")
+                         "calibration); flux: the heat-flux post-processing (calc_flux) of the composite's "
+                         "solution")
     return ap.parse_args()
 
 
@@ -304,6 +306,72 @@ def bench_spmv(args):
     print(json.dumps(line), flush=True)
 
 
+FP64_PEAK_TFLOPS = 78.6
+
+
+def bench_flux(args):
+    """Heat flux of the composite's temperature: gradient and -lambda grad T at the 4M quadrature points,
+    the nonlocal average over every element's neighbours (the dominant kernel: one influence
+    evaluation and two FMAs per pair of quadrature points), node values.  value = seconds per
+    nlf_heat_flux call (device buffers allocated by the call included)."""
+    import torch
+    from nonlocal_amd import _abi as A
+    from nonlocal_amd import api, configs
+    from nonlocal_amd._lib import load
+    import ctypes as C
+    api.torch = torch
+    problem = configs.cfg4(args.n)
+    ctx = api.default_context()
+    mesh = api.mesh_2d(problem.mesh, ctx)
+    log(f"[flux] solving the {args.n}x{args.n} composite for its temperature")
+    sol, _ = run_step(api, A, mesh, problem, A.NLF_PRECOND_LEARNED, A.NLF_MODE_FP64, args.tol)
+    total = C.c_int64()
+    ctx.check(load().nlf_neighbours_count(mesh.h, C.byref(total)))
+    nq = 4
+    pairs = int(total.value) * nq * nq                 # (qL, qNL) pairs: every group is nonlocal here
+    for _ in range(args.warmup):
+        api.heat_equation_solution_2d(mesh, problem.models, problem.materials, sol.x).calc_flux()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        h = api.heat_equation_solution_2d(mesh, problem.models, problem.materials, sol.x)
+        h.calc_flux()
+    torch.cuda.synchronize()
+    sec = (time.perf_counter() - t0) / args.steps
+    fx, fy = h.flux()
+    flops = pairs * 12.0                              # phi (8) + two multiply-adds (4) per pair
+    cpu = None
+    if not args.no_cpu:
+        from oracle import oracle as O
+        cores = O.set_threads(0)
+        n_s = 300
+        ps = configs.composite(n_s, r=10.0 / n_s)
+        om = O.OracleMesh(ps.mesh)
+        off, ids, _ = om.find_neighbours([[m.radius[0], m.radius[1]] for m in ps.models])
+        T = np.random.default_rng(0).standard_normal(ps.mesh.n_nodes)
+        t1 = time.perf_counter()
+        om.heat_flux(ps.models, ps.materials, T)
+        tc = time.perf_counter() - t1
+        scale = pairs / (ids.size * nq * nq)
+        cpu = dict(value=tc * scale, unit="s", cores=cores, kind="port",
+                   sample=f"oracle calc_flux on the {n_s}x{n_s} composite with the same r/h = 10 "
+                          f"({ids.size * nq * nq:.3g} quadrature-point pairs, {tc:.2f} s), scaled by pairs x{scale:.1f}")
+    line = {
+        "metric": "heat-flux post-processing seconds (calc_flux) on the 1000x1000 composite",
+        "value": round(sec, 5), "unit": "s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(sec * 1e3, 3), "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"composite {args.n}x{args.n} Q4, 16 inclusions, p1=0.5, bell r=0.01: flux of the "
+                               f"solved temperature", "quadrature_point_pairs": pairs},
+        "pairs_per_s": round(pairs / sec, 1),
+        "flux_abs_max": float(torch.maximum(fx.abs().max(), fy.abs().max())),
+        "fp64_rate_upper_bound": {"achieved_TFLOPs_whole_call": round(flops / sec * 1e-12, 2),
+                                  "peak": FP64_PEAK_TFLOPS, "flops_per_pair": 12},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -318,6 +386,9 @@ def main():
         return
     if args.workload == "spmv":
         bench_spmv(args)
+        return
+    if args.workload == "flux":
+        bench_flux(args)
         return
     import torch
     from nonlocal_amd import _abi as A

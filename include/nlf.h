@@ -179,6 +179,16 @@ int nlf_matrix_make_split(nlf_matrix a);
 /* No reference counterpart.  Distance of the FP32 values from the FP64 ones, computed on the
  * device over the full storage: out[4] = {max |v32 - v64|, max |v64|, ||v32 - v64||_2, ||v64||_2}. */
 int nlf_matrix_compare_fp32(nlf_matrix a, double* out);
+
+/* ---- heat flux: replaces heat_equation_solution_2d::calc_flux
+ * (solvers/solver_2d/thermal/heat_equation_solution_2d.hpp:99-137) with gradient_in_qnodes and
+ * qnodes_to_nodes (mesh/mesh_2d/mesh_2d_utils.hpp:39-63,82-101).  T_dev[N] node temperatures;
+ * flux_dev[2N] node fluxes (x block, then y block, like the reference's array of two vectors);
+ * qflux_dev[2*E*nq] quadrature-point fluxes (x block, then y block; index e*nq + q).  Either output
+ * may be NULL.  A nonlocal group needs nlf_find_neighbours first (NLF_ERR_STATE otherwise). */
+int nlf_heat_flux(nlf_mesh m, const nlf_model* models, const nlf_thermal_material* materials, const double* T_dev,
+                  double* flux_dev, double* qflux_dev);
+
 /* inner_nodes (base/solvers_utils.hpp:31-40), boundary_condition_second_kind_2d
  * (base/boundary_condition_second_kind_2d.hpp:12-33) and the first-kind values (:17-26), host arrays */
 int nlf_boundary_data(nlf_mesh m, int dof, const nlf_boundary* bcs, double* f, double* xD, uint8_t* is_inner);

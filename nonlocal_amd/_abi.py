@@ -2,6 +2,7 @@
 import ctypes as C
 
 NLF_OK = 0
+NLF_ERR_ARGUMENT, NLF_ERR_HIP, NLF_ERR_STATE, NLF_ERR_UNSUPPORTED, NLF_ERR_NUMERIC = -1, -2, -3, -4, -5
 NLF_ELEMENT_Q4 = 4
 NLF_ELEMENT_Q8 = 8
 NLF_INFLUENCE_POLYNOMIAL = 0

@@ -373,6 +373,65 @@ class Solution:
     rhs: torch.Tensor = None
     timings: dict = field(default_factory=dict)
     stats: object = None
+    solution: object = None        # heat_equation_solution_2d of a thermal solve (flux on demand)
+
+
+class heat_equation_solution_2d:
+    """Temperature and heat flux of a thermal solve (solvers/solver_2d/thermal/
+    heat_equation_solution_2d.hpp:10-137), device resident.
+
+    calc_flux() computes, on the GPU, the flux -lambda grad T at the quadrature points, its nonlocal
+    average p1 q + (1 - p1) int phi q over each element's neighbours for nonlocal groups, and the
+    node values as the area-weighted mean of the nearest quadrature points; it returns the pair
+    (flux_x, flux_y) of node vectors and keeps it, like the reference's calc_flux.  The reference's
+    solver calls calc_flux before returning; here the thermal solve returns this object with the
+    flux not yet computed, so a solve that does not need the flux does not pay for it.
+    """
+
+    def __init__(self, mesh: mesh_2d, models, materials, temperature: torch.Tensor):
+        n = mesh.nodes_count
+        if temperature.numel() < n:
+            raise NLFError(_abi.NLF_ERR_ARGUMENT, "The temperature vector is shorter than the number of nodes.")
+        self.mesh = mesh
+        self._models = list(models)
+        self._materials = list(materials)
+        self._T = temperature.reshape(-1)[:n].to(device=mesh.ctx.device, dtype=torch.float64).contiguous()
+        self._flux = None
+        self._qflux = None
+
+    def temperature(self) -> torch.Tensor:
+        return self._T
+
+    def is_flux_calculated(self) -> bool:
+        return self._flux is not None
+
+    def flux(self):
+        if not self.is_flux_calculated():
+            raise RuntimeError("Flux wasn't calculated")
+        return self._flux
+
+    def flux_in_qnodes(self):
+        """Quadrature-point fluxes (x, y), index e*nq + q, after the nonlocal averaging."""
+        if not self.is_flux_calculated():
+            raise RuntimeError("Flux wasn't calculated")
+        return self._qflux
+
+    def calc_flux(self):
+        if self.is_flux_calculated():
+            return self.flux()
+        mesh = self.mesh
+        if any(m.local_weight < 0.999 for m in self._models) and mesh.influences is None:
+            find_neighbours(mesh, _radii(self._models))
+        n, E = mesh.nodes_count, mesh.elements_count
+        nq = 4 if mesh.mesh.nn == 4 else 9
+        out = torch.empty(2 * n, dtype=torch.float64, device=mesh.ctx.device)
+        q = torch.empty(2 * E * nq, dtype=torch.float64, device=mesh.ctx.device)
+        marr = _abi.array_of(_abi.Model, self._models)
+        matarr = _abi.array_of(_abi.ThermalMaterial, self._materials)
+        mesh.ctx.check(load().nlf_heat_flux(mesh.h, marr, matarr, _dev(self._T), _dev(out), _dev(q)))
+        self._flux = (out[:n], out[n:])
+        self._qflux = (q[:E * nq], q[E * nq:])
+        return self.flux()
 
 
 def _radii(models):
@@ -454,8 +513,9 @@ def _solve_driver(mesh: mesh_2d, physics, models, materials, bcs, precond, tol, 
     ctx.synchronize()
     tm["solve"] = time.perf_counter() - t3
     st = cg.stats
+    heat = heat_equation_solution_2d(mesh, models, materials, x) if physics == 0 else None
     return Solution(x=x, iterations=st.iterations, residual=st.residual, true_residual=st.true_residual, matrix=K,
-                    local_matrix=KL, rhs=fd, timings=tm, stats=st)
+                    local_matrix=KL, rhs=fd, timings=tm, stats=st, solution=heat)
 
 
 def stationary_heat_equation_solver_2d(mesh: mesh_2d, models, materials, bcs, precond=NLF_PRECOND_ILLT, tol=1e-12,

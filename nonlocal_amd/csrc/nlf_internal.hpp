@@ -82,6 +82,7 @@ struct FeTables {
     double w[MAX_NQ];
     double qN[MAX_NN * MAX_NQ], qNxi[MAX_NN * MAX_NQ], qNeta[MAX_NN * MAX_NQ];  // [i*nq + q]
     double N0[MAX_NN];
+    int nearest[MAX_NN];                 // quadrature point nearest to each node (nearest_qnode)
 };
 
 struct BoundaryTables {

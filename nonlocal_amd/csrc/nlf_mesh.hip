@@ -87,6 +87,20 @@ FeTables make_fe_tables(int nn) {
                 t.qNeta[a * t.nq + i * k + j] = Ny[a];
             }
         }
+    // nearest_qnode (element_2d_integrate.hpp:87-105): the first quadrature point at the smallest
+    // distance from the node in the reference element
+    static const double q4_nodes[4][2] = {{-1, -1}, {1, -1}, {1, 1}, {-1, 1}};
+    static const double q8_nodes[8][2] = {{-1, -1}, {0, -1}, {1, -1}, {1, 0}, {1, 1}, {0, 1}, {-1, 1}, {-1, 0}};
+    for (int a = 0; a < nn; ++a) {
+        const double* v = nn == NLF_ELEMENT_Q4 ? q4_nodes[a] : q8_nodes[a];
+        double best = 1e300;
+        t.nearest[a] = 0;
+        for (int i = 0; i < k; ++i)
+            for (int j = 0; j < k; ++j) {
+                const double d = std::sqrt((v[0] - px[i]) * (v[0] - px[i]) + (v[1] - px[j]) * (v[1] - px[j]));
+                if (best > d) { best = d; t.nearest[a] = i * k + j; }
+            }
+    }
     // centre weights N_i(0,0): 1/4 (Q4); (9p-1)/16 corners and -(9p-5)/16 midsides (Q8)
     if (nn == NLF_ELEMENT_Q4) for (int a = 0; a < 4; ++a) t.N0[a] = 0.25;
     else {

@@ -424,10 +424,52 @@ struct boundary_condition {
 
 using boundaries_conditions_2d = std::unordered_map<std::string, boundary_condition>;
 
-struct heat_equation_solution_2d {
-    std::vector<double> temperature;
-    uintmax_t iterations = 0;
+// heat_equation_solution_2d (thermal/heat_equation_solution_2d.hpp:10-137): the temperature and the
+// heat flux; calc_flux runs nlf_heat_flux on the GPU (gradient, -lambda grad T, nonlocal averaging
+// over the neighbour lists, nearest-quadrature-point node values) and keeps the result.
+template<class T = double, class I = int64_t>
+class heat_equation_solution_2d {
+    std::shared_ptr<mesh::mesh_2d<T, I>> mesh_;
+    std::vector<nlf_model> models_;
+    std::vector<nlf_thermal_material> materials_;
+    std::vector<T> temperature_;
+    std::array<std::vector<T>, 2> flux_;
+
+public:
+    uintmax_t iterations = 0;                          // of the solve that produced the temperature
     double residual = 0, true_residual = 0;
+
+    heat_equation_solution_2d(const std::shared_ptr<mesh::mesh_2d<T, I>>& mesh, const parameters_2d& parameters,
+                              std::vector<T> temperature)
+        : mesh_(mesh), temperature_(std::move(temperature)) {
+        if (temperature_.size() < mesh_->nodes_count())
+            throw nlf_error(NLF_ERR_ARGUMENT, "The temperature vector is shorter than the number of nodes.");
+        temperature_.resize(mesh_->nodes_count());
+        for (size_t g = 0; g < mesh_->groups_count(); ++g) {
+            const auto& p = parameters.at(mesh_->data().group_names[g]);
+            models_.push_back(p.model.model);
+            materials_.push_back(p.conductivity);
+        }
+    }
+
+    const std::vector<T>& temperature() const noexcept { return temperature_; }
+    bool is_flux_calculated() const noexcept { return !flux_[0].empty() && !flux_[1].empty(); }
+    const std::array<std::vector<T>, 2>& flux() const {
+        if (!is_flux_calculated()) throw std::runtime_error{"Flux wasn't calculated"};
+        return flux_;
+    }
+    // The neighbour lists of a nonlocal group must be on the mesh (find_neighbours), as in the reference.
+    const std::array<std::vector<T>, 2>& calc_flux() {
+        if (is_flux_calculated()) return flux_;
+        const size_t n = mesh_->nodes_count();
+        device_vector dT(temperature_), dflux(2 * n);
+        nlf_check(nlf_heat_flux(mesh_->handle(), models_.data(), materials_.data(), dT.data(), dflux.data(), nullptr),
+                  mesh_->ctx().handle());
+        const std::vector<double> h = dflux.host();
+        flux_[0].assign(h.begin(), h.begin() + std::ptrdiff_t(n));
+        flux_[1].assign(h.begin() + std::ptrdiff_t(n), h.end());
+        return flux_;
+    }
 };
 
 // stationary_heat_equation_solver_2d (thermal/stationary_heat_equation_solver_2d.hpp:46-120):
@@ -435,10 +477,11 @@ struct heat_equation_solution_2d {
 // failure), CG.  The system is solved directly (the reference's correction form with T0 = 0 is
 // the same system).
 template<class T = double, class I = int64_t, class J = int64_t>
-heat_equation_solution_2d stationary_heat_equation_solver_2d(const std::shared_ptr<mesh::mesh_2d<T, I>>& mesh,
-                                                             const parameters_2d& parameters,
-                                                             const boundaries_conditions_2d& conditions,
-                                                             double tolerance = 1e-15, uintmax_t max_iterations = 10000) {
+heat_equation_solution_2d<T, I> stationary_heat_equation_solver_2d(const std::shared_ptr<mesh::mesh_2d<T, I>>& mesh,
+                                                                   const parameters_2d& parameters,
+                                                                   const boundaries_conditions_2d& conditions,
+                                                                   double tolerance = 1e-15,
+                                                                   uintmax_t max_iterations = 10000) {
     const size_t G = mesh->groups_count(), GB = mesh->data().boundary_names.size();
     nlf_ctx ctx = mesh->ctx().handle();
     mesh::influences<T> radii;
@@ -476,7 +519,12 @@ heat_equation_solution_2d stationary_heat_equation_solver_2d(const std::shared_p
     if (!P.computation_info()) cg.init_preconditioner<slae::eigen_identity_preconditioner>(K.matrix());
     dx.upload(std::vector<double>(rows, 0.0));
     cg.solve(df, dx);
-    return heat_equation_solution_2d{dx.host(), cg.iterations(), cg.residual(), cg.statistics().true_residual};
+    heat_equation_solution_2d<T, I> solution(mesh, parameters, dx.host());
+    solution.iterations = cg.iterations();
+    solution.residual = cg.residual();
+    solution.true_residual = cg.statistics().true_residual;
+    solution.calc_flux();                              // as the reference's solver does (:117-119)
+    return solution;
 }
 
 }  // namespace thermal

@@ -44,6 +44,7 @@ struct Tables2D {
     int nn = 0, nq = 0;
     std::vector<double> w, qN, qNxi, qNeta;   // [i*nq + q]
     std::vector<double> N0;                   // N_i(0, 0) (centre weights)
+    std::vector<int> nearest;                 // nearest quadrature point of each node (element_2d_integrate.hpp:87-105)
 };
 
 void gauss_rule(int k, std::vector<double>& x, std::vector<double>& w) {
@@ -123,6 +124,21 @@ Tables2D make_tables(int nn) {
                 t.qNeta[a * t.nq + i * k + j] = Ny[a];
             }
         }
+    // nearest_qnode: the first Gauss point at the smallest Euclidean distance from node i of the
+    // reference element (strict comparison keeps the first minimum)
+    static const double q4_nodes[4][2] = {{-1, -1}, {1, -1}, {1, 1}, {-1, 1}};
+    static const double q8_nodes[8][2] = {{-1, -1}, {0, -1}, {1, -1}, {1, 0}, {1, 1}, {0, 1}, {-1, 1}, {-1, 0}};
+    t.nearest.assign(nn, 0);
+    for (int a = 0; a < nn; ++a) {
+        const double* node = nn == 4 ? q4_nodes[a] : q8_nodes[a];
+        double length = std::numeric_limits<double>::max();
+        for (int i = 0; i < k; ++i)
+            for (int j = 0; j < k; ++j) {
+                const double dx = node[0] - px[i], dy = node[1] - py[j];
+                const double curr = std::sqrt(dx * dx + dy * dy);
+                if (length > curr) { length = curr; t.nearest[a] = i * k + j; }
+            }
+    }
     basis(nn, 0.0, 0.0, N, Nx, Ny);
     t.N0.assign(N, N + nn);
     if (nn == 8) {   // exact reference weights (9p-1)/16 and -(9p-5)/16 at xi0 = 0
@@ -963,6 +979,110 @@ CGResult cg(const CSR& A, const Precond& M, const double* b, double* x, double t
 
 thread_local std::string g_error;
 
+
+// ---------------------------------------------------------------------------------------------
+// Heat flux post-processing: heat_equation_solution_2d::calc_flux (heat_equation_solution_2d.hpp:99-137)
+// ---------------------------------------------------------------------------------------------
+// mesh_2d::area (mesh_2d.hpp:226-235): sum over q of w_q |J_q| sum_i N_i(q), accumulated q-outer,
+// i-inner.
+double element_area(const Mesh& m, i64 e) {
+    double area = 0.0;
+    for (int q = 0; q < m.t.nq; ++q) {
+        const double factor = m.t.w[q] * m.jac[e * m.t.nq + q];
+        for (int i = 0; i < m.nn; ++i) area += factor * m.t.qN[i * m.t.nq + q];
+    }
+    return area;
+}
+
+// qnodes_to_nodes (mesh_2d_utils.hpp:82-101): area-weighted mean, over the elements of each node in
+// ascending id order, of the value at the element's quadrature point nearest to the node.
+void qnodes_to_nodes(const Mesh& m, const double* x, double* out) {
+#pragma omp parallel for schedule(static)
+    for (i64 node = 0; node < m.N; ++node) {
+        double value = 0.0, node_area = 0.0;
+        for (i64 k = m.ne_off[node]; k < m.ne_off[node + 1]; ++k) {
+            const i64 e = m.ne_ids[k];
+            const double area = element_area(m, e);
+            const int i = m.local(e, node);
+            value += area * x[e * m.t.nq + m.t.nearest[i]];
+            node_area += area;
+        }
+        out[node] = value / node_area;
+    }
+}
+
+// qflux: [2][E*nq] quadrature-point fluxes (x block, then y block); nflux: [2][N] node fluxes.
+void heat_flux(const Mesh& m, const std::vector<Group>& groups, const double* T, double* qflux, double* nflux) {
+    const int nq = m.t.nq, nn = m.nn;
+    const i64 Q = m.E * nq;
+    // gradient_in_qnodes (mesh_2d_utils.hpp:39-63): sum_i T_i D_i(q), then divided by |J_q|
+    std::vector<double> fx(Q, 0.0), fy(Q, 0.0);
+#pragma omp parallel for schedule(static)
+    for (i64 e = 0; e < m.E; ++e)
+        for (int q = 0; q < nq; ++q) {
+            const i64 qs = e * nq + q;
+            for (int i = 0; i < nn; ++i) {
+                const double val = T[m.el[e * nn + i]];
+                fx[qs] += val * m.Dx(e, i, q);
+                fy[qs] += val * m.Dy(e, i, q);
+            }
+            fx[qs] /= m.jac[qs];
+            fy[qs] /= m.jac[qs];
+        }
+    // local_flux_in_qnodes (heat_equation_solution_2d.hpp:72-97): -lambda grad T per group
+    for (int g = 0; g < m.G(); ++g) {
+        const nlf_thermal_material& c = groups[g].th;
+        for (i64 e = m.gbeg[g]; e < m.gbeg[g + 1]; ++e)
+            for (int q = 0; q < nq; ++q) {
+                const i64 qs = e * nq + q;
+                if (c.kind == NLF_CONDUCTIVITY_ISOTROPIC) { fx[qs] *= -c.c[0]; fy[qs] *= -c.c[0]; }
+                else if (c.kind == NLF_CONDUCTIVITY_ORTHOTROPIC) { fx[qs] *= -c.c[0]; fy[qs] *= -c.c[1]; }
+                else {
+                    const double x = fx[qs], y = fy[qs];
+                    fx[qs] = -c.c[0] * x - c.c[2] * y;
+                    fy[qs] = -c.c[2] * x - c.c[1] * y;
+                }
+            }
+    }
+    // nonlocal groups: p1 q(x) + (1 - p1) sum_{eNL in N(eL)} sum_qNL w |J| phi(x, x') q(x')
+    std::vector<double> gx = fx, gy = fy;
+    for (int g = 0; g < m.G(); ++g) {
+        const Group& gr = groups[g];
+        if (!gr.nonlocal) continue;
+        if (m.nb_off.empty()) throw std::logic_error("heat flux of a nonlocal group needs the neighbour lists");
+#pragma omp parallel for schedule(dynamic, 64)
+        for (i64 eL = m.gbeg[g]; eL < m.gbeg[g + 1]; ++eL)
+            for (int qL = 0; qL < nq; ++qL) {
+                const i64 qsL = eL * nq + qL;
+                double ng[2] = {0.0, 0.0};
+                const double* xl = m.xqp(eL, qL);
+                for (i64 s = m.nb_off[eL]; s < m.nb_off[eL + 1]; ++s) {
+                    const i64 eNL = m.nb_ids[s];
+                    for (int qN = 0; qN < nq; ++qN) {
+                        const i64 qsN = eNL * nq + qN;
+                        const double iw = m.t.w[qN] * m.jac[qsN] * influence_eval(gr.inf, xl, m.xqp(eNL, qN));
+                        ng[0] += iw * fx[qsN];
+                        ng[1] += iw * fy[qsN];
+                    }
+                }
+                ng[0] *= 1.0 - gr.p1;
+                ng[1] *= 1.0 - gr.p1;
+                gx[qsL] *= gr.p1;
+                gy[qsL] *= gr.p1;
+                gx[qsL] += ng[0];
+                gy[qsL] += ng[1];
+            }
+    }
+    if (qflux) {
+        std::copy(gx.begin(), gx.end(), qflux);
+        std::copy(gy.begin(), gy.end(), qflux + Q);
+    }
+    if (nflux) {
+        qnodes_to_nodes(m, gx.data(), nflux);
+        qnodes_to_nodes(m, gy.data(), nflux + m.N);
+    }
+}
+
 }  // namespace
 
 // =============================================================================================
@@ -1112,6 +1232,30 @@ i64 orc_cg(void* a, void* p, const double* b, double* x, double tol, i64 max_it,
     const auto r = cg(static_cast<Assembly*>(a)->A[INNER], *static_cast<Precond*>(p), b, x, tol, max_it);
     if (residual) *residual = r.residual;
     return r.iterations;
+}
+
+void orc_nearest_qnodes(int nn, int* out) {
+    const Tables2D t = make_tables(nn);
+    std::copy(t.nearest.begin(), t.nearest.end(), out);
+}
+
+void orc_element_areas(void* h, double* out) {
+    const Mesh& m = *static_cast<Mesh*>(h);
+    for (i64 e = 0; e < m.E; ++e) out[e] = element_area(m, e);
+}
+
+void orc_qnodes_to_nodes(void* h, const double* x, double* out) { qnodes_to_nodes(*static_cast<Mesh*>(h), x, out); }
+
+// Thermal flux of temperature T (N values): qflux [2][E*nq] and nflux [2][N], either may be NULL.
+int orc_heat_flux(void* h, const nlf_model* models, const nlf_thermal_material* materials, const double* T,
+                  double* qflux, double* nflux) {
+    try {
+        const Mesh& m = *static_cast<Mesh*>(h);
+        std::vector<Group> groups;
+        for (int g = 0; g < m.G(); ++g) groups.push_back(make_group(models[g], 0, &materials[g], nullptr, false));
+        heat_flux(m, groups, T, qflux, nflux);
+        return 0;
+    } catch (const std::exception& e) { g_error = e.what(); return -1; }
 }
 
 }  // extern "C"

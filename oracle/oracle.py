@@ -69,6 +69,12 @@ def lib():
         L.orc_cg.restype = _i64
         L.orc_cg.argtypes = [C.c_void_p, C.c_void_p, _f64p, _f64p, C.c_double, _i64, _f64p]
         L.orc_set_threads.argtypes = [C.c_int]
+        L.orc_nearest_qnodes.argtypes = [C.c_int, _i32p]
+        L.orc_element_areas.argtypes = [C.c_void_p, _f64p]
+        L.orc_qnodes_to_nodes.argtypes = [C.c_void_p, _f64p, _f64p]
+        L.orc_heat_flux.restype = C.c_int
+        L.orc_heat_flux.argtypes = [C.c_void_p, C.POINTER(_abi.Model), C.POINTER(_abi.ThermalMaterial), _f64p, _f64p,
+                                    _f64p]
         L.orc_last_error.restype = C.c_char_p
         _LIB = L
     return _LIB
@@ -87,6 +93,13 @@ def tables(nn: int):
     N0 = np.zeros(nn)
     lib().orc_tables(nn, _ptr(w, _f64p), _ptr(qN, _f64p), _ptr(qx, _f64p), _ptr(qy, _f64p), _ptr(N0, _f64p))
     return w, qN.reshape(nn, nq), qx.reshape(nn, nq), qy.reshape(nn, nq), N0
+
+
+def nearest_qnodes(nn: int) -> np.ndarray:
+    """Index of the quadrature point nearest to each node of the reference element."""
+    out = np.zeros(nn, dtype=np.int32)
+    lib().orc_nearest_qnodes(nn, _ptr(out, _i32p))
+    return out
 
 
 def basis(nn: int, x: float, y: float):
@@ -224,6 +237,31 @@ class OracleMesh:
         if not h:
             raise RuntimeError(lib().orc_last_error().decode())
         return OracleMatrix(h, inner.size)
+
+    def element_areas(self):
+        a = np.zeros(self.mesh.n_elems)
+        lib().orc_element_areas(self.h, _ptr(a, _f64p))
+        return a
+
+    def qnodes_to_nodes(self, x):
+        """Node values from quadrature-point values [E*nq] (mesh_2d_utils.hpp:82-101)."""
+        x = np.ascontiguousarray(x, dtype=np.float64).ravel()
+        out = np.zeros(self.mesh.n_nodes)
+        lib().orc_qnodes_to_nodes(self.h, _ptr(x, _f64p), _ptr(out, _f64p))
+        return out
+
+    def heat_flux(self, models, materials, T):
+        """heat_equation_solution_2d::calc_flux (heat_equation_solution_2d.hpp:99-137).  Needs the
+        neighbour lists when a group is nonlocal.  Returns (node flux (2, N), quad-point flux (2, E*nq))."""
+        nq = 4 if self.mesh.nn == 4 else 9
+        T = np.ascontiguousarray(T, dtype=np.float64)
+        q = np.zeros((2, self.mesh.n_elems * nq))
+        n = np.zeros((2, self.mesh.n_nodes))
+        marr = _abi.array_of(_abi.Model, models)
+        matarr = _abi.array_of(_abi.ThermalMaterial, materials)
+        if lib().orc_heat_flux(self.h, marr, matarr, _ptr(T, _f64p), _ptr(q, _f64p), _ptr(n, _f64p)) != 0:
+            raise RuntimeError(lib().orc_last_error().decode())
+        return n, q
 
     def convection(self, matrix: OracleMatrix, bcs, is_inner):
         arr = _abi.array_of(_abi.Boundary, bcs)

@@ -5,7 +5,8 @@
 //   heat_q4 <n> <p1> <r> <out.bin>
 //
 // T = 0 on x = 0, unit inward flux on x = 1, insulated elsewhere, lambda = 1.  Writes the
-// temperature (n+1)^2 doubles to out.bin and prints iterations and residuals.
+// temperature, then the node fluxes x and y ((n+1)^2 doubles each) to out.bin and prints iterations
+// and residuals.
 #include <nonlocal/nonlocal_amd.hpp>
 
 #include <cstdio>
@@ -44,8 +45,13 @@ int main(int argc, char** argv) {
         conditions["left"] = thermal::boundary_condition::temperature(0.0);
         conditions["right"] = thermal::boundary_condition::flux(1.0);
         const auto sol = thermal::stationary_heat_equation_solver_2d(m, parameters, conditions, 1e-13, 100000);
-        std::ofstream(argv[4], std::ios::binary)
-            .write(reinterpret_cast<const char*>(sol.temperature.data()), std::streamsize(sol.temperature.size() * sizeof(double)));
+        std::ofstream out(argv[4], std::ios::binary);
+        auto write = [&out](const std::vector<double>& v) {
+            out.write(reinterpret_cast<const char*>(v.data()), std::streamsize(v.size() * sizeof(double)));
+        };
+        write(sol.temperature());
+        write(sol.flux()[0]);
+        write(sol.flux()[1]);
         std::printf("iterations %ju residual %.3e true_residual %.3e\n", sol.iterations, sol.residual, sol.true_residual);
         return 0;
     } catch (const std::exception& e) {

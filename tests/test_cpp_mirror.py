@@ -38,7 +38,10 @@ def test_cpp_heat_solver_matches_oracle(tmp_path):
     res = subprocess.run([exe, "40", "0.5", "0.1", out], capture_output=True, text=True, timeout=120)
     assert res.returncode == 0, res.stderr
     print(res.stdout.strip(), flush=True)
-    T = np.fromfile(out, dtype=np.float64)
+    T, fx, fy = np.fromfile(out, dtype=np.float64).reshape(3, -1)
     p = configs.cfg0()
     om, K, f, xD, inner = oracle_assembly(p)
     assert rel_l2(T, direct_solution(K, f)) < 1e-10
+    # the solver computed the flux (calc_flux, as the reference's solver does) of its temperature
+    n_ref, _ = om.heat_flux(p.models, p.materials, T)
+    assert np.abs(np.stack([fx, fy]) - n_ref).max() <= 1e-12 * np.abs(n_ref).max()

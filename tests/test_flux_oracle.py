@@ -1,0 +1,186 @@
+"""CPU pins of the oracle's heat-flux post-processing and of the SU2 reader.
+
+calc_flux (heat_equation_solution_2d.hpp:99-137) has three parts; each is pinned by something the
+formulas do not restate:
+  * gradient and local flux: a linear temperature has the exact flux -lambda grad T at every
+    quadrature point of any (jittered) Q4/Q8 mesh, for every conductivity kind;
+  * nonlocal averaging: the weak divergence of the quadrature-point flux equals minus the
+    assembled operator times T, sum_e sum_q w_q D_i(q) . q(x_q) = -(K T)_i, because K's local and
+    nonlocal terms are exactly the integrals of grad N_i against the local and averaged fluxes;
+  * qnodes_to_nodes: the nearest-quadrature-point table against the geometry of the Gauss points,
+    and symmetry (on a uniform grid the four nearest points around an interior node average to it).
+The reference's own solver tests (isotropic_solid_ring.cpp, isotropic_composite_ring.cpp) run on
+its SU2 meshes, with its tolerances on the temperature and the flux.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from nonlocal_amd import _abi as A
+from nonlocal_amd import configs, meshgen, su2
+from oracle import oracle as O
+from tests import helpers as H
+
+MATERIALS = {
+    "isotropic": A.thermal_material(A.NLF_CONDUCTIVITY_ISOTROPIC, (1.7, 0.0, 0.0)),
+    "orthotropic": A.thermal_material(A.NLF_CONDUCTIVITY_ORTHOTROPIC, (1.3, 0.4, 0.0)),
+    "anisotropic": A.thermal_material(A.NLF_CONDUCTIVITY_ANISOTROPIC, (1.0, 0.25, 0.1)),
+}
+LAMBDA = {"isotropic": np.array([[1.7, 0.0], [0.0, 1.7]]),
+          "orthotropic": np.array([[1.3, 0.0], [0.0, 0.4]]),
+          "anisotropic": np.array([[1.0, 0.1], [0.1, 0.25]])}
+
+
+# ---- SU2 reader ---------------------------------------------------------------------------------
+def test_su2_meshes_of_the_reference_tests():
+    counts = {"solid_ring": (920, 280, 8, ["DEFAULT"], 6),
+              "composite_ring": (1040, 320, 8, ["Inner_Material", "Outer_Material"], 7),
+              "plate_10x1_h0.125": (729, 640, 4, ["DEFAULT"], 4),
+              "square_1x1_16el_uniform_mesh": (25, 16, 4, ["Layer1"], 4)}
+    for name, (n, e, nn, groups, markers) in counts.items():
+        m = su2.read(f"{H.MESH_DIR}/{name}.su2")
+        assert (m.n_nodes, m.n_elems, m.nn, m.group_names, len(m.bgroup_names)) == (n, e, nn, groups, markers)
+        assert m.group_begin[-1] == m.n_elems and m.bgroup_begin[-1] == m.bedges.shape[0]
+        # every element is a valid quad: |det J| > 0 with one sign at all quadrature points
+        om = O.OracleMesh(m)
+        xq, jac, D = om.quadrature()
+        assert jac.min() > 0
+        # the areas add up to the domain's
+        area = om.element_areas().sum()
+        expect = {"solid_ring": math.pi * (1 - 0.25), "composite_ring": math.pi * (1 - 0.25),
+                  "plate_10x1_h0.125": 10.0, "square_1x1_16el_uniform_mesh": 1.0}[name]
+        assert abs(area - expect) <= 1e-4 * expect, (name, area, expect)
+
+
+def test_su2_node_orders():
+    """Serendipity quads: midside nodes between their corners (a wrong remap puts a corner there);
+    quadratic edges stored (end, mid, end)."""
+    m = su2.read(f"{H.MESH_DIR}/solid_ring.su2")
+    xy = m.xy[m.elems]                                    # (E, 8, 2), local order c0 m01 c1 m12 c2 m23 c3 m30
+    for c0, mid, c1 in [(0, 1, 2), (2, 3, 4), (4, 5, 6), (6, 7, 0)]:
+        a, b, c = xy[:, c0], xy[:, mid], xy[:, c1]
+        assert np.abs(np.linalg.norm(b - a, axis=1) - np.linalg.norm(c - b, axis=1)).max() < 1e-3
+    e = m.xy[m.bedges]
+    assert np.abs(np.linalg.norm(e[:, 1] - e[:, 0], axis=1) - np.linalg.norm(e[:, 2] - e[:, 1], axis=1)).max() < 1e-3
+
+
+def test_su2_reserved_group_and_duplicates():
+    text = ("NDIME= 2\nNELEM= 2\n9 0 1 4 3 0\n9 1 2 5 4 1\nNPOIN= 6\n0 0 0\n1 0 1\n2 0 2\n0 1 3\n1 1 4\n2 1 5\n"
+            "NMARK= 1\nMARKER_TAG= DEFAULT\nMARKER_ELEMS= 1\n3 0 1\n")
+    with pytest.raises(ValueError, match="reserved"):
+        su2.parse(text)
+    text = ("NDIME= 2\nNELEM= 3\n9 0 1 4 3 0\n9 1 2 5 4 1\n9 0 1 4 3 2\nNPOIN= 6\n0 0 0\n1 0 1\n2 0 2\n0 1 3\n"
+            "1 1 4\n2 1 5\nNMARK= 2\nMARKER_TAG= Right\nMARKER_ELEMS= 1\n3 2 5\nMARKER_TAG= Mat\nMARKER_ELEMS= 1\n"
+            "9 1 2 5 4 \n")
+    m = su2.parse(text)
+    assert m.n_elems == 2                                 # a repeated element line counts once
+    assert m.group_names == ["Mat", "DEFAULT"] and m.elems[0].tolist() == [1, 2, 5, 4]
+    assert m.bgroup_names == ["Right"] and m.bedges.tolist() == [[2, 5]]
+
+
+# ---- nearest quadrature points and qnodes_to_nodes ------------------------------------------------
+@pytest.mark.parametrize("nn", [4, 8])
+def test_nearest_qnodes_from_gauss_geometry(nn):
+    g = [-1 / math.sqrt(3), 1 / math.sqrt(3)] if nn == 4 else [-math.sqrt(0.6), 0.0, math.sqrt(0.6)]
+    pts = [(a, b) for a in g for b in g]                 # q = ix * ny + iy
+    nodes = [(-1, -1), (1, -1), (1, 1), (-1, 1)] if nn == 4 else \
+        [(-1, -1), (0, -1), (1, -1), (1, 0), (1, 1), (0, 1), (-1, 1), (-1, 0)]
+    expect = [int(np.argmin([math.dist(v, p) for p in pts])) for v in nodes]
+    assert O.nearest_qnodes(nn).tolist() == expect
+
+
+@pytest.mark.parametrize("nn", [4, 8])
+def test_qnodes_to_nodes_symmetry(nn):
+    n = 6
+    mesh = meshgen.structured_q4(n) if nn == 4 else meshgen.structured_q8(n)
+    om = O.OracleMesh(mesh)
+    xq, jac, D = om.quadrature()
+    for comp in range(2):
+        v = om.qnodes_to_nodes(xq[:, :, comp].ravel())
+        interior = np.all((mesh.xy > 1e-12) & (mesh.xy < 1 - 1e-12), axis=1)
+        assert np.abs(v[interior] - mesh.xy[interior, comp]).max() < 1e-14
+    # a constant is reproduced everywhere
+    assert np.abs(om.qnodes_to_nodes(np.full(xq.shape[0] * xq.shape[1], 2.5)) - 2.5).max() < 1e-14
+
+
+# ---- gradient and local flux ----------------------------------------------------------------------
+@pytest.mark.parametrize("nn", [4, 8])
+@pytest.mark.parametrize("kind", list(MATERIALS))
+def test_local_flux_of_linear_field(nn, kind):
+    mesh = meshgen.structured_q4(8, jitter=0.2, seed=5) if nn == 4 else meshgen.structured_q8(6, jitter=0.2, seed=5)
+    a, b, c = 0.7, -1.3, 0.25
+    T = a * mesh.xy[:, 0] + b * mesh.xy[:, 1] + c
+    om = O.OracleMesh(mesh)
+    nflux, qflux = om.heat_flux([A.model(1.0, 0.0)], [MATERIALS[kind]], T)
+    expect = -LAMBDA[kind] @ np.array([a, b])
+    assert np.abs(qflux - expect[:, None]).max() < 1e-13
+    assert np.abs(nflux - expect[:, None]).max() < 1e-13
+
+
+# ---- nonlocal flux: the weak divergence of the flux is -K T -------------------------------------
+def _weak_divergence(om, mesh, qflux):
+    xq, jac, D = om.quadrature()
+    w = O.tables(mesh.nn)[0]
+    E, nn, nq = D.shape[0], D.shape[1], D.shape[2]
+    q = qflux.reshape(2, E, nq)
+    contrib = (D[..., 0] * (w * q[0])[:, None, :] + D[..., 1] * (w * q[1])[:, None, :]).sum(axis=2)   # (E, nn)
+    out = np.zeros(mesh.n_nodes)
+    np.add.at(out, mesh.elems.ravel(), contrib.ravel())
+    return out
+
+
+CASES = {
+    "q4_bell_iso": (lambda: meshgen.structured_q4(14), [configs.bell(0.4, 0.2)], [MATERIALS["isotropic"]]),
+    "q4_jitter_aniso_const": (lambda: meshgen.structured_q4(12, jitter=0.2, seed=2),
+                              [A.model(0.3, 0.25, influence=A.NLF_INFLUENCE_CONSTANT)], [MATERIALS["anisotropic"]]),
+    "q8_jitter_ortho_fast": (lambda: meshgen.structured_q8(7, jitter=0.2, seed=3),
+                             [A.model(0.5, 0.3, q=1.0)], [MATERIALS["orthotropic"]]),
+    "q4_composite_groups": (lambda: meshgen.composite_q4(20, count=3, rmin=0.12, rmax=0.2, seed=7)[0],
+                            None, None),
+}
+
+
+@pytest.mark.parametrize("case", list(CASES))
+def test_nonlocal_flux_weak_divergence(case):
+    build, models, materials = CASES[case]
+    mesh = build()
+    if models is None:                                   # matrix local, inclusions nonlocal, two conductivities
+        models = [A.model(1.0, 0.0)] + [configs.bell(0.5, 0.15)] * (mesh.n_groups - 1)
+        materials = [MATERIALS["isotropic"]] + [A.thermal_material(A.NLF_CONDUCTIVITY_ISOTROPIC, (10.0, 0, 0))] * (
+            mesh.n_groups - 1)
+    om = O.OracleMesh(mesh)
+    om.find_neighbours(H.radii(models))
+    inner = np.ones(mesh.n_nodes, dtype=np.uint8)          # no boundary conditions: the whole operator
+    K = om.assemble(0, models, materials, inner)
+    T = np.random.default_rng(11).standard_normal(mesh.n_nodes)
+    nflux, qflux = om.heat_flux(models, materials, T)
+    lhs = _weak_divergence(om, mesh, qflux)
+    rhs = -(H.upper_to_full(K.export(0), mesh.n_nodes) @ T)
+    assert np.linalg.norm(lhs - rhs) <= 1e-12 * np.linalg.norm(rhs)
+    # the nonlocal part is really there: the local flux alone does not satisfy it
+    local = [A.model(1.0, 0.0)] * len(models)
+    _, q_local = om.heat_flux(local, materials, T)
+    assert np.linalg.norm(_weak_divergence(om, mesh, q_local) - rhs) > 1e-3 * np.linalg.norm(rhs)
+
+
+def test_nonlocal_flux_needs_neighbours():
+    mesh = meshgen.structured_q4(4)
+    om = O.OracleMesh(mesh)
+    with pytest.raises(RuntimeError, match="neighbour"):
+        om.heat_flux([configs.bell(0.5, 0.3)], [MATERIALS["isotropic"]], np.zeros(mesh.n_nodes))
+
+
+# ---- the reference's solver tests on its SU2 meshes ------------------------------------------------
+@pytest.mark.parametrize("composite", [False, True], ids=["solid_ring", "composite_ring"])
+def test_ring_known_answers(composite):
+    mesh, models, materials, bcs, temperature, flux = H.ring_problem(composite)
+    r = O.thermal_problem(mesh, models, materials, bcs, tol=1e-14)
+    T = r["T"]
+    x, y = mesh.xy[:, 0], mesh.xy[:, 1]
+    nflux, _ = r["mesh"].heat_flux(models, materials, T)
+    fx, fy = flux(x, y)
+    tol_T, tol_q = H.RING_TOLERANCES[composite]
+    assert H.norm_error(T, temperature(x, y)) <= tol_T
+    assert H.norm_error(nflux[0], fx) <= tol_q
+    assert H.norm_error(nflux[1], fy) <= tol_q
