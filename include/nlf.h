@@ -188,10 +188,25 @@ int nlf_matrix_compare_fp32(nlf_matrix a, double* out);
  * may be NULL.  A nonlocal group needs nlf_find_neighbours first (NLF_ERR_STATE otherwise). */
 int nlf_heat_flux(nlf_mesh m, const nlf_model* models, const nlf_thermal_material* materials, const double* T_dev,
                   double* flux_dev, double* qflux_dev);
+/* ---- strains and stresses: replaces mechanical_solution_2d::calc_strain_and_stress
+ * (solvers/solver_2d/mechanical/mechanical_solution_2d.hpp:192-232) without thermal strains.  u_dev[2N]
+ * the displacement as the solve returns it (x, y per node); strain_dev[3N] and stress_dev[3N] node
+ * values (xx, yy, xy blocks; xy strain is the tensor shear (u_x,y + u_y,x)/2, xy stress 2 h66 eps_xy);
+ * qstress_dev[3*E*nq] quadrature-point stresses after the nonlocal average.  Any output may be NULL.
+ * A nonlocal group needs nlf_find_neighbours first (NLF_ERR_STATE otherwise). */
+int nlf_strain_stress(nlf_mesh m, const nlf_model* models, const nlf_elastic_material* materials, const double* u_dev,
+                      double* strain_dev, double* stress_dev, double* qstress_dev);
 
 /* inner_nodes (base/solvers_utils.hpp:31-40), boundary_condition_second_kind_2d
  * (base/boundary_condition_second_kind_2d.hpp:12-33) and the first-kind values (:17-26), host arrays */
 int nlf_boundary_data(nlf_mesh m, int dof, const nlf_boundary* bcs, double* f, double* xD, uint8_t* is_inner);
+/* The same with position-dependent values, as the reference's conditions are functions of the point
+ * (first_kind_2d / second_kind_2d, base/boundary_conditions_2d.hpp): value(x, y, group, dof, user) is
+ * called on the host at each node of a first-kind group and at each Gauss point of the edges of a
+ * second-kind (NLF_BC_FLUX) group, in place of bcs[group].value[dof].  Convection stays constant. */
+typedef double (*nlf_boundary_value_fn)(double x, double y, int group, int dof, void* user);
+int nlf_boundary_data_fn(nlf_mesh m, int dof, const nlf_boundary* bcs, nlf_boundary_value_fn value, void* user,
+                         double* f, double* xD, uint8_t* is_inner);
 /* boundary_condition_first_kind_2d (base/boundary_condition_first_kind_2d.hpp:50-59): f -= K_bound xD; f[D] = xD */
 int nlf_apply_first_kind(nlf_matrix a, const double* xD_dev, double* f_dev);
 

@@ -92,3 +92,15 @@ def array_of(ctype, items):
     for i, it in enumerate(items):
         arr[i] = it
     return arr
+
+
+# nlf_boundary_value_fn: double (*)(double x, double y, int group, int dof, void* user)
+BoundaryValueFn = C.CFUNCTYPE(C.c_double, C.c_double, C.c_double, C.c_int, C.c_int, C.c_void_p)
+
+
+def boundary_callback(values):
+    """ctypes callback of a Python callable values(x, y, group, dof) -> float (None -> None).  Keep the
+    returned object alive while the library may call it."""
+    if values is None:
+        return None
+    return BoundaryValueFn(lambda x, y, g, d, _user: float(values(x, y, g, d)))

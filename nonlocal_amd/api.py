@@ -373,7 +373,7 @@ class Solution:
     rhs: torch.Tensor = None
     timings: dict = field(default_factory=dict)
     stats: object = None
-    solution: object = None        # heat_equation_solution_2d of a thermal solve (flux on demand)
+    solution: object = None        # heat_equation_solution_2d / mechanical_solution_2d (post-processing on demand)
 
 
 class heat_equation_solution_2d:
@@ -434,6 +434,81 @@ class heat_equation_solution_2d:
         return self.flux()
 
 
+class mechanical_solution_2d:
+    """Displacement, strains and stresses of an elastic solve (solvers/solver_2d/mechanical/
+    mechanical_solution_2d.hpp:11-232), device resident.
+
+    calc_strain_and_stress() computes on the GPU the strains (u_x,x, u_y,y, (u_x,y + u_y,x)/2) at the
+    quadrature points, the stresses C eps and their nonlocal average p1 s + (1 - p1) int phi s for
+    nonlocal groups, and the node values as the area-weighted mean of the nearest quadrature points.
+    Thermal strains (a temperature change) are not included.
+    """
+
+    def __init__(self, mesh: mesh_2d, models, materials, displacement: torch.Tensor):
+        n = mesh.nodes_count
+        if displacement.numel() < 2 * n:
+            raise NLFError(_abi.NLF_ERR_ARGUMENT, "The displacement vector is shorter than twice the number of nodes.")
+        self.mesh = mesh
+        self._models = list(models)
+        self._materials = list(materials)
+        self._u = displacement.reshape(-1)[:2 * n].to(device=mesh.ctx.device, dtype=torch.float64).contiguous()
+        self._strain = self._stress = self._qstress = None
+
+    def displacement(self):
+        """(u_x, u_y) node vectors."""
+        return self._u[0::2], self._u[1::2]
+
+    def is_strain_and_stress_calculated(self) -> bool:
+        return self._stress is not None
+
+    def strain(self):
+        """(xx, yy, xy) node vectors; empty before calc_strain_and_stress, like the reference."""
+        return self._strain if self._strain is not None else ()
+
+    def stress(self):
+        return self._stress if self._stress is not None else ()
+
+    def stress_in_qnodes(self):
+        if self._qstress is None:
+            raise RuntimeError("Stress wasn't calculated")
+        return self._qstress
+
+    def calc_strain_and_stress(self):
+        if self.is_strain_and_stress_calculated():
+            return
+        mesh = self.mesh
+        if any(m.local_weight < 0.999 for m in self._models) and mesh.influences is None:
+            find_neighbours(mesh, _radii(self._models))
+        n, E = mesh.nodes_count, mesh.elements_count
+        nq = 4 if mesh.mesh.nn == 4 else 9
+        dev = mesh.ctx.device
+        strain = torch.empty(3 * n, dtype=torch.float64, device=dev)
+        stress = torch.empty(3 * n, dtype=torch.float64, device=dev)
+        q = torch.empty(3 * E * nq, dtype=torch.float64, device=dev)
+        marr = _abi.array_of(_abi.Model, self._models)
+        matarr = _abi.array_of(_abi.ElasticMaterial, self._materials)
+        mesh.ctx.check(load().nlf_strain_stress(mesh.h, marr, matarr, _dev(self._u), _dev(strain), _dev(stress),
+                                                _dev(q)))
+        self._strain = tuple(strain[k * n:(k + 1) * n] for k in range(3))
+        self._stress = tuple(stress[k * n:(k + 1) * n] for k in range(3))
+        self._qstress = tuple(q[k * E * nq:(k + 1) * E * nq] for k in range(3))
+
+
+def boundary_data(mesh: mesh_2d, dof, bcs, values=None):
+    """Host boundary data (f, x_D, is_inner) of one boundary condition per boundary group; values is
+    an optional callable (x, y, group, dof) -> value for the first-kind and second-kind groups, as
+    the reference's conditions are functions of the point."""
+    rows = dof * mesh.nodes_count
+    f = np.zeros(rows)
+    xD = np.zeros(rows)
+    inner = np.ones(rows, dtype=np.uint8)
+    arr = _abi.array_of(_abi.Boundary, bcs)
+    fn = _abi.boundary_callback(values)
+    mesh.ctx.check(load().nlf_boundary_data_fn(mesh.h, dof, arr, C.cast(fn, C.c_void_p) if fn else None, None,
+                                               _ptr(f, _f64p), _ptr(xD, _f64p), _ptr(inner, _u8p)))
+    return f, xD, inner
+
+
 def _radii(models):
     out = []
     for m in models:
@@ -447,7 +522,7 @@ def _radii(models):
 
 
 def _solve_driver(mesh: mesh_2d, physics, models, materials, bcs, precond, tol, max_it, mode, weights=None,
-                  assembly32=False, coarse=None):
+                  assembly32=False, coarse=None, values=None):
     import time
     ctx = mesh.ctx
     dof = 1 if physics == 0 else 2
@@ -456,12 +531,7 @@ def _solve_driver(mesh: mesh_2d, physics, models, materials, bcs, precond, tol, 
     if any(m.local_weight < 0.999 for m in models):
         find_neighbours(mesh, _radii(models))
     tm["neighbours"] = time.perf_counter() - t0
-    rows = dof * mesh.nodes_count
-    f = np.zeros(rows)
-    xD = np.zeros(rows)
-    inner = np.ones(rows, dtype=np.uint8)
-    arr = _abi.array_of(_abi.Boundary, bcs)
-    ctx.check(load().nlf_boundary_data(mesh.h, dof, arr, _ptr(f, _f64p), _ptr(xD, _f64p), _ptr(inner, _u8p)))
+    f, xD, inner = boundary_data(mesh, dof, bcs, values)
     t1 = time.perf_counter()
     K = _assemble(mesh, physics, models, materials, inner)
     if physics == 0:
@@ -513,27 +583,32 @@ def _solve_driver(mesh: mesh_2d, physics, models, materials, bcs, precond, tol, 
     ctx.synchronize()
     tm["solve"] = time.perf_counter() - t3
     st = cg.stats
-    heat = heat_equation_solution_2d(mesh, models, materials, x) if physics == 0 else None
+    post = (heat_equation_solution_2d if physics == 0 else mechanical_solution_2d)(mesh, models, materials, x)
     return Solution(x=x, iterations=st.iterations, residual=st.residual, true_residual=st.true_residual, matrix=K,
-                    local_matrix=KL, rhs=fd, timings=tm, stats=st, solution=heat)
+                    local_matrix=KL, rhs=fd, timings=tm, stats=st, solution=post)
 
 
 def stationary_heat_equation_solver_2d(mesh: mesh_2d, models, materials, bcs, precond=NLF_PRECOND_ILLT, tol=1e-12,
                                        max_it=100000, mode=NLF_MODE_FP64, weights=None, assembly32=False,
-                                       coarse=None) -> Solution:
+                                       coarse=None, values=None) -> Solution:
     """K T = f with K = p1 K_L + (1 - p1) K_NL (solved directly; the reference's correction form
     T = T0 - K^-1 (K T0 - f) with T0 = 0 is the same system).  precond NLF_PRECOND_LEARNED runs
     flexible CG with the learned preconditioner (weights file `weights`, default the packaged one)
     and Jacobi-PCG as its fallback; tol is then on the FP64 true residual.  In the mixed mode the
     inner FP32 operator is the rounded FP64 matrix, or with assembly32 an FP32 assembly.  coarse sets
     the learned preconditioner's aggregation coarse correction (about that many aggregates; 0 = one
-    level; default 4096)."""
-    return _solve_driver(mesh, 0, models, materials, bcs, precond, tol, max_it, mode, weights, assembly32, coarse)
+    level; default 4096).  values: optional callable (x, y, group, dof) -> boundary value at a point
+    for the first-kind and flux groups."""
+    return _solve_driver(mesh, 0, models, materials, bcs, precond, tol, max_it, mode, weights, assembly32, coarse,
+                         values)
 
 
 def equilibrium_equation(mesh: mesh_2d, models, materials, bcs, precond=None, tol=1e-12, max_it=100000,
-                         mode=NLF_MODE_FP64, assembly32=False) -> Solution:
-    """Plane-stress static elasticity; ILLT of the local stiffness when nonlocal, identity otherwise."""
+                         mode=NLF_MODE_FP64, assembly32=False, values=None) -> Solution:
+    """Plane-stress static elasticity; ILLT of the local stiffness when nonlocal, identity otherwise.
+    values: optional callable (x, y, group, dof) -> displacement or pressure of a boundary group at a
+    point.  The solution's `solution` is a mechanical_solution_2d."""
     if precond is None:
         precond = NLF_PRECOND_ILLT if any(m.local_weight < 0.999 for m in models) else NLF_PRECOND_IDENTITY
-    return _solve_driver(mesh, 1, models, materials, bcs, precond, tol, max_it, mode, assembly32=assembly32)
+    return _solve_driver(mesh, 1, models, materials, bcs, precond, tol, max_it, mode, assembly32=assembly32,
+                         values=values)

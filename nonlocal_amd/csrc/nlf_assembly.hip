@@ -1463,51 +1463,64 @@ int nlf_matrix_assemble_fp32(nlf_matrix a) {
 
 // Boundary data on the host (O(boundary)): second-kind RHS f (boundary_condition_second_kind_2d.hpp:12-33,
 // convection h*T_a), Dirichlet values xD (first wins) and the inner mask (solvers_utils.hpp:31-40).
-int nlf_boundary_data(nlf_mesh m, int dof, const nlf_boundary* bcs, double* f, double* xD, uint8_t* is_inner) {
-    if (!m) return NLF_ERR_ARGUMENT;
+int nlf_boundary_data_fn(nlf_mesh m, int dof, const nlf_boundary* bcs, nlf_boundary_value_fn value, void* user,
+                         double* f, double* xD, uint8_t* is_inner) {
+    if (!m || !bcs || (dof != 1 && dof != 2)) return NLF_ERR_ARGUMENT;
     return guarded(m->ctx, [&] {
         const i64 rows = dof * m->N;
         const BoundaryTables& t = m->bt;
         if (is_inner) std::fill(is_inner, is_inner + rows, uint8_t(1));
         if (xD) std::fill(xD, xD + rows, 0.0);
         if (f) std::fill(f, f + rows, 0.0);
+        // first kind: the value at the node; the first group giving a node a nonzero value keeps it
+        // (boundary_condition_first_kind_2d.hpp:17-26)
         for (int b = 0; b < m->GB; ++b)
             for (int d = 0; d < dof; ++d) {
                 if (bcs[b].kind[d] != NLF_BC_FIRST_KIND) continue;
                 for (i64 be = m->bbeg_h[b]; be < m->bbeg_h[b + 1]; ++be)
                     for (int i = 0; i < t.nb; ++i) {
                         const i64 v = m->be_h[be * t.nb + i];
-                        if (xD && xD[dof * v + d] == 0.0) xD[dof * v + d] = bcs[b].value[d];
+                        if (xD && xD[dof * v + d] == 0.0)
+                            xD[dof * v + d] = value ? value(m->xy_h[2 * v], m->xy_h[2 * v + 1], b, d, user) : bcs[b].value[d];
                         if (is_inner) is_inner[dof * v + d] = 0;
                     }
             }
         if (!f) return;
+        // second kind: int_edge N_i g |J| over the edge's Gauss points, g at the point
+        // (boundary_condition_second_kind_2d.hpp:16-23); convection adds h T_a the same way
         for (int b = 0; b < m->GB; ++b)
             for (int d = 0; d < dof; ++d) {
-                double g;
-                if (bcs[b].kind[d] == NLF_BC_FLUX) g = bcs[b].value[d];
-                else if (dof == 1 && bcs[b].kind[d] == NLF_BC_CONVECTION) g = bcs[b].heat_transfer * bcs[b].ambient;
-                else continue;
+                const int kind = bcs[b].kind[d];
+                const bool flux = kind == NLF_BC_FLUX;
+                if (!flux && !(dof == 1 && kind == NLF_BC_CONVECTION)) continue;
+                const double g0 = flux ? bcs[b].value[d] : bcs[b].heat_transfer * bcs[b].ambient;
                 for (i64 be = m->bbeg_h[b]; be < m->bbeg_h[b + 1]; ++be) {
-                    double jq[2];
+                    double jq[2], gq[2];
                     for (int q = 0; q < t.nq; ++q) {
-                        double jx = 0, jy = 0;
+                        double jx = 0, jy = 0, px = 0, py = 0;
                         for (int i = 0; i < t.nb; ++i) {
                             const i64 v = m->be_h[be * t.nb + i];
                             jx = jx + m->xy_h[2 * v] * t.qNxi[i * t.nq + q];
                             jy = jy + m->xy_h[2 * v + 1] * t.qNxi[i * t.nq + q];
+                            px = px + m->xy_h[2 * v] * t.qN[i * t.nq + q];
+                            py = py + m->xy_h[2 * v + 1] * t.qN[i * t.nq + q];
                         }
                         jq[q] = std::sqrt(jx * jx + jy * jy);
+                        gq[q] = flux && value ? value(px, py, b, d, user) : g0;
                     }
                     for (int i = 0; i < t.nb; ++i) {
                         const i64 v = m->be_h[be * t.nb + i];
                         double integral = 0.0;
-                        for (int q = 0; q < t.nq; ++q) integral += t.w[q] * t.qN[i * t.nq + q] * g * jq[q];
+                        for (int q = 0; q < t.nq; ++q) integral += t.w[q] * t.qN[i * t.nq + q] * gq[q] * jq[q];
                         f[dof * v + d] += integral;
                     }
                 }
             }
     });
+}
+
+int nlf_boundary_data(nlf_mesh m, int dof, const nlf_boundary* bcs, double* f, double* xD, uint8_t* is_inner) {
+    return nlf_boundary_data_fn(m, dof, bcs, nullptr, nullptr, f, xD, is_inner);
 }
 
 }  // extern "C"

@@ -16,6 +16,7 @@
 
 #include <array>
 #include <cstdint>
+#include <functional>
 #include <map>
 #include <memory>
 #include <stdexcept>
@@ -410,15 +411,47 @@ public:
 
 }  // namespace slae
 
+// A boundary value as a function of the point, like the reference's first_kind_2d / second_kind_2d.
+using point_function = std::function<double(const std::array<double, 2>&)>;
+
+namespace detail {
+
+// Boundary data of one condition per boundary group through nlf_boundary_data_fn: functions[b][d],
+// when set, give the value of group b, degree of freedom d at a point.
+inline void boundary_data(nlf_mesh mesh, nlf_ctx ctx, int dof, const std::vector<nlf_boundary>& bcs,
+                          const std::vector<std::array<point_function, 2>>& functions, std::vector<double>& f,
+                          std::vector<double>& xD, std::vector<uint8_t>& inner) {
+    struct closure {
+        const std::vector<nlf_boundary>* bcs;
+        const std::vector<std::array<point_function, 2>>* fn;
+    } c{&bcs, &functions};
+    const nlf_boundary_value_fn value = [](double x, double y, int group, int d, void* user) -> double {
+        const auto& cl = *static_cast<const closure*>(user);
+        const point_function& g = (*cl.fn)[size_t(group)][size_t(d)];
+        return g ? g({x, y}) : (*cl.bcs)[size_t(group)].value[d];
+    };
+    nlf_check(nlf_boundary_data_fn(mesh, dof, bcs.data(), value, &c, f.data(), xD.data(), inner.data()), ctx);
+}
+
+}  // namespace detail
+
 namespace thermal {
 
-// Boundary condition of one boundary group (thermal_boundary_conditions_2d.hpp).
+// Boundary condition of one boundary group (thermal_boundary_conditions_2d.hpp); the temperature
+// and the flux may be functions of the point.
 struct boundary_condition {
     nlf_boundary c{{NLF_BC_NONE, NLF_BC_NONE}, {0.0, 0.0}, 0.0, 0.0};
-    static boundary_condition temperature(double t) { return {{{NLF_BC_FIRST_KIND, NLF_BC_NONE}, {t, 0.0}, 0.0, 0.0}}; }
-    static boundary_condition flux(double q) { return {{{NLF_BC_FLUX, NLF_BC_NONE}, {q, 0.0}, 0.0, 0.0}}; }
+    point_function value;
+    static boundary_condition temperature(double t) { return {{{NLF_BC_FIRST_KIND, NLF_BC_NONE}, {t, 0.0}, 0.0, 0.0}, {}}; }
+    static boundary_condition temperature(point_function t) {
+        return {{{NLF_BC_FIRST_KIND, NLF_BC_NONE}, {0.0, 0.0}, 0.0, 0.0}, std::move(t)};
+    }
+    static boundary_condition flux(double q) { return {{{NLF_BC_FLUX, NLF_BC_NONE}, {q, 0.0}, 0.0, 0.0}, {}}; }
+    static boundary_condition flux(point_function q) {
+        return {{{NLF_BC_FLUX, NLF_BC_NONE}, {0.0, 0.0}, 0.0, 0.0}, std::move(q)};
+    }
     static boundary_condition convection(double h, double ambient) {
-        return {{{NLF_BC_CONVECTION, NLF_BC_NONE}, {0.0, 0.0}, h, ambient}};
+        return {{{NLF_BC_CONVECTION, NLF_BC_NONE}, {0.0, 0.0}, h, ambient}, {}};
     }
 };
 
@@ -496,14 +529,16 @@ heat_equation_solution_2d<T, I> stationary_heat_equation_solver_2d(const std::sh
     }
     if (nonlocal) mesh::find_neighbours(*mesh, radii);
     std::vector<nlf_boundary> bcs(GB);
+    std::vector<std::array<point_function, 2>> functions(GB);
     for (size_t b = 0; b < GB; ++b) {
         const auto it = conditions.find(mesh->data().boundary_names[b]);
         bcs[b] = it == conditions.end() ? boundary_condition{}.c : it->second.c;
+        if (it != conditions.end()) functions[b][0] = it->second.value;
     }
     const size_t rows = mesh->nodes_count();
     std::vector<double> f(rows), xD(rows);
     std::vector<uint8_t> inner8(rows);
-    nlf_check(nlf_boundary_data(mesh->handle(), 1, bcs.data(), f.data(), xD.data(), inner8.data()), ctx);
+    detail::boundary_data(mesh->handle(), ctx, 1, bcs, functions, f, xD, inner8);
     const std::vector<bool> is_inner(inner8.begin(), inner8.end());
     conductivity_matrix_2d<T, I, J> K(mesh);
     K.compute(parameters, is_inner);
@@ -528,6 +563,144 @@ heat_equation_solution_2d<T, I> stationary_heat_equation_solver_2d(const std::sh
 }
 
 }  // namespace thermal
+
+namespace mechanical {
+
+// Condition of one degree of freedom of a boundary group (mechanical_boundary_conditions_2d.hpp):
+// displacement_2d (first kind) or pressure_2d (second kind), a constant or a function of the point.
+struct boundary_condition {
+    int kind = NLF_BC_NONE;
+    double value = 0.0;
+    point_function function;
+    static boundary_condition displacement(double u) { return {NLF_BC_FIRST_KIND, u, {}}; }
+    static boundary_condition displacement(point_function u) { return {NLF_BC_FIRST_KIND, 0.0, std::move(u)}; }
+    static boundary_condition pressure(double p) { return {NLF_BC_FLUX, p, {}}; }
+    static boundary_condition pressure(point_function p) { return {NLF_BC_FLUX, 0.0, std::move(p)}; }
+};
+
+using boundaries_conditions_2d = std::unordered_map<std::string, std::array<boundary_condition, 2>>;
+
+// mechanical_solution_2d (mechanical/mechanical_solution_2d.hpp:11-232) without thermal strains:
+// calc_strain_and_stress runs nlf_strain_stress on the GPU and keeps the node values.
+template<class T = double, class I = int64_t>
+class mechanical_solution_2d {
+    std::shared_ptr<mesh::mesh_2d<T, I>> mesh_;
+    std::vector<nlf_model> models_;
+    std::vector<nlf_elastic_material> materials_;
+    std::vector<T> solution_;                           // x, y per node, as the solve returns it
+    std::array<std::vector<T>, 2> displacement_;
+    std::array<std::vector<T>, 3> strain_, stress_;
+
+public:
+    uintmax_t iterations = 0;
+    double residual = 0, true_residual = 0;
+
+    mechanical_solution_2d(const std::shared_ptr<mesh::mesh_2d<T, I>>& mesh, const parameters_2d& parameters,
+                           std::vector<T> displacement)
+        : mesh_(mesh), solution_(std::move(displacement)) {
+        const size_t n = mesh_->nodes_count();
+        if (solution_.size() < 2 * n)
+            throw nlf_error(NLF_ERR_ARGUMENT, "The displacement vector is shorter than twice the number of nodes.");
+        solution_.resize(2 * n);
+        for (size_t g = 0; g < mesh_->groups_count(); ++g) {
+            const auto& p = parameters.at(mesh_->data().group_names[g]);
+            models_.push_back(p.model.model);
+            materials_.push_back(p.material);
+        }
+        for (auto& d : displacement_) d.resize(n);
+        for (size_t i = 0; i < n; ++i) {
+            displacement_[0][i] = solution_[2 * i];
+            displacement_[1][i] = solution_[2 * i + 1];
+        }
+    }
+
+    const std::array<std::vector<T>, 2>& displacement() const noexcept { return displacement_; }
+    const std::array<std::vector<T>, 3>& strain() const noexcept { return strain_; }
+    const std::array<std::vector<T>, 3>& stress() const noexcept { return stress_; }
+    bool is_strain_and_stress_calculated() const noexcept {
+        for (size_t k = 0; k < 3; ++k)
+            if (strain_[k].empty() || stress_[k].empty()) return false;
+        return true;
+    }
+    // The neighbour lists of a nonlocal group must be on the mesh (find_neighbours), as in the reference.
+    void calc_strain_and_stress() {
+        if (is_strain_and_stress_calculated()) return;
+        const size_t n = mesh_->nodes_count();
+        device_vector du(solution_), dstrain(3 * n), dstress(3 * n);
+        nlf_check(nlf_strain_stress(mesh_->handle(), models_.data(), materials_.data(), du.data(), dstrain.data(),
+                                    dstress.data(), nullptr),
+                  mesh_->ctx().handle());
+        const std::vector<double> hs = dstrain.host(), ht = dstress.host();
+        for (size_t k = 0; k < 3; ++k) {
+            strain_[k].assign(hs.begin() + std::ptrdiff_t(k * n), hs.begin() + std::ptrdiff_t((k + 1) * n));
+            stress_[k].assign(ht.begin() + std::ptrdiff_t(k * n), ht.begin() + std::ptrdiff_t((k + 1) * n));
+        }
+    }
+};
+
+// equilibrium_equation (mechanical/equilibrium_equation_2d.hpp:20-62): neighbours, stiffness,
+// boundary conditions, CG with ILLT of the local stiffness when nonlocal (identity otherwise, and
+// on a failed factorisation), then the strains and stresses.
+template<class T = double, class I = int64_t, class J = int64_t>
+mechanical_solution_2d<T, I> equilibrium_equation(const std::shared_ptr<mesh::mesh_2d<T, I>>& mesh,
+                                                  const parameters_2d& parameters,
+                                                  const boundaries_conditions_2d& conditions,
+                                                  double tolerance = 1e-15, uintmax_t max_iterations = 10000) {
+    const size_t G = mesh->groups_count(), GB = mesh->data().boundary_names.size();
+    nlf_ctx ctx = mesh->ctx().handle();
+    mesh::influences<T> radii;
+    bool nonlocal = false;
+    for (size_t g = 0; g < G; ++g) {
+        const auto& m = parameters.at(mesh->data().group_names[g]).model.model;
+        if (m.local_weight < 0.999) {
+            nonlocal = true;
+            radii[mesh->data().group_names[g]] = mesh::influence<T>{{m.search_radius[0] > 0 ? m.search_radius[0] : m.radius[0],
+                                                                    m.search_radius[1] > 0 ? m.search_radius[1] : m.radius[1]}};
+        }
+    }
+    if (nonlocal) mesh::find_neighbours(*mesh, radii);
+    std::vector<nlf_boundary> bcs(GB, nlf_boundary{{NLF_BC_NONE, NLF_BC_NONE}, {0.0, 0.0}, 0.0, 0.0});
+    std::vector<std::array<point_function, 2>> functions(GB);
+    for (size_t b = 0; b < GB; ++b) {
+        const auto it = conditions.find(mesh->data().boundary_names[b]);
+        if (it == conditions.end()) continue;
+        for (int d = 0; d < 2; ++d) {
+            bcs[b].kind[d] = it->second[size_t(d)].kind;
+            bcs[b].value[d] = it->second[size_t(d)].value;
+            functions[b][size_t(d)] = it->second[size_t(d)].function;
+        }
+    }
+    const size_t rows = 2 * mesh->nodes_count();
+    std::vector<double> f(rows), xD(rows);
+    std::vector<uint8_t> inner8(rows);
+    detail::boundary_data(mesh->handle(), ctx, 2, bcs, functions, f, xD, inner8);
+    const std::vector<bool> is_inner(inner8.begin(), inner8.end());
+    stiffness_matrix<T, I, J> K(mesh);
+    K.compute(parameters, is_inner);
+    device_vector df(f), dxD(xD), dx(rows);
+    nlf_check(nlf_apply_first_kind(K.matrix().handle(), dxD.data(), df.data()), ctx);
+    slae::conjugate_gradient cg(K.matrix());
+    cg.set_tolerance(tolerance);
+    cg.set_max_interations(max_iterations);
+    stiffness_matrix<T, I, J> KL(mesh);
+    if (nonlocal) {
+        KL.compute(parameters, is_inner, true, assemble_part::LOCAL);
+        auto& P = cg.init_preconditioner<slae::eigen_ILLT_preconditioner>(KL.matrix());
+        if (!P.computation_info()) cg.init_preconditioner<slae::eigen_identity_preconditioner>(K.matrix());
+    } else {
+        cg.init_preconditioner<slae::eigen_identity_preconditioner>(K.matrix());
+    }
+    dx.upload(std::vector<double>(rows, 0.0));
+    cg.solve(df, dx);
+    mechanical_solution_2d<T, I> solution(mesh, parameters, dx.host());
+    solution.iterations = cg.iterations();
+    solution.residual = cg.residual();
+    solution.true_residual = cg.statistics().true_residual;
+    solution.calc_strain_and_stress();
+    return solution;
+}
+
+}  // namespace mechanical
 
 }  // namespace nonlocal
 

@@ -758,19 +758,41 @@ double boundary_jac(const Mesh& m, i64 be, int q) {  // element_data_1d::jacobi_
 
 // boundary_condition_second_kind_2d (boundary_condition_second_kind_2d.hpp:12-33) with constant values;
 // thermal convection also adds h*T_a (thermal_boundary_conditions_2d.hpp:54-56).
-void second_kind(const Mesh& m, int dof, const nlf_boundary* bc, double* f) {
+// A position-dependent value: condition(element.quad_coord(q)) for the second kind, and
+// condition(node_coord) for the first kind (boundary_condition_first_kind_2d.hpp:17-26).
+using ValueFn = double (*)(double, double, int, int, void*);
+
+void boundary_point(const Mesh& m, i64 be, int q, double* p) {   // element_1d_data::quad_coord
+    double x = 0.0, y = 0.0;
+    for (int i = 0; i < m.nb; ++i) {
+        const i64 node = m.be[be * m.nb + i];
+        x += m.xy[2 * node] * m.t1.qN[i * m.t1.nq + q];
+        y += m.xy[2 * node + 1] * m.t1.qN[i * m.t1.nq + q];
+    }
+    p[0] = x; p[1] = y;
+}
+
+void second_kind(const Mesh& m, int dof, const nlf_boundary* bc, double* f, ValueFn fn = nullptr, void* user = nullptr) {
     for (size_t b = 0; b + 1 < m.bbeg.size(); ++b)
         for (int d = 0; d < dof; ++d) {
             double g;
-            if (bc[b].kind[d] == NLF_BC_FLUX) g = bc[b].value[d];
+            const bool flux = bc[b].kind[d] == NLF_BC_FLUX;
+            if (flux) g = bc[b].value[d];
             else if (dof == 1 && bc[b].kind[d] == NLF_BC_CONVECTION) g = bc[b].heat_transfer * bc[b].ambient;
             else continue;
             for (i64 be = m.bbeg[b]; be < m.bbeg[b + 1]; ++be)
                 for (int i = 0; i < m.nb; ++i) {
                     const i64 node = m.be[be * m.nb + i];
                     double integral = 0.0;
-                    for (int q = 0; q < m.t1.nq; ++q)
-                        integral += m.t1.w[q] * m.t1.qN[i * m.t1.nq + q] * g * boundary_jac(m, be, q);
+                    for (int q = 0; q < m.t1.nq; ++q) {
+                        double gq = g;
+                        if (flux && fn) {
+                            double p[2];
+                            boundary_point(m, be, q, p);
+                            gq = fn(p[0], p[1], int(b), d, user);
+                        }
+                        integral += m.t1.w[q] * m.t1.qN[i * m.t1.nq + q] * gq * boundary_jac(m, be, q);
+                    }
                     f[dof * node + d] += integral;
                 }
         }
@@ -798,14 +820,16 @@ void convection(const Mesh& m, const nlf_boundary* bc, const std::vector<uint8_t
 }
 
 // Dirichlet values x_D (calc_vector :17-26) and inner mask (solvers_utils.hpp:31-40)
-void first_kind_vector(const Mesh& m, int dof, const nlf_boundary* bc, double* x, uint8_t* inner) {
+void first_kind_vector(const Mesh& m, int dof, const nlf_boundary* bc, double* x, uint8_t* inner, ValueFn fn = nullptr,
+                       void* user = nullptr) {
     for (size_t b = 0; b + 1 < m.bbeg.size(); ++b)
         for (int d = 0; d < dof; ++d) {
             if (bc[b].kind[d] != NLF_BC_FIRST_KIND) continue;
             for (i64 be = m.bbeg[b]; be < m.bbeg[b + 1]; ++be)
                 for (int i = 0; i < m.nb; ++i) {
                     const i64 node = m.be[be * m.nb + i];
-                    if (x && x[dof * node + d] == 0.0) x[dof * node + d] = bc[b].value[d];
+                    if (x && x[dof * node + d] == 0.0)
+                        x[dof * node + d] = fn ? fn(m.xy[2 * node], m.xy[2 * node + 1], int(b), d, user) : bc[b].value[d];
                     if (inner) inner[dof * node + d] = 0;
                 }
         }
@@ -1011,24 +1035,32 @@ void qnodes_to_nodes(const Mesh& m, const double* x, double* out) {
     }
 }
 
-// qflux: [2][E*nq] quadrature-point fluxes (x block, then y block); nflux: [2][N] node fluxes.
-void heat_flux(const Mesh& m, const std::vector<Group>& groups, const double* T, double* qflux, double* nflux) {
+// gradient_in_qnodes (mesh_2d_utils.hpp:39-63): sum_i x_i D_i(q), then divided by |J_q|; x is read
+// with a stride (1 for the temperature, 2 for one component of the displacement).
+void gradient_in_qnodes(const Mesh& m, const double* x, int stride, std::vector<double>& gx, std::vector<double>& gy) {
     const int nq = m.t.nq, nn = m.nn;
-    const i64 Q = m.E * nq;
-    // gradient_in_qnodes (mesh_2d_utils.hpp:39-63): sum_i T_i D_i(q), then divided by |J_q|
-    std::vector<double> fx(Q, 0.0), fy(Q, 0.0);
+    gx.assign(m.E * nq, 0.0);
+    gy.assign(m.E * nq, 0.0);
 #pragma omp parallel for schedule(static)
     for (i64 e = 0; e < m.E; ++e)
         for (int q = 0; q < nq; ++q) {
             const i64 qs = e * nq + q;
             for (int i = 0; i < nn; ++i) {
-                const double val = T[m.el[e * nn + i]];
-                fx[qs] += val * m.Dx(e, i, q);
-                fy[qs] += val * m.Dy(e, i, q);
+                const double val = x[i64(stride) * m.el[e * nn + i]];
+                gx[qs] += val * m.Dx(e, i, q);
+                gy[qs] += val * m.Dy(e, i, q);
             }
-            fx[qs] /= m.jac[qs];
-            fy[qs] /= m.jac[qs];
+            gx[qs] /= m.jac[qs];
+            gy[qs] /= m.jac[qs];
         }
+}
+
+// qflux: [2][E*nq] quadrature-point fluxes (x block, then y block); nflux: [2][N] node fluxes.
+void heat_flux(const Mesh& m, const std::vector<Group>& groups, const double* T, double* qflux, double* nflux) {
+    const int nq = m.t.nq;
+    const i64 Q = m.E * nq;
+    std::vector<double> fx, fy;
+    gradient_in_qnodes(m, T, 1, fx, fy);
     // local_flux_in_qnodes (heat_equation_solution_2d.hpp:72-97): -lambda grad T per group
     for (int g = 0; g < m.G(); ++g) {
         const nlf_thermal_material& c = groups[g].th;
@@ -1081,6 +1113,63 @@ void heat_flux(const Mesh& m, const std::vector<Group>& groups, const double* T,
         qnodes_to_nodes(m, gx.data(), nflux);
         qnodes_to_nodes(m, gy.data(), nflux + m.N);
     }
+}
+
+// calc_stress for isotropic Hooke {h11, h12, h66} (mechanical_parameters_2d.hpp:27-35)
+void calc_stress(const double* h, double e11, double e22, double e12, double* out) {
+    out[0] = h[0] * e11 + h[1] * e22;
+    out[1] = h[1] * e11 + h[0] * e22;
+    out[2] = 2 * h[2] * e12;
+}
+
+// mechanical_solution_2d::calc_strain_and_stress (mechanical_solution_2d.hpp:192-232) without thermal
+// strains.  u: displacement (x, y per node).  qstress [3][E*nq]; nstrain, nstress [3][N] (xx, yy, xy).
+void strain_stress(const Mesh& m, const std::vector<Group>& groups, const double* u, double* qstress, double* nstrain,
+                   double* nstress) {
+    const int nq = m.t.nq;
+    const i64 Q = m.E * nq;
+    // strains_in_quadratures (:150-157)
+    std::vector<double> g11, g12, g21, g22;
+    gradient_in_qnodes(m, u, 2, g11, g12);
+    gradient_in_qnodes(m, u + 1, 2, g21, g22);
+    for (i64 q = 0; q < Q; ++q) g12[q] = 0.5 * (g12[q] + g21[q]);
+    const double* eps[3] = {g11.data(), g22.data(), g12.data()};
+    if (nstrain)
+        for (int c = 0; c < 3; ++c) qnodes_to_nodes(m, eps[c], nstrain + c * m.N);
+    std::vector<double> st(3 * Q, 0.0);
+    for (int g = 0; g < m.G(); ++g) {
+        const Group& gr = groups[g];
+        if (gr.nonlocal && m.nb_off.empty()) throw std::logic_error("stress of a nonlocal group needs the neighbour lists");
+        const double w1 = 1.0 - gr.p1;
+#pragma omp parallel for schedule(dynamic, 64)
+        for (i64 eL = m.gbeg[g]; eL < m.gbeg[g + 1]; ++eL)
+            for (int qL = 0; qL < nq; ++qL) {
+                const i64 qsL = eL * nq + qL;
+                double local[3];
+                calc_stress(gr.hooke, eps[0][qsL], eps[1][qsL], eps[2][qsL], local);
+                double stress[3] = {gr.p1 * local[0], gr.p1 * local[1], gr.p1 * local[2]};
+                if (gr.nonlocal) {
+                    // calc_nonlocal_stress (:173-189): influence_weight = w |J| (1 - p1) phi
+                    double ns[3] = {0.0, 0.0, 0.0};
+                    const double* xl = m.xqp(eL, qL);
+                    for (i64 k = m.nb_off[eL]; k < m.nb_off[eL + 1]; ++k) {
+                        const i64 eNL = m.nb_ids[k];
+                        for (int qN = 0; qN < nq; ++qN) {
+                            const i64 qs = eNL * nq + qN;
+                            const double iw = m.t.w[qN] * m.jac[qs] * (w1 * influence_eval(gr.inf, xl, m.xqp(eNL, qN)));
+                            double sn[3];
+                            calc_stress(gr.hooke, eps[0][qs], eps[1][qs], eps[2][qs], sn);
+                            for (int c = 0; c < 3; ++c) ns[c] += iw * sn[c];
+                        }
+                    }
+                    for (int c = 0; c < 3; ++c) stress[c] += ns[c];
+                }
+                for (int c = 0; c < 3; ++c) st[c * Q + qsL] += stress[c];
+            }
+    }
+    if (qstress) std::copy(st.begin(), st.end(), qstress);
+    if (nstress)
+        for (int c = 0; c < 3; ++c) qnodes_to_nodes(m, st.data() + c * Q, nstress + c * m.N);
 }
 
 }  // namespace
@@ -1190,13 +1279,18 @@ void orc_matrix_set_values(void* a, int part, const double* val) {
 }
 
 // Thermal boundary data: f (second kind + convection RHS), x_D, inner mask; and the convection matrix term.
-void orc_boundary(void* h, int dof, const nlf_boundary* bc, double* f, double* xD, uint8_t* inner) {
+void orc_boundary_fn(void* h, int dof, const nlf_boundary* bc, ValueFn fn, void* user, double* f, double* xD,
+                     uint8_t* inner) {
     const Mesh& m = *static_cast<Mesh*>(h);
     const i64 rows = dof * m.N;
     if (inner) std::fill(inner, inner + rows, uint8_t(1));
     if (xD) std::fill(xD, xD + rows, 0.0);
-    first_kind_vector(m, dof, bc, xD, inner);
-    if (f) { std::fill(f, f + rows, 0.0); second_kind(m, dof, bc, f); }
+    first_kind_vector(m, dof, bc, xD, inner, fn, user);
+    if (f) { std::fill(f, f + rows, 0.0); second_kind(m, dof, bc, f, fn, user); }
+}
+
+void orc_boundary(void* h, int dof, const nlf_boundary* bc, double* f, double* xD, uint8_t* inner) {
+    orc_boundary_fn(h, dof, bc, nullptr, nullptr, f, xD, inner);
 }
 
 void orc_convection(void* h, void* a, const nlf_boundary* bc, const uint8_t* is_inner) {
@@ -1254,6 +1348,17 @@ int orc_heat_flux(void* h, const nlf_model* models, const nlf_thermal_material* 
         std::vector<Group> groups;
         for (int g = 0; g < m.G(); ++g) groups.push_back(make_group(models[g], 0, &materials[g], nullptr, false));
         heat_flux(m, groups, T, qflux, nflux);
+        return 0;
+    } catch (const std::exception& e) { g_error = e.what(); return -1; }
+}
+
+int orc_strain_stress(void* h, const nlf_model* models, const nlf_elastic_material* materials, const double* u,
+                      double* qstress, double* nstrain, double* nstress) {
+    try {
+        const Mesh& m = *static_cast<Mesh*>(h);
+        std::vector<Group> groups;
+        for (int g = 0; g < m.G(); ++g) groups.push_back(make_group(models[g], 1, nullptr, &materials[g], false));
+        strain_stress(m, groups, u, qstress, nstrain, nstress);
         return 0;
     } catch (const std::exception& e) { g_error = e.what(); return -1; }
 }

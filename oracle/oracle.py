@@ -72,6 +72,11 @@ def lib():
         L.orc_nearest_qnodes.argtypes = [C.c_int, _i32p]
         L.orc_element_areas.argtypes = [C.c_void_p, _f64p]
         L.orc_qnodes_to_nodes.argtypes = [C.c_void_p, _f64p, _f64p]
+        L.orc_strain_stress.restype = C.c_int
+        L.orc_strain_stress.argtypes = [C.c_void_p, C.POINTER(_abi.Model), C.POINTER(_abi.ElasticMaterial), _f64p,
+                                        _f64p, _f64p, _f64p]
+        L.orc_boundary_fn.argtypes = [C.c_void_p, C.c_int, C.POINTER(_abi.Boundary), C.c_void_p, C.c_void_p, _f64p,
+                                      _f64p, _u8p]
         L.orc_heat_flux.restype = C.c_int
         L.orc_heat_flux.argtypes = [C.c_void_p, C.POINTER(_abi.Model), C.POINTER(_abi.ThermalMaterial), _f64p, _f64p,
                                     _f64p]
@@ -216,14 +221,18 @@ class OracleMesh:
         lib().orc_neighbours_export(self.h, _ptr(off, _i64p), _ptr(ids, _i32p))
         return off, ids, expanded
 
-    def boundary(self, bcs, dof=1):
-        """bcs: list of nlf_boundary per boundary group.  Returns (f, xD, is_inner)."""
+    def boundary(self, bcs, dof=1, values=None):
+        """bcs: list of nlf_boundary per boundary group; values: optional callable (x, y, group, dof) ->
+        value of the first-kind and second-kind (flux / pressure) groups at a point.  Returns
+        (f, xD, is_inner)."""
         rows = dof * self.mesh.n_nodes
         arr = _abi.array_of(_abi.Boundary, bcs)
         f = np.zeros(rows)
         xD = np.zeros(rows)
         inner = np.ones(rows, dtype=np.uint8)
-        lib().orc_boundary(self.h, dof, arr, _ptr(f, _f64p), _ptr(xD, _f64p), _ptr(inner, _u8p))
+        fn = _abi.boundary_callback(values)
+        lib().orc_boundary_fn(self.h, dof, arr, C.cast(fn, C.c_void_p) if fn else None, None, _ptr(f, _f64p),
+                              _ptr(xD, _f64p), _ptr(inner, _u8p))
         return f, xD, inner
 
     def assemble(self, physics, models, materials, is_inner, symmetric=True, part=_abi.NLF_PART_FULL,
@@ -262,6 +271,22 @@ class OracleMesh:
         if lib().orc_heat_flux(self.h, marr, matarr, _ptr(T, _f64p), _ptr(q, _f64p), _ptr(n, _f64p)) != 0:
             raise RuntimeError(lib().orc_last_error().decode())
         return n, q
+
+    def strain_stress(self, models, materials, u):
+        """mechanical_solution_2d::calc_strain_and_stress (mechanical_solution_2d.hpp:192-232).  u: the
+        displacement (x, y per node).  Returns (node strains (3, N), node stresses (3, N),
+        quad-point stresses (3, E*nq)), components xx, yy, xy."""
+        nq = 4 if self.mesh.nn == 4 else 9
+        u = np.ascontiguousarray(u, dtype=np.float64)
+        q = np.zeros((3, self.mesh.n_elems * nq))
+        ns = np.zeros((3, self.mesh.n_nodes))
+        nst = np.zeros((3, self.mesh.n_nodes))
+        marr = _abi.array_of(_abi.Model, models)
+        matarr = _abi.array_of(_abi.ElasticMaterial, materials)
+        if lib().orc_strain_stress(self.h, marr, matarr, _ptr(u, _f64p), _ptr(q, _f64p), _ptr(ns, _f64p),
+                                   _ptr(nst, _f64p)) != 0:
+            raise RuntimeError(lib().orc_last_error().decode())
+        return ns, nst, q
 
     def convection(self, matrix: OracleMatrix, bcs, is_inner):
         arr = _abi.array_of(_abi.Boundary, bcs)
@@ -311,14 +336,14 @@ def cg(matrix: OracleMatrix, precond: OraclePrecond, b, x0=None, tol=1e-15, max_
 # Problem drivers (stationary_heat_equation_solver_2d.hpp:46-120, equilibrium_equation_2d.hpp:20-62)
 # ---------------------------------------------------------------------------------------------
 def thermal_problem(mesh, models, materials, bcs, precond=_abi.NLF_PRECOND_ILLT, tol=1e-12, max_it=100000,
-                    radii=None, faithful_reset=False):
+                    radii=None, faithful_reset=False, values=None):
     """Solve the stationary thermal problem; returns a dict with neighbours, matrices, f and T."""
     om = OracleMesh(mesh)
     if radii is None:
         radii = [[m.search_radius[0] or m.radius[0], m.search_radius[1] or m.radius[1]]
                  if m.local_weight < 0.999 else [0.0, 0.0] for m in models]
     off, ids, expanded = om.find_neighbours(radii)
-    f, xD, inner = om.boundary(bcs, dof=1)
+    f, xD, inner = om.boundary(bcs, dof=1, values=values)
     K = om.assemble(0, models, materials, inner, symmetric=True, faithful_reset=faithful_reset)
     om.convection(K, bcs, inner)
     f = first_kind_rhs(K, inner, xD, f)
@@ -335,12 +360,12 @@ def thermal_problem(mesh, models, materials, bcs, precond=_abi.NLF_PRECOND_ILLT,
                 iterations=it, residual=res)
 
 
-def elastic_problem(mesh, models, materials, bcs, tol=1e-12, max_it=100000, radii=None):
+def elastic_problem(mesh, models, materials, bcs, tol=1e-12, max_it=100000, radii=None, values=None):
     om = OracleMesh(mesh)
     if radii is None:
         radii = [[m.radius[0], m.radius[1]] if m.local_weight < 0.999 else [0.0, 0.0] for m in models]
     off, ids, expanded = om.find_neighbours(radii)
-    f, xD, inner = om.boundary(bcs, dof=2)
+    f, xD, inner = om.boundary(bcs, dof=2, values=values)
     K = om.assemble(1, models, materials, inner, symmetric=True)
     f = first_kind_rhs(K, inner, xD, f)
     nonlocal_ = any(m.local_weight < 0.999 for m in models)

@@ -92,3 +92,111 @@ def ring_problem(composite: bool):
 
 # tolerances of the reference tests: (temperature, flux)
 RING_TOLERANCES = {False: (1.4e-3, 1.7e-2), True: (1.2e-3, 2.1e-2)}
+
+
+def lame_ring_problem(composite: bool):
+    """isotropic_lame_solid_ring.cpp / isotropic_lame_composite_ring.cpp: plane stress, u_y = 0 on
+    "Horizontal", u_x = 0 on "Vertical", radial pressure 0.05 on "Inner" and -0.01 on "Outer" (as
+    functions of the point).  Returns (mesh, models, materials, bcs, values, expected) where
+    expected maps "ux", "uy", "exx", "eyy", "exy", "sxx", "syy", "sxy" to functions of (x, y)."""
+    from nonlocal_amd import su2
+    mesh = su2.read(os.path.join(MESH_DIR, "composite_ring.su2" if composite else "solid_ring.su2"))
+    names = mesh.bgroup_names
+    bcs = [A.boundary() for _ in names]
+    bcs[names.index("Horizontal")] = A.boundary(A.NLF_BC_NONE, 0.0, A.NLF_BC_FIRST_KIND, 0.0)
+    bcs[names.index("Vertical")] = A.boundary(A.NLF_BC_FIRST_KIND, 0.0, A.NLF_BC_NONE, 0.0)
+    bcs[names.index("Inner")] = A.boundary(A.NLF_BC_FLUX, 0.0, A.NLF_BC_FLUX, 0.0)
+    bcs[names.index("Outer")] = A.boundary(A.NLF_BC_FLUX, 0.0, A.NLF_BC_FLUX, 0.0)
+    gi, go = names.index("Inner"), names.index("Outer")
+
+    def values(x, y, g, d):
+        c = (x, y)[d] / np.hypot(x, y)
+        return 0.05 * c if g == gi else (-0.01 * c if g == go else 0.0)
+
+    models = [A.model(1.0, 0.0) for _ in mesh.group_names]
+    young = {"DEFAULT": (350.0, 0.25), "Inner_Material": (350.0, 0.25), "Outer_Material": (150.0, 0.2)}
+    materials = [A.elastic_material(*young[g]) for g in mesh.group_names]
+    if not composite:
+        def r2(x, y):
+            return x * x + y * y
+        expected = {
+            "ux": lambda x, y: x * (20 + 3 * r2(x, y)) / (420000 * r2(x, y)),
+            "uy": lambda x, y: y * (20 + 3 * r2(x, y)) / (420000 * r2(x, y)),
+            "exx": lambda x, y: (3 * x**4 + y**2 * (20 + 3 * y**2) + x**2 * (-20 + 6 * y**2)) / (420000 * r2(x, y)**2),
+            "eyy": lambda x, y: (3 * x**4 + y**2 * (-20 + 3 * y**2) + x**2 * (20 + 6 * y**2)) / (420000 * r2(x, y)**2),
+            "exy": lambda x, y: -np.sin(2 * np.arctan2(y, x)) / (21000 * r2(x, y)),
+            "sxx": lambda x, y: (x**4 + 2 * x**2 * (-2 + y**2) + y**2 * (4 + y**2)) / (300 * r2(x, y)**2),
+            "syy": lambda x, y: (x**4 + 2 * x**2 * (2 + y**2) + y**2 * (-4 + y**2)) / (300 * r2(x, y)**2),
+            "sxy": lambda x, y: -np.sin(2 * np.arctan2(y, x)) / (75 * r2(x, y)),
+        }
+        return mesh, models, materials, bcs, values, expected
+    rc2 = 0.75 ** 2
+
+    def piece(inner, outer, contact=True):
+        """inner solution for r^2 < 0.75^2, outer beyond; on the contact circle the mean of the two
+        (is_contact, isotropic_lame_composite_ring.cpp:25-28), for the strains and stresses only"""
+        def f(x, y):
+            r = x * x + y * y
+            fi, fo = inner(x, y, r), outer(x, y, r)
+            v = np.where(r < rc2, fi, fo)
+            return np.where(np.abs(r - rc2) < 1e-12, 0.5 * (fi + fo), v) if contact else v
+        return f
+
+    expected = {
+        "ux": piece(lambda x, y, r: x * (2.5284 + r) / (47303 * r), lambda x, y, r: x * (9.7572 - r) / (140715 * r),
+                    False),
+        "uy": piece(lambda x, y, r: y * (2.5284 + r) / (47303 * r), lambda x, y, r: y * (9.7572 - r) / (140715 * r),
+                    False),
+        "exx": piece(lambda x, y, r: (1 + 2.5284 * (y * y - x * x) / r**2) / 47303,
+                     lambda x, y, r: (-1 + 9.7572 * (y * y - x * x) / r**2) / 140715),
+        "eyy": piece(lambda x, y, r: (1 - 2.5284 * (y * y - x * x) / r**2) / 47303,
+                     lambda x, y, r: (-1 - 9.7572 * (y * y - x * x) / r**2) / 140715),
+        "exy": piece(lambda x, y, r: -x * y / (9354.31 * r**2), lambda x, y, r: -x * y / (7210.83 * r**2)),
+        "sxx": piece(lambda x, y, r: (1 + 1.5170 * (y * y - x * x) / r**2) / 101.364,
+                     lambda x, y, r: -(1 - 6.5048 * (y * y - x * x) / r**2) / 750.481),
+        "syy": piece(lambda x, y, r: (1 - 1.5170 * (y * y - x * x) / r**2) / 101.364,
+                     lambda x, y, r: -(1 + 6.5048 * (y * y - x * x) / r**2) / 750.481),
+        "sxy": piece(lambda x, y, r: -x * y / (33.4082 * r**2), lambda x, y, r: -x * y / (57.6866 * r**2)),
+    }
+    return mesh, models, materials, bcs, values, expected
+
+
+# tolerances of the reference tests
+LAME_TOLERANCES = {
+    False: {"ux": 1.1e-3, "uy": 1.1e-3, "exx": 3.1e-2, "eyy": 3.1e-2, "exy": 3.2e-2,
+            "sxx": 3.3e-2, "syy": 3.3e-2, "sxy": 3.3e-2},
+    True: {"ux": 1.6e-3, "uy": 1.6e-3, "exx": 2.7e-2, "eyy": 2.7e-2, "exy": 2.8e-2,
+           "sxx": 2.9e-2, "syy": 2.9e-2, "sxy": 2.9e-2},
+}
+
+
+def saint_venant_problem():
+    """saint_venant.cpp:26-49 on plate_10x1_h0.125.su2: E = 1, nu = 0.3; u_x = 0 and no y traction on
+    "Vertical"; no x traction and u_y = 0 on "Horizontal"; x traction -4|y| on "Left", +4|y| on
+    "Right".  Returns (mesh, models, materials, bcs, values)."""
+    from nonlocal_amd import su2
+    mesh = su2.read(os.path.join(MESH_DIR, "plate_10x1_h0.125.su2"))
+    names = mesh.bgroup_names
+    bcs = [A.boundary() for _ in names]
+    bcs[names.index("Vertical")] = A.boundary(A.NLF_BC_FIRST_KIND, 0.0, A.NLF_BC_FLUX, 0.0)
+    bcs[names.index("Horizontal")] = A.boundary(A.NLF_BC_FLUX, 0.0, A.NLF_BC_FIRST_KIND, 0.0)
+    bcs[names.index("Left")] = A.boundary(A.NLF_BC_FLUX, 0.0, A.NLF_BC_FLUX, 0.0)
+    bcs[names.index("Right")] = A.boundary(A.NLF_BC_FLUX, 0.0, A.NLF_BC_FLUX, 0.0)
+    gl, gr = names.index("Left"), names.index("Right")
+
+    def values(x, y, g, d):
+        if d == 0 and g == gl:
+            return -4 * abs(y)
+        if d == 0 and g == gr:
+            return 4 * abs(y)
+        return 0.0
+
+    return mesh, [A.model(1.0, 0.0)], [A.elastic_material(1.0, 0.3)], bcs, values
+
+
+def integrate_nodes(om, mesh, v):
+    """mesh::utils::integrate (mesh_2d_utils.hpp:135-149): sum_e sum_q w_q |J_q| sum_i N_i(q) v_i."""
+    xq, jac, D = om.quadrature()
+    w, qN = O.tables(mesh.nn)[:2]
+    vals = np.asarray(v)[mesh.elems]                              # (E, nn)
+    return float(np.einsum("q,eq,iq,ei->", w, jac, qN, vals))

@@ -1,6 +1,6 @@
 """The C++ mirror of the reference API (nonlocal_amd/include/nonlocal/nonlocal_amd.hpp): it compiles
-against include/nlf.h and links libnlf.so (CPU), and its heat solver driver reproduces the oracle's
-solution (GPU)."""
+against include/nlf.h and links libnlf.so (CPU); its heat solver driver reproduces the oracle's solution
+and flux, and its elastic driver meets saint_venant.cpp's bars (GPU)."""
 import os
 import subprocess
 
@@ -14,16 +14,17 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 CPP = os.path.join(ROOT, "tests", "cpp")
 
 
-def _compile(out):
+def _compile(out, source="heat_q4.cpp"):
     cmd = ["g++", "-O1", "-std=c++17", "-Wall", "-Wextra", "-Werror", f"-I{ROOT}/include",
            f"-I{ROOT}/nonlocal_amd/include", "-isystem", "/opt/rocm/include", "-D__HIP_PLATFORM_AMD__",
-           os.path.join(CPP, "heat_q4.cpp"), "-o", out, f"-L{ROOT}/nonlocal_amd", "-lnlf", "-L/opt/rocm/lib",
+           os.path.join(CPP, source), "-o", out, f"-L{ROOT}/nonlocal_amd", "-lnlf", "-L/opt/rocm/lib",
            "-lamdhip64", f"-Wl,-rpath,{ROOT}/nonlocal_amd", "-Wl,-rpath,/opt/rocm/lib"]
     return subprocess.run(cmd, capture_output=True, text=True)
 
 
-def test_mirror_compiles_and_links(tmp_path):
-    res = _compile(str(tmp_path / "heat_q4"))
+@pytest.mark.parametrize("source", ["heat_q4.cpp", "saint_venant.cpp"])
+def test_mirror_compiles_and_links(tmp_path, source):
+    res = _compile(str(tmp_path / "prog"), source)
     assert res.returncode == 0, res.stderr
 
 
@@ -45,3 +46,21 @@ def test_cpp_heat_solver_matches_oracle(tmp_path):
     # the solver computed the flux (calc_flux, as the reference's solver does) of its temperature
     n_ref, _ = om.heat_flux(p.models, p.materials, T)
     assert np.abs(np.stack([fx, fy]) - n_ref).max() <= 1e-12 * np.abs(n_ref).max()
+
+
+@pytest.mark.gpu
+def test_cpp_saint_venant_known_answers():
+    """saint_venant.cpp's centre-line checks through the C++ mirror (equilibrium_equation with
+    tractions given as functions of the point, then calc_strain_and_stress)."""
+    exe = os.path.join(CPP, "saint_venant")
+    if not os.path.exists(exe):
+        res = _compile(exe, "saint_venant.cpp")
+        assert res.returncode == 0, res.stderr
+    print("[cpp] saint_venant", flush=True)
+    res = subprocess.run([exe], capture_output=True, text=True, timeout=120)
+    assert res.returncode == 0, res.stderr
+    print(res.stdout.strip(), flush=True)
+    fields = res.stdout.split()
+    ux = float(fields[fields.index("ux_centre") + 1])
+    sxx = float(fields[fields.index("sxx_centre") + 1])
+    assert ux <= 2e-14 and sxx <= 1.3e-9
