@@ -180,6 +180,17 @@ int nlf_matrix_make_split(nlf_matrix a);
  * device over the full storage: out[4] = {max |v32 - v64|, max |v64|, ||v32 - v64||_2, ||v64||_2}. */
 int nlf_matrix_compare_fp32(nlf_matrix a, double* out);
 
+/* ---- nonstationary heat conduction and source terms.
+ * nlf_assemble with physics 2 assembles the heat capacity matrix C_ij = rho c int N_i N_j
+ * (heat_capacity_matrix_2d.hpp:31-62; `materials` = double[G] of density * capacity, local theory).
+ * nlf_matrix_combine: K := alpha K + beta C on K's pattern, K's bound part times alpha, Dirichlet rows
+ * back to identity rows -- the tau K + C of nonstationary_heat_equation_solver_2d::compute
+ * (thermal/nonstationary_heat_equation_solver_2d.hpp:66-87); C's pattern must lie within K's.
+ * nlf_right_part: f_dev[dof*N] += int s N_i (integrate_right_part, base/right_part_2d.hpp:10-35) with
+ * the source values at the quadrature points, values_dev[(e*nq + q)*dof + d] (device arrays). */
+int nlf_matrix_combine(nlf_matrix K, double alpha, nlf_matrix C, double beta);
+int nlf_right_part(nlf_mesh m, int dof, const double* values_dev, double* f_dev);
+
 /* ---- heat flux: replaces heat_equation_solution_2d::calc_flux
  * (solvers/solver_2d/thermal/heat_equation_solution_2d.hpp:99-137) with gradient_in_qnodes and
  * qnodes_to_nodes (mesh/mesh_2d/mesh_2d_utils.hpp:39-63,82-101).  T_dev[N] node temperatures;

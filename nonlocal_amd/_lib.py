@@ -58,6 +58,8 @@ SIGNATURES = {
     "nlf_boundary_data": (_int, [_p, _int, C.POINTER(_abi.Boundary), _f64p, _f64p, _u8p]),
     "nlf_apply_first_kind": (_int, [_p, _p, _p]),
     "nlf_heat_flux": (_int, [_p, C.POINTER(_abi.Model), C.POINTER(_abi.ThermalMaterial), _p, _p, _p]),
+    "nlf_matrix_combine": (_int, [_p, _dbl, _p, _dbl]),
+    "nlf_right_part": (_int, [_p, _int, _p, _p]),
     "nlf_strain_stress": (_int, [_p, C.POINTER(_abi.Model), C.POINTER(_abi.ElasticMaterial), _p, _p, _p, _p]),
     "nlf_boundary_data_fn": (_int, [_p, _int, C.POINTER(_abi.Boundary), _p, _p, _f64p, _f64p, _u8p]),
     "nlf_spmv": (_int, [_p, _p, _p]),

@@ -241,15 +241,57 @@ class finite_element_matrix:
         self.ctx.check(load().nlf_apply_first_kind(self.h, _dev(xD), _dev(f)))
 
 
+PHYSICS_CAPACITY = 2      # nlf_assemble physics of the heat capacity matrix (materials: rho c per group)
+
+
 def _assemble(mesh: mesh_2d, physics, models, materials, is_inner, symmetric=True, part=NLF_PART_FULL):
     marr = _abi.array_of(_abi.Model, models)
-    mat_t = _abi.ThermalMaterial if physics == 0 else _abi.ElasticMaterial
-    matarr = _abi.array_of(mat_t, materials)
+    if physics == PHYSICS_CAPACITY:
+        matarr = (C.c_double * len(materials))(*[float(v) for v in materials])
+    else:
+        mat_t = _abi.ThermalMaterial if physics == 0 else _abi.ElasticMaterial
+        matarr = _abi.array_of(mat_t, materials)
     inner = np.ascontiguousarray(is_inner, dtype=np.uint8)
     h = C.c_void_p()
     mesh.ctx.check(load().nlf_assemble(mesh.h, physics, marr, C.cast(matarr, C.c_void_p), _ptr(inner, _u8p),
                                        int(symmetric), part, C.byref(h)))
-    return finite_element_matrix(h, mesh.ctx, 1 if physics == 0 else 2)
+    return finite_element_matrix(h, mesh.ctx, 2 if physics == 1 else 1)
+
+
+def combine(K: finite_element_matrix, alpha, Cm: finite_element_matrix, beta):
+    """K := alpha K + beta C on K's pattern (nlf_matrix_combine); C's pattern must lie within K's."""
+    K.ctx.check(load().nlf_matrix_combine(K.h, float(alpha), Cm.h, float(beta)))
+    return K
+
+
+def right_part(mesh: mesh_2d, dof, source, f: torch.Tensor):
+    """f += int s N_i (integrate_right_part, base/right_part_2d.hpp:10-35).  source(x, y) takes arrays of
+    quadrature-point coordinates and returns the values (dof 2: a pair of arrays)."""
+    xq = mesh.quadrature()[0].reshape(-1, 2)
+    v = source(xq[:, 0], xq[:, 1])
+    v = np.stack([np.broadcast_to(np.asarray(c, dtype=np.float64), xq.shape[:1]) for c in v], axis=1) if dof == 2 \
+        else np.broadcast_to(np.asarray(v, dtype=np.float64), xq.shape[:1])
+    vd = torch.from_numpy(np.array(v, dtype=np.float64).ravel()).to(mesh.ctx.device)
+    mesh.ctx.check(load().nlf_right_part(mesh.h, dof, _dev(vd), _dev(f)))
+    return f
+
+
+class heat_capacity_matrix_2d:
+    """Heat capacity matrix assembler (heat_capacity_matrix_2d<T, I, J>, thermal/heat_capacity_matrix_2d.hpp:
+    8-62): C_ij = rho c int N_i N_j, local theory, same separation into inner and bound parts."""
+
+    def __init__(self, mesh: mesh_2d):
+        self.mesh = mesh
+        self._matrix = None
+
+    def calc_matrix(self, capacities, is_inner):
+        """capacities: density * capacity per element group."""
+        models = [_abi.model(1.0, 0.0) for _ in capacities]
+        self._matrix = _assemble(self.mesh, PHYSICS_CAPACITY, models, capacities, is_inner, True, NLF_PART_LOCAL)
+        return self._matrix
+
+    def matrix(self) -> finite_element_matrix:
+        return self._matrix
 
 
 class conductivity_matrix_2d:
@@ -522,7 +564,7 @@ def _radii(models):
 
 
 def _solve_driver(mesh: mesh_2d, physics, models, materials, bcs, precond, tol, max_it, mode, weights=None,
-                  assembly32=False, coarse=None, values=None):
+                  assembly32=False, coarse=None, values=None, source=None):
     import time
     ctx = mesh.ctx
     dof = 1 if physics == 0 else 2
@@ -537,6 +579,8 @@ def _solve_driver(mesh: mesh_2d, physics, models, materials, bcs, precond, tol, 
     if physics == 0:
         K.add_convection(bcs)
     fd = torch.from_numpy(f).to(ctx.device)
+    if source is not None:
+        right_part(mesh, dof, source, fd)
     xDd = torch.from_numpy(xD).to(ctx.device)
     K.apply_first_kind(xDd, fd)
     ctx.synchronize()
@@ -590,7 +634,7 @@ def _solve_driver(mesh: mesh_2d, physics, models, materials, bcs, precond, tol, 
 
 def stationary_heat_equation_solver_2d(mesh: mesh_2d, models, materials, bcs, precond=NLF_PRECOND_ILLT, tol=1e-12,
                                        max_it=100000, mode=NLF_MODE_FP64, weights=None, assembly32=False,
-                                       coarse=None, values=None) -> Solution:
+                                       coarse=None, values=None, source=None) -> Solution:
     """K T = f with K = p1 K_L + (1 - p1) K_NL (solved directly; the reference's correction form
     T = T0 - K^-1 (K T0 - f) with T0 = 0 is the same system).  precond NLF_PRECOND_LEARNED runs
     flexible CG with the learned preconditioner (weights file `weights`, default the packaged one)
@@ -598,17 +642,85 @@ def stationary_heat_equation_solver_2d(mesh: mesh_2d, models, materials, bcs, pr
     inner FP32 operator is the rounded FP64 matrix, or with assembly32 an FP32 assembly.  coarse sets
     the learned preconditioner's aggregation coarse correction (about that many aggregates; 0 = one
     level; default 4096).  values: optional callable (x, y, group, dof) -> boundary value at a point
-    for the first-kind and flux groups."""
+    for the first-kind and flux groups.  source: optional heat source s(x, y) (arrays in, array out),
+    integrated as int s N_i."""
     return _solve_driver(mesh, 0, models, materials, bcs, precond, tol, max_it, mode, weights, assembly32, coarse,
-                         values)
+                         values, source)
 
 
 def equilibrium_equation(mesh: mesh_2d, models, materials, bcs, precond=None, tol=1e-12, max_it=100000,
-                         mode=NLF_MODE_FP64, assembly32=False, values=None) -> Solution:
+                         mode=NLF_MODE_FP64, assembly32=False, values=None, source=None) -> Solution:
     """Plane-stress static elasticity; ILLT of the local stiffness when nonlocal, identity otherwise.
     values: optional callable (x, y, group, dof) -> displacement or pressure of a boundary group at a
     point.  The solution's `solution` is a mechanical_solution_2d."""
     if precond is None:
         precond = NLF_PRECOND_ILLT if any(m.local_weight < 0.999 for m in models) else NLF_PRECOND_IDENTITY
     return _solve_driver(mesh, 1, models, materials, bcs, precond, tol, max_it, mode, assembly32=assembly32,
-                         values=values)
+                         values=values, source=source)
+
+
+class nonstationary_heat_equation_solver_2d:
+    """Implicit Euler heat conduction (nonstationary_heat_equation_solver_2d, thermal/
+    nonstationary_heat_equation_solver_2d.hpp:17-106):
+
+        (C + tau K) T_{n+1} = C T_n + tau (f_boundary + f_source),
+
+    K the conductivity matrix (with convection), C the heat capacity matrix, both on the device.
+    compute() assembles tau K + C once (nlf_matrix_combine); each calc_step() forms the right-hand
+    side (second-kind conditions and the source at this step, plus C T_n), applies the first-kind
+    conditions and runs CG from T_n.  As in the reference the CG has no preconditioner unless one is
+    asked for; precond=NLF_PRECOND_JACOBI takes the diagonal of tau K + C.
+    """
+
+    def __init__(self, mesh: mesh_2d, time_step: float):
+        self.mesh = mesh
+        self._tau = float(time_step)
+        self._T = None
+        self._K = self._C = self._cg = None
+
+    def time_step(self):
+        return self._tau
+
+    def temperature(self) -> torch.Tensor:
+        return self._T
+
+    def compute(self, models, materials, capacities, bcs, init_dist, values=None, precond=NLF_PRECOND_IDENTITY,
+                tol=1e-15, max_it=10000):
+        """init_dist: callable (x, y) on arrays of node coordinates, or an array of N temperatures."""
+        mesh = self.mesh
+        if any(m.local_weight < 0.999 for m in models):
+            find_neighbours(mesh, _radii(models))
+        _, _, inner = boundary_data(mesh, 1, bcs, values)
+        self._inner = inner
+        K = _assemble(mesh, 0, models, materials, inner)
+        K.add_convection(bcs)
+        Cm = heat_capacity_matrix_2d(mesh).calc_matrix(capacities, inner)
+        self._K = combine(K, self._tau, Cm, 1.0)
+        self._C = Cm
+        xy = mesh.mesh.xy
+        T0 = init_dist(xy[:, 0], xy[:, 1]) if callable(init_dist) else init_dist
+        T0 = np.array(np.broadcast_to(np.asarray(T0, dtype=np.float64), (mesh.nodes_count,)))
+        self._T = torch.from_numpy(T0).to(mesh.ctx.device)
+        self._cg = conjugate_gradient(self._K)
+        self._cg.set_tolerance(tol)
+        self._cg.set_max_interations(max_it)
+        if precond != NLF_PRECOND_IDENTITY:
+            self._cg.init_preconditioner(precond)
+
+    def calc_step(self, bcs, values=None, source=None):
+        """One step; bcs / values / source may change from step to step (the first-kind node set was
+        fixed by compute)."""
+        mesh = self.mesh
+        f, xD, inner = boundary_data(mesh, 1, bcs, values)
+        if not np.array_equal(inner, self._inner):
+            raise NLFError(_abi.NLF_ERR_ARGUMENT, "the first-kind boundary groups changed after compute()")
+        fd = torch.from_numpy(f).to(mesh.ctx.device)
+        if source is not None:
+            right_part(mesh, 1, source, fd)
+        rhs = fd.mul_(self._tau).add_(self._C.spmv(self._T))
+        self._K.apply_first_kind(torch.from_numpy(xD).to(mesh.ctx.device), rhs)
+        self._T = self._cg.solve(rhs, self._T)
+        return self._T
+
+    def stats(self):
+        return self._cg.stats

@@ -310,6 +310,8 @@ struct AssemblyArgs {
     i64 plane;           // pairs[i * plane + (eoff[eL] + t) * 4 + j], t = position in eL's neighbour list
     const i64* eoff;
     i64 node0;           // nodes [node0, node0 + grid) of this launch
+    int mass;            // heat capacity matrix: rho c int N_i N_j (groups' c[0] = rho c)
+    double qN[MAX_NN * MAX_NQ];
 };
 
 template<class T>
@@ -334,6 +336,17 @@ template<int DOF, int NN, int NQ, class T>
 __device__ void local_row(const AssemblyArgs& a, const GroupParams& g, i64 e, int i, T (&v)[NN][DOF * DOF]) {
     for (int j = 0; j < NN; ++j)
         for (int k = 0; k < DOF * DOF; ++k) v[j][k] = T(0);
+    if (DOF == 1 && a.mass) {
+        // heat_capacity_matrix_2d::integrate_basic_pair (heat_capacity_matrix_2d.hpp:31-37), times
+        // density * capacity (:55-56)
+        for (int q = 0; q < NQ; ++q) {
+            const T f = T(a.w[q] * a.qN[i * NQ + q]);
+            const T jq = T(a.jac[e * NQ + q]);
+            for (int j = 0; j < NN; ++j) v[j][0] += f * T(a.qN[j * NQ + q]) * jq;
+        }
+        for (int j = 0; j < NN; ++j) v[j][0] *= T(g.c[0]);
+        return;
+    }
     const T c0 = T(g.c[0]), c1 = T(g.c[1]), c2 = T(g.c[2]);
     for (int q = 0; q < NQ; ++q) {
         const T f = T(a.w[q] / a.jac[e * NQ + q]);
@@ -924,6 +937,12 @@ namespace {
 constexpr int PATTERN_MAX_WORDS = 12288;   // 48 KiB bitmap: column node ids within a span of 393k
 
 GroupParams make_group(const nlf_model& md, int physics, const void* materials, int g, bool only_local) {
+    if (physics == 2) {                                  // heat capacity: local theory, c[0] = rho c
+        GroupParams gp = model_group_params(md, true);
+        gp.p1 = 1.0;
+        gp.c[0] = static_cast<const double*>(materials)[g];
+        return gp;
+    }
     GroupParams gp = model_group_params(md, only_local);
     if (physics == 0) {
         const nlf_thermal_material& t = static_cast<const nlf_thermal_material*>(materials)[g];
@@ -981,6 +1000,8 @@ void assemble_values(nlf_matrix_s* mat, hipStream_t s) {
     aa.el_maxnode = m->el_maxnode.p;
     aa.el_dirichlet = el_dir.p; aa.inner = mat->inner.p; aa.xq = m->xq.p; aa.jac = m->jac.p; aa.D = m->D.p;
     for (int q = 0; q < m->fe.nq; ++q) aa.w[q] = m->fe.w[q];
+    for (int k = 0; k < m->fe.nn * m->fe.nq; ++k) aa.qN[k] = m->fe.qN[k];
+    aa.mass = mat->physics == 2;
     aa.outer = mat->outer.p; aa.cols = mat->cols.p; aa.diag = mat->diag.p; aa.b_outer = mat->b_outer.p;
     aa.b_cols = mat->b_cols.p; aa.val = mat->val.p; aa.b_val = mat->b_val.p; aa.val32 = mat->val32.p;
     aa.local_only = only_local;
@@ -1001,8 +1022,8 @@ void assemble_values(nlf_matrix_s* mat, hipStream_t s) {
     }
     if ((!any_nonlocal || only_local) && max_upper <= 64 && max_bound <= 64) {
         const unsigned nb = unsigned((m->N + 3) / 4);
-        if (mat->physics == 0 && m->nn == 4) hipLaunchKernelGGL((k_assemble_local<1, 4, 4, T>), dim3(nb), dim3(256), 0, s, aa);
-        else if (mat->physics == 0 && m->nn == 8) hipLaunchKernelGGL((k_assemble_local<1, 8, 9, T>), dim3(nb), dim3(256), 0, s, aa);
+        if (mat->physics != 1 && m->nn == 4) hipLaunchKernelGGL((k_assemble_local<1, 4, 4, T>), dim3(nb), dim3(256), 0, s, aa);
+        else if (mat->physics != 1 && m->nn == 8) hipLaunchKernelGGL((k_assemble_local<1, 8, 9, T>), dim3(nb), dim3(256), 0, s, aa);
         else if (mat->physics == 1 && m->nn == 4) hipLaunchKernelGGL((k_assemble_local<2, 4, 4, T>), dim3(nb), dim3(256), 0, s, aa);
         else hipLaunchKernelGGL((k_assemble_local<2, 8, 9, T>), dim3(nb), dim3(256), 0, s, aa);
     } else if (use_pairs) {
@@ -1083,8 +1104,8 @@ void assemble_values(nlf_matrix_s* mat, hipStream_t s) {
             NLF_HIP(hipStreamSynchronize(s));          // the next chunk rewrites the list and the offsets
             n0 = n1;
         }
-    } else if (mat->physics == 0 && m->nn == 4) launch_assemble<1, 4, 4, T>(aa, det, lds, m->N, s, m->ctx->assembly_threads);
-    else if (mat->physics == 0 && m->nn == 8) launch_assemble<1, 8, 9, T>(aa, det, lds, m->N, s, m->ctx->assembly_threads);
+    } else if (mat->physics != 1 && m->nn == 4) launch_assemble<1, 4, 4, T>(aa, det, lds, m->N, s, m->ctx->assembly_threads);
+    else if (mat->physics != 1 && m->nn == 8) launch_assemble<1, 8, 9, T>(aa, det, lds, m->N, s, m->ctx->assembly_threads);
     else if (mat->physics == 1 && m->nn == 4) launch_assemble<2, 4, 4, T>(aa, det, lds, m->N, s, m->ctx->assembly_threads);
     else launch_assemble<2, 8, 9, T>(aa, det, lds, m->N, s, m->ctx->assembly_threads);
     NLF_HIP(hipGetLastError());
@@ -1146,11 +1167,13 @@ int nlf_assemble(nlf_mesh m, int physics, const nlf_model* models, const void* m
     if (!m) return NLF_ERR_ARGUMENT;
     return guarded(m->ctx, [&] {
         NLF_CHECK(out && models && materials && is_inner, NLF_ERR_ARGUMENT, "null argument");
-        NLF_CHECK(physics == 0 || physics == 1, NLF_ERR_ARGUMENT, "physics must be 0 (thermal) or 1 (elastic)");
+        NLF_CHECK(physics >= 0 && physics <= 2, NLF_ERR_ARGUMENT,
+                  "physics must be 0 (thermal), 1 (elastic) or 2 (heat capacity)");
+        if (physics == 2) part = NLF_PART_LOCAL;            // heat_capacity_matrix_2d uses local theories
         NLF_CHECK(symmetric, NLF_ERR_UNSUPPORTED, "only symmetric assembly is implemented");
         const auto t0 = std::chrono::steady_clock::now();
         hipStream_t s = m->ctx->stream;
-        const int dof = physics == 0 ? 1 : 2;
+        const int dof = physics == 1 ? 2 : 1;
         const i64 rows = dof * m->N;
         const bool only_local = part == NLF_PART_LOCAL;
         std::vector<GroupParams> gps(m->G);
@@ -1442,6 +1465,7 @@ int nlf_matrix_compare_fp32(nlf_matrix a, double* out) {
 int nlf_matrix_assemble_fp32(nlf_matrix a) {
     if (!a) return NLF_ERR_ARGUMENT;
     return guarded(a->ctx, [&] {
+        NLF_CHECK(!a->combined, NLF_ERR_STATE, "the matrix was combined with another one (nlf_matrix_combine)");
         NLF_CHECK(a->val.p && !a->groups.empty(), NLF_ERR_STATE, "the matrix has no FP64 assembly to repeat");
         hipStream_t s = a->ctx->stream;
         assemble_values<float>(a, s);

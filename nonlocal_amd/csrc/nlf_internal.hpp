@@ -245,6 +245,7 @@ struct nlf_matrix_s {
     std::vector<double> extra_v;
     bool val32_assembled = false;        // val32 holds FP32-assembled values (not the rounded copy)
     bool split_ready = false;            // val32 + val_lo is the split of the current val
+    bool combined = false;               // values changed by nlf_matrix_combine (no FP32 value pass)
 };
 
 namespace nlf {

@@ -242,9 +242,11 @@ struct model_parameters {
 
 namespace thermal {
 
-struct parameter_2d {                                  // thermal_parameters_2d.hpp:14-17 (constant)
+struct parameter_2d {                                  // thermal_parameters_2d.hpp:14-17, 54-55 (constant)
     model_parameters model;
     nlf_thermal_material conductivity{NLF_CONDUCTIVITY_ISOTROPIC, 0, {1.0, 0.0, 0.0}};
+    double capacity = 1.0;
+    double density = 1.0;
 };
 
 using parameters_2d = std::unordered_map<std::string, parameter_2d>;   // by element group name
@@ -561,6 +563,117 @@ heat_equation_solution_2d<T, I> stationary_heat_equation_solver_2d(const std::sh
     solution.calc_flux();                              // as the reference's solver does (:117-119)
     return solution;
 }
+
+// integrate_right_part (base/right_part_2d.hpp:10-35): f += int s N_i, s evaluated on the host at the
+// quadrature points, integrated on the GPU.
+template<class T, class I>
+void integrate_right_part(device_vector& f, const mesh::mesh_2d<T, I>& mesh, const point_function& s) {
+    int64_t N = 0, E = 0;
+    int nn = 0, nq = 0;
+    nlf_check(nlf_mesh_info(mesh.handle(), &N, &E, &nn, &nq), mesh.ctx().handle());
+    std::vector<double> xq(size_t(2 * E * nq)), values(size_t(E * nq));
+    nlf_check(nlf_mesh_export_quadrature(mesh.handle(), xq.data(), nullptr, nullptr, nullptr), mesh.ctx().handle());
+    for (size_t k = 0; k < values.size(); ++k) values[k] = s({xq[2 * k], xq[2 * k + 1]});
+    device_vector dv(values);
+    nlf_check(nlf_right_part(mesh.handle(), 1, dv.data(), f.data()), mesh.ctx().handle());
+}
+
+// nonstationary_heat_equation_solver_2d (thermal/nonstationary_heat_equation_solver_2d.hpp:17-106):
+// implicit Euler (C + tau K) T_{n+1} = C T_n + tau (f_boundary + f_source).  compute() assembles
+// tau K + C on the GPU (nlf_matrix_combine); calc_step() forms the right-hand side, applies the
+// first-kind conditions and runs CG (no preconditioner, as in the reference) from T_n.
+template<class T = double, class I = int64_t, class J = int64_t>
+class nonstationary_heat_equation_solver_2d {
+    std::shared_ptr<mesh::mesh_2d<T, I>> mesh_;
+    conductivity_matrix_2d<T, I, J> conductivity_;
+    finite_element_matrix capacity_;
+    std::unique_ptr<slae::conjugate_gradient> solver_;
+    device_vector temperature_;
+    std::vector<uint8_t> inner_;
+    T time_step_ = 1;
+
+    void boundary(const boundaries_conditions_2d& conditions, std::vector<double>& f, std::vector<double>& xD,
+                  std::vector<uint8_t>& inner) const {
+        const size_t GB = mesh_->data().boundary_names.size();
+        std::vector<nlf_boundary> bcs(GB, boundary_condition{}.c);
+        std::vector<std::array<point_function, 2>> functions(GB);
+        for (size_t b = 0; b < GB; ++b) {
+            const auto it = conditions.find(mesh_->data().boundary_names[b]);
+            if (it == conditions.end()) continue;
+            bcs[b] = it->second.c;
+            functions[b][0] = it->second.value;
+        }
+        const size_t rows = mesh_->nodes_count();
+        f.assign(rows, 0.0); xD.assign(rows, 0.0); inner.assign(rows, 1);
+        detail::boundary_data(mesh_->handle(), mesh_->ctx().handle(), 1, bcs, functions, f, xD, inner);
+    }
+
+public:
+    nonstationary_heat_equation_solver_2d(const std::shared_ptr<mesh::mesh_2d<T, I>>& mesh, T time_step)
+        : mesh_(mesh), conductivity_(mesh), temperature_(mesh->nodes_count()), time_step_(time_step) {}
+
+    std::vector<T> temperature() const { return temperature_.host(); }
+    T time_step() const noexcept { return time_step_; }
+
+    template<class Init_Dist>
+    void compute(const parameters_2d& parameters, const boundaries_conditions_2d& conditions, const Init_Dist& init_dist) {
+        const size_t G = mesh_->groups_count();
+        mesh::influences<T> radii;
+        bool nonlocal = false;
+        std::vector<nlf_model> models(G);
+        std::vector<double> capacities(G);
+        for (size_t g = 0; g < G; ++g) {
+            const auto& p = parameters.at(mesh_->data().group_names[g]);
+            const auto& m = p.model.model;
+            capacities[g] = p.density * p.capacity;
+            models[g] = nlf_model{1.0, {0.0, 0.0}, {0.0, 0.0}, NLF_INFLUENCE_POLYNOMIAL, 2, 2.0, 1.0};
+            if (m.local_weight < 0.999) {
+                nonlocal = true;
+                radii[mesh_->data().group_names[g]] = mesh::influence<T>{{m.search_radius[0] > 0 ? m.search_radius[0] : m.radius[0],
+                                                                         m.search_radius[1] > 0 ? m.search_radius[1] : m.radius[1]}};
+            }
+        }
+        if (nonlocal) mesh::find_neighbours(*mesh_, radii);
+        std::vector<double> f, xD;
+        boundary(conditions, f, xD, inner_);
+        const std::vector<bool> is_inner(inner_.begin(), inner_.end());
+        conductivity_.compute(parameters, is_inner);
+        std::vector<nlf_boundary> bcs(mesh_->data().boundary_names.size(), boundary_condition{}.c);
+        for (size_t b = 0; b < bcs.size(); ++b) {
+            const auto it = conditions.find(mesh_->data().boundary_names[b]);
+            if (it != conditions.end()) bcs[b] = it->second.c;
+        }
+        nlf_check(nlf_matrix_add_convection(conductivity_.matrix().handle(), bcs.data()), mesh_->ctx().handle());
+        nlf_matrix hc = nullptr;                       // heat_capacity_matrix_2d::calc_matrix
+        nlf_check(nlf_assemble(mesh_->handle(), 2, models.data(), capacities.data(), inner_.data(), 1, NLF_PART_LOCAL, &hc),
+                  mesh_->ctx().handle());
+        capacity_ = finite_element_matrix(hc, mesh_->ctx().handle());
+        nlf_check(nlf_matrix_combine(conductivity_.matrix().handle(), time_step_, capacity_.handle(), 1.0),
+                  mesh_->ctx().handle());
+        std::vector<double> T0(mesh_->nodes_count());
+        for (size_t node = 0; node < T0.size(); ++node) T0[node] = init_dist(mesh_->data().nodes[node]);
+        temperature_.upload(T0);
+        solver_ = std::make_unique<slae::conjugate_gradient>(conductivity_.matrix());
+    }
+
+    void calc_step(const boundaries_conditions_2d& conditions, const point_function& right_part = {}) {
+        std::vector<double> f, xD;
+        std::vector<uint8_t> inner;
+        boundary(conditions, f, xD, inner);
+        if (inner != inner_) throw nlf_error(NLF_ERR_ARGUMENT, "the first-kind boundary groups changed after compute()");
+        device_vector df(f), dxD(xD), ct(mesh_->nodes_count());
+        if (right_part) integrate_right_part(df, *mesh_, right_part);
+        nlf_check(nlf_spmv(capacity_.handle(), temperature_.data(), ct.data()), mesh_->ctx().handle());
+        std::vector<double> rhs = df.host();
+        const std::vector<double> c = ct.host();
+        for (size_t i = 0; i < rhs.size(); ++i) rhs[i] = time_step_ * rhs[i] + c[i];
+        df.upload(rhs);
+        nlf_check(nlf_apply_first_kind(conductivity_.matrix().handle(), dxD.data(), df.data()), mesh_->ctx().handle());
+        solver_->solve(df, temperature_);              // x0 = T_n, result T_{n+1}
+    }
+
+    uintmax_t iterations() const { return solver_ ? solver_->iterations() : 0; }
+};
 
 }  // namespace thermal
 

@@ -464,6 +464,7 @@ struct Group {
     Influence inf;
     nlf_thermal_material th{};
     double hooke[3] = {0, 0, 0};   // {h11, h12, h66} isotropic plane stress
+    double capacity = 0.0;         // density * capacity (heat capacity matrix)
 };
 
 struct Assembly {
@@ -616,6 +617,15 @@ void thermal_nonlocal(const Mesh& m, const Group& g, i64 eL, i64 eNL, int iL, in
     out[0] = (1.0 - g.p1) * integral;
 }
 
+// heat_capacity_matrix_2d::integrate_basic_pair (heat_capacity_matrix_2d.hpp:31-37) times density *
+// capacity (:55-56)
+void capacity_local(const Mesh& m, const Group& g, i64 e, int i, int j, double* out) {
+    double integral = 0.0;
+    for (int q = 0; q < m.t.nq; ++q)
+        integral += m.t.w[q] * m.t.qN[i * m.t.nq + q] * m.t.qN[j * m.t.nq + q] * m.jac[e * m.t.nq + q];
+    out[0] = g.capacity * integral;
+}
+
 void elastic_local(const Mesh& m, const Group& g, i64 e, int i, int j, double* out) {
     double b[4] = {0, 0, 0, 0};
     const double h11 = g.hooke[0], h12 = g.hooke[1], h66 = g.hooke[2];
@@ -669,7 +679,9 @@ struct ValueRunner {                                 // integrator.hpp:36-91
                 if (p == NO) continue;
                 if (!have) {
                     have = true;
-                    if (local_theory) {
+                    if (physics == 2) {
+                        capacity_local(*m, g, eL, iL, jNL, block);
+                    } else if (local_theory) {
                         if (physics == 0) thermal_local(*m, g, eL, iL, jNL, block); else elastic_local(*m, g, eL, iL, jNL, block);
                     } else {
                         if (physics == 0) thermal_nonlocal(*m, g, eL, eNL, iL, jNL, block); else elastic_nonlocal(*m, g, eL, eNL, iL, jNL, block);
@@ -704,6 +716,7 @@ Group make_group(const nlf_model& md, int physics, const nlf_thermal_material* t
         nlf_model m2 = md; m2.radius[0] = r0; m2.radius[1] = r1;
         g.inf = make_influence(m2);
     }
+    if (physics == 2) return g;
     if (physics == 0) g.th = *th;
     else {
         const double E = el->young, nu = el->poisson;
@@ -716,7 +729,7 @@ Group make_group(const nlf_model& md, int physics, const nlf_thermal_material* t
 Assembly* assemble(const Mesh& m, int physics, const std::vector<Group>& groups, const std::vector<uint8_t>& is_inner,
                    bool symmetric, bool faithful_reset) {
     auto* a = new Assembly();
-    a->dof = physics == 0 ? 1 : 2;
+    a->dof = physics == 1 ? 2 : 1;
     a->symmetric = symmetric;
     const i64 rows = i64(a->dof) * m.N;
     for (int p = 0; p < 2; ++p) { a->A[p].rows = rows; a->A[p].cols = rows; a->A[p].outer.assign(rows + 1, 0); }
@@ -1247,12 +1260,15 @@ void* orc_assemble(void* h, int physics, const nlf_model* models, const void* ma
     try {
         const Mesh& m = *static_cast<Mesh*>(h);
         std::vector<Group> groups;
-        for (int g = 0; g < m.G(); ++g)
+        // physics 2: heat capacity matrix, local theories (heat_capacity_matrix_2d.hpp:50-53), materials = rho c
+        for (int g = 0; g < m.G(); ++g) {
             groups.push_back(make_group(models[g], physics,
                                         physics == 0 ? &static_cast<const nlf_thermal_material*>(materials)[g] : nullptr,
                                         physics == 1 ? &static_cast<const nlf_elastic_material*>(materials)[g] : nullptr,
-                                        part == NLF_PART_LOCAL));
-        const i64 rows = (physics == 0 ? 1 : 2) * m.N;
+                                        part == NLF_PART_LOCAL || physics == 2));
+            if (physics == 2) groups.back().capacity = static_cast<const double*>(materials)[g];
+        }
+        const i64 rows = (physics == 1 ? 2 : 1) * m.N;
         std::vector<uint8_t> inner(is_inner, is_inner + rows);
         return assemble(m, physics, groups, inner, symmetric != 0, faithful_reset != 0);
     } catch (const std::exception& e) { g_error = e.what(); return nullptr; }
@@ -1350,6 +1366,26 @@ int orc_heat_flux(void* h, const nlf_model* models, const nlf_thermal_material* 
         heat_flux(m, groups, T, qflux, nflux);
         return 0;
     } catch (const std::exception& e) { g_error = e.what(); return -1; }
+}
+
+// integrate_right_part (base/right_part_2d.hpp:10-35): for every node, over its elements in
+// ascending order, f[dof*node + d] += sum_q w_q N_i(q) |J_q| s_d(x_q); s from fn(x, y, group, d, user).
+void orc_right_part(void* h, int dof, ValueFn fn, void* user, double* f) {
+    const Mesh& m = *static_cast<Mesh*>(h);
+    const int nq = m.t.nq;
+    for (i64 node = 0; node < m.N; ++node)
+        for (i64 k = m.ne_off[node]; k < m.ne_off[node + 1]; ++k) {
+            const i64 e = m.ne_ids[k];
+            const int i = m.local(e, node);
+            for (int d = 0; d < dof; ++d) {
+                double integral = 0.0;
+                for (int q = 0; q < nq; ++q) {
+                    const double* x = m.xqp(e, q);
+                    integral += m.t.w[q] * m.t.qN[i * nq + q] * m.jac[e * nq + q] * fn(x[0], x[1], m.group_of[e], d, user);
+                }
+                f[dof * node + d] += integral;
+            }
+        }
 }
 
 int orc_strain_stress(void* h, const nlf_model* models, const nlf_elastic_material* materials, const double* u,

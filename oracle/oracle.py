@@ -77,6 +77,7 @@ def lib():
                                         _f64p, _f64p, _f64p]
         L.orc_boundary_fn.argtypes = [C.c_void_p, C.c_int, C.POINTER(_abi.Boundary), C.c_void_p, C.c_void_p, _f64p,
                                       _f64p, _u8p]
+        L.orc_right_part.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, _f64p]
         L.orc_heat_flux.restype = C.c_int
         L.orc_heat_flux.argtypes = [C.c_void_p, C.POINTER(_abi.Model), C.POINTER(_abi.ThermalMaterial), _f64p, _f64p,
                                     _f64p]
@@ -247,6 +248,14 @@ class OracleMesh:
             raise RuntimeError(lib().orc_last_error().decode())
         return OracleMatrix(h, inner.size)
 
+    def right_part(self, dof, source):
+        """int s N_i (integrate_right_part, base/right_part_2d.hpp:10-35); source(x, y) -> value (dof 2:
+        a pair), evaluated point by point."""
+        f = np.zeros(dof * self.mesh.n_nodes)
+        fn = _abi.boundary_callback(lambda x, y, g, d: source(x, y) if dof == 1 else source(x, y)[d])
+        lib().orc_right_part(self.h, dof, C.cast(fn, C.c_void_p), None, _ptr(f, _f64p))
+        return f
+
     def element_areas(self):
         a = np.zeros(self.mesh.n_elems)
         lib().orc_element_areas(self.h, _ptr(a, _f64p))
@@ -336,7 +345,7 @@ def cg(matrix: OracleMatrix, precond: OraclePrecond, b, x0=None, tol=1e-15, max_
 # Problem drivers (stationary_heat_equation_solver_2d.hpp:46-120, equilibrium_equation_2d.hpp:20-62)
 # ---------------------------------------------------------------------------------------------
 def thermal_problem(mesh, models, materials, bcs, precond=_abi.NLF_PRECOND_ILLT, tol=1e-12, max_it=100000,
-                    radii=None, faithful_reset=False, values=None):
+                    radii=None, faithful_reset=False, values=None, source=None):
     """Solve the stationary thermal problem; returns a dict with neighbours, matrices, f and T."""
     om = OracleMesh(mesh)
     if radii is None:
@@ -346,6 +355,8 @@ def thermal_problem(mesh, models, materials, bcs, precond=_abi.NLF_PRECOND_ILLT,
     f, xD, inner = om.boundary(bcs, dof=1, values=values)
     K = om.assemble(0, models, materials, inner, symmetric=True, faithful_reset=faithful_reset)
     om.convection(K, bcs, inner)
+    if source is not None:
+        f = f + om.right_part(1, source)
     f = first_kind_rhs(K, inner, xD, f)
     if precond == _abi.NLF_PRECOND_ILLT:
         KL = om.assemble(0, models, materials, inner, symmetric=True, part=_abi.NLF_PART_LOCAL)
@@ -387,3 +398,61 @@ def influence(model, x, y) -> float:
     xa = np.ascontiguousarray(x, dtype=np.float64)
     ya = np.ascontiguousarray(y, dtype=np.float64)
     return lib().orc_influence(C.byref(model), _ptr(xa, _f64p), _ptr(ya, _f64p))
+
+
+# ---------------------------------------------------------------------------------------------
+# Nonstationary heat conduction (nonstationary_heat_equation_solver_2d.hpp:17-106), restated with
+# the oracle's matrices and scipy's sparse direct solver.
+# ---------------------------------------------------------------------------------------------
+def capacity_matrix(om: OracleMesh, capacities, is_inner) -> OracleMatrix:
+    """heat_capacity_matrix_2d::calc_matrix (heat_capacity_matrix_2d.hpp:50-62): rho c int N_i N_j."""
+    models = [_abi.model(1.0, 0.0) for _ in capacities]
+    marr = _abi.array_of(_abi.Model, models)
+    caps = (C.c_double * len(capacities))(*[float(v) for v in capacities])
+    inner = np.ascontiguousarray(is_inner, dtype=np.uint8)
+    h = lib().orc_assemble(om.h, 2, marr, C.cast(caps, C.c_void_p), _ptr(inner, _u8p), 1, _abi.NLF_PART_LOCAL, 0)
+    if not h:
+        raise RuntimeError(lib().orc_last_error().decode())
+    return OracleMatrix(h, inner.size)
+
+
+def transient(mesh, models, materials, capacities, bcs, tau, T0, steps, values=None, source=None):
+    """compute() then `steps` calc_step()s: inner() := tau K + C with the Dirichlet diagonal set to 1
+    (:76-83), bound() := tau K_bound; each step rhs = tau (f_2nd + f_source) + C_inner T_n (:96-101),
+    f -= K_bound x_D and f[D] = x_D (:102), then the linear system (solved directly here; the
+    reference runs CG to 1e-15).  Returns the list of temperatures after each step."""
+    import scipy.sparse as sp
+    import scipy.sparse.linalg as spla
+    om = OracleMesh(mesh)
+    radii = [[m.search_radius[0] or m.radius[0], m.search_radius[1] or m.radius[1]]
+             if m.local_weight < 0.999 else [0.0, 0.0] for m in models]
+    om.find_neighbours(radii)
+    n = mesh.n_nodes
+    f2, xD, inner = om.boundary(bcs, dof=1, values=values)
+    K = om.assemble(0, models, materials, inner)
+    om.convection(K, bcs, inner)
+    Cm = capacity_matrix(om, capacities, inner)
+
+    def full(csr):
+        U = sp.csr_matrix((csr.val, csr.inner, csr.outer), shape=(n, n))
+        return (U + sp.triu(U, 1).T).tocsr()
+
+    A = (tau * full(K.export(0)) + full(Cm.export(0))).tolil()
+    dirichlet = np.flatnonzero(inner == 0)
+    for r in dirichlet:
+        A[r, r] = 1.0
+    A = A.tocsc()
+    kb = K.export(1)
+    Kb = tau * sp.csr_matrix((kb.val, kb.inner, kb.outer), shape=(n, n))
+    Cf = full(Cm.export(0))
+    fs = om.right_part(1, source) if source is not None else np.zeros(n)
+    lu = spla.splu(A)
+    T = np.array(T0, dtype=np.float64)
+    out = []
+    for _ in range(steps):
+        rhs = tau * (f2 + fs) + Cf @ T
+        rhs = rhs - Kb @ xD
+        rhs[dirichlet] = xD[dirichlet]
+        T = lu.solve(rhs)
+        out.append(T.copy())
+    return out

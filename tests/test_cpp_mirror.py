@@ -22,7 +22,7 @@ def _compile(out, source="heat_q4.cpp"):
     return subprocess.run(cmd, capture_output=True, text=True)
 
 
-@pytest.mark.parametrize("source", ["heat_q4.cpp", "saint_venant.cpp"])
+@pytest.mark.parametrize("source", ["heat_q4.cpp", "saint_venant.cpp", "transient_q4.cpp"])
 def test_mirror_compiles_and_links(tmp_path, source):
     res = _compile(str(tmp_path / "prog"), source)
     assert res.returncode == 0, res.stderr
@@ -64,3 +64,27 @@ def test_cpp_saint_venant_known_answers():
     ux = float(fields[fields.index("ux_centre") + 1])
     sxx = float(fields[fields.index("sxx_centre") + 1])
     assert ux <= 2e-14 and sxx <= 1.3e-9
+
+
+@pytest.mark.gpu
+def test_cpp_transient_matches_oracle(tmp_path):
+    """nonstationary_heat_equation_solver_2d of the C++ mirror: 5 steps against the oracle's implicit
+    Euler on the same problem."""
+    from nonlocal_amd import _abi as A
+    from nonlocal_amd import meshgen
+    from oracle import oracle as O
+    exe = os.path.join(CPP, "transient_q4")
+    if not os.path.exists(exe):
+        res = _compile(exe, "transient_q4.cpp")
+        assert res.returncode == 0, res.stderr
+    out = str(tmp_path / "T.bin")
+    print("[cpp] transient_q4 16 0.5 0.2 0.01 5", flush=True)
+    res = subprocess.run([exe, "16", "0.5", "0.2", "0.01", "5", out], capture_output=True, text=True, timeout=120)
+    assert res.returncode == 0, res.stderr
+    print(res.stdout.strip(), flush=True)
+    T = np.fromfile(out, dtype=np.float64)
+    mesh = meshgen.structured_q4(16)
+    bcs = [A.boundary(A.NLF_BC_FIRST_KIND, 0.0), A.boundary(A.NLF_BC_FIRST_KIND, 0.0), A.boundary(), A.boundary()]
+    ref = O.transient(mesh, [configs.bell(0.5, 0.2)], [A.thermal_material()], [2.0], bcs, 0.01,
+                      np.sin(np.pi * mesh.xy[:, 0]), 5, source=lambda x, y: 1.0 + y)
+    assert rel_l2(T, ref[-1]) < 1e-9

@@ -184,3 +184,39 @@ def test_ring_known_answers(composite):
     assert H.norm_error(T, temperature(x, y)) <= tol_T
     assert H.norm_error(nflux[0], fx) <= tol_q
     assert H.norm_error(nflux[1], fy) <= tol_q
+
+
+# ---- VTK / CSV writers -----------------------------------------------------------------------------
+def test_vtk_and_csv_round_trip(tmp_path):
+    """The VTK cells of a serendipity mesh read back through the SU2 node order give the elements
+    again; every written number parses back to the same double."""
+    from nonlocal_amd import io
+    mesh = su2.read(f"{H.MESH_DIR}/solid_ring.su2")
+    T = np.sin(mesh.xy[:, 0]) * 1e3
+    flux = (mesh.xy[:, 0] / 3, -mesh.xy[:, 1])
+    io.save_as_vtk(tmp_path / "r.vtk", mesh, scalars=[("temperature", T)], vectors=[("flux", flux)],
+                   tensors=[("stress", (T, 2 * T, 3 * T))])
+    lines = (tmp_path / "r.vtk").read_text().splitlines()
+    assert lines[0] == "# vtk DataFile Version 4.2" and lines[3] == "DATASET UNSTRUCTURED_GRID"
+    assert lines[4] == f"POINTS {mesh.n_nodes} double"
+    pts = np.array([[float(v) for v in l.split()] for l in lines[5:5 + mesh.n_nodes]])
+    assert np.array_equal(pts[:, :2], mesh.xy) and not pts[:, 2].any()
+    k = 5 + mesh.n_nodes
+    assert lines[k] == f"CELLS {mesh.n_elems} {mesh.n_elems * 9}"
+    cells = np.array([[int(v) for v in l.split()] for l in lines[k + 1:k + 1 + mesh.n_elems]])
+    assert (cells[:, 0] == 8).all()
+    # VTK's corners-then-midsides order is the SU2 file order: the reader's remap undoes it
+    local = np.empty_like(cells[:, 1:])
+    local[:, [0, 2, 4, 6, 1, 3, 5, 7]] = cells[:, 1:]
+    assert np.array_equal(local, mesh.elems)
+    assert lines[k + 1 + mesh.n_elems] == f"CELL_TYPES {mesh.n_elems}"
+    j = lines.index(f"POINT_DATA {mesh.n_nodes}")
+    assert lines[j + 1] == "SCALARS temperature double 1" and lines[j + 2] == "LOOKUP_TABLE default"
+    assert np.array_equal([float(v) for v in lines[j + 3:j + 3 + mesh.n_nodes]], T)
+    io.save_as_csv(tmp_path / "r.csv", mesh, [("temperature", T), ("flux_x", flux[0])])
+    rows = (tmp_path / "r.csv").read_text().splitlines()
+    assert rows[0] == "x,y,temperature,flux_x"
+    back = np.array([[float(v) for v in r.split(",")] for r in rows[1:]])
+    assert np.array_equal(back, np.column_stack([mesh.xy, T, flux[0]]))
+    with pytest.raises(ValueError, match="do not match"):
+        io.save_as_csv(tmp_path / "bad.csv", mesh, [("t", T[:-1])])
