@@ -211,29 +211,62 @@ __global__ __launch_bounds__(PATTERN_THREADS) void k_pattern(PatternArgs a) {
             for (int w = threadIdx.x; w < words; w += blockDim.x) a.saved[node * a.stride + w] = bitmap[w];
     }
     __syncthreads();
-    // each thread owns a contiguous chunk of words; columns are emitted in ascending order
+    // The set bits (column nodes, ascending) are dealt out evenly: thread t takes ranks
+    // [t k, t k + k) of the L set bits.  A prefix of the bits per thread-owned run of words locates
+    // the first rank; the walk then goes bit by bit.  (The node's columns cluster in a few runs of
+    // words, so dealing out words instead would leave most threads idle.)
+    __shared__ int tpre[PATTERN_THREADS + 1];
     const int per = (words + blockDim.x - 1) / blockDim.x;
     const int w0 = min(words, int(threadIdx.x) * per), w1 = min(words, w0 + per);
+    {
+        i64 c4[4] = {0, 0, 0, 0}, tot4[4];
+        for (int w = w0; w < w1; ++w) c4[0] += __popc(bitmap[w]);
+        block_scan4(c4, tot4, scan);
+        tpre[threadIdx.x] = int(c4[0]);
+        if (threadIdx.x == 0) tpre[blockDim.x] = int(tot4[0]);
+    }
+    __syncthreads();
+    const int L = tpre[blockDim.x];
+    const int k = (L + int(blockDim.x) - 1) / int(blockDim.x);
+    const int r0 = min(L, int(threadIdx.x) * k), r1 = min(L, r0 + k);
+    int wstart = 0;
+    unsigned int bstart = 0u;
+    if (r0 < r1) {
+        int u = 0, hi_u = int(blockDim.x) - 1;                 // last thread whose prefix is <= r0
+        while (u < hi_u) {
+            const int mid = (u + hi_u + 1) >> 1;
+            if (tpre[mid] <= r0) u = mid; else hi_u = mid - 1;
+        }
+        int rank = tpre[u];
+        wstart = u * per;
+        while (rank + __popc(bitmap[wstart]) <= r0) rank += __popc(bitmap[wstart++]);
+        bstart = bitmap[wstart];
+        for (int skip = r0 - rank; skip > 0; --skip) bstart &= bstart - 1;
+    }
+    auto walk = [&](auto&& f) {                              // column nodes of ranks [r0, r1), ascending
+        int w = wstart;
+        unsigned int bits = bstart;
+        for (int r = r0; r < r1; ++r) {
+            while (!bits) bits = bitmap[++w];
+            const int b = __ffs(bits) - 1;
+            bits &= bits - 1;
+            f(lo + i64(w) * 32 + b);
+        }
+    };
     const int D = a.dof;
     i64 rowv[2], rowinner[2];
     for (int d = 0; d < D; ++d) { rowv[d] = i64(D) * node + d; rowinner[d] = a.inner[rowv[d]]; }
     i64 cnt[4] = {0, 0, 0, 0};   // inner d0, inner d1, bound d0, bound d1
-    for (int w = w0; w < w1; ++w) {
-        unsigned int bits = bitmap[w];
-        while (bits) {
-            const int b = __ffs(bits) - 1;
-            bits &= bits - 1;
-            const i64 m = lo + i64(w) * 32 + b;
-            for (int dc = 0; dc < D; ++dc) {
-                const i64 c = i64(D) * m + dc;
-                const bool ci = a.inner[c];
-                for (int d = 0; d < D; ++d) {
-                    if (ci) { if (rowinner[d]) cnt[d]++; }
-                    else if (c != rowv[d]) cnt[2 + d]++;
-                }
+    walk([&](i64 m) {
+        for (int dc = 0; dc < D; ++dc) {
+            const i64 c = i64(D) * m + dc;
+            const bool ci = a.inner[c];
+            for (int d = 0; d < D; ++d) {
+                if (ci) { if (rowinner[d]) cnt[d]++; }
+                else if (c != rowv[d]) cnt[2 + d]++;
             }
         }
-    }
+    });
     if (PASS == 0) {
         i64 tot[4];
         block_scan4(cnt, tot, scan);
@@ -248,28 +281,22 @@ __global__ __launch_bounds__(PATTERN_THREADS) void k_pattern(PatternArgs a) {
     block_scan4(cnt, tot, scan);
     i64 pos[4];
     for (int d = 0; d < D; ++d) { pos[d] = a.outer[rowv[d]] + cnt[d]; pos[2 + d] = a.b_outer[rowv[d]] + cnt[2 + d]; }
-    for (int w = w0; w < w1; ++w) {
-        unsigned int bits = bitmap[w];
-        while (bits) {
-            const int b = __ffs(bits) - 1;
-            bits &= bits - 1;
-            const i64 m = lo + i64(w) * 32 + b;
-            for (int dc = 0; dc < D; ++dc) {
-                const i64 c = i64(D) * m + dc;
-                const bool ci = a.inner[c];
-                for (int d = 0; d < D; ++d) {
-                    if (ci) {
-                        if (rowinner[d]) {
-                            if (c == rowv[d]) a.diag[rowv[d]] = pos[d];
-                            a.cols[pos[d]++] = i32(c);
-                        }
-                    } else if (c != rowv[d]) {
-                        a.b_cols[pos[2 + d]++] = i32(c);
+    walk([&](i64 m) {
+        for (int dc = 0; dc < D; ++dc) {
+            const i64 c = i64(D) * m + dc;
+            const bool ci = a.inner[c];
+            for (int d = 0; d < D; ++d) {
+                if (ci) {
+                    if (rowinner[d]) {
+                        if (c == rowv[d]) a.diag[rowv[d]] = pos[d];
+                        a.cols[pos[d]++] = i32(c);
                     }
+                } else if (c != rowv[d]) {
+                    a.b_cols[pos[2 + d]++] = i32(c);
                 }
             }
         }
-    }
+    });
     if (threadIdx.x == 0)
         for (int d = 0; d < D; ++d)
             if (!rowinner[d]) { a.cols[a.outer[rowv[d]]] = i32(rowv[d]); a.diag[rowv[d]] = a.outer[rowv[d]]; }

@@ -182,7 +182,7 @@ struct nlf_ctx_s {
     int assembly_pairs = 1;              // NLF_OPT_ASSEMBLY_PAIRS
     int row_patterns = 1;                // NLF_OPT_ROW_PATTERNS
     int assembly_threads = 64;           // NLF_OPT_ASSEMBLY_THREADS
-    int pattern_threads = 256;           // NLF_OPT_PATTERN_THREADS
+    int pattern_threads = 128;           // NLF_OPT_PATTERN_THREADS
 };
 
 struct nlf_mesh_s {
