@@ -608,6 +608,7 @@ __global__ __launch_bounds__(THREADS) void k_assemble(AssemblyArgs a) {
     __shared__ T hl[2 * NQ];
     __shared__ int row_len[DOF], brow_len[DOF];
     __shared__ int s_off[2 * DOF + 1];
+    __shared__ int s_woff[DOF + 1];
     const i64 node = a.node0 + blockIdx.x;
     if (node >= a.N) return;
     i64 rows[DOF];
@@ -622,11 +623,24 @@ __global__ __launch_bounds__(THREADS) void k_assemble(AssemblyArgs a) {
         for (int d = 0; d < DOF; ++d) { s_off[d] = off; off += row_len[d]; }
         for (int d = 0; d < DOF; ++d) { s_off[DOF + d] = off; off += brow_len[d]; }
         s_off[2 * DOF] = off;
+        // words of the bitmap of each upper row over its column span [row, last column]
+        int woff = 0;
+        for (int d = 0; d < DOF; ++d) {
+            s_woff[d] = woff;
+            const i64 span = a.cols[a.outer[rows[d] + 1] - 1] - rows[d];
+            woff += int(span / 32) + 1;
+        }
+        s_woff[DOF] = woff;
     }
     __syncthreads();
     const int total = s_off[2 * DOF];
     T* acc = reinterpret_cast<T*>(smem);
     i32* cl = reinterpret_cast<i32*>(acc + total);
+    // position of an upper column c of row d: bit (c - row) of the row's bitmap, and the row
+    // position = the first position in its word (wpos) + the set bits below it in the word
+    unsigned int* bm = reinterpret_cast<unsigned int*>(cl + total);
+    uint16_t* wpos = reinterpret_cast<uint16_t*>(bm + s_woff[DOF]);
+    for (int w = threadIdx.x; w < s_woff[DOF]; w += blockDim.x) bm[w] = 0u;
     for (int d = 0; d < DOF; ++d) {
         for (int k = threadIdx.x; k < row_len[d]; k += blockDim.x) {
             cl[s_off[d] + k] = a.cols[a.diag[rows[d]] + k];
@@ -639,6 +653,13 @@ __global__ __launch_bounds__(THREADS) void k_assemble(AssemblyArgs a) {
     }
     bool rinner[DOF];
     for (int d = 0; d < DOF; ++d) rinner[d] = a.inner[rows[d]];
+    __syncthreads();
+    for (int d = 0; d < DOF; ++d)
+        for (int k = threadIdx.x; k < row_len[d]; k += blockDim.x) {
+            const int off = int(cl[s_off[d] + k] - rows[d]);
+            atomicOr(&bm[s_woff[d] + (off >> 5)], 1u << (off & 31));
+            if (k == 0 || ((cl[s_off[d] + k - 1] - rows[d]) >> 5) != (off >> 5)) wpos[s_woff[d] + (off >> 5)] = uint16_t(k);
+        }
     __syncthreads();
     for (i32 k = a.ne_off[node]; k < a.ne_off[node + 1]; ++k) {
         const i32 eL = a.ne_ids[k];
@@ -697,12 +718,16 @@ __global__ __launch_bounds__(THREADS) void k_assemble(AssemblyArgs a) {
                         for (int dc = 0; dc < DOF; ++dc) {
                             const i64 c = i64(DOF) * m + dc;
                             int p = -1;
-                            if (a.inner[c]) {          // INNER: inner row, c >= row (upper storage)
-                                if (rinner[d] && c >= rows[d]) {
-                                    const int q = lower_bound_i32(cl + s_off[d], row_len[d], i32(c));
-                                    if (q < row_len[d] && cl[s_off[d] + q] == i32(c)) p = s_off[d] + q;
-                                }
-                            } else if (c != rows[d]) { // BOUND
+                            // INNER: inner row, inner column c >= row -- exactly the columns in the
+                            // row's upper part, so the bitmap decides membership
+                            const i64 off = c - rows[d];
+                            if (rinner[d] && off >= 0 && off < i64(s_woff[d + 1] - s_woff[d]) * 32) {
+                                const unsigned int word = bm[s_woff[d] + int(off >> 5)];
+                                const unsigned int bit = 1u << (off & 31);
+                                if (word & bit) p = s_off[d] + wpos[s_woff[d] + int(off >> 5)] + __popc(word & (bit - 1u));
+                            }
+                            // BOUND: Dirichlet column c != row (the bound part holds only those)
+                            if (p < 0 && brow_len[d] > 0 && c != rows[d]) {
                                 const int q = lower_bound_i32(cl + s_off[DOF + d], brow_len[d], i32(c));
                                 if (q < brow_len[d] && cl[s_off[DOF + d] + q] == i32(c)) p = s_off[DOF + d] + q;
                             }
@@ -844,16 +869,21 @@ __global__ void k_el_dirichlet(i64 E, int nn, int dof, const i32* __restrict__ e
     out[e] = has;
 }
 
+// Per node: entries of its rows (upper + bound) -> mx[0]; bitmap words of its upper rows' column
+// spans -> mx[1] (the value pass's LDS).
 __global__ void k_row_max(i64 rows, const i64* __restrict__ outer, const i64* __restrict__ diag,
-                          const i64* __restrict__ b_outer, int dof, unsigned long long* __restrict__ mx) {
+                          const i64* __restrict__ b_outer, const i32* __restrict__ cols, int dof,
+                          unsigned long long* __restrict__ mx) {
     const i64 n = i64(blockIdx.x) * blockDim.x + threadIdx.x;
     if (n * dof >= rows) return;
-    i64 s = 0;
+    i64 s = 0, w = 0;
     for (int d = 0; d < dof; ++d) {
         const i64 r = n * dof + d;
         s += (outer[r + 1] - diag[r]) + (b_outer[r + 1] - b_outer[r]);
+        w += (cols[outer[r + 1] - 1] - r) / 32 + 1;
     }
     atomicMax(mx, static_cast<unsigned long long>(s));
+    atomicMax(mx + 1, static_cast<unsigned long long>(w));
 }
 
 // eoff[list[i]] = ofs[i]: chunk slots of the chunk's elements.
@@ -1005,13 +1035,13 @@ void assemble_values(nlf_matrix_s* mat, hipStream_t s) {
     DevBuf<uint8_t> el_dir{size_t(m->E)};
     hipLaunchKernelGGL(k_el_dirichlet, dim3(blocks_for(m->E, 256)), dim3(256), 0, s, m->E, m->nn, dof, m->el.p,
                        mat->inner.p, el_dir.p);
-    DevBuf<unsigned long long> d_max{1};
-    NLF_HIP(hipMemsetAsync(d_max.p, 0, sizeof(unsigned long long), s));
+    DevBuf<unsigned long long> d_max{2};
+    NLF_HIP(hipMemsetAsync(d_max.p, 0, 2 * sizeof(unsigned long long), s));
     hipLaunchKernelGGL(k_row_max, dim3(blocks_for(m->N, 256)), dim3(256), 0, s, rows, mat->outer.p, mat->diag.p,
-                       mat->b_outer.p, dof, d_max.p);
-    unsigned long long maxrow = 0;
-    download(&maxrow, d_max.p, 1, s);
-    const size_t lds = size_t(maxrow) * (sizeof(T) + sizeof(i32)) + 16;
+                       mat->b_outer.p, mat->cols.p, dof, d_max.p);
+    unsigned long long maxes[2] = {0, 0};
+    download(maxes, d_max.p, 2, s);
+    const size_t lds = size_t(maxes[0]) * (sizeof(T) + sizeof(i32)) + size_t(maxes[1]) * (sizeof(unsigned int) + 2) + 16;
     NLF_CHECK(lds <= 60 * 1024, NLF_ERR_UNSUPPORTED, "matrix rows too long for the LDS accumulator");
     if constexpr (F64) {
         mat->val.alloc(size_t(std::max<i64>(mat->nnz_full, 1)));
