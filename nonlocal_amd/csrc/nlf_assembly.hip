@@ -821,16 +821,25 @@ __global__ __launch_bounds__(256) void k_assemble_local(AssemblyArgs a) {
     }
 }
 
-// Lower triangle of the full storage: a(r, c) = a(c, r) for c < r (binary search of r in row c).
+// Lower triangle of the full storage: a(r, c) = a(c, r) for c < r.  Where rows r and c have
+// mirror-image patterns (the interior of a structured mesh: the j-th entry left of r's diagonal
+// and the j-th entry right of c's are each other's transposes) the guess diag[c] + j is checked
+// with one load; otherwise r is found in row c by binary search.
 template<class T>
 __global__ void k_mirror(i64 rows, const i64* __restrict__ outer, const i32* __restrict__ cols,
                          const i64* __restrict__ diag, T* __restrict__ val) {
     const i64 r = i64(blockIdx.x) * (blockDim.x / 64) + (threadIdx.x >> 6);
     if (r >= rows) return;
     const int lane = threadIdx.x & 63;
-    for (i64 p = outer[r] + lane; p < diag[r]; p += 64) {
+    const i64 dr = diag[r];
+    for (i64 p = outer[r] + lane; p < dr; p += 64) {
         const i32 c = cols[p];
         const i64 b = diag[c], e = outer[c + 1];
+        const i64 guess = b + (dr - p);
+        if (guess < e && cols[guess] == i32(r)) {
+            val[p] = val[guess];
+            continue;
+        }
         const int q = lower_bound_i32(cols + b, int(e - b), i32(r));
         if (q < e - b && cols[b + q] == i32(r)) val[p] = val[b + q];
     }

@@ -1067,31 +1067,67 @@ __global__ __launch_bounds__(CP_THREADS) void k_lf_compile(i64 n, const i64* __r
         return;
     }
     const float* L1 = w + P_LAYER0 + P_LAYER_SIZE;
+    // the basis contracted with the coefficients of every j in P1, once per node (the bitmap's
+    // LDS is free after the unions): cd[j][c] = sum_m cf[j][m] d1[m][c], ct[j][c] likewise with t1
+    static_assert(2 * CP_P1 * MP_H <= CP_WORDS, "contracted basis must fit the bitmap's LDS");
+    float (*cd)[MP_H] = reinterpret_cast<float (*)[MP_H]>(bm);
+    float (*ct)[MP_H] = cd + CP_P1;
+    for (int idx = threadIdx.x; idx < n1 * MP_H; idx += blockDim.x) {
+        const int j = idx / MP_H, cc = idx - j * MP_H;
+        float a = 0.0f, b = 0.0f;
+        for (int m = 0; m < 5; ++m) {
+            a += cf[j][m] * B->d1[m][cc];
+            b += cf[j][m] * B->t1[m][cc];
+        }
+        cd[j][cc] = a;
+        ct[j][cc] = b;
+    }
+    __syncthreads();
     // g1(w) for w in P2: lanes 0..31 channels of one w, lanes 32..63 the next w
     const int c = threadIdx.x & 31, half = threadIdx.x >> 5;
+    const float e0 = L1[P_WE + c * MP_F + 0], e1 = L1[P_WE + c * MP_F + 1], e2 = L1[P_WE + c * MP_F + 2],
+                e3 = L1[P_WE + c * MP_F + 3];
+    const float alpha_c = B->alpha[c];
+    float lam_c[MP_F];
+    for (int f = 0; f < MP_F; ++f) lam_c[f] = B->lam[c][f];
+    // row wn of each half is staged in LDS 32 entries at a time, one entry per lane, so its
+    // column, P1 index and features are loaded in parallel rather than one after another
+    __shared__ float4 rf[2][32];
+    __shared__ int rj[2][32];
     for (int i0 = 0; i0 < n2; i0 += 2) {
         const int i = i0 + half;
         float g = 0.0f;
+        i64 rb = 0, re = 0;
         if (i < n2) {
             const i32 wn = p2[i];
             const int j = find_sorted(p1, n1, wn);
-            if (j >= 0)
-                for (int m = 0; m < 5; ++m) g += cf[j][m] * B->d1[m][c];
-            const float e0 = L1[P_WE + c * MP_F + 0], e1 = L1[P_WE + c * MP_F + 1], e2 = L1[P_WE + c * MP_F + 2],
-                        e3 = L1[P_WE + c * MP_F + 3];
-            for (i64 p = outer[wn]; p < outer[wn + 1]; ++p) {
-                const int jv = find_sorted(p1, n1, cols[p]);
-                if (jv < 0) continue;
-                const float4 f = feat[p];
-                float t = 0.0f;
-                for (int m = 0; m < 5; ++m) t += cf[jv][m] * B->t1[m][c];
-                g += (e0 * f.x + e1 * f.y + e2 * f.z + e3 * f.w) * t;
+            if (j >= 0) g += cd[j][c];
+            rb = outer[wn];
+            re = outer[wn + 1];
+        }
+        // both halves step through their rows together (the block is one wave)
+        i64 len = re - rb, other = __shfl_xor(len, 32);
+        const i64 steps = len > other ? len : other;
+        for (i64 k0 = 0; k0 < steps; k0 += 32) {
+            if (k0 + c < len) {
+                const i64 p = rb + k0 + c;
+                rj[half][c] = find_sorted(p1, n1, cols[p]);
+                rf[half][c] = feat[p];
             }
+            __syncthreads();
+            const int cnt = int(len - k0 < 32 ? len - k0 : 32);
+            for (int k = 0; k < cnt; ++k) {
+                const int jv = rj[half][k];
+                if (jv < 0) continue;
+                const float4 f = rf[half][k];
+                g += (e0 * f.x + e1 * f.y + e2 * f.z + e3 * f.w) * ct[jv][c];
+            }
+            __syncthreads();
         }
         // alpha . g and Lambda^T g over the 32 channels of each half
-        float a = B->alpha[c] * g;
+        float a = alpha_c * g;
         float kf[MP_F];
-        for (int f = 0; f < MP_F; ++f) kf[f] = B->lam[c][f] * g;
+        for (int f = 0; f < MP_F; ++f) kf[f] = lam_c[f] * g;
         for (int o = 16; o > 0; o >>= 1) {
             a += __shfl_down(a, o, 32);
             for (int f = 0; f < MP_F; ++f) kf[f] += __shfl_down(kf[f], o, 32);
@@ -1107,11 +1143,20 @@ __global__ __launch_bounds__(CP_THREADS) void k_lf_compile(i64 n, const i64* __r
         const i32 x = p3[k];
         const int ix = find_sorted(p2, n2, x);
         float y = ix >= 0 ? aw[ix] : 0.0f;
-        for (i64 p = outer[x]; p < outer[x + 1]; ++p) {
-            const int iw = find_sorted(p2, n2, cols[p]);
-            if (iw < 0) continue;
-            const float4 f = feat[p];
-            y += f.x * kw[iw][0] + f.y * kw[iw][1] + f.z * kw[iw][2] + f.w * kw[iw][3];
+        // four entries of row x loaded together, then used in order
+        const i64 xe = outer[x + 1];
+        for (i64 p = outer[x]; p < xe; p += 4) {
+            i32 cc[4];
+            float4 ff[4];
+            for (int t = 0; t < 4; ++t)
+                if (p + t < xe) { cc[t] = cols[p + t]; ff[t] = feat[p + t]; }
+            for (int t = 0; t < 4; ++t) {
+                if (p + t >= xe) break;
+                const int iw = find_sorted(p2, n2, cc[t]);
+                if (iw < 0) continue;
+                const float4 f = ff[t];
+                y += f.x * kw[iw][0] + f.y * kw[iw][1] + f.z * kw[iw][2] + f.w * kw[iw][3];
+            }
         }
         m_cols[o + k] = x;
         yrow[o + k] = y;
