@@ -108,7 +108,7 @@ class mesh_2d:
         self.h = h
         self.influences = None
 
-    def __del__(self):
+    def __del__(self, _destroy=_destroy):      # bound now: module globals go first at exit
         if getattr(self, "h", None):
             _destroy("nlf_mesh_destroy", self.h)
             self.h = None
@@ -166,7 +166,7 @@ class finite_element_matrix:
         self.ctx = ctx
         self.dof = dof
 
-    def __del__(self):
+    def __del__(self, _destroy=_destroy):      # bound now: module globals go first at exit
         if getattr(self, "h", None):
             _destroy("nlf_matrix_destroy", self.h)
             self.h = None
@@ -336,7 +336,7 @@ class preconditioner:
         self.h = h
         self.kind = kind
 
-    def __del__(self):
+    def __del__(self, _destroy=_destroy):      # bound now: module globals go first at exit
         if getattr(self, "h", None):
             _destroy("nlf_precond_destroy", self.h)
             self.h = None

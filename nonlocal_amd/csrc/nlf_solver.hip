@@ -25,6 +25,7 @@
 namespace nlf {
 
 constexpr int SPMV_THREADS = 256;
+constexpr int SPMV_PAIRS_IN_FLIGHT = 4;  // nonzero pairs each lane of the pair kernel loads per step (2 or 4)
 constexpr int LEVEL_THREADS = 1024;
 
 struct CGState {
@@ -144,7 +145,7 @@ template<> struct Pair<float> { typedef float T __attribute__((ext_vector_type(2
 // XCD: each block takes a contiguous chunk of row groups, and the blocks that the hardware deals
 // round-robin to one XCD take adjacent chunks, so each XCD's L2 holds the x window of its own
 // eighth of the rows (without it, concurrently active rows span the whole matrix on every XCD).
-template<int LPR, class V, class X, class Acc, bool DOT, bool SPLIT = false, bool XCD = false>
+template<int LPR, class V, class X, class Acc, bool DOT, bool SPLIT = false, bool XCD = false, int PF = SPMV_PAIRS_IN_FLIGHT>
 __global__ __launch_bounds__(SPMV_THREADS) void k_spmv2(i64 rows, const i64* __restrict__ outer,
                                                         const int16_t* __restrict__ off, const i64* __restrict__ pstart,
                                                         const V* __restrict__ val, const int16_t* __restrict__ lo,
@@ -182,18 +183,27 @@ __global__ __launch_bounds__(SPMV_THREADS) void k_spmv2(i64 rows, const i64* __r
             if ((b & 1) && sub == 0 && b < e)
                 acc += value(val[b], SPLIT ? lo[b] : int16_t(0)) * Acc(x[r + o[b]]);
             i64 p = ((b + 1) & ~i64(1)) + 2 * sub;
-            for (; p + 2 * LPR + 1 < e; p += 4 * LPR) {
-                const V2 v0 = *reinterpret_cast<const V2*>(val + p);
-                const V2 v1 = *reinterpret_cast<const V2*>(val + p + 2 * LPR);
-                const int c0 = *reinterpret_cast<const int*>(o + p);
-                const int c1 = *reinterpret_cast<const int*>(o + p + 2 * LPR);
-                const int l0 = lo_pair(p), l1 = lo_pair(p + 2 * LPR);
-                const X x0 = x[r + int16_t(c0 & 0xffff)], x1 = x[r + int16_t(c0 >> 16)];
-                const X x2 = x[r + int16_t(c1 & 0xffff)], x3 = x[r + int16_t(c1 >> 16)];
-                acc += value(v0.x, int16_t(l0 & 0xffff)) * Acc(x0);
-                acc += value(v0.y, int16_t(l0 >> 16)) * Acc(x1);
-                acc += value(v1.x, int16_t(l1 & 0xffff)) * Acc(x2);
-                acc += value(v1.y, int16_t(l1 >> 16)) * Acc(x3);
+            // PF pairs in flight per lane; a lane's pairs are summed in order whatever PF is
+            for (; p + 2 * LPR * (PF - 1) + 1 < e; p += 2 * LPR * PF) {
+                V2 v[PF];
+                int cw[PF], lw[PF];
+#pragma unroll
+                for (int u = 0; u < PF; ++u) {
+                    v[u] = *reinterpret_cast<const V2*>(val + p + 2 * LPR * u);
+                    cw[u] = *reinterpret_cast<const int*>(o + p + 2 * LPR * u);
+                    lw[u] = lo_pair(p + 2 * LPR * u);
+                }
+                X xs[2 * PF];
+#pragma unroll
+                for (int u = 0; u < PF; ++u) {
+                    xs[2 * u] = x[r + int16_t(cw[u] & 0xffff)];
+                    xs[2 * u + 1] = x[r + int16_t(cw[u] >> 16)];
+                }
+#pragma unroll
+                for (int u = 0; u < PF; ++u) {
+                    acc += value(v[u].x, int16_t(lw[u] & 0xffff)) * Acc(xs[2 * u]);
+                    acc += value(v[u].y, int16_t(lw[u] >> 16)) * Acc(xs[2 * u + 1]);
+                }
             }
             for (; p + 1 < e; p += 2 * LPR) {
                 const V2 v0 = *reinterpret_cast<const V2*>(val + p);
@@ -990,7 +1000,7 @@ unsigned spmv_blocks(int lpr, i64 rows) {
 template<class V, class X, class Acc, bool DOT>
 void launch_spmv(int lpr, i64 rows, const i64* outer, SpmvCols cs, const V* val, const X* x, X* y,
                  double* partial, const CGState* st, hipStream_t s, bool xcd = false, const int16_t* lo = nullptr,
-                 bool pairs = true) {
+                 bool pairs = true, int pf = SPMV_PAIRS_IN_FLIGHT) {
     const i32* cols = cs.cols;
     const int16_t* off16 = cs.off;
     const i64* pstart = cs.pstart;
@@ -1030,8 +1040,12 @@ void launch_spmv(int lpr, i64 rows, const i64* outer, SpmvCols cs, const V* val,
         } else {
 #define NLF_SPMV2_CASE(L)                                                                                          \
     case L:                                                                                                        \
-        hipLaunchKernelGGL((k_spmv2<L, V, X, Acc, DOT>), dim3(blocks), dim3(SPMV_THREADS), 0, s, rows, outer, off16,       \
-                           pstart, val, static_cast<const int16_t*>(nullptr), x, y, partial, st);                 \
+        if (pf == 4)                                                                                               \
+            hipLaunchKernelGGL((k_spmv2<L, V, X, Acc, DOT, false, false, 4>), dim3(blocks), dim3(SPMV_THREADS), 0, s, rows, \
+                               outer, off16, pstart, val, static_cast<const int16_t*>(nullptr), x, y, partial, st);   \
+        else                                                                                                       \
+            hipLaunchKernelGGL((k_spmv2<L, V, X, Acc, DOT, false, false, 2>), dim3(blocks), dim3(SPMV_THREADS), 0, s, rows, \
+                               outer, off16, pstart, val, static_cast<const int16_t*>(nullptr), x, y, partial, st);   \
         return;
             switch (lpr) {
                 NLF_SPMV2_CASE(4)
@@ -1385,7 +1399,8 @@ int nlf_spmv_fp32(nlf_matrix a, const float* x, float* y) {
 // the XCD-contiguous workgroup numbering, bit 9 the 32-bit column indices instead of the offsets,
 // bit 12 the per-nonzero 16-bit offsets instead of the row-pattern dictionary, bit 13 the split
 // values (FP32 + 16-bit remainder, FP64 vectors; fp32 = 0), bit 14 the one-nonzero-per-load kernel
-// instead of the pair kernel; bits 10 / 11 (FP64 only, LPR 32 or 64, per-nonzero offsets) select 8
+// instead of the pair kernel, bit 15 two nonzero pairs in flight per lane in the pair kernel instead of
+// four; bits 10 / 11 (FP64 only, LPR 32 or 64, per-nonzero offsets) select 8
 // nonzeros in flight per lane / non-temporal loads of the matrix stream.
 int nlf_spmv_lpr(nlf_matrix a, int lpr, int fp32, const void* x, void* y) {
     if (!a) return NLF_ERR_ARGUMENT;
@@ -1404,6 +1419,7 @@ int nlf_spmv_lpr(nlf_matrix a, int lpr, int fp32, const void* x, void* y) {
         const bool xcd = (lpr & 0x100) != 0;
         const bool split = (lpr & 0x2000) != 0;
         const bool pairs = (lpr & 0x4000) == 0;
+        const int pf = (lpr & 0x8000) ? 2 : SPMV_PAIRS_IN_FLIGHT;
         SpmvCols cs = spmv_cols(a);
         if (lpr & 0x1000) cs = SpmvCols{a->cols.p, a->off16.p, nullptr};
         if (lpr & 0x200) cs = SpmvCols{a->cols.p, nullptr, nullptr};
@@ -1417,11 +1433,11 @@ int nlf_spmv_lpr(nlf_matrix a, int lpr, int fp32, const void* x, void* y) {
             NLF_CHECK(a->val32.p, NLF_ERR_STATE, "no FP32 values");
             launch_spmv<float, float, float, false>(lpr, a->rows, a->outer.p, cs, a->val32.p, static_cast<const float*>(x),
                                                     static_cast<float*>(y), nullptr, nullptr, a->ctx->stream, xcd, nullptr,
-                                                    pairs);
+                                                    pairs, pf);
         } else {
             launch_spmv<double, double, double, false>(lpr, a->rows, a->outer.p, cs, a->val.p, static_cast<const double*>(x),
                                                        static_cast<double*>(y), nullptr, nullptr, a->ctx->stream, xcd, nullptr,
-                                                       pairs);
+                                                       pairs, pf);
         }
         NLF_HIP(hipGetLastError());
     });
