@@ -44,6 +44,19 @@ GroupParams model_group_params(const nlf_model& md, bool only_local) {
     return gp;
 }
 
+int influence_class(const std::vector<GroupParams>& gp) {
+    int poly = -1;
+    for (const GroupParams& g : gp) {
+        if (!g.nonlocal) continue;
+        const InfluenceParams& f = g.inf;
+        int cls = 0;
+        if (f.kind == NLF_INFLUENCE_POLYNOMIAL && (f.fast || (f.p_int && f.p_i == 1 && f.q_int && f.q_i <= 2)))
+            cls = f.fast ? 1 : f.q_i;
+        poly = poly < 0 ? cls : (poly == cls ? poly : 0);
+    }
+    return poly;
+}
+
 InfluenceParams make_influence(const nlf_model& md) {
     InfluenceParams f;
     f.kind = md.influence;
@@ -500,6 +513,10 @@ __global__ void k_pair_prep(i64 E, const double* __restrict__ D, Weights4 w4, co
     HN[e * 32 + 16 + i * 4 + q] = T(w * cy);
 }
 
+// A wave first reads 64 neighbour ids and their "needed" flags in one round, then takes the needed
+// ones eight at a time (two MFMA sets of four blocks), so each round of gathers serves eight pairs.
+// A pair's block does not depend on which set or block slot it lands in.  POLY: influence_poly.
+template<int POLY>
 __global__ __launch_bounds__(256) void k_pairs_mfma(PairArgs<double> a) {
     const int l = threadIdx.x & 63;
     const int g = (l >> 2) & 3, c = l & 3, h = l >> 4;
@@ -515,26 +532,44 @@ __global__ __launch_bounds__(256) void k_pairs_mfma(PairArgs<double> a) {
         const i64 nb_b = a.nb_off[eL], cnt = a.nb_off[eL + 1] - nb_b;
         const i32 emin = a.el_min[eL];
         const double s = 1.0 - gp.p1;
-        for (i64 base = 0; base < cnt; base += 4) {
-            const i64 t = base + g;
-            bool need = false;
-            i64 eNL = eL;
-            if (t < cnt) {
-                eNL = a.nb_ids[nb_b + t];
+        for (i64 b64 = 0; b64 < cnt; b64 += 64) {
+            i32 id = 0;
+            bool nd = false;
+            if (b64 + l < cnt) {
+                id = a.nb_ids[nb_b + b64 + l];
                 // the pair reaches the upper triangle of some row of eL, or a Dirichlet column
-                need = a.el_max[eNL] >= emin || a.el_dir[eNL];
+                nd = a.el_max[id] >= emin || a.el_dir[id];
             }
-            double phi = 0.0, hn0 = 0.0, hn1 = 0.0;
-            if (need) {
-                phi = influence(gp.inf, xlx, xly, a.xq[2 * (eNL * 4 + c)], a.xq[2 * (eNL * 4 + c) + 1]);
-                hn0 = a.HN[eNL * 32 + c * 4 + h];                  // HN_0[j = c][qn = h]
-                hn1 = a.HN[eNL * 32 + 16 + c * 4 + h];
+            unsigned long long mask = __ballot(nd);
+            while (mask) {
+                int pos[2] = {-1, -1};                            // this block's neighbour in each set
+                for (int k = 0; k < 8; ++k) {
+                    const int p = mask ? int(__builtin_ctzll(mask)) : -1;
+                    mask &= mask - 1;
+                    if (k == g) pos[0] = p;
+                    if (k == 4 + g) pos[1] = p;
+                }
+                double phi[2], hn0[2], hn1[2];
+#pragma unroll
+                for (int u = 0; u < 2; ++u) {
+                    const i64 eNL = __shfl(id, pos[u] < 0 ? 0 : pos[u]);
+                    phi[u] = hn0[u] = hn1[u] = 0.0;
+                    if (pos[u] >= 0) {
+                        phi[u] = influence_poly<POLY>(gp.inf, xlx, xly, a.xq[2 * (eNL * 4 + c)],
+                                                      a.xq[2 * (eNL * 4 + c) + 1]);
+                        hn0[u] = a.HN[eNL * 32 + c * 4 + h];      // HN_0[j = c][qn = h]
+                        hn1[u] = a.HN[eNL * 32 + 16 + c * 4 + h];
+                    }
+                }
+#pragma unroll
+                for (int u = 0; u < 2; ++u) {
+                    const double v0 = __builtin_amdgcn_mfma_f64_4x4x4f64(phi[u], hl0, 0.0, 0, 0, 0);
+                    const double v1 = __builtin_amdgcn_mfma_f64_4x4x4f64(phi[u], hl1, 0.0, 0, 0, 0);
+                    double k = __builtin_amdgcn_mfma_f64_4x4x4f64(v0, hn0[u], 0.0, 0, 0, 0);
+                    k = __builtin_amdgcn_mfma_f64_4x4x4f64(v1, hn1[u], k, 0, 0, 0);
+                    if (pos[u] >= 0) a.pairs[i64(h) * a.plane + (slot + b64 + pos[u]) * 4 + c] = s * k;  // K[i = h][j = c]
+                }
             }
-            const double v0 = __builtin_amdgcn_mfma_f64_4x4x4f64(phi, hl0, 0.0, 0, 0, 0);
-            const double v1 = __builtin_amdgcn_mfma_f64_4x4x4f64(phi, hl1, 0.0, 0, 0, 0);
-            double k = __builtin_amdgcn_mfma_f64_4x4x4f64(v0, hn0, 0.0, 0, 0, 0);
-            k = __builtin_amdgcn_mfma_f64_4x4x4f64(v1, hn1, k, 0, 0, 0);
-            if (need) a.pairs[i64(h) * a.plane + (slot + t) * 4 + c] = s * k;     // K[i = h][j = c]
         }
     }
 }
@@ -600,7 +635,7 @@ __global__ __launch_bounds__(256) void k_pairs_mfma32(PairArgs<float> a) {
 
 // T = double: the FP64 value pass (inner and bound parts).  T = float: the FP32 value pass with
 // FP32 accumulation into val32 (inner part only).
-template<int DOF, int NN, int NQ, bool DETERMINISTIC, class T, int THREADS = ASM_THREADS>
+template<int DOF, int NN, int NQ, bool DETERMINISTIC, class T, int THREADS = ASM_THREADS, bool PAIRS = false>
 __global__ __launch_bounds__(THREADS) void k_assemble(AssemblyArgs a) {
     constexpr bool F64 = sizeof(T) == sizeof(double);
     extern __shared__ unsigned char smem[];
@@ -677,43 +712,59 @@ __global__ __launch_bounds__(THREADS) void k_assemble(AssemblyArgs a) {
         __syncthreads();
         const i64 nb_b = nonlocal ? a.nb_off[eL] : 0;
         const i64 cnt = nonlocal ? a.nb_off[eL + 1] - nb_b : 1;
-        for (i64 base = 0; base < cnt; base += blockDim.x) {
-            const i64 t = base + threadIdx.x;
-            T v[NN][DOF * DOF];
-            int pos[NN][DOF * DOF];
-            bool active = false;
-            i64 eNL = -1;
-            if (t < cnt) {
-                eNL = nonlocal ? a.nb_ids[nb_b + t] : eL;
+        // ITEMS neighbours per thread and step: item u covers [base + u blockDim, base + (u + 1) blockDim),
+        // their loads are issued before either is used, and they are added item after item, so the
+        // order of the additions is that of ITEMS consecutive single-item steps
+        // (PAIRS: the nonlocal rows come from the pair blocks of k_pairs_mfma, thermal Q4 only)
+        constexpr int ITEMS = PAIRS ? 2 : 1;
+        const T* pairs = nullptr;
+        if constexpr (F64) pairs = a.pairs; else pairs = a.pairs32;
+        const bool from_pairs = PAIRS && nonlocal;
+        const i64 eoff = from_pairs ? a.eoff[eL] : 0;
+        for (i64 base = 0; base < cnt; base += ITEMS * i64(blockDim.x)) {
+            T v[ITEMS][NN][DOF * DOF];
+            int pos[ITEMS][NN][DOF * DOF];
+            bool active[ITEMS];
+            i64 eNL[ITEMS];
+            i32 mj[ITEMS][NN];
+#pragma unroll
+            for (int u = 0; u < ITEMS; ++u) {
+                const i64 t = base + u * i64(blockDim.x) + threadIdx.x;
+                eNL[u] = -1;
+                if (t < cnt) eNL[u] = nonlocal ? a.nb_ids[nb_b + t] : eL;
+            }
+#pragma unroll
+            for (int u = 0; u < ITEMS; ++u) {
                 // pairs whose columns all lie left of the node's rows and have no Dirichlet column
                 // contribute nothing to the upper triangle nor to the bound part
-                active = a.el_maxnode[eNL] >= node || a.el_dirichlet[eNL];
+                active[u] = eNL[u] >= 0 && (a.el_maxnode[eNL[u]] >= node || a.el_dirichlet[eNL[u]]);
             }
-            if (active) {
+#pragma unroll
+            for (int u = 0; u < ITEMS; ++u) {
+                if (!active[u]) continue;
+                const i64 t = base + u * i64(blockDim.x) + threadIdx.x;
+                if (from_pairs) {
+                    const T* pr = pairs + i64(iL) * a.plane + (eoff + t) * 4;
+                    for (int j = 0; j < NN; ++j) v[u][j][0] = pr[j];
+                }
+                for (int j = 0; j < NN; ++j) mj[u][j] = a.el[eNL[u] * NN + j];
+            }
+#pragma unroll
+            for (int u = 0; u < ITEMS; ++u) {
+                if (!active[u]) continue;
                 if (nonlocal) {
-                    const T* pairs = nullptr;
-                    if constexpr (F64) pairs = a.pairs; else pairs = a.pairs32;
-                    if constexpr (DOF == 1 && NN == 4 && NQ == 4) {
-                        if (pairs) {
-                            const T* pr = pairs + i64(iL) * a.plane + (a.eoff[eL] + t) * 4;
-                            for (int j = 0; j < NN; ++j) v[j][0] = pr[j];
-                        } else {
-                            nonlocal_row<DOF, NN, NQ, T>(a, g, xl, hl, eNL, v);
-                        }
-                    } else {
-                        nonlocal_row<DOF, NN, NQ, T>(a, g, xl, hl, eNL, v);
-                    }
-                    if (eNL == eL) {
+                    if constexpr (!PAIRS) nonlocal_row<DOF, NN, NQ, T>(a, g, xl, hl, eNL[u], v[u]);
+                    if (eNL[u] == eL) {
                         T lv[NN][DOF * DOF];
                         local_row<DOF, NN, NQ, T>(a, g, eL, iL, lv);
                         for (int j = 0; j < NN; ++j)
-                            for (int q = 0; q < DOF * DOF; ++q) v[j][q] += lv[j][q];
+                            for (int q = 0; q < DOF * DOF; ++q) v[u][j][q] += lv[j][q];
                     }
                 } else {
-                    local_row<DOF, NN, NQ, T>(a, g, eL, iL, v);
+                    local_row<DOF, NN, NQ, T>(a, g, eL, iL, v[u]);
                 }
                 for (int j = 0; j < NN; ++j) {
-                    const i64 m = a.el[eNL * NN + j];
+                    const i64 m = mj[u][j];
                     for (int d = 0; d < DOF; ++d)
                         for (int dc = 0; dc < DOF; ++dc) {
                             const i64 c = i64(DOF) * m + dc;
@@ -731,21 +782,22 @@ __global__ __launch_bounds__(THREADS) void k_assemble(AssemblyArgs a) {
                                 const int q = lower_bound_i32(cl + s_off[DOF + d], brow_len[d], i32(c));
                                 if (q < brow_len[d] && cl[s_off[DOF + d] + q] == i32(c)) p = s_off[DOF + d] + q;
                             }
-                            pos[j][d * DOF + dc] = p;
+                            pos[u][j][d * DOF + dc] = p;
                         }
                 }
             }
-            for (int j = 0; j < NN; ++j) {
-                if (DETERMINISTIC) __syncthreads();
-                if (active)
-                    for (int q = 0; q < DOF * DOF; ++q) {
-                        const int p = pos[j][q];
-                        if (p >= 0) {
-                            if (DETERMINISTIC) acc[p] += v[j][q];
-                            else atomicAdd(&acc[p], v[j][q]);
+            for (int u = 0; u < ITEMS; ++u)
+                for (int j = 0; j < NN; ++j) {
+                    if (DETERMINISTIC) __syncthreads();
+                    if (active[u])
+                        for (int q = 0; q < DOF * DOF; ++q) {
+                            const int p = pos[u][j][q];
+                            if (p >= 0) {
+                                if (DETERMINISTIC) acc[p] += v[u][j][q];
+                                else atomicAdd(&acc[p], v[u][j][q]);
+                            }
                         }
-                    }
-            }
+                }
         }
         __syncthreads();
     }
@@ -975,23 +1027,35 @@ __global__ __launch_bounds__(256) void k_compare32(i64 n, const double* __restri
 }
 
 // nodes [a.node0, a.node0 + count); threads per workgroup 64, 128, 256 or 512
+template<int DOF, int NN, int NQ, class T, bool PAIRS>
+void launch_assemble_kernel(const AssemblyArgs& a, bool det, size_t lds, dim3 g, hipStream_t s, int threads) {
+    if (threads == 512) {
+        if (det) hipLaunchKernelGGL((k_assemble<DOF, NN, NQ, true, T, 512, PAIRS>), g, dim3(512), lds, s, a);
+        else hipLaunchKernelGGL((k_assemble<DOF, NN, NQ, false, T, 512, PAIRS>), g, dim3(512), lds, s, a);
+    } else if (threads == 128) {
+        if (det) hipLaunchKernelGGL((k_assemble<DOF, NN, NQ, true, T, 128, PAIRS>), g, dim3(128), lds, s, a);
+        else hipLaunchKernelGGL((k_assemble<DOF, NN, NQ, false, T, 128, PAIRS>), g, dim3(128), lds, s, a);
+    } else if (threads == 64) {
+        if (det) hipLaunchKernelGGL((k_assemble<DOF, NN, NQ, true, T, 64, PAIRS>), g, dim3(64), lds, s, a);
+        else hipLaunchKernelGGL((k_assemble<DOF, NN, NQ, false, T, 64, PAIRS>), g, dim3(64), lds, s, a);
+    } else {
+        if (det) hipLaunchKernelGGL((k_assemble<DOF, NN, NQ, true, T, ASM_THREADS, PAIRS>), g, dim3(ASM_THREADS), lds, s, a);
+        else hipLaunchKernelGGL((k_assemble<DOF, NN, NQ, false, T, ASM_THREADS, PAIRS>), g, dim3(ASM_THREADS), lds, s, a);
+    }
+}
+
+// The pair-block variant runs when the arguments carry pair blocks (thermal Q4).
 template<int DOF, int NN, int NQ, class T>
 void launch_assemble(const AssemblyArgs& a, bool det, size_t lds, i64 count, hipStream_t s, int threads = ASM_THREADS) {
     if (count <= 0) return;
     const dim3 g(static_cast<unsigned>(count));
-    if (threads == 512) {
-        if (det) hipLaunchKernelGGL((k_assemble<DOF, NN, NQ, true, T, 512>), g, dim3(512), lds, s, a);
-        else hipLaunchKernelGGL((k_assemble<DOF, NN, NQ, false, T, 512>), g, dim3(512), lds, s, a);
-    } else if (threads == 128) {
-        if (det) hipLaunchKernelGGL((k_assemble<DOF, NN, NQ, true, T, 128>), g, dim3(128), lds, s, a);
-        else hipLaunchKernelGGL((k_assemble<DOF, NN, NQ, false, T, 128>), g, dim3(128), lds, s, a);
-    } else if (threads == 64) {
-        if (det) hipLaunchKernelGGL((k_assemble<DOF, NN, NQ, true, T, 64>), g, dim3(64), lds, s, a);
-        else hipLaunchKernelGGL((k_assemble<DOF, NN, NQ, false, T, 64>), g, dim3(64), lds, s, a);
-    } else {
-        if (det) hipLaunchKernelGGL((k_assemble<DOF, NN, NQ, true, T>), g, dim3(ASM_THREADS), lds, s, a);
-        else hipLaunchKernelGGL((k_assemble<DOF, NN, NQ, false, T>), g, dim3(ASM_THREADS), lds, s, a);
+    if constexpr (DOF == 1 && NN == 4 && NQ == 4) {
+        if (sizeof(T) == sizeof(double) ? a.pairs != nullptr : a.pairs32 != nullptr) {
+            launch_assemble_kernel<DOF, NN, NQ, T, true>(a, det, lds, g, s, threads);
+            return;
+        }
     }
+    launch_assemble_kernel<DOF, NN, NQ, T, false>(a, det, lds, g, s, threads);
 }
 
 }  // namespace nlf
@@ -1155,7 +1219,10 @@ void assemble_values(nlf_matrix_s* mat, hipStream_t s) {
                 pr.count = i64(list.size()); pr.list = d_list.p; pr.pairs = pairs.p; pr.plane = total * 4;
                 const unsigned pb = unsigned(std::min<i64>((pr.count + 3) / 4, 16384));
                 if constexpr (F64) {
-                    hipLaunchKernelGGL(k_pairs_mfma, dim3(pb), dim3(256), 0, s, pr);
+                    const int poly = influence_class(mat->groups);
+                    if (poly == 1) hipLaunchKernelGGL(k_pairs_mfma<1>, dim3(pb), dim3(256), 0, s, pr);
+                    else if (poly == 2) hipLaunchKernelGGL(k_pairs_mfma<2>, dim3(pb), dim3(256), 0, s, pr);
+                    else hipLaunchKernelGGL(k_pairs_mfma<0>, dim3(pb), dim3(256), 0, s, pr);
                     ca.pairs = pairs.p;
                 } else {
                     hipLaunchKernelGGL(k_pairs_mfma32, dim3(pb), dim3(256), 0, s, pr);

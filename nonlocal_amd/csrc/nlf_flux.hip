@@ -109,24 +109,6 @@ __global__ __launch_bounds__(256) void k_stress_local(i64 E, const i32* __restri
     rec2[idx] = wj * s12;
 }
 
-// phi for the gather.  POLY = 1 or 2: polynomial influence with p = n (d^(p/n) = d) and q = POLY
-// (the fast polynomial, the bell), evaluated with the same operations as influence() -- the same
-// division form near the cutoff, (1 - d) and its square as one multiply -- without the code of
-// the other kinds, which keeps the gather loop small; POLY = 0: influence() itself.
-template<int POLY>
-__device__ __forceinline__ double gather_phi(const InfluenceParams& f, double ax, double ay, double bx, double by) {
-    if (POLY == 0) return influence(f, ax, ay, bx, by);
-    const double dx = ax - bx, dy = ay - by;
-    const double zx = dx * f.inv_r[0], zy = dy * f.inv_r[1];
-    double d = zx * zx + zy * zy;
-    if (fabs(d - 1.0) < 1e-12) {
-        const double ex = dx / f.r[0], ey = dy / f.r[1];
-        d = ex * ex + ey * ey;
-    }
-    const double s = 1.0 - d;
-    return d < 1.0 ? f.norm * (POLY == 1 ? s : s * s) : 0.0;
-}
-
 template<int NQ, int POLY, int C>
 __global__ __launch_bounds__(256) void k_nonlocal_average(i64 E, const i64* __restrict__ nb_off,
                                                           const i32* __restrict__ nb_ids,
@@ -181,7 +163,7 @@ __global__ __launch_bounds__(256) void k_nonlocal_average(i64 E, const i64* __re
         }
         if (t + 128 < pairs) id_after = nb_ids[b + (t + 128) / NQ];
         for (int q = 0; q < NQ; ++q) {
-            const double phi = gather_phi<POLY>(inf, xl[q], yl[q], r.x, r.y);
+            const double phi = influence_poly<POLY>(inf, xl[q], yl[q], r.x, r.y);
             acc[0][q] += phi * r.z;
             acc[1][q] += phi * r.w;
             if (C == 3) acc[C - 1][q] += phi * r2;
@@ -273,19 +255,6 @@ void launch_stress(nlf_mesh m, const GroupParams* groups, int poly, const double
 
 // Influence class of the nonlocal groups for gather_phi: 2 (bell), 1 (fast polynomial) when all
 // nonlocal groups share it, else 0 (general influence()); -1 when no group is nonlocal.
-int influence_class(const std::vector<GroupParams>& gp) {
-    int poly = -1;
-    for (const GroupParams& g : gp) {
-        if (!g.nonlocal) continue;
-        const InfluenceParams& f = g.inf;
-        int cls = 0;
-        if (f.kind == NLF_INFLUENCE_POLYNOMIAL && (f.fast || (f.p_int && f.p_i == 1 && f.q_int && f.q_i <= 2)))
-            cls = f.fast ? 1 : f.q_i;
-        poly = poly < 0 ? cls : (poly == cls ? poly : 0);
-    }
-    return poly;
-}
-
 bool any_nonlocal(const std::vector<GroupParams>& gp) {
     for (const GroupParams& g : gp)
         if (g.nonlocal) return true;

@@ -137,6 +137,24 @@ __device__ __forceinline__ double influence(const InfluenceParams& f, double ax,
     return f.norm * exp(f.exp_q * exp_pow(d, f.p_int, f.p_i, f.p_d));
 }
 
+// phi specialised by influence class (influence_class).  POLY = 1 or 2: polynomial influence with p = n (d^(p/n) = d) and q = POLY
+// (the fast polynomial, the bell), evaluated with the same operations as influence() -- the same
+// division form near the cutoff, (1 - d) and its square as one multiply -- without the code of
+// the other kinds, which keeps the loops that call it small; POLY = 0: influence() itself.
+template<int POLY>
+__device__ __forceinline__ double influence_poly(const InfluenceParams& f, double ax, double ay, double bx, double by) {
+    if (POLY == 0) return influence(f, ax, ay, bx, by);
+    const double dx = ax - bx, dy = ay - by;
+    const double zx = dx * f.inv_r[0], zy = dy * f.inv_r[1];
+    double d = zx * zx + zy * zy;
+    if (fabs(d - 1.0) < 1e-12) {
+        const double ex = dx / f.r[0], ey = dy / f.r[1];
+        d = ex * ex + ey * ey;
+    }
+    const double s = 1.0 - d;
+    return d < 1.0 ? f.norm * (POLY == 1 ? s : s * s) : 0.0;
+}
+
 // FP32 phi for the FP32 assembly: the distance and the polynomial are evaluated in FP32; within
 // 4e-6 of the cutoff the FP64 evaluation decides, so that the support (and the discontinuity of
 // the constant influence) is the FP64 one.
@@ -170,6 +188,11 @@ struct GroupParams {                     // per element group, copied to device
 };
 
 GroupParams model_group_params(const nlf_model& md, bool only_local);
+
+// Influence class shared by the nonlocal groups, for influence_poly<POLY>: 1 = fast polynomial,
+// 2 = bell-shaped polynomial (p = n, q = 2), 0 = anything else (or groups of different classes),
+// -1 = no nonlocal group.
+int influence_class(const std::vector<GroupParams>& gp);
 
 }  // namespace nlf
 
