@@ -408,6 +408,7 @@ def main():
         "fp64_jacobi": (A.NLF_PRECOND_JACOBI, A.NLF_MODE_FP64),
         "learned_fcg": (A.NLF_PRECOND_LEARNED, A.NLF_MODE_FP64),
         "learned_fcg_split": (A.NLF_PRECOND_LEARNED, A.NLF_MODE_SPLIT),
+        "learned_fcg_mixed": (A.NLF_PRECOND_LEARNED, A.NLF_MODE_MIXED),
         "split_jacobi": (A.NLF_PRECOND_JACOBI, A.NLF_MODE_SPLIT),
         "learned_fcg_onelevel": (A.NLF_PRECOND_LEARNED, A.NLF_MODE_FP64),
     }

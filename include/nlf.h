@@ -124,6 +124,8 @@ int         nlf_ctx_synchronize(nlf_ctx ctx);
 #define NLF_OPT_ROW_PATTERNS   2     /* 1: SpMV column offsets from a dictionary of distinct rows, 0: per nonzero */
 #define NLF_OPT_ASSEMBLY_THREADS 3   /* threads per node workgroup of the value pass: 64 (default), 128, 256, 512 */
 #define NLF_OPT_PATTERN_THREADS  4   /* threads per node workgroup of the pattern build: 64, 128 (default), 256 */
+#define NLF_OPT_PAIR_BUDGET_MIB  5   /* MiB for the MFMA pair blocks of one assembly pass; 0 (default): 8 GiB;
+                                        a mesh whose blocks do not fit is assembled in node chunks */
 int         nlf_ctx_set_option(nlf_ctx ctx, int option, int value);
 void*       nlf_ctx_stream(nlf_ctx ctx);
 

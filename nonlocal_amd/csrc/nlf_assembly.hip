@@ -473,7 +473,7 @@ __device__ void nonlocal_row(const AssemblyArgs& a, const GroupParams& g, const 
 template<class T>
 struct PairArgs {
     i64 count;                 // elements of this chunk: list[0 .. count)
-    const i32* list;
+    const i32* list;           // null: element li itself
     const i64* eoff;           // chunk slot of each listed element's first neighbour pair
     const i64* nb_off;
     const i32* nb_ids;
@@ -522,7 +522,7 @@ __global__ __launch_bounds__(256) void k_pairs_mfma(PairArgs<double> a) {
     const int g = (l >> 2) & 3, c = l & 3, h = l >> 4;
     const i64 waves = i64(gridDim.x) * (blockDim.x >> 6);
     for (i64 li = i64(blockIdx.x) * (blockDim.x >> 6) + (threadIdx.x >> 6); li < a.count; li += waves) {
-        const i64 eL = a.list[li];
+        const i64 eL = a.list ? a.list[li] : li;
         const i64 slot = a.eoff[eL];
         const GroupParams gp = a.groups[a.group_of[eL]];          // one group per wave
         if (!gp.nonlocal) continue;
@@ -589,7 +589,7 @@ __global__ __launch_bounds__(256) void k_pairs_mfma32(PairArgs<float> a) {
     const int b = l >> 2, c = l & 3;
     const i64 waves = i64(gridDim.x) * (blockDim.x >> 6);
     for (i64 li = i64(blockIdx.x) * (blockDim.x >> 6) + (threadIdx.x >> 6); li < a.count; li += waves) {
-        const i64 eL = a.list[li];
+        const i64 eL = a.list ? a.list[li] : li;
         const i64 slot = a.eoff[eL];
         const GroupParams gp = a.groups[a.group_of[eL]];          // one group per wave
         if (!gp.nonlocal) continue;
@@ -1092,8 +1092,69 @@ GroupParams make_group(const nlf_model& md, int physics, const void* materials, 
 // nonlocal blocks go through the MFMA pair kernel when its pair buffer (16 values per neighbour
 // pair) fits in half of the free device memory; option NLF_OPT_ASSEMBLY_PAIRS = 0 forces the
 // node-centric contraction.
+// Node chunks of the MFMA pair assembly.  single: every element's blocks (all_pairs of them) fit
+// the budget, one pass over all elements.  Otherwise a chunk is a run of nodes [n0, n1) whose
+// elements' blocks fit the budget: list[lbeg, lend) are its elements (each once, ascending by first
+// touching node) and cofs their first block slot within the chunk.
+struct PairChunk {
+    i64 n0, n1, lbeg, lend, total;
+};
+struct PairPlan {
+    i64 per_pair = 0, all_pairs = 0, max_total = 0;
+    bool single = true;
+    std::vector<PairChunk> chunks;
+    std::vector<i32> list;
+    std::vector<i64> cofs;
+};
+
+i64 pair_budget(const nlf_mesh_s* m) {
+    return m->ctx->pair_budget_mib > 0 ? i64(m->ctx->pair_budget_mib) << 20 : i64(8) << 30;
+}
+
+// nb_off: host copy of the neighbour offsets (E + 1).
+PairPlan plan_pairs(const nlf_mesh_s* m, const std::vector<i64>& nb_off, i64 per_pair) {
+    PairPlan plan;
+    const i64 E = m->E, N = m->N, budget = pair_budget(m);
+    plan.per_pair = per_pair;
+    plan.all_pairs = nb_off[size_t(E)];
+    plan.single = plan.all_pairs * per_pair <= budget;
+    if (plan.single) return plan;
+    const std::vector<i32>& noff = m->ne_off_h;
+    const std::vector<i32>& nel = m->ne_ids_h;
+    std::vector<i32> stamp(size_t(E), -1);
+    plan.list.reserve(size_t(E) + size_t(E) / 4);
+    plan.cofs.reserve(size_t(E) + size_t(E) / 4);
+    for (i64 n0 = 0; n0 < N;) {
+        const int chunk = int(plan.chunks.size());
+        const i64 lbeg = i64(plan.list.size());
+        i64 total = 0, n1 = n0;
+        for (; n1 < N; ++n1) {
+            i64 add = 0;
+            for (i32 k = noff[size_t(n1)]; k < noff[size_t(n1 + 1)]; ++k) {
+                const i32 e = nel[size_t(k)];
+                if (stamp[size_t(e)] != chunk) add += nb_off[size_t(e + 1)] - nb_off[size_t(e)];
+            }
+            if (n1 > n0 && (total + add) * per_pair > budget) break;
+            for (i32 k = noff[size_t(n1)]; k < noff[size_t(n1 + 1)]; ++k) {
+                const i32 e = nel[size_t(k)];
+                if (stamp[size_t(e)] == chunk) continue;
+                stamp[size_t(e)] = chunk;
+                plan.list.push_back(e);
+                plan.cofs.push_back(total);
+                total += nb_off[size_t(e + 1)] - nb_off[size_t(e)];
+            }
+        }
+        plan.chunks.push_back(PairChunk{n0, n1, lbeg, i64(plan.list.size()), total});
+        plan.max_total = std::max(plan.max_total, total);
+        n0 = n1;
+    }
+    return plan;
+}
+
+// plan: the pair chunks, when the caller planned them (while the pattern kernels ran); null or
+// planned for another value size: planned here.
 template<class T>
-void assemble_values(nlf_matrix_s* mat, hipStream_t s) {
+void assemble_values(nlf_matrix_s* mat, hipStream_t s, const PairPlan* plan = nullptr) {
     constexpr bool F64 = sizeof(T) == sizeof(double);
     const nlf_mesh m = mat->mesh;
     const int dof = mat->dof;
@@ -1157,86 +1218,83 @@ void assemble_values(nlf_matrix_s* mat, hipStream_t s) {
         else if (mat->physics == 1 && m->nn == 4) hipLaunchKernelGGL((k_assemble_local<2, 4, 4, T>), dim3(nb), dim3(256), 0, s, aa);
         else hipLaunchKernelGGL((k_assemble_local<2, 8, 9, T>), dim3(nb), dim3(256), 0, s, aa);
     } else if (use_pairs) {
-        // thermal Q4 nonlocal blocks through the MFMA pair kernel, node chunk by node chunk: a chunk
-        // is a run of nodes whose elements' pair blocks (16 values per neighbour pair) take at most
-        // PAIR_BUDGET bytes, so the buffer is bounded whatever the mesh size.  Elements shared by
-        // two chunks are contracted once in each.
-        constexpr size_t PAIR_BUDGET = size_t(8) << 30;
-        const i64 E = m->E, N = m->N, nn = m->nn;
-        std::vector<i64> nb_off(size_t(E + 1));
-        download(nb_off.data(), m->nb_off.p, nb_off.size(), s);
-        std::vector<i32> noff(size_t(N + 1), 0), nel(size_t(E * nn));     // node -> elements (ascending)
-        for (i64 e = 0; e < E * nn; ++e) ++noff[size_t(m->el_h[size_t(e)]) + 1];
-        for (i64 v = 0; v < N; ++v) noff[size_t(v + 1)] += noff[size_t(v)];
-        {
-            std::vector<i32> fill(noff.begin(), noff.end() - 1);
-            for (i64 e = 0; e < E; ++e)
-                for (i64 k = 0; k < nn; ++k) nel[size_t(fill[size_t(m->el_h[size_t(e * nn + k)])]++)] = i32(e);
+        // thermal Q4 nonlocal blocks through the MFMA pair kernel.  When every element's blocks (16
+        // values per neighbour pair) fit the budget -- 8 GiB, or NLF_OPT_PAIR_BUDGET_MIB -- one pass
+        // lists every element once with its blocks at its neighbour-list offset; otherwise node chunk
+        // by node chunk (plan_pairs): the buffer is bounded whatever the mesh size, and elements
+        // shared by two chunks are contracted once in each.  The chunks run back to back on the
+        // stream (each chunk's offsets are written after the previous chunk's kernels, in stream order).
+        const i64 E = m->E, N = m->N;
+        const i64 per_pair = 16 * i64(sizeof(T));
+        PairPlan own;
+        if (!plan || plan->per_pair != per_pair) {
+            std::vector<i64> nb_off(size_t(E + 1));
+            download(nb_off.data(), m->nb_off.p, nb_off.size(), s);
+            own = plan_pairs(m, nb_off, per_pair);
+            plan = &own;
         }
+        const i64 all_pairs = plan->all_pairs;
         DevBuf<T> HL{size_t(E) * 32}, HN{size_t(E) * 32}, pairs;
         Weights4 w4{};
         for (int q = 0; q < 4; ++q) w4.w[q] = m->fe.w[q];
         hipLaunchKernelGGL(k_pair_prep<T>, dim3(blocks_for(E * 16, 256)), dim3(256), 0, s, E, m->D.p, w4, m->group_of.p,
                            d_groups.p, HL.p, HN.p);
-        DevBuf<i64> d_eoff{size_t(E)}, d_cofs;
-        DevBuf<i32> d_list;
         PairArgs<T> pr{};
         pr.nb_off = m->nb_off.p; pr.nb_ids = m->nb_ids.p; pr.group_of = m->group_of.p;
         pr.groups = d_groups.p; pr.xq = m->xq.p; pr.HL = HL.p; pr.HN = HN.p; pr.el_min = el_min.p;
-        pr.el_max = m->el_maxnode.p; pr.el_dir = el_dir.p; pr.eoff = d_eoff.p;
-        const i64 per_pair = 16 * i64(sizeof(T));
-        std::vector<i32> stamp(size_t(E), -1), list;
-        std::vector<i64> cofs;
-        int chunk = 0;
-        for (i64 n0 = 0; n0 < N; ++chunk) {
-            list.clear();
-            cofs.clear();
-            i64 total = 0, n1 = n0;
-            for (; n1 < N; ++n1) {
-                i64 add = 0;
-                for (i32 k = noff[size_t(n1)]; k < noff[size_t(n1 + 1)]; ++k) {
-                    const i32 e = nel[size_t(k)];
-                    if (stamp[size_t(e)] != chunk) add += nb_off[size_t(e + 1)] - nb_off[size_t(e)];
-                }
-                if (n1 > n0 && (total + add) * per_pair > i64(PAIR_BUDGET)) break;
-                for (i32 k = noff[size_t(n1)]; k < noff[size_t(n1 + 1)]; ++k) {
-                    const i32 e = nel[size_t(k)];
-                    if (stamp[size_t(e)] == chunk) continue;
-                    stamp[size_t(e)] = chunk;
-                    list.push_back(e);
-                    cofs.push_back(total);
-                    total += nb_off[size_t(e + 1)] - nb_off[size_t(e)];
-                }
+        pr.el_max = m->el_maxnode.p; pr.el_dir = el_dir.p;
+        const int poly = influence_class(mat->groups);
+        auto contract = [&](i64 count) {
+            const unsigned pb = unsigned(std::min<i64>((count + 3) / 4, 16384));
+            pr.count = count;
+            if constexpr (F64) {
+                if (poly == 1) hipLaunchKernelGGL(k_pairs_mfma<1>, dim3(pb), dim3(256), 0, s, pr);
+                else if (poly == 2) hipLaunchKernelGGL(k_pairs_mfma<2>, dim3(pb), dim3(256), 0, s, pr);
+                else hipLaunchKernelGGL(k_pairs_mfma<0>, dim3(pb), dim3(256), 0, s, pr);
+            } else {
+                hipLaunchKernelGGL(k_pairs_mfma32, dim3(pb), dim3(256), 0, s, pr);
             }
-            AssemblyArgs ca = aa;
-            ca.node0 = n0;
-            if (!list.empty() && total > 0) {
-                upload(d_list, list.data(), list.size(), s);
-                upload(d_cofs, cofs.data(), cofs.size(), s);
-                hipLaunchKernelGGL(k_scatter_offsets, dim3(blocks_for(i64(list.size()), 256)), dim3(256), 0, s,
-                                   i64(list.size()), d_list.p, d_cofs.p, d_eoff.p);
-                if (pairs.n < size_t(total) * 16) pairs.alloc(size_t(total) * 16);
-                pr.count = i64(list.size()); pr.list = d_list.p; pr.pairs = pairs.p; pr.plane = total * 4;
-                const unsigned pb = unsigned(std::min<i64>((pr.count + 3) / 4, 16384));
-                if constexpr (F64) {
-                    const int poly = influence_class(mat->groups);
-                    if (poly == 1) hipLaunchKernelGGL(k_pairs_mfma<1>, dim3(pb), dim3(256), 0, s, pr);
-                    else if (poly == 2) hipLaunchKernelGGL(k_pairs_mfma<2>, dim3(pb), dim3(256), 0, s, pr);
-                    else hipLaunchKernelGGL(k_pairs_mfma<0>, dim3(pb), dim3(256), 0, s, pr);
-                    ca.pairs = pairs.p;
-                } else {
-                    hipLaunchKernelGGL(k_pairs_mfma32, dim3(pb), dim3(256), 0, s, pr);
-                    ca.pairs32 = pairs.p;
-                }
-                NLF_HIP(hipGetLastError());
-                ca.plane = total * 4;
-                ca.eoff = d_eoff.p;
-            }
-            launch_assemble<1, 4, 4, T>(ca, det, lds, n1 - n0, s, m->ctx->assembly_threads);
             NLF_HIP(hipGetLastError());
-            NLF_HIP(hipStreamSynchronize(s));          // the next chunk rewrites the list and the offsets
-            n0 = n1;
+        };
+        if (plan->single) {
+            AssemblyArgs ca = aa;
+            ca.node0 = 0;
+            if (all_pairs > 0) {
+                pairs.alloc(size_t(all_pairs) * 16);
+                pr.list = nullptr; pr.eoff = m->nb_off.p; pr.pairs = pairs.p; pr.plane = all_pairs * 4;
+                contract(E);
+                if constexpr (F64) ca.pairs = pairs.p; else ca.pairs32 = pairs.p;
+                ca.plane = all_pairs * 4;
+                ca.eoff = m->nb_off.p;
+            }
+            launch_assemble<1, 4, 4, T>(ca, det, lds, N, s, m->ctx->assembly_threads);
+            NLF_HIP(hipGetLastError());
+        } else {
+            DevBuf<i64> d_eoff{size_t(E)}, d_cofs;
+            DevBuf<i32> d_list;
+            upload(d_list, plan->list.data(), plan->list.size(), s);
+            upload(d_cofs, plan->cofs.data(), plan->cofs.size(), s);
+            if (plan->max_total > 0) pairs.alloc(size_t(plan->max_total) * 16);
+            pr.eoff = d_eoff.p;
+            for (const PairChunk& c : plan->chunks) {
+                AssemblyArgs ca = aa;
+                ca.node0 = c.n0;
+                const i64 count = c.lend - c.lbeg;
+                if (count > 0 && c.total > 0) {
+                    hipLaunchKernelGGL(k_scatter_offsets, dim3(blocks_for(count, 256)), dim3(256), 0, s, count,
+                                       d_list.p + c.lbeg, d_cofs.p + c.lbeg, d_eoff.p);
+                    pr.list = d_list.p + c.lbeg; pr.pairs = pairs.p; pr.plane = c.total * 4;
+                    contract(count);
+                    if constexpr (F64) ca.pairs = pairs.p; else ca.pairs32 = pairs.p;
+                    ca.plane = c.total * 4;
+                    ca.eoff = d_eoff.p;
+                }
+                launch_assemble<1, 4, 4, T>(ca, det, lds, c.n1 - c.n0, s, m->ctx->assembly_threads);
+                NLF_HIP(hipGetLastError());
+            }
+            NLF_HIP(hipStreamSynchronize(s));          // before the chunk offsets and lists are released
         }
+        NLF_HIP(hipStreamSynchronize(s));              // before the pair buffer is released
     } else if (mat->physics != 1 && m->nn == 4) launch_assemble<1, 4, 4, T>(aa, det, lds, m->N, s, m->ctx->assembly_threads);
     else if (mat->physics != 1 && m->nn == 8) launch_assemble<1, 8, 9, T>(aa, det, lds, m->N, s, m->ctx->assembly_threads);
     else if (mat->physics == 1 && m->nn == 4) launch_assemble<2, 4, 4, T>(aa, det, lds, m->N, s, m->ctx->assembly_threads);
@@ -1367,8 +1425,18 @@ int nlf_assemble(nlf_mesh m, int physics, const nlf_model* models, const void* m
             pa.saved = saved.p;
             pa.stride = int(std::max<unsigned>(words, 1));
         }
+        // thermal Q4 nonlocal assembly takes the MFMA pair path: its chunk plan needs the neighbour
+        // offsets on the host and is computed while pass 0 of the pattern runs
+        const bool pairs_path = physics == 0 && m->nn == 4 && any_nonlocal && !only_local && m->ctx->assembly_pairs != 0;
+        std::vector<i64> nb_off_h;
+        if (pairs_path) {
+            nb_off_h.resize(size_t(m->E + 1));
+            download(nb_off_h.data(), m->nb_off.p, nb_off_h.size(), s);
+        }
         hipLaunchKernelGGL(k_pattern<0>, dim3(unsigned(m->N)), dim3(pth), lds_bitmap, s, pa);
         NLF_HIP(hipGetLastError());
+        PairPlan plan;
+        if (pairs_path) plan = plan_pairs(m, nb_off_h, 16 * i64(sizeof(double)));
         int ovf = 0;
         download(&ovf, overflow.p, 1, s);
         NLF_CHECK(!ovf, NLF_ERR_UNSUPPORTED, "column node ids of a row span more than 393k ids (renumber the mesh)");
@@ -1390,7 +1458,7 @@ int nlf_assemble(nlf_mesh m, int physics, const nlf_model* models, const void* m
         mat->physics = physics;
         mat->only_local = only_local;
         mat->groups = gps;
-        assemble_values<double>(mat.get(), s);
+        assemble_values<double>(mat.get(), s, pairs_path ? &plan : nullptr);
         mat->nnz_upper = (mat->nnz_full + rows) / 2;
         mat->seconds_values = std::chrono::duration<double>(std::chrono::steady_clock::now() - t1).count();
         *out = mat.release();

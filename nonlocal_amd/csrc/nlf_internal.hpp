@@ -206,6 +206,7 @@ struct nlf_ctx_s {
     int row_patterns = 1;                // NLF_OPT_ROW_PATTERNS
     int assembly_threads = 64;           // NLF_OPT_ASSEMBLY_THREADS
     int pattern_threads = 128;           // NLF_OPT_PATTERN_THREADS
+    int pair_budget_mib = 0;             // NLF_OPT_PAIR_BUDGET_MIB (0: automatic)
 };
 
 struct nlf_mesh_s {
@@ -219,6 +220,7 @@ struct nlf_mesh_s {
     std::vector<double> xy_h;
     std::vector<nlf::i32> el_h, be_h;
     std::vector<nlf::i64> gbeg_h, bbeg_h;
+    std::vector<nlf::i32> ne_off_h, ne_ids_h;   // node -> elements (ascending), as ne_off / ne_ids
     // device geometry and precompute
     nlf::DevBuf<double> xy;              // 2N
     nlf::DevBuf<nlf::i32> el;            // E*nn

@@ -223,7 +223,9 @@ const char* nlf_last_error(nlf_ctx ctx) { return ctx ? ctx->error.c_str() : "nul
 // NLF_OPT_ROW_PATTERNS 1 = shared row-pattern dictionary for the SpMV of matrices assembled
 // afterwards (default), 0 = per-nonzero 16-bit offsets; NLF_OPT_ASSEMBLY_THREADS = threads per
 // node workgroup of the node-centric value pass (64 default, 128, 256, 512); NLF_OPT_PATTERN_THREADS
-// = threads per node workgroup of the pattern build (64, 128, 256 default).
+// = threads per node workgroup of the pattern build (64, 128 default, 256); NLF_OPT_PAIR_BUDGET_MIB =
+// device memory for the MFMA pair blocks of one assembly pass in MiB (0 = 8 GiB); a mesh whose blocks
+// do not fit is assembled in node chunks.
 int nlf_ctx_set_option(nlf_ctx ctx, int option, int value) {
     if (!ctx) return NLF_ERR_ARGUMENT;
     if (option == NLF_OPT_ASSEMBLY_PAIRS) ctx->assembly_pairs = value != 0;
@@ -234,6 +236,9 @@ int nlf_ctx_set_option(nlf_ctx ctx, int option, int value) {
     } else if (option == NLF_OPT_PATTERN_THREADS) {
         if (value != 64 && value != 128 && value != 256) return NLF_ERR_ARGUMENT;
         ctx->pattern_threads = value;
+    } else if (option == NLF_OPT_PAIR_BUDGET_MIB) {
+        if (value < 0) return NLF_ERR_ARGUMENT;
+        ctx->pair_budget_mib = value;
     }
     else return NLF_ERR_ARGUMENT;
     return NLF_OK;
@@ -301,6 +306,8 @@ int nlf_mesh_create(nlf_ctx ctx, int64_t n_nodes, const double* xy, int64_t n_el
         upload(m->el, elems, size_t(n_elems * nn), s);
         upload(m->ne_off, off.data(), off.size(), s);
         upload(m->ne_ids, ids.data(), ids.size(), s);
+        m->ne_off_h = std::move(off);
+        m->ne_ids_h = std::move(ids);
         upload(m->group_of, group_of.data(), group_of.size(), s);
         const int nq = m->fe.nq;
         m->xq.alloc(size_t(2 * n_elems * nq));

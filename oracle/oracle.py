@@ -191,7 +191,7 @@ class OracleMesh:
         if not self.h:
             raise RuntimeError(lib().orc_last_error().decode())
 
-    def __del__(self):
+    def __del__(self, lib=lib):             # bound now: module globals go first at exit
         if getattr(self, "h", None):
             lib().orc_mesh_destroy(self.h)
             self.h = None

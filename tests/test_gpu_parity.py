@@ -124,6 +124,28 @@ def test_mfma_pair_path_matches_node_path(gpu_ctx, problem):
     assert np.array_equal(a.val, c.val), "the MFMA path must be bitwise reproducible"
 
 
+@pytest.mark.parametrize("problem", [PROBLEMS[0], PROBLEMS[3]], ids=[IDS[0], IDS[3]])
+def test_pair_chunks_match_single_pass(gpu_ctx, problem):
+    """Thermal Q4 pair blocks assembled in node chunks (a 1 MiB budget) against the single pass:
+    every value bitwise equal, the bound part included."""
+    print(f"[pair chunks] {problem.name}", flush=True)
+    g = _gpu_mesh(problem, gpu_ctx)
+    api.find_neighbours(g, H.radii(problem.models))
+    inner = O.OracleMesh(problem.mesh).boundary(problem.bcs, problem.dof)[2]
+    whole = api._assemble(g, 0, problem.models, problem.materials, inner)
+    pairs = g.neighbours()[1].size
+    assert pairs * 16 * 8 > 2 << 20, "the problem must need several 1 MiB chunks"
+    gpu_ctx.set_option(A.NLF_OPT_PAIR_BUDGET_MIB, 1)
+    try:
+        chunked = api._assemble(g, 0, problem.models, problem.materials, inner)
+    finally:
+        gpu_ctx.set_option(A.NLF_OPT_PAIR_BUDGET_MIB, 0)
+    for part in (1, 2):
+        a, b = whole.export(part), chunked.export(part)
+        assert np.array_equal(a.inner, b.inner) and np.array_equal(a.val, b.val), f"part {part}"
+    print(f"  {pairs} neighbour pairs, {pairs * 128 / 2**20:.1f} MiB of blocks: equal", flush=True)
+
+
 @pytest.mark.parametrize("problem", PROBLEMS[:2], ids=IDS[:2])
 def test_assembly_deterministic(gpu_ctx, problem):
     print(f"[determinism] {problem.name}", flush=True)
