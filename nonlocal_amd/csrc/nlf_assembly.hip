@@ -513,11 +513,15 @@ __global__ void k_pair_prep(i64 E, const double* __restrict__ D, Weights4 w4, co
     HN[e * 32 + 16 + i * 4 + q] = T(w * cy);
 }
 
-// A wave first reads 64 neighbour ids and their "needed" flags in one round, then takes the needed
-// ones eight at a time (two MFMA sets of four blocks), so each round of gathers serves eight pairs.
-// A pair's block does not depend on which set or block slot it lands in.  POLY: influence_poly.
+// A wave first reads 64 neighbour ids and their "needed" flags in one round, then takes the runs
+// of four consecutive neighbours that hold a needed one, four runs at a time (four MFMA sets of four
+// blocks), so each round of gathers serves sixteen pairs.  Every block of a run is computed and
+// stored: with an element's first slot a multiple of four (padded offsets) a run fills whole
+// 128-byte lines of each row plane, so no line is written in part.  A pair's block does not depend
+// on which set or block slot it lands in.  POLY: influence_poly.
 template<int POLY>
-__global__ __launch_bounds__(256) void k_pairs_mfma(PairArgs<double> a) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void k_pairs_mfma(PairArgs<double> a) {
+    constexpr int SETS = 4;
     const int l = threadIdx.x & 63;
     const int g = (l >> 2) & 3, c = l & 3, h = l >> 4;
     const i64 waves = i64(gridDim.x) * (blockDim.x >> 6);
@@ -532,39 +536,41 @@ __global__ __launch_bounds__(256) void k_pairs_mfma(PairArgs<double> a) {
         const i64 nb_b = a.nb_off[eL], cnt = a.nb_off[eL + 1] - nb_b;
         const i32 emin = a.el_min[eL];
         const double s = 1.0 - gp.p1;
+        // the ids of the next batch of 64 are loaded while the current batch is contracted
+        i32 id_next = l < cnt ? a.nb_ids[nb_b + l] : 0;
         for (i64 b64 = 0; b64 < cnt; b64 += 64) {
-            i32 id = 0;
-            bool nd = false;
-            if (b64 + l < cnt) {
-                id = a.nb_ids[nb_b + b64 + l];
-                // the pair reaches the upper triangle of some row of eL, or a Dirichlet column
-                nd = a.el_max[id] >= emin || a.el_dir[id];
-            }
-            unsigned long long mask = __ballot(nd);
-            while (mask) {
-                int pos[2] = {-1, -1};                            // this block's neighbour in each set
-                for (int k = 0; k < 8; ++k) {
-                    const int p = mask ? int(__builtin_ctzll(mask)) : -1;
-                    mask &= mask - 1;
-                    if (k == g) pos[0] = p;
-                    if (k == 4 + g) pos[1] = p;
+            const i32 id = id_next;
+            // the pair reaches the upper triangle of some row of eL, or a Dirichlet column
+            const bool nd = b64 + l < cnt && (a.el_max[id] >= emin || a.el_dir[id]);
+            if (b64 + 64 + l < cnt) id_next = a.nb_ids[nb_b + b64 + 64 + l];
+            unsigned long long runs = __ballot(nd);
+            runs |= runs >> 1;
+            runs |= runs >> 2;
+            runs &= 0x1111111111111111ull;                       // bit 4q: run q holds a needed pair
+            while (runs) {
+                int pos[SETS];                                    // this block's neighbour in each set
+                for (int u = 0; u < SETS; ++u) {
+                    pos[u] = runs ? int(__builtin_ctzll(runs)) + g : -1;
+                    runs &= runs - 1;
+                    if (pos[u] >= 0 && b64 + pos[u] >= cnt) pos[u] = -1;
                 }
-                double phi[2], hn0[2], hn1[2];
+                double xnx[SETS], xny[SETS], hn0[SETS], hn1[SETS];
 #pragma unroll
-                for (int u = 0; u < 2; ++u) {
+                for (int u = 0; u < SETS; ++u) {
                     const i64 eNL = __shfl(id, pos[u] < 0 ? 0 : pos[u]);
-                    phi[u] = hn0[u] = hn1[u] = 0.0;
+                    xnx[u] = xny[u] = hn0[u] = hn1[u] = 0.0;
                     if (pos[u] >= 0) {
-                        phi[u] = influence_poly<POLY>(gp.inf, xlx, xly, a.xq[2 * (eNL * 4 + c)],
-                                                      a.xq[2 * (eNL * 4 + c) + 1]);
+                        xnx[u] = a.xq[2 * (eNL * 4 + c)];
+                        xny[u] = a.xq[2 * (eNL * 4 + c) + 1];
                         hn0[u] = a.HN[eNL * 32 + c * 4 + h];      // HN_0[j = c][qn = h]
                         hn1[u] = a.HN[eNL * 32 + 16 + c * 4 + h];
                     }
                 }
 #pragma unroll
-                for (int u = 0; u < 2; ++u) {
-                    const double v0 = __builtin_amdgcn_mfma_f64_4x4x4f64(phi[u], hl0, 0.0, 0, 0, 0);
-                    const double v1 = __builtin_amdgcn_mfma_f64_4x4x4f64(phi[u], hl1, 0.0, 0, 0, 0);
+                for (int u = 0; u < SETS; ++u) {
+                    const double phi = pos[u] >= 0 ? influence_poly<POLY>(gp.inf, xlx, xly, xnx[u], xny[u]) : 0.0;
+                    const double v0 = __builtin_amdgcn_mfma_f64_4x4x4f64(phi, hl0, 0.0, 0, 0, 0);
+                    const double v1 = __builtin_amdgcn_mfma_f64_4x4x4f64(phi, hl1, 0.0, 0, 0, 0);
                     double k = __builtin_amdgcn_mfma_f64_4x4x4f64(v0, hn0[u], 0.0, 0, 0, 0);
                     k = __builtin_amdgcn_mfma_f64_4x4x4f64(v1, hn1[u], k, 0, 0, 0);
                     if (pos[u] >= 0) a.pairs[i64(h) * a.plane + (slot + b64 + pos[u]) * 4 + c] = s * k;  // K[i = h][j = c]
@@ -947,6 +953,12 @@ __global__ void k_row_max(i64 rows, const i64* __restrict__ outer, const i64* __
     atomicMax(mx + 1, static_cast<unsigned long long>(w));
 }
 
+// slots[e] = neighbour count of e rounded up to a multiple of four (the pair-block slots of e).
+__global__ void k_padded_slots(i64 E, const i64* __restrict__ nb_off, i64* __restrict__ slots) {
+    const i64 e = i64(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (e < E) slots[e] = (nb_off[e + 1] - nb_off[e] + 3) & ~i64(3);
+}
+
 // eoff[list[i]] = ofs[i]: chunk slots of the chunk's elements.
 __global__ void k_scatter_offsets(i64 n, const i32* __restrict__ list, const i64* __restrict__ ofs, i64* __restrict__ eoff) {
     const i64 i = i64(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -1092,8 +1104,9 @@ GroupParams make_group(const nlf_model& md, int physics, const void* materials, 
 // nonlocal blocks go through the MFMA pair kernel when its pair buffer (16 values per neighbour
 // pair) fits in half of the free device memory; option NLF_OPT_ASSEMBLY_PAIRS = 0 forces the
 // node-centric contraction.
-// Node chunks of the MFMA pair assembly.  single: every element's blocks (all_pairs of them) fit
-// the budget, one pass over all elements.  Otherwise a chunk is a run of nodes [n0, n1) whose
+// Node chunks of the MFMA pair assembly.  An element's blocks take its neighbour count rounded up
+// to a multiple of four slots (k_pairs_mfma writes runs of four).  single: every element's blocks
+// (all_pairs slots) fit the budget, one pass over all elements.  Otherwise a chunk is a run of nodes [n0, n1) whose
 // elements' blocks fit the budget: list[lbeg, lend) are its elements (each once, ascending by first
 // touching node) and cofs their first block slot within the chunk.
 struct PairChunk {
@@ -1105,6 +1118,10 @@ struct PairPlan {
     std::vector<PairChunk> chunks;
     std::vector<i32> list;
     std::vector<i64> cofs;
+    // order in which the pair kernel visits the elements, sorted on the device: (chunk << 32 |
+    // Morton key of the element) for every entry of list (single: every element, its Morton key)
+    std::vector<uint64_t> order_key;
+    std::vector<i32> order_val;
 };
 
 i64 pair_budget(const nlf_mesh_s* m) {
@@ -1116,9 +1133,19 @@ PairPlan plan_pairs(const nlf_mesh_s* m, const std::vector<i64>& nb_off, i64 per
     PairPlan plan;
     const i64 E = m->E, N = m->N, budget = pair_budget(m);
     plan.per_pair = per_pair;
-    plan.all_pairs = nb_off[size_t(E)];
+    auto slots = [&](i64 e) { return (nb_off[size_t(e + 1)] - nb_off[size_t(e)] + 3) & ~i64(3); };
+    for (i64 e = 0; e < E; ++e) plan.all_pairs += slots(e);
     plan.single = plan.all_pairs * per_pair <= budget;
-    if (plan.single) return plan;
+    const std::vector<uint32_t>& key = m->el_key_h;
+    if (plan.single) {
+        plan.order_key.resize(size_t(E));
+        plan.order_val.resize(size_t(E));
+        for (i64 e = 0; e < E; ++e) {
+            plan.order_key[size_t(e)] = key[size_t(e)];
+            plan.order_val[size_t(e)] = i32(e);
+        }
+        return plan;
+    }
     const std::vector<i32>& noff = m->ne_off_h;
     const std::vector<i32>& nel = m->ne_ids_h;
     std::vector<i32> stamp(size_t(E), -1);
@@ -1132,7 +1159,7 @@ PairPlan plan_pairs(const nlf_mesh_s* m, const std::vector<i64>& nb_off, i64 per
             i64 add = 0;
             for (i32 k = noff[size_t(n1)]; k < noff[size_t(n1 + 1)]; ++k) {
                 const i32 e = nel[size_t(k)];
-                if (stamp[size_t(e)] != chunk) add += nb_off[size_t(e + 1)] - nb_off[size_t(e)];
+                if (stamp[size_t(e)] != chunk) add += slots(e);
             }
             if (n1 > n0 && (total + add) * per_pair > budget) break;
             for (i32 k = noff[size_t(n1)]; k < noff[size_t(n1 + 1)]; ++k) {
@@ -1141,13 +1168,18 @@ PairPlan plan_pairs(const nlf_mesh_s* m, const std::vector<i64>& nb_off, i64 per
                 stamp[size_t(e)] = chunk;
                 plan.list.push_back(e);
                 plan.cofs.push_back(total);
-                total += nb_off[size_t(e + 1)] - nb_off[size_t(e)];
+                total += slots(e);
             }
         }
         plan.chunks.push_back(PairChunk{n0, n1, lbeg, i64(plan.list.size()), total});
         plan.max_total = std::max(plan.max_total, total);
         n0 = n1;
     }
+    plan.order_key.resize(plan.list.size());
+    for (const PairChunk& c : plan.chunks)
+        for (i64 i = c.lbeg; i < c.lend; ++i)
+            plan.order_key[size_t(i)] = (uint64_t(&c - plan.chunks.data()) << 32) | key[size_t(plan.list[size_t(i)])];
+    plan.order_val = plan.list;
     return plan;
 }
 
@@ -1234,6 +1266,16 @@ void assemble_values(nlf_matrix_s* mat, hipStream_t s, const PairPlan* plan = nu
             plan = &own;
         }
         const i64 all_pairs = plan->all_pairs;
+        // the pair kernel's element order: Morton order within each chunk
+        DevBuf<i32> d_order;
+        {
+            DevBuf<uint64_t> k_in, k_out{plan->order_key.size()};
+            DevBuf<i32> v_in;
+            upload(k_in, plan->order_key.data(), plan->order_key.size(), s);
+            upload(v_in, plan->order_val.data(), plan->order_val.size(), s);
+            d_order.alloc(plan->order_val.size());
+            sort_pairs_u64(k_in.p, k_out.p, v_in.p, d_order.p, i64(plan->order_val.size()), s);
+        }
         DevBuf<T> HL{size_t(E) * 32}, HN{size_t(E) * 32}, pairs;
         Weights4 w4{};
         for (int q = 0; q < 4; ++q) w4.w[q] = m->fe.w[q];
@@ -1259,13 +1301,16 @@ void assemble_values(nlf_matrix_s* mat, hipStream_t s, const PairPlan* plan = nu
         if (plan->single) {
             AssemblyArgs ca = aa;
             ca.node0 = 0;
+            DevBuf<i64> d_eoff{size_t(E + 1)};
             if (all_pairs > 0) {
+                hipLaunchKernelGGL(k_padded_slots, dim3(blocks_for(E, 256)), dim3(256), 0, s, E, m->nb_off.p, d_eoff.p);
+                exclusive_scan_i64(d_eoff.p, E, s);
                 pairs.alloc(size_t(all_pairs) * 16);
-                pr.list = nullptr; pr.eoff = m->nb_off.p; pr.pairs = pairs.p; pr.plane = all_pairs * 4;
+                pr.list = d_order.p; pr.eoff = d_eoff.p; pr.pairs = pairs.p; pr.plane = all_pairs * 4;
                 contract(E);
                 if constexpr (F64) ca.pairs = pairs.p; else ca.pairs32 = pairs.p;
                 ca.plane = all_pairs * 4;
-                ca.eoff = m->nb_off.p;
+                ca.eoff = d_eoff.p;
             }
             launch_assemble<1, 4, 4, T>(ca, det, lds, N, s, m->ctx->assembly_threads);
             NLF_HIP(hipGetLastError());
@@ -1283,7 +1328,7 @@ void assemble_values(nlf_matrix_s* mat, hipStream_t s, const PairPlan* plan = nu
                 if (count > 0 && c.total > 0) {
                     hipLaunchKernelGGL(k_scatter_offsets, dim3(blocks_for(count, 256)), dim3(256), 0, s, count,
                                        d_list.p + c.lbeg, d_cofs.p + c.lbeg, d_eoff.p);
-                    pr.list = d_list.p + c.lbeg; pr.pairs = pairs.p; pr.plane = c.total * 4;
+                    pr.list = d_order.p + c.lbeg; pr.pairs = pairs.p; pr.plane = c.total * 4;
                     contract(count);
                     if constexpr (F64) ca.pairs = pairs.p; else ca.pairs32 = pairs.p;
                     ca.plane = c.total * 4;

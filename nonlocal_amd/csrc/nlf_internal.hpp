@@ -221,6 +221,7 @@ struct nlf_mesh_s {
     std::vector<nlf::i32> el_h, be_h;
     std::vector<nlf::i64> gbeg_h, bbeg_h;
     std::vector<nlf::i32> ne_off_h, ne_ids_h;   // node -> elements (ascending), as ne_off / ne_ids
+    std::vector<uint32_t> el_key_h;             // Morton (Z-order) key of each element's centroid
     // device geometry and precompute
     nlf::DevBuf<double> xy;              // 2N
     nlf::DevBuf<nlf::i32> el;            // E*nn
@@ -309,6 +310,8 @@ inline SpmvCols spmv_cols(const nlf_matrix_s* a) {
     return {a->cols.p, a->off16.p, nullptr};
 }
 void exclusive_scan_i64(i64* data, i64 n, hipStream_t s);   // in place over n+1 entries, data[n] = total
+// (keys, values) sorted by key, stable (radix sort); n pairs
+void sort_pairs_u64(const uint64_t* keys_in, uint64_t* keys_out, const i32* vals_in, i32* vals_out, i64 n, hipStream_t s);
 void set_error(nlf_ctx ctx, const std::string& msg);
 
 // Runs f, translating exceptions into status codes and the context's last error.

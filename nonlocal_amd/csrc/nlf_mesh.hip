@@ -308,6 +308,36 @@ int nlf_mesh_create(nlf_ctx ctx, int64_t n_nodes, const double* xy, int64_t n_el
         upload(m->ne_ids, ids.data(), ids.size(), s);
         m->ne_off_h = std::move(off);
         m->ne_ids_h = std::move(ids);
+        // Morton key of each element's centroid on a 2^16 x 2^16 grid over the bounding box: the
+        // pair kernel visits elements in this order, so that the elements in flight form compact
+        // patches whose neighbour data stay in L2
+        {
+            double x0 = 1e308, x1 = -1e308, y0 = 1e308, y1 = -1e308;
+            for (i64 v = 0; v < n_nodes; ++v) {
+                x0 = std::min(x0, xy[2 * v]); x1 = std::max(x1, xy[2 * v]);
+                y0 = std::min(y0, xy[2 * v + 1]); y1 = std::max(y1, xy[2 * v + 1]);
+            }
+            const double sx = 65535.0 / std::max(x1 - x0, 1e-300), sy = 65535.0 / std::max(y1 - y0, 1e-300);
+            auto spread = [](uint32_t v) {         // bit k of v to bit 2k
+                v &= 0xffffu;
+                v = (v | (v << 8)) & 0x00ff00ffu;
+                v = (v | (v << 4)) & 0x0f0f0f0fu;
+                v = (v | (v << 2)) & 0x33333333u;
+                v = (v | (v << 1)) & 0x55555555u;
+                return v;
+            };
+            m->el_key_h.resize(size_t(n_elems));
+            for (i64 e = 0; e < n_elems; ++e) {
+                double cx = 0.0, cy = 0.0;
+                for (int i = 0; i < nn; ++i) {
+                    cx += xy[2 * i64(elems[e * nn + i])];
+                    cy += xy[2 * i64(elems[e * nn + i]) + 1];
+                }
+                const uint32_t qx = uint32_t(std::clamp((cx / nn - x0) * sx, 0.0, 65535.0));
+                const uint32_t qy = uint32_t(std::clamp((cy / nn - y0) * sy, 0.0, 65535.0));
+                m->el_key_h[size_t(e)] = spread(qx) | (spread(qy) << 1);
+            }
+        }
         upload(m->group_of, group_of.data(), group_of.size(), s);
         const int nq = m->fe.nq;
         m->xq.alloc(size_t(2 * n_elems * nq));

@@ -30,6 +30,15 @@ void exclusive_scan_i64(i64* data, i64 n, hipStream_t s) {
     NLF_HIP(hipStreamSynchronize(s));
 }
 
+void sort_pairs_u64(const uint64_t* keys_in, uint64_t* keys_out, const i32* vals_in, i32* vals_out, i64 n, hipStream_t s) {
+    if (n <= 0) return;
+    size_t bytes = 0;
+    NLF_HIP(rocprim::radix_sort_pairs(nullptr, bytes, keys_in, keys_out, vals_in, vals_out, size_t(n), 0, 64, s));
+    DevBuf<uint8_t> tmp{bytes + 16};
+    NLF_HIP(rocprim::radix_sort_pairs(tmp.p, bytes, keys_in, keys_out, vals_in, vals_out, size_t(n), 0, 64, s));
+    NLF_HIP(hipStreamSynchronize(s));          // before the temporary storage is released
+}
+
 struct SearchGroup {
     double r[2];        // expanded radii
     double inv_r[2];
