@@ -134,19 +134,56 @@ __global__ __launch_bounds__(256) void k_gj_rowpanel(double* __restrict__ a, i64
         for (int v = 0; v < 4; ++v) blk[i64(ty * 4 + u) * ncp + tx * 4 + v] = acc[u][v];
 }
 
-// Trailing update: A(i, j) -= A(i, k) A(k, j) for tile rows i != k and tile columns j != k.
-__global__ __launch_bounds__(256) void k_gj_update(double* __restrict__ a, i64 ncp, i64 k) {
-    __shared__ double sx[64][33], sy[32][65];
+// Trailing update: A(i, j) -= A(i, k) A(k, j) for tile rows i != k and tile
+// columns j != k with V_MFMA_F64_16X16X4_F64.  Lane l of a wave holds A[l % 16][l / 16] and
+// B[l / 16][l % 16] of a 16 x 4 by 4 x 16 product and D[l / 16 + 4 r][l % 16] in register r
+// (measured on gfx950).
+// Wave w computes the 32 x 32 quadrant (w / 2, w % 2) of the tile as 2 x 2 blocks of 16 x 16.  The
+// inner dimension is staged through LDS in two halves of 32, XOR-swizzled by row so that the
+// operand reads of one instruction hit distinct banks.
+__global__ __launch_bounds__(256) void k_gj_update_mfma(double* __restrict__ a, i64 ncp, i64 k) {
+    typedef double d4 __attribute__((ext_vector_type(4)));
+    __shared__ double sx[64][32];                                 // X(i, kk) at [i][kk ^ (2 i & 31)]
+    __shared__ double sy[32][64];                                 // Y(kk, j) at [kk][j ^ 16 (kk & 3)]
     const i64 bi = blockIdx.y < k ? blockIdx.y : blockIdx.y + 1;
     const i64 bj = blockIdx.x < k ? blockIdx.x : blockIdx.x + 1;
     const i64 nb = ncp / TB;
     if (bi >= nb || bj >= nb) return;
-    double acc[4][4] = {};
-    tile_mm(a + (bi * TB) * ncp + k * TB, ncp, a + (k * TB) * ncp + bj * TB, ncp, acc, sx, sy);
-    const int t = threadIdx.x, tx = t % 16, ty = t / 16;
-    double* blk = a + (bi * TB) * ncp + bj * TB;
-    for (int u = 0; u < 4; ++u)
-        for (int v = 0; v < 4; ++v) blk[i64(ty * 4 + u) * ncp + tx * 4 + v] -= acc[u][v];
+    const int t = threadIdx.x, w = t >> 6, l = t & 63;
+    const int li = l & 15, lk = l >> 4;
+    const int r0 = 32 * (w >> 1), c0 = 32 * (w & 1);
+    const double* X = a + (bi * TB) * ncp + k * TB;
+    const double* Y = a + (k * TB) * ncp + bj * TB;
+    d4 acc[2][2];
+    for (int u = 0; u < 2; ++u)
+        for (int v = 0; v < 2; ++v) acc[u][v] = d4{0.0, 0.0, 0.0, 0.0};
+    for (int half = 0; half < 2; ++half) {
+        __syncthreads();
+        for (int e = t; e < 64 * 32; e += 256) {
+            const int i = e >> 5, j = e & 31;
+            sx[i][j ^ ((2 * i) & 31)] = X[i64(i) * ncp + 32 * half + j];
+        }
+        for (int e = t; e < 32 * 64; e += 256) {
+            const int kk = e >> 6, j = e & 63;
+            sy[kk][j ^ (16 * (kk & 3))] = Y[i64(32 * half + kk) * ncp + j];
+        }
+        __syncthreads();
+        for (int ks = 0; ks < 32; ks += 4) {
+            const int kk = ks + lk;
+            double av[2], bv[2];
+            for (int b = 0; b < 2; ++b) {
+                const int i = r0 + 16 * b + li;
+                av[b] = sx[i][kk ^ ((2 * i) & 31)];
+                bv[b] = sy[kk][(c0 + 16 * b + li) ^ (16 * (kk & 3))];
+            }
+            for (int u = 0; u < 2; ++u)
+                for (int v = 0; v < 2; ++v) acc[u][v] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[u], bv[v], acc[u][v], 0, 0, 0);
+        }
+    }
+    double* C = a + (bi * TB) * ncp + bj * TB;
+    for (int u = 0; u < 2; ++u)
+        for (int v = 0; v < 2; ++v)
+            for (int r = 0; r < 4; ++r) C[i64(r0 + 16 * u + lk + 4 * r) * ncp + c0 + 16 * v + li] -= acc[u][v][r];
 }
 
 // Column panel: A(i, k) = -A(i, k) D^-1 for tile rows i != k.
@@ -275,7 +312,7 @@ std::unique_ptr<CoarseSpace> build_coarse(const nlf_matrix_s* a, i64 target, hip
         hipLaunchKernelGGL(k_gj_diag, dim3(1), dim3(1024), 0, s, cs->inv.p, cs->ncp, k, dinv.p);
         if (nb > 1) {
             hipLaunchKernelGGL(k_gj_rowpanel, dim3(unsigned(nb - 1)), dim3(256), 0, s, cs->inv.p, cs->ncp, k, dinv.p);
-            hipLaunchKernelGGL(k_gj_update, dim3(unsigned(nb - 1), unsigned(nb - 1)), dim3(256), 0, s, cs->inv.p, cs->ncp, k);
+            hipLaunchKernelGGL(k_gj_update_mfma, dim3(unsigned(nb - 1), unsigned(nb - 1)), dim3(256), 0, s, cs->inv.p, cs->ncp, k);
             hipLaunchKernelGGL(k_gj_colpanel, dim3(unsigned(nb - 1)), dim3(256), 0, s, cs->inv.p, cs->ncp, k, dinv.p);
         }
         hipLaunchKernelGGL(k_gj_setdiag, dim3(1), dim3(256), 0, s, cs->inv.p, cs->ncp, k, dinv.p);

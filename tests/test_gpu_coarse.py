@@ -51,16 +51,19 @@ def problem(gpu_ctx):
     return tp
 
 
-def test_coarse_apply_matches_dense_algebra(problem, gpu_ctx):
+@pytest.mark.parametrize("target", [128, 1024])
+def test_coarse_apply_matches_dense_algebra(problem, gpu_ctx, target):
+    """128 aggregates: two 64-row tiles of the blocked inverse; 1024 (264 after the 16-nodes cap):
+    five tiles, so the trailing updates cover 4 x 4 tiles."""
     tp = problem
     n = tp.b.numel()
     r = torch.from_numpy(np.random.default_rng(0).standard_normal(n)).cuda()
     tp.learned.set_coarse(0)
     z_m = tp.learned.solve(r).cpu().numpy()
-    tp.learned.set_coarse(128)
+    tp.learned.set_coarse(target)
     rows, sec = tp.learned.coarse_info()
     z = tp.learned.solve(r).cpu().numpy()
-    agg = _aggregates(tp.problem.mesh.xy, 128)
+    agg = _aggregates(tp.problem.mesh.xy, target)
     assert rows == agg.max() + 1
     csr = tp.K.export(2)
     K = sp.csr_matrix((csr.val, csr.inner, csr.outer), shape=(n, n))
@@ -71,7 +74,7 @@ def test_coarse_apply_matches_dense_algebra(problem, gpu_ctx):
     print(f"[coarse] {rows} coarse rows, set-up {sec * 1e3:.1f} ms, apply vs dense algebra {err:.2e}", flush=True)
     assert err <= 1e-10
     # reproducible: a second build gives the same operator bit for bit
-    tp.learned.set_coarse(128)
+    tp.learned.set_coarse(target)
     assert np.array_equal(tp.learned.solve(r).cpu().numpy(), z)
     tp.learned.set_coarse(0)
 
