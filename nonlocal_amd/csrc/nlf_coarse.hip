@@ -235,6 +235,37 @@ __global__ void k_coarse_prolong(i64 n, const i32* __restrict__ agg, const doubl
     for (i64 i = i64(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += i64(gridDim.x) * blockDim.x) z[i] += yc[agg[i]];
 }
 
+// Cell of every node on the cx x cy grid (the operations of the host form: offset, division,
+// truncation, clamp to the last cell), and the used cells marked.
+__global__ void k_cells(i64 N, const double* __restrict__ xy, double xmin, double ymin, double hx, double hy, i64 cx,
+                        i64 cy, i64* __restrict__ cell, i64* __restrict__ used) {
+    const i64 v = i64(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (v >= N) return;
+    const i64 fx = i64((xy[2 * v] - xmin) / hx), fy = i64((xy[2 * v + 1] - ymin) / hy);
+    const i64 ix = fx < cx - 1 ? fx : cx - 1, iy = fy < cy - 1 ? fy : cy - 1;
+    cell[v] = ix * cy + iy;
+    used[ix * cy + iy] = 1;
+}
+
+// Coarse row of every fine row (the used cells numbered in cell order, times dof, plus the fine
+// row's component), the sort key of the members list, and the per-aggregate counts.
+__global__ void k_aggregate(i64 n, int dof, const i64* __restrict__ cell, const i64* __restrict__ cid,
+                            i32* __restrict__ agg, uint64_t* __restrict__ key, i32* __restrict__ row,
+                            unsigned long long* __restrict__ count) {
+    const i64 r = i64(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (r >= n) return;
+    const i32 I = i32(cid[cell[r / dof]] * dof + r % dof);
+    agg[r] = I;
+    key[r] = uint64_t(I);
+    row[r] = i32(r);
+    atomicAdd(&count[I], 1ull);
+}
+
+__global__ void k_to_i64(i64 n, const unsigned long long* __restrict__ in, i64* __restrict__ out) {
+    const i64 i = i64(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = i64(in[i]);
+}
+
 }  // namespace
 
 std::unique_ptr<CoarseSpace> build_coarse(const nlf_matrix_s* a, i64 target, hipStream_t s) {
@@ -247,45 +278,42 @@ std::unique_ptr<CoarseSpace> build_coarse(const nlf_matrix_s* a, i64 target, hip
     // Gauss-Jordan, one 32 KiB LDS row per coarse row in the Galerkin product)
     target = std::min<i64>(target, std::max<i64>(1, N / 16));
     target = std::min<i64>(target, 4096 / dof);
-    double xmin = 1e308, xmax = -1e308, ymin = 1e308, ymax = -1e308;
-    for (i64 v = 0; v < N; ++v) {
-        xmin = std::min(xmin, m->xy_h[2 * v]); xmax = std::max(xmax, m->xy_h[2 * v]);
-        ymin = std::min(ymin, m->xy_h[2 * v + 1]); ymax = std::max(ymax, m->xy_h[2 * v + 1]);
-    }
+    const double xmin = m->bbox[0], xmax = m->bbox[1], ymin = m->bbox[2], ymax = m->bbox[3];
     const double w = std::max(xmax - xmin, 1e-300), h = std::max(ymax - ymin, 1e-300);
     const i64 cx = std::min<i64>(target, std::max<i64>(1, i64(std::lround(std::sqrt(double(target) * w / h)))));
     const i64 cy = std::max<i64>(1, target / cx);
     const double hx = w / double(cx), hy = h / double(cy);
-    // cell of every node, then the non-empty cells numbered in cell order
-    std::vector<i64> cell(static_cast<size_t>(N));
-    std::vector<i32> used(static_cast<size_t>(cx * cy), 0);
-    for (i64 v = 0; v < N; ++v) {
-        const i64 ix = std::min<i64>(cx - 1, i64((m->xy_h[2 * v] - xmin) / hx));
-        const i64 iy = std::min<i64>(cy - 1, i64((m->xy_h[2 * v + 1] - ymin) / hy));
-        cell[size_t(v)] = ix * cy + iy;
-        used[size_t(cell[size_t(v)])] = 1;
-    }
-    std::vector<i64> cid(static_cast<size_t>(cx * cy), -1);
+    // on the device: the cell of every node, the non-empty cells numbered in cell order, the coarse
+    // row of every fine row, and the members of every aggregate, rows ascending (stable sort by
+    // coarse row)
+    DevBuf<i64> cell{size_t(N)}, cid{size_t(cx * cy + 1)};
+    NLF_HIP(hipMemsetAsync(cid.p, 0, cid.bytes(), s));
+    hipLaunchKernelGGL(k_cells, dim3(blocks_for(N, 256)), dim3(256), 0, s, N, m->xy.p, xmin, ymin, hx, hy, cx, cy, cell.p,
+                       cid.p);
+    exclusive_scan_i64(cid.p, cx * cy, s);
     i64 nagg = 0;
-    for (i64 c = 0; c < cx * cy; ++c)
-        if (used[size_t(c)]) cid[size_t(c)] = nagg++;
+    download(&nagg, cid.p + cx * cy, 1, s);
     auto cs = std::make_unique<CoarseSpace>();
     cs->n = n;
     cs->nc = nagg * dof;
     cs->ncp = (cs->nc + TB - 1) / TB * TB;
-    std::vector<i32> agg(static_cast<size_t>(n));
-    std::vector<i64> cnt(static_cast<size_t>(cs->nc + 1), 0);
-    for (i64 r = 0; r < n; ++r) {
-        agg[size_t(r)] = i32(cid[size_t(cell[size_t(r / dof)])] * dof + r % dof);
-        ++cnt[size_t(agg[size_t(r)]) + 1];
+    cs->agg.alloc(size_t(n));
+    cs->mem.alloc(size_t(n));
+    cs->mem_off.alloc(size_t(cs->nc + 1));
+    {
+        DevBuf<uint64_t> key{size_t(n)}, key_sorted{size_t(n)};
+        DevBuf<i32> row{size_t(n)};
+        DevBuf<unsigned long long> count{size_t(cs->nc)};
+        NLF_HIP(hipMemsetAsync(count.p, 0, count.bytes(), s));
+        hipLaunchKernelGGL(k_aggregate, dim3(blocks_for(n, 256)), dim3(256), 0, s, n, dof, cell.p, cid.p, cs->agg.p, key.p,
+                           row.p, count.p);
+        hipLaunchKernelGGL(k_to_i64, dim3(blocks_for(cs->nc, 256)), dim3(256), 0, s, cs->nc, count.p, cs->mem_off.p);
+        NLF_HIP(hipGetLastError());
+        exclusive_scan_i64(cs->mem_off.p, cs->nc, s);
+        sort_pairs_u64(key.p, key_sorted.p, row.p, cs->mem.p, n, s);
     }
-    for (i64 I = 0; I < cs->nc; ++I) cnt[size_t(I + 1)] += cnt[size_t(I)];
-    std::vector<i32> mem(static_cast<size_t>(n));
-    std::vector<i64> fill(cnt.begin(), cnt.end() - 1);
-    for (i64 r = 0; r < n; ++r) mem[size_t(fill[size_t(agg[size_t(r)])]++)] = i32(r);   // rows ascending per aggregate
-    upload(cs->agg, agg.data(), agg.size(), s);
-    upload(cs->mem_off, cnt.data(), cnt.size(), s);
-    upload(cs->mem, mem.data(), mem.size(), s);
+    std::vector<i64> cnt(size_t(cs->nc + 1));
+    download(cnt.data(), cs->mem_off.p, cnt.size(), s);
     // fixed-point scale: every partial sum of a coarse entry is bounded by (rows per aggregate) x
     // (largest absolute row sum); keep it below 2^61
     i64 maxmem = 0;

@@ -222,6 +222,7 @@ struct nlf_mesh_s {
     std::vector<nlf::i64> gbeg_h, bbeg_h;
     std::vector<nlf::i32> ne_off_h, ne_ids_h;   // node -> elements (ascending), as ne_off / ne_ids
     std::vector<uint32_t> el_key_h;             // Morton (Z-order) key of each element's centroid
+    double bbox[4] = {0.0, 0.0, 0.0, 0.0};       // x min, x max, y min, y max of the nodes
     // device geometry and precompute
     nlf::DevBuf<double> xy;              // 2N
     nlf::DevBuf<nlf::i32> el;            // E*nn

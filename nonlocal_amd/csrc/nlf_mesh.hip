@@ -317,6 +317,7 @@ int nlf_mesh_create(nlf_ctx ctx, int64_t n_nodes, const double* xy, int64_t n_el
                 x0 = std::min(x0, xy[2 * v]); x1 = std::max(x1, xy[2 * v]);
                 y0 = std::min(y0, xy[2 * v + 1]); y1 = std::max(y1, xy[2 * v + 1]);
             }
+            m->bbox[0] = x0; m->bbox[1] = x1; m->bbox[2] = y0; m->bbox[3] = y1;
             const double sx = 65535.0 / std::max(x1 - x0, 1e-300), sy = 65535.0 / std::max(y1 - y0, 1e-300);
             auto spread = [](uint32_t v) {         // bit k of v to bit 2k
                 v &= 0xffffu;
