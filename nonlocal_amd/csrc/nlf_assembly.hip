@@ -214,9 +214,15 @@ __global__ __launch_bounds__(PATTERN_THREADS) void k_pattern(PatternArgs a) {
         for (int w = threadIdx.x; w < words; w += blockDim.x) bitmap[w] = 0u;
         __syncthreads();
         for_each_column_element(a, node, [&](i32 e) {
-            for (int j = 0; j < a.nn; ++j) {
-                const i64 v = a.el[i64(e) * a.nn + j] - lo;
-                atomicOr(&bitmap[v >> 5], 1u << (v & 31));
+            // the element's node ids in 16-byte loads (rows of el are 16-byte aligned: nn = 4 or 8)
+            const int4* row = reinterpret_cast<const int4*>(a.el + i64(e) * a.nn);
+            for (int q = 0; q < a.nn / 4; ++q) {
+                const int4 ids = row[q];
+                const i32 id[4] = {ids.x, ids.y, ids.z, ids.w};
+                for (int j = 0; j < 4; ++j) {
+                    const i64 v = id[j] - lo;
+                    atomicOr(&bitmap[v >> 5], 1u << (v & 31));
+                }
             }
         });
         __syncthreads();
@@ -753,7 +759,15 @@ __global__ __launch_bounds__(THREADS) void k_assemble(AssemblyArgs a) {
                     const T* pr = pairs + i64(iL) * a.plane + (eoff + t) * 4;
                     for (int j = 0; j < NN; ++j) v[u][j][0] = pr[j];
                 }
-                for (int j = 0; j < NN; ++j) mj[u][j] = a.el[eNL[u] * NN + j];
+                if constexpr (NN % 4 == 0) {
+                    const int4* row = reinterpret_cast<const int4*>(a.el + eNL[u] * NN);
+                    for (int q = 0; q < NN / 4; ++q) {
+                        const int4 ids = row[q];
+                        mj[u][4 * q] = ids.x; mj[u][4 * q + 1] = ids.y; mj[u][4 * q + 2] = ids.z; mj[u][4 * q + 3] = ids.w;
+                    }
+                } else {
+                    for (int j = 0; j < NN; ++j) mj[u][j] = a.el[eNL[u] * NN + j];
+                }
             }
 #pragma unroll
             for (int u = 0; u < ITEMS; ++u) {
