@@ -981,18 +981,51 @@ __device__ __forceinline__ int find_sorted(const i32* a, int n, i32 key) {
 // (a multiple of CP_THREADS) starting at id `base`; returns its size, or -1 when it exceeds cap or
 // the id window.
 __device__ int union_rows(const i64* __restrict__ outer, const i32* __restrict__ cols, const i32* src, int ns, i64 base,
-                          unsigned* bm, int words, i32* dst, int cap) {
+                          unsigned* bm, int words, i32* dst, int cap, i64* rs, int* pre) {
     for (int i = threadIdx.x; i < words; i += blockDim.x) bm[i] = 0u;
     __shared__ int bad;
+    // the rows of src flattened (rs[k]: start of row k in cols, rs[CP_P2], pre[CP_P2 + 1]): row k's
+    // entries are [pre[k], pre[k + 1]) of the concatenation, so every entry is loaded in parallel
+    // instead of row after row
+    static_assert(CP_P2 <= 3 * CP_THREADS && CP_THREADS == 64, "three rows per lane in the scan, one wave");
     if (threadIdx.x == 0) bad = 0;
-    __syncthreads();
-    for (int k = 0; k < ns; ++k) {
-        const i64 v = src[k];
-        for (i64 p = outer[v] + threadIdx.x; p < outer[v + 1]; p += blockDim.x) {
-            const i64 off = i64(cols[p]) - base;
-            if (off < 0 || off >= i64(words) * 32) bad = 1;
-            else atomicOr(&bm[off >> 5], 1u << (off & 31));
+    {
+        const int l = threadIdx.x;
+        int len[3] = {0, 0, 0}, sum = 0;
+        for (int q = 0; q < 3; ++q) {
+            const int k = 3 * l + q;
+            if (k < ns) {
+                const i64 v = src[k];
+                rs[k] = outer[v];
+                len[q] = int(outer[v + 1] - rs[k]);
+            }
+            sum += len[q];
         }
+        int incl = sum;                                  // inclusive wave scan of the lane sums
+        for (int o = 1; o < 64; o <<= 1) {
+            const int y = __shfl_up(incl, o);
+            if (l >= o) incl += y;
+        }
+        int run = incl - sum;
+        for (int q = 0; q < 3; ++q) {
+            const int k = 3 * l + q;
+            if (k < ns) pre[k] = run;
+            run += len[q];
+        }
+        if (l == 63) pre[ns] = incl;
+    }
+    __syncthreads();
+    const int entries = pre[ns];
+    for (int e = threadIdx.x; e < entries; e += blockDim.x) {
+        int lo = 0, hi = ns - 1;                         // the row k with pre[k] <= e < pre[k + 1]
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (pre[mid] <= e) lo = mid; else hi = mid - 1;
+        }
+        const i64 p = rs[lo] + (e - pre[lo]);
+        const i64 off = i64(cols[p]) - base;
+        if (off < 0 || off >= i64(words) * 32) bad = 1;
+        else atomicOr(&bm[off >> 5], 1u << (off & 31));
     }
     __syncthreads();
     if (bad) return -1;
@@ -1052,12 +1085,14 @@ __global__ __launch_bounds__(CP_THREADS) void k_lf_compile(i64 n, const i64* __r
     }
     __syncthreads();
     const i64 base2 = std::max<i64>(u - 2 * band, 0), base3 = std::max<i64>(u - 3 * band, 0);
-    const int n2 = union_rows(outer, cols, p1, n1, base2, bm, words, p2, CP_P2);
+    __shared__ i64 rs[CP_P2];
+    __shared__ int pre[CP_P2 + 1];
+    const int n2 = union_rows(outer, cols, p1, n1, base2, bm, words, p2, CP_P2, rs, pre);
     if (n2 < 0) {
         if (threadIdx.x == 0) { if (PASS == 0) cnt[u] = 0; *overflow = 1; }
         return;
     }
-    const int n3 = union_rows(outer, cols, p2, n2, base3, bm, words, p3, CP_P3);
+    const int n3 = union_rows(outer, cols, p2, n2, base3, bm, words, p3, CP_P3, rs, pre);
     if (n3 < 0) {
         if (threadIdx.x == 0) { if (PASS == 0) cnt[u] = 0; *overflow = 1; }
         return;
