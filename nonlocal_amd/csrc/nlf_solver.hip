@@ -25,7 +25,7 @@
 namespace nlf {
 
 constexpr int SPMV_THREADS = 256;
-constexpr int SPMV_PAIRS_IN_FLIGHT = 4;  // nonzero pairs each lane of the pair kernel loads per step (2 or 4)
+constexpr int SPMV_PAIRS_IN_FLIGHT = 4;  // nonzero pairs each lane of the pair kernel loads per step (8: no faster)
 constexpr int LEVEL_THREADS = 1024;
 
 struct CGState {
@@ -1040,12 +1040,13 @@ void launch_spmv(int lpr, i64 rows, const i64* outer, SpmvCols cs, const V* val,
         } else {
 #define NLF_SPMV2_CASE(L)                                                                                          \
     case L:                                                                                                        \
-        if (pf == 4)                                                                                               \
-            hipLaunchKernelGGL((k_spmv2<L, V, X, Acc, DOT, false, false, 4>), dim3(blocks), dim3(SPMV_THREADS), 0, s, rows, \
-                               outer, off16, pstart, val, static_cast<const int16_t*>(nullptr), x, y, partial, st);   \
-        else                                                                                                       \
+        if (pf == 2)                                                                                               \
             hipLaunchKernelGGL((k_spmv2<L, V, X, Acc, DOT, false, false, 2>), dim3(blocks), dim3(SPMV_THREADS), 0, s, rows, \
                                outer, off16, pstart, val, static_cast<const int16_t*>(nullptr), x, y, partial, st);   \
+        else                                                                                                       \
+            hipLaunchKernelGGL((k_spmv2<L, V, X, Acc, DOT, false, false, SPMV_PAIRS_IN_FLIGHT>), dim3(blocks),        \
+                               dim3(SPMV_THREADS), 0, s, rows, outer, off16, pstart, val,                           \
+                               static_cast<const int16_t*>(nullptr), x, y, partial, st);                           \
         return;
             switch (lpr) {
                 NLF_SPMV2_CASE(4)
@@ -1400,7 +1401,7 @@ int nlf_spmv_fp32(nlf_matrix a, const float* x, float* y) {
 // bit 12 the per-nonzero 16-bit offsets instead of the row-pattern dictionary, bit 13 the split
 // values (FP32 + 16-bit remainder, FP64 vectors; fp32 = 0), bit 14 the one-nonzero-per-load kernel
 // instead of the pair kernel, bit 15 two nonzero pairs in flight per lane in the pair kernel instead of
-// four; bits 10 / 11 (FP64 only, LPR 32 or 64, per-nonzero offsets) select 8
+// SPMV_PAIRS_IN_FLIGHT; bits 10 / 11 (FP64 only, LPR 32 or 64, per-nonzero offsets) select 8
 // nonzeros in flight per lane / non-temporal loads of the matrix stream.
 int nlf_spmv_lpr(nlf_matrix a, int lpr, int fp32, const void* x, void* y) {
     if (!a) return NLF_ERR_ARGUMENT;
