@@ -274,6 +274,7 @@ std::unique_ptr<CoarseSpace> build_coarse(const nlf_matrix_s* a, i64 target, hip
     const int dof = a->dof;
     const i64 N = m->N, n = a->rows;
     NLF_CHECK(target > 0, NLF_ERR_ARGUMENT, "coarse target must be positive");
+    NLF_CHECK(N > 0 && n > 0, NLF_ERR_ARGUMENT, "the coarse space needs a mesh with nodes");
     // at least 16 nodes per aggregate on average, at most 4096 coarse rows (dense inverse in LDS-tiled
     // Gauss-Jordan, one 32 KiB LDS row per coarse row in the Galerkin product)
     target = std::min<i64>(target, std::max<i64>(1, N / 16));
