@@ -1522,7 +1522,11 @@ i64 fcg(hipStream_t s, nlf_matrix A, const V* val, nlf_precond M, const double* 
                 if (!std::isfinite(est) || !std::isfinite(hs.beta) || !std::isfinite(hs.pq)) { it = -it; break; }
                 if (est < 0.9 * best) { best = est; best_at = double(it); }
                 if (double(it) - best_at > window) { it = -it; break; }         // stagnation
-                if (est < 0.1 * last_update || est <= tol * bn || it >= max_it) {
+                // the recursive residual is replaced by the FP64 true residual at every tenfold drop
+                // when the SpMV reads reduced-precision values, every thousandfold drop with FP64
+                // values (where it drifts far less), and always before stopping
+                const double refresh = sizeof(V) == sizeof(double) ? 1e-3 : 0.1;
+                if (est < refresh * last_update || est <= tol * bn || it >= max_it) {
                     auto tr = true_residual();
                     ++nupd;
                     rel = tr.first / bn;
