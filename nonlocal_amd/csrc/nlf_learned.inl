@@ -1064,8 +1064,11 @@ __global__ __launch_bounds__(CP_THREADS) void k_lf_compile(i64 n, const i64* __r
                                                            const CompileBasis* __restrict__ B,
                                                            const float* __restrict__ w, i64* __restrict__ cnt,
                                                            const i64* __restrict__ m_outer, i32* __restrict__ m_cols,
-                                                           float* __restrict__ yrow, int* __restrict__ overflow) {
-    __shared__ unsigned bm[CP_WORDS];
+                                                           float* __restrict__ yrow, int* __restrict__ overflow,
+                                                           int cdrows) {
+    // bitmap of `words` words, reused after the unions for the contracted basis of cdrows 1-hop
+    // entries (cdrows >= the longest row of the graph); sized by the launch
+    extern __shared__ unsigned bm[];
     __shared__ i32 p1[CP_P1], p2[CP_P2], p3[CP_P3];
     __shared__ float cf[CP_P1][5];
     __shared__ float aw[CP_P2], kw[CP_P2][MP_F];
@@ -1073,7 +1076,7 @@ __global__ __launch_bounds__(CP_THREADS) void k_lf_compile(i64 n, const i64* __r
     if (u >= n) return;
     const i64 r0 = outer[u];
     const int n1 = int(outer[u + 1] - r0);
-    if (n1 > CP_P1) {
+    if (n1 > CP_P1 || n1 > cdrows) {
         if (threadIdx.x == 0) { if (PASS == 0) cnt[u] = 0; *overflow = 1; }
         return;
     }
@@ -1104,9 +1107,8 @@ __global__ __launch_bounds__(CP_THREADS) void k_lf_compile(i64 n, const i64* __r
     const float* L1 = w + P_LAYER0 + P_LAYER_SIZE;
     // the basis contracted with the coefficients of every j in P1, once per node (the bitmap's
     // LDS is free after the unions): cd[j][c] = sum_m cf[j][m] d1[m][c], ct[j][c] likewise with t1
-    static_assert(2 * CP_P1 * MP_H <= CP_WORDS, "contracted basis must fit the bitmap's LDS");
     float (*cd)[MP_H] = reinterpret_cast<float (*)[MP_H]>(bm);
-    float (*ct)[MP_H] = cd + CP_P1;
+    float (*ct)[MP_H] = cd + cdrows;
     for (int idx = threadIdx.x; idx < n1 * MP_H; idx += blockDim.x) {
         const int j = idx / MP_H, cc = idx - j * MP_H;
         float a = 0.0f, b = 0.0f;
@@ -1236,8 +1238,12 @@ void learned_compile(LearnedGraph& g, const float* w, hipStream_t s, bool ctx_ro
     NLF_HIP(hipMemsetAsync(overflow.p, 0, sizeof(int), s));
     NLF_HIP(hipMemsetAsync(cnt.p, 0, size_t(n + 1) * sizeof(i64), s));
     const float4* feat = reinterpret_cast<const float4*>(g.feat.p);
-    hipLaunchKernelGGL(k_lf_compile<0>, dim3(unsigned(n)), dim3(CP_THREADS), 0, s, n, g.outer.p, g.cols.p, feat, g.band,
-                       words, basis.p, w, cnt.p, (const i64*)nullptr, (i32*)nullptr, (float*)nullptr, overflow.p);
+    // dynamic LDS: the bitmap, or the contracted basis of up to ell_d 1-hop entries after it
+    const int cdrows = std::min(g.ell_d, CP_P1);
+    const size_t lds = size_t(std::max(words, 2 * cdrows * MP_H)) * sizeof(unsigned);
+    hipLaunchKernelGGL(k_lf_compile<0>, dim3(unsigned(n)), dim3(CP_THREADS), size_t(words) * sizeof(unsigned), s, n,
+                       g.outer.p, g.cols.p, feat, g.band, words, basis.p, w, cnt.p, (const i64*)nullptr, (i32*)nullptr,
+                       (float*)nullptr, overflow.p, cdrows);
     NLF_HIP(hipGetLastError());
     int ovf = 0;
     download(&ovf, overflow.p, 1, s);
@@ -1250,8 +1256,8 @@ void learned_compile(LearnedGraph& g, const float* w, hipStream_t s, bool ctx_ro
     g.m_cols.alloc(size_t(std::max<i64>(nnz, 1)));
     g.m_val.alloc(size_t(std::max<i64>(nnz, 1)));
     DevBuf<float> yrow{size_t(std::max<i64>(nnz, 1))};
-    hipLaunchKernelGGL(k_lf_compile<1>, dim3(unsigned(n)), dim3(CP_THREADS), 0, s, n, g.outer.p, g.cols.p, feat, g.band,
-                       words, basis.p, w, (i64*)nullptr, g.m_outer.p, g.m_cols.p, yrow.p, overflow.p);
+    hipLaunchKernelGGL(k_lf_compile<1>, dim3(unsigned(n)), dim3(CP_THREADS), lds, s, n, g.outer.p, g.cols.p, feat, g.band,
+                       words, basis.p, w, (i64*)nullptr, g.m_outer.p, g.m_cols.p, yrow.p, overflow.p, cdrows);
     hipLaunchKernelGGL(k_lf_symmetrize, dim3(blocks_for(n, 4)), dim3(256), 0, s, n, g.m_outer.p, g.m_cols.p, yrow.p, g.s.p,
                        g.m_val.p);
     g.m_off16.release();
