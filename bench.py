@@ -10,8 +10,9 @@ value = assembly+solve seconds of the fastest accurate GPU method (normally flex
 learned preconditioner: FP32 network, FP64 Krylov vectors, SpMV and true residual); the plain FP64
 ILLT-CG time and the other methods are reported beside it.  roofline = the dominant kernel (CSR SpMV inside the
 solve) against 8 TB/s HBM, timed live with HIP events on the solver stream.  cpu_baseline = the
-oracle (the reference algorithm restated in C++/OpenMP) timed on this host on a smaller instance of
-the same workload family and scaled to the full size (see the `sample` field).
+oracle (the reference algorithm restated in C++/OpenMP) timed on this host on the full workload
+(neighbours, assembly and preconditioner set-up whole; the ILLT-CG for a bounded number of
+iterations, scaled by the iteration count); the same oracle system checks the GPU result.
 """
 from __future__ import annotations
 
@@ -54,7 +55,8 @@ def parse():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--n", type=int, default=1000, help="mesh side (1000 = the benchmark workload)")
     ap.add_argument("--tol", type=float, default=1e-10)
-    ap.add_argument("--cpu-sample-n", type=int, default=480)
+    ap.add_argument("--cpu-cg-iterations", type=int, default=40,
+                    help="ILLT-CG iterations the oracle runs on the full workload for its per-iteration time")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--methods", default="fp64_jacobi,mixed_jacobi,learned_fcg_onelevel,learned_fcg")
     ap.add_argument("--workload", default="composite1000",
@@ -79,34 +81,82 @@ def run_step(api, A, mesh, problem, precond, mode, tol, coarse=None):
     return sol, wall
 
 
-def cpu_baseline(n_sample, full_nodes, full_iterations, tol):
-    """Oracle (reference algorithm restated) on an n_sample^2 composite with the same r/h, scaled up."""
-    import numpy as np  # noqa: F401
-    from nonlocal_amd import configs
+def host_description():
+    """The host the CPU leg runs on: logical CPUs, the ones this process may use, the CPU model, the
+    NUMA nodes with their CPU lists, and OMP_NUM_THREADS."""
+    import glob
+    d = {"nproc": os.cpu_count(), "usable_cpus": len(os.sched_getaffinity(0)),
+         "OMP_NUM_THREADS": os.environ.get("OMP_NUM_THREADS")}
+    try:
+        with open("/proc/cpuinfo") as f:
+            d["cpu_model"] = next((ln.split(":", 1)[1].strip() for ln in f if ln.startswith("model name")), None)
+    except OSError:
+        d["cpu_model"] = None
+    numa = {}
+    for node in sorted(glob.glob("/sys/devices/system/node/node[0-9]*")):
+        try:
+            with open(os.path.join(node, "cpulist")) as f:
+                numa[os.path.basename(node)] = f.read().strip()
+        except OSError:
+            pass
+    d["numa_nodes"] = numa
+    return d
+
+
+def cpu_baseline(problem, plain_iterations, check, cg_iterations):
+    """The oracle (the reference algorithm restated in C++/OpenMP, with its O(N^2) flag resets) on
+    the full workload: neighbour search, assembly of K, boundary conditions, assembly of the local
+    matrix and its IC(0) factor are timed whole; the ILLT-CG is timed for `cg_iterations`
+    iterations and its per-iteration time is multiplied by the iteration count the same IC(0)-CG
+    needs on this system (the GPU's plain FP64 ILLT-CG count).  The same oracle system then checks
+    the GPU result: the residual of the GPU solution on the oracle's K and f, the oracle's K v
+    against the GPU's for one random v, and sampled rows of the two upper CSRs."""
     from nonlocal_amd import _abi as A
     from oracle import oracle as O
     cores = O.set_threads(0)
-    p = configs.composite(n_sample, r=10.0 / n_sample)
+    radii = [[m.radius[0], m.radius[1]] for m in problem.models]
     t0 = time.perf_counter()
-    om = O.OracleMesh(p.mesh)
-    om.find_neighbours([[m.radius[0], m.radius[1]] for m in p.models])
-    f, xD, inner = om.boundary(p.bcs, 1)
-    K = om.assemble(0, p.models, p.materials, inner, faithful_reset=True)
-    f = O.first_kind_rhs(K, inner, xD, f)
+    om = O.OracleMesh(problem.mesh)
+    om.find_neighbours(radii)
     t1 = time.perf_counter()
-    KL = om.assemble(0, p.models, p.materials, inner, part=A.NLF_PART_LOCAL, faithful_reset=True)
-    P = O.OraclePrecond(A.NLF_PRECOND_ILLT, KL)
-    x, it, res = O.cg(K, P, f, tol=tol, max_it=100000)
+    f, xD, inner = om.boundary(problem.bcs, 1)
+    K = om.assemble(0, problem.models, problem.materials, inner, faithful_reset=True)
+    om.convection(K, problem.bcs, inner)
+    f = O.first_kind_rhs(K, inner, xD, f)
     t2 = time.perf_counter()
-    ns = p.mesh.n_nodes
-    scale = full_nodes / ns
-    est = (t1 - t0) * scale + (t2 - t1) * scale * (full_iterations / max(it, 1))
-    sample = (f"oracle (reference algorithm restated, OpenMP, faithful O(N^2) flag resets) on the "
-              f"{n_sample}x{n_sample} composite with the same r/h = 10: assembly {t1 - t0:.2f} s, ILLT-CG "
-              f"{t2 - t1:.2f} s ({it} iterations); scaled linearly by nodes x{scale:.1f} and by iterations "
-              f"x{full_iterations / max(it, 1):.2f} to the full workload (a lower bound: the flag resets are O(N^2))")
-    return dict(value=est, unit="s", cores=cores, kind="port", sample=sample,
-                measured_seconds=round(t2 - t0, 3))
+    KL = om.assemble(0, problem.models, problem.materials, inner, part=A.NLF_PART_LOCAL, faithful_reset=True)
+    P = O.OraclePrecond(A.NLF_PRECOND_ILLT, KL)
+    del KL
+    t3 = time.perf_counter()
+    _, it, _ = O.cg(K, P, f, tol=1e-300, max_it=cg_iterations)
+    t4 = time.perf_counter()
+    per_it = (t4 - t3) / max(it, 1)
+    value = (t1 - t0) + (t2 - t1) + (t3 - t2) + per_it * plain_iterations
+    # the GPU result against the oracle's system
+    x = check["x"]
+    res = float(np.linalg.norm(f - K.spmv(x)) / np.linalg.norm(f))
+    kv = K.spmv(check["v"])
+    kv_err = float(np.linalg.norm(check["Kv"] - kv) / np.linalg.norm(kv))
+    worst = 0.0
+    nnz_checked = 0
+    for r0, r1, g in check["rows"]:
+        o = K.export_rows(r0, r1)
+        if not (np.array_equal(o.outer, g.outer) and np.array_equal(o.inner, g.inner)):
+            raise RuntimeError(f"GPU pattern of rows [{r0}, {r1}) differs from the oracle's")
+        worst = max(worst, float(np.abs(o.val - g.val).max() / np.abs(o.val).max()))
+        nnz_checked += o.val.size
+    sample = (f"oracle on the full {problem.mesh.n_nodes}-node composite, {cores} OpenMP threads: neighbours "
+              f"{t1 - t0:.2f} s, assembly of K with boundary conditions {t2 - t1:.2f} s, local matrix and IC(0) "
+              f"{t3 - t2:.2f} s, ILLT-CG {t4 - t3:.2f} s for {it} iterations ({per_it * 1e3:.1f} ms each) "
+              f"scaled to the {plain_iterations} iterations of the same IC(0)-CG")
+    return dict(value=round(value, 3), unit="s", cores=cores, kind="port", sample=sample,
+                measured_seconds=round(t4 - t0, 3), host=host_description(),
+                phases={"neighbours": round(t1 - t0, 3), "assembly": round(t2 - t1, 3),
+                        "preconditioner": round(t3 - t2, 3), "cg_per_iteration": round(per_it, 5),
+                        "cg_estimated": round(per_it * plain_iterations, 3)},
+                oracle_check={"gpu_solution_residual_on_oracle_system": res,
+                              "Kv_rel_diff_gpu_vs_oracle": kv_err,
+                              "sampled_rows_max_rel_entry_diff": worst, "sampled_nonzeros": int(nnz_checked)})
 
 
 def bench_batch(args):
@@ -420,7 +470,9 @@ def main():
     log(f"[bench] plain: {t_plain:.3f} s, it {ref.iterations}, true res {ref.true_residual:.2e}, "
         f"timings {ref.timings}")
     results = {"plain_fp64_illt": dict(seconds=t_plain, iterations=int(ref.iterations),
-                                       true_residual=ref.true_residual)}
+                                       true_residual=ref.true_residual,
+                                       phases={k: round(v, 4) for k, v in ref.timings.items()})}
+    del ref                                   # its matrices and factor are several GB: free them now
     best = None
     for name in [m for m in args.methods.split(",") if m]:
         pc, mode = methods[name]
@@ -429,7 +481,8 @@ def main():
         log(f"[bench] {name}: {t:.3f} s, it {sol.iterations}, outer {sol.stats.outer_iterations}, "
             f"true res {sol.true_residual:.2e}, rel err vs FP64 {err:.2e}, timings {sol.timings}")
         results[name] = dict(seconds=t, iterations=int(sol.iterations), true_residual=sol.true_residual,
-                             rel_err_vs_fp64=err)
+                             rel_err_vs_fp64=err, phases={k: round(v, 4) for k, v in sol.timings.items()})
+        del sol
         if err <= 1e-8 and (best is None or t < results[best]["seconds"]):
             best = name
     if best is None:
@@ -439,18 +492,36 @@ def main():
     for _ in range(args.warmup):
         run_step(api, A, mesh, problem, pc, mode, args.tol, coarse_of.get(best))
     torch.cuda.synchronize()
+    # Each step's Solution (matrix, local matrix, preconditioner) is released before the next step
+    # starts, so every step runs with the same free device memory; only the last step's results are
+    # kept for the line.
+    step_s = []
+    last = None
     t0 = time.perf_counter()
-    sols = []
-    for _ in range(args.steps):
-        sol, _t = run_step(api, A, mesh, problem, pc, mode, args.tol, coarse_of.get(best))
-        sols.append(sol)
+    for k in range(args.steps):
+        last = None
+        sol, t_step = run_step(api, A, mesh, problem, pc, mode, args.tol, coarse_of.get(best))
+        step_s.append(t_step)
+        if k == args.steps - 1:
+            last = dict(x=sol.x.clone(), info=sol.matrix.info(), stats=sol.stats, timings=dict(sol.timings),
+                        iterations=int(sol.iterations), true_residual=sol.true_residual)
+            if not args.no_cpu:
+                # the last step's matrix, for the oracle check after the timed region: K v for one
+                # random v and a few sampled row ranges of the upper CSR
+                n_rows = last["info"]["rows"]
+                v = torch.from_numpy(np.random.default_rng(5).standard_normal(n_rows)).to(sol.x.device)
+                starts = np.linspace(0, n_rows - 64, 48).astype(np.int64)
+                last["check"] = dict(v=v.cpu().numpy(), Kv=sol.matrix.spmv(v).cpu().numpy(),
+                                     x=sol.x.cpu().numpy(),
+                                     rows=[(int(r), int(r) + 64, sol.matrix.export_rows(int(r), int(r) + 64))
+                                           for r in starts])
+        del sol
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
     sec_per_step = elapsed / args.steps
-    last = sols[-1]
-    err = float(torch.linalg.norm(last.x - x_ref) / torch.linalg.norm(x_ref))
-    info = last.matrix.info()
-    st = last.stats
+    err = float(torch.linalg.norm(last["x"] - x_ref) / torch.linalg.norm(x_ref))
+    info = last["info"]
+    st = last["stats"]
     rows = info["rows"]
     sv = {A.NLF_MODE_MIXED: 4, A.NLF_MODE_SPLIT: 6}.get(mode, 8)      # matrix value bytes (vectors stay FP64)
     sx = 8
@@ -460,8 +531,8 @@ def main():
     cpu = None
     if not args.no_cpu:
         log("[bench] CPU baseline (oracle)")
-        cpu = cpu_baseline(args.cpu_sample_n, problem.mesh.n_nodes, results["plain_fp64_illt"]["iterations"],
-                           args.tol)
+        cpu = cpu_baseline(problem, results["plain_fp64_illt"]["iterations"], last["check"], args.cpu_cg_iterations)
+        log(f"[bench] CPU baseline: {cpu['value']} s ({cpu['sample']}); oracle check {cpu['oracle_check']}")
     line = {
         "metric": "assembly+solve s & %roofline, learned/mixed vs plain FP64 GPU at <=1e-8 rel err",
         "value": round(sec_per_step, 4),
@@ -479,13 +550,15 @@ def main():
                                f"bell r=0.01, solve to {args.tol:g} true residual",
                    "method": best, "nodes": int(problem.mesh.n_nodes), "nnz_upper": int(info["nnz_upper"]),
                    "nnz_full": int(info["nnz_full"])},
+        "step_seconds": {"mean": round(sec_per_step, 4), "median": round(float(np.median(step_s)), 4),
+                         "min": round(min(step_s), 4), "max": round(max(step_s), 4)},
         "plain_fp64_illt_s": round(t_plain, 4),
         "speedup_vs_plain_fp64_illt": round(t_plain / sec_per_step, 3),
         "rel_err_vs_fp64": err,
-        "true_residual": last.true_residual,
-        "iterations": int(last.iterations),
+        "true_residual": last["true_residual"],
+        "iterations": last["iterations"],
         "methods": results,
-        "phase_seconds": {k: round(v, 4) for k, v in last.timings.items()},
+        "phase_seconds": {k: round(v, 4) for k, v in last["timings"].items()},
         "precond_apply_ms": round(st.precond_ms, 4),
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": counter_traffic(args.n, sv == 8),

@@ -78,10 +78,24 @@ typedef struct nlf_thermal_material {
     double  c[3];              /* iso: c[0]; ortho: {l11, l22}; aniso: {l11, l22, l12} */
 } nlf_thermal_material;
 
+#define NLF_ELASTIC_ISOTROPIC   0      /* isotropic_elastic_parameters (elastic_parameters.hpp:62-76) */
+#define NLF_ELASTIC_ORTHOTROPIC 1      /* orthotropic_elastic_parameters (elastic_parameters.hpp:78-96) */
+#define NLF_ELASTIC_ANISOTROPIC 2      /* anisotropic_elastic_parameters: orthotropic rotated by an angle (:98-133) */
+
+#define NLF_EXPANSION_NONE        0    /* no thermal expansion: the group takes no thermal load */
+#define NLF_EXPANSION_ISOTROPIC   1    /* thermal_expansion_parameters.hpp:7-18: alpha */
+#define NLF_EXPANSION_ORTHOTROPIC 2    /* {a_xx, a_yy} */
+#define NLF_EXPANSION_ANISOTROPIC 3    /* {a_xx, a_yy, a_xy} */
+
+/* Plane-stress material of one element group (raw_mechanical_parameters_t, mechanical_parameters_2d.hpp:8-12).
+ * Isotropic: E = young, nu = poisson.  Orthotropic: E = {young, young_y}, nu = {poisson, poisson_y},
+ * G = shear.  Anisotropic: the orthotropic Hooke matrix rotated by `angle` (radians). */
 typedef struct nlf_elastic_material {
-    int32_t kind;              /* 0 = isotropic plane stress (elastic_parameters.hpp:62-76) */
-    int32_t pad_;
+    int32_t kind;              /* NLF_ELASTIC_* */
+    int32_t expansion_kind;    /* NLF_EXPANSION_* */
     double  young, poisson;
+    double  young_y, poisson_y, shear, angle;
+    double  expansion[3];      /* thermal expansion coefficients of expansion_kind */
 } nlf_elastic_material;
 
 /* One boundary-group condition; for elasticity `value` holds the two components. */
@@ -153,6 +167,26 @@ int nlf_neighbours_device(nlf_mesh m, const int64_t** offsets, const int32_t** i
  * materials = nlf_thermal_material[G] or nlf_elastic_material[G], is_inner[DoF*N] host mask. */
 int nlf_assemble(nlf_mesh m, int physics, const nlf_model* models, const void* materials, const uint8_t* is_inner,
                  int symmetric, int part, nlf_matrix* out);
+/* Options of conductivity_matrix_2d::compute / stiffness_matrix::compute beyond nlf_assemble's.
+ * neumann: is_neumann of conductivity_matrix_2d::compute (conductivity_matrix_2d.hpp:39-72, 83-104):
+ *   thermal only and no first-kind row; the matrix gets the Lagrange row and column N (rows = N + 1):
+ *   K(r, N) = int N_r, K(N, N) = 0.  The right-hand side then has N + 1 entries, f[N] = the energy
+ *   (stationary_heat_equation_solver_2d.hpp:28-44), and CG runs on the indefinite bordered system.
+ * coefficients: NULL, or per element group NULL (the material's constant) or that group's values at
+ *   its quadrature points (evaluated_parameters' Variable alternative; evaluate_conductivity.hpp:16-37,
+ *   evaluate_mechanical_parameters.hpp:9-30): index ((e - group_begin) * nq + q) * ncomp + k, ncomp
+ *   1 / 2 / 3 for isotropic / orthotropic / anisotropic conductivity {l11, l22, l12}, 3 / 4 / 6 for the
+ *   Hooke matrix of an isotropic / orthotropic / anisotropic material in the reference's index order
+ *   ({11, 12, 66}, {11, 12, 66, 22}, {11, 12, 66, 22, 16, 26}).  Only for problems whose groups are all
+ *   local: varying coefficients make a nonlocal problem asymmetric (problem_settings.hpp:32-34), which
+ *   the reference solves with BiCGSTAB (NLF_ERR_UNSUPPORTED here). */
+typedef struct nlf_assemble_options {
+    int32_t neumann;
+    int32_t pad_;
+    const double* const* coefficients;
+} nlf_assemble_options;
+int nlf_assemble_ex(nlf_mesh m, int physics, const nlf_model* models, const void* materials, const uint8_t* is_inner,
+                    int symmetric, int part, const nlf_assemble_options* options, nlf_matrix* out);
 int nlf_matrix_destroy(nlf_matrix a);
 int nlf_matrix_info(nlf_matrix a, int64_t* rows, int64_t* nnz_upper, int64_t* nnz_full, int64_t* nnz_bound);
 int nlf_matrix_timings(nlf_matrix a, double* pattern_seconds, double* values_seconds);
@@ -160,6 +194,9 @@ int nlf_matrix_timings(nlf_matrix a, double* pattern_seconds, double* values_sec
  * part 0 = inner upper CSR, 1 = bound CSR, 2 = inner full storage; 3 and 4 = parts 0 and 2 of the
  * FP32 values (nlf_matrix_make_fp32 / nlf_matrix_assemble_fp32), widened to double */
 int nlf_matrix_export(nlf_matrix a, int part, int64_t* outer, int64_t* inner, double* values);
+/* The inner upper CSR of rows [row0, row1) only (finite_element_matrix::inner() rows, as
+ * SparseMatrix::middleRows would give them): outer[row1 - row0 + 1] from 0, inner/values may be NULL. */
+int nlf_matrix_export_rows(nlf_matrix a, int64_t row0, int64_t row1, int64_t* outer, int64_t* inner, double* values);
 int nlf_matrix_device(nlf_matrix a, const int64_t** outer, const int32_t** cols, double** values, const int64_t** diag);
 /* No reference counterpart.  Row-pattern dictionary of the SpMV (NLF_OPT_ROW_PATTERNS): number of
  * distinct offset rows and their total length (both 0 when the per-nonzero offsets are used). */
@@ -202,13 +239,21 @@ int nlf_right_part(nlf_mesh m, int dof, const double* values_dev, double* f_dev)
 int nlf_heat_flux(nlf_mesh m, const nlf_model* models, const nlf_thermal_material* materials, const double* T_dev,
                   double* flux_dev, double* qflux_dev);
 /* ---- strains and stresses: replaces mechanical_solution_2d::calc_strain_and_stress
- * (solvers/solver_2d/mechanical/mechanical_solution_2d.hpp:192-232) without thermal strains.  u_dev[2N]
- * the displacement as the solve returns it (x, y per node); strain_dev[3N] and stress_dev[3N] node
- * values (xx, yy, xy blocks; xy strain is the tensor shear (u_x,y + u_y,x)/2, xy stress 2 h66 eps_xy);
+ * (solvers/solver_2d/mechanical/mechanical_solution_2d.hpp:192-232).  u_dev[2N] the displacement as the
+ * solve returns it (x, y per node); delta_temperature_dev[N] the node temperature change or NULL: its
+ * thermal strain (alpha dT of each group's expansion) is subtracted before the stresses
+ * (substract_temperature_strains :135-166); strain_dev[3N] and stress_dev[3N] node values (xx, yy, xy
+ * blocks; xy strain is the tensor shear (u_x,y + u_y,x)/2, xy stress 2 h66 eps_xy + h16 eps_xx + h26 eps_yy);
  * qstress_dev[3*E*nq] quadrature-point stresses after the nonlocal average.  Any output may be NULL.
  * A nonlocal group needs nlf_find_neighbours first (NLF_ERR_STATE otherwise). */
 int nlf_strain_stress(nlf_mesh m, const nlf_model* models, const nlf_elastic_material* materials, const double* u_dev,
-                      double* strain_dev, double* stress_dev, double* qstress_dev);
+                      const double* delta_temperature_dev, double* strain_dev, double* stress_dev, double* qstress_dev);
+/* ---- thermal load: replaces temperature_condition (solvers/solver_2d/mechanical/temperature_condition_2d.hpp:
+ * 128-170).  f_dev[2N] += the load of the node temperature change delta_temperature_dev[N] for the groups
+ * with a thermal expansion: p1 times the local integral of grad N_i . C alpha dT plus (1 - p1) times the
+ * nonlocal one over the neighbour lists, in the reference's form. */
+int nlf_temperature_condition(nlf_mesh m, const nlf_model* models, const nlf_elastic_material* materials,
+                              const double* delta_temperature_dev, double* f_dev);
 
 /* inner_nodes (base/solvers_utils.hpp:31-40), boundary_condition_second_kind_2d
  * (base/boundary_condition_second_kind_2d.hpp:12-33) and the first-kind values (:17-26), host arrays */

@@ -12,6 +12,8 @@ NLF_EXPAND_NO, NLF_EXPAND_MAX, NLF_EXPAND_MIN, NLF_EXPAND_MEAN = 0, 1, 2, 3
 NLF_PART_LOCAL, NLF_PART_FULL = 0, 2
 NLF_CONDUCTIVITY_ISOTROPIC, NLF_CONDUCTIVITY_ORTHOTROPIC, NLF_CONDUCTIVITY_ANISOTROPIC = 0, 1, 2
 NLF_BC_NONE, NLF_BC_FIRST_KIND, NLF_BC_FLUX, NLF_BC_CONVECTION = 0, 1, 2, 3
+NLF_ELASTIC_ISOTROPIC, NLF_ELASTIC_ORTHOTROPIC, NLF_ELASTIC_ANISOTROPIC = 0, 1, 2
+NLF_EXPANSION_NONE, NLF_EXPANSION_ISOTROPIC, NLF_EXPANSION_ORTHOTROPIC, NLF_EXPANSION_ANISOTROPIC = 0, 1, 2, 3
 NLF_PRECOND_IDENTITY, NLF_PRECOND_JACOBI, NLF_PRECOND_ILLT, NLF_PRECOND_LEARNED = 0, 1, 2, 3
 NLF_MODE_FP64, NLF_MODE_MIXED, NLF_MODE_SPLIT = 0, 1, 2
 NLF_OPT_ASSEMBLY_PAIRS = 1
@@ -31,7 +33,14 @@ class ThermalMaterial(C.Structure):
 
 
 class ElasticMaterial(C.Structure):
-    _fields_ = [("kind", C.c_int32), ("pad_", C.c_int32), ("young", C.c_double), ("poisson", C.c_double)]
+    _fields_ = [("kind", C.c_int32), ("expansion_kind", C.c_int32), ("young", C.c_double), ("poisson", C.c_double),
+                ("young_y", C.c_double), ("poisson_y", C.c_double), ("shear", C.c_double), ("angle", C.c_double),
+                ("expansion", C.c_double * 3)]
+
+
+class AssembleOptions(C.Structure):
+    """nlf_assemble_options: the Neumann border and coefficients per quadrature point."""
+    _fields_ = [("neumann", C.c_int32), ("pad_", C.c_int32), ("coefficients", C.POINTER(C.POINTER(C.c_double)))]
 
 
 class Boundary(C.Structure):
@@ -71,12 +80,44 @@ def thermal_material(kind=NLF_CONDUCTIVITY_ISOTROPIC, c=(1.0, 0.0, 0.0)):
     return t
 
 
-def elastic_material(young=1.0, poisson=0.3):
+def elastic_material(young=1.0, poisson=0.3, expansion=None):
+    """Isotropic plane-stress material (isotropic_elastic_parameters); `expansion` an optional thermal
+    expansion coefficient: a scalar (isotropic), a pair (orthotropic) or a triple (anisotropic)."""
     e = ElasticMaterial()
-    e.kind = 0
+    e.kind = NLF_ELASTIC_ISOTROPIC
     e.young = float(young)
     e.poisson = float(poisson)
+    _set_expansion(e, expansion)
     return e
+
+
+def orthotropic_material(young, poisson, shear, expansion=None):
+    """orthotropic_elastic_parameters: young = (E_x, E_y), poisson = (nu_0, nu_1), shear = G."""
+    e = ElasticMaterial()
+    e.kind = NLF_ELASTIC_ORTHOTROPIC
+    e.young, e.young_y = float(young[0]), float(young[1])
+    e.poisson, e.poisson_y = float(poisson[0]), float(poisson[1])
+    e.shear = float(shear)
+    _set_expansion(e, expansion)
+    return e
+
+
+def anisotropic_material(young, poisson, shear, angle, expansion=None):
+    """anisotropic_elastic_parameters: the orthotropic material rotated by `angle` (radians)."""
+    e = orthotropic_material(young, poisson, shear, expansion)
+    e.kind = NLF_ELASTIC_ANISOTROPIC
+    e.angle = float(angle)
+    return e
+
+
+def _set_expansion(e, expansion):
+    if expansion is None:
+        e.expansion_kind = NLF_EXPANSION_NONE
+        return
+    v = [expansion] if not hasattr(expansion, "__len__") else list(expansion)
+    e.expansion_kind = {1: NLF_EXPANSION_ISOTROPIC, 2: NLF_EXPANSION_ORTHOTROPIC, 3: NLF_EXPANSION_ANISOTROPIC}[len(v)]
+    for i, x in enumerate(v):
+        e.expansion[i] = float(x)
 
 
 def boundary(kind=NLF_BC_NONE, value=0.0, kind2=NLF_BC_NONE, value2=0.0, heat_transfer=0.0, ambient=0.0):

@@ -41,10 +41,13 @@ SIGNATURES = {
     "nlf_neighbours_export": (_int, [_p, _i64p, _i32p]),
     "nlf_neighbours_device": (_int, [_p, _pp, _pp]),
     "nlf_assemble": (_int, [_p, _int, C.POINTER(_abi.Model), _p, _u8p, _int, _int, _pp]),
+    "nlf_assemble_ex": (_int, [_p, _int, C.POINTER(_abi.Model), _p, _u8p, _int, _int, C.POINTER(_abi.AssembleOptions),
+                               _pp]),
     "nlf_matrix_destroy": (_int, [_p]),
     "nlf_matrix_info": (_int, [_p, _i64p, _i64p, _i64p, _i64p]),
     "nlf_matrix_timings": (_int, [_p, _f64p, _f64p]),
     "nlf_matrix_export": (_int, [_p, _int, _i64p, _i64p, _f64p]),
+    "nlf_matrix_export_rows": (_int, [_p, _i64, _i64, _i64p, _i64p, _f64p]),
     "nlf_matrix_device": (_int, [_p, _pp, _pp, _pp, _pp]),
     "nlf_matrix_add_pairs": (_int, [_p, _i64, _i64p, _f64p]),
     "nlf_matrix_add_convection": (_int, [_p, C.POINTER(_abi.Boundary)]),
@@ -60,7 +63,8 @@ SIGNATURES = {
     "nlf_heat_flux": (_int, [_p, C.POINTER(_abi.Model), C.POINTER(_abi.ThermalMaterial), _p, _p, _p]),
     "nlf_matrix_combine": (_int, [_p, _dbl, _p, _dbl]),
     "nlf_right_part": (_int, [_p, _int, _p, _p]),
-    "nlf_strain_stress": (_int, [_p, C.POINTER(_abi.Model), C.POINTER(_abi.ElasticMaterial), _p, _p, _p, _p]),
+    "nlf_strain_stress": (_int, [_p, C.POINTER(_abi.Model), C.POINTER(_abi.ElasticMaterial), _p, _p, _p, _p, _p]),
+    "nlf_temperature_condition": (_int, [_p, C.POINTER(_abi.Model), C.POINTER(_abi.ElasticMaterial), _p, _p]),
     "nlf_boundary_data_fn": (_int, [_p, _int, C.POINTER(_abi.Boundary), _p, _p, _f64p, _f64p, _u8p]),
     "nlf_spmv": (_int, [_p, _p, _p]),
     "nlf_spmv_fp32": (_int, [_p, _p, _p]),

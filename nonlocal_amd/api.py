@@ -199,6 +199,17 @@ class finite_element_matrix:
         self.ctx.check(load().nlf_matrix_export(self.h, part, _ptr(outer, _i64p), _ptr(inner, _i64p), _ptr(val, _f64p)))
         return CSRHost(outer, inner[:nnz], val[:nnz])
 
+    def export_rows(self, row0, row1) -> CSRHost:
+        """Rows [row0, row1) of the inner upper CSR, offsets from 0 (only those rows are copied)."""
+        outer = np.zeros(row1 - row0 + 1, dtype=np.int64)
+        self.ctx.check(load().nlf_matrix_export_rows(self.h, row0, row1, _ptr(outer, _i64p), None, None))
+        nnz = int(outer[-1])
+        inner = np.zeros(max(nnz, 1), dtype=np.int64)
+        val = np.zeros(max(nnz, 1))
+        self.ctx.check(load().nlf_matrix_export_rows(self.h, row0, row1, _ptr(outer, _i64p), _ptr(inner, _i64p),
+                                                     _ptr(val, _f64p)))
+        return CSRHost(outer, inner[:nnz], val[:nnz])
+
     def inner(self):
         return self.export(0)
 
@@ -486,7 +497,7 @@ class mechanical_solution_2d:
     Thermal strains (a temperature change) are not included.
     """
 
-    def __init__(self, mesh: mesh_2d, models, materials, displacement: torch.Tensor):
+    def __init__(self, mesh: mesh_2d, models, materials, displacement: torch.Tensor, delta_temperature=None):
         n = mesh.nodes_count
         if displacement.numel() < 2 * n:
             raise NLFError(_abi.NLF_ERR_ARGUMENT, "The displacement vector is shorter than twice the number of nodes.")
@@ -494,7 +505,12 @@ class mechanical_solution_2d:
         self._models = list(models)
         self._materials = list(materials)
         self._u = displacement.reshape(-1)[:2 * n].to(device=mesh.ctx.device, dtype=torch.float64).contiguous()
+        self._dT = _node_vector(mesh, delta_temperature)
         self._strain = self._stress = self._qstress = None
+
+    def delta_temperature(self):
+        """The node temperature change whose thermal strains the stresses exclude (None: no thermal strain)."""
+        return self._dT
 
     def displacement(self):
         """(u_x, u_y) node vectors."""
@@ -529,11 +545,35 @@ class mechanical_solution_2d:
         q = torch.empty(3 * E * nq, dtype=torch.float64, device=dev)
         marr = _abi.array_of(_abi.Model, self._models)
         matarr = _abi.array_of(_abi.ElasticMaterial, self._materials)
-        mesh.ctx.check(load().nlf_strain_stress(mesh.h, marr, matarr, _dev(self._u), _dev(strain), _dev(stress),
-                                                _dev(q)))
+        mesh.ctx.check(load().nlf_strain_stress(mesh.h, marr, matarr, _dev(self._u),
+                                                _dev(self._dT) if self._dT is not None else None, _dev(strain),
+                                                _dev(stress), _dev(q)))
         self._strain = tuple(strain[k * n:(k + 1) * n] for k in range(3))
         self._stress = tuple(stress[k * n:(k + 1) * n] for k in range(3))
         self._qstress = tuple(q[k * E * nq:(k + 1) * E * nq] for k in range(3))
+
+
+def _node_vector(mesh: mesh_2d, v):
+    """A node vector (array, tensor or None) as an FP64 device tensor of length N (None stays None)."""
+    if v is None:
+        return None
+    t = v if isinstance(v, torch.Tensor) else torch.from_numpy(np.ascontiguousarray(v, dtype=np.float64))
+    if t.numel() != mesh.nodes_count:
+        raise NLFError(_abi.NLF_ERR_ARGUMENT, "It is impossible to calculate temperature deformations due to "
+                                              "different dimensions of the mesh and the delta temperature vector.")
+    return t.reshape(-1).to(device=mesh.ctx.device, dtype=torch.float64).contiguous()
+
+
+def temperature_condition(mesh: mesh_2d, models, materials, delta_temperature, f: torch.Tensor):
+    """f (2N, x and y per node) += the thermal load of the node temperature change
+    (temperature_condition, mechanical/temperature_condition_2d.hpp:128-170)."""
+    dT = _node_vector(mesh, delta_temperature)
+    if any(m.local_weight < 0.999 for m in models) and mesh.influences is None:
+        find_neighbours(mesh, _radii(models))
+    marr = _abi.array_of(_abi.Model, models)
+    matarr = _abi.array_of(_abi.ElasticMaterial, materials)
+    mesh.ctx.check(load().nlf_temperature_condition(mesh.h, marr, matarr, _dev(dT), _dev(f)))
+    return f
 
 
 def boundary_data(mesh: mesh_2d, dof, bcs, values=None):
@@ -564,7 +604,7 @@ def _radii(models):
 
 
 def _solve_driver(mesh: mesh_2d, physics, models, materials, bcs, precond, tol, max_it, mode, weights=None,
-                  assembly32=False, coarse=None, values=None, source=None):
+                  assembly32=False, coarse=None, values=None, source=None, delta_temperature=None):
     import time
     ctx = mesh.ctx
     dof = 1 if physics == 0 else 2
@@ -581,6 +621,8 @@ def _solve_driver(mesh: mesh_2d, physics, models, materials, bcs, precond, tol, 
     fd = torch.from_numpy(f).to(ctx.device)
     if source is not None:
         right_part(mesh, dof, source, fd)
+    if delta_temperature is not None and physics == 1:
+        temperature_condition(mesh, models, materials, delta_temperature, fd)
     xDd = torch.from_numpy(xD).to(ctx.device)
     K.apply_first_kind(xDd, fd)
     ctx.synchronize()
@@ -627,7 +669,10 @@ def _solve_driver(mesh: mesh_2d, physics, models, materials, bcs, precond, tol, 
     ctx.synchronize()
     tm["solve"] = time.perf_counter() - t3
     st = cg.stats
-    post = (heat_equation_solution_2d if physics == 0 else mechanical_solution_2d)(mesh, models, materials, x)
+    if physics == 0:
+        post = heat_equation_solution_2d(mesh, models, materials, x)
+    else:
+        post = mechanical_solution_2d(mesh, models, materials, x, delta_temperature)
     return Solution(x=x, iterations=st.iterations, residual=st.residual, true_residual=st.true_residual, matrix=K,
                     local_matrix=KL, rhs=fd, timings=tm, stats=st, solution=post)
 
@@ -649,14 +694,19 @@ def stationary_heat_equation_solver_2d(mesh: mesh_2d, models, materials, bcs, pr
 
 
 def equilibrium_equation(mesh: mesh_2d, models, materials, bcs, precond=None, tol=1e-12, max_it=100000,
-                         mode=NLF_MODE_FP64, assembly32=False, values=None, source=None) -> Solution:
-    """Plane-stress static elasticity; ILLT of the local stiffness when nonlocal, identity otherwise.
-    values: optional callable (x, y, group, dof) -> displacement or pressure of a boundary group at a
-    point.  The solution's `solution` is a mechanical_solution_2d."""
+                         mode=NLF_MODE_FP64, assembly32=False, values=None, source=None,
+                         delta_temperature=None) -> Solution:
+    """Plane-stress static elasticity (equilibrium_equation, mechanical/equilibrium_equation_2d.hpp:20-62);
+    ILLT of the local stiffness when nonlocal, identity otherwise.  materials: nlf_elastic_material per
+    group (isotropic, orthotropic or anisotropic Hooke, optional thermal expansion).  values: optional
+    callable (x, y, group, dof) -> displacement or pressure of a boundary group at a point.
+    delta_temperature: node temperature changes; their thermal load enters the right-hand side and
+    their thermal strains are left out of the stresses.  The solution's `solution` is a
+    mechanical_solution_2d."""
     if precond is None:
         precond = NLF_PRECOND_ILLT if any(m.local_weight < 0.999 for m in models) else NLF_PRECOND_IDENTITY
     return _solve_driver(mesh, 1, models, materials, bcs, precond, tol, max_it, mode, assembly32=assembly32,
-                         values=values, source=source)
+                         values=values, source=source, delta_temperature=delta_temperature)
 
 
 class nonstationary_heat_equation_solver_2d:

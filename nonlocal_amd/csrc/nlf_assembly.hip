@@ -11,7 +11,9 @@
 // handles one node, visiting its elements eL and their neighbours eNL.  Values are accumulated in
 // LDS in barrier-separated rounds (element slot, neighbour batch, local column index j): within a
 // round two contributions never share a position because e -> node(e, j) is injective, so the sum
-// order is fixed and the result is bitwise reproducible.  No atomics touch the matrix.
+// order is fixed and the result is bitwise reproducible.  On a mesh where one node holds the same
+// local index in two elements the rounds go by the element's rank at the node instead, with the
+// same property.  No atomics touch the matrix.
 #include "nlf_internal.hpp"
 
 #include <algorithm>
@@ -42,6 +44,43 @@ GroupParams model_group_params(const nlf_model& md, bool only_local) {
         gp.inf = make_influence(m2);
     }
     return gp;
+}
+
+void elastic_group_params(const nlf_elastic_material& e, GroupParams& gp) {
+    for (int k = 0; k < 6; ++k) gp.c[k] = 0.0;
+    if (e.kind == NLF_ELASTIC_ISOTROPIC) {
+        const double value = e.young / (1.0 - e.poisson * e.poisson);           // :70-75
+        gp.c[0] = value; gp.c[1] = e.poisson * value; gp.c[2] = 0.5 * value * (1.0 - e.poisson);
+        gp.c[3] = value;
+    } else {
+        NLF_CHECK(e.kind == NLF_ELASTIC_ORTHOTROPIC || e.kind == NLF_ELASTIC_ANISOTROPIC, NLF_ERR_ARGUMENT,
+                  "unknown elastic material kind");
+        const double value = 1.0 / (1.0 - e.poisson * e.poisson_y);            // :89-95
+        const double m11 = e.young * value, m12 = e.young * e.poisson_y * value, m66 = e.shear;
+        const double m22 = e.young_y * value;
+        if (e.kind == NLF_ELASTIC_ORTHOTROPIC) {
+            gp.c[0] = m11; gp.c[1] = m12; gp.c[2] = m66; gp.c[3] = m22;
+        } else {                                                                // rotate, :102-123
+            const double a2 = e.angle + e.angle;
+            const double s2 = std::sin(a2), c2 = std::cos(a2);
+            const double c4 = 0.125 * (c2 * c2 - s2 * s2);
+            const double diff = m11 - m22, core = m11 + m22 - 2 * m12 - 4 * m66;
+            const double plus = core * (c4 + 0.125), minus = core * (0.125 - c4);
+            const double cdiff = diff * (c2 + 0.5), minor = core * c2, s4 = 0.25 * s2;
+            gp.c[0] = plus + 0.5 * (m12 + m22 + cdiff) + m66;
+            gp.c[1] = minus + m12;
+            gp.c[2] = minus + m66;
+            gp.c[3] = plus + 0.5 * (m12 + m11 - cdiff) + m66;
+            gp.c[4] = s4 * (diff + minor);
+            gp.c[5] = s4 * (diff - minor);
+        }
+    }
+    NLF_CHECK(e.expansion_kind >= NLF_EXPANSION_NONE && e.expansion_kind <= NLF_EXPANSION_ANISOTROPIC,
+              NLF_ERR_ARGUMENT, "unknown thermal expansion kind");
+    gp.ekind = e.expansion_kind;
+    gp.alpha[0] = e.expansion[0];
+    gp.alpha[1] = e.expansion_kind == NLF_EXPANSION_ISOTROPIC ? e.expansion[0] : e.expansion[1];
+    gp.alpha[2] = e.expansion_kind == NLF_EXPANSION_ANISOTROPIC ? e.expansion[2] : 0.0;
 }
 
 int influence_class(const std::vector<GroupParams>& gp) {
@@ -357,6 +396,9 @@ struct AssemblyArgs {
     const i64* eoff;
     i64 node0;           // nodes [node0, node0 + grid) of this launch
     int mass;            // heat capacity matrix: rho c int N_i N_j (groups' c[0] = rho c)
+    const double* cq;        // coefficients per quadrature point [(e * NQ + q) * 6 + k] of groups with varying
+    const uint8_t* el_rank;  // non-injective meshes: rank of element e in its node j's element list
+    int rounds;              // and the number of ranks (most elements at one node)
     double qN[MAX_NN * MAX_NQ];
 };
 
@@ -393,8 +435,13 @@ __device__ void local_row(const AssemblyArgs& a, const GroupParams& g, i64 e, in
         for (int j = 0; j < NN; ++j) v[j][0] *= T(g.c[0]);
         return;
     }
-    const T c0 = T(g.c[0]), c1 = T(g.c[1]), c2 = T(g.c[2]);
     for (int q = 0; q < NQ; ++q) {
+        // the coefficient at the quadrature point: the group's constant, or its value per point
+        // (evaluated_parameters' Variable alternative, conductivity_matrix_2d.hpp:117-118)
+        double cp[6];
+#pragma unroll
+        for (int k = 0; k < 6; ++k) cp[k] = g.varying ? a.cq[(e * NQ + q) * 6 + k] : g.c[k];
+        const T c0 = T(cp[0]), c1 = T(cp[1]), c2 = T(cp[2]);
         const T f = T(a.w[q] / a.jac[e * NQ + q]);
         const T ix = T(a.D[2 * ((e * NN + i) * NQ + q)]), iy = T(a.D[2 * ((e * NN + i) * NQ + q) + 1]);
         for (int j = 0; j < NN; ++j) {
@@ -406,12 +453,16 @@ __device__ void local_row(const AssemblyArgs& a, const GroupParams& g, i64 e, in
                 else value = ix * (c0 * jx + c2 * jy) + iy * (c2 * jx + c1 * jy);
                 v[j][0] += f * value;
             } else {
-                const T h11 = c0, h12 = c1, h66 = c2;
+                // B_i^T C B_j with C = [[h11, h12, h16], [h12, h22, h26], [h16, h26, h66]] (the
+                // anisotropic integrand, stiffness_matrix_2d.hpp:66-75; isotropic and orthotropic
+                // matrices have h16 = h26 = 0)
+                const T h11 = c0, h12 = c1, h66 = c2, h22 = T(cp[3]), h16 = T(cp[4]), h26 = T(cp[5]);
                 const T wx = f * ix, wy = f * iy;
-                v[j][0] += h11 * wx * jx + h66 * wy * jy;
-                v[j][1] += h12 * wx * jy + h66 * wy * jx;
-                v[j][2] += h12 * wy * jx + h66 * wx * jy;
-                v[j][DOF * DOF - 1] += h11 * wy * jy + h66 * wx * jx;
+                const T s1 = h66 * wy + h16 * wx, s2 = h66 * wx + h26 * wy;
+                v[j][0] += (h11 * wx + h16 * wy) * jx + s1 * jy;
+                v[j][1] += (h12 * wx + h26 * wy) * jy + s1 * jx;
+                v[j][2] += (h12 * wy + h16 * wx) * jx + s2 * jy;
+                v[j][DOF * DOF - 1] += (h22 * wy + h26 * wx) * jy + s2 * jx;
             }
         }
     }
@@ -448,12 +499,12 @@ __device__ void nonlocal_row(const AssemblyArgs& a, const GroupParams& g, const 
                 if (g.ckind == NLF_CONDUCTIVITY_ISOTROPIC) v[j][0] += c0 * (ux * jx + uy * jy);
                 else if (g.ckind == NLF_CONDUCTIVITY_ORTHOTROPIC) v[j][0] += c0 * ux * jx + c1 * uy * jy;
                 else v[j][0] += ux * (c0 * jx + c2 * jy) + uy * (c2 * jx + c1 * jy);
-            } else {
-                const T h11 = c0, h12 = c1, h66 = c2;
-                v[j][0] += h11 * ux * jx + h66 * uy * jy;
-                v[j][1] += h12 * ux * jy + h66 * uy * jx;
-                v[j][2] += h12 * uy * jx + h66 * ux * jy;
-                v[j][DOF * DOF - 1] += h11 * uy * jy + h66 * ux * jx;
+            } else {                             // stiffness_matrix_2d.hpp:112-117, any Hooke kind
+                const T h11 = c0, h12 = c1, h66 = c2, h22 = T(g.c[3]), h16 = T(g.c[4]), h26 = T(g.c[5]);
+                v[j][0] += (h11 * ux + h16 * uy) * jx + (h16 * ux + h66 * uy) * jy;
+                v[j][1] += (h16 * ux + h66 * uy) * jx + (h12 * ux + h26 * uy) * jy;
+                v[j][2] += (h16 * ux + h12 * uy) * jx + (h66 * ux + h26 * uy) * jy;
+                v[j][DOF * DOF - 1] += (h66 * ux + h26 * uy) * jx + (h26 * ux + h22 * uy) * jy;
             }
         }
     }
@@ -647,7 +698,7 @@ __global__ __launch_bounds__(256) void k_pairs_mfma32(PairArgs<float> a) {
 
 // T = double: the FP64 value pass (inner and bound parts).  T = float: the FP32 value pass with
 // FP32 accumulation into val32 (inner part only).
-template<int DOF, int NN, int NQ, bool DETERMINISTIC, class T, int THREADS = ASM_THREADS, bool PAIRS = false>
+template<int DOF, int NN, int NQ, bool INJECTIVE, class T, int THREADS = ASM_THREADS, bool PAIRS = false>
 __global__ __launch_bounds__(THREADS) void k_assemble(AssemblyArgs a) {
     constexpr bool F64 = sizeof(T) == sizeof(double);
     extern __shared__ unsigned char smem[];
@@ -806,18 +857,39 @@ __global__ __launch_bounds__(THREADS) void k_assemble(AssemblyArgs a) {
                         }
                 }
             }
-            for (int u = 0; u < ITEMS; ++u)
-                for (int j = 0; j < NN; ++j) {
-                    if (DETERMINISTIC) __syncthreads();
-                    if (active[u])
-                        for (int q = 0; q < DOF * DOF; ++q) {
-                            const int p = pos[u][j][q];
-                            if (p >= 0) {
-                                if (DETERMINISTIC) acc[p] += v[u][j][q];
-                                else atomicAdd(&acc[p], v[u][j][q]);
+            if constexpr (INJECTIVE) {
+                // rounds by local index j: e -> el[e][j] is injective, so no two threads of a round
+                // add to one position
+                for (int u = 0; u < ITEMS; ++u)
+                    for (int j = 0; j < NN; ++j) {
+                        __syncthreads();
+                        if (active[u])
+                            for (int q = 0; q < DOF * DOF; ++q) {
+                                const int p = pos[u][j][q];
+                                if (p >= 0) acc[p] += v[u][j][q];
                             }
-                        }
+                    }
+            } else {
+                // rounds by the element's rank at the column node: two different elements never
+                // hold the same rank at one node, so a round again has at most one addition per
+                // position, and the order of additions is fixed by (step, item, rank)
+                for (int u = 0; u < ITEMS; ++u) {
+                    int rk[NN];
+                    if (active[u])
+                        for (int j = 0; j < NN; ++j) rk[j] = a.el_rank[eNL[u] * NN + j];
+                    for (int rho = 0; rho < a.rounds; ++rho) {
+                        __syncthreads();
+                        if (active[u])
+                            for (int j = 0; j < NN; ++j) {
+                                if (rk[j] != rho) continue;
+                                for (int q = 0; q < DOF * DOF; ++q) {
+                                    const int p = pos[u][j][q];
+                                    if (p >= 0) acc[p] += v[u][j][q];
+                                }
+                            }
+                    }
                 }
+            }
         }
         __syncthreads();
     }
@@ -1054,34 +1126,34 @@ __global__ __launch_bounds__(256) void k_compare32(i64 n, const double* __restri
 
 // nodes [a.node0, a.node0 + count); threads per workgroup 64, 128, 256 or 512
 template<int DOF, int NN, int NQ, class T, bool PAIRS>
-void launch_assemble_kernel(const AssemblyArgs& a, bool det, size_t lds, dim3 g, hipStream_t s, int threads) {
+void launch_assemble_kernel(const AssemblyArgs& a, bool injective, size_t lds, dim3 g, hipStream_t s, int threads) {
     if (threads == 512) {
-        if (det) hipLaunchKernelGGL((k_assemble<DOF, NN, NQ, true, T, 512, PAIRS>), g, dim3(512), lds, s, a);
+        if (injective) hipLaunchKernelGGL((k_assemble<DOF, NN, NQ, true, T, 512, PAIRS>), g, dim3(512), lds, s, a);
         else hipLaunchKernelGGL((k_assemble<DOF, NN, NQ, false, T, 512, PAIRS>), g, dim3(512), lds, s, a);
     } else if (threads == 128) {
-        if (det) hipLaunchKernelGGL((k_assemble<DOF, NN, NQ, true, T, 128, PAIRS>), g, dim3(128), lds, s, a);
+        if (injective) hipLaunchKernelGGL((k_assemble<DOF, NN, NQ, true, T, 128, PAIRS>), g, dim3(128), lds, s, a);
         else hipLaunchKernelGGL((k_assemble<DOF, NN, NQ, false, T, 128, PAIRS>), g, dim3(128), lds, s, a);
     } else if (threads == 64) {
-        if (det) hipLaunchKernelGGL((k_assemble<DOF, NN, NQ, true, T, 64, PAIRS>), g, dim3(64), lds, s, a);
+        if (injective) hipLaunchKernelGGL((k_assemble<DOF, NN, NQ, true, T, 64, PAIRS>), g, dim3(64), lds, s, a);
         else hipLaunchKernelGGL((k_assemble<DOF, NN, NQ, false, T, 64, PAIRS>), g, dim3(64), lds, s, a);
     } else {
-        if (det) hipLaunchKernelGGL((k_assemble<DOF, NN, NQ, true, T, ASM_THREADS, PAIRS>), g, dim3(ASM_THREADS), lds, s, a);
+        if (injective) hipLaunchKernelGGL((k_assemble<DOF, NN, NQ, true, T, ASM_THREADS, PAIRS>), g, dim3(ASM_THREADS), lds, s, a);
         else hipLaunchKernelGGL((k_assemble<DOF, NN, NQ, false, T, ASM_THREADS, PAIRS>), g, dim3(ASM_THREADS), lds, s, a);
     }
 }
 
 // The pair-block variant runs when the arguments carry pair blocks (thermal Q4).
 template<int DOF, int NN, int NQ, class T>
-void launch_assemble(const AssemblyArgs& a, bool det, size_t lds, i64 count, hipStream_t s, int threads = ASM_THREADS) {
+void launch_assemble(const AssemblyArgs& a, bool injective, size_t lds, i64 count, hipStream_t s, int threads = ASM_THREADS) {
     if (count <= 0) return;
     const dim3 g(static_cast<unsigned>(count));
     if constexpr (DOF == 1 && NN == 4 && NQ == 4) {
         if (sizeof(T) == sizeof(double) ? a.pairs != nullptr : a.pairs32 != nullptr) {
-            launch_assemble_kernel<DOF, NN, NQ, T, true>(a, det, lds, g, s, threads);
+            launch_assemble_kernel<DOF, NN, NQ, T, true>(a, injective, lds, g, s, threads);
             return;
         }
     }
-    launch_assemble_kernel<DOF, NN, NQ, T, false>(a, det, lds, g, s, threads);
+    launch_assemble_kernel<DOF, NN, NQ, T, false>(a, injective, lds, g, s, threads);
 }
 
 }  // namespace nlf
@@ -1105,12 +1177,126 @@ GroupParams make_group(const nlf_model& md, int physics, const void* materials, 
         gp.ckind = t.kind;
         gp.c[0] = t.c[0]; gp.c[1] = t.c[1]; gp.c[2] = t.c[2];
     } else {
-        const nlf_elastic_material& e = static_cast<const nlf_elastic_material*>(materials)[g];
-        NLF_CHECK(e.kind == 0, NLF_ERR_UNSUPPORTED, "only isotropic plane-stress elasticity");
-        const double value = e.young / (1.0 - e.poisson * e.poisson);           // elastic_parameters.hpp:70-75
-        gp.c[0] = value; gp.c[1] = e.poisson * value; gp.c[2] = 0.5 * value * (1.0 - e.poisson);
+        elastic_group_params(static_cast<const nlf_elastic_material*>(materials)[g], gp);
     }
     return gp;
+}
+
+// 16-bit column offsets for the SpMV when every |col - row| < 2^15, and the row-pattern dictionary
+// over them (NLF_OPT_ROW_PATTERNS); any earlier ones are dropped first.
+void finish_spmv_storage(nlf_matrix_s* mat, hipStream_t s) {
+    const i64 rows = mat->rows;
+    mat->off16.release();
+    mat->pat.release();
+    mat->pstart.release();
+    mat->pat_size = mat->pat_groups = 0;
+    DevBuf<unsigned> bw{1};
+    NLF_HIP(hipMemsetAsync(bw.p, 0, sizeof(unsigned), s));
+    hipLaunchKernelGGL(k_bandwidth, dim3(blocks_for(rows, 256)), dim3(256), 0, s, rows, mat->outer.p, mat->cols.p, bw.p);
+    unsigned hb = 0;
+    download(&hb, bw.p, 1, s);
+    if (hb < 32768u && mat->nnz_full > 0) {
+        mat->off16.alloc(size_t(mat->nnz_full));
+        hipLaunchKernelGGL(k_to_off16, dim3(blocks_for(rows, 4)), dim3(256), 0, s, rows, mat->outer.p, mat->cols.p,
+                           mat->off16.p);
+        NLF_HIP(hipGetLastError());
+        if (mat->ctx->row_patterns) build_row_patterns(mat, s);
+    }
+}
+
+// ---- the Neumann problem's Lagrange row and column (conductivity_matrix_2d.hpp:39-72, 83-104) ----
+// c[node] = sum over the node's elements (ascending) of integrate_basic = sum_q w_q N_i(q) |J_q|
+template<int NN, int NQ>
+__global__ __launch_bounds__(256) void k_basic_integral(i64 N, const i32* __restrict__ ne_off, const i32* __restrict__ ne_ids,
+                                                        const i32* __restrict__ el, const double* __restrict__ jac, FeTables t,
+                                                        double* __restrict__ c) {
+    const i64 node = i64(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (node >= N) return;
+    double acc = 0.0;
+    for (i32 k = ne_off[node]; k < ne_off[node + 1]; ++k) {
+        const i64 e = ne_ids[k];
+        int i = 0;
+        for (int j = 0; j < NN; ++j)
+            if (el[e * NN + j] == node) i = j;
+        double integral = 0.0;
+        for (int q = 0; q < NQ; ++q) integral += t.w[q] * t.qN[i * NQ + q] * jac[e * NQ + q];
+        acc += integral;
+    }
+    c[node] = acc;
+}
+
+// The bordered full storage: row r < n copies its entries and appends (r, n) = c[r]; row n is
+// (n, 0..n-1) = c and (n, n) = 0.  One wave per row.
+__global__ __launch_bounds__(256) void k_border_rows(i64 n, const i64* __restrict__ outer, const i32* __restrict__ cols,
+                                                     const double* __restrict__ val, const double* __restrict__ c,
+                                                     i64* __restrict__ outer2, i32* __restrict__ cols2,
+                                                     double* __restrict__ val2, i64* __restrict__ diag2,
+                                                     const i64* __restrict__ diag) {
+    const i64 r = i64(blockIdx.x) * 4 + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (r > n) return;
+    if (r < n) {
+        const i64 b = outer[r], e = outer[r + 1], b2 = b + r;
+        for (i64 p = b + lane; p < e; p += 64) {
+            cols2[b2 + (p - b)] = cols[p];
+            val2[b2 + (p - b)] = val[p];
+        }
+        if (lane == 0) {
+            cols2[b2 + (e - b)] = i32(n);
+            val2[b2 + (e - b)] = c[r];
+            outer2[r] = b2;
+            diag2[r] = diag[r] + r;
+        }
+    } else {
+        const i64 b2 = outer[n] + n;
+        for (i64 k = lane; k <= n; k += 64) {
+            cols2[b2 + k] = i32(k);
+            val2[b2 + k] = k < n ? c[k] : 0.0;
+        }
+        if (lane == 0) {
+            outer2[n] = b2;
+            outer2[n + 1] = b2 + n + 1;
+            diag2[n] = b2 + n;
+        }
+    }
+}
+
+void border_neumann(nlf_matrix_s* mat, hipStream_t s) {
+    const nlf_mesh m = mat->mesh;
+    const i64 n = mat->rows;
+    DevBuf<double> c{size_t(std::max<i64>(n, 1))};
+    if (m->nn == NLF_ELEMENT_Q4)
+        hipLaunchKernelGGL((k_basic_integral<4, 4>), dim3(blocks_for(n, 256)), dim3(256), 0, s, n, m->ne_off.p, m->ne_ids.p,
+                           m->el.p, m->jac.p, m->fe, c.p);
+    else
+        hipLaunchKernelGGL((k_basic_integral<8, 9>), dim3(blocks_for(n, 256)), dim3(256), 0, s, n, m->ne_off.p, m->ne_ids.p,
+                           m->el.p, m->jac.p, m->fe, c.p);
+    const i64 nnz2 = mat->nnz_full + 2 * n + 1;
+    NLF_CHECK(nnz2 < (i64(1) << 40), NLF_ERR_ARGUMENT, "pattern too large");
+    DevBuf<i64> outer2{size_t(n + 2)}, diag2{size_t(n + 1)};
+    DevBuf<i32> cols2{size_t(nnz2)};
+    DevBuf<double> val2{size_t(nnz2)};
+    hipLaunchKernelGGL(k_border_rows, dim3(blocks_for(n + 1, 4)), dim3(256), 0, s, n, mat->outer.p, mat->cols.p, mat->val.p,
+                       c.p, outer2.p, cols2.p, val2.p, diag2.p, mat->diag.p);
+    NLF_HIP(hipGetLastError());
+    // the Lagrange row is an inner row with no bound entries
+    std::vector<uint8_t> inner(size_t(n + 1), 1);
+    DevBuf<uint8_t> inner2;
+    upload(inner2, inner.data(), inner.size(), s);
+    DevBuf<i64> b_outer2{size_t(n + 2)};
+    NLF_HIP(hipMemsetAsync(b_outer2.p, 0, b_outer2.bytes(), s));
+    NLF_HIP(hipStreamSynchronize(s));
+    mat->outer = std::move(outer2);
+    mat->diag = std::move(diag2);
+    mat->cols = std::move(cols2);
+    mat->val = std::move(val2);
+    mat->inner = std::move(inner2);
+    mat->b_outer = std::move(b_outer2);
+    mat->rows = n + 1;
+    mat->nnz_full = nnz2;
+    mat->neumann = true;
+    finish_spmv_storage(mat, s);
+    NLF_HIP(hipStreamSynchronize(s));
 }
 
 // Value pass on the matrix's pattern.  T = double fills val and b_val (and the 16-bit column
@@ -1242,7 +1428,10 @@ void assemble_values(nlf_matrix_s* mat, hipStream_t s, const PairPlan* plan = nu
     aa.outer = mat->outer.p; aa.cols = mat->cols.p; aa.diag = mat->diag.p; aa.b_outer = mat->b_outer.p;
     aa.b_cols = mat->b_cols.p; aa.val = mat->val.p; aa.b_val = mat->b_val.p; aa.val32 = mat->val32.p;
     aa.local_only = only_local;
-    const bool det = m->injective;
+    aa.el_rank = m->el_rank.p;
+    aa.cq = mat->cq.p;
+    aa.rounds = m->max_degree;
+    const bool injective = m->injective;
     const bool node_only = m->ctx->assembly_pairs == 0;
     const bool use_pairs = mat->physics == 0 && m->nn == 4 && any_nonlocal && !only_local && !node_only;
     // local-only matrices whose rows fit a wave: one wave per node
@@ -1326,7 +1515,7 @@ void assemble_values(nlf_matrix_s* mat, hipStream_t s, const PairPlan* plan = nu
                 ca.plane = all_pairs * 4;
                 ca.eoff = d_eoff.p;
             }
-            launch_assemble<1, 4, 4, T>(ca, det, lds, N, s, m->ctx->assembly_threads);
+            launch_assemble<1, 4, 4, T>(ca, injective, lds, N, s, m->ctx->assembly_threads);
             NLF_HIP(hipGetLastError());
         } else {
             DevBuf<i64> d_eoff{size_t(E)}, d_cofs;
@@ -1348,37 +1537,23 @@ void assemble_values(nlf_matrix_s* mat, hipStream_t s, const PairPlan* plan = nu
                     ca.plane = c.total * 4;
                     ca.eoff = d_eoff.p;
                 }
-                launch_assemble<1, 4, 4, T>(ca, det, lds, c.n1 - c.n0, s, m->ctx->assembly_threads);
+                launch_assemble<1, 4, 4, T>(ca, injective, lds, c.n1 - c.n0, s, m->ctx->assembly_threads);
                 NLF_HIP(hipGetLastError());
             }
             NLF_HIP(hipStreamSynchronize(s));          // before the chunk offsets and lists are released
         }
         NLF_HIP(hipStreamSynchronize(s));              // before the pair buffer is released
-    } else if (mat->physics != 1 && m->nn == 4) launch_assemble<1, 4, 4, T>(aa, det, lds, m->N, s, m->ctx->assembly_threads);
-    else if (mat->physics != 1 && m->nn == 8) launch_assemble<1, 8, 9, T>(aa, det, lds, m->N, s, m->ctx->assembly_threads);
-    else if (mat->physics == 1 && m->nn == 4) launch_assemble<2, 4, 4, T>(aa, det, lds, m->N, s, m->ctx->assembly_threads);
-    else launch_assemble<2, 8, 9, T>(aa, det, lds, m->N, s, m->ctx->assembly_threads);
+    } else if (mat->physics != 1 && m->nn == 4) launch_assemble<1, 4, 4, T>(aa, injective, lds, m->N, s, m->ctx->assembly_threads);
+    else if (mat->physics != 1 && m->nn == 8) launch_assemble<1, 8, 9, T>(aa, injective, lds, m->N, s, m->ctx->assembly_threads);
+    else if (mat->physics == 1 && m->nn == 4) launch_assemble<2, 4, 4, T>(aa, injective, lds, m->N, s, m->ctx->assembly_threads);
+    else launch_assemble<2, 8, 9, T>(aa, injective, lds, m->N, s, m->ctx->assembly_threads);
     NLF_HIP(hipGetLastError());
     T* values = nullptr;
     if constexpr (F64) values = mat->val.p; else values = mat->val32.p;
     hipLaunchKernelGGL(k_mirror<T>, dim3(blocks_for(rows, 4)), dim3(256), 0, s, rows, mat->outer.p, mat->cols.p,
                        mat->diag.p, values);
     NLF_HIP(hipGetLastError());
-    if constexpr (F64) {   // 16-bit column offsets for the SpMV when every |col - row| < 2^15
-        DevBuf<unsigned> bw{1};
-        NLF_HIP(hipMemsetAsync(bw.p, 0, sizeof(unsigned), s));
-        hipLaunchKernelGGL(k_bandwidth, dim3(blocks_for(rows, 256)), dim3(256), 0, s, rows, mat->outer.p, mat->cols.p,
-                           bw.p);
-        unsigned hb = 0;
-        download(&hb, bw.p, 1, s);
-        if (hb < 32768u && mat->nnz_full > 0) {
-            mat->off16.alloc(size_t(mat->nnz_full));
-            hipLaunchKernelGGL(k_to_off16, dim3(blocks_for(rows, 4)), dim3(256), 0, s, rows, mat->outer.p, mat->cols.p,
-                               mat->off16.p);
-            NLF_HIP(hipGetLastError());
-            if (m->ctx->row_patterns) build_row_patterns(mat, s);
-        }
-    }
+    if constexpr (F64) finish_spmv_storage(mat, s);
     NLF_HIP(hipStreamSynchronize(s));
 }
 
@@ -1414,8 +1589,14 @@ extern "C" {
 // are supported (the reference solves the asymmetric case with BiCGSTAB, out of scope here).
 int nlf_assemble(nlf_mesh m, int physics, const nlf_model* models, const void* materials, const uint8_t* is_inner,
                  int symmetric, int part, nlf_matrix* out) {
+    return nlf_assemble_ex(m, physics, models, materials, is_inner, symmetric, part, nullptr, out);
+}
+
+int nlf_assemble_ex(nlf_mesh m, int physics, const nlf_model* models, const void* materials, const uint8_t* is_inner,
+                    int symmetric, int part, const nlf_assemble_options* opt, nlf_matrix* out) {
     if (!m) return NLF_ERR_ARGUMENT;
     return guarded(m->ctx, [&] {
+        const bool neumann = opt && opt->neumann;
         NLF_CHECK(out && models && materials && is_inner, NLF_ERR_ARGUMENT, "null argument");
         NLF_CHECK(physics >= 0 && physics <= 2, NLF_ERR_ARGUMENT,
                   "physics must be 0 (thermal), 1 (elastic) or 2 (heat capacity)");
@@ -1443,6 +1624,40 @@ int nlf_assemble(nlf_mesh m, int physics, const nlf_model* models, const void* m
         auto mat = std::make_unique<nlf_matrix_s>();
         mat->ctx = m->ctx; mat->mesh = m; mat->dof = dof; mat->rows = rows;
         upload(mat->inner, is_inner, size_t(rows), s);
+        if (neumann) {
+            NLF_CHECK(physics == 0, NLF_ERR_ARGUMENT, "Neumann problems are thermal (conductivity_matrix_2d)");
+            for (i64 r = 0; r < rows; ++r)
+                NLF_CHECK(is_inner[r], NLF_ERR_ARGUMENT, "a Neumann problem has no first-kind boundary condition");
+        }
+        // coefficients per quadrature point: only where every group is local -- a nonlocal problem with
+        // varying coefficients is asymmetric (problem_settings.hpp:32-34) and solved by BiCGSTAB there
+        if (opt && opt->coefficients) {
+            const int nq = m->fe.nq;
+            std::vector<double> cq;
+            for (int g = 0; g < m->G; ++g) {
+                const double* v = opt->coefficients[g];
+                if (!v) continue;
+                NLF_CHECK(physics != 2, NLF_ERR_ARGUMENT, "the heat capacity takes no coefficients per point");
+                NLF_CHECK(!any_nonlocal, NLF_ERR_UNSUPPORTED,
+                          "coefficients per point make a nonlocal problem asymmetric (BiCGSTAB in the reference)");
+                if (cq.empty()) cq.assign(size_t(m->E) * nq * 6, 0.0);
+                int nc;
+                if (physics == 0) nc = gps[g].ckind == NLF_CONDUCTIVITY_ISOTROPIC ? 1 : gps[g].ckind == NLF_CONDUCTIVITY_ORTHOTROPIC ? 2 : 3;
+                else {
+                    const int k = static_cast<const nlf_elastic_material*>(materials)[g].kind;
+                    nc = k == NLF_ELASTIC_ISOTROPIC ? 3 : k == NLF_ELASTIC_ORTHOTROPIC ? 4 : 6;
+                }
+                for (i64 e = m->gbeg_h[g]; e < m->gbeg_h[g + 1]; ++e)
+                    for (int q = 0; q < nq; ++q) {
+                        const double* src = v + ((e - m->gbeg_h[g]) * nq + q) * nc;
+                        double* dst = cq.data() + (size_t(e) * nq + q) * 6;
+                        for (int k = 0; k < nc; ++k) dst[k] = src[k];
+                        if (physics == 1 && nc == 3) dst[3] = src[0];       // isotropic h22 = h11
+                    }
+                gps[g].varying = 1;
+            }
+            if (!cq.empty()) upload(mat->cq, cq.data(), cq.size(), s);
+        }
         DevBuf<int> d_gnl;
         upload(d_gnl, gnl.data(), gnl.size(), s);
         // ---- pattern ----
@@ -1518,7 +1733,8 @@ int nlf_assemble(nlf_mesh m, int physics, const nlf_model* models, const void* m
         mat->only_local = only_local;
         mat->groups = gps;
         assemble_values<double>(mat.get(), s, pairs_path ? &plan : nullptr);
-        mat->nnz_upper = (mat->nnz_full + rows) / 2;
+        if (neumann) border_neumann(mat.get(), s);
+        mat->nnz_upper = (mat->nnz_full + mat->rows) / 2;
         mat->seconds_values = std::chrono::duration<double>(std::chrono::steady_clock::now() - t1).count();
         *out = mat.release();
     });
@@ -1549,6 +1765,35 @@ int nlf_matrix_timings(nlf_matrix a, double* pattern_seconds, double* values_sec
 // Host export in the reference's layout (finite_element_matrix::inner()/bound(): RowMajor CSR with
 // int64 indices): part 0 = inner upper triangle, 1 = bound, 2 = inner full storage; parts 3 and 4
 // are parts 0 and 2 of the FP32 values, widened.
+// Upper CSR of rows [row0, row1) only: outer[row1 - row0 + 1] starts at 0; inner / values may be
+// NULL (a first call with both NULL gives the sizes through outer).
+int nlf_matrix_export_rows(nlf_matrix a, int64_t row0, int64_t row1, int64_t* outer, int64_t* inner, double* values) {
+    if (!a || !outer) return NLF_ERR_ARGUMENT;
+    return guarded(a->ctx, [&] {
+        NLF_CHECK(row0 >= 0 && row0 <= row1 && row1 <= a->rows, NLF_ERR_ARGUMENT, "row range out of bounds");
+        hipStream_t s = a->ctx->stream;
+        const i64 n = row1 - row0;
+        outer[0] = 0;
+        if (n == 0) return;
+        std::vector<i64> o(static_cast<size_t>(n + 1)), dg(static_cast<size_t>(n));
+        download(o.data(), a->outer.p + row0, size_t(n + 1), s);
+        download(dg.data(), a->diag.p + row0, size_t(n), s);
+        const i64 b = dg[0], e = o[n];                 // full-storage span holding the rows' upper tails
+        std::vector<i32> c;
+        std::vector<double> v;
+        if (inner) { c.resize(size_t(e - b)); download(c.data(), a->cols.p + b, c.size(), s); }
+        if (values) { v.resize(size_t(e - b)); download(v.data(), a->val.p + b, v.size(), s); }
+        i64 pos = 0;
+        for (i64 r = 0; r < n; ++r) {
+            for (i64 p = dg[r]; p < o[r + 1]; ++p, ++pos) {
+                if (inner) inner[pos] = c[size_t(p - b)];
+                if (values) values[pos] = v[size_t(p - b)];
+            }
+            outer[r + 1] = pos;
+        }
+    });
+}
+
 int nlf_matrix_export(nlf_matrix a, int part, int64_t* outer, int64_t* inner, double* values) {
     if (!a) return NLF_ERR_ARGUMENT;
     return guarded(a->ctx, [&] {
@@ -1726,6 +1971,7 @@ int nlf_matrix_assemble_fp32(nlf_matrix a) {
     if (!a) return NLF_ERR_ARGUMENT;
     return guarded(a->ctx, [&] {
         NLF_CHECK(!a->combined, NLF_ERR_STATE, "the matrix was combined with another one (nlf_matrix_combine)");
+        NLF_CHECK(!a->neumann, NLF_ERR_UNSUPPORTED, "the FP32 value pass does not repeat the Neumann border (use nlf_matrix_make_fp32)");
         NLF_CHECK(a->val.p && !a->groups.empty(), NLF_ERR_STATE, "the matrix has no FP64 assembly to repeat");
         hipStream_t s = a->ctx->stream;
         assemble_values<float>(a, s);

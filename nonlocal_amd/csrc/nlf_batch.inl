@@ -468,6 +468,7 @@ int nlf_batch_create(nlf_matrix a, int count, const int64_t* row_begin, nlf_batc
     if (!a || !row_begin || !out || count <= 0) return NLF_ERR_ARGUMENT;
     return guarded(a->ctx, [&] {
         NLF_CHECK(row_begin[0] == 0 && row_begin[count] == a->rows, NLF_ERR_ARGUMENT, "row ranges must cover the matrix");
+        NLF_CHECK(!a->neumann, NLF_ERR_UNSUPPORTED, "batched solves of bordered Neumann systems are not implemented");
         for (int i = 0; i < count; ++i) NLF_CHECK(row_begin[i] <= row_begin[i + 1], NLF_ERR_ARGUMENT, "row ranges not ascending");
         hipStream_t s = a->ctx->stream;
         auto B = std::make_unique<nlf_batch_s>();

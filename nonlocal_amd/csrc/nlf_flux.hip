@@ -63,15 +63,36 @@ __global__ __launch_bounds__(256) void k_flux_local(i64 E, const i32* __restrict
     rec[idx] = make_double4(xq[2 * idx], xq[2 * idx + 1], wj * fx, wj * fy);
 }
 
-// u is the solution vector (x, y per node).  Hooke {h11, h12, h66} of isotropic plane stress.
+// calc_stress (mechanical_parameters_2d.hpp:27-48) with the Hooke matrix in the anisotropic layout
+// {h11, h12, h66, h22, h16, h26} (isotropic and orthotropic: h16 = h26 = 0); tensor shear strain.
+__device__ __forceinline__ void hooke_stress(const GroupParams& g, double e11, double e22, double e12, double& s11,
+                                             double& s22, double& s12) {
+    s11 = g.c[0] * e11 + g.c[1] * e22 + 2 * g.c[4] * e12;
+    s22 = g.c[1] * e11 + g.c[3] * e22 + 2 * g.c[5] * e12;
+    s12 = g.c[4] * e11 + g.c[5] * e22 + 2 * g.c[2] * e12;
+}
+
+// temperature change at quadrature point q of element e: sum_i N_i(q) dT_i (nodes_to_qnodes,
+// mesh_2d_utils.hpp:65-80)
+template<int NN, int NQ>
+__device__ __forceinline__ double temperature_at(const i32* __restrict__ el, const FeTables& t,
+                                                 const double* __restrict__ dT, i64 e, int q) {
+    double v = 0.0;
+    for (int i = 0; i < NN; ++i) v += dT[el[e * NN + i]] * t.qN[i * NQ + q];
+    return v;
+}
+
+// u is the solution vector (x, y per node).  The strains written to qstrain are those of u; with a
+// temperature change dT (node values, or null) the thermal strain alpha dT of the group is
+// subtracted before the stress (substract_temperature_strains, mechanical_solution_2d.hpp:135-166).
 template<int NN, int NQ>
 __global__ __launch_bounds__(256) void k_stress_local(i64 E, const i32* __restrict__ el, const double* __restrict__ D,
                                                       const double* __restrict__ jac, const double* __restrict__ xq,
                                                       const i32* __restrict__ group_of,
                                                       const GroupParams* __restrict__ groups, FeTables t,
-                                                      const double* __restrict__ u, double* __restrict__ qstrain,
-                                                      double* __restrict__ qloc, double4* __restrict__ rec,
-                                                      double* __restrict__ rec2) {
+                                                      const double* __restrict__ u, const double* __restrict__ dT,
+                                                      double* __restrict__ qstrain, double* __restrict__ qloc,
+                                                      double4* __restrict__ rec, double* __restrict__ rec2) {
     const i64 idx = i64(blockIdx.x) * blockDim.x + threadIdx.x;
     const i64 Q = E * NQ;
     if (idx >= Q) return;
@@ -93,20 +114,99 @@ __global__ __launch_bounds__(256) void k_stress_local(i64 E, const i32* __restri
     g12 /= j;
     g21 /= j;
     g22 /= j;
-    const double e11 = g11, e22 = g22, e12 = 0.5 * (g12 + g21);
+    double e11 = g11, e22 = g22, e12 = 0.5 * (g12 + g21);
     qstrain[idx] = e11;
     qstrain[Q + idx] = e22;
     qstrain[2 * Q + idx] = e12;
     const GroupParams& g = groups[group_of[e]];
-    const double s11 = g.c[0] * e11 + g.c[1] * e22;
-    const double s22 = g.c[1] * e11 + g.c[0] * e22;
-    const double s12 = 2 * g.c[2] * e12;
+    if (dT && g.ekind != NLF_EXPANSION_NONE) {
+        const double dt = temperature_at<NN, NQ>(el, t, dT, e, q);
+        e11 -= g.alpha[0] * dt;
+        e22 -= g.alpha[1] * dt;
+        e12 -= g.alpha[2] * dt;
+    }
+    double s11, s22, s12;
+    hooke_stress(g, e11, e22, e12, s11, s22, s12);
     qloc[idx] = s11;
     qloc[Q + idx] = s22;
     qloc[2 * Q + idx] = s12;
     const double wj = t.w[q] * j;
     rec[idx] = make_double4(xq[2 * idx], xq[2 * idx + 1], wj * s11, wj * s22);
     rec2[idx] = wj * s12;
+}
+
+// The stress of the thermal strain at each quadrature point (_temperature_condition::calc_stress,
+// temperature_condition_2d.hpp:20-74: C (alpha dT), the shear part with the tensor convention), in
+// the layout of k_stress_local, so that k_nonlocal_average forms its nonlocal average.
+template<int NN, int NQ>
+__global__ __launch_bounds__(256) void k_thermal_stress(i64 E, const i32* __restrict__ el,
+                                                        const double* __restrict__ jac, const double* __restrict__ xq,
+                                                        const i32* __restrict__ group_of,
+                                                        const GroupParams* __restrict__ groups, FeTables t,
+                                                        const double* __restrict__ dT, double* __restrict__ qloc,
+                                                        double4* __restrict__ rec, double* __restrict__ rec2) {
+    const i64 idx = i64(blockIdx.x) * blockDim.x + threadIdx.x;
+    const i64 Q = E * NQ;
+    if (idx >= Q) return;
+    const i64 e = idx / NQ;
+    const int q = int(idx - e * NQ);
+    const GroupParams& g = groups[group_of[e]];
+    double s11 = 0.0, s22 = 0.0, s12 = 0.0;
+    if (g.ekind != NLF_EXPANSION_NONE) {
+        const double dt = temperature_at<NN, NQ>(el, t, dT, e, q);
+        hooke_stress(g, g.alpha[0] * dt, g.alpha[1] * dt, g.alpha[2] * dt, s11, s22, s12);
+    }
+    qloc[idx] = s11;
+    qloc[Q + idx] = s22;
+    qloc[2 * Q + idx] = s12;
+    const double wj = t.w[q] * jac[idx];
+    rec[idx] = make_double4(xq[2 * idx], xq[2 * idx + 1], wj * s11, wj * s22);
+    rec2[idx] = wj * s12;
+}
+
+// temperature_condition (temperature_condition_2d.hpp:128-170), one thread per node: over the node's
+// elements (ascending), p1 times the local load sum_q w_q D_i(q) . s(q) (:80-92) plus, for a nonlocal
+// group, (1 - p1) times sum_q w_q (D_i,x (S_xx + S_xy), D_i,y (S_yy + S_xy)) with S the influence
+// average of the thermal stress over the neighbours -- the reference's nonlocal form (:118-121).
+// qs holds s, qavg holds S (k_nonlocal_average with p1 = 0).
+template<int NN, int NQ>
+__global__ __launch_bounds__(256) void k_thermal_load(i64 N, const i32* __restrict__ ne_off,
+                                                      const i32* __restrict__ ne_ids, const i32* __restrict__ el,
+                                                      const double* __restrict__ D, const i32* __restrict__ group_of,
+                                                      const GroupParams* __restrict__ groups, FeTables t, i64 Q,
+                                                      const double* __restrict__ qs, const double* __restrict__ qavg,
+                                                      double* __restrict__ f) {
+    const i64 node = i64(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (node >= N) return;
+    double fx = 0.0, fy = 0.0;
+    for (i32 k = ne_off[node]; k < ne_off[node + 1]; ++k) {
+        const i64 e = ne_ids[k];
+        const GroupParams& g = groups[group_of[e]];
+        if (g.ekind == NLF_EXPANSION_NONE) continue;
+        int i = 0;
+        for (int j = 0; j < NN; ++j)
+            if (el[e * NN + j] == node) i = j;
+        double lx = 0.0, ly = 0.0, nx = 0.0, ny = 0.0;
+        for (int q = 0; q < NQ; ++q) {
+            const i64 d = 2 * ((e * NN + i) * NQ + q);
+            const double wx = t.w[q] * D[d], wy = t.w[q] * D[d + 1];
+            const i64 at = e * NQ + q;
+            lx += wx * qs[at] + wy * qs[2 * Q + at];
+            ly += wy * qs[Q + at] + wx * qs[2 * Q + at];
+            if (g.nonlocal) {
+                nx += wx * (qavg[at] + qavg[2 * Q + at]);
+                ny += wy * (qavg[Q + at] + qavg[2 * Q + at]);
+            }
+        }
+        if (g.nonlocal) {
+            fx += (1.0 - g.p1) * nx;
+            fy += (1.0 - g.p1) * ny;
+        }
+        fx += g.p1 * lx;
+        fy += g.p1 * ly;
+    }
+    f[2 * node] += fx;
+    f[2 * node + 1] += fy;
 }
 
 template<int NQ, int POLY, int C>
@@ -242,10 +342,11 @@ void launch_flux(nlf_mesh m, const GroupParams* groups, int poly, const double* 
 }
 
 template<int NN, int NQ>
-void launch_stress(nlf_mesh m, const GroupParams* groups, int poly, const double* u, double* qstrain, double* qloc,
-                   double4* rec, double* rec2, double* qout, double* strain, double* stress, hipStream_t s) {
+void launch_stress(nlf_mesh m, const GroupParams* groups, int poly, const double* u, const double* dT, double* qstrain,
+                   double* qloc, double4* rec, double* rec2, double* qout, double* strain, double* stress,
+                   hipStream_t s) {
     hipLaunchKernelGGL((k_stress_local<NN, NQ>), dim3(blocks_for(m->E * NQ, 256)), dim3(256), 0, s, m->E, m->el.p,
-                       m->D.p, m->jac.p, m->xq.p, m->group_of.p, groups, m->fe, u, qstrain, qloc, rec, rec2);
+                       m->D.p, m->jac.p, m->xq.p, m->group_of.p, groups, m->fe, u, dT, qstrain, qloc, rec, rec2);
     NLF_HIP(hipGetLastError());
     if (strain) launch_to_nodes<NN, NQ, 3>(m, qstrain, strain, s);
     if (!qout) return;
@@ -302,17 +403,15 @@ extern "C" int nlf_heat_flux(nlf_mesh m, const nlf_model* models, const nlf_ther
 }
 
 extern "C" int nlf_strain_stress(nlf_mesh m, const nlf_model* models, const nlf_elastic_material* materials,
-                                 const double* u_dev, double* strain_dev, double* stress_dev, double* qstress_dev) {
+                                 const double* u_dev, const double* delta_temperature_dev, double* strain_dev,
+                                 double* stress_dev, double* qstress_dev) {
     if (!m || !models || !materials || !u_dev || (!strain_dev && !stress_dev && !qstress_dev)) return NLF_ERR_ARGUMENT;
     return guarded(m->ctx, [&] {
         hipStream_t s = m->ctx->stream;
         std::vector<GroupParams> gp(size_t(m->G));
         for (int g = 0; g < m->G; ++g) {
             gp[g] = model_group_params(models[g], false);
-            const nlf_elastic_material& e = materials[g];
-            NLF_CHECK(e.kind == 0, NLF_ERR_UNSUPPORTED, "only isotropic plane-stress elasticity");
-            const double value = e.young / (1.0 - e.poisson * e.poisson);        // elastic_parameters.hpp:70-75
-            gp[g].c[0] = value; gp[g].c[1] = e.poisson * value; gp[g].c[2] = 0.5 * value * (1.0 - e.poisson);
+            elastic_group_params(materials[g], gp[g]);
         }
         NLF_CHECK(!any_nonlocal(gp) || m->has_neighbours, NLF_ERR_STATE,
                   "the stress of a nonlocal group needs the neighbour lists (nlf_find_neighbours)");
@@ -330,11 +429,57 @@ extern "C" int nlf_strain_stress(nlf_mesh m, const nlf_model* models, const nlf_
         }
         const int poly = influence_class(gp);
         if (m->nn == NLF_ELEMENT_Q4)
-            launch_stress<4, 4>(m, d_groups.p, poly, u_dev, qstrain.p, qloc.p, rec.p, rec2.p, qout, strain_dev,
-                                stress_dev, s);
+            launch_stress<4, 4>(m, d_groups.p, poly, u_dev, delta_temperature_dev, qstrain.p, qloc.p, rec.p, rec2.p, qout,
+                                strain_dev, stress_dev, s);
         else
-            launch_stress<8, 9>(m, d_groups.p, poly, u_dev, qstrain.p, qloc.p, rec.p, rec2.p, qout, strain_dev,
-                                stress_dev, s);
+            launch_stress<8, 9>(m, d_groups.p, poly, u_dev, delta_temperature_dev, qstrain.p, qloc.p, rec.p, rec2.p, qout,
+                                strain_dev, stress_dev, s);
+        NLF_HIP(hipStreamSynchronize(s));
+    });
+}
+
+extern "C" int nlf_temperature_condition(nlf_mesh m, const nlf_model* models, const nlf_elastic_material* materials,
+                                         const double* delta_temperature_dev, double* f_dev) {
+    if (!m || !models || !materials || !delta_temperature_dev || !f_dev) return NLF_ERR_ARGUMENT;
+    return guarded(m->ctx, [&] {
+        hipStream_t s = m->ctx->stream;
+        std::vector<GroupParams> gp(size_t(m->G)), avg(size_t(m->G));
+        bool any = false;
+        for (int g = 0; g < m->G; ++g) {
+            gp[g] = model_group_params(models[g], false);
+            elastic_group_params(materials[g], gp[g]);
+            any |= gp[g].ekind != NLF_EXPANSION_NONE;
+            avg[g] = gp[g];
+            avg[g].p1 = 0.0;                     // k_nonlocal_average then returns the average alone
+        }
+        NLF_CHECK(!any_nonlocal(gp) || m->has_neighbours, NLF_ERR_STATE,
+                  "the thermal load of a nonlocal group needs the neighbour lists (nlf_find_neighbours)");
+        if (m->E == 0 || !any) return;
+        const i64 Q = m->E * m->fe.nq;
+        DevBuf<GroupParams> d_groups, d_avg;
+        upload(d_groups, gp.data(), gp.size(), s);
+        upload(d_avg, avg.data(), avg.size(), s);
+        DevBuf<double> qs{size_t(3 * Q)}, rec2{size_t(Q)}, qavg;
+        DevBuf<double4> rec{size_t(Q)};
+        const bool nonlocal = any_nonlocal(gp);
+        if (nonlocal) qavg.alloc(size_t(3 * Q));
+        const int poly = influence_class(gp);
+        if (m->nn == NLF_ELEMENT_Q4) {
+            hipLaunchKernelGGL((k_thermal_stress<4, 4>), dim3(blocks_for(Q, 256)), dim3(256), 0, s, m->E, m->el.p, m->jac.p,
+                               m->xq.p, m->group_of.p, d_groups.p, m->fe, delta_temperature_dev, qs.p, rec.p, rec2.p);
+            if (nonlocal) launch_average<4, 3>(m, d_avg.p, poly, rec.p, rec2.p, qs.p, qavg.p, s);
+            hipLaunchKernelGGL((k_thermal_load<4, 4>), dim3(blocks_for(m->N, 256)), dim3(256), 0, s, m->N, m->ne_off.p,
+                               m->ne_ids.p, m->el.p, m->D.p, m->group_of.p, d_groups.p, m->fe, Q, qs.p,
+                               nonlocal ? qavg.p : qs.p, f_dev);
+        } else {
+            hipLaunchKernelGGL((k_thermal_stress<8, 9>), dim3(blocks_for(Q, 256)), dim3(256), 0, s, m->E, m->el.p, m->jac.p,
+                               m->xq.p, m->group_of.p, d_groups.p, m->fe, delta_temperature_dev, qs.p, rec.p, rec2.p);
+            if (nonlocal) launch_average<9, 3>(m, d_avg.p, poly, rec.p, rec2.p, qs.p, qavg.p, s);
+            hipLaunchKernelGGL((k_thermal_load<8, 9>), dim3(blocks_for(m->N, 256)), dim3(256), 0, s, m->N, m->ne_off.p,
+                               m->ne_ids.p, m->el.p, m->D.p, m->group_of.p, d_groups.p, m->fe, Q, qs.p,
+                               nonlocal ? qavg.p : qs.p, f_dev);
+        }
+        NLF_HIP(hipGetLastError());
         NLF_HIP(hipStreamSynchronize(s));
     });
 }

@@ -181,13 +181,23 @@ __device__ __forceinline__ float influence32(const InfluenceParams& f, double ax
 
 struct GroupParams {                     // per element group, copied to device
     double p1;                           // local weight (1 for LOCAL groups)
-    double c[3];                         // thermal conductivity components or Hooke {h11, h12, h66}
+    double c[6];                         // thermal conductivity {l11, l22, l12}, or the Hooke matrix of any
+                                         // kind in the anisotropic layout {h11, h12, h66, h22, h16, h26}
+    double alpha[3];                     // thermal expansion {a_xx, a_yy, a_xy} (elasticity)
     int nonlocal;                        // theory NONLOCAL
     int ckind;                           // conductivity kind
+    int ekind;                           // thermal expansion kind (NLF_EXPANSION_*)
+    int varying;                         // coefficients per quadrature point (AssemblyArgs::cq) instead of c
     InfluenceParams inf;
 };
 
 GroupParams model_group_params(const nlf_model& md, bool only_local);
+
+// Hooke matrix and thermal expansion of a plane-stress material into gp.c / gp.alpha / gp.ekind
+// (elastic_parameters.hpp:62-133, thermal_expansion_parameters.hpp:7-18).  Isotropic and orthotropic
+// matrices take the anisotropic layout with h22 = h11 (isotropic) and h16 = h26 = 0, so one
+// integrand serves every kind.
+void elastic_group_params(const nlf_elastic_material& e, GroupParams& gp);
 
 // Influence class shared by the nonlocal groups, for influence_poly<POLY>: 1 = fast polynomial,
 // 2 = bell-shaped polynomial (p = n, q = 2), 0 = anything else (or groups of different classes),
@@ -214,6 +224,7 @@ struct nlf_mesh_s {
     nlf::i64 N = 0, E = 0;
     int nn = 0, nb = 0, G = 0, GB = 0;
     bool injective = true;               // for every local index j, e -> elems[e][j] is injective
+    int max_degree = 0;                  // most elements at one node
     nlf::FeTables fe;
     nlf::BoundaryTables bt;
     // host copies used by boundary integrals and level analysis
@@ -229,6 +240,8 @@ struct nlf_mesh_s {
     nlf::DevBuf<nlf::i32> group_of;      // E
     nlf::DevBuf<nlf::i32> ne_off, ne_ids; // node -> elements (ascending), int32 offsets (4*E < 2^31)
     nlf::DevBuf<nlf::i32> el_maxnode;    // max node id of each element
+    nlf::DevBuf<uint8_t> el_rank;        // E*nn, non-injective meshes only: position of e in the
+                                         // ascending element list of its node j
     nlf::DevBuf<double> xq, jac, D;      // 2*E*nq, E*nq, 2*E*nn*nq ([e][i][q])
     nlf::DevBuf<double> centres;         // 2E
     // neighbours
@@ -273,6 +286,8 @@ struct nlf_matrix_s {
     bool val32_assembled = false;        // val32 holds FP32-assembled values (not the rounded copy)
     bool split_ready = false;            // val32 + val_lo is the split of the current val
     bool combined = false;               // values changed by nlf_matrix_combine (no FP32 value pass)
+    nlf::DevBuf<double> cq;              // coefficients per quadrature point, [(e * nq + q) * 6 + k], or empty
+    bool neumann = false;                // bordered by the Lagrange row and column of a Neumann problem
 };
 
 namespace nlf {

@@ -1598,6 +1598,8 @@ int nlf_learned_create(nlf_matrix A, nlf_matrix A_local, const nlf_model* models
     return guarded(A->ctx, [&] {
         NLF_CHECK(A->dof == 1 && A_local->dof == 1 && A->rows == A_local->rows, NLF_ERR_UNSUPPORTED,
                   "the learned preconditioner is implemented for the thermal (1 DoF) problem");
+        NLF_CHECK(!A->neumann && !A_local->neumann, NLF_ERR_UNSUPPORTED,
+                  "the learned preconditioner works on mesh nodes: not on the bordered Neumann system");
         hipStream_t s = A->ctx->stream;
         const nlf_mesh m = A->mesh;
         auto P = std::make_unique<nlf_precond_s>();

@@ -301,7 +301,23 @@ int nlf_mesh_create(nlf_ctx ctx, int64_t n_nodes, const double* xy, int64_t n_el
                 }
             }
         }
+        for (i64 v = 0; v < n_nodes; ++v) m->max_degree = std::max(m->max_degree, off[v + 1] - off[v]);
         hipStream_t s = ctx->stream;
+        if (!m->injective) {
+            // rank of each element at each of its nodes: the assembly accumulates a node's row in
+            // rounds by this rank instead of by local index, so no two contributions of one round
+            // share a position on any mesh
+            NLF_CHECK(m->max_degree <= 255, NLF_ERR_UNSUPPORTED, "a node shared by more than 255 elements");
+            std::vector<uint8_t> rank(size_t(n_elems) * nn);
+            for (i64 e = 0; e < n_elems; ++e)
+                for (int i = 0; i < nn; ++i) {
+                    const i32 v = elems[e * nn + i];
+                    int r = 0;
+                    while (ids[off[v] + r] != e) ++r;
+                    rank[e * nn + i] = uint8_t(r);
+                }
+            upload(m->el_rank, rank.data(), rank.size(), s);
+        }
         upload(m->xy, xy, size_t(2 * n_nodes), s);
         upload(m->el, elems, size_t(n_elems * nn), s);
         upload(m->ne_off, off.data(), off.size(), s);

@@ -124,6 +124,77 @@ __global__ void k_expand(i64 E, int nn, int strategy, const i32* __restrict__ el
     if (is_min) atomicMin(&out[g], bits); else atomicMax(&out[g], bits);
 }
 
+// MEAN strategy (find_neighbours.hpp:55-71): the mean of |c_e - c_a| over the distinct unordered
+// pairs {e, a} met by the loop "e in the group, node of e, element a at that node" (self pairs
+// included), summed in the order of that loop.  A pair is counted where the loop meets it first:
+// at its lower element when both lie in the group (the outer loop reaches that one first), and at
+// the first node of e that a shares.  PASS 0 counts the terms of each element, PASS 1 writes them
+// (sqrt of the squared distance, evaluated as the reference does) at the element's scanned offset,
+// so each group's terms are one contiguous run in loop order.
+template<int PASS>
+__global__ void k_mean_terms(i64 E, int nn, const i32* __restrict__ el, const i32* __restrict__ group_of,
+                             const int* __restrict__ searched, const i32* __restrict__ ne_off,
+                             const i32* __restrict__ ne_ids, const double* __restrict__ c, i64* __restrict__ cnt,
+                             const i64* __restrict__ off, double* __restrict__ terms) {
+    const i64 e = i64(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (e >= E) return;
+    const int g = group_of[e];
+    if (!searched[g]) {
+        if (PASS == 0) cnt[e] = 0;
+        return;
+    }
+    i64 n = 0;
+    const i64 pos = PASS ? off[e] : 0;
+    for (int i = 0; i < nn; ++i) {
+        const i32 v = el[e * nn + i];
+        for (i32 k = ne_off[v]; k < ne_off[v + 1]; ++k) {
+            const i32 a = ne_ids[k];
+            if (a != e && a < e && group_of[a] == g) continue;     // met first in a's own iteration
+            bool earlier = false;                                  // a shares an earlier node of e
+            for (int i2 = 0; i2 < i && !earlier; ++i2) {
+                const i32 v2 = el[e * nn + i2];
+                for (int j = 0; j < nn; ++j) earlier |= el[i64(a) * nn + j] == v2;
+            }
+            if (earlier) continue;
+            if (PASS == 1) {
+                const double z0 = c[2 * e] - c[2 * i64(a)], z1 = c[2 * e + 1] - c[2 * i64(a) + 1];
+                double d = 0.0;
+                d = d + z0 * z0;
+                d = d + z1 * z1;
+                terms[pos + n] = sqrt(d);
+            }
+            ++n;
+        }
+    }
+    if (PASS == 0) cnt[e] = n;
+}
+
+// Serial sum of each group's run of MEAN terms in loop order (one block per group): the block
+// stages tiles of the run in LDS with coalesced loads and one thread adds them one after the
+// other, so the rounding is that of the reference's running sum.  out[g] = sum / count.
+constexpr int MEAN_TILE = 2048;
+__global__ __launch_bounds__(256) void k_mean_sum(const i64* __restrict__ gbeg, const int* __restrict__ searched,
+                                                  const i64* __restrict__ off, const double* __restrict__ terms,
+                                                  double* __restrict__ out) {
+    __shared__ double tile[MEAN_TILE];
+    const int g = blockIdx.x;
+    if (!searched[g]) {
+        if (threadIdx.x == 0) out[g] = 0.0;
+        return;
+    }
+    const i64 b = off[gbeg[g]], e = off[gbeg[g + 1]];
+    double sum = 0.0;
+    for (i64 t0 = b; t0 < e; t0 += MEAN_TILE) {
+        const int len = int(e - t0 < MEAN_TILE ? e - t0 : MEAN_TILE);
+        for (int t = threadIdx.x; t < len; t += blockDim.x) tile[t] = terms[t0 + t];
+        __syncthreads();
+        if (threadIdx.x == 0)
+            for (int t = 0; t < len; ++t) sum = sum + tile[t];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[g] = e > b ? sum / double(e - b) : 0.0;
+}
+
 // group_corners (find_neighbours.hpp:40-47): min/max over the nodes of the group's elements.
 __global__ void k_bbox(i64 E, int nn, const i32* __restrict__ el, const i32* __restrict__ group_of,
                        const int* __restrict__ searched, const double* __restrict__ xy,
@@ -257,13 +328,13 @@ extern "C" {
 // find_neighbours (find_neighbours.hpp:119-160) + mesh_2d::neighbours (mesh_2d.hpp:205-212).
 // search_radii[2G]: per-group search radius (search_radius, else nonlocal_radius; read_model.hpp:226-244);
 // a group with a radius <= 0 is not searched and its elements get empty lists.  expanded_out[2G]
-// (optional) receives the expanded radii.  Strategies NO, MAX, MIN are supported.
+// (optional) receives the expanded radii.  Strategies NO, MAX, MIN and MEAN.
 int nlf_find_neighbours(nlf_mesh m, const double* search_radii, int strategy, double* expanded_out) {
     if (!m) return NLF_ERR_ARGUMENT;
     return guarded(m->ctx, [&] {
         NLF_CHECK(search_radii, NLF_ERR_ARGUMENT, "null radii");
-        NLF_CHECK(strategy == NLF_EXPAND_NO || strategy == NLF_EXPAND_MAX || strategy == NLF_EXPAND_MIN,
-                  NLF_ERR_UNSUPPORTED, "diameter expanding strategy MEAN is not implemented");
+        NLF_CHECK(strategy == NLF_EXPAND_NO || strategy == NLF_EXPAND_MAX || strategy == NLF_EXPAND_MIN ||
+                      strategy == NLF_EXPAND_MEAN, NLF_ERR_ARGUMENT, "unknown diameter expanding strategy");
         const auto t0 = std::chrono::steady_clock::now();
         hipStream_t s = m->ctx->stream;
         const int G = m->G;
@@ -275,7 +346,29 @@ int nlf_find_neighbours(nlf_mesh m, const double* search_radii, int strategy, do
         upload(d_searched, searched.data(), size_t(G), s);
         // expansion
         std::vector<double> add(G, 0.0);
-        if (strategy != NLF_EXPAND_NO) {
+        if (strategy == NLF_EXPAND_MEAN) {
+            DevBuf<i64> d_off{size_t(E + 1)};
+            hipLaunchKernelGGL(k_mean_terms<0>, dim3(blocks_for(E, 256)), dim3(256), 0, s, E, m->nn, m->el.p,
+                               m->group_of.p, d_searched.p, m->ne_off.p, m->ne_ids.p, m->centres.p, d_off.p,
+                               (const i64*)nullptr, (double*)nullptr);
+            NLF_HIP(hipGetLastError());
+            exclusive_scan_i64(d_off.p, E, s);
+            i64 terms_total = 0;
+            download(&terms_total, d_off.p + E, 1, s);
+            DevBuf<double> d_terms{size_t(std::max<i64>(terms_total, 1))};
+            hipLaunchKernelGGL(k_mean_terms<1>, dim3(blocks_for(E, 256)), dim3(256), 0, s, E, m->nn, m->el.p,
+                               m->group_of.p, d_searched.p, m->ne_off.p, m->ne_ids.p, m->centres.p, (i64*)nullptr,
+                               d_off.p, d_terms.p);
+            NLF_HIP(hipGetLastError());
+            std::vector<i64> gbeg(m->gbeg_h.begin(), m->gbeg_h.end());
+            DevBuf<i64> d_gbeg;
+            upload(d_gbeg, gbeg.data(), gbeg.size(), s);
+            DevBuf<double> d_mean{size_t(G)};
+            hipLaunchKernelGGL(k_mean_sum, dim3(G), dim3(256), 0, s, d_gbeg.p, d_searched.p, d_off.p, d_terms.p,
+                               d_mean.p);
+            NLF_HIP(hipGetLastError());
+            download(add.data(), d_mean.p, size_t(G), s);
+        } else if (strategy != NLF_EXPAND_NO) {
             std::vector<unsigned long long> init(G, strategy == NLF_EXPAND_MIN ? 0x7fefffffffffffffull : 0ull);
             DevBuf<unsigned long long> d_exp;
             upload(d_exp, init.data(), size_t(G), s);

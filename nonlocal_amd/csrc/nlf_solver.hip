@@ -30,7 +30,7 @@ constexpr int LEVEL_THREADS = 1024;
 
 struct CGState {
     double rz, pq, alpha, beta, bnorm, res, tol;
-    int done, pad;
+    int done, indefinite;                // indefinite: p'Ap < 0 is not a breakdown (bordered Neumann systems)
     i64 it, max_it;
     int breakdown, pad2;
 };
@@ -244,7 +244,8 @@ __global__ void k_alpha(const double* __restrict__ partial, int n, CGState* __re
     if (threadIdx.x == 0) {
         st->pq = s;
         st->alpha = st->rz / s;
-        if (!(s > 0.0) || !isfinite(st->alpha)) { st->breakdown = 1; st->done = 1; }
+        const bool bad_curvature = st->indefinite ? s == 0.0 : !(s > 0.0);
+        if (bad_curvature || !isfinite(st->alpha)) { st->breakdown = 1; st->done = 1; }
     }
 }
 
@@ -312,7 +313,7 @@ __global__ void k_update_p(i64 n, X* __restrict__ p, const X* __restrict__ z, co
 
 // CG start: rz = r.z, bnorm from partials, residual, done.
 __global__ void k_init_state(const double* __restrict__ prz, const double* __restrict__ pbb, int n, double tol, i64 max_it,
-                             CGState* __restrict__ st) {
+                             int indefinite, CGState* __restrict__ st) {
     __shared__ double sh[1024 / 64];
     double a = 0.0, b = 0.0;
     for (int i = threadIdx.x; i < n; i += blockDim.x) { a += prz[i]; b += pbb[i]; }
@@ -325,6 +326,7 @@ __global__ void k_init_state(const double* __restrict__ prz, const double* __res
         st->it = 0;
         st->max_it = max_it;
         st->breakdown = 0;
+        st->indefinite = indefinite;
         st->res = st->bnorm > 0.0 ? sqrt(fabs(a)) / st->bnorm : 0.0;
         st->done = (st->res <= tol || max_it <= 0) ? 1 : 0;
     }
@@ -344,7 +346,8 @@ __global__ void k_add_to(i64 n, const S* __restrict__ x, Y* __restrict__ y) {
 
 // rz from partial sums; `first` also initialises the state of a reliable-update solve (no
 // convergence test inside: the host decides on the FP64 true residual).
-__global__ void k_set_rz(const double* __restrict__ partial, int n, int first, i64 max_it, CGState* __restrict__ st) {
+__global__ void k_set_rz(const double* __restrict__ partial, int n, int first, i64 max_it, int indefinite,
+                         CGState* __restrict__ st) {
     __shared__ double sh[1024 / 64];
     double s = 0.0;
     for (int i = threadIdx.x; i < n; i += blockDim.x) s += partial[i];
@@ -358,6 +361,7 @@ __global__ void k_set_rz(const double* __restrict__ partial, int n, int first, i
             st->tol = -1.0;
             st->bnorm = 1.0;
             st->breakdown = 0;
+            st->indefinite = indefinite;
         }
         st->res = sqrt(fabs(s));
     }
@@ -569,65 +573,87 @@ __global__ void k_sweep_gather(i64 n, const i32* __restrict__ row, const X* __re
     }
 }
 
-// One level-scheduled triangular sweep by one workgroup (blockDim >= widest level).  The loads of
-// level L+1 are issued before level L is solved, so only the LDS reads of the dependencies and the
-// barrier remain on the critical path.  out[row] = v; out2[row] = scale2[rec] * v when given.
-template<class X, int WE>
-__global__ __launch_bounds__(LEVEL_THREADS) void k_ic_sweep(EllSweep<X> s, const X* __restrict__ rhs, X* __restrict__ out,
-                                                            X* __restrict__ out2, const X* __restrict__ scale2,
-                                                            const CGState* __restrict__ st) {
+// One level-scheduled triangular sweep by one workgroup (blockDim >= widest level).  Each thread owns
+// record t of every level; the record data of level L + D (right-hand side, 1/U_kk, the WE entries
+// and their dependency slots, the row, the backward scale) are loaded D levels ahead into a ring of
+// D register slots, and the level offsets are staged in LDS, so a level's critical path is the LDS
+// reads of its dependencies, a few FMAs and the barrier: one exposed global-load latency per level
+// (about 1.7 us) becomes one per D levels.  The arithmetic of a record is unchanged.
+// out[row] = v; out2[row] = scale2[rec] * v when given.
+constexpr size_t SWEEP_LDS_BYTES = 64 * 1024;
+
+// Levels the ring holds, sized so that the ring and the rest of the kernel fit the registers a
+// lane has at MAXT threads per workgroup (256 VGPRs at 512, 128 at 1024) without spilling.
+template<int WE, int MAXT>
+constexpr int sweep_depth() {
+    return MAXT <= 512 ? (WE <= 4 ? 8 : WE <= 8 ? 4 : WE <= 16 ? 2 : 1) : (WE <= 4 ? 4 : WE <= 8 ? 2 : 1);
+}
+
+template<class X, int WE, int MAXT>
+__global__ __launch_bounds__(MAXT) void k_ic_sweep(EllSweep<X> s, const X* __restrict__ rhs, X* __restrict__ out,
+                                                   X* __restrict__ out2, const X* __restrict__ scale2,
+                                                   const CGState* __restrict__ st, int lptr_in_lds) {
+    constexpr int D = sweep_depth<WE, MAXT>();
     extern __shared__ unsigned char smem_raw[];
     X* win = reinterpret_cast<X*>(smem_raw);
+    i32* lp = reinterpret_cast<i32*>(smem_raw + ((size_t(s.K) * size_t(s.W) * sizeof(X) + 15) & ~size_t(15)));
     if (st && st->done) return;
     const int t = threadIdx.x;
     if (s.K)
         for (int i = t; i < s.K * s.W; i += blockDim.x) win[i] = X(0);
-    X c_rhs = 0, c_dinv = 0, c_val[WE];
-    i32 c_dep[WE], c_row = 0;
-    auto load = [&](int L, X& rv, X& dv, X* vv, i32* dd, i32& rw) {
-        const i32 b = s.lptr[L], cnt = s.lptr[L + 1] - b;
-        if (t < cnt) {
-            const i32 rec = b + t;
-            rv = rhs[rec];
-            dv = s.dinv[rec];
-            rw = s.row[rec];
-            const i64 e0 = i64(WE) * b + t;
+    if (lptr_in_lds)
+        for (int i = t; i <= s.levels; i += blockDim.x) lp[i] = s.lptr[i];
+    __syncthreads();
+    auto level_begin = [&](int L) { return lptr_in_lds ? lp[L] : s.lptr[L]; };
+    X r_rhs[D], r_dinv[D], r_val[D][WE], r_sc[D];
+    i32 r_dep[D][WE], r_row[D];
+    // the loads of a level are issued by every thread (past the level's width a thread reads the
+    // level's last record again), so that each thread issues the same loads in the same order
+    auto load = [&](int L, X& rv, X& dv, X (&vv)[WE], i32 (&dd)[WE], i32& rw, X& sc) {
+        const i32 b = level_begin(L), cnt = level_begin(L + 1) - b;
+        const i32 tt = t < cnt ? t : cnt - 1;
+        const i32 rec = b + tt;
+        rv = rhs[rec];
+        dv = s.dinv[rec];
+        rw = s.row[rec];
+        if (scale2) sc = scale2[rec];
+        const i64 e0 = i64(WE) * b + tt;
 #pragma unroll
-            for (int j = 0; j < WE; ++j) {
-                vv[j] = s.val[e0 + i64(j) * cnt];
-                dd[j] = s.dep[e0 + i64(j) * cnt];
-            }
+        for (int j = 0; j < WE; ++j) {
+            vv[j] = s.val[e0 + i64(j) * cnt];
+            dd[j] = s.dep[e0 + i64(j) * cnt];
         }
     };
-    if (s.levels > 0) load(0, c_rhs, c_dinv, c_val, c_dep, c_row);
-    __syncthreads();
-    for (int L = 0; L < s.levels; ++L) {
-        X n_rhs = 0, n_dinv = 0, n_val[WE];
-        i32 n_dep[WE], n_row = 0;
-        if (L + 1 < s.levels) load(L + 1, n_rhs, n_dinv, n_val, n_dep, n_row);
-        const i32 cnt = s.lptr[L + 1] - s.lptr[L];
-        if (t < cnt) {
-            X v = c_rhs;
 #pragma unroll
-            for (int j = 0; j < WE; ++j) {
-                const i32 d = c_dep[j];
-                v -= c_val[j] * (d >= 0 ? win[d] : out[-d - 1]);
+    for (int u = 0; u < D; ++u)
+        if (u < s.levels) load(u, r_rhs[u], r_dinv[u], r_val[u], r_dep[u], r_row[u], r_sc[u]);
+    for (int L0 = 0; L0 < s.levels; L0 += D) {
+#pragma unroll
+        for (int u = 0; u < D; ++u) {
+            const int L = L0 + u;
+            if (L >= s.levels) break;
+            const i32 b = level_begin(L), cnt = level_begin(L + 1) - b;
+            if (t < cnt) {
+                X v = r_rhs[u];
+#pragma unroll
+                for (int j = 0; j < WE; ++j) {
+                    const i32 d = r_dep[u][j];
+                    v -= r_val[u][j] * (d >= 0 ? win[d] : out[-d - 1]);
+                }
+                v *= r_dinv[u];
+                if (s.K) win[(L % s.K) * s.W + t] = v;
+                out[r_row[u]] = v;
+                if (out2) out2[r_row[u]] = r_sc[u] * v;
             }
-            v *= c_dinv;
-            if (s.K) win[(L % s.K) * s.W + t] = v;
-            out[c_row] = v;
-            if (out2) out2[c_row] = scale2[s.lptr[L] + t] * v;
+            if (s.K) {
+                // every dependency lives in the LDS window: wait for LDS traffic only, so the global
+                // loads prefetched for the next levels stay in flight across the barrier
+                asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
+            } else {
+                __syncthreads();
+            }
+            if (L + D < s.levels) load(L + D, r_rhs[u], r_dinv[u], r_val[u], r_dep[u], r_row[u], r_sc[u]);
         }
-        if (s.K) {
-            // every dependency lives in the LDS window: wait for LDS traffic only, so the global
-            // loads prefetched for the next level stay in flight across the barrier
-            asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
-        } else {
-            __syncthreads();
-        }
-        c_rhs = n_rhs; c_dinv = n_dinv; c_row = n_row;
-#pragma unroll
-        for (int j = 0; j < WE; ++j) { c_val[j] = n_val[j]; c_dep[j] = n_dep[j]; }
     }
 }
 
@@ -918,13 +944,27 @@ template<> const float* ell_scale<float>(const nlf_precond_s::Ell& e) { return e
 template<class X>
 void ell_sweep(const nlf_precond_s::Ell& e, const X* rhs, X* out, X* out2, const X* scale2, const CGState* st,
                hipStream_t s) {
-    const size_t lds = size_t(std::max(e.K * e.W, 1)) * sizeof(X);
+    // LDS: the dependency window, then the level offsets when both fit SWEEP_LDS_BYTES
+    const size_t win = (size_t(std::max(e.K * e.W, 1)) * sizeof(X) + 15) & ~size_t(15);
+    const size_t offs = size_t(e.levels + 1) * sizeof(i32);
+    const int in_lds = win + offs <= SWEEP_LDS_BYTES ? 1 : 0;
+    const size_t lds = win + (in_lds ? offs : 0);
     const EllSweep<X> a = e.template get<X>();
-    switch (e.WE) {
-        case 4: hipLaunchKernelGGL((k_ic_sweep<X, 4>), dim3(1), dim3(e.threads), lds, s, a, rhs, out, out2, scale2, st); break;
-        case 8: hipLaunchKernelGGL((k_ic_sweep<X, 8>), dim3(1), dim3(e.threads), lds, s, a, rhs, out, out2, scale2, st); break;
-        case 16: hipLaunchKernelGGL((k_ic_sweep<X, 16>), dim3(1), dim3(e.threads), lds, s, a, rhs, out, out2, scale2, st); break;
-        default: hipLaunchKernelGGL((k_ic_sweep<X, 32>), dim3(1), dim3(e.threads), lds, s, a, rhs, out, out2, scale2, st); break;
+    const dim3 g(1), b(unsigned(e.threads));
+    if (e.threads <= 512) {
+        switch (e.WE) {
+            case 4: hipLaunchKernelGGL((k_ic_sweep<X, 4, 512>), g, b, lds, s, a, rhs, out, out2, scale2, st, in_lds); break;
+            case 8: hipLaunchKernelGGL((k_ic_sweep<X, 8, 512>), g, b, lds, s, a, rhs, out, out2, scale2, st, in_lds); break;
+            case 16: hipLaunchKernelGGL((k_ic_sweep<X, 16, 512>), g, b, lds, s, a, rhs, out, out2, scale2, st, in_lds); break;
+            default: hipLaunchKernelGGL((k_ic_sweep<X, 32, 512>), g, b, lds, s, a, rhs, out, out2, scale2, st, in_lds); break;
+        }
+    } else {
+        switch (e.WE) {
+            case 4: hipLaunchKernelGGL((k_ic_sweep<X, 4, 1024>), g, b, lds, s, a, rhs, out, out2, scale2, st, in_lds); break;
+            case 8: hipLaunchKernelGGL((k_ic_sweep<X, 8, 1024>), g, b, lds, s, a, rhs, out, out2, scale2, st, in_lds); break;
+            case 16: hipLaunchKernelGGL((k_ic_sweep<X, 16, 1024>), g, b, lds, s, a, rhs, out, out2, scale2, st, in_lds); break;
+            default: hipLaunchKernelGGL((k_ic_sweep<X, 32, 1024>), g, b, lds, s, a, rhs, out, out2, scale2, st, in_lds); break;
+        }
     }
 }
 
@@ -1167,7 +1207,21 @@ void residual64(i64 n, const i64* outer, SpmvCols cs, const double* val, int lpr
     hipLaunchKernelGGL(k_resid_finish, dim3(nb), dim3(SPMV_THREADS), 0, s, n, b, r, prr, pbb);
 }
 
+// Work memory of one CG solve: the vectors r, z, p (n entries of X each, 256-byte aligned), five
+// arrays of per-block partial sums (SPMV_MAX_BLOCKS doubles each: every launch of the solve writes at
+// most that many), two reduced sums and the solver state.
+struct CGWork {
+    static size_t align(size_t b) { return (b + 255) & ~size_t(255); }
+    template<class X>
+    static size_t bytes(i64 n) {
+        return 3 * align(size_t(n) * sizeof(X)) + 5 * align(SPMV_MAX_BLOCKS * sizeof(double)) + align(2 * sizeof(double)) +
+               align(sizeof(CGState));
+    }
+};
+
 // Preconditioned CG on device vectors (conjugate_gradient::solve).  Vectors of type X, matrix values V.
+// The work memory (CGWork) is allocated by the object, or given by the caller (no allocation then;
+// `timed` = false also records no HIP events).
 template<class V, class X>
 struct CG {
     hipStream_t s;
@@ -1179,24 +1233,37 @@ struct CG {
     const V* val;
     const int16_t* lo = nullptr;        // split values: 16-bit remainders (NLF_MODE_SPLIT)
     const nlf_precond_s* P;
-    DevBuf<X> r, z, p;
-    DevBuf<double> part1, part2;
-    DevBuf<CGState> st;
+    DevBuf<unsigned char> own;
+    X *r = nullptr, *z = nullptr, *p = nullptr;
+    double *part1 = nullptr, *part2 = nullptr, *pbb = nullptr, *prz = nullptr, *pb2 = nullptr, *sums = nullptr;
+    CGState* st = nullptr;
     unsigned sblocks, vblocks;
+    bool timed = true;
+    bool indefinite = false;            // a bordered (Neumann) system: negative curvature allowed
     KernelTimer timer;
 
-    CG(hipStream_t s_, i64 n_, i64 nnz, const i64* o, const i32* c, const V* v, const nlf_precond_s* P_)
+    CG(hipStream_t s_, i64 n_, i64 nnz, const i64* o, const i32* c, const V* v, const nlf_precond_s* P_,
+       void* work = nullptr)
         : s(s_), n(n_), outer(o), cols(c), val(v), P(P_) {
         cs.cols = c;
         lpr = pick_lpr(double(nnz) / double(std::max<i64>(n, 1)));
-        r.alloc(size_t(n)); z.alloc(size_t(n)); p.alloc(size_t(n));
+        if (!work) {
+            own.alloc(CGWork::bytes<X>(n));
+            work = own.p;
+        } else {
+            timed = false;
+        }
+        unsigned char* w = static_cast<unsigned char*>(work);
+        auto take = [&](size_t b) { unsigned char* q = w; w += CGWork::align(b); return q; };
+        r = reinterpret_cast<X*>(take(size_t(n) * sizeof(X)));
+        z = reinterpret_cast<X*>(take(size_t(n) * sizeof(X)));
+        p = reinterpret_cast<X*>(take(size_t(n) * sizeof(X)));
+        double** parts[5] = {&part1, &part2, &pbb, &prz, &pb2};
+        for (double** q : parts) *q = reinterpret_cast<double*>(take(SPMV_MAX_BLOCKS * sizeof(double)));
+        sums = reinterpret_cast<double*>(take(2 * sizeof(double)));
+        st = reinterpret_cast<CGState*>(take(sizeof(CGState)));
         sblocks = spmv_blocks(lpr, n);
         vblocks = grid_stride_blocks(n);
-        // k_spmv (sblocks), vector kernels (vblocks) and k_residual (one wave per row) write partials
-        const size_t np = std::max<size_t>(std::max<size_t>(sblocks, vblocks), spmv_blocks(64, n));
-        part1.alloc(np);
-        part2.alloc(np);
-        st.alloc(1);
     }
 
     bool jacobi() const { return P && P->kind == NLF_PRECOND_JACOBI; }
@@ -1214,28 +1281,27 @@ struct CG {
     i64 solve(const X* b, X* x, double tol, i64 max_it, double* residual, int* breakdown, int check_every = 8) {
         // r = b - A x ; z = M r ; rz = r.z ; bnorm
         hipLaunchKernelGGL((k_residual<V, X>), dim3(spmv_blocks(64, n)), dim3(SPMV_THREADS), 0, s, n, outer, cols, val, x, b,
-                           r.p, part1.p, part2.p);
+                           r, part1, part2);
         const int nb_res = int(spmv_blocks(64, n));
-        DevBuf<double> pbb{size_t(nb_res)};
-        NLF_HIP(hipMemcpyAsync(pbb.p, part2.p, size_t(nb_res) * sizeof(double), hipMemcpyDeviceToDevice, s));
-        precond_apply<X>(P, n, r.p, p.p, nullptr, s);
-        hipLaunchKernelGGL(k_dot<X>, dim3(vblocks), dim3(SPMV_THREADS), 0, s, n, r.p, p.p, part1.p, (const CGState*)nullptr);
-        // pad partial arrays: k_init_state sums vblocks of r.z and nb_res of b.b
-        DevBuf<double> prz{size_t(std::max<unsigned>(vblocks, unsigned(nb_res)))};
-        NLF_HIP(hipMemsetAsync(prz.p, 0, prz.bytes(), s));
-        NLF_HIP(hipMemcpyAsync(prz.p, part1.p, size_t(vblocks) * sizeof(double), hipMemcpyDeviceToDevice, s));
-        DevBuf<double> pb2{size_t(std::max<unsigned>(vblocks, unsigned(nb_res)))};
-        NLF_HIP(hipMemsetAsync(pb2.p, 0, pb2.bytes(), s));
-        NLF_HIP(hipMemcpyAsync(pb2.p, pbb.p, size_t(nb_res) * sizeof(double), hipMemcpyDeviceToDevice, s));
-        hipLaunchKernelGGL(k_init_state, dim3(1), dim3(1024), 0, s, prz.p, pb2.p, int(prz.n), tol, max_it, st.p);
+        NLF_HIP(hipMemcpyAsync(pbb, part2, size_t(nb_res) * sizeof(double), hipMemcpyDeviceToDevice, s));
+        precond_apply<X>(P, n, r, p, nullptr, s);
+        hipLaunchKernelGGL(k_dot<X>, dim3(vblocks), dim3(SPMV_THREADS), 0, s, n, r, p, part1, (const CGState*)nullptr);
+        // pad partial arrays: k_init_state sums vblocks of r.z and nb_res of b.b (cleared by a kernel,
+        // not a memset, so that the sequence may be captured)
+        const int npad = int(std::max<unsigned>(vblocks, unsigned(nb_res)));
+        hipLaunchKernelGGL(k_zero<double>, dim3(grid_stride_blocks(npad)), dim3(256), 0, s, i64(npad), prz);
+        hipLaunchKernelGGL(k_zero<double>, dim3(grid_stride_blocks(npad)), dim3(256), 0, s, i64(npad), pb2);
+        NLF_HIP(hipMemcpyAsync(prz, part1, size_t(vblocks) * sizeof(double), hipMemcpyDeviceToDevice, s));
+        NLF_HIP(hipMemcpyAsync(pb2, pbb, size_t(nb_res) * sizeof(double), hipMemcpyDeviceToDevice, s));
+        hipLaunchKernelGGL(k_init_state, dim3(1), dim3(1024), 0, s, prz, pb2, npad, tol, max_it, indefinite ? 1 : 0, st);
         CGState hs{};
-        NLF_HIP(hipMemcpyAsync(&hs, st.p, sizeof(CGState), hipMemcpyDeviceToHost, s));
+        NLF_HIP(hipMemcpyAsync(&hs, st, sizeof(CGState), hipMemcpyDeviceToHost, s));
         NLF_HIP(hipStreamSynchronize(s));
         while (!hs.done) {
             // the first iteration of each chunk starts with done == 0, so its kernels run in full:
             // HIP events around its SpMV and preconditioner give the live kernel timings
-            for (int k = 0; k < check_every; ++k) iteration(x, k == 0);
-            NLF_HIP(hipMemcpyAsync(&hs, st.p, sizeof(CGState), hipMemcpyDeviceToHost, s));
+            for (int k = 0; k < check_every; ++k) iteration(x, timed && k == 0);
+            NLF_HIP(hipMemcpyAsync(&hs, st, sizeof(CGState), hipMemcpyDeviceToHost, s));
             NLF_HIP(hipStreamSynchronize(s));
         }
         timer.collect();
@@ -1252,24 +1318,25 @@ struct CG {
     i64 solve_reliable(const i64* o64, const i32* c64, const double* v64, const double* b, double* x, double tol,
                        i64 max_it, double delta, double* true_res, i64* updates, int* breakdown) {
         const unsigned rb = spmv_blocks(64, n);
-        DevBuf<double> r64{size_t(n)}, sums{2};
+        DevBuf<double> r64{size_t(n)};
         DevBuf<X> xh{size_t(n)};
         auto true_residual = [&]() {
-            residual64(n, o64, cs, v64, lpr, x, b, r64.p, part1.p, part2.p, rb, s);
-            hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(1024), 0, s, part1.p, int(rb), sums.p);
-            hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(1024), 0, s, part2.p, int(rb), sums.p + 1);
+            residual64(n, o64, cs, v64, lpr, x, b, r64.p, part1, part2, rb, s);
+            hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(1024), 0, s, part1, int(rb), sums);
+            hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(1024), 0, s, part2, int(rb), sums + 1);
             double h[2];
-            download(h, sums.p, 2, s);
+            download(h, sums, 2, s);
             return h[1] > 0.0 ? std::sqrt(h[0] / h[1]) : 0.0;
         };
         // r = M^-1-preconditioned restart data from the FP64 residual: r32 = r64, z = M r32, rz = r32.z
         auto load_residual = [&](bool first) {
-            hipLaunchKernelGGL((k_convert<double, X>), dim3(vblocks), dim3(256), 0, s, n, r64.p, r.p);
-            precond_apply<X>(P, n, r.p, z.p, nullptr, s);
-            if (first) NLF_HIP(hipMemcpyAsync(p.p, z.p, size_t(n) * sizeof(X), hipMemcpyDeviceToDevice, s));
-            hipLaunchKernelGGL(k_dot<X>, dim3(vblocks), dim3(SPMV_THREADS), 0, s, n, r.p, z.p, part1.p,
+            hipLaunchKernelGGL((k_convert<double, X>), dim3(vblocks), dim3(256), 0, s, n, r64.p, r);
+            precond_apply<X>(P, n, r, z, nullptr, s);
+            if (first) NLF_HIP(hipMemcpyAsync(p, z, size_t(n) * sizeof(X), hipMemcpyDeviceToDevice, s));
+            hipLaunchKernelGGL(k_dot<X>, dim3(vblocks), dim3(SPMV_THREADS), 0, s, n, r, z, part1,
                                (const CGState*)nullptr);
-            hipLaunchKernelGGL(k_set_rz, dim3(1), dim3(1024), 0, s, part1.p, int(vblocks), first ? 1 : 0, max_it, st.p);
+            hipLaunchKernelGGL(k_set_rz, dim3(1), dim3(1024), 0, s, part1, int(vblocks), first ? 1 : 0, max_it,
+                               indefinite ? 1 : 0, st);
         };
         i64 nupd = 0;
         double rel = true_residual();
@@ -1277,7 +1344,7 @@ struct CG {
         CGState hs{};
         if (rel > tol && max_it > 0) {
             load_residual(true);
-            NLF_HIP(hipMemcpyAsync(&hs, st.p, sizeof(CGState), hipMemcpyDeviceToHost, s));
+            NLF_HIP(hipMemcpyAsync(&hs, st, sizeof(CGState), hipMemcpyDeviceToHost, s));
             NLF_HIP(hipStreamSynchronize(s));
             double last = std::sqrt(std::fabs(hs.rz));
             // the recursive estimate sqrt(r'z) is preconditioned; q maps it to the relative true
@@ -1286,7 +1353,7 @@ struct CG {
             double q = last > 0.0 ? rel / last : 0.0;
             for (;;) {
                 for (int k = 0; k < 8; ++k) iteration(xh.p, k == 0);
-                NLF_HIP(hipMemcpyAsync(&hs, st.p, sizeof(CGState), hipMemcpyDeviceToHost, s));
+                NLF_HIP(hipMemcpyAsync(&hs, st, sizeof(CGState), hipMemcpyDeviceToHost, s));
                 NLF_HIP(hipStreamSynchronize(s));
                 const double est = std::sqrt(std::fabs(hs.rz));
                 const bool stop = hs.it >= max_it || hs.breakdown;
@@ -1297,7 +1364,7 @@ struct CG {
                 rel = true_residual();
                 if (rel <= tol || stop) break;
                 load_residual(false);
-                NLF_HIP(hipMemcpyAsync(&hs, st.p, sizeof(CGState), hipMemcpyDeviceToHost, s));
+                NLF_HIP(hipMemcpyAsync(&hs, st, sizeof(CGState), hipMemcpyDeviceToHost, s));
                 NLF_HIP(hipStreamSynchronize(s));
                 last = std::sqrt(std::fabs(hs.rz));
                 q = last > 0.0 ? rel / last : 0.0;
@@ -1313,27 +1380,27 @@ struct CG {
     void iteration(X* x, bool sample = false) {
         // z = A p ; alpha = rz / p.z
         if (sample) timer.start(0, s);
-        launch_spmv<V, X, double, true>(lpr, n, outer, cs, val, p.p, z.p, part1.p, st.p, s, false, lo);
+        launch_spmv<V, X, double, true>(lpr, n, outer, cs, val, p, z, part1, st, s, false, lo);
         if (sample) timer.stop(0, s);
-        hipLaunchKernelGGL(k_alpha, dim3(1), dim3(1024), 0, s, part1.p, int(sblocks), st.p);
+        hipLaunchKernelGGL(k_alpha, dim3(1), dim3(1024), 0, s, part1, int(sblocks), st);
         if (!P || P->kind == NLF_PRECOND_IDENTITY || jacobi()) {
             // x += a p ; r -= a z ; z = M r ; partial r.z  (fused)
             if (jacobi())
-                hipLaunchKernelGGL((k_update_xr<X, true, true>), dim3(vblocks), dim3(SPMV_THREADS), 0, s, n, x, r.p, p.p, z.p,
-                                   z.p, invd(), part2.p, st.p);
+                hipLaunchKernelGGL((k_update_xr<X, true, true>), dim3(vblocks), dim3(SPMV_THREADS), 0, s, n, x, r, p, z,
+                                   z, invd(), part2, st);
             else
-                hipLaunchKernelGGL((k_update_xr<X, false, true>), dim3(vblocks), dim3(SPMV_THREADS), 0, s, n, x, r.p, p.p,
-                                   z.p, z.p, (const X*)nullptr, part2.p, st.p);
+                hipLaunchKernelGGL((k_update_xr<X, false, true>), dim3(vblocks), dim3(SPMV_THREADS), 0, s, n, x, r, p,
+                                   z, z, (const X*)nullptr, part2, st);
         } else {
-            hipLaunchKernelGGL((k_update_xr<X, false, false>), dim3(vblocks), dim3(SPMV_THREADS), 0, s, n, x, r.p, p.p, z.p,
-                               (X*)nullptr, (const X*)nullptr, part2.p, st.p);
+            hipLaunchKernelGGL((k_update_xr<X, false, false>), dim3(vblocks), dim3(SPMV_THREADS), 0, s, n, x, r, p, z,
+                               (X*)nullptr, (const X*)nullptr, part2, st);
             if (sample) timer.start(1, s);
-            precond_apply<X>(P, n, r.p, z.p, st.p, s);
+            precond_apply<X>(P, n, r, z, st, s);
             if (sample) timer.stop(1, s);
-            hipLaunchKernelGGL(k_dot<X>, dim3(vblocks), dim3(SPMV_THREADS), 0, s, n, r.p, z.p, part2.p, st.p);
+            hipLaunchKernelGGL(k_dot<X>, dim3(vblocks), dim3(SPMV_THREADS), 0, s, n, r, z, part2, st);
         }
-        hipLaunchKernelGGL(k_beta, dim3(1), dim3(1024), 0, s, part2.p, int(vblocks), st.p);
-        hipLaunchKernelGGL(k_update_p<X>, dim3(vblocks), dim3(256), 0, s, n, p.p, z.p, st.p);
+        hipLaunchKernelGGL(k_beta, dim3(1), dim3(1024), 0, s, part2, int(vblocks), st);
+        hipLaunchKernelGGL(k_update_p<X>, dim3(vblocks), dim3(256), 0, s, n, p, z, st);
     }
 };
 
@@ -1600,6 +1667,7 @@ int nlf_cg_solve(nlf_matrix a, nlf_precond p, const double* b, double* x, double
         if (mode == NLF_MODE_FP64) {
             CG<double, double> cg(s, n, a->nnz_full, a->outer.p, a->cols.p, a->val.p, p);
             cg.set_cols(a);
+            cg.indefinite = a->neumann;
             out.iterations = cg.solve(b, x, tol, max_it, &out.residual, &breakdown);
             timings(cg.timer, sizeof(double), sizeof(double));
         } else {
@@ -1612,6 +1680,7 @@ int nlf_cg_solve(nlf_matrix a, nlf_precond p, const double* b, double* x, double
             // residual with the FP64 matrix replaces the recursive one at every reliable update.
             CG<float, double> cg(s, n, a->nnz_full, a->outer.p, a->cols.p, a->val32.p, p);
             cg.set_cols(a);
+            cg.indefinite = a->neumann;
             if (mode == NLF_MODE_SPLIT) cg.lo = a->val_lo.p;
             out.iterations = cg.solve_reliable(a->outer.p, a->cols.p, a->val.p, b, x, tol, max_it, 0.1,
                                                &out.true_residual, &out.outer_iterations, &breakdown);

@@ -8,7 +8,8 @@
 //   * nlf_matrix_combine: K := alpha K + beta C on K's pattern, the bound part scaled by alpha, the
 //     Dirichlet rows kept as identity rows -- the "inner() *= tau; bound() *= tau; inner() += C;
 //     diagonal = 1" of compute() (:76-83).  C's pattern must lie within K's (a local matrix in a
-//     nonlocal one); an entry of C outside K's pattern is an argument error.
+//     nonlocal one); an entry of C outside K's pattern is an argument error, found by a pass with
+//     no writes before K is touched.
 //
 // The time step itself (right-hand side tau f + C T_n, first-kind conditions, CG from T_n) is
 // composed by the host from these, nlf_spmv, nlf_apply_first_kind and nlf_cg_solve.
@@ -45,7 +46,9 @@ __global__ void k_scale(i64 n, double alpha, double* __restrict__ v) {
     if (i < n) v[i] *= alpha;
 }
 
-// one wave per row: every entry of C's row r is added into K's row r (binary search of its column)
+// one wave per row: every entry of C's row r is found in K's row r by a binary search of its column;
+// WRITE = false only flags an entry that K's row lacks, WRITE = true adds beta C_rc into K
+template<bool WRITE>
 __global__ __launch_bounds__(256) void k_add_rows(i64 rows, const i64* __restrict__ k_outer,
                                                   const i32* __restrict__ k_cols, double* __restrict__ k_val,
                                                   const i64* __restrict__ c_outer, const i32* __restrict__ c_cols,
@@ -62,8 +65,11 @@ __global__ __launch_bounds__(256) void k_add_rows(i64 rows, const i64* __restric
             const int mid = (lo + hi) >> 1;
             if (k_cols[kb + mid] < c) lo = mid + 1; else hi = mid;
         }
-        if (lo < kn && k_cols[kb + lo] == c) k_val[kb + lo] += beta * c_val[p];
-        else *bad = 1;
+        if (lo < kn && k_cols[kb + lo] == c) {
+            if (WRITE) k_val[kb + lo] += beta * c_val[p];
+        } else {
+            *bad = 1;
+        }
     }
 }
 
@@ -101,6 +107,15 @@ extern "C" int nlf_matrix_combine(nlf_matrix K, double alpha, nlf_matrix C, doub
         NLF_CHECK(K != C, NLF_ERR_ARGUMENT, "a matrix cannot be combined with itself");
         hipStream_t s = K->ctx->stream;
         const i64 rows = K->rows;
+        // containment check first, with no writes, so that K is left untouched when it fails
+        DevBuf<int> bad{1};
+        NLF_HIP(hipMemsetAsync(bad.p, 0, sizeof(int), s));
+        hipLaunchKernelGGL(k_add_rows<false>, dim3(blocks_for(rows, 4)), dim3(256), 0, s, rows, K->outer.p, K->cols.p,
+                           K->val.p, C->outer.p, C->cols.p, C->val.p, beta, bad.p);
+        NLF_HIP(hipGetLastError());
+        int h_bad = 0;
+        download(&h_bad, bad.p, 1, s);
+        NLF_CHECK(!h_bad, NLF_ERR_ARGUMENT, "the second matrix has entries outside the first one's pattern");
         if (K->nnz_full) {
             hipLaunchKernelGGL(k_scale, dim3(blocks_for(K->nnz_full, 256)), dim3(256), 0, s, K->nnz_full, alpha, K->val.p);
             NLF_HIP(hipGetLastError());
@@ -110,16 +125,12 @@ extern "C" int nlf_matrix_combine(nlf_matrix K, double alpha, nlf_matrix C, doub
                                K->b_val.p);
             NLF_HIP(hipGetLastError());
         }
-        DevBuf<int> bad{1};
-        NLF_HIP(hipMemsetAsync(bad.p, 0, sizeof(int), s));
-        hipLaunchKernelGGL(k_add_rows, dim3(blocks_for(rows, 4)), dim3(256), 0, s, rows, K->outer.p, K->cols.p, K->val.p,
-                           C->outer.p, C->cols.p, C->val.p, beta, bad.p);
+        hipLaunchKernelGGL(k_add_rows<true>, dim3(blocks_for(rows, 4)), dim3(256), 0, s, rows, K->outer.p, K->cols.p,
+                           K->val.p, C->outer.p, C->cols.p, C->val.p, beta, bad.p);
         hipLaunchKernelGGL(k_identity_rows, dim3(blocks_for(rows, 256)), dim3(256), 0, s, rows, K->inner.p, K->diag.p,
                            K->val.p);
         NLF_HIP(hipGetLastError());
-        int h_bad = 0;
-        download(&h_bad, bad.p, 1, s);
-        NLF_CHECK(!h_bad, NLF_ERR_ARGUMENT, "the second matrix has entries outside the first one's pattern");
+        NLF_HIP(hipStreamSynchronize(s));
         // derived values no longer match: drop them; the FP32 value pass cannot repeat a combination
         K->val32.release();
         K->val_lo.release();

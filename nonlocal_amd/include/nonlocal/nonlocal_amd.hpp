@@ -285,9 +285,28 @@ public:
 
 namespace mechanical {
 
+// isotropic_elastic_parameters / orthotropic_elastic_parameters / anisotropic_elastic_parameters
+// (elastic_parameters.hpp:62-133) as a plain material record; expansion_kind NLF_EXPANSION_NONE
+inline nlf_elastic_material isotropic_elastic_parameters(double young_modulus, double poissons_ratio) {
+    return {NLF_ELASTIC_ISOTROPIC, NLF_EXPANSION_NONE, young_modulus, poissons_ratio, 0.0, 0.0, 0.0, 0.0, {0.0, 0.0, 0.0}};
+}
+inline nlf_elastic_material orthotropic_elastic_parameters(std::array<double, 2> young_modulus,
+                                                           std::array<double, 2> poissons_ratio, double shear_modulus) {
+    return {NLF_ELASTIC_ORTHOTROPIC, NLF_EXPANSION_NONE, young_modulus[0], poissons_ratio[0], young_modulus[1],
+            poissons_ratio[1], shear_modulus, 0.0, {0.0, 0.0, 0.0}};
+}
+inline nlf_elastic_material anisotropic_elastic_parameters(std::array<double, 2> young_modulus,
+                                                           std::array<double, 2> poissons_ratio, double shear_modulus,
+                                                           double angle) {
+    nlf_elastic_material m = orthotropic_elastic_parameters(young_modulus, poissons_ratio, shear_modulus);
+    m.kind = NLF_ELASTIC_ANISOTROPIC;
+    m.angle = angle;
+    return m;
+}
+
 struct parameter_2d {
     model_parameters model;
-    nlf_elastic_material material{0, 0, 1.0, 0.3};
+    nlf_elastic_material material{NLF_ELASTIC_ISOTROPIC, NLF_EXPANSION_NONE, 1.0, 0.3, 0.0, 0.0, 0.0, 0.0, {0.0, 0.0, 0.0}};
 };
 
 using parameters_2d = std::unordered_map<std::string, parameter_2d>;
@@ -740,8 +759,8 @@ public:
         if (is_strain_and_stress_calculated()) return;
         const size_t n = mesh_->nodes_count();
         device_vector du(solution_), dstrain(3 * n), dstress(3 * n);
-        nlf_check(nlf_strain_stress(mesh_->handle(), models_.data(), materials_.data(), du.data(), dstrain.data(),
-                                    dstress.data(), nullptr),
+        nlf_check(nlf_strain_stress(mesh_->handle(), models_.data(), materials_.data(), du.data(), nullptr,
+                                    dstrain.data(), dstress.data(), nullptr),
                   mesh_->ctx().handle());
         const std::vector<double> hs = dstrain.host(), ht = dstress.host();
         for (size_t k = 0; k < 3; ++k) {

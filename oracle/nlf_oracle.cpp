@@ -461,9 +461,23 @@ struct Separator {                                   // matrix_separator_base::p
 struct Group {
     bool nonlocal = false;
     double p1 = 1.0;
+    // coefficients per quadrature point (evaluated_parameters' Variable alternative,
+    // equation_parameters.hpp; evaluate_conductivity.hpp:16-37): cq[((e - begin) * nq + q) * ncomp + k]
+    // in the layout of the constant (conductivity c[], or the Hooke matrix of the kind), or null
+    const double* cq = nullptr;
+    i64 begin = 0;
+    int ncomp = 0;
+    const double* coeff(i64 e, int q, int nq, const double* constant) const {
+        return cq ? cq + ((e - begin) * nq + q) * ncomp : constant;
+    }
     Influence inf;
     nlf_thermal_material th{};
-    double hooke[3] = {0, 0, 0};   // {h11, h12, h66} isotropic plane stress
+    // Hooke matrix in the reference's index order of its kind (elastic_parameters.hpp:9-11):
+    // isotropic {11, 12, 66}, orthotropic {11, 12, 66, 22}, anisotropic {11, 12, 66, 22, 16, 26}
+    int hkind = NLF_ELASTIC_ISOTROPIC;
+    double hooke[6] = {0, 0, 0, 0, 0, 0};
+    int ekind = NLF_EXPANSION_NONE;  // thermal expansion (thermal_expansion_parameters.hpp:7-18)
+    double alpha[3] = {0, 0, 0};
     double capacity = 0.0;         // density * capacity (heat capacity matrix)
 };
 
@@ -579,14 +593,16 @@ struct IndexRunner {                                 // matrix_index_initializer
 
 // Local and nonlocal integrands.
 // conductivity_matrix_2d.hpp:106-164, stiffness_matrix_2d.hpp:44-123.
+// conductivity_matrix_2d::integrate_local (conductivity_matrix_2d.hpp:106-130), the conductivity at q
 void thermal_local(const Mesh& m, const Group& g, i64 e, int i, int j, double* out) {
     double integral = 0.0;
     for (int q = 0; q < m.t.nq; ++q) {
         const double ix = m.Dx(e, i, q), iy = m.Dy(e, i, q), jx = m.Dx(e, j, q), jy = m.Dy(e, j, q);
+        const double* c = g.coeff(e, q, m.t.nq, g.th.c);
         double value;
-        if (g.th.kind == NLF_CONDUCTIVITY_ISOTROPIC) value = g.th.c[0] * (ix * jx + iy * jy);
-        else if (g.th.kind == NLF_CONDUCTIVITY_ORTHOTROPIC) value = g.th.c[0] * ix * jx + g.th.c[1] * iy * jy;
-        else value = ix * (g.th.c[0] * jx + g.th.c[2] * jy) + iy * (g.th.c[2] * jx + g.th.c[1] * jy);
+        if (g.th.kind == NLF_CONDUCTIVITY_ISOTROPIC) value = c[0] * (ix * jx + iy * jy);
+        else if (g.th.kind == NLF_CONDUCTIVITY_ORTHOTROPIC) value = c[0] * ix * jx + c[1] * iy * jy;
+        else value = ix * (c[0] * jx + c[2] * jy) + iy * (c[2] * jx + c[1] * jy);
         integral += m.t.w[q] * value / m.jac[e * m.t.nq + q];
     }
     out[0] = g.p1 * integral;
@@ -626,25 +642,36 @@ void capacity_local(const Mesh& m, const Group& g, i64 e, int i, int j, double* 
     out[0] = g.capacity * integral;
 }
 
+// stiffness_matrix::integrate_local (stiffness_matrix_2d.hpp:44-80), the branch of the group's Hooke kind
 void elastic_local(const Mesh& m, const Group& g, i64 e, int i, int j, double* out) {
     double b[4] = {0, 0, 0, 0};
-    const double h11 = g.hooke[0], h12 = g.hooke[1], h66 = g.hooke[2];
     for (int q = 0; q < m.t.nq; ++q) {
+        const double* h = g.coeff(e, q, m.t.nq, g.hooke);
         const double f = m.t.w[q] / m.jac[e * m.t.nq + q];
         const double wx = f * m.Dx(e, i, q), wy = f * m.Dy(e, i, q);
         const double jx = m.Dx(e, j, q), jy = m.Dy(e, j, q);
-        const double s1 = h66 * wy, s2 = h66 * wx;
-        b[0] += h11 * wx * jx + s1 * jy;
-        b[1] += h12 * wx * jy + s1 * jx;
-        b[2] += h12 * wy * jx + s2 * jy;
-        b[3] += h11 * wy * jy + s2 * jx;
+        if (g.hkind != NLF_ELASTIC_ANISOTROPIC) {               // indices 11 12 66 (22)
+            const double s1 = h[2] * wy, s2 = h[2] * wx;
+            b[0] += h[0] * wx * jx + s1 * jy;
+            b[1] += h[1] * wx * jy + s1 * jx;
+            b[2] += h[1] * wy * jx + s2 * jy;
+            b[3] += (g.hkind == NLF_ELASTIC_ISOTROPIC ? h[0] * wy * jy : h[3] * wy * jy) + s2 * jx;
+        } else {                                                 // 11 12 66 22 16 26
+            const double h16 = h[4] * wx, h26 = h[5] * wy;
+            const double s1 = h[2] * wy + h16, s2 = h[2] * wx + h26;
+            b[0] += (h[0] * wx + h[4] * wy) * jx + s1 * jy;
+            b[1] += (h[1] * wx + h26) * jy + s1 * jx;
+            b[2] += (h[1] * wy + h16) * jx + s2 * jy;
+            b[3] += (h[3] * wy + h[5] * wx) * jy + s2 * jx;
+        }
     }
     for (int k = 0; k < 4; ++k) out[k] = g.p1 * b[k];
 }
 
+// stiffness_matrix::integrate_nonlocal (stiffness_matrix_2d.hpp:82-123); Hooke at the nonlocal point
 void elastic_nonlocal(const Mesh& m, const Group& g, i64 eL, i64 eNL, int iL, int jNL, double* out) {
     double b[4] = {0, 0, 0, 0};
-    const double h11 = g.hooke[0], h12 = g.hooke[1], h66 = g.hooke[2];
+    const double* h = g.hooke;
     const int nq = m.t.nq;
     for (int qL = 0; qL < nq; ++qL) {
         const double* xl = m.xqp(eL, qL);
@@ -652,13 +679,59 @@ void elastic_nonlocal(const Mesh& m, const Group& g, i64 eL, i64 eNL, int iL, in
         for (int qN = 0; qN < nq; ++qN) {
             const double wgt = m.t.w[qN] * influence_eval(g.inf, xl, m.xqp(eNL, qN));
             const double jx = wgt * m.Dx(eNL, jNL, qN), jy = wgt * m.Dy(eNL, jNL, qN);
-            b[0] += h11 * ix * jx + h66 * iy * jy;
-            b[1] += h12 * ix * jy + h66 * iy * jx;
-            b[2] += h12 * iy * jx + h66 * ix * jy;
-            b[3] += h11 * iy * jy + h66 * ix * jx;
+            if (g.hkind != NLF_ELASTIC_ANISOTROPIC) {
+                const double h22 = g.hkind == NLF_ELASTIC_ISOTROPIC ? h[0] : h[3];
+                b[0] += h[0] * ix * jx + h[2] * iy * jy;
+                b[1] += h[1] * ix * jy + h[2] * iy * jx;
+                b[2] += h[1] * iy * jx + h[2] * ix * jy;
+                b[3] += h22 * iy * jy + h[2] * ix * jx;
+            } else {
+                b[0] += (h[0] * ix + h[4] * iy) * jx + (h[4] * ix + h[2] * iy) * jy;
+                b[1] += (h[4] * ix + h[2] * iy) * jx + (h[1] * ix + h[5] * iy) * jy;
+                b[2] += (h[4] * ix + h[1] * iy) * jx + (h[2] * ix + h[5] * iy) * jy;
+                b[3] += (h[2] * ix + h[5] * iy) * jx + (h[5] * ix + h[3] * iy) * jy;
+            }
         }
     }
     for (int k = 0; k < 4; ++k) out[k] = (1.0 - g.p1) * b[k];
+}
+
+// Hooke matrix of a material (elastic_parameters.hpp:62-133), in the index order of its kind
+void hooke_matrix(const nlf_elastic_material& e, int& kind, double h[6]) {
+    kind = e.kind;
+    if (e.kind == NLF_ELASTIC_ISOTROPIC) {                       // :70-75
+        const double E = e.young, nu = e.poisson;
+        const double value = E / (1.0 - nu * nu);
+        h[0] = value; h[1] = nu * value; h[2] = 0.5 * value * (1.0 - nu);
+        return;
+    }
+    if (e.kind != NLF_ELASTIC_ORTHOTROPIC && e.kind != NLF_ELASTIC_ANISOTROPIC)
+        throw std::domain_error("unknown elastic material kind");
+    // orthotropic :89-95: E = {young, young_y}, nu = {poisson, poisson_y}
+    const double value = 1.0 / (1.0 - e.poisson * e.poisson_y);
+    const double m11 = e.young * value, m12 = e.young * e.poisson_y * value, m66 = e.shear, m22 = e.young_y * value;
+    if (e.kind == NLF_ELASTIC_ORTHOTROPIC) {
+        h[0] = m11; h[1] = m12; h[2] = m66; h[3] = m22;
+        return;
+    }
+    // anisotropic_elastic_parameters::rotate :102-123
+    const double angle_2 = e.angle + e.angle;
+    const double sin_2 = std::sin(angle_2);
+    const double cos_2 = std::cos(angle_2);
+    const double cos_4 = 0.125 * (cos_2 * cos_2 - sin_2 * sin_2);
+    const double stiff_diff = m11 - m22;
+    const double stiff_core = m11 + m22 - 2 * m12 - 4 * m66;
+    const double main_part_plus = stiff_core * (cos_4 + 0.125);
+    const double main_part_minus = stiff_core * (0.125 - cos_4);
+    const double cos_diff = stiff_diff * (cos_2 + 0.5);
+    const double minor_part = stiff_core * cos_2;
+    const double sin_2_025 = 0.25 * sin_2;
+    h[0] = main_part_plus + 0.5 * (m12 + m22 + cos_diff) + m66;
+    h[1] = main_part_minus + m12;
+    h[2] = main_part_minus + m66;
+    h[3] = main_part_plus + 0.5 * (m12 + m11 - cos_diff) + m66;
+    h[4] = sin_2_025 * (stiff_diff + minor_part);
+    h[5] = sin_2_025 * (stiff_diff - minor_part);
 }
 
 struct ValueRunner {                                 // integrator.hpp:36-91
@@ -719,9 +792,9 @@ Group make_group(const nlf_model& md, int physics, const nlf_thermal_material* t
     if (physics == 2) return g;
     if (physics == 0) g.th = *th;
     else {
-        const double E = el->young, nu = el->poisson;
-        const double value = E / (1.0 - nu * nu);
-        g.hooke[0] = value; g.hooke[1] = nu * value; g.hooke[2] = 0.5 * value * (1.0 - nu);
+        hooke_matrix(*el, g.hkind, g.hooke);
+        g.ekind = el->expansion_kind;
+        for (int k = 0; k < 3; ++k) g.alpha[k] = el->expansion[k];
     }
     return g;
 }
@@ -1128,17 +1201,133 @@ void heat_flux(const Mesh& m, const std::vector<Group>& groups, const double* T,
     }
 }
 
-// calc_stress for isotropic Hooke {h11, h12, h66} (mechanical_parameters_2d.hpp:27-35)
-void calc_stress(const double* h, double e11, double e22, double e12, double* out) {
-    out[0] = h[0] * e11 + h[1] * e22;
-    out[1] = h[1] * e11 + h[0] * e22;
-    out[2] = 2 * h[2] * e12;
+// calc_stress (mechanical_parameters_2d.hpp:27-48) for the group's Hooke kind; tensor shear strain
+void calc_stress(const Group& g, double e11, double e22, double e12, double* out) {
+    const double* h = g.hooke;
+    if (g.hkind == NLF_ELASTIC_ISOTROPIC) {
+        out[0] = h[0] * e11 + h[1] * e22;
+        out[1] = h[1] * e11 + h[0] * e22;
+        out[2] = 2 * h[2] * e12;
+    } else if (g.hkind == NLF_ELASTIC_ORTHOTROPIC) {
+        out[0] = h[0] * e11 + h[1] * e22;
+        out[1] = h[1] * e11 + h[3] * e22;
+        out[2] = 2 * h[2] * e12;
+    } else {
+        out[0] = h[0] * e11 + h[1] * e22 + 2 * h[4] * e12;
+        out[1] = h[1] * e11 + h[3] * e22 + 2 * h[5] * e12;
+        out[2] = h[4] * e11 + h[5] * e22 + 2 * h[2] * e12;
+    }
 }
 
-// mechanical_solution_2d::calc_strain_and_stress (mechanical_solution_2d.hpp:192-232) without thermal
-// strains.  u: displacement (x, y per node).  qstress [3][E*nq]; nstrain, nstress [3][N] (xx, yy, xy).
-void strain_stress(const Mesh& m, const std::vector<Group>& groups, const double* u, double* qstress, double* nstrain,
-                   double* nstress) {
+// nodes_to_qnodes (mesh_2d_utils.hpp:65-80): sum_i x_i N_i(q)
+std::vector<double> nodes_to_qnodes(const Mesh& m, const double* x) {
+    const int nq = m.t.nq;
+    std::vector<double> v(size_t(m.E * nq), 0.0);
+#pragma omp parallel for schedule(static)
+    for (i64 e = 0; e < m.E; ++e)
+        for (int q = 0; q < nq; ++q)
+            for (int i = 0; i < m.nn; ++i) v[e * nq + q] += x[m.el[e * m.nn + i]] * m.t.qN[i * nq + q];
+    return v;
+}
+
+// the thermal strain of a group at a point of temperature change dt, {xx, yy, xy} (substract_temperature_strains,
+// mechanical_solution_2d.hpp:135-166)
+void thermal_strain(const Group& g, double dt, double* out) {
+    if (g.ekind == NLF_EXPANSION_ISOTROPIC) { out[0] = g.alpha[0] * dt; out[1] = out[0]; out[2] = 0.0; }
+    else if (g.ekind == NLF_EXPANSION_ORTHOTROPIC) { out[0] = g.alpha[0] * dt; out[1] = g.alpha[1] * dt; out[2] = 0.0; }
+    else { out[0] = g.alpha[0] * dt; out[1] = g.alpha[1] * dt; out[2] = g.alpha[2] * dt; }
+}
+
+// _temperature_condition::calc_stress (temperature_condition_2d.hpp:20-74): the stress of the thermal
+// strain for each pair of Hooke and expansion kinds, written as the reference writes each branch
+void thermal_stress(const Group& g, double dt, double* out) {
+    const double* h = g.hooke;
+    const double* a = g.alpha;
+    if (g.ekind == NLF_EXPANSION_ISOTROPIC) {
+        const double ts = dt * a[0];
+        if (g.hkind == NLF_ELASTIC_ISOTROPIC) { const double v = (h[0] + h[1]) * ts; out[0] = v; out[1] = v; out[2] = 0.0; }
+        else if (g.hkind == NLF_ELASTIC_ORTHOTROPIC) { out[0] = (h[0] + h[1]) * ts; out[1] = (h[1] + h[3]) * ts; out[2] = 0.0; }
+        else { out[0] = (h[0] + h[1]) * ts; out[1] = (h[1] + h[3]) * ts; out[2] = (h[4] + h[5]) * ts; }
+        return;
+    }
+    const double sxx = dt * a[0], syy = dt * a[1], sxy = g.ekind == NLF_EXPANSION_ANISOTROPIC ? dt * a[2] : 0.0;
+    const double h22 = g.hkind == NLF_ELASTIC_ISOTROPIC ? h[0] : h[3];
+    if (g.hkind != NLF_ELASTIC_ANISOTROPIC) {
+        out[0] = h[0] * sxx + h[1] * syy;
+        out[1] = h[1] * sxx + h22 * syy;
+        out[2] = g.ekind == NLF_EXPANSION_ANISOTROPIC ? 2 * h[2] * sxy : 0.0;
+    } else if (g.ekind == NLF_EXPANSION_ORTHOTROPIC) {
+        out[0] = h[0] * sxx + h[1] * syy;
+        out[1] = h[1] * sxx + h[3] * syy;
+        out[2] = h[4] * sxx + h[5] * syy;
+    } else {
+        out[0] = h[0] * sxx + 2 * h[4] * sxy + h[1] * syy;
+        out[1] = h[1] * sxx + 2 * h[5] * sxy + h[3] * syy;
+        out[2] = h[4] * sxx + 2 * h[2] * sxy + h[5] * syy;
+    }
+}
+
+// temperature_condition (temperature_condition_2d.hpp:128-170): f[2 node + d] += the thermal load.  Per
+// node, the node's elements in ascending order; per element, for a nonlocal group first the sum over
+// its neighbours of (1 - p1) times the nonlocal integral (:106-124), then p1 times the local one
+// (:80-92).  The nonlocal integral adds wdN_x (s_xx + s_xy) and wdN_y (s_yy + s_xy) of the averaged
+// thermal stress s, as the reference writes it (:120-121).
+void temperature_condition(const Mesh& m, const std::vector<Group>& groups, const double* dT, double* f) {
+    const int nq = m.t.nq;
+    const std::vector<double> tq = nodes_to_qnodes(m, dT);
+#pragma omp parallel for schedule(dynamic, 64)
+    for (i64 node = 0; node < m.N; ++node) {
+        double integral[2] = {0.0, 0.0};
+        for (i64 k = m.ne_off[node]; k < m.ne_off[node + 1]; ++k) {
+            const i64 eL = m.ne_ids[k];
+            const Group& g = groups[m.group_of[eL]];
+            if (g.ekind == NLF_EXPANSION_NONE) continue;
+            int iL = 0;
+            for (int i = 0; i < m.nn; ++i) if (m.el[eL * m.nn + i] == node) iL = i;
+            if (g.nonlocal) {
+                const double w1 = 1.0 - g.p1;
+                for (i64 s = m.nb_off[eL]; s < m.nb_off[eL + 1]; ++s) {
+                    const i64 eNL = m.nb_ids[s];
+                    double in[2] = {0.0, 0.0};
+                    for (int qL = 0; qL < nq; ++qL) {
+                        double acc[3] = {0.0, 0.0, 0.0};
+                        const double* xl = m.xqp(eL, qL);
+                        for (int qN = 0; qN < nq; ++qN) {
+                            const i64 qs = eNL * nq + qN;
+                            const double wgt = m.t.w[qN] * influence_eval(g.inf, xl, m.xqp(eNL, qN)) * m.jac[qs];
+                            double st[3];
+                            thermal_stress(g, tq[qs], st);
+                            for (int c = 0; c < 3; ++c) acc[c] += wgt * st[c];
+                        }
+                        const double wx = m.t.w[qL] * m.Dx(eL, iL, qL), wy = m.t.w[qL] * m.Dy(eL, iL, qL);
+                        in[0] += wx * (acc[0] + acc[2]);
+                        in[1] += wy * (acc[1] + acc[2]);
+                    }
+                    integral[0] += w1 * in[0];
+                    integral[1] += w1 * in[1];
+                }
+            }
+            double lo[2] = {0.0, 0.0};
+            for (int q = 0; q < nq; ++q) {
+                double st[3];
+                thermal_stress(g, tq[eL * nq + q], st);
+                const double wx = m.t.w[q] * m.Dx(eL, iL, q), wy = m.t.w[q] * m.Dy(eL, iL, q);
+                lo[0] += wx * st[0] + wy * st[2];
+                lo[1] += wy * st[1] + wx * st[2];
+            }
+            integral[0] += g.p1 * lo[0];
+            integral[1] += g.p1 * lo[1];
+        }
+        f[2 * node] += integral[0];
+        f[2 * node + 1] += integral[1];
+    }
+}
+
+// mechanical_solution_2d::calc_strain_and_stress (mechanical_solution_2d.hpp:192-232).  u: displacement
+// (x, y per node); dT: the temperature change per node or null (no thermal strain).  qstress [3][E*nq];
+// nstrain, nstress [3][N] (xx, yy, xy).
+void strain_stress(const Mesh& m, const std::vector<Group>& groups, const double* u, const double* dT, double* qstress,
+                   double* nstrain, double* nstress) {
     const int nq = m.t.nq;
     const i64 Q = m.E * nq;
     // strains_in_quadratures (:150-157)
@@ -1146,9 +1335,21 @@ void strain_stress(const Mesh& m, const std::vector<Group>& groups, const double
     gradient_in_qnodes(m, u, 2, g11, g12);
     gradient_in_qnodes(m, u + 1, 2, g21, g22);
     for (i64 q = 0; q < Q; ++q) g12[q] = 0.5 * (g12[q] + g21[q]);
-    const double* eps[3] = {g11.data(), g22.data(), g12.data()};
+    double* eps[3] = {g11.data(), g22.data(), g12.data()};
     if (nstrain)
         for (int c = 0; c < 3; ++c) qnodes_to_nodes(m, eps[c], nstrain + c * m.N);
+    if (dT) {                                        // substract_temperature_strains (:135-166)
+        const std::vector<double> tq = nodes_to_qnodes(m, dT);
+        for (i64 e = 0; e < m.E; ++e) {
+            const Group& g = groups[m.group_of[e]];
+            if (g.ekind == NLF_EXPANSION_NONE) continue;
+            for (int q = 0; q < nq; ++q) {
+                double ts[3];
+                thermal_strain(g, tq[e * nq + q], ts);
+                for (int c = 0; c < 3; ++c) eps[c][e * nq + q] -= ts[c];
+            }
+        }
+    }
     std::vector<double> st(3 * Q, 0.0);
     for (int g = 0; g < m.G(); ++g) {
         const Group& gr = groups[g];
@@ -1159,7 +1360,7 @@ void strain_stress(const Mesh& m, const std::vector<Group>& groups, const double
             for (int qL = 0; qL < nq; ++qL) {
                 const i64 qsL = eL * nq + qL;
                 double local[3];
-                calc_stress(gr.hooke, eps[0][qsL], eps[1][qsL], eps[2][qsL], local);
+                calc_stress(gr, eps[0][qsL], eps[1][qsL], eps[2][qsL], local);
                 double stress[3] = {gr.p1 * local[0], gr.p1 * local[1], gr.p1 * local[2]};
                 if (gr.nonlocal) {
                     // calc_nonlocal_stress (:173-189): influence_weight = w |J| (1 - p1) phi
@@ -1171,7 +1372,7 @@ void strain_stress(const Mesh& m, const std::vector<Group>& groups, const double
                             const i64 qs = eNL * nq + qN;
                             const double iw = m.t.w[qN] * m.jac[qs] * (w1 * influence_eval(gr.inf, xl, m.xqp(eNL, qN)));
                             double sn[3];
-                            calc_stress(gr.hooke, eps[0][qs], eps[1][qs], eps[2][qs], sn);
+                            calc_stress(gr, eps[0][qs], eps[1][qs], eps[2][qs], sn);
                             for (int c = 0; c < 3; ++c) ns[c] += iw * sn[c];
                         }
                     }
@@ -1274,6 +1475,80 @@ void* orc_assemble(void* h, int physics, const nlf_model* models, const void* ma
     } catch (const std::exception& e) { g_error = e.what(); return nullptr; }
 }
 
+// The Neumann problem's portrait and integral condition (conductivity_matrix_2d.hpp:39-72, 83-104),
+// symmetric case: one more row and column, N; every row r < N ends with (r, N) = sum over the node's
+// elements (ascending) of integrate_basic = sum_q w_q N_i(q) |J_q|; row N holds (N, N) = 0.
+void neumann_border(const Mesh& m, Assembly& a) {
+    CSR& A = a.A[INNER];
+    const i64 n = A.rows;
+    const int nq = m.t.nq;
+    std::vector<double> c(size_t(n), 0.0);
+    for (i64 node = 0; node < n; ++node)
+        for (i64 k = m.ne_off[node]; k < m.ne_off[node + 1]; ++k) {
+            const i64 e = m.ne_ids[k];
+            const int i = m.local(e, node);
+            double integral = 0.0;
+            for (int q = 0; q < nq; ++q) integral += m.t.w[q] * m.t.qN[i * nq + q] * m.jac[e * nq + q];
+            c[size_t(node)] += integral;
+        }
+    CSR B;
+    B.rows = B.cols = n + 1;
+    B.outer.assign(size_t(n + 2), 0);
+    for (i64 r = 0; r < n; ++r) {
+        for (i64 p = A.outer[r]; p < A.outer[r + 1]; ++p) { B.inner.push_back(A.inner[p]); B.val.push_back(A.val[p]); }
+        B.inner.push_back(n);
+        B.val.push_back(c[size_t(r)]);
+        B.outer[r + 1] = i64(B.inner.size());
+    }
+    B.inner.push_back(n);
+    B.val.push_back(0.0);
+    B.outer[n + 1] = i64(B.inner.size());
+    A = std::move(B);
+    CSR& D = a.A[BOUND];
+    D.rows = D.cols = n + 1;
+    D.outer.push_back(D.outer.back());
+}
+
+// nlf_assemble_ex's options: neumann (thermal only, no first-kind rows), coefficients per quadrature point
+// for each group (coeffs[g] null or E_g * nq * ncomp values in the constant's layout)
+void* orc_assemble_ex(void* h, int physics, const nlf_model* models, const void* materials, const uint8_t* is_inner,
+                      int symmetric, int part, int faithful_reset, int neumann, const double* const* coeffs) {
+    try {
+        const Mesh& m = *static_cast<Mesh*>(h);
+        std::vector<Group> groups;
+        bool any_nonlocal = false, any_varying = false;
+        for (int g = 0; g < m.G(); ++g) {
+            groups.push_back(make_group(models[g], physics,
+                                        physics == 0 ? &static_cast<const nlf_thermal_material*>(materials)[g] : nullptr,
+                                        physics == 1 ? &static_cast<const nlf_elastic_material*>(materials)[g] : nullptr,
+                                        part == NLF_PART_LOCAL || physics == 2));
+            if (physics == 2) groups.back().capacity = static_cast<const double*>(materials)[g];
+            Group& gr = groups.back();
+            any_nonlocal |= gr.nonlocal;
+            if (coeffs && coeffs[g]) {
+                if (physics == 2) throw std::domain_error("the heat capacity takes no coefficients per point");
+                gr.cq = coeffs[g];
+                gr.begin = m.gbeg[g];
+                gr.ncomp = physics == 0 ? (gr.th.kind == NLF_CONDUCTIVITY_ISOTROPIC ? 1 :
+                                           gr.th.kind == NLF_CONDUCTIVITY_ORTHOTROPIC ? 2 : 3)
+                                        : (gr.hkind == NLF_ELASTIC_ISOTROPIC ? 3 : gr.hkind == NLF_ELASTIC_ORTHOTROPIC ? 4 : 6);
+                any_varying = true;
+            }
+        }
+        // problem_settings::is_symmetric (problem_settings.hpp:32-34): a nonlocal problem with coefficients
+        // that vary is asymmetric (BiCGSTAB in the reference)
+        if (any_varying && any_nonlocal) throw std::domain_error("coefficients per point make a nonlocal problem asymmetric");
+        const i64 rows = (physics == 1 ? 2 : 1) * m.N;
+        std::vector<uint8_t> inner(is_inner, is_inner + rows);
+        Assembly* a = assemble(m, physics, groups, inner, symmetric != 0, faithful_reset != 0);
+        if (neumann) {
+            if (physics != 0) { delete a; throw std::domain_error("Neumann problems are thermal"); }
+            neumann_border(m, *a);
+        }
+        return a;
+    } catch (const std::exception& e) { g_error = e.what(); return nullptr; }
+}
+
 void orc_matrix_destroy(void* a) { delete static_cast<Assembly*>(a); }
 
 void orc_matrix_info(void* a, int part, i64* rows, i64* nnz) {
@@ -1287,6 +1562,16 @@ void orc_matrix_export(void* a, int part, i64* outer, i64* inner, double* val) {
     if (outer) std::copy(A.outer.begin(), A.outer.end(), outer);
     if (inner) std::copy(A.inner.begin(), A.inner.end(), inner);
     if (val) std::copy(A.val.begin(), A.val.end(), val);
+}
+
+// rows [row0, row1) of one part: outer[row1 - row0 + 1] from 0, inner / val may be null
+void orc_matrix_export_rows(void* a, int part, i64 row0, i64 row1, i64* outer, i64* inner, double* val) {
+    const CSR& A = static_cast<Assembly*>(a)->A[part];
+    const i64 b = A.outer[row0];
+    for (i64 r = row0; r <= row1; ++r) outer[r - row0] = A.outer[r] - b;
+    const i64 e = A.outer[row1];
+    if (inner) std::copy(A.inner.begin() + b, A.inner.begin() + e, inner);
+    if (val) std::copy(A.val.begin() + b, A.val.begin() + e, val);
 }
 
 void orc_matrix_set_values(void* a, int part, const double* val) {
@@ -1389,12 +1674,36 @@ void orc_right_part(void* h, int dof, ValueFn fn, void* user, double* f) {
 }
 
 int orc_strain_stress(void* h, const nlf_model* models, const nlf_elastic_material* materials, const double* u,
-                      double* qstress, double* nstrain, double* nstress) {
+                      const double* dT, double* qstress, double* nstrain, double* nstress) {
     try {
         const Mesh& m = *static_cast<Mesh*>(h);
         std::vector<Group> groups;
         for (int g = 0; g < m.G(); ++g) groups.push_back(make_group(models[g], 1, nullptr, &materials[g], false));
-        strain_stress(m, groups, u, qstress, nstrain, nstress);
+        strain_stress(m, groups, u, dT, qstress, nstrain, nstress);
+        return 0;
+    } catch (const std::exception& e) { g_error = e.what(); return -1; }
+}
+
+// the Hooke matrix of a material in its kind's index order (elastic_parameters.hpp:62-133)
+int orc_hooke(const nlf_elastic_material* e, int* kind, double* h) {
+    try {
+        hooke_matrix(*e, *kind, h);
+        return 0;
+    } catch (const std::exception& ex) { g_error = ex.what(); return -1; }
+}
+
+// f[2N] += the thermal load of the temperature change dT[N] (temperature_condition)
+int orc_temperature_condition(void* h, const nlf_model* models, const nlf_elastic_material* materials, const double* dT,
+                              double* f) {
+    try {
+        const Mesh& m = *static_cast<Mesh*>(h);
+        std::vector<Group> groups;
+        for (int g = 0; g < m.G(); ++g) {
+            groups.push_back(make_group(models[g], 1, nullptr, &materials[g], false));
+            if (groups.back().nonlocal && m.nb_off.empty())
+                throw std::logic_error("the thermal load of a nonlocal group needs the neighbour lists");
+        }
+        temperature_condition(m, groups, dT, f);
         return 0;
     } catch (const std::exception& e) { g_error = e.what(); return -1; }
 }

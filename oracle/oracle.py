@@ -56,6 +56,7 @@ def lib():
         L.orc_matrix_info.argtypes = [C.c_void_p, C.c_int, _i64p, _i64p]
         L.orc_matrix_export.argtypes = [C.c_void_p, C.c_int, _i64p, _i64p, _f64p]
         L.orc_matrix_set_values.argtypes = [C.c_void_p, C.c_int, _f64p]
+        L.orc_matrix_export_rows.argtypes = [C.c_void_p, C.c_int, _i64, _i64, _i64p, _i64p, _f64p]
         L.orc_boundary.argtypes = [C.c_void_p, C.c_int, C.POINTER(_abi.Boundary), _f64p, _f64p, _u8p]
         L.orc_convection.argtypes = [C.c_void_p, C.c_void_p, C.POINTER(_abi.Boundary), _u8p]
         L.orc_first_kind_rhs.argtypes = [C.c_void_p, _u8p, _f64p, _f64p]
@@ -74,7 +75,12 @@ def lib():
         L.orc_qnodes_to_nodes.argtypes = [C.c_void_p, _f64p, _f64p]
         L.orc_strain_stress.restype = C.c_int
         L.orc_strain_stress.argtypes = [C.c_void_p, C.POINTER(_abi.Model), C.POINTER(_abi.ElasticMaterial), _f64p,
-                                        _f64p, _f64p, _f64p]
+                                        _f64p, _f64p, _f64p, _f64p]
+        L.orc_hooke.restype = C.c_int
+        L.orc_hooke.argtypes = [C.POINTER(_abi.ElasticMaterial), C.POINTER(C.c_int), _f64p]
+        L.orc_temperature_condition.restype = C.c_int
+        L.orc_temperature_condition.argtypes = [C.c_void_p, C.POINTER(_abi.Model), C.POINTER(_abi.ElasticMaterial),
+                                                _f64p, _f64p]
         L.orc_boundary_fn.argtypes = [C.c_void_p, C.c_int, C.POINTER(_abi.Boundary), C.c_void_p, C.c_void_p, _f64p,
                                       _f64p, _u8p]
         L.orc_right_part.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, _f64p]
@@ -148,8 +154,8 @@ class OracleMatrix:
         self.h = handle
         self.rows = rows
 
-    def __del__(self):
-        if getattr(self, "h", None):
+    def __del__(self, lib=lib):             # bound now: module globals go first at exit
+        if getattr(self, "h", None) and _LIB is not None:
             lib().orc_matrix_destroy(self.h)
             self.h = None
 
@@ -162,6 +168,16 @@ class OracleMatrix:
         val = np.zeros(nnz.value)
         lib().orc_matrix_export(self.h, part, _ptr(outer, _i64p), _ptr(inner, _i64p), _ptr(val, _f64p))
         return CSR(outer, inner, val)
+
+    def export_rows(self, row0, row1, part=0) -> CSR:
+        """Rows [row0, row1) of one part, offsets from 0."""
+        outer = np.zeros(row1 - row0 + 1, dtype=np.int64)
+        lib().orc_matrix_export_rows(self.h, part, row0, row1, _ptr(outer, _i64p), None, None)
+        inner = np.zeros(max(int(outer[-1]), 1), dtype=np.int64)
+        val = np.zeros(max(int(outer[-1]), 1))
+        lib().orc_matrix_export_rows(self.h, part, row0, row1, _ptr(outer, _i64p), _ptr(inner, _i64p),
+                                     _ptr(val, _f64p))
+        return CSR(outer, inner[:outer[-1]], val[:outer[-1]])
 
     def set_values(self, val, part=0):
         val = np.ascontiguousarray(val, dtype=np.float64)
@@ -192,7 +208,7 @@ class OracleMesh:
             raise RuntimeError(lib().orc_last_error().decode())
 
     def __del__(self, lib=lib):             # bound now: module globals go first at exit
-        if getattr(self, "h", None):
+        if getattr(self, "h", None) and _LIB is not None:
             lib().orc_mesh_destroy(self.h)
             self.h = None
 
@@ -281,21 +297,34 @@ class OracleMesh:
             raise RuntimeError(lib().orc_last_error().decode())
         return n, q
 
-    def strain_stress(self, models, materials, u):
+    def strain_stress(self, models, materials, u, delta_temperature=None):
         """mechanical_solution_2d::calc_strain_and_stress (mechanical_solution_2d.hpp:192-232).  u: the
-        displacement (x, y per node).  Returns (node strains (3, N), node stresses (3, N),
+        displacement (x, y per node); delta_temperature: node temperature changes whose thermal strains
+        are subtracted before the stresses.  Returns (node strains (3, N), node stresses (3, N),
         quad-point stresses (3, E*nq)), components xx, yy, xy."""
         nq = 4 if self.mesh.nn == 4 else 9
         u = np.ascontiguousarray(u, dtype=np.float64)
+        dT = None if delta_temperature is None else np.ascontiguousarray(delta_temperature, dtype=np.float64)
         q = np.zeros((3, self.mesh.n_elems * nq))
         ns = np.zeros((3, self.mesh.n_nodes))
         nst = np.zeros((3, self.mesh.n_nodes))
         marr = _abi.array_of(_abi.Model, models)
         matarr = _abi.array_of(_abi.ElasticMaterial, materials)
-        if lib().orc_strain_stress(self.h, marr, matarr, _ptr(u, _f64p), _ptr(q, _f64p), _ptr(ns, _f64p),
-                                   _ptr(nst, _f64p)) != 0:
+        if lib().orc_strain_stress(self.h, marr, matarr, _ptr(u, _f64p), None if dT is None else _ptr(dT, _f64p),
+                                   _ptr(q, _f64p), _ptr(ns, _f64p), _ptr(nst, _f64p)) != 0:
             raise RuntimeError(lib().orc_last_error().decode())
         return ns, nst, q
+
+    def temperature_condition(self, models, materials, delta_temperature, f=None):
+        """temperature_condition (temperature_condition_2d.hpp:128-170): f (2N, x and y per node) plus the
+        thermal load of the node temperature changes."""
+        f = np.zeros(2 * self.mesh.n_nodes) if f is None else np.array(f, dtype=np.float64)
+        dT = np.ascontiguousarray(delta_temperature, dtype=np.float64)
+        marr = _abi.array_of(_abi.Model, models)
+        matarr = _abi.array_of(_abi.ElasticMaterial, materials)
+        if lib().orc_temperature_condition(self.h, marr, matarr, _ptr(dT, _f64p), _ptr(f, _f64p)) != 0:
+            raise RuntimeError(lib().orc_last_error().decode())
+        return f
 
     def convection(self, matrix: OracleMatrix, bcs, is_inner):
         arr = _abi.array_of(_abi.Boundary, bcs)
@@ -316,8 +345,8 @@ class OraclePrecond:
         self.h = lib().orc_precond_create(kind, matrix.h if matrix is not None else None)
         self.kind = kind
 
-    def __del__(self):
-        if getattr(self, "h", None):
+    def __del__(self, lib=lib):
+        if getattr(self, "h", None) and _LIB is not None:
             lib().orc_precond_destroy(self.h)
             self.h = None
 
@@ -371,13 +400,18 @@ def thermal_problem(mesh, models, materials, bcs, precond=_abi.NLF_PRECOND_ILLT,
                 iterations=it, residual=res)
 
 
-def elastic_problem(mesh, models, materials, bcs, tol=1e-12, max_it=100000, radii=None, values=None):
+def elastic_problem(mesh, models, materials, bcs, tol=1e-12, max_it=100000, radii=None, values=None,
+                    delta_temperature=None):
+    """equilibrium_equation (equilibrium_equation_2d.hpp:20-62): second-kind conditions, the thermal load
+    of delta_temperature (node values) when given, first-kind conditions, CG with ILLT when nonlocal."""
     om = OracleMesh(mesh)
     if radii is None:
         radii = [[m.radius[0], m.radius[1]] if m.local_weight < 0.999 else [0.0, 0.0] for m in models]
     off, ids, expanded = om.find_neighbours(radii)
     f, xD, inner = om.boundary(bcs, dof=2, values=values)
     K = om.assemble(1, models, materials, inner, symmetric=True)
+    if delta_temperature is not None:
+        f = om.temperature_condition(models, materials, delta_temperature, f)
     f = first_kind_rhs(K, inner, xD, f)
     nonlocal_ = any(m.local_weight < 0.999 for m in models)
     if nonlocal_:
@@ -391,6 +425,16 @@ def elastic_problem(mesh, models, materials, bcs, tol=1e-12, max_it=100000, radi
     u, it, res = cg(K, P, f, tol=tol, max_it=max_it)
     return dict(mesh=om, neighbours=(off, ids, expanded), K=K, KL=KL, f=f, xD=xD, inner=inner, u=u,
                 iterations=it, residual=res)
+
+
+def hooke(material):
+    """(kind, values) of the Hooke matrix of a material, in the reference's index order of the kind:
+    isotropic {11, 12, 66}, orthotropic {11, 12, 66, 22}, anisotropic {11, 12, 66, 22, 16, 26}."""
+    kind = C.c_int()
+    h = np.zeros(6)
+    if lib().orc_hooke(C.byref(material), C.byref(kind), _ptr(h, _f64p)) != 0:
+        raise RuntimeError(lib().orc_last_error().decode())
+    return kind.value, h[:{0: 3, 1: 4, 2: 6}[kind.value]]
 
 
 def influence(model, x, y) -> float:

@@ -34,7 +34,7 @@ int main() {
         auto m = std::make_shared<mesh::mesh_2d<double, int64_t>>(d);
 
         mechanical::parameters_2d parameters;
-        parameters["DEFAULT"] = mechanical::parameter_2d{model_parameters{}, {0, 0, 1.0, 0.3}};
+        parameters["DEFAULT"] = mechanical::parameter_2d{model_parameters{}, mechanical::isotropic_elastic_parameters(1.0, 0.3)};
         using bc = mechanical::boundary_condition;
         mechanical::boundaries_conditions_2d conditions;
         conditions["Vertical"] = {bc::displacement(0.0), bc::pressure(0.0)};

@@ -200,3 +200,34 @@ def integrate_nodes(om, mesh, v):
     w, qN = O.tables(mesh.nn)[:2]
     vals = np.asarray(v)[mesh.elems]                              # (E, nn)
     return float(np.einsum("q,eq,iq,ei->", w, jac, qN, vals))
+
+
+def rotate_local_order(mesh, every=2, shift=1):
+    """A copy of the mesh in which every `every`-th element starts its counter-clockwise local order
+    at another corner (Q4 nodes rolled by `shift`, Q8 nodes by 2 * shift, so corners stay corners).
+    The element matrices are the same integrals, but a node then holds one local index in two
+    elements, which is the mesh shape SU2 files give."""
+    import copy
+    out = copy.deepcopy(mesh)
+    step = shift if mesh.nn == 4 else 2 * shift
+    rows = np.arange(0, mesh.n_elems, every)
+    out.elems[rows] = np.roll(mesh.elems[rows], -step, axis=1)
+    return out
+
+
+def is_injective(mesh):
+    """True when for every local index j no node holds index j in two elements."""
+    for j in range(mesh.nn):
+        col = mesh.elems[:, j]
+        if np.unique(col).size != col.size:
+            return False
+    return True
+
+
+def oracle_cg_solution(problem, om, K, f, inner, tol=1e-15):
+    """The oracle's ILLT-CG run to the reference criterion tol (cheap where a direct solve is not);
+    it agrees with the direct solution to ~5e-12 at r/h = 10."""
+    KL = om.assemble(problem.physics, problem.models, problem.materials, inner, part=A.NLF_PART_LOCAL)
+    P = O.OraclePrecond(A.NLF_PRECOND_ILLT, KL)
+    x, it, res = O.cg(K, P, f, tol=tol, max_it=200000)
+    return x
