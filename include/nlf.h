@@ -287,6 +287,16 @@ int nlf_precond_export_factor(nlf_precond p, double* values, double* scale);
 int nlf_cg_solve(nlf_matrix a, nlf_precond p, const double* b_dev, double* x_dev, double tol, int64_t max_it, int mode,
                  nlf_solve_stats* stats);
 
+/* No reference counterpart.  For callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc): nlf_cg_solve in NLF_MODE_FP64 whose
+ * work vectors, partial sums and solver state live in a fixed device array of the library, sized for
+ * at most NLF_NOALLOC_MAX_ROWS rows (about 48 MiB, held per GPU until the process ends).  It calls no
+ * hipMalloc or hipFree and records no HIP events (stats carry no kernel timings).  A larger matrix is
+ * an argument error.  Every call on one GPU shares that array: calls that overlap on one GPU race. */
+#define NLF_NOALLOC_MAX_ROWS 2097152
+int nlf_cg_solve_noalloc(nlf_matrix a, nlf_precond p, const double* b_dev, double* x_dev, double tol, int64_t max_it,
+                         nlf_solve_stats* stats);
+
 /* ---- learned preconditioner (no reference counterpart: an additional preconditioner_base
  * implementation, used first with ILLT / Jacobi as the fallback).  A message-passing network (3
  * layers, width 32) over the graph of the local matrix; weights are nlf_learned_num_params() floats. */

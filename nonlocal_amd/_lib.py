@@ -75,6 +75,7 @@ SIGNATURES = {
     "nlf_precond_apply": (_int, [_p, _p, _p]),
     "nlf_precond_export_factor": (_int, [_p, _f64p, _f64p]),
     "nlf_cg_solve": (_int, [_p, _p, _p, _p, _dbl, _i64, _int, C.POINTER(_abi.SolveStats)]),
+    "nlf_cg_solve_noalloc": (_int, [_p, _p, _p, _p, _dbl, _i64, C.POINTER(_abi.SolveStats)]),
     "nlf_learned_num_params": (_int, []),
     "nlf_learned_init_weights": (_int, [C.c_uint64, _f32p]),
     "nlf_learned_create": (_int, [_p, _p, C.POINTER(_abi.Model), _pp]),

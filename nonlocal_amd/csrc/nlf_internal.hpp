@@ -326,6 +326,7 @@ inline SpmvCols spmv_cols(const nlf_matrix_s* a) {
     return {a->cols.p, a->off16.p, nullptr};
 }
 void exclusive_scan_i64(i64* data, i64 n, hipStream_t s);   // in place over n+1 entries, data[n] = total
+int noalloc_scratch_init(int device);   // device address of nlf_cg_solve_noalloc's work memory, per device
 // (keys, values) sorted by key, stable (radix sort); n pairs
 void sort_pairs_u64(const uint64_t* keys_in, uint64_t* keys_out, const i32* vals_in, i32* vals_out, i64 n, hipStream_t s);
 void set_error(nlf_ctx ctx, const std::string& msg);

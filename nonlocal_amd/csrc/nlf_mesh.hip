@@ -200,6 +200,9 @@ int nlf_ctx_create(int device, void* stream, nlf_ctx* out) {
     c->device = device;
     hipError_t e = hipSetDevice(device);
     if (e != hipSuccess) { delete c; return NLF_ERR_HIP; }
+    // the fixed work memory of nlf_cg_solve_noalloc; a failed look-up leaves it unset, and only that
+    // entry point then reports NLF_ERR_STATE
+    (void)nlf::noalloc_scratch_init(device);
     if (stream) c->stream = static_cast<hipStream_t>(stream);
     else {
         if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) { delete c; return NLF_ERR_HIP; }

@@ -13,6 +13,8 @@
 // barrier between them.
 #include "nlf_internal.hpp"
 
+#include <mutex>
+
 #include <algorithm>
 #include <array>
 #include <chrono>
@@ -1646,6 +1648,74 @@ int nlf_precond_export_factor(nlf_precond p, double* values, double* scale) {
 // stopping rule sqrt(r'z)/||b|| <= tol.  NLF_MODE_MIXED: FP64 iterative refinement around FP32 PCG
 // (x += A32^-1 (b - A x)) until the FP64 true residual ||b - Ax||/||b|| <= tol.  NLF_MODE_SPLIT: the
 // same with the values read as FP32 + 16-bit remainder.  x holds the initial guess on entry.
+// ---- CG with no allocation -------------------------------------------------------------------
+// The work memory of nlf_cg_solve_noalloc: one file-scope device array (every GPU the library runs
+// on holds its own copy until the process ends).  Host code never passes the array's name to a
+// kernel: its device address, per device, comes from hipGetSymbolAddress at context creation
+// (noalloc_scratch_init) and is kept in g_scratch_ptr.
+constexpr size_t NOALLOC_SCRATCH_BYTES = 3 * (size_t(NLF_NOALLOC_MAX_ROWS) * sizeof(double)) +
+                                         5 * (SPMV_MAX_BLOCKS * sizeof(double)) + 3 * 256;
+static __device__ __align__(256) unsigned char scratch[NOALLOC_SCRATCH_BYTES];
+
+constexpr int NOALLOC_MAX_DEVICES = 64;
+static void* g_scratch_ptr[NOALLOC_MAX_DEVICES] = {};
+static std::mutex g_scratch_mu;
+
+}  // extern "C"
+
+namespace nlf {
+// Called by nlf_ctx_create with `device` current: looks up this device's copy of the scratch array once.
+int noalloc_scratch_init(int device) {
+    if (device < 0 || device >= NOALLOC_MAX_DEVICES) return NLF_ERR_ARGUMENT;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (g_scratch_ptr[device]) return NLF_OK;
+    void* ptr = nullptr;
+    if (hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)) != hipSuccess || !ptr) return NLF_ERR_HIP;
+    g_scratch_ptr[device] = ptr;
+    return NLF_OK;
+}
+}  // namespace nlf
+
+extern "C" {
+
+int nlf_cg_solve_noalloc(nlf_matrix a, nlf_precond p, const double* b, double* x, double tol, int64_t max_it,
+                         nlf_solve_stats* stats) {
+    if (!a) return NLF_ERR_ARGUMENT;
+    return guarded(a->ctx, [&] {
+        NLF_CHECK(b && x, NLF_ERR_ARGUMENT, "null vector");
+        NLF_CHECK(!p || p->kind != NLF_PRECOND_LEARNED, NLF_ERR_ARGUMENT, "the learned preconditioner needs nlf_fcg_solve");
+        NLF_CHECK(a->rows <= NLF_NOALLOC_MAX_ROWS, NLF_ERR_ARGUMENT,
+                  "nlf_cg_solve_noalloc: the matrix has more rows than its fixed work memory holds (NLF_NOALLOC_MAX_ROWS = "
+                  "2097152); use nlf_cg_solve");
+        NLF_CHECK(CGWork::bytes<double>(a->rows) <= NOALLOC_SCRATCH_BYTES, NLF_ERR_ARGUMENT, "work memory too small");
+        const int dev = a->ctx->device;
+        void* work = (dev >= 0 && dev < NOALLOC_MAX_DEVICES) ? g_scratch_ptr[dev] : nullptr;
+        NLF_CHECK(work, NLF_ERR_STATE, "the fixed work memory of this device was not set up (nlf_ctx_create)");
+        hipStream_t s = a->ctx->stream;
+        const auto t0 = std::chrono::steady_clock::now();
+        const i64 n = a->rows;
+        nlf_solve_stats out{};
+        out.preconditioner = p ? p->kind : NLF_PRECOND_IDENTITY;
+        int breakdown = 0;
+        CG<double, double> cg(s, n, a->nnz_full, a->outer.p, a->cols.p, a->val.p, p, work);
+        cg.set_cols(a);
+        cg.indefinite = a->neumann;
+        out.iterations = cg.solve(b, x, tol, max_it, &out.residual, &breakdown);
+        out.spmv_bytes = spmv_stream_bytes(a, sizeof(double), sizeof(double));
+        // FP64 true residual ||b - A x|| / ||b||, with z and the partial arrays of the work memory
+        const unsigned rb = spmv_blocks(64, n);
+        residual64(n, a->outer.p, spmv_cols(a), a->val.p, pick_lpr(a), x, b, cg.z, cg.part1, cg.part2, rb, s);
+        hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(1024), 0, s, cg.part1, int(rb), cg.sums);
+        hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(1024), 0, s, cg.part2, int(rb), cg.sums + 1);
+        double h[2];
+        download(h, cg.sums, 2, s);
+        out.true_residual = h[1] > 0 ? std::sqrt(h[0]) / std::sqrt(h[1]) : 0.0;
+        out.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        if (stats) *stats = out;
+        NLF_CHECK(!breakdown, NLF_ERR_NUMERIC, "CG breakdown (non-positive curvature or non-finite residual)");
+    });
+}
+
 int nlf_cg_solve(nlf_matrix a, nlf_precond p, const double* b, double* x, double tol, int64_t max_it, int mode,
                  nlf_solve_stats* stats) {
     if (!a) return NLF_ERR_ARGUMENT;
