@@ -119,17 +119,21 @@ def cpu_baseline(problem, plain_iterations, check, cg_iterations):
     om = O.OracleMesh(problem.mesh)
     om.find_neighbours(radii)
     t1 = time.perf_counter()
+    log(f"[cpu] neighbours {t1 - t0:.2f} s ({cores} threads)")
     f, xD, inner = om.boundary(problem.bcs, 1)
     K = om.assemble(0, problem.models, problem.materials, inner, faithful_reset=True)
     om.convection(K, problem.bcs, inner)
     f = O.first_kind_rhs(K, inner, xD, f)
     t2 = time.perf_counter()
+    log(f"[cpu] assembly {t2 - t1:.2f} s")
     KL = om.assemble(0, problem.models, problem.materials, inner, part=A.NLF_PART_LOCAL, faithful_reset=True)
     P = O.OraclePrecond(A.NLF_PRECOND_ILLT, KL)
     del KL
     t3 = time.perf_counter()
+    log(f"[cpu] local matrix and IC(0) {t3 - t2:.2f} s")
     _, it, _ = O.cg(K, P, f, tol=1e-300, max_it=cg_iterations)
     t4 = time.perf_counter()
+    log(f"[cpu] {it} ILLT-CG iterations {t4 - t3:.2f} s")
     per_it = (t4 - t3) / max(it, 1)
     value = (t1 - t0) + (t2 - t1) + (t3 - t2) + per_it * plain_iterations
     # the GPU result against the oracle's system

@@ -255,7 +255,10 @@ class finite_element_matrix:
 PHYSICS_CAPACITY = 2      # nlf_assemble physics of the heat capacity matrix (materials: rho c per group)
 
 
-def _assemble(mesh: mesh_2d, physics, models, materials, is_inner, symmetric=True, part=NLF_PART_FULL):
+def _assemble(mesh: mesh_2d, physics, models, materials, is_inner, symmetric=True, part=NLF_PART_FULL,
+              neumann=False, coefficients=None):
+    """nlf_assemble, or nlf_assemble_ex with neumann (the Lagrange row and column of a Neumann
+    problem) and coefficients (per group None or its values at its quadrature points)."""
     marr = _abi.array_of(_abi.Model, models)
     if physics == PHYSICS_CAPACITY:
         matarr = (C.c_double * len(materials))(*[float(v) for v in materials])
@@ -264,9 +267,29 @@ def _assemble(mesh: mesh_2d, physics, models, materials, is_inner, symmetric=Tru
         matarr = _abi.array_of(mat_t, materials)
     inner = np.ascontiguousarray(is_inner, dtype=np.uint8)
     h = C.c_void_p()
-    mesh.ctx.check(load().nlf_assemble(mesh.h, physics, marr, C.cast(matarr, C.c_void_p), _ptr(inner, _u8p),
-                                       int(symmetric), part, C.byref(h)))
+    if neumann or coefficients is not None:
+        keep = [None if c is None else np.ascontiguousarray(c, dtype=np.float64).ravel()
+                for c in (coefficients or [None] * len(models))]
+        arr = (C.POINTER(C.c_double) * len(keep))(*[C.POINTER(C.c_double)() if c is None else _ptr(c, _f64p)
+                                                   for c in keep])
+        opt = _abi.AssembleOptions()
+        opt.neumann = int(bool(neumann))
+        opt.coefficients = C.cast(arr, C.POINTER(C.POINTER(C.c_double)))
+        mesh.ctx.check(load().nlf_assemble_ex(mesh.h, physics, marr, C.cast(matarr, C.c_void_p), _ptr(inner, _u8p),
+                                              int(symmetric), part, C.byref(opt), C.byref(h)))
+    else:
+        mesh.ctx.check(load().nlf_assemble(mesh.h, physics, marr, C.cast(matarr, C.c_void_p), _ptr(inner, _u8p),
+                                           int(symmetric), part, C.byref(h)))
     return finite_element_matrix(h, mesh.ctx, 2 if physics == 1 else 1)
+
+
+def is_neumann(bcs):
+    """init_problem_settings (thermal/init_problem_settings.hpp:43): a stationary thermal problem whose
+    conditions are all fluxes (insulated groups hold none) is a Neumann problem."""
+    return all(b.kind[0] in (_abi.NLF_BC_NONE, _abi.NLF_BC_FLUX) for b in bcs)
+
+
+NEUMANN_PROBLEM_ERROR_THRESHOLD = 1e-10          # nonlocal_constants.hpp:45
 
 
 def combine(K: finite_element_matrix, alpha, Cm: finite_element_matrix, beta):
@@ -604,7 +627,8 @@ def _radii(models):
 
 
 def _solve_driver(mesh: mesh_2d, physics, models, materials, bcs, precond, tol, max_it, mode, weights=None,
-                  assembly32=False, coarse=None, values=None, source=None, delta_temperature=None):
+                  assembly32=False, coarse=None, values=None, source=None, delta_temperature=None, energy=0.0,
+                  coefficients=None):
     import time
     ctx = mesh.ctx
     dof = 1 if physics == 0 else 2
@@ -614,8 +638,11 @@ def _solve_driver(mesh: mesh_2d, physics, models, materials, bcs, precond, tol, 
         find_neighbours(mesh, _radii(models))
     tm["neighbours"] = time.perf_counter() - t0
     f, xD, inner = boundary_data(mesh, dof, bcs, values)
+    neumann = physics == 0 and is_neumann(bcs)
+    if neumann and precond == _abi.NLF_PRECOND_LEARNED:
+        raise NLFError(_abi.NLF_ERR_UNSUPPORTED, "the learned preconditioner does not take the bordered Neumann system")
     t1 = time.perf_counter()
-    K = _assemble(mesh, physics, models, materials, inner)
+    K = _assemble(mesh, physics, models, materials, inner, neumann=neumann, coefficients=coefficients)
     if physics == 0:
         K.add_convection(bcs)
     fd = torch.from_numpy(f).to(ctx.device)
@@ -623,6 +650,13 @@ def _solve_driver(mesh: mesh_2d, physics, models, materials, bcs, precond, tol, 
         right_part(mesh, dof, source, fd)
     if delta_temperature is not None and physics == 1:
         temperature_condition(mesh, models, materials, delta_temperature, fd)
+    if neumann:
+        # init_right_part (stationary_heat_equation_solver_2d.hpp:28-44): solvable only when the
+        # fluxes balance; the Lagrange row's right-hand side is the energy
+        if abs(float(fd.sum())) > NEUMANN_PROBLEM_ERROR_THRESHOLD:
+            raise NLFError(_abi.NLF_ERR_ARGUMENT, "It's unsolvable Neumann problem.")
+        fd = torch.cat([fd, torch.tensor([float(energy)], dtype=torch.float64, device=ctx.device)])
+        xD = np.append(xD, 0.0)
     xDd = torch.from_numpy(xD).to(ctx.device)
     K.apply_first_kind(xDd, fd)
     ctx.synchronize()
@@ -634,7 +668,8 @@ def _solve_driver(mesh: mesh_2d, physics, models, materials, bcs, precond, tol, 
     cg.set_max_interations(max_it)
     learned_pc = None
     if precond == NLF_PRECOND_ILLT:
-        KL = _assemble(mesh, physics, models, materials, inner, part=NLF_PART_LOCAL)
+        KL = _assemble(mesh, physics, models, materials, inner, part=NLF_PART_LOCAL, neumann=neumann,
+                       coefficients=coefficients)
         P = cg.init_preconditioner(NLF_PRECOND_ILLT, KL)
         if not P.computation_info():          # fall back to identity like the reference
             cg.init_preconditioner(NLF_PRECOND_IDENTITY)
@@ -679,7 +714,8 @@ def _solve_driver(mesh: mesh_2d, physics, models, materials, bcs, precond, tol, 
 
 def stationary_heat_equation_solver_2d(mesh: mesh_2d, models, materials, bcs, precond=NLF_PRECOND_ILLT, tol=1e-12,
                                        max_it=100000, mode=NLF_MODE_FP64, weights=None, assembly32=False,
-                                       coarse=None, values=None, source=None) -> Solution:
+                                       coarse=None, values=None, source=None, energy=0.0,
+                                       coefficients=None) -> Solution:
     """K T = f with K = p1 K_L + (1 - p1) K_NL (solved directly; the reference's correction form
     T = T0 - K^-1 (K T0 - f) with T0 = 0 is the same system).  precond NLF_PRECOND_LEARNED runs
     flexible CG with the learned preconditioner (weights file `weights`, default the packaged one)
@@ -688,14 +724,17 @@ def stationary_heat_equation_solver_2d(mesh: mesh_2d, models, materials, bcs, pr
     the learned preconditioner's aggregation coarse correction (about that many aggregates; 0 = one
     level; default 4096).  values: optional callable (x, y, group, dof) -> boundary value at a point
     for the first-kind and flux groups.  source: optional heat source s(x, y) (arrays in, array out),
-    integrated as int s N_i."""
+    integrated as int s N_i.  A problem whose conditions are all fluxes is a Neumann problem: it is
+    solved with the Lagrange row (sum of T weighted by int N_i = energy) and its solution vector has
+    N + 1 entries (the temperature, then the multiplier).  coefficients: per element group None or its
+    conductivity at its quadrature points (all groups local)."""
     return _solve_driver(mesh, 0, models, materials, bcs, precond, tol, max_it, mode, weights, assembly32, coarse,
-                         values, source)
+                         values, source, energy=energy, coefficients=coefficients)
 
 
 def equilibrium_equation(mesh: mesh_2d, models, materials, bcs, precond=None, tol=1e-12, max_it=100000,
                          mode=NLF_MODE_FP64, assembly32=False, values=None, source=None,
-                         delta_temperature=None) -> Solution:
+                         delta_temperature=None, coefficients=None) -> Solution:
     """Plane-stress static elasticity (equilibrium_equation, mechanical/equilibrium_equation_2d.hpp:20-62);
     ILLT of the local stiffness when nonlocal, identity otherwise.  materials: nlf_elastic_material per
     group (isotropic, orthotropic or anisotropic Hooke, optional thermal expansion).  values: optional
@@ -706,7 +745,8 @@ def equilibrium_equation(mesh: mesh_2d, models, materials, bcs, precond=None, to
     if precond is None:
         precond = NLF_PRECOND_ILLT if any(m.local_weight < 0.999 for m in models) else NLF_PRECOND_IDENTITY
     return _solve_driver(mesh, 1, models, materials, bcs, precond, tol, max_it, mode, assembly32=assembly32,
-                         values=values, source=source, delta_temperature=delta_temperature)
+                         values=values, source=source, delta_temperature=delta_temperature,
+                         coefficients=coefficients)
 
 
 class nonstationary_heat_equation_solver_2d:

@@ -52,6 +52,9 @@ def lib():
         L.orc_assemble.restype = C.c_void_p
         L.orc_assemble.argtypes = [C.c_void_p, C.c_int, C.POINTER(_abi.Model), C.c_void_p, _u8p, C.c_int, C.c_int,
                                    C.c_int]
+        L.orc_assemble_ex.restype = C.c_void_p
+        L.orc_assemble_ex.argtypes = [C.c_void_p, C.c_int, C.POINTER(_abi.Model), C.c_void_p, _u8p, C.c_int, C.c_int,
+                                      C.c_int, C.c_int, C.POINTER(C.POINTER(C.c_double))]
         L.orc_matrix_destroy.argtypes = [C.c_void_p]
         L.orc_matrix_info.argtypes = [C.c_void_p, C.c_int, _i64p, _i64p]
         L.orc_matrix_export.argtypes = [C.c_void_p, C.c_int, _i64p, _i64p, _f64p]
@@ -253,16 +256,27 @@ class OracleMesh:
         return f, xD, inner
 
     def assemble(self, physics, models, materials, is_inner, symmetric=True, part=_abi.NLF_PART_FULL,
-                 faithful_reset=False) -> OracleMatrix:
+                 faithful_reset=False, neumann=False, coefficients=None) -> OracleMatrix:
+        """neumann: the Lagrange row and column of a Neumann problem (conductivity_matrix_2d::compute
+        with is_neumann); coefficients: per group None or the group's values at its quadrature points,
+        ((e - group_begin) * nq + q) * ncomp + k."""
         marr = _abi.array_of(_abi.Model, models)
         mat_t = _abi.ThermalMaterial if physics == 0 else _abi.ElasticMaterial
         matarr = _abi.array_of(mat_t, materials)
         inner = np.ascontiguousarray(is_inner, dtype=np.uint8)
-        h = lib().orc_assemble(self.h, physics, marr, C.cast(matarr, C.c_void_p), _ptr(inner, _u8p),
-                               int(symmetric), part, int(faithful_reset))
+        if neumann or coefficients is not None:
+            keep = [None if c is None else np.ascontiguousarray(c, dtype=np.float64).ravel()
+                    for c in (coefficients or [None] * len(models))]
+            arr = (C.POINTER(C.c_double) * len(keep))(*[C.POINTER(C.c_double)() if c is None else _ptr(c, _f64p)
+                                                       for c in keep])
+            h = lib().orc_assemble_ex(self.h, physics, marr, C.cast(matarr, C.c_void_p), _ptr(inner, _u8p),
+                                      int(symmetric), part, int(faithful_reset), int(neumann), arr)
+        else:
+            h = lib().orc_assemble(self.h, physics, marr, C.cast(matarr, C.c_void_p), _ptr(inner, _u8p),
+                                   int(symmetric), part, int(faithful_reset))
         if not h:
             raise RuntimeError(lib().orc_last_error().decode())
-        return OracleMatrix(h, inner.size)
+        return OracleMatrix(h, inner.size + int(bool(neumann)))
 
     def right_part(self, dof, source):
         """int s N_i (integrate_right_part, base/right_part_2d.hpp:10-35); source(x, y) -> value (dof 2:
@@ -373,31 +387,51 @@ def cg(matrix: OracleMatrix, precond: OraclePrecond, b, x0=None, tol=1e-15, max_
 # ---------------------------------------------------------------------------------------------
 # Problem drivers (stationary_heat_equation_solver_2d.hpp:46-120, equilibrium_equation_2d.hpp:20-62)
 # ---------------------------------------------------------------------------------------------
+def is_neumann(bcs):
+    """init_problem_settings (thermal/init_problem_settings.hpp:43): a stationary problem whose conditions
+    are all fluxes (insulated groups hold none) is a Neumann problem."""
+    return all(b.kind[0] in (_abi.NLF_BC_NONE, _abi.NLF_BC_FLUX) for b in bcs)
+
+
+NEUMANN_PROBLEM_ERROR_THRESHOLD = 1e-10          # nonlocal_constants.hpp:45
+
+
 def thermal_problem(mesh, models, materials, bcs, precond=_abi.NLF_PRECOND_ILLT, tol=1e-12, max_it=100000,
-                    radii=None, faithful_reset=False, values=None, source=None):
-    """Solve the stationary thermal problem; returns a dict with neighbours, matrices, f and T."""
+                    radii=None, faithful_reset=False, values=None, source=None, energy=0.0, coefficients=None):
+    """Solve the stationary thermal problem; returns a dict with neighbours, matrices, f and T.  A
+    Neumann problem (every condition a flux) is solved with the Lagrange row (f[N] = energy,
+    stationary_heat_equation_solver_2d.hpp:28-44); T is then the first N entries."""
     om = OracleMesh(mesh)
     if radii is None:
         radii = [[m.search_radius[0] or m.radius[0], m.search_radius[1] or m.radius[1]]
                  if m.local_weight < 0.999 else [0.0, 0.0] for m in models]
     off, ids, expanded = om.find_neighbours(radii)
     f, xD, inner = om.boundary(bcs, dof=1, values=values)
-    K = om.assemble(0, models, materials, inner, symmetric=True, faithful_reset=faithful_reset)
+    neumann = is_neumann(bcs)
+    K = om.assemble(0, models, materials, inner, symmetric=True, faithful_reset=faithful_reset, neumann=neumann,
+                    coefficients=coefficients)
     om.convection(K, bcs, inner)
     if source is not None:
         f = f + om.right_part(1, source)
+    if neumann:
+        if abs(np.sum(f)) > NEUMANN_PROBLEM_ERROR_THRESHOLD:
+            raise ValueError("It's unsolvable Neumann problem.")
+        f = np.append(f, energy)
+        inner = np.append(inner, 1).astype(np.uint8)
+        xD = np.append(xD, 0.0)
     f = first_kind_rhs(K, inner, xD, f)
     if precond == _abi.NLF_PRECOND_ILLT:
-        KL = om.assemble(0, models, materials, inner, symmetric=True, part=_abi.NLF_PART_LOCAL)
+        KL = om.assemble(0, models, materials, inner[:mesh.n_nodes], symmetric=True, part=_abi.NLF_PART_LOCAL,
+                         neumann=neumann, coefficients=coefficients)
         P = OraclePrecond(precond, KL)
         if not P.ok()[0]:
             P = OraclePrecond(_abi.NLF_PRECOND_IDENTITY, None)
     else:
         KL = None
         P = OraclePrecond(precond, K if precond == _abi.NLF_PRECOND_JACOBI else None)
-    T, it, res = cg(K, P, f, tol=tol, max_it=max_it)
-    return dict(mesh=om, neighbours=(off, ids, expanded), K=K, KL=KL, f=f, xD=xD, inner=inner, T=T,
-                iterations=it, residual=res)
+    x, it, res = cg(K, P, f, tol=tol, max_it=max_it)
+    return dict(mesh=om, neighbours=(off, ids, expanded), K=K, KL=KL, f=f, xD=xD, inner=inner, T=x[:mesh.n_nodes],
+                x=x, iterations=it, residual=res, neumann=neumann)
 
 
 def elastic_problem(mesh, models, materials, bcs, tol=1e-12, max_it=100000, radii=None, values=None,
