@@ -279,6 +279,11 @@ int nlf_spmv_lpr(nlf_matrix a, int lpr, int fp32, const void* x_dev, void* y_dev
 int nlf_precond_create(nlf_matrix a, int kind, nlf_precond* out);
 int nlf_precond_destroy(nlf_precond p);
 int nlf_precond_info(nlf_precond p, int* ok, double* shift, int* fw_levels, int* bw_levels, double* setup_seconds);
+/* No reference counterpart: the layout of the ILLT sweeps, for diagnostics.  Per sweep (forward,
+ * then backward): out[0/4] levels as swept (wide levels split), out[1/5] compute width,
+ * out[2/6] levels held in the LDS dependency window (0: none), out[3/7] kernel (0 packed,
+ * 1 ELL one level at a time, 2 ELL with prefetch). */
+int nlf_precond_sweep_info(nlf_precond p, int32_t out[8]);
 int nlf_precond_apply(nlf_precond p, const double* r_dev, double* z_dev);
 int nlf_precond_export_factor(nlf_precond p, double* values, double* scale);
 

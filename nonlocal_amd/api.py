@@ -381,6 +381,15 @@ class preconditioner:
                                                C.byref(sec)))
         return dict(ok=bool(ok.value), shift=shift.value, fw_levels=fw.value, bw_levels=bw.value, seconds=sec.value)
 
+    def sweep_info(self):
+        """ILLT sweep layout (nlf_precond_sweep_info): per sweep the levels as swept, the compute
+        width, the levels of the LDS dependency window and the kernel (0 packed, 1 ELL one level at a
+        time, 2 ELL with prefetch)."""
+        out = (C.c_int32 * 8)()
+        self.ctx.check(load().nlf_precond_sweep_info(self.h, out))
+        keys = ("levels", "width", "window_levels", "kernel")
+        return {"forward": dict(zip(keys, out[0:4])), "backward": dict(zip(keys, out[4:8]))}
+
     def computation_info(self):
         return self.info()["ok"]
 

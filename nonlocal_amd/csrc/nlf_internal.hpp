@@ -16,6 +16,7 @@ namespace nlf {
 
 using i64 = int64_t;
 using i32 = int32_t;
+using u32 = uint32_t;
 
 struct Error : std::runtime_error {
     int code;

@@ -549,15 +549,16 @@ __global__ __launch_bounds__(LEVEL_THREADS) void k_ic_solve(PackedSweep<X> fw, P
 }
 
 // ELL form of one sweep for the fast path: every level is a block of `cnt` records with exactly WE
-// entry slots each, stored slot-major inside the level (entry j of record t of level L at
-// WE * lptr[L] + j * cnt + t), so all loads of a level are coalesced and independent of each other;
-// padding slots carry value 0 and a dependency on slot 0 of the (zero-initialised) window.
+// entry slots each, stored record-major (entry j of record rec at WE * rec + j), so that a lane
+// loads a record's values and dependency slots with 16-byte loads and a wave's loads of one level
+// cover one contiguous span; padding slots carry value 0 and a dependency on slot 0 of the
+// (zero-initialised) window.
 template<class X>
 struct EllSweep {
     int levels, K, W;
     const i32* lptr;
     const i32* row;
-    const i32* dep;
+    const i32* dep;          // window slot (K > 0) or -(row + 1) of a dependency read from the output
     const X* val;
     const X* dinv;
 };
@@ -575,87 +576,199 @@ __global__ void k_sweep_gather(i64 n, const i32* __restrict__ row, const X* __re
     }
 }
 
-// One level-scheduled triangular sweep by one workgroup (blockDim >= widest level).  Each thread owns
-// record t of every level; the record data of level L + D (right-hand side, 1/U_kk, the WE entries
-// and their dependency slots, the row, the backward scale) are loaded D levels ahead into a ring of
-// D register slots, and the level offsets are staged in LDS, so a level's critical path is the LDS
-// reads of its dependencies, a few FMAs and the barrier: one exposed global-load latency per level
-// (about 1.7 us) becomes one per D levels.  The arithmetic of a record is unchanged.
-// out[row] = v; out2[row] = scale2[rec] * v when given.
+// One level-scheduled triangular sweep by one workgroup.  The first `ct` threads (ct = blockDim - 64
+// SW, at least the widest level) compute: thread t owns record t of every level, and the record data
+// of level L + D (right-hand side, 1/U_kk, the WE entries and their dependency slots) are loaded D
+// levels ahead into a ring of D register slots, while the level offsets and every dependency live in
+// LDS; a level's critical path is the LDS reads of its dependencies, a few FMAs, the LDS write and
+// the barrier.  The last SW waves store: after the barrier of level L they copy the level's values
+// from the LDS window to out_lvl (level order), while level L + 1 is computed.
+//
+// Why the split: a wave's vector-memory counter covers loads and stores alike, and loads and stores
+// may complete out of order relative to each other, so while a wave has stores in flight any wait
+// for one of its loads must wait for everything (vmcnt 0), which drains the prefetch ring.  Compute
+// waves that never store wait for exactly the level they need.  For the same reason no load of the
+// level loop goes through a generic (flat) address.  The arithmetic of a record is unchanged; the
+// row-order results come from k_sweep_scatter.
 constexpr size_t SWEEP_LDS_BYTES = 64 * 1024;
+// Compute widths of the two prefetching sweeps and their store waves (workgroups of CT + 64 SW).
+#ifndef NLF_SWEEP_SW_SMALL
+#define NLF_SWEEP_SW_SMALL 2
+#endif
+#ifndef NLF_SWEEP_SW_LARGE
+#define NLF_SWEEP_SW_LARGE 2
+#endif
+constexpr int SWEEP_SW_SMALL = NLF_SWEEP_SW_SMALL, SWEEP_SW_LARGE = NLF_SWEEP_SW_LARGE;
+constexpr int SWEEP_CT_SMALL = 512, SWEEP_CT_LARGE = 1024 - 64 * SWEEP_SW_LARGE;
 
-// Levels the ring holds, sized so that the ring and the rest of the kernel fit the registers a
-// lane has at MAXT threads per workgroup (256 VGPRs at 512, 128 at 1024) without spilling.
-template<int WE, int MAXT>
+// Levels the ring holds, sized so that the ring and the rest of a compute lane fit the registers a
+// lane has with CT compute lanes plus the store waves per workgroup (168 VGPRs up to 12 waves, 128
+// at 16) without spilling.
+template<int WE, int CT>
 constexpr int sweep_depth() {
-    return MAXT <= 512 ? (WE <= 4 ? 8 : WE <= 8 ? 4 : WE <= 16 ? 2 : 1) : (WE <= 4 ? 4 : WE <= 8 ? 2 : 1);
+    return CT <= SWEEP_CT_SMALL ? (WE <= 4 ? 8 : WE <= 8 ? 4 : WE <= 16 ? 2 : 1) : (WE <= 4 ? 4 : WE <= 8 ? 2 : 1);
 }
 
-template<class X, int WE, int MAXT>
-__global__ __launch_bounds__(MAXT) void k_ic_sweep(EllSweep<X> s, const X* __restrict__ rhs, X* __restrict__ out,
-                                                   X* __restrict__ out2, const X* __restrict__ scale2,
-                                                   const CGState* __restrict__ st, int lptr_in_lds) {
-    constexpr int D = sweep_depth<WE, MAXT>();
+// N consecutive elements of type T by 16-byte loads (8-byte when N * sizeof(T) is 8); p is aligned
+// to the load size
+template<class T, int N>
+__device__ __forceinline__ void load16(const T* __restrict__ p, T (&v)[N]) {
+    constexpr int B = int(N * sizeof(T));
+    static_assert(B == 8 || B % 16 == 0, "record slots are whole 8- or 16-byte words");
+    if constexpr (B == 8) {
+        const uint2 w = *reinterpret_cast<const uint2*>(p);
+        __builtin_memcpy(v, &w, 8);
+    } else {
+        const uint4* q = reinterpret_cast<const uint4*>(p);
+#pragma unroll
+        for (int k = 0; k < B / 16; ++k) {
+            const uint4 w = q[k];
+            __builtin_memcpy(reinterpret_cast<unsigned char*>(v) + 16 * k, &w, 16);
+        }
+    }
+}
+
+template<class X, int WE, int CT, int SW>
+__global__ __launch_bounds__(CT + 64 * SW) void k_ic_sweep(EllSweep<X> s, const X* __restrict__ rhs,
+                                                           X* __restrict__ out_lvl, const CGState* __restrict__ st) {
+    constexpr int D = sweep_depth<WE, CT>();
     extern __shared__ unsigned char smem_raw[];
     X* win = reinterpret_cast<X*>(smem_raw);
     i32* lp = reinterpret_cast<i32*>(smem_raw + ((size_t(s.K) * size_t(s.W) * sizeof(X) + 15) & ~size_t(15)));
     if (st && st->done) return;
     const int t = threadIdx.x;
-    if (s.K)
-        for (int i = t; i < s.K * s.W; i += blockDim.x) win[i] = X(0);
-    if (lptr_in_lds)
-        for (int i = t; i <= s.levels; i += blockDim.x) lp[i] = s.lptr[i];
+    const int ct = int(blockDim.x) - 64 * SW;
+    for (int i = t; i < s.K * s.W; i += blockDim.x) win[i] = X(0);
+    for (int i = t; i <= s.levels; i += blockDim.x) lp[i] = s.lptr[i];
     __syncthreads();
-    auto level_begin = [&](int L) { return lptr_in_lds ? lp[L] : s.lptr[L]; };
-    X r_rhs[D], r_dinv[D], r_val[D][WE], r_sc[D];
-    i32 r_dep[D][WE], r_row[D];
-    // the loads of a level are issued by every thread (past the level's width a thread reads the
-    // level's last record again), so that each thread issues the same loads in the same order
-    auto load = [&](int L, X& rv, X& dv, X (&vv)[WE], i32 (&dd)[WE], i32& rw, X& sc) {
-        const i32 b = level_begin(L), cnt = level_begin(L + 1) - b;
-        const i32 tt = t < cnt ? t : cnt - 1;
-        const i32 rec = b + tt;
+    // the compute waves run whole trips of D levels (the phantom levels past the last one recompute
+    // the last level's records into window slots no record reads): an exit inside the unrolled trip
+    // would make the compiler wait for every prefetched load at the top of each trip
+    const int padded = (s.levels + D - 1) / D * D;
+    if (t >= ct) {
+        // store waves: one barrier per level, like the compute waves, then that level's values
+        constexpr int PER = (CT + 64 * SW - 1) / (64 * SW);
+        const int lane = t - ct;
+        for (int L = 0; L < padded; ++L) {
+            asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
+            if (L >= s.levels) continue;
+            const i32 b = lp[L], cnt = lp[L + 1] - b;
+            const X* src = win + (L % s.K) * s.W;
+            X v[PER];
+#pragma unroll
+            for (int k = 0; k < PER; ++k) {
+                const int i = lane + k * 64 * SW;
+                v[k] = i < cnt ? src[i] : X(0);
+            }
+#pragma unroll
+            for (int k = 0; k < PER; ++k) {
+                const int i = lane + k * 64 * SW;
+                if (i < cnt) out_lvl[b + i] = v[k];
+            }
+        }
+        return;
+    }
+    // The ring.  The loads of a level are issued by every compute thread (past the level's width a
+    // thread reads the level's last record again), so that each thread issues the same loads in the
+    // same order.  (Masking the idle lanes' loads and 16-bit dependency slots were both measured
+    // slower: the loads cost per instruction, and unpacking the slots lengthens the level.)
+    X r_rhs[D], r_dinv[D], r_val[D][WE];
+    i32 r_dep[D][WE];
+    auto load_at = [&](i32 b, i32 cnt, X& rv, X& dv, X (&vv)[WE], i32 (&dd)[WE]) {
+        const i32 rec = b + (t < cnt ? t : cnt - 1);
         rv = rhs[rec];
         dv = s.dinv[rec];
-        rw = s.row[rec];
-        if (scale2) sc = scale2[rec];
-        const i64 e0 = i64(WE) * b + tt;
-#pragma unroll
-        for (int j = 0; j < WE; ++j) {
-            vv[j] = s.val[e0 + i64(j) * cnt];
-            dd[j] = s.dep[e0 + i64(j) * cnt];
-        }
+        load16(s.val + i64(WE) * rec, vv);
+        load16(s.dep + i64(WE) * rec, dd);
     };
+    auto load = [&](int L, X& rv, X& dv, X (&vv)[WE], i32 (&dd)[WE]) {
+        load_at(lp[L], lp[L + 1] - lp[L], rv, dv, vv, dd);
+    };
+    // the prologue issues the ring in slot order (the scheduling barrier keeps the compiler from
+    // grouping the loads by kind): the wait for slot u in the loop then counts only later slots
 #pragma unroll
-    for (int u = 0; u < D; ++u)
-        if (u < s.levels) load(u, r_rhs[u], r_dinv[u], r_val[u], r_dep[u], r_row[u], r_sc[u]);
-    for (int L0 = 0; L0 < s.levels; L0 += D) {
+    for (int u = 0; u < D; ++u) {
+        load(min(u, s.levels - 1), r_rhs[u], r_dinv[u], r_val[u], r_dep[u]);
+        __builtin_amdgcn_sched_barrier(0);
+    }
+    // at least one trip (levels >= 1): a do-while has no zero-trip test, so no block for the
+    // optimiser to sink the prologue's loads into, out of slot order
+    int L0 = 0;
+    do {
 #pragma unroll
         for (int u = 0; u < D; ++u) {
             const int L = L0 + u;
-            if (L >= s.levels) break;
-            const i32 b = level_begin(L), cnt = level_begin(L + 1) - b;
-            if (t < cnt) {
-                X v = r_rhs[u];
+            // offsets of the level this slot is refilled with, read beside the dependencies
+            const int Ln = min(L + D, s.levels - 1);
+            const i32 nb = lp[Ln], ncnt = lp[Ln + 1] - nb;
+            // branch-free: a lane past the level's width holds the level's last record and computes
+            // that record's value again into its own window slot t >= width, which no record reads
+            // (the window stride is the compute width); the store waves copy only the first cnt
+            X v = r_rhs[u];
 #pragma unroll
-                for (int j = 0; j < WE; ++j) {
-                    const i32 d = r_dep[u][j];
-                    v -= r_val[u][j] * (d >= 0 ? win[d] : out[-d - 1]);
-                }
-                v *= r_dinv[u];
-                if (s.K) win[(L % s.K) * s.W + t] = v;
-                out[r_row[u]] = v;
-                if (out2) out2[r_row[u]] = r_sc[u] * v;
-            }
-            if (s.K) {
-                // every dependency lives in the LDS window: wait for LDS traffic only, so the global
-                // loads prefetched for the next levels stay in flight across the barrier
-                asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
-            } else {
-                __syncthreads();
-            }
-            if (L + D < s.levels) load(L + D, r_rhs[u], r_dinv[u], r_val[u], r_dep[u], r_row[u], r_sc[u]);
+            for (int j = 0; j < WE; ++j) v -= r_val[u][j] * win[r_dep[u][j]];
+            v *= r_dinv[u];
+            win[(L % s.K) * s.W + t] = v;
+            // the slot is free once its level is computed: refill it before the barrier, so the
+            // address arithmetic and the load issue overlap the wait for the other waves
+            load_at(nb, ncnt, r_rhs[u], r_dinv[u], r_val[u], r_dep[u]);
+            // every dependency lives in the LDS window: wait for LDS traffic only, so the global
+            // loads prefetched for the next levels stay in flight across the barrier.  The
+            // scheduling barriers keep the compiler from hoisting a later level's address
+            // arithmetic above it, which would wait for that level's loads here.
+            __builtin_amdgcn_sched_barrier(0);
+            asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
+            __builtin_amdgcn_sched_barrier(0);
         }
+        L0 += D;
+    } while (L0 < padded);
+}
+
+// Level order to row order: out[row[rec]] = v_lvl[rec]; out2[row[rec]] = scale2[rec] * v_lvl[rec].
+template<class X>
+__global__ void k_sweep_scatter(i64 n, const i32* __restrict__ row, const X* __restrict__ lvl,
+                                const X* __restrict__ scale2, X* __restrict__ out, X* __restrict__ out2,
+                                const CGState* __restrict__ st) {
+    if (st && st->done) return;
+    for (i64 t = i64(blockIdx.x) * blockDim.x + threadIdx.x; t < n; t += i64(gridDim.x) * blockDim.x) {
+        const i32 k = row[t];
+        const X v = lvl[t];
+        out[k] = v;
+        if (out2) out2[k] = scale2[t] * v;
+    }
+}
+
+// The same sweep when the dependency window or the level offsets do not fit LDS: dependencies
+// outside the window are read back from `out` (written by earlier levels of this workgroup), one
+// level at a time with no prefetch.
+template<class X, int WE>
+__global__ __launch_bounds__(LEVEL_THREADS) void k_ic_sweep_generic(EllSweep<X> s, const X* __restrict__ rhs,
+                                                                    X* __restrict__ out, X* __restrict__ out2,
+                                                                    const X* __restrict__ scale2,
+                                                                    const CGState* __restrict__ st) {
+    extern __shared__ unsigned char smem_raw[];
+    X* win = reinterpret_cast<X*>(smem_raw);
+    if (st && st->done) return;
+    const int t = threadIdx.x;
+    for (int i = t; i < s.K * s.W; i += blockDim.x) win[i] = X(0);
+    __syncthreads();
+    for (int L = 0; L < s.levels; ++L) {
+        const i32 b = s.lptr[L], cnt = s.lptr[L + 1] - b;
+        if (t < cnt) {
+            const i32 rec = b + t;
+            X v = rhs[rec];
+#pragma unroll
+            for (int j = 0; j < WE; ++j) {
+                const i32 d = s.dep[i64(WE) * rec + j];
+                v -= s.val[i64(WE) * rec + j] * (s.K ? win[d] : out[-d - 1]);
+            }
+            v *= s.dinv[rec];
+            if (s.K) win[(L % s.K) * s.W + t] = v;
+            const i32 k = s.row[rec];
+            out[k] = v;
+            if (out2) out2[k] = scale2[rec] * v;
+        }
+        __syncthreads();
     }
 }
 
@@ -745,14 +858,15 @@ struct nlf_precond_s {
     int solve_threads = 64;
     struct Ell {
         int levels = 0, K = 0, W = 0, WE = 0, threads = 64;
+        nlf::i64 rows = 0;
         nlf::DevBuf<nlf::i32> lptr, row, dep;
         nlf::DevBuf<double> val, dinv, scale;
         nlf::DevBuf<float> val32, dinv32, scale32;
         template<class X> nlf::EllSweep<X> get() const;
     } efw, ebw;
     bool ell = false;
-    nlf::DevBuf<double> rhs_lvl, y1, y2;
-    nlf::DevBuf<float> rhs_lvl32, y1_32, y2_32;
+    nlf::DevBuf<double> rhs_lvl, y_lvl, y1, y2;
+    nlf::DevBuf<float> rhs_lvl32, y_lvl32, y1_32, y2_32;
     // learned (set by nlf_learned.hip)
     void* learned = nullptr;
     void (*learned_destroy)(void*) = nullptr;
@@ -876,9 +990,28 @@ static void build_sweep(const std::vector<i64>& outer, const std::vector<i32>& c
 // ELL sweep for the fast kernel; false when a level is wider than one workgroup or rows have too
 // many entries (the generic packed sweep is used then).
 static bool build_ell(const std::vector<i64>& outer, const std::vector<i32>& cols, const std::vector<i64>& diag,
-                      const std::vector<i32>& lrows, const std::vector<i32>& lptr, bool forward, const double* F,
+                      const std::vector<i32>& lrows, const std::vector<i32>& level_ptr, bool forward, const double* F,
                       const double* scale, const i64* d_diag, nlf_precond_s::Ell& el, hipStream_t s) {
     const i64 n = i64(diag.size());
+    // Levels wider than the sweep's compute width are split into consecutive parts: the records of a
+    // level are independent, so any split keeps the dependency order.  The widest level is often
+    // level 0, the rows with no dependency at all (first-kind rows among them), several times wider
+    // than the rest.  The width is the one of the two prefetching sweeps that needs fewer levels,
+    // preferring the narrower (deeper prefetch) unless it needs over 25% more.
+    std::vector<i32> lptr(1, 0);
+    {
+        i64 c_small = 0, c_large = 0;
+        for (size_t L = 0; L + 1 < level_ptr.size(); ++L) {
+            const i64 w = level_ptr[L + 1] - level_ptr[L];
+            c_small += (w + SWEEP_CT_SMALL - 1) / SWEEP_CT_SMALL;
+            c_large += (w + SWEEP_CT_LARGE - 1) / SWEEP_CT_LARGE;
+        }
+        const i32 cap = 4 * c_small <= 5 * c_large ? SWEEP_CT_SMALL : SWEEP_CT_LARGE;
+        for (size_t L = 0; L + 1 < level_ptr.size(); ++L) {
+            for (i32 x = level_ptr[L] + cap; x < level_ptr[L + 1]; x += cap) lptr.push_back(x);
+            lptr.push_back(level_ptr[L + 1]);
+        }
+    }
     const int levels = int(lptr.size()) - 1;
     std::vector<i32> level_of(n), pos_of(n);
     int W = 1, WEmax = 0;
@@ -897,6 +1030,10 @@ static bool build_ell(const std::vector<i64>& outer, const std::vector<i32>& col
     int WE = 4;
     while (WE < WEmax) WE *= 2;
     if (W > LEVEL_THREADS || WE > 32) return false;
+    // one thread per record of the widest level; the window's slot blocks are one workgroup wide,
+    // so that the sweep's lanes past a level's width have slots of their own to write
+    const int threads = std::min(LEVEL_THREADS, std::max(64, (W + 63) / 64 * 64));
+    W = threads;
     const bool window = size_t(K) * size_t(W) * sizeof(double) <= IC_WINDOW_BYTES;
     std::vector<i32> dep(size_t(WE) * n);
     std::vector<i64> src(size_t(WE) * n, -1);
@@ -906,7 +1043,7 @@ static bool build_ell(const std::vector<i64>& outer, const std::vector<i32>& col
             const i32 k = lrows[b + t];
             const i64 pb = forward ? outer[k] : diag[k] + 1, pe = forward ? diag[k] : outer[k + 1];
             for (int j = 0; j < WE; ++j) {
-                const size_t e = size_t(WE) * b + size_t(j) * cnt + t;
+                const size_t e = size_t(WE) * size_t(b + t) + size_t(j);
                 if (pb + j < pe) {
                     const i32 i = cols[pb + j];
                     dep[e] = window ? (level_of[i] % K) * W + pos_of[i] : -(i + 1);
@@ -921,7 +1058,8 @@ static bool build_ell(const std::vector<i64>& outer, const std::vector<i32>& col
     el.K = window ? K : 0;
     el.W = W;
     el.WE = WE;
-    el.threads = std::min(LEVEL_THREADS, std::max(64, (W + 63) / 64 * 64));
+    el.threads = threads;
+    el.rows = n;
     upload(el.lptr, lptr.data(), lptr.size(), s);
     upload(el.row, lrows.data(), lrows.size(), s);
     upload(el.dep, dep.data(), dep.size(), s);
@@ -943,41 +1081,70 @@ const X* ell_scale(const nlf_precond_s::Ell& e);
 template<> const double* ell_scale<double>(const nlf_precond_s::Ell& e) { return e.scale.p; }
 template<> const float* ell_scale<float>(const nlf_precond_s::Ell& e) { return e.scale32.p; }
 
+// LDS bytes of the dependency window, and whether the prefetching kernel takes the sweep (it needs
+// the window and the level offsets in LDS and at most SWEEP_CT_LARGE compute lanes).
+static size_t ell_window_bytes(const nlf_precond_s::Ell& e, size_t xsize) {
+    return (size_t(std::max(e.K * e.W, 1)) * xsize + 15) & ~size_t(15);
+}
+static bool ell_prefetching(const nlf_precond_s::Ell& e, size_t xsize) {
+    return e.K != 0 && ell_window_bytes(e, xsize) + size_t(e.levels + 1) * sizeof(i32) <= SWEEP_LDS_BYTES &&
+           e.threads <= SWEEP_CT_LARGE;
+}
+
 template<class X>
-void ell_sweep(const nlf_precond_s::Ell& e, const X* rhs, X* out, X* out2, const X* scale2, const CGState* st,
+void ell_sweep(const nlf_precond_s::Ell& e, const X* rhs, X* lvl, X* out, X* out2, const X* scale2, const CGState* st,
                hipStream_t s) {
-    // LDS: the dependency window, then the level offsets when both fit SWEEP_LDS_BYTES
-    const size_t win = (size_t(std::max(e.K * e.W, 1)) * sizeof(X) + 15) & ~size_t(15);
+    // LDS: the dependency window, then the level offsets; the prefetching kernel needs both there
+    const size_t win = ell_window_bytes(e, sizeof(X));
     const size_t offs = size_t(e.levels + 1) * sizeof(i32);
-    const int in_lds = win + offs <= SWEEP_LDS_BYTES ? 1 : 0;
-    const size_t lds = win + (in_lds ? offs : 0);
     const EllSweep<X> a = e.template get<X>();
-    const dim3 g(1), b(unsigned(e.threads));
-    if (e.threads <= 512) {
+    const dim3 g(1);
+    if (e.levels == 0) return;
+    if (!ell_prefetching(e, sizeof(X))) {
+        const size_t lds = e.K ? win : 16;
+        const dim3 b(unsigned(e.threads));
         switch (e.WE) {
-            case 4: hipLaunchKernelGGL((k_ic_sweep<X, 4, 512>), g, b, lds, s, a, rhs, out, out2, scale2, st, in_lds); break;
-            case 8: hipLaunchKernelGGL((k_ic_sweep<X, 8, 512>), g, b, lds, s, a, rhs, out, out2, scale2, st, in_lds); break;
-            case 16: hipLaunchKernelGGL((k_ic_sweep<X, 16, 512>), g, b, lds, s, a, rhs, out, out2, scale2, st, in_lds); break;
-            default: hipLaunchKernelGGL((k_ic_sweep<X, 32, 512>), g, b, lds, s, a, rhs, out, out2, scale2, st, in_lds); break;
+            case 4: hipLaunchKernelGGL((k_ic_sweep_generic<X, 4>), g, b, lds, s, a, rhs, out, out2, scale2, st); break;
+            case 8: hipLaunchKernelGGL((k_ic_sweep_generic<X, 8>), g, b, lds, s, a, rhs, out, out2, scale2, st); break;
+            case 16: hipLaunchKernelGGL((k_ic_sweep_generic<X, 16>), g, b, lds, s, a, rhs, out, out2, scale2, st); break;
+            default: hipLaunchKernelGGL((k_ic_sweep_generic<X, 32>), g, b, lds, s, a, rhs, out, out2, scale2, st); break;
+        }
+        return;
+    }
+    const size_t lds = win + offs;
+    if (e.threads <= SWEEP_CT_SMALL) {
+        constexpr int CT = SWEEP_CT_SMALL, SW = SWEEP_SW_SMALL;
+        const dim3 b(unsigned(e.threads + 64 * SW));
+        switch (e.WE) {
+            case 4: hipLaunchKernelGGL((k_ic_sweep<X, 4, CT, SW>), g, b, lds, s, a, rhs, lvl, st); break;
+            case 8: hipLaunchKernelGGL((k_ic_sweep<X, 8, CT, SW>), g, b, lds, s, a, rhs, lvl, st); break;
+            case 16: hipLaunchKernelGGL((k_ic_sweep<X, 16, CT, SW>), g, b, lds, s, a, rhs, lvl, st); break;
+            default: hipLaunchKernelGGL((k_ic_sweep<X, 32, CT, SW>), g, b, lds, s, a, rhs, lvl, st); break;
         }
     } else {
+        constexpr int CT = SWEEP_CT_LARGE, SW = SWEEP_SW_LARGE;
+        const dim3 b(unsigned(e.threads + 64 * SW));
         switch (e.WE) {
-            case 4: hipLaunchKernelGGL((k_ic_sweep<X, 4, 1024>), g, b, lds, s, a, rhs, out, out2, scale2, st, in_lds); break;
-            case 8: hipLaunchKernelGGL((k_ic_sweep<X, 8, 1024>), g, b, lds, s, a, rhs, out, out2, scale2, st, in_lds); break;
-            case 16: hipLaunchKernelGGL((k_ic_sweep<X, 16, 1024>), g, b, lds, s, a, rhs, out, out2, scale2, st, in_lds); break;
-            default: hipLaunchKernelGGL((k_ic_sweep<X, 32, 1024>), g, b, lds, s, a, rhs, out, out2, scale2, st, in_lds); break;
+            case 4: hipLaunchKernelGGL((k_ic_sweep<X, 4, CT, SW>), g, b, lds, s, a, rhs, lvl, st); break;
+            case 8: hipLaunchKernelGGL((k_ic_sweep<X, 8, CT, SW>), g, b, lds, s, a, rhs, lvl, st); break;
+            case 16: hipLaunchKernelGGL((k_ic_sweep<X, 16, CT, SW>), g, b, lds, s, a, rhs, lvl, st); break;
+            default: hipLaunchKernelGGL((k_ic_sweep<X, 32, CT, SW>), g, b, lds, s, a, rhs, lvl, st); break;
         }
     }
+    const i64 n = i64(e.rows);
+    hipLaunchKernelGGL(k_sweep_scatter<X>, dim3(grid_stride_blocks(n)), dim3(256), 0, s, n, e.row.p, (const X*)lvl,
+                       scale2, out, out2, st);
 }
 
 // z = S U^-1 U^-T S r through the ELL sweeps: gather, forward sweep, gather, backward sweep.
 template<class X>
-void ell_apply(const nlf_precond_s* P, i64 n, const X* r, X* z, X* rhs, X* y1, X* y2, const CGState* st, hipStream_t s) {
+void ell_apply(const nlf_precond_s* P, i64 n, const X* r, X* z, X* rhs, X* lvl, X* y1, X* y2, const CGState* st,
+               hipStream_t s) {
     const unsigned g = grid_stride_blocks(n);
     hipLaunchKernelGGL(k_sweep_gather<X>, dim3(g), dim3(256), 0, s, n, P->efw.row.p, ell_scale<X>(P->efw), r, rhs, y1, st);
-    ell_sweep<X>(P->efw, rhs, y1, (X*)nullptr, (const X*)nullptr, st, s);
+    ell_sweep<X>(P->efw, rhs, lvl, y1, (X*)nullptr, (const X*)nullptr, st, s);
     hipLaunchKernelGGL(k_sweep_gather<X>, dim3(g), dim3(256), 0, s, n, P->ebw.row.p, (const X*)nullptr, y1, rhs, y2, st);
-    ell_sweep<X>(P->ebw, rhs, y2, z, ell_scale<X>(P->ebw), st, s);
+    ell_sweep<X>(P->ebw, rhs, lvl, y2, z, ell_scale<X>(P->ebw), st, s);
 }
 
 // Applies the preconditioner z = M^-1 r on the stream (FP64 or FP32 vectors).
@@ -989,7 +1156,7 @@ void precond_apply<double>(const nlf_precond_s* P, i64 n, const double* r, doubl
     if (P && P->kind == NLF_PRECOND_JACOBI)
         hipLaunchKernelGGL(k_scale<double>, dim3(grid_stride_blocks(n)), dim3(256), 0, s, n, P->invd.p, r, z);
     else if (P && P->kind == NLF_PRECOND_ILLT && P->ell)
-        ell_apply<double>(P, n, r, z, P->rhs_lvl.p, P->y1.p, P->y2.p, st, s);
+        ell_apply<double>(P, n, r, z, P->rhs_lvl.p, P->y_lvl.p, P->y1.p, P->y2.p, st, s);
     else if (P && P->kind == NLF_PRECOND_ILLT) {
         const size_t lds = size_t(std::max(P->fw.K * P->fw.W, P->bw.K * P->bw.W)) * sizeof(double);
         hipLaunchKernelGGL((k_ic_solve<double>), dim3(1), dim3(P->solve_threads), lds, s, P->fw.f64(), P->bw.f64(), r,
@@ -1003,7 +1170,7 @@ void precond_apply<float>(const nlf_precond_s* P, i64 n, const float* r, float* 
     if (P && P->kind == NLF_PRECOND_JACOBI)
         hipLaunchKernelGGL(k_scale<float>, dim3(grid_stride_blocks(n)), dim3(256), 0, s, n, P->invd32.p, r, z);
     else if (P && P->kind == NLF_PRECOND_ILLT && P->ell)
-        ell_apply<float>(P, n, r, z, P->rhs_lvl32.p, P->y1_32.p, P->y2_32.p, st, s);
+        ell_apply<float>(P, n, r, z, P->rhs_lvl32.p, P->y_lvl32.p, P->y1_32.p, P->y2_32.p, st, s);
     else if (P && P->kind == NLF_PRECOND_ILLT) {
         const size_t lds = size_t(std::max(P->fw.K * P->fw.W, P->bw.K * P->bw.W)) * sizeof(float);
         hipLaunchKernelGGL((k_ic_solve<float>), dim3(1), dim3(P->solve_threads), lds, s, P->fw.f32(), P->bw.f32(), r,
@@ -1592,8 +1759,9 @@ int nlf_precond_create(nlf_matrix a, int kind, nlf_precond* out) {
             P->ell = build_ell(outer, cols, diag, fr, fp, true, P->F.p, P->scale.p, P->diag.p, P->efw, s) &&
                      build_ell(outer, cols, diag, br, bp, false, P->F.p, P->scale.p, P->diag.p, P->ebw, s);
             if (P->ell) {
-                P->rhs_lvl.alloc(size_t(n)); P->y1.alloc(size_t(n)); P->y2.alloc(size_t(n));
-                P->rhs_lvl32.alloc(size_t(n)); P->y1_32.alloc(size_t(n)); P->y2_32.alloc(size_t(n));
+                P->rhs_lvl.alloc(size_t(n)); P->y_lvl.alloc(size_t(n)); P->y1.alloc(size_t(n)); P->y2.alloc(size_t(n));
+                P->rhs_lvl32.alloc(size_t(n)); P->y_lvl32.alloc(size_t(n)); P->y1_32.alloc(size_t(n));
+                P->y2_32.alloc(size_t(n));
             }
         } else {
             throw Error(NLF_ERR_UNSUPPORTED, "use nlf_learned_precond_create for the learned preconditioner");
@@ -1617,6 +1785,28 @@ int nlf_precond_info(nlf_precond p, int* ok, double* shift, int* fw_levels, int*
     if (fw_levels) *fw_levels = p->fw_levels;
     if (bw_levels) *bw_levels = p->bw_levels;
     if (setup_seconds) *setup_seconds = p->setup_seconds;
+    return NLF_OK;
+}
+
+int nlf_precond_sweep_info(nlf_precond p, int32_t out[8]) {
+    if (!p || !out) return NLF_ERR_ARGUMENT;
+    for (int i = 0; i < 8; ++i) out[i] = 0;
+    if (p->kind != NLF_PRECOND_ILLT) return NLF_ERR_ARGUMENT;
+    const nlf_precond_s::Ell* e[2] = {&p->efw, &p->ebw};
+    for (int k = 0; k < 2; ++k) {
+        if (p->ell) {
+            out[4 * k] = e[k]->levels;
+            out[4 * k + 1] = e[k]->threads;
+            out[4 * k + 2] = e[k]->K;
+            out[4 * k + 3] = nlf::ell_prefetching(*e[k], sizeof(double)) ? 2 : 1;
+        } else {
+            const auto& w = k ? p->bw : p->fw;
+            out[4 * k] = w.levels;
+            out[4 * k + 1] = p->solve_threads;
+            out[4 * k + 2] = w.K;
+            out[4 * k + 3] = 0;
+        }
+    }
     return NLF_OK;
 }
 
