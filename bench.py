@@ -58,7 +58,7 @@ def parse():
     ap.add_argument("--cpu-cg-iterations", type=int, default=40,
                     help="ILLT-CG iterations the oracle runs on the full workload for its per-iteration time")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--methods", default="fp64_jacobi,mixed_jacobi,learned_fcg_onelevel,learned_fcg")
+    ap.add_argument("--methods", default="fp64_jacobi,mixed_jacobi,learned_fcg_onelevel,coarse_jacobi,learned_fcg")
     ap.add_argument("--workload", default="composite1000",
                     choices=["composite1000", "batch64", "precision", "spmv", "flux"],
                     help="composite1000: the 1000x1000 composite (the metric's workload); batch64: 64 uneven "
@@ -70,12 +70,12 @@ def parse():
     return ap.parse_args()
 
 
-def run_step(api, A, mesh, problem, precond, mode, tol, coarse=None):
+def run_step(api, A, mesh, problem, precond, mode, tol, coarse=None, smoother=None):
     """One pass of the hot path; returns (solution, timings)."""
     torch = api.torch
     t0 = time.perf_counter()
     sol = api._solve_driver(mesh, problem.physics, problem.models, problem.materials, problem.bcs, precond, tol,
-                            30000, mode, coarse=coarse)
+                            30000, mode, coarse=coarse, smoother=smoother)
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     return sol, wall
@@ -465,8 +465,12 @@ def main():
         "learned_fcg_mixed": (A.NLF_PRECOND_LEARNED, A.NLF_MODE_MIXED),
         "split_jacobi": (A.NLF_PRECOND_JACOBI, A.NLF_MODE_SPLIT),
         "learned_fcg_onelevel": (A.NLF_PRECOND_LEARNED, A.NLF_MODE_FP64),
+        # the flagship's two-level form with the network switched off (its fixed Jacobi term and the
+        # same coarse space): the difference to learned_fcg is what the trained network adds
+        "coarse_jacobi": (A.NLF_PRECOND_LEARNED, A.NLF_MODE_FP64),
     }
     coarse_of = {"learned_fcg_onelevel": 0}
+    smoother_of = {"coarse_jacobi": "jacobi"}
     # plain FP64 ILLT-CG: the baseline the metric compares against; also the FP64 reference solution
     log("[bench] plain FP64 ILLT-CG")
     ref, t_plain = run_step(api, A, mesh, problem, *methods["plain_fp64_illt"], args.tol)
@@ -480,7 +484,7 @@ def main():
     best = None
     for name in [m for m in args.methods.split(",") if m]:
         pc, mode = methods[name]
-        sol, t = run_step(api, A, mesh, problem, pc, mode, args.tol, coarse_of.get(name))
+        sol, t = run_step(api, A, mesh, problem, pc, mode, args.tol, coarse_of.get(name), smoother_of.get(name))
         err = float(torch.linalg.norm(sol.x - x_ref) / torch.linalg.norm(x_ref))
         log(f"[bench] {name}: {t:.3f} s, it {sol.iterations}, outer {sol.stats.outer_iterations}, "
             f"true res {sol.true_residual:.2e}, rel err vs FP64 {err:.2e}, timings {sol.timings}")
@@ -494,7 +498,7 @@ def main():
     pc, mode = methods[best]
     log(f"[bench] flagship method: {best}; warmup {args.warmup}, steps {args.steps}")
     for _ in range(args.warmup):
-        run_step(api, A, mesh, problem, pc, mode, args.tol, coarse_of.get(best))
+        run_step(api, A, mesh, problem, pc, mode, args.tol, coarse_of.get(best), smoother_of.get(best))
     torch.cuda.synchronize()
     # Each step's Solution (matrix, local matrix, preconditioner) is released before the next step
     # starts, so every step runs with the same free device memory; only the last step's results are
@@ -504,7 +508,7 @@ def main():
     t0 = time.perf_counter()
     for k in range(args.steps):
         last = None
-        sol, t_step = run_step(api, A, mesh, problem, pc, mode, args.tol, coarse_of.get(best))
+        sol, t_step = run_step(api, A, mesh, problem, pc, mode, args.tol, coarse_of.get(best), smoother_of.get(best))
         step_s.append(t_step)
         if k == args.steps - 1:
             last = dict(x=sol.x.clone(), info=sol.matrix.info(), stats=sol.stats, timings=dict(sol.timings),

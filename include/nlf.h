@@ -314,8 +314,10 @@ int nlf_learned_get_weights(nlf_precond p, float* w_host);
  * three-hop pattern (applied as one SpMV); compiled = 0 when a row's node sets exceed the kernel's
  * capacities, the network kernels are then used */
 int nlf_learned_info(nlf_precond p, int* compiled, int64_t* nnz, double* compile_seconds);
-/* 0 = apply the compiled operator when available (default), 1 = always run the network kernels */
-int nlf_learned_set_apply(nlf_precond p, int network);
+/* 0 = apply the compiled operator when available (default), 1 = always run the network kernels,
+ * 2 = apply only the network's fixed Jacobi term s^2 r (s = diag^-1/2; the learned correction off,
+ * the coarse part unchanged: what the trained network adds is the difference from this) */
+int nlf_learned_set_apply(nlf_precond p, int mode);
 /* No reference counterpart.  Two-level form of the learned preconditioner: B r = M r + P A_c^-1 P^T r
  * with P the 0/1 map of rows to about `aggregates` aggregates (a grid of cells over the node
  * coordinates, at most 4096 coarse rows and at least 16 nodes per aggregate on average) and

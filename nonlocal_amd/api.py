@@ -637,7 +637,7 @@ def _radii(models):
 
 def _solve_driver(mesh: mesh_2d, physics, models, materials, bcs, precond, tol, max_it, mode, weights=None,
                   assembly32=False, coarse=None, values=None, source=None, delta_temperature=None, energy=0.0,
-                  coefficients=None):
+                  coefficients=None, smoother=None):
     import time
     ctx = mesh.ctx
     dof = 1 if physics == 0 else 2
@@ -688,6 +688,8 @@ def _solve_driver(mesh: mesh_2d, physics, models, materials, bcs, precond, tol, 
         KL = _assemble(mesh, physics, models, materials, inner, part=NLF_PART_LOCAL)
         w, _ = learned.load_weights(weights or learned.DEFAULT_WEIGHTS)
         learned_pc = learned.LearnedPreconditioner(K, KL, models, w)
+        if smoother is not None:
+            learned_pc.set_apply(smoother)
         learned_pc.set_coarse(learned.DEFAULT_AGGREGATES if coarse is None else coarse)
         fallback = preconditioner(K, NLF_PRECOND_JACOBI)
     else:
@@ -724,7 +726,7 @@ def _solve_driver(mesh: mesh_2d, physics, models, materials, bcs, precond, tol, 
 def stationary_heat_equation_solver_2d(mesh: mesh_2d, models, materials, bcs, precond=NLF_PRECOND_ILLT, tol=1e-12,
                                        max_it=100000, mode=NLF_MODE_FP64, weights=None, assembly32=False,
                                        coarse=None, values=None, source=None, energy=0.0,
-                                       coefficients=None) -> Solution:
+                                       coefficients=None, smoother=None) -> Solution:
     """K T = f with K = p1 K_L + (1 - p1) K_NL (solved directly; the reference's correction form
     T = T0 - K^-1 (K T0 - f) with T0 = 0 is the same system).  precond NLF_PRECOND_LEARNED runs
     flexible CG with the learned preconditioner (weights file `weights`, default the packaged one)
@@ -736,9 +738,11 @@ def stationary_heat_equation_solver_2d(mesh: mesh_2d, models, materials, bcs, pr
     integrated as int s N_i.  A problem whose conditions are all fluxes is a Neumann problem: it is
     solved with the Lagrange row (sum of T weighted by int N_i = energy) and its solution vector has
     N + 1 entries (the temperature, then the multiplier).  coefficients: per element group None or its
-    conductivity at its quadrature points (all groups local)."""
+    conductivity at its quadrature points (all groups local).  smoother "jacobi" (learned only) keeps
+    the coarse part and replaces the network by its fixed Jacobi term, the form that shows what the
+    trained network adds."""
     return _solve_driver(mesh, 0, models, materials, bcs, precond, tol, max_it, mode, weights, assembly32, coarse,
-                         values, source, energy=energy, coefficients=coefficients)
+                         values, source, energy=energy, coefficients=coefficients, smoother=smoother)
 
 
 def equilibrium_equation(mesh: mesh_2d, models, materials, bcs, precond=None, tol=1e-12, max_it=100000,

@@ -58,6 +58,7 @@ struct LearnedGraph {
     DevBuf<float> m_val;
     double compile_seconds = 0.0;
     bool network_apply = false;                 // run the network kernels even when M is compiled
+    bool jacobi_apply = false;                  // apply only the network's fixed term s^2 r (Jacobi)
     std::unique_ptr<CoarseSpace> coarse;        // two-level form: + P A_c^-1 P^T r (nlf_learned_set_coarse)
     DevBuf<float> g0, g1, gdir, gm;             // backward work
     DevBuf<double> part, sigma, zbuf, az;
@@ -1301,10 +1302,24 @@ void learned_apply_ell(LearnedGraph& g, const float* w, const double* r, Z* z, h
                        g.y.p, g.s.p, g.sigma.p, r, z);
 }
 
+// z = s^2 r: the fixed first term of the network's output alone, i.e. Jacobi with the learned
+// correction switched off (the comparison that shows what the trained network adds)
+template<class Z>
+__global__ void k_lf_jacobi(i64 n, const float* __restrict__ s, const double* __restrict__ r, Z* __restrict__ z) {
+    for (i64 v = i64(blockIdx.x) * blockDim.x + threadIdx.x; v < n; v += i64(gridDim.x) * blockDim.x) {
+        const double sv = double(s[v]);
+        z[v] = Z(sv * sv * r[v]);
+    }
+}
+
 // z = M r, the symmetric part of N, on the lane-per-node kernels: forward layers (A, B channel-major
 // ping-pong), y; transposed layers (d in place, q through A then B) ending in the output.
 template<class Z>
 void learned_apply_one(LearnedGraph& g, const float* w, const double* r, Z* z, hipStream_t s) {
+    if (g.jacobi_apply) {
+        hipLaunchKernelGGL(k_lf_jacobi<Z>, dim3(grid_stride_blocks(g.n)), dim3(256), 0, s, g.n, g.s.p, r, z);
+        return;
+    }
     if (g.m_ready && !g.network_apply) {
         const int lpr = pick_lpr(double(g.m_nnz) / double(std::max<i64>(g.n, 1)), g.m_off16.p != nullptr);
         const SpmvCols cs = g.m_pstart.p ? SpmvCols{g.m_cols.p, g.m_pat.p, g.m_pstart.p}
@@ -1704,10 +1719,15 @@ int nlf_learned_coarse_info(nlf_precond p, int64_t* coarse_rows, double* setup_s
     });
 }
 
-// Application path: 0 = the compiled sparse operator when available (default), 1 = the network kernels.
-int nlf_learned_set_apply(nlf_precond p, int network) {
-    if (!p) return NLF_ERR_ARGUMENT;
-    return guarded(p->ctx, [&] { learned_of(p)->network_apply = network != 0; });
+// Application path: 0 = the compiled sparse operator when available (default), 1 = the network
+// kernels, 2 = the network's Jacobi term s^2 r alone (learned correction off).
+int nlf_learned_set_apply(nlf_precond p, int mode) {
+    if (!p || mode < 0 || mode > 2) return NLF_ERR_ARGUMENT;
+    return guarded(p->ctx, [&] {
+        LearnedGraph& g = *learned_of(p);
+        g.network_apply = mode == 1;
+        g.jacobi_apply = mode == 2;
+    });
 }
 
 int nlf_learned_get_weights(nlf_precond p, float* w) {

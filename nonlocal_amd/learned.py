@@ -97,9 +97,11 @@ class LearnedPreconditioner:
         self.ctx.check(load().nlf_learned_info(self.h, C.byref(c), C.byref(nnz), C.byref(sec)))
         return bool(c.value), nnz.value, sec.value
 
-    def set_apply(self, network: bool):
-        """Apply through the network kernels (True) or the compiled sparse operator (False, default)."""
-        self.ctx.check(load().nlf_learned_set_apply(self.h, int(network)))
+    def set_apply(self, network):
+        """Apply through the network kernels (True), the compiled sparse operator (False, default), or
+        "jacobi": the network's fixed term diag^-1 r alone (learned correction off, coarse part kept)."""
+        mode = 2 if network == "jacobi" else int(bool(network))
+        self.ctx.check(load().nlf_learned_set_apply(self.h, mode))
 
     def set_coarse(self, aggregates: int = DEFAULT_AGGREGATES, matrix: api.finite_element_matrix | None = None):
         """Two-level form B r = M r + P A_c^-1 P^T r with about `aggregates` aggregates of the system

@@ -215,3 +215,46 @@ def test_trained_network_gain(gpu_ctx):
     assert st.fallbacks == 0 and st.true_residual <= 1e-10
     assert st.iterations < 0.8 * st0.iterations
     assert st.iterations < 0.8 * cg.iterations()
+
+
+def test_jacobi_apply_mode_is_jacobi(small, gpu_ctx):
+    """set_apply("jacobi") applies the network's fixed term alone: diag^-1 r (FP32 scale s = diag^-1/2)."""
+    tp, Kf = small
+    tp.learned.set_weights(_gpu_weights(5))
+    tp.learned.set_apply("jacobi")
+    r = np.random.default_rng(8).standard_normal(Kf.shape[0])
+    z = tp.learned.solve(torch.from_numpy(r).to(gpu_ctx.device)).cpu().numpy()
+    tp.learned.set_apply(False)
+    assert rel_l2(z, r / Kf.diagonal()) < 1e-6
+
+
+def test_trained_network_gain_two_level(gpu_ctx):
+    """The flagship's two-level form on a held-out 128 x 128 composite: flexible CG with the trained
+    network against the same two-level FCG with the network switched off (its fixed Jacobi term,
+    set_apply("jacobi")), both with one aggregation coarse space.  The coarse space is sized as on the
+    1000 x 1000 benchmark (about 245 nodes per aggregate: 4096 aggregates of 1002001 nodes); there the
+    network must need fewer than 0.8x the iterations.  With the default density at this size (16
+    nodes per aggregate, a coarse space rich enough to leave the smoother little to do) the counts
+    are printed for reference."""
+    if not os.path.exists(learned.DEFAULT_WEIGHTS):
+        pytest.skip("no trained weights committed")
+    w, meta = learned.load_weights()
+    tp = learned.TrainingProblem(128, seed=1234, ctx=gpu_ctx)
+    tp.learned.set_weights(w)
+    n = tp.problem.mesh.n_nodes
+    counts = {}
+    for label, aggregates in (("benchmark density", max(1, round(n / 245))), ("default", learned.DEFAULT_AGGREGATES)):
+        tp.learned.set_coarse(aggregates)
+        rows, _ = tp.learned.coarse_info()
+        tp.learned.set_apply(False)
+        _, st = learned.fcg_solve(tp.K, tp.learned, tp.b, tol=1e-10, max_it=20000)
+        tp.learned.set_apply("jacobi")
+        _, st0 = learned.fcg_solve(tp.K, tp.learned, tp.b, tol=1e-10, max_it=20000)
+        tp.learned.set_apply(False)
+        print(f"[learned] held-out 128x128, {label}: {rows} coarse rows: network {st.iterations} it, "
+              f"network off (Jacobi term + coarse) {st0.iterations} it", flush=True)
+        assert rows > 0
+        assert st.fallbacks == 0 and st.true_residual <= 1e-10 and st0.true_residual <= 1e-10
+        counts[label] = (st.iterations, st0.iterations)
+    it, it0 = counts["benchmark density"]
+    assert it < 0.8 * it0
