@@ -154,6 +154,9 @@ int nlf_mesh_info(nlf_mesh m, int64_t* n_nodes, int64_t* n_elems, int* nn, int* 
 /* quad coords [E][nq][2], |det J| [E][nq], det J * grad N [E][nn][nq][2], centres [E][2]
  * (mesh_2d::quad_coord / jacobian / derivatives, mesh_2d.hpp:141-179; centres mesh_container_2d.hpp:130-139) */
 int nlf_mesh_export_quadrature(nlf_mesh m, double* xq, double* jac, double* D, double* centres);
+/* the element's quadrature weights [nq] and shape values N_i(q) [nn][nq] (element_2d::weight / qN,
+ * as mesh::utils::integrate uses them, mesh_2d_utils.hpp:135-149) */
+int nlf_mesh_export_basis(nlf_mesh m, double* weights, double* qN);
 
 /* ---- neighbours: replaces find_neighbours (libraries/mesh/mesh_2d/find_neighbours.hpp:119-160) and
  * mesh_2d::neighbours (mesh_2d.hpp:205-212).  search_radii[2G]; <= 0 skips the group. */

@@ -36,6 +36,7 @@ SIGNATURES = {
     "nlf_mesh_destroy": (_int, [_p]),
     "nlf_mesh_info": (_int, [_p, _i64p, _i64p, C.POINTER(_int), C.POINTER(_int)]),
     "nlf_mesh_export_quadrature": (_int, [_p, _f64p, _f64p, _f64p, _f64p]),
+    "nlf_mesh_export_basis": (_int, [_p, _f64p, _f64p]),
     "nlf_find_neighbours": (_int, [_p, _f64p, _int, _f64p]),
     "nlf_neighbours_count": (_int, [_p, _i64p]),
     "nlf_neighbours_export": (_int, [_p, _i64p, _i32p]),

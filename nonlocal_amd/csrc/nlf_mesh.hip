@@ -393,6 +393,19 @@ int nlf_mesh_info(nlf_mesh m, int64_t* n_nodes, int64_t* n_elems, int* nn, int* 
 }
 
 // Host export of the precompute (parity checks against the oracle's mesh_2d tables).
+// The element's quadrature weights [nq] and shape values N_i at its quadrature points [nn][nq]
+// (element_2d::weight / qN of the reference's mesh container), host tables.
+int nlf_mesh_export_basis(nlf_mesh m, double* weights, double* qN) {
+    if (!m) return NLF_ERR_ARGUMENT;
+    return guarded(m->ctx, [&] {
+        const nlf::FeTables& fe = m->fe;
+        if (weights)
+            for (int q = 0; q < fe.nq; ++q) weights[q] = fe.w[q];
+        if (qN)
+            for (int k = 0; k < fe.nn * fe.nq; ++k) qN[k] = fe.qN[k];
+    });
+}
+
 int nlf_mesh_export_quadrature(nlf_mesh m, double* xq, double* jac, double* D, double* centres) {
     if (!m) return NLF_ERR_ARGUMENT;
     return guarded(m->ctx, [&] {

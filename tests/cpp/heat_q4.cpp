@@ -1,6 +1,6 @@
-// Stationary nonlocal heat conduction on an n x n Q4 mesh through the C++ mirror of the reference API
-// (mesh_2d, find_neighbours, conductivity_matrix_2d, conjugate_gradient with ILLT), written the way
-// the reference's own tests drive the solver (isotropic_solid_ring.cpp-style set-up).
+// Stationary nonlocal heat conduction on an n x n Q4 mesh through the C++ mirror of the reference API,
+// set up the way the reference's NonLocFEM driver does it: the mesh, the parameters with a bell
+// influence, the neighbour search adopted by the mesh, the boundary conditions, the solver.
 //
 //   heat_q4 <n> <p1> <r> <out.bin>
 //
@@ -20,9 +20,13 @@ int main(int argc, char** argv) {
     }
     const int n = std::atoi(argv[1]);
     const double p1 = std::atof(argv[2]), r = std::atof(argv[3]);
+    using T = double;
+    using I = int64_t;
     using namespace nonlocal;
+    using namespace nonlocal::mesh;
+    using namespace nonlocal::solver_2d::thermal;
     try {
-        mesh::mesh_2d_data d;
+        mesh_2d_data d;
         auto v = [n](int i, int j) { return int32_t(j * (n + 1) + i); };
         for (int j = 0; j <= n; ++j)
             for (int i = 0; i <= n; ++i) d.nodes.push_back({double(i) / n, double(j) / n});
@@ -37,21 +41,23 @@ int main(int argc, char** argv) {
         for (int i = 0; i < n; ++i) { d.boundary_edges.push_back(v(i, 0)); d.boundary_edges.push_back(v(i + 1, 0)); }
         for (int i = 0; i < n; ++i) { d.boundary_edges.push_back(v(i, n)); d.boundary_edges.push_back(v(i + 1, n)); }
         d.boundary_begin = {0, n, 2 * int64_t(n), 3 * int64_t(n), 4 * int64_t(n)};
-        auto m = std::make_shared<mesh::mesh_2d<double, int64_t>>(d);
+        const auto mesh = std::make_shared<mesh_2d<T, I>>(d);
 
-        thermal::parameters_2d parameters;
-        parameters["material"] = thermal::parameter_2d{model_parameters::bell(p1, r), {NLF_CONDUCTIVITY_ISOTROPIC, 0, {1.0, 0, 0}}};
-        thermal::boundaries_conditions_2d conditions;
-        conditions["left"] = thermal::boundary_condition::temperature(0.0);
-        conditions["right"] = thermal::boundary_condition::flux(1.0);
-        const auto sol = thermal::stationary_heat_equation_solver_2d(m, parameters, conditions, 1e-13, 100000);
+        const parameters_2d<T> parameters = {
+            {"material", {.model = {.influence = solver_2d::influence::polynomial<T>({r, r}, 2, 2, 2), .local_weight = p1},
+                          .physical = {.conductivity = T{1}}}}};
+        mesh->neighbours(find_neighbours(*mesh, influences<T>{{"material", {.radius = {r, r}}}}));
+        thermal_boundaries_conditions_2d<T> boundaries_conditions;
+        boundaries_conditions["left"] = std::make_unique<temperature_2d<T>>(T{0});
+        boundaries_conditions["right"] = std::make_unique<flux_2d<T>>(T{1});
+        const auto sol = stationary_heat_equation_solver_2d<I>(mesh, parameters, boundaries_conditions, {});
         std::ofstream out(argv[4], std::ios::binary);
         auto write = [&out](const std::vector<double>& v) {
             out.write(reinterpret_cast<const char*>(v.data()), std::streamsize(v.size() * sizeof(double)));
         };
         write(sol.temperature());
-        write(sol.flux()[0]);
-        write(sol.flux()[1]);
+        write(sol.flux()[X]);
+        write(sol.flux()[Y]);
         std::printf("iterations %ju residual %.3e true_residual %.3e\n", sol.iterations, sol.residual, sol.true_residual);
         return 0;
     } catch (const std::exception& e) {

@@ -1,58 +1,82 @@
-// saint_venant.cpp of the reference (tests/solvers_2d/mechanical_2d/saint_venant.cpp:26-77) through the
-// C++ mirror of its API: a 10 x 1 plate of 80 x 8 bilinear quads (h = 0.125, x in [-5, 5], y in
-// [-0.5, 0.5]), E = 1, nu = 0.3; u_x = 0 on the vertical centre line, u_y = 0 on the horizontal centre
-// line, x traction -4|y| on the left edge and +4|y| on the right edge, as functions of the point.
-//
-//   saint_venant
-//
-// Prints the largest |u_x| and |sigma_xx - 1| on the centre line x = 0, and the iterations.
+// tests/solvers_2d/mechanical_2d/saint_venant.cpp of the reference, written against the C++ mirror of
+// its API: the 10 x 1 plate of its SU2 mesh (80 x 8 bilinear quads, h = 0.125), E = 1, nu = 0.3;
+// u_x = 0 on "Vertical", u_y = 0 on "Horizontal", x traction -4|y| on "Left" and +4|y| on "Right".
+// The checks and bars are the reference's: u_x = 0 (2e-14) and sigma_xx = 1 (1.3e-9) on the centre
+// line, the integrals of u_x (0, 7.5e-13) and sigma_xx (10, 1e-9).  Exit status 0 when all pass.
+#include "test_utils.hpp"
+
 #include <nonlocal/nonlocal_amd.hpp>
 
 #include <cmath>
-#include <cstdio>
+#include <limits>
+#include <memory>
+#include <sstream>
 
-int main() {
-    using namespace nonlocal;
-    try {
-        const int nx = 80, ny = 8;
-        const double h = 0.125;
-        mesh::mesh_2d_data d;
-        auto v = [nx](int i, int j) { return int32_t(j * (nx + 1) + i); };
-        for (int j = 0; j <= ny; ++j)
-            for (int i = 0; i <= nx; ++i) d.nodes.push_back({-5.0 + i * h, -0.5 + j * h});
-        for (int j = 0; j < ny; ++j)
-            for (int i = 0; i < nx; ++i)
-                for (int32_t id : {v(i, j), v(i + 1, j), v(i + 1, j + 1), v(i, j + 1)}) d.elements.push_back(id);
-        d.group_names = {"DEFAULT"};
-        d.group_begin = {0, int64_t(nx) * ny};
-        d.boundary_names = {"Left", "Right", "Vertical", "Horizontal"};
-        for (int j = 0; j < ny; ++j) { d.boundary_edges.push_back(v(0, j)); d.boundary_edges.push_back(v(0, j + 1)); }
-        for (int j = 0; j < ny; ++j) { d.boundary_edges.push_back(v(nx, j)); d.boundary_edges.push_back(v(nx, j + 1)); }
-        for (int j = 0; j < ny; ++j) { d.boundary_edges.push_back(v(nx / 2, j)); d.boundary_edges.push_back(v(nx / 2, j + 1)); }
-        for (int i = 0; i < nx; ++i) { d.boundary_edges.push_back(v(i, ny / 2)); d.boundary_edges.push_back(v(i + 1, ny / 2)); }
-        d.boundary_begin = {0, ny, 2 * ny, 3 * ny, 3 * ny + nx};
-        auto m = std::make_shared<mesh::mesh_2d<double, int64_t>>(d);
+#ifndef NLF_MESH_DIR
+#define NLF_MESH_DIR "tests/golden/meshes"
+#endif
 
-        mechanical::parameters_2d parameters;
-        parameters["DEFAULT"] = mechanical::parameter_2d{model_parameters{}, mechanical::isotropic_elastic_parameters(1.0, 0.3)};
-        using bc = mechanical::boundary_condition;
-        mechanical::boundaries_conditions_2d conditions;
-        conditions["Vertical"] = {bc::displacement(0.0), bc::pressure(0.0)};
-        conditions["Horizontal"] = {bc::pressure(0.0), bc::displacement(0.0)};
-        conditions["Left"] = {bc::pressure([](const std::array<double, 2>& p) { return -4 * std::abs(p[1]); }), bc::pressure(0.0)};
-        conditions["Right"] = {bc::pressure([](const std::array<double, 2>& p) { return 4 * std::abs(p[1]); }), bc::pressure(0.0)};
-        const auto sol = mechanical::equilibrium_equation(m, parameters, conditions, 1e-15, 100000);
+namespace {
 
-        double ux = 0.0, sxx = 0.0;
-        for (int j = 0; j <= ny; ++j) {
-            const size_t node = size_t(v(nx / 2, j));
-            ux = std::max(ux, std::abs(sol.displacement()[0][node]));
-            sxx = std::max(sxx, std::abs(sol.stress()[0][node] - 1.0));
-        }
-        std::printf("iterations %ju ux_centre %.3e sxx_centre %.3e\n", sol.iterations, ux, sxx);
-        return 0;
-    } catch (const std::exception& e) {
-        std::fprintf(stderr, "error: %s\n", e.what());
-        return 1;
-    }
+using T = double;
+using I = int64_t;
+using namespace nonlocal;
+using namespace unit_tests;
+using namespace mesh;
+using namespace solver_2d::mechanical;
+
+// the reference embeds the file at build time (embedded_files/plate_10x1_h0_125_su2.h)
+const std::string plate_10x1_h0_125_su2_data = file_text(NLF_MESH_DIR "/plate_10x1_h0.125.su2");
+
+std::vector<T> get_values_on_center_line(const mesh_container_2d<T, I>& mesh, const std::vector<T>& solution) {
+    std::vector<T> values;
+    for (const size_t node : mesh.nodes())
+        if (std::abs(mesh.node_coord(node)[X]) < std::numeric_limits<T>::epsilon())
+            values.push_back(solution[node]);
+    return values;
+}
+
+}  // namespace
+
+int main() try {
+    checks expect;
+    std::stringstream stream{plate_10x1_h0_125_su2_data};
+    const auto mesh = std::make_shared<mesh_2d<T, I>>(stream, mesh_format::SU2);
+    const raw_mechanical_parameters<T> parameters = {
+        {"DEFAULT", { .physical = { .elastic = isotropic_elastic_parameters<T>{ .young_modulus = 1., .poissons_ratio = 0.3 } } }}
+    };
+    mechanical_boundaries_conditions_2d<T> boundaries_conditions;
+    boundaries_conditions["Vertical"] = {
+        std::make_unique<displacement_2d<T>>(T{0}),
+        std::make_unique<pressure_2d<T>>(T{0})
+    };
+    boundaries_conditions["Horizontal"] = {
+        std::make_unique<pressure_2d<T>>(T{0}),
+        std::make_unique<displacement_2d<T>>(T{0})
+    };
+    boundaries_conditions["Left"] = {
+        std::make_unique<pressure_2d<T>>([](const std::array<T, 2>& point) { return -4 * std::abs(point[Y]); }),
+        std::make_unique<pressure_2d<T>>(T{0})
+    };
+    boundaries_conditions["Right"] = {
+        std::make_unique<pressure_2d<T>>([](const std::array<T, 2>& point) { return  4 * std::abs(point[Y]); }),
+        std::make_unique<pressure_2d<T>>(T{0})
+    };
+    auto solution = equilibrium_equation<I>(mesh, parameters, boundaries_conditions);
+    std::printf("iterations %ju\n", solution.iterations);
+
+    double worst = 0;
+    for (const T value : get_values_on_center_line(mesh->container(), solution.displacement()[X]))
+        worst = std::max(worst, std::abs(value));
+    expect.approx("displacement_x_on_center_line", worst, 0, 2e-14);
+    worst = 0;
+    for (const T value : get_values_on_center_line(mesh->container(), solution.stress()[0]))
+        worst = std::max(worst, std::abs(value - 1));
+    expect.approx("stress_xx_on_center_line", 1 + worst, 1, 1.3e-9);
+    expect.approx("displacement_x_integral", mesh::utils::integrate(*mesh, solution.displacement()[X]), 0, 7.5e-13);
+    expect.approx("stress_xx_integral", mesh::utils::integrate(*mesh, solution.stress()[0]), 10, 1e-9);
+    return expect.failed ? 1 : 0;
+} catch (const std::exception& e) {
+    std::fprintf(stderr, "error: %s\n", e.what());
+    return 2;
 }

@@ -37,16 +37,16 @@ int main(int argc, char** argv) {
         for (int i = 0; i < n; ++i) { d.boundary_edges.push_back(v(i, 0)); d.boundary_edges.push_back(v(i + 1, 0)); }
         for (int i = 0; i < n; ++i) { d.boundary_edges.push_back(v(i, n)); d.boundary_edges.push_back(v(i + 1, n)); }
         d.boundary_begin = {0, n, 2 * n, 3 * n, 4 * int64_t(n)};
-        auto m = std::make_shared<mesh::mesh_2d<double, int64_t>>(d);
-
-        thermal::parameters_2d parameters;
-        thermal::parameter_2d p{model_parameters::bell(p1, r), {NLF_CONDUCTIVITY_ISOTROPIC, 0, {1.0, 0, 0}}};
-        p.capacity = 2.0;
-        parameters["material"] = p;
-        thermal::boundaries_conditions_2d conditions;
-        conditions["left"] = thermal::boundary_condition::temperature(0.0);
-        conditions["right"] = thermal::boundary_condition::temperature(0.0);
-        thermal::nonstationary_heat_equation_solver_2d<> solver(m, tau);
+        const auto mesh = std::make_shared<mesh::mesh_2d<double, int64_t>>(d);
+        using namespace nonlocal::solver_2d::thermal;
+        const parameters_2d<double> parameters = {
+            {"material", {.model = {.influence = solver_2d::influence::polynomial<double>({r, r}, 2, 2, 2), .local_weight = p1},
+                          .physical = {.conductivity = 1.0, .capacity = 2.0}}}};
+        mesh->neighbours(mesh::find_neighbours(*mesh, mesh::influences<double>{{"material", {.radius = {r, r}}}}));
+        thermal_boundaries_conditions_2d<double> conditions;
+        conditions["left"] = std::make_unique<temperature_2d<double>>(0.0);
+        conditions["right"] = std::make_unique<temperature_2d<double>>(0.0);
+        nonstationary_heat_equation_solver_2d<double, int64_t, int64_t> solver(mesh, tau);
         solver.compute(parameters, conditions, [](const std::array<double, 2>& x) { return std::sin(M_PI * x[0]); });
         for (int k = 0; k < steps; ++k)
             solver.calc_step(conditions, [](const std::array<double, 2>& x) { return 1.0 + x[1]; });
