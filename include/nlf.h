@@ -34,6 +34,11 @@ extern "C" {
 #define NLF_INFLUENCE_CONSTANT    1   /* influence::constant (influence_functions_2d.hpp:39-59) */
 #define NLF_INFLUENCE_EXPONENTIAL 2   /* influence::exponential (influence_functions_2d.hpp:94-125) */
 
+/* Distance of an influence (read_distance_2d, config/read_model.hpp:120-131) */
+#define NLF_DISTANCE_LP                    0   /* powered_distance (mesh_2d/search_function.hpp:28-73) */
+#define NLF_DISTANCE_ELLIPSE_WITH_ROTATION 1   /* powered_distance_with_rotation (search_function.hpp:75-90) */
+#define NLF_DISTANCE_N_INFINITY           -1   /* distance_n of the maximum norm (metamath Infinity<size_t>) */
+
 #define NLF_EXPAND_NO   0            /* diameter_expanding_strategy (find_neighbours.hpp:7) */
 #define NLF_EXPAND_MAX  1
 #define NLF_EXPAND_MIN  2
@@ -68,9 +73,22 @@ typedef struct nlf_model {
     double  radius[2];         /* nonlocal radius of the influence function */
     double  search_radius[2];  /* neighbour-search radius before expansion; 0 => use radius */
     int32_t influence;         /* NLF_INFLUENCE_* */
-    int32_t distance_n;        /* Lp exponent of the distance; only 2 is implemented */
+    int32_t distance_n;        /* integer Lp exponent n of the distance (size_t_or<T> holding size_t):
+                                  0 (= 2), 1, 2, 3, ... or NLF_DISTANCE_N_INFINITY */
     double  p, q;              /* influence parameters (polynomial: p, q; exponential: p, q) */
+    int32_t distance;          /* NLF_DISTANCE_*; ELLIPSE_WITH_ROTATION takes n = 2 */
+    int32_t pad_;
+    double  distance_real;     /* > 0: a real Lp exponent n (size_t_or<T> holding T); distance_n is then
+                                  ignored.  0: distance_n decides */
 } nlf_model;
+
+/* Distance of one element group's neighbour search: the distance member of mesh::influence
+ * (search_function.hpp:16-20); kind, n and n_real as the distance fields of nlf_model. */
+typedef struct nlf_distance {
+    int32_t kind;              /* NLF_DISTANCE_* */
+    int32_t n;                 /* integer exponent, 0 (= 2) or NLF_DISTANCE_N_INFINITY */
+    double  n_real;            /* > 0: real exponent */
+} nlf_distance;
 
 typedef struct nlf_thermal_material {
     int32_t kind;              /* NLF_CONDUCTIVITY_* */
@@ -161,6 +179,11 @@ int nlf_mesh_export_basis(nlf_mesh m, double* weights, double* qN);
 /* ---- neighbours: replaces find_neighbours (libraries/mesh/mesh_2d/find_neighbours.hpp:119-160) and
  * mesh_2d::neighbours (mesh_2d.hpp:205-212).  search_radii[2G]; <= 0 skips the group. */
 int nlf_find_neighbours(nlf_mesh m, const double* search_radii, int strategy, double* expanded_out);
+/* the same search with a distance per group (distances[G]; NULL = Euclidean everywhere): the predicate
+ * distance(c_eL, c_eNL, r) <= 1 of find_neighbours.hpp:152-155 with each group's distance function.
+ * The radius expansion stays Euclidean (diameter_between_elements_centers, :49-72). */
+int nlf_find_neighbours_ex(nlf_mesh m, const double* search_radii, const nlf_distance* distances, int strategy,
+                           double* expanded_out);
 int nlf_neighbours_count(nlf_mesh m, int64_t* total);
 int nlf_neighbours_export(nlf_mesh m, int64_t* offsets, int32_t* ids);
 int nlf_neighbours_device(nlf_mesh m, const int64_t** offsets, const int32_t** ids);

@@ -1,5 +1,6 @@
 """ctypes mirror of the plain-data types declared in include/nlf.h."""
 import ctypes as C
+import math
 
 NLF_OK = 0
 NLF_ERR_ARGUMENT, NLF_ERR_HIP, NLF_ERR_STATE, NLF_ERR_UNSUPPORTED, NLF_ERR_NUMERIC = -1, -2, -3, -4, -5
@@ -8,6 +9,8 @@ NLF_ELEMENT_Q8 = 8
 NLF_INFLUENCE_POLYNOMIAL = 0
 NLF_INFLUENCE_CONSTANT = 1
 NLF_INFLUENCE_EXPONENTIAL = 2
+NLF_DISTANCE_LP, NLF_DISTANCE_ELLIPSE_WITH_ROTATION = 0, 1
+NLF_DISTANCE_N_INFINITY = -1
 NLF_EXPAND_NO, NLF_EXPAND_MAX, NLF_EXPAND_MIN, NLF_EXPAND_MEAN = 0, 1, 2, 3
 NLF_PART_LOCAL, NLF_PART_FULL = 0, 2
 NLF_CONDUCTIVITY_ISOTROPIC, NLF_CONDUCTIVITY_ORTHOTROPIC, NLF_CONDUCTIVITY_ANISOTROPIC = 0, 1, 2
@@ -25,7 +28,49 @@ NLF_OPT_PAIR_BUDGET_MIB = 5
 
 class Model(C.Structure):
     _fields_ = [("local_weight", C.c_double), ("radius", C.c_double * 2), ("search_radius", C.c_double * 2),
-                ("influence", C.c_int32), ("distance_n", C.c_int32), ("p", C.c_double), ("q", C.c_double)]
+                ("influence", C.c_int32), ("distance_n", C.c_int32), ("p", C.c_double), ("q", C.c_double),
+                ("distance", C.c_int32), ("pad_", C.c_int32), ("distance_real", C.c_double)]
+
+
+class Distance(C.Structure):
+    """nlf_distance: the distance function of one group's neighbour search."""
+    _fields_ = [("kind", C.c_int32), ("n", C.c_int32), ("n_real", C.c_double)]
+
+
+def distance_exponent(n):
+    """(distance_n, distance_real) of an exponent held like the reference's size_t_or<T>
+    (read_influence_parameter, read_model.hpp:97-114): an int is the integer alternative, a float the
+    real one, math.inf or "inf" / "infinity" the maximum norm."""
+    if isinstance(n, str):
+        if n in ("inf", "infinity"):
+            return NLF_DISTANCE_N_INFINITY, 0.0
+        raise ValueError(f"distance exponent {n!r} shall be a number or \"infinity\"")
+    if isinstance(n, bool):
+        raise ValueError("distance exponent shall be a number")
+    if isinstance(n, float) and math.isinf(n):
+        return NLF_DISTANCE_N_INFINITY, 0.0
+    if isinstance(n, int) or type(n).__name__.startswith(("int", "uint")):
+        if int(n) <= 0:
+            raise ValueError("distance exponent shall be greater than 0")
+        return int(n), 0.0
+    if not float(n) > 0.0:
+        raise ValueError("distance exponent shall be greater than 0")
+    return 0, float(n)
+
+
+def distance(kind=NLF_DISTANCE_LP, n=2):
+    """nlf_distance of a kind (NLF_DISTANCE_*) and exponent n (ignored by the rotated ellipse)."""
+    d = Distance()
+    d.kind = kind
+    d.n, d.n_real = distance_exponent(n) if kind == NLF_DISTANCE_LP else (2, 0.0)
+    return d
+
+
+def distance_of(m):
+    """The nlf_distance of an nlf_model."""
+    d = Distance()
+    d.kind, d.n, d.n_real = m.distance, m.distance_n, m.distance_real
+    return d
 
 
 class ThermalMaterial(C.Structure):
@@ -55,8 +100,10 @@ class SolveStats(C.Structure):
                 ("spmv_samples", C.c_int64), ("spmv_bytes", C.c_int64)]
 
 
-def model(local_weight, radius, search_radius=None, influence=NLF_INFLUENCE_POLYNOMIAL, p=2.0, q=1.0):
-    """Build an nlf_model; `radius` may be a scalar (isotropic) or a pair."""
+def model(local_weight, radius, search_radius=None, influence=NLF_INFLUENCE_POLYNOMIAL, p=2.0, q=1.0,
+          distance=NLF_DISTANCE_LP, n=2):
+    """Build an nlf_model; `radius` may be a scalar (isotropic) or a pair; `distance` NLF_DISTANCE_* with
+    the exponent `n` of the Lp distance (see distance_exponent)."""
     m = Model()
     m.local_weight = float(local_weight)
     r = (radius, radius) if not hasattr(radius, "__len__") else radius
@@ -66,7 +113,8 @@ def model(local_weight, radius, search_radius=None, influence=NLF_INFLUENCE_POLY
     s = (search_radius, search_radius) if not hasattr(search_radius, "__len__") else search_radius
     m.search_radius[0], m.search_radius[1] = float(s[0]), float(s[1])
     m.influence = influence
-    m.distance_n = 2
+    m.distance = distance
+    m.distance_n, m.distance_real = distance_exponent(n) if distance == NLF_DISTANCE_LP else (2, 0.0)
     m.p = float(p)
     m.q = float(q)
     return m

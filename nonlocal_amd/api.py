@@ -142,11 +142,18 @@ class mesh_2d:
         return off, ids[:total.value]
 
 
-def find_neighbours(mesh: mesh_2d, search_radii, strategy=NLF_EXPAND_MAX):
-    """find_neighbours + mesh_2d::neighbours: search_radii (G, 2); returns the expanded radii."""
+def find_neighbours(mesh: mesh_2d, search_radii, strategy=NLF_EXPAND_MAX, distances=None):
+    """find_neighbours + mesh_2d::neighbours: search_radii (G, 2); distances: one nlf_distance per group
+    (_abi.distance / _abi.distance_of), None = Euclidean; returns the expanded radii."""
     r = np.ascontiguousarray(np.asarray(search_radii, dtype=np.float64).reshape(-1, 2))
     expanded = np.zeros_like(r)
-    mesh.ctx.check(load().nlf_find_neighbours(mesh.h, _ptr(r, _f64p), strategy, _ptr(expanded, _f64p)))
+    if distances is None:
+        mesh.ctx.check(load().nlf_find_neighbours(mesh.h, _ptr(r, _f64p), strategy, _ptr(expanded, _f64p)))
+    else:
+        if len(distances) != r.shape[0]:
+            raise ValueError("one distance per group")
+        arr = _abi.array_of(_abi.Distance, distances)
+        mesh.ctx.check(load().nlf_find_neighbours_ex(mesh.h, _ptr(r, _f64p), arr, strategy, _ptr(expanded, _f64p)))
     mesh.influences = expanded
     return expanded
 
@@ -506,7 +513,7 @@ class heat_equation_solution_2d:
             return self.flux()
         mesh = self.mesh
         if any(m.local_weight < 0.999 for m in self._models) and mesh.influences is None:
-            find_neighbours(mesh, _radii(self._models))
+            _search(mesh, self._models)
         n, E = mesh.nodes_count, mesh.elements_count
         nq = 4 if mesh.mesh.nn == 4 else 9
         out = torch.empty(2 * n, dtype=torch.float64, device=mesh.ctx.device)
@@ -568,7 +575,7 @@ class mechanical_solution_2d:
             return
         mesh = self.mesh
         if any(m.local_weight < 0.999 for m in self._models) and mesh.influences is None:
-            find_neighbours(mesh, _radii(self._models))
+            _search(mesh, self._models)
         n, E = mesh.nodes_count, mesh.elements_count
         nq = 4 if mesh.mesh.nn == 4 else 9
         dev = mesh.ctx.device
@@ -601,7 +608,7 @@ def temperature_condition(mesh: mesh_2d, models, materials, delta_temperature, f
     (temperature_condition, mechanical/temperature_condition_2d.hpp:128-170)."""
     dT = _node_vector(mesh, delta_temperature)
     if any(m.local_weight < 0.999 for m in models) and mesh.influences is None:
-        find_neighbours(mesh, _radii(models))
+        _search(mesh, models)
     marr = _abi.array_of(_abi.Model, models)
     matarr = _abi.array_of(_abi.ElasticMaterial, materials)
     mesh.ctx.check(load().nlf_temperature_condition(mesh.h, marr, matarr, _dev(dT), _dev(f)))
@@ -621,6 +628,16 @@ def boundary_data(mesh: mesh_2d, dof, bcs, values=None):
     mesh.ctx.check(load().nlf_boundary_data_fn(mesh.h, dof, arr, C.cast(fn, C.c_void_p) if fn else None, None,
                                                _ptr(f, _f64p), _ptr(xD, _f64p), _ptr(inner, _u8p)))
     return f, xD, inner
+
+
+def _distances(models):
+    """The distance function of each group's influence (read_influences, read_model.hpp:226-244)."""
+    return [_abi.distance_of(m) for m in models]
+
+
+def _search(mesh: mesh_2d, models):
+    """Neighbour search of the models' groups: their search radii and distances."""
+    return find_neighbours(mesh, _radii(models), distances=_distances(models))
 
 
 def _radii(models):
@@ -644,7 +661,7 @@ def _solve_driver(mesh: mesh_2d, physics, models, materials, bcs, precond, tol, 
     tm = {}
     t0 = time.perf_counter()
     if any(m.local_weight < 0.999 for m in models):
-        find_neighbours(mesh, _radii(models))
+        _search(mesh, models)
     tm["neighbours"] = time.perf_counter() - t0
     f, xD, inner = boundary_data(mesh, dof, bcs, values)
     neumann = physics == 0 and is_neumann(bcs)
@@ -792,7 +809,7 @@ class nonstationary_heat_equation_solver_2d:
         """init_dist: callable (x, y) on arrays of node coordinates, or an array of N temperatures."""
         mesh = self.mesh
         if any(m.local_weight < 0.999 for m in models):
-            find_neighbours(mesh, _radii(models))
+            _search(mesh, models)
         _, _, inner = boundary_data(mesh, 1, bcs, values)
         self._inner = inner
         K = _assemble(mesh, 0, models, materials, inner)

@@ -67,7 +67,7 @@ class BatchSolver:
         self.mesh = api.mesh_2d(mesh, ctx)
         radii = api._radii(models)
         if any(m.local_weight < 0.999 for m in models):
-            api.find_neighbours(self.mesh, radii)
+            api.find_neighbours(self.mesh, radii, distances=api._distances(models))
         rows = mesh.n_nodes
         f = np.zeros(rows)
         xD = np.zeros(rows)

@@ -40,7 +40,6 @@ GroupParams model_group_params(const nlf_model& md, bool only_local) {
     if (gp.nonlocal) {
         nlf_model m2 = md;
         m2.radius[0] = r0; m2.radius[1] = r1;
-        NLF_CHECK(md.distance_n == 2 || md.distance_n == 0, NLF_ERR_UNSUPPORTED, "only the L2 distance is implemented");
         gp.inf = make_influence(m2);
     }
     return gp;
@@ -89,36 +88,83 @@ int influence_class(const std::vector<GroupParams>& gp) {
         if (!g.nonlocal) continue;
         const InfluenceParams& f = g.inf;
         int cls = 0;
-        if (f.kind == NLF_INFLUENCE_POLYNOMIAL && (f.fast || (f.p_int && f.p_i == 1 && f.q_int && f.q_i <= 2)))
+        if (f.dist.kind == DIST_L2 && f.kind == NLF_INFLUENCE_POLYNOMIAL &&
+            (f.fast || (f.p_int && f.p_i == 1 && f.q_int && f.q_i <= 2)))
             cls = f.fast ? 1 : f.q_i;
         poly = poly < 0 ? cls : (poly == cls ? poly : 0);
     }
     return poly;
 }
 
+DistanceParams make_distance(int kind, int n, double n_real) {
+    DistanceParams d;
+    if (kind == NLF_DISTANCE_ELLIPSE_WITH_ROTATION) {      // n is fixed at 2 (search_function.hpp:77)
+        d.kind = DIST_ROTATION;
+        return d;
+    }
+    NLF_CHECK(kind == NLF_DISTANCE_LP, NLF_ERR_ARGUMENT, "unknown distance kind");
+    // read_influence_parameter (read_model.hpp:97-114): n > 0, a number or infinity
+    NLF_CHECK(n_real >= 0.0 && std::isfinite(n_real), NLF_ERR_ARGUMENT, "the distance exponent shall be greater than 0");
+    if (n_real > 0.0) {
+        d.kind = DIST_LN_REAL;
+        d.n_real = n_real;
+        return d;
+    }
+    NLF_CHECK(n >= 0 || n == NLF_DISTANCE_N_INFINITY, NLF_ERR_ARGUMENT, "the distance exponent shall be greater than 0");
+    if (n == 0 || n == 2) d.kind = DIST_L2;
+    else if (n == 1) d.kind = DIST_L1;
+    else if (n == NLF_DISTANCE_N_INFINITY) d.kind = DIST_LINF;
+    else { d.kind = DIST_LN_INT; d.n = n; }
+    return d;
+}
+
+// The influence function of a model, selected like read_influence_2d (read_model.hpp:144-165): a
+// polynomial with n = p = 2 (integer n) and q = 1 is the fast polynomial.  Normalisations follow
+// influence_functions_2d.hpp:44-48 (constant), 70-75 (polynomial), 103-109 (exponential), with n as
+// get_value(n) reads it (Infinity<size_t> as a double in the constant's area); the exponent p/n is
+// _influence_function_2d::division (:8-17): p itself for the maximum norm, an integer when p and n are
+// integers and n divides p, p / n otherwise.  A p or q with an integral value counts as an integer.
 InfluenceParams make_influence(const nlf_model& md) {
     InfluenceParams f;
     f.kind = md.influence;
     f.r[0] = md.radius[0]; f.r[1] = md.radius[1];
     f.inv_r[0] = 1.0 / f.r[0]; f.inv_r[1] = 1.0 / f.r[1];
-    const double n = 2.0, p = md.p, q = md.q;
+    f.dist = make_distance(md.distance, md.distance_n, md.distance_real);
+    const int dk = f.dist.kind;
+    const bool infinity = dk == DIST_LINF;
+    const bool real_n = dk == DIST_LN_REAL;
+    const long n_i = dk == DIST_L1 ? 1 : dk == DIST_LN_INT ? f.dist.n : 2;
+    const double n = real_n ? f.dist.n_real : infinity ? 18446744073709551615.0 : double(n_i);
+    const double p = md.p, q = md.q;
     auto integral = [](double v) { return v >= 0 && std::floor(v) == v && v < 1e9; };
-    if (f.kind == NLF_INFLUENCE_CONSTANT) {              // influence_functions_2d.hpp:44-48
+    auto divide = [&](double v, int& is_int, int& vi, double& vd) {       // division(p, n)
+        if (infinity) {
+            is_int = integral(v);
+            if (is_int) vi = int(v); else vd = v;
+        } else if (!real_n && integral(v) && long(v) % n_i == 0) {
+            is_int = 1; vi = int(long(v) / n_i);
+        } else {
+            is_int = 0; vd = v / n;
+        }
+    };
+    if (f.kind == NLF_INFLUENCE_CONSTANT) {
         f.norm = std::tgamma(1 + 2 / n) / (4 * f.r[0] * f.r[1] * std::pow(std::tgamma(1 + 1 / n), 2));
     } else if (f.kind == NLF_INFLUENCE_POLYNOMIAL) {
-        if (p == 2.0 && q == 1.0) {                      // fast_polynomial (read_model.hpp:160-161)
-            f.fast = 1;
+        if (!real_n && n_i == 2 && !infinity && p == 2.0 && q == 1.0) {
+            f.fast = 1;                                  // fast_polynomial (:127-148)
             f.norm = 2.0 / (M_PI * f.r[0] * f.r[1]);
-        } else {                                         // polynomial :70-75, exponents :215-216
-            if (integral(p) && long(p) % 2 == 0) { f.p_int = 1; f.p_i = int(p / 2); } else { f.p_int = 0; f.p_d = p / n; }
+        } else {
+            divide(p, f.p_int, f.p_i, f.p_d);
             if (integral(q)) { f.q_int = 1; f.q_i = int(q); } else { f.q_int = 0; f.q_d = q; }
-            f.norm = n * p / (4 * f.r[0] * f.r[1] * std::beta(1 / n, 1 / n) * std::beta(2 / p, q + 1));
+            f.norm = infinity ? p / (8 * f.r[0] * f.r[1] * std::beta(2 / p, q + 1))
+                              : n * p / (4 * f.r[0] * f.r[1] * std::beta(1 / n, 1 / n) * std::beta(2 / p, q + 1));
         }
-    } else if (f.kind == NLF_INFLUENCE_EXPONENTIAL) {    // :234-251
-        if (integral(p) && long(p) % 2 == 0) { f.p_int = 1; f.p_i = int(p / 2); } else { f.p_int = 0; f.p_d = p / n; }
+    } else if (f.kind == NLF_INFLUENCE_EXPONENTIAL) {
+        divide(p, f.p_int, f.p_i, f.p_d);
         f.exp_q = -q;
-        f.norm = (n * p * std::pow(4.0, 1 / n) * std::pow(q, 2 / p)) /
-                 (8 * f.r[0] * f.r[1] * std::tgamma(2 / p) * std::beta(0.5, 1 / n));
+        f.norm = infinity ? p * std::pow(q, 2 / p) / (8 * f.r[0] * f.r[1] * std::tgamma(2 / p))
+                          : (n * p * std::pow(4.0, 1 / n) * std::pow(q, 2 / p)) /
+                                (8 * f.r[0] * f.r[1] * std::tgamma(2 / p) * std::beta(0.5, 1 / n));
     } else {
         throw Error(NLF_ERR_ARGUMENT, "unknown influence function");
     }
@@ -989,6 +1035,79 @@ __global__ void k_mirror(i64 rows, const i64* __restrict__ outer, const i32* __r
     }
 }
 
+// ---- symmetric pattern from the upper rows (neighbour lists that are not a symmetric relation) ----
+// With the rotated-ellipse distance e' in N(e) does not imply e in N(e'), so the columns a row reaches
+// left of its diagonal need not be the transpose of the upper rows.  The reference keeps only the
+// upper triangle of each row's own traversal (matrix_separator_base.hpp:50-58) and reads the matrix as
+// symmetric; here the lower part of every row is rebuilt as the transpose of the upper rows: the
+// (col, row) keys of the entries right of the diagonal are sorted, which lists each row's lower
+// columns in ascending order.
+__global__ void k_lower_keys(i64 rows, const i64* __restrict__ outer, const i32* __restrict__ cols,
+                             const i64* __restrict__ diag, unsigned long long* __restrict__ keys,
+                             i64* __restrict__ lower_cnt) {
+    const i64 r = i64(blockIdx.x) * (blockDim.x / 64) + (threadIdx.x >> 6);
+    if (r >= rows) return;
+    const int lane = threadIdx.x & 63;
+    const i64 dr = diag[r];
+    for (i64 p = outer[r] + lane; p < outer[r + 1]; p += 64) {
+        unsigned long long key = ~0ull;
+        if (p > dr) {
+            const i32 c = cols[p];
+            key = (static_cast<unsigned long long>(c) << 32) | static_cast<unsigned long long>(r);
+            atomicAdd(reinterpret_cast<unsigned long long*>(&lower_cnt[c]), 1ull);
+        }
+        keys[p] = key;
+    }
+}
+
+__global__ void k_sym_rows(i64 rows, const i64* __restrict__ outer, const i64* __restrict__ diag,
+                           const i64* __restrict__ lower_cnt, i64* __restrict__ len) {
+    const i64 r = i64(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (r < rows) len[r] = lower_cnt[r] + (outer[r + 1] - diag[r]);
+}
+
+__global__ void k_sym_fill(i64 rows, const i64* __restrict__ outer, const i32* __restrict__ cols,
+                           const i64* __restrict__ diag, const i64* __restrict__ lower_off,
+                           const unsigned long long* __restrict__ sorted, const i64* __restrict__ outer2,
+                           i32* __restrict__ cols2, i64* __restrict__ diag2) {
+    const i64 r = i64(blockIdx.x) * (blockDim.x / 64) + (threadIdx.x >> 6);
+    if (r >= rows) return;
+    const int lane = threadIdx.x & 63;
+    const i64 lb = lower_off[r], nl = lower_off[r + 1] - lb, o2 = outer2[r];
+    for (i64 k = lane; k < nl; k += 64) cols2[o2 + k] = i32(sorted[lb + k] & 0xffffffffull);
+    const i64 ub = diag[r], nu = outer[r + 1] - ub;
+    for (i64 k = lane; k < nu; k += 64) cols2[o2 + nl + k] = cols[ub + k];
+    if (lane == 0) diag2[r] = o2 + nl;
+}
+
+void symmetrize_pattern(nlf_matrix_s* mat, hipStream_t s) {
+    const i64 rows = mat->rows, nnz = mat->nnz_full;
+    DevBuf<unsigned long long> keys{size_t(std::max<i64>(nnz, 1))}, sorted{size_t(std::max<i64>(nnz, 1))};
+    DevBuf<i64> lower_off{size_t(rows + 1)}, len{size_t(rows + 1)};
+    NLF_HIP(hipMemsetAsync(lower_off.p, 0, lower_off.bytes(), s));
+    hipLaunchKernelGGL(k_lower_keys, dim3(blocks_for(rows, 4)), dim3(256), 0, s, rows, mat->outer.p, mat->cols.p,
+                       mat->diag.p, keys.p, lower_off.p);
+    NLF_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_sym_rows, dim3(blocks_for(rows, 256)), dim3(256), 0, s, rows, mat->outer.p, mat->diag.p,
+                       lower_off.p, len.p);
+    NLF_HIP(hipGetLastError());
+    exclusive_scan_i64(lower_off.p, rows, s);
+    exclusive_scan_i64(len.p, rows, s);
+    sort_keys_u64(keys.p, sorted.p, nnz, s);
+    i64 nnz2 = 0;
+    download(&nnz2, len.p + rows, 1, s);
+    DevBuf<i32> cols2{size_t(std::max<i64>(nnz2, 1))};
+    DevBuf<i64> diag2{size_t(rows)};
+    hipLaunchKernelGGL(k_sym_fill, dim3(blocks_for(rows, 4)), dim3(256), 0, s, rows, mat->outer.p, mat->cols.p,
+                       mat->diag.p, lower_off.p, sorted.p, len.p, cols2.p, diag2.p);
+    NLF_HIP(hipGetLastError());
+    NLF_HIP(hipStreamSynchronize(s));
+    mat->outer = std::move(len);
+    mat->cols = std::move(cols2);
+    mat->diag = std::move(diag2);
+    mat->nnz_full = nnz2;
+}
+
 // Largest |col - row| of a sorted-row pattern (rows are ascending: first and last entries decide).
 __global__ void k_bandwidth(i64 rows, const i64* __restrict__ outer, const i32* __restrict__ cols, unsigned* __restrict__ mx) {
     const i64 r = i64(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -1726,6 +1845,7 @@ int nlf_assemble_ex(nlf_mesh m, int physics, const nlf_model* models, const void
         hipLaunchKernelGGL(k_pattern<1>, dim3(unsigned(m->N)), dim3(pth), lds_bitmap, s, pa);
         NLF_HIP(hipGetLastError());
         NLF_HIP(hipStreamSynchronize(s));
+        if (any_nonlocal && !only_local && !m->nb_symmetric) symmetrize_pattern(mat.get(), s);
         const auto t1 = std::chrono::steady_clock::now();
         mat->seconds_pattern = std::chrono::duration<double>(t1 - t0).count();
         // ---- values ----

@@ -95,6 +95,20 @@ FeTables make_fe_tables(int nn);
 BoundaryTables make_boundary_tables(int nb);
 
 // ---- influence function (device evaluation parameters) ---------------------------------------
+// Distance kinds: the Euclidean one (n = 2, the common case, with its own fast path) and the others
+// of powered_distance (search_function.hpp:28-73) and powered_distance_with_rotation (:75-90).
+enum : int { DIST_L2 = 0, DIST_L1 = 1, DIST_LINF = 2, DIST_LN_INT = 3, DIST_LN_REAL = 4, DIST_ROTATION = 5 };
+
+struct DistanceParams {
+    int kind = DIST_L2;
+    int n = 2;                           // DIST_LN_INT exponent
+    double n_real = 2.0;                 // DIST_LN_REAL exponent
+    bool symmetric() const { return kind != DIST_ROTATION; }
+};
+
+// Validated distance of an nlf_distance / the distance fields of an nlf_model.
+DistanceParams make_distance(int kind, int n, double n_real);
+
 struct InfluenceParams {
     int kind = NLF_INFLUENCE_POLYNOMIAL;
     int fast = 0;                        // fast_polynomial: norm*(1-d)
@@ -105,11 +119,12 @@ struct InfluenceParams {
     double r[2] = {0, 0};
     double inv_r[2] = {0, 0};
     double norm = 1.0;
+    DistanceParams dist;
 };
 
 InfluenceParams make_influence(const nlf_model& m);
 
-__device__ __forceinline__ double ipow(double x, int e) {     // power.hpp:132-147 (integral exponent)
+__device__ __forceinline__ double ipow(double x, int e) {     // metamath/functions/power.hpp:25-40 (integral exponent)
     double r = 1.0;
     while (e > 0) {
         if (e & 1) { r *= x; --e; } else { x *= x; e >>= 1; }
@@ -122,14 +137,56 @@ __device__ __forceinline__ double exp_pow(double v, int is_int, int ei, double e
     return ei == 1 ? v : ipow(v, ei);
 }
 
+// powered_distance(x, y, r) for the kinds other than DIST_L2, operation for operation like the
+// reference (search_function.hpp:28-90; metamath distance.hpp:17-31, norm.hpp:15-43, power.hpp:9-46):
+// z_i = (x_i - y_i) / r_i, then n = 1: (0 + |z0|) + |z1|; the maximum norm: the larger |z_i| (a norm,
+// not powered); integer n: (0 + |z0|^n) + |z1|^n by repeated squaring; real n: the same with pow.
+// Rotation (an ellipse whose r0 axis points along x from the origin and whose r1 axis is
+// perpendicular to it): (r0 (x0 y1 - x1 y0))^2 + (r1 (x0 (x0 - y0) + x1 (x1 - y1)))^2 over
+// (r0 r1)^2 |x|^2 -- not symmetric in x and y, and NaN at x = 0, like the reference.
+__device__ __forceinline__ double general_distance(const DistanceParams& d, double r0, double r1, double ax, double ay,
+                                                   double bx, double by) {
+    if (d.kind == DIST_ROTATION) {
+        const double u = r0 * (ax * by - ay * bx);
+        const double v = r1 * (ax * (ax - bx) + ay * (ay - by));
+        const double rr = r0 * r1;
+        double xx = 0.0;
+        xx = xx + ax * ax;
+        xx = xx + ay * ay;
+        return (u * u + v * v) / (rr * rr * xx);
+    }
+    const double z0 = fabs((ax - bx) / r0), z1 = fabs((ay - by) / r1);
+    if (d.kind == DIST_LINF) return z0 < z1 ? z1 : z0;
+    double s = 0.0;
+    if (d.kind == DIST_L1) {
+        s = s + z0;
+        s = s + z1;
+    } else if (d.kind == DIST_LN_INT) {
+        s = s + ipow(z0, d.n);
+        s = s + ipow(z1, d.n);
+    } else if (d.kind == DIST_LN_REAL) {
+        s = s + pow(z0, d.n_real);
+        s = s + pow(z1, d.n_real);
+    } else {
+        s = s + z0 * z0;
+        s = s + z1 * z1;
+    }
+    return s;
+}
+
 // phi(x, y): normalised influence of the nonlocal radius (influence_functions_2d.hpp:39-148).
 __device__ __forceinline__ double influence(const InfluenceParams& f, double ax, double ay, double bx, double by) {
-    const double dx = ax - bx, dy = ay - by;
-    const double zx = dx * f.inv_r[0], zy = dy * f.inv_r[1];
-    double d = zx * zx + zy * zy;
-    if (fabs(d - 1.0) < 1e-12) {      // near the cutoff: the reference's division form decides
-        const double ex = dx / f.r[0], ey = dy / f.r[1];
-        d = ex * ex + ey * ey;
+    double d;
+    if (f.dist.kind == DIST_L2) {
+        const double dx = ax - bx, dy = ay - by;
+        const double zx = dx * f.inv_r[0], zy = dy * f.inv_r[1];
+        d = zx * zx + zy * zy;
+        if (fabs(d - 1.0) < 1e-12) {      // near the cutoff: the reference's division form decides
+            const double ex = dx / f.r[0], ey = dy / f.r[1];
+            d = ex * ex + ey * ey;
+        }
+    } else {
+        d = general_distance(f.dist, f.r[0], f.r[1], ax, ay, bx, by);
     }
     if (f.fast) return d < 1.0 ? f.norm * (1.0 - d) : 0.0;
     if (f.kind == NLF_INFLUENCE_POLYNOMIAL)
@@ -169,6 +226,7 @@ __device__ __forceinline__ float exp_pow32(float v, int is_int, int ei, double e
 }
 
 __device__ __forceinline__ float influence32(const InfluenceParams& f, double ax, double ay, double bx, double by) {
+    if (f.dist.kind != DIST_L2) return float(influence(f, ax, ay, bx, by));
     const float zx = float(ax - bx) * float(f.inv_r[0]), zy = float(ay - by) * float(f.inv_r[1]);
     const float d = zx * zx + zy * zy;
     if (fabsf(d - 1.0f) < 4e-6f) return float(influence(f, ax, ay, bx, by));
@@ -251,6 +309,7 @@ struct nlf_mesh_s {
     nlf::DevBuf<nlf::i32> nb_ids;
     std::vector<double> expanded;        // 2G
     std::vector<int> searched;           // group searched (radius > 0)
+    bool nb_symmetric = true;            // e' in N(e) <=> e in N(e') (false after a rotated-distance search)
     double last_neighbour_seconds = 0.0;
 };
 
@@ -330,6 +389,7 @@ void exclusive_scan_i64(i64* data, i64 n, hipStream_t s);   // in place over n+1
 int noalloc_scratch_init(int device);   // device address of nlf_cg_solve_noalloc's work memory, per device
 // (keys, values) sorted by key, stable (radix sort); n pairs
 void sort_pairs_u64(const uint64_t* keys_in, uint64_t* keys_out, const i32* vals_in, i32* vals_out, i64 n, hipStream_t s);
+void sort_keys_u64(const unsigned long long* keys_in, unsigned long long* keys_out, i64 n, hipStream_t s);
 void set_error(nlf_ctx ctx, const std::string& msg);
 
 // Runs f, translating exceptions into status codes and the context's last error.

@@ -89,7 +89,9 @@ __global__ void k_lf_features(i64 n, const i64* __restrict__ lo, const i32* __re
     if (u >= n) return;
     const GroupParams gu = groups[node_group(u, ne_off, ne_ids, group_of)];
     const double xu = xy[2 * u], yu = xy[2 * u + 1];
-    const double phi0u = gu.nonlocal ? influence(gu.inf, xu, yu, xu, yu) : 1.0;
+    // phi(x, x) is the influence's norm for every kind and distance (the rotated distance is NaN at the
+    // origin, so it is not evaluated); a NaN weight there counts as no influence
+    const double phi0u = gu.nonlocal ? gu.inf.norm : 1.0;
     for (i64 p = lo[u]; p < lo[u + 1]; ++p) {
         const i32 v = lc[p];
         // K(u, v) by binary search in K's row u
@@ -101,9 +103,11 @@ __global__ void k_lf_features(i64 n, const i64* __restrict__ lo, const i32* __re
         const double kuv = (b < ko[u + 1] && kc[b] == v) ? kv[b] : 0.0;
         const GroupParams gv = groups[node_group(v, ne_off, ne_ids, group_of)];
         const double xv = xy[2 * v], yv = xy[2 * v + 1];
-        const double wu = gu.nonlocal ? (1.0 - gu.p1) * influence(gu.inf, xu, yu, xv, yv) / phi0u : 0.0;
-        const double phi0v = gv.nonlocal ? influence(gv.inf, xv, yv, xv, yv) : 1.0;
-        const double wv = gv.nonlocal ? (1.0 - gv.p1) * influence(gv.inf, xv, yv, xu, yu) / phi0v : 0.0;
+        double wu = gu.nonlocal ? (1.0 - gu.p1) * influence(gu.inf, xu, yu, xv, yv) / phi0u : 0.0;
+        const double phi0v = gv.nonlocal ? gv.inf.norm : 1.0;
+        double wv = gv.nonlocal ? (1.0 - gv.p1) * influence(gv.inf, xv, yv, xu, yu) / phi0v : 0.0;
+        wu = isfinite(wu) ? wu : 0.0;
+        wv = isfinite(wv) ? wv : 0.0;
         const double ss = double(s[u]) * double(s[v]);
         feat[4 * p + 0] = float(lv[p] * ss);
         feat[4 * p + 1] = float(kuv * ss);

@@ -7,10 +7,14 @@
 // Layout: cells of every searched group are numbered consecutively; (cell << 32 | element) keys are
 // radix-sorted once, which yields every cell's element list in ascending id order.  A wave (64
 // lanes) handles one element and tests 64 candidates per step; ballot + popcount give the ordered
-// write position.  The predicate is evaluated exactly as powered_distance<2>(x, y, r) <= 1
+// write position.  The Euclidean predicate is evaluated exactly as powered_distance<2>(x, y, r) <= 1
 // (distance.hpp:17-23, norm.hpp:15-22): z_i = (a_i - b_i) / r_i, s = (0 + z0*z0) + z1*z1.  A
 // reciprocal-multiply estimate decides candidates whose value is more than 1e-12 away from 1
 // (the estimate differs from the exact value by < 2e-15 relative), the rest take the division path.
+// The other distances (Lp with n != 2, the maximum norm, the rotated ellipse) go through
+// general_distance, operation for operation like the reference; the 3x3 cell window is the
+// reference's for every distance (each keeps |z_i| <= 1 inside its unit ball, so a cell side of
+// max(r0, r1) covers it).
 #include "nlf_internal.hpp"
 
 #include <rocprim/rocprim.hpp>
@@ -39,6 +43,15 @@ void sort_pairs_u64(const uint64_t* keys_in, uint64_t* keys_out, const i32* vals
     NLF_HIP(hipStreamSynchronize(s));          // before the temporary storage is released
 }
 
+void sort_keys_u64(const unsigned long long* keys_in, unsigned long long* keys_out, i64 n, hipStream_t s) {
+    if (n <= 0) return;
+    size_t bytes = 0;
+    NLF_HIP(rocprim::radix_sort_keys(nullptr, bytes, keys_in, keys_out, size_t(n), 0, 64, s));
+    DevBuf<uint8_t> tmp{bytes + 16};
+    NLF_HIP(rocprim::radix_sort_keys(tmp.p, bytes, keys_in, keys_out, size_t(n), 0, 64, s));
+    NLF_HIP(hipStreamSynchronize(s));          // before the temporary storage is released
+}
+
 struct SearchGroup {
     double r[2];        // expanded radii
     double inv_r[2];
@@ -48,6 +61,7 @@ struct SearchGroup {
     i64 base;           // first global cell id
     i64 begin, end;     // element range
     int searched;
+    DistanceParams dist;
 };
 
 __device__ __forceinline__ unsigned long long ordered_bits(double v) {
@@ -261,6 +275,7 @@ __global__ void k_cell_keys(i64 E, const i32* __restrict__ group_of, const Searc
 }
 
 __device__ __forceinline__ bool inside(const SearchGroup& sg, double ax, double ay, double bx, double by) {
+    if (sg.dist.kind != DIST_L2) return general_distance(sg.dist, sg.r[0], sg.r[1], ax, ay, bx, by) <= 1.0;
     const double dx = ax - bx, dy = ay - by;
     const double ex = dx * sg.inv_r[0], ey = dy * sg.inv_r[1];
     double est = 0.0;
@@ -327,12 +342,22 @@ extern "C" {
 
 // find_neighbours (find_neighbours.hpp:119-160) + mesh_2d::neighbours (mesh_2d.hpp:205-212).
 // search_radii[2G]: per-group search radius (search_radius, else nonlocal_radius; read_model.hpp:226-244);
-// a group with a radius <= 0 is not searched and its elements get empty lists.  expanded_out[2G]
-// (optional) receives the expanded radii.  Strategies NO, MAX, MIN and MEAN.
+// a group with a radius <= 0 is not searched and its elements get empty lists.  distances[G] (null:
+// Euclidean) the distance function of each group's influence.  expanded_out[2G] (optional) receives
+// the expanded radii.  Strategies NO, MAX, MIN and MEAN.
 int nlf_find_neighbours(nlf_mesh m, const double* search_radii, int strategy, double* expanded_out) {
+    return nlf_find_neighbours_ex(m, search_radii, nullptr, strategy, expanded_out);
+}
+
+int nlf_find_neighbours_ex(nlf_mesh m, const double* search_radii, const nlf_distance* distances, int strategy,
+                           double* expanded_out) {
     if (!m) return NLF_ERR_ARGUMENT;
     return guarded(m->ctx, [&] {
         NLF_CHECK(search_radii, NLF_ERR_ARGUMENT, "null radii");
+        std::vector<DistanceParams> dist(size_t(m->G));
+        if (distances)
+            for (int g = 0; g < m->G; ++g)
+                dist[size_t(g)] = make_distance(distances[g].kind, distances[g].n, distances[g].n_real);
         NLF_CHECK(strategy == NLF_EXPAND_NO || strategy == NLF_EXPAND_MAX || strategy == NLF_EXPAND_MIN ||
                       strategy == NLF_EXPAND_MEAN, NLF_ERR_ARGUMENT, "unknown diameter expanding strategy");
         const auto t0 = std::chrono::steady_clock::now();
@@ -398,6 +423,7 @@ int nlf_find_neighbours(nlf_mesh m, const double* search_radii, int strategy, do
         download(box.data(), d_box.p, box.size(), s);
         std::vector<SearchGroup> sg(G);
         m->expanded.assign(2 * G, 0.0);
+        bool nb_symmetric = true;
         i64 cells = 0;
         for (int g = 0; g < G; ++g) {
             SearchGroup& q = sg[g];
@@ -406,7 +432,9 @@ int nlf_find_neighbours(nlf_mesh m, const double* search_radii, int strategy, do
             q.r[0] = search_radii[2 * g] + add[g];
             q.r[1] = search_radii[2 * g + 1] + add[g];
             q.searched = searched[g] && q.r[0] > 0.0 && q.r[1] > 0.0;
+            q.dist = dist[size_t(g)];
             if (!q.searched) continue;
+            nb_symmetric = nb_symmetric && q.dist.symmetric();
             m->expanded[2 * g] = q.r[0]; m->expanded[2 * g + 1] = q.r[1];
             q.inv_r[0] = 1.0 / q.r[0]; q.inv_r[1] = 1.0 / q.r[1];
             q.R = std::max(q.r[0], q.r[1]);
@@ -456,6 +484,7 @@ int nlf_find_neighbours(nlf_mesh m, const double* search_radii, int strategy, do
         NLF_HIP(hipGetLastError());
         NLF_HIP(hipStreamSynchronize(s));
         m->has_neighbours = true;
+        m->nb_symmetric = nb_symmetric;
         if (expanded_out) std::copy(m->expanded.begin(), m->expanded.end(), expanded_out);
         m->last_neighbour_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     });

@@ -13,7 +13,8 @@
 // from the reference interface, all forced by the boundary:
 //   * T = double, indices int64_t (as the reference's CLI and tests use);
 //   * influence functions are the families the GPU evaluates (solver_2d::influence::constant,
-//     polynomial, exponential, fast_polynomial) instead of arbitrary callables, with the L2 distance;
+//     polynomial, exponential, fast_polynomial) instead of arbitrary callables, and a distance is one
+//     of the two the reference defines (mesh::powered_distance with any n, the rotated ellipse);
 //   * neighbours_t holds the lists on the GPU: find_neighbours runs there and mesh_2d::neighbours
 //     adopts its result; a list is copied to the host only when read (neighbours(e));
 //   * coefficients are constants or functions of the point (solution-dependent ones, and
@@ -137,13 +138,116 @@ template<class T>
 using evaluated_parameters = std::variant<T, std::vector<T>>;
 inline constexpr size_t Constant = 0, Variable = 1;
 
+// size_t_or (metamath/types/size_t_or.hpp:7-17): an exponent held as an integer or a real;
+// Infinity (metamath/utils/constants.hpp) is the largest size_t and selects the maximum norm.
+namespace metamath {
+namespace types {
+template<class T>
+using size_t_or = std::variant<size_t, T>;
+template<class T>
+T get_value(const size_t_or<T>& value) {
+    return std::holds_alternative<size_t>(value) ? T(std::get<size_t>(value)) : std::get<T>(value);
+}
+}  // namespace types
+namespace constants {
+template<class T>
+inline constexpr T Infinity = std::numeric_limits<T>::max();
+}  // namespace constants
+}  // namespace metamath
+
+namespace mesh {
+
+// powered_distance (search_function.hpp:28-73): the Lp distance sum |(x_i - y_i) / r_i|^n with n an
+// integer, a real or Infinity (then the maximum norm, not powered).  operator() evaluates it on the
+// host as the reference does; the GPU evaluates the same operations in the search and the influences.
+template<class T>
+struct powered_distance {
+    metamath::types::size_t_or<T> n = size_t{2};
+
+    explicit powered_distance() noexcept = default;
+    explicit powered_distance(const metamath::types::size_t_or<T> parameter) : n{parameter} {
+        if (!(metamath::types::get_value(n) > T{0})) throw std::domain_error{"the distance exponent shall be greater than 0"};
+    }
+
+    T operator()(const std::array<T, 2>& x, const std::array<T, 2>& y, const std::array<T, 2>& r) const {
+        const std::array<T, 2> z = {(x[0] - y[0]) / r[0], (x[1] - y[1]) / r[1]};
+        if (std::holds_alternative<size_t>(n) && std::get<size_t>(n) == metamath::constants::Infinity<size_t>)
+            return std::max(std::abs(z[0]), std::abs(z[1]));
+        T sum = T{0};
+        for (const T v : z) {
+            if (std::holds_alternative<T>(n)) sum = sum + std::pow(std::abs(v), std::get<T>(n));
+            else if (std::get<size_t>(n) == 2) sum = sum + v * v;
+            else {
+                T base = std::abs(v), result = T{1};              // power by squaring (power.hpp:25-40)
+                for (size_t e = std::get<size_t>(n); e > 0;)
+                    if (e & 1) { result *= base; --e; } else { base *= base; e >>= 1; }
+                sum = sum + result;
+            }
+        }
+        return sum;
+    }
+};
+
+// powered_distance_with_rotation (search_function.hpp:75-90): an ellipse whose r0 axis points along
+// x from the origin and whose r1 axis is perpendicular to it; n is 2.
+template<class T>
+struct powered_distance_with_rotation {
+    static constexpr metamath::types::size_t_or<T> n = size_t{2};
+
+    explicit powered_distance_with_rotation(const metamath::types::size_t_or<T>& = {}) noexcept {}
+
+    T operator()(const std::array<T, 2>& x, const std::array<T, 2>& y, const std::array<T, 2>& r) const {
+        const auto square = [](const T v) { return v * v; };
+        T norm_x = T{0};
+        norm_x = norm_x + square(x[0]);
+        norm_x = norm_x + square(x[1]);
+        return (square(r[0] * (x[0] * y[1] - x[1] * y[0])) + square(r[1] * (x[0] * (x[0] - y[0]) + x[1] * (x[1] - y[1])))) /
+               (square(r[0] * r[1]) * norm_x);
+    }
+};
+
+// distance_function (search_function.hpp:13-14): the reference's std::function, here one of the two
+// distances the GPU evaluates
+template<class T>
+using distance_function = std::variant<powered_distance<T>, powered_distance_with_rotation<T>>;
+
+}  // namespace mesh
+
+namespace detail {
+
+// (kind, n, n_real) of the C ABI for a distance
+template<class T>
+nlf_distance to_nlf(const mesh::distance_function<T>& distance) {
+    nlf_distance d{NLF_DISTANCE_LP, 2, 0.0};
+    if (std::holds_alternative<mesh::powered_distance_with_rotation<T>>(distance)) {
+        d.kind = NLF_DISTANCE_ELLIPSE_WITH_ROTATION;
+        return d;
+    }
+    const auto& n = std::get<mesh::powered_distance<T>>(distance).n;
+    if (std::holds_alternative<T>(n)) {
+        d.n = 0;
+        d.n_real = double(std::get<T>(n));
+    } else if (std::get<size_t>(n) == metamath::constants::Infinity<size_t>) {
+        d.n = NLF_DISTANCE_N_INFINITY;
+    } else {
+        if (std::get<size_t>(n) > size_t(std::numeric_limits<int32_t>::max()))
+            throw nlf_error(NLF_ERR_ARGUMENT, "integer distance exponent out of range");
+        d.n = int32_t(std::get<size_t>(n));
+    }
+    return d;
+}
+
+}  // namespace detail
+
 // The influence functions the GPU evaluates (influence_functions_2d.hpp:39-148), as a descriptor;
-// the reference stores a callable in model_parameters::influence.
+// the reference stores a callable in model_parameters::influence.  p and q with an integral value
+// act as the integer alternative of size_t_or.
 template<class T>
 struct influence_function_2d {
     int32_t kind = NLF_INFLUENCE_POLYNOMIAL;
     std::array<T, 2> radius = {T{0}, T{0}};
-    T n = T{2}, p = T{2}, q = T{1};
+    mesh::distance_function<T> distance;
+    T p = T{2}, q = T{1};
     bool nonlocal = false;                             // false: the default (zero) influence
 };
 
@@ -163,32 +267,37 @@ struct equation_parameters final {
 namespace solver_2d::influence {
 
 // constant (influence_functions_2d.hpp:39-59)
-template<class T>
+template<class T, class Distance = mesh::powered_distance<T>>
 struct constant : influence_function_2d<T> {
-    explicit constant(const std::array<T, 2>& radius, const T n = T{2}) {
-        this->kind = NLF_INFLUENCE_CONSTANT; this->radius = radius; this->n = n; this->nonlocal = true;
+    explicit constant(const std::array<T, 2>& radius, const metamath::types::size_t_or<T>& n = size_t{2}) {
+        this->kind = NLF_INFLUENCE_CONSTANT; this->radius = radius; this->distance = Distance{n};
+        this->nonlocal = true;
     }
 };
-// polynomial (:61-92): norm (1 - d^p)^q inside the ellipse
-template<class T>
+// polynomial (:61-92): norm (1 - d^(p/n))^q inside the unit ball of the distance
+template<class T, class Distance = mesh::powered_distance<T>>
 struct polynomial : influence_function_2d<T> {
-    explicit polynomial(const std::array<T, 2>& radius, const T n = T{2}, const T p = T{2}, const T q = T{1}) {
-        this->kind = NLF_INFLUENCE_POLYNOMIAL; this->radius = radius; this->n = n; this->p = p; this->q = q;
+    explicit polynomial(const std::array<T, 2>& radius, const metamath::types::size_t_or<T>& n = size_t{2},
+                        const metamath::types::size_t_or<T>& p = size_t{2}, const metamath::types::size_t_or<T>& q = size_t{1}) {
+        this->kind = NLF_INFLUENCE_POLYNOMIAL; this->radius = radius; this->distance = Distance{n};
+        this->p = metamath::types::get_value(p); this->q = metamath::types::get_value(q);
         this->nonlocal = true;
     }
 };
 // exponential (:94-125); the default parameters give the normal distribution
-template<class T>
+template<class T, class Distance = mesh::powered_distance<T>>
 struct exponential : influence_function_2d<T> {
-    explicit exponential(const std::array<T, 2>& radius, const T n = T{2}, const T p = T{2}, const T q = T{0.5}) {
-        this->kind = NLF_INFLUENCE_EXPONENTIAL; this->radius = radius; this->n = n; this->p = p; this->q = q;
+    explicit exponential(const std::array<T, 2>& radius, const metamath::types::size_t_or<T>& n = size_t{2},
+                         const metamath::types::size_t_or<T>& p = size_t{2}, const T q = T{0.5}) {
+        this->kind = NLF_INFLUENCE_EXPONENTIAL; this->radius = radius; this->distance = Distance{n};
+        this->p = metamath::types::get_value(p); this->q = q;
         this->nonlocal = true;
     }
 };
 // fast_polynomial (:127-148): the polynomial with n = 2, p = 2, q = 1
-template<class T>
-struct fast_polynomial : polynomial<T> {
-    explicit fast_polynomial(const std::array<T, 2>& radius) : polynomial<T>(radius) {}
+template<class T, class Distance = mesh::powered_distance<T>>
+struct fast_polynomial : polynomial<T, Distance> {
+    explicit fast_polynomial(const std::array<T, 2>& radius) : polynomial<T, Distance>(radius) {}
 };
 
 }  // namespace solver_2d::influence
@@ -199,8 +308,16 @@ template<size_t Dimension, class T>
 nlf_model to_nlf(const model_parameters<Dimension, T>& m) {
     const auto& f = m.influence;
     const bool nonlocal = f.nonlocal && m.local_weight < 0.999;
-    return nlf_model{nonlocal ? double(m.local_weight) : 1.0, {double(f.radius[0]), double(f.radius[1])}, {0.0, 0.0},
-                     f.kind, int32_t(std::lround(double(f.n))), double(f.p), double(f.q)};
+    const nlf_distance d = to_nlf(f.distance);
+    nlf_model out{};
+    out.local_weight = nonlocal ? double(m.local_weight) : 1.0;
+    out.radius[0] = double(f.radius[0]); out.radius[1] = double(f.radius[1]);
+    out.influence = f.kind;
+    out.distance_n = d.n;
+    out.p = double(f.p); out.q = double(f.q);
+    out.distance = d.kind;
+    out.distance_real = d.n_real;
+    return out;
 }
 
 template<size_t Dimension, class T>
@@ -395,15 +512,9 @@ public:
     const mesh_2d_data& data() const noexcept { return *d_; }
 };
 
-// powered_distance (search_function.hpp:28-40) with its exponent; the GPU search is the L2 one
-template<class T>
-struct powered_distance {
-    T n = T{2};
-};
-
 template<class T>
 struct influence final {                          // search_function.hpp:15-19
-    powered_distance<T> distance;
+    distance_function<T> distance;
     std::array<T, 2> radius = {T{0}, T{0}};
 };
 
@@ -526,20 +637,24 @@ neighbours_t<T, I> find_neighbours(const mesh_2d<T, I>& mesh, const influences<T
                                    const diameter_expanding_strategy strategy = diameter_expanding_strategy::MAX) {
     const size_t G = mesh.groups_count();
     std::vector<double> r(2 * G, 0.0), expanded(2 * G, 0.0);
+    std::vector<nlf_distance> distances(G, nlf_distance{NLF_DISTANCE_LP, 2, 0.0});
     for (size_t g = 0; g < G; ++g) {
         const auto it = influence.find(mesh.data().group_names[g]);
         if (it == influence.end()) continue;
-        if (std::abs(it->second.distance.n - T{2}) > 0)
-            throw nlf_error(NLF_ERR_UNSUPPORTED, "the GPU neighbour search uses the L2 distance (n = 2)");
+        distances[g] = ::nonlocal::detail::to_nlf(it->second.distance);
         r[2 * g] = it->second.radius[0];
         r[2 * g + 1] = it->second.radius[1];
     }
-    nlf_check(nlf_find_neighbours(mesh.handle(), r.data(), int(strategy), expanded.data()), mesh.ctx().handle());
+    nlf_check(nlf_find_neighbours_ex(mesh.handle(), r.data(), distances.data(), int(strategy), expanded.data()),
+              mesh.ctx().handle());
     auto& m = const_cast<mesh_2d<T, I>&>(mesh);       // the search's generation: the lists now on the GPU
     ++m.searches_;
     influences<T> out;
     for (size_t g = 0; g < G; ++g)
-        if (r[2 * g] > 0) out[mesh.data().group_names[g]] = mesh::influence<T>{{}, {expanded[2 * g], expanded[2 * g + 1]}};
+        if (r[2 * g] > 0) {
+            const auto& name = mesh.data().group_names[g];
+            out[name] = mesh::influence<T>{influence.at(name).distance, {expanded[2 * g], expanded[2 * g + 1]}};
+        }
     return {std::move(out), neighbours_list<I>(mesh.handle(), mesh.ctx().handle(), mesh.elements_count(), m.searches_)};
 }
 

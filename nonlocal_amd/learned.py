@@ -193,7 +193,7 @@ class TrainingProblem:
         self.problem = configs.composite(n, r=r, seed=seed, count=count)
         p = self.problem
         self.mesh = api.mesh_2d(p.mesh, ctx)
-        api.find_neighbours(self.mesh, [[m.radius[0], m.radius[1]] for m in p.models])
+        api.find_neighbours(self.mesh, [[m.radius[0], m.radius[1]] for m in p.models], distances=api._distances(p.models))
         rows = p.mesh.n_nodes
         f = np.zeros(rows)
         xD = np.zeros(rows)

@@ -278,6 +278,58 @@ double powered_distance2r(const double* a, const double* b, const double* r) {  
     return s;
 }
 
+double ipow(double x, i64 e);
+
+// powered_distance (search_function.hpp:28-73) with n held like metamath::types::size_t_or<T>
+// (an integer, Infinity<size_t> among them, or a real), and powered_distance_with_rotation (:75-90).
+// x: the point the distance is measured from (centre of eL / quadrature point of eL), y: the other,
+// r: the radii.
+struct Distance {
+    int kind = NLF_DISTANCE_LP;
+    bool real = false, infinity = false;
+    i64 n_int = 2;
+    double n_real = 0.0;
+
+    double operator()(const double* x, const double* y, const double* r) const {
+        if (kind == NLF_DISTANCE_ELLIPSE_WITH_ROTATION) {
+            const auto square = [](double v) { return v * v; };            // power<2> (power.hpp:9-23)
+            double norm_x = 0.0;                                            // powered_norm<2>(x) (norm.hpp:15-22)
+            norm_x = norm_x + square(x[0]);
+            norm_x = norm_x + square(x[1]);
+            return (square(r[0] * (x[0] * y[1] - x[1] * y[0])) + square(r[1] * (x[0] * (x[0] - y[0]) + x[1] * (x[1] - y[1])))) /
+                   (square(r[0] * r[1]) * norm_x);
+        }
+        if (!real && !infinity && n_int == 2) return powered_distance2r(x, y, r);   // calculate<2>
+        const double z[2] = {(x[0] - y[0]) / r[0], (x[1] - y[1]) / r[1]};   // distance.hpp:17-31
+        if (infinity) {                                                     // distance<Infinity> -> norm<Infinity> (norm.hpp:29-40)
+            const double* it = std::max_element(z, z + 2, [](double a, double b) { return std::abs(a) < std::abs(b); });
+            return std::abs(*it);
+        }
+        double sum = 0.0;                                                   // powered_norm (norm.hpp:15-29)
+        for (const double v : z) {
+            if (real) sum = sum + std::pow(std::abs(v), n_real);            // power(x, floating exp) = std::pow
+            else if (n_int == 1) sum = sum + std::abs(v);                   // power<1>(|x|)
+            else sum = sum + ipow(std::abs(v), n_int);                      // power(|x|, integral exp)
+        }
+        return sum;
+    }
+};
+
+// read_distance_2d / read_influence_parameter (read_model.hpp:97-131): n > 0 or infinity; the
+// rotated ellipse ignores n.
+Distance make_distance(int kind, int n, double n_real) {
+    Distance d;
+    d.kind = kind;
+    if (kind == NLF_DISTANCE_ELLIPSE_WITH_ROTATION) return d;
+    if (kind != NLF_DISTANCE_LP) throw std::domain_error("unknown distance");
+    if (!(n_real >= 0.0) || !std::isfinite(n_real)) throw std::domain_error("distance exponent");
+    if (n_real > 0.0) { d.real = true; d.n_real = n_real; return d; }
+    if (n == NLF_DISTANCE_N_INFINITY) { d.infinity = true; return d; }
+    if (n < 0) throw std::domain_error("distance exponent");
+    d.n_int = n == 0 ? 2 : n;
+    return d;
+}
+
 double expansion(const Mesh& m, int g, int strategy) {                  // find_neighbours.hpp:49-72
     if (strategy == NLF_EXPAND_NO) return 0.0;
     double diameter = strategy == NLF_EXPAND_MIN ? std::numeric_limits<double>::max() : 0.0;
@@ -296,7 +348,7 @@ double expansion(const Mesh& m, int g, int strategy) {                  // find_
     return strategy == NLF_EXPAND_MEAN ? diameter / double(pairs.size()) : std::sqrt(diameter);
 }
 
-i64 find_neighbours(Mesh& m, const double* radii, int strategy, double* expanded_out) {
+i64 find_neighbours(Mesh& m, const double* radii, const nlf_distance* distances, int strategy, double* expanded_out) {
     m.nb_off.assign(m.E + 1, 0);
     std::vector<std::vector<i32>> lists(m.E);
     m.expanded.assign(2 * m.G(), 0.0);
@@ -311,6 +363,7 @@ i64 find_neighbours(Mesh& m, const double* radii, int strategy, double* expanded
             continue;
         }
         if (r[0] <= 0.0 || r[1] <= 0.0) continue;                       // :131-134
+        const Distance distance = distances ? make_distance(distances[g].kind, distances[g].n, distances[g].n_real) : Distance{};
         // group_corners :40-47
         double left = std::numeric_limits<double>::max(), down = left;
         double right = std::numeric_limits<double>::lowest(), up = right;
@@ -340,7 +393,7 @@ i64 find_neighbours(Mesh& m, const double* radii, int strategy, double* expanded
             for (size_t i = r0; i < r1; ++i)
                 for (size_t j = c0; j < c1; ++j)
                     for (const i32 eNL : cells[i * cols + j])
-                        if (powered_distance2r(&m.centres[2 * eL], &m.centres[2 * eNL], r) <= 1.0)
+                        if (distance(&m.centres[2 * eL], &m.centres[2 * eNL], r) <= 1.0)
                             out.push_back(eNL);
         }
     }
@@ -365,11 +418,12 @@ struct Influence {
     i64 p_i = 1, q_i = 1;
     double p_d = 1.0, q_d = 1.0;
     double exp_q = 0.5;
+    Distance distance;
 };
 
 bool is_integral(double v) { return v >= 0 && std::floor(v) == v && v < 1e9; }
 
-double ipow(double x, i64 e) {                       // power.hpp:132-147 (integral exponent)
+double ipow(double x, i64 e) {                       // metamath/functions/power.hpp:25-40 (integral exponent)
     if (e < 0) return 1.0 / ipow(x, -e);
     double result = 1.0;
     while (e > 0) {
@@ -379,37 +433,55 @@ double ipow(double x, i64 e) {                       // power.hpp:132-147 (integ
     return result;
 }
 
-double exp_pow(double v, bool is_int, i64 ei, double ed) {   // _influence_function_2d::power :151-158
+double exp_pow(double v, bool is_int, i64 ei, double ed) {   // _influence_function_2d::power (influence_functions_2d.hpp:19-27)
     if (!is_int) return std::pow(v, ed);
     if (ei == 1) return v;
     return ipow(v, ei);
 }
 
+// The influence of a model (read_influence_2d, read_model.hpp:144-165; influence_functions_2d.hpp).
+// n enters as get_value(n): Infinity<size_t> converted to double.  The exponent is
+// _influence_function_2d::division(p, n) (:8-17).
 Influence make_influence(const nlf_model& md) {
     Influence f;
     f.kind = md.influence;
     f.r[0] = md.radius[0]; f.r[1] = md.radius[1];
-    const double n = 2.0;
+    f.distance = make_distance(md.distance, md.distance_n, md.distance_real);
+    const Distance& d = f.distance;
+    const bool rotation = d.kind == NLF_DISTANCE_ELLIPSE_WITH_ROTATION;   // its n is 2zu (search_function.hpp:77)
+    const bool infinity = !rotation && d.infinity;
+    const bool real = !rotation && d.real;
+    const i64 n_int = rotation ? 2 : d.n_int;
+    const double n = real ? d.n_real : infinity ? double(std::numeric_limits<size_t>::max()) : double(n_int);
     const double p = md.p, q = md.q;
-    if (f.kind == NLF_INFLUENCE_CONSTANT) {
+    auto division = [&](double v, bool& is_int, i64& vi, double& vd) {
+        if (infinity) {                                         // "used infinity norm": p unchanged
+            is_int = is_integral(v);
+            if (is_int) vi = i64(v); else vd = v;
+        } else if (!real && is_integral(v) && i64(v) % n_int == 0) {
+            is_int = true; vi = i64(v) / n_int;
+        } else {
+            is_int = false; vd = v / n;
+        }
+    };
+    if (f.kind == NLF_INFLUENCE_CONSTANT) {                                    // :44-48
         f.norm = std::tgamma(1 + 2 / n) / (4 * f.r[0] * f.r[1] * std::pow(std::tgamma(1 + 1 / n), 2));
     } else if (f.kind == NLF_INFLUENCE_POLYNOMIAL) {
-        if (p == 2.0 && q == 1.0) {
+        if (!real && !infinity && n_int == 2 && p == 2.0 && q == 1.0) {       // fast_polynomial (read_model.hpp:160-161)
             f.fast = true;
             f.norm = 2.0 / (M_PI * f.r[0] * f.r[1]);
-        } else {
-            // _p = division(p, n): integral when p, n integral and n | p
-            if (is_integral(p) && i64(p) % 2 == 0) { f.p_int = true; f.p_i = i64(p) / 2; }
-            else { f.p_int = false; f.p_d = p / n; }
+        } else {                                                               // :70-75
+            division(p, f.p_int, f.p_i, f.p_d);
             if (is_integral(q)) { f.q_int = true; f.q_i = i64(q); } else { f.q_int = false; f.q_d = q; }
-            f.norm = n * p / (4 * f.r[0] * f.r[1] * std::beta(1 / n, 1 / n) * std::beta(2 / p, q + 1));
+            f.norm = infinity ? p / (8 * f.r[0] * f.r[1] * std::beta(2 / p, q + 1))
+                              : n * p / (4 * f.r[0] * f.r[1] * std::beta(1 / n, 1 / n) * std::beta(2 / p, q + 1));
         }
-    } else if (f.kind == NLF_INFLUENCE_EXPONENTIAL) {
-        if (is_integral(p) && i64(p) % 2 == 0) { f.p_int = true; f.p_i = i64(p) / 2; }
-        else { f.p_int = false; f.p_d = p / n; }
+    } else if (f.kind == NLF_INFLUENCE_EXPONENTIAL) {                          // :103-109
+        division(p, f.p_int, f.p_i, f.p_d);
         f.exp_q = -q;
-        f.norm = (n * p * std::pow(4.0, 1 / n) * std::pow(q, 2 / p)) /
-                 (8 * f.r[0] * f.r[1] * std::tgamma(2 / p) * std::beta(0.5, 1 / n));
+        f.norm = infinity ? p * std::pow(q, 2 / p) / (8 * f.r[0] * f.r[1] * std::tgamma(2 / p))
+                          : (n * p * std::pow(4.0, 1 / n) * std::pow(q, 2 / p)) /
+                                (8 * f.r[0] * f.r[1] * std::tgamma(2 / p) * std::beta(0.5, 1 / n));
     } else {
         throw std::domain_error("unknown influence");
     }
@@ -417,7 +489,7 @@ Influence make_influence(const nlf_model& md) {
 }
 
 double influence_eval(const Influence& f, const double* x, const double* y) {
-    const double dist = powered_distance2r(x, y, f.r);
+    const double dist = f.distance(x, y, f.r);
     if (f.kind == NLF_INFLUENCE_CONSTANT) return dist <= 1.0 ? f.norm : 0.0;
     if (f.kind == NLF_INFLUENCE_EXPONENTIAL) return f.norm * std::exp(f.exp_q * exp_pow(dist, f.p_int, f.p_i, f.p_d));
     if (f.fast) return dist < 1.0 ? f.norm * (1.0 - dist) : 0.0;
@@ -1439,7 +1511,12 @@ void orc_tables(int nn, double* w, double* qN, double* qNxi, double* qNeta, doub
 }
 
 i64 orc_find_neighbours(void* h, const double* radii, int strategy, double* expanded) {
-    try { return find_neighbours(*static_cast<Mesh*>(h), radii, strategy, expanded); }
+    try { return find_neighbours(*static_cast<Mesh*>(h), radii, nullptr, strategy, expanded); }
+    catch (const std::exception& e) { g_error = e.what(); return -1; }
+}
+
+i64 orc_find_neighbours_ex(void* h, const double* radii, const nlf_distance* distances, int strategy, double* expanded) {
+    try { return find_neighbours(*static_cast<Mesh*>(h), radii, distances, strategy, expanded); }
     catch (const std::exception& e) { g_error = e.what(); return -1; }
 }
 
@@ -1454,6 +1531,25 @@ double orc_influence(const nlf_model* md, const double* x, const double* y) {
 }
 
 double orc_influence_norm(const nlf_model* md) { return make_influence(*md).norm; }
+
+// phi(x, ys[k]) for n points ys[2n] (quadrature sums over a grid in the tests); 0 on success.
+int orc_influence_points(const nlf_model* md, const double* x, i64 n, const double* ys, double* out) {
+    try {
+        const Influence f = make_influence(*md);
+#pragma omp parallel for
+        for (i64 k = 0; k < n; ++k) out[k] = influence_eval(f, x, ys + 2 * k);
+        return 0;
+    } catch (const std::exception& e) { g_error = e.what(); return -1; }
+}
+
+// distance(x, ys[k], r) of one group's search (nlf_distance) for n points; 0 on success.
+int orc_distance_points(const nlf_distance* d, const double* x, const double* r, i64 n, const double* ys, double* out) {
+    try {
+        const Distance dist = make_distance(d->kind, d->n, d->n_real);
+        for (i64 k = 0; k < n; ++k) out[k] = dist(x, ys + 2 * k, r);
+        return 0;
+    } catch (const std::exception& e) { g_error = e.what(); return -1; }
+}
 
 // physics 0 = thermal (materials: nlf_thermal_material[G]), 1 = elastic (nlf_elastic_material[G]).
 void* orc_assemble(void* h, int physics, const nlf_model* models, const void* materials, const uint8_t* is_inner,

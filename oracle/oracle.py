@@ -44,11 +44,15 @@ def lib():
         L.orc_basis.argtypes = [C.c_int, C.c_double, C.c_double, _f64p, _f64p, _f64p]
         L.orc_find_neighbours.restype = _i64
         L.orc_find_neighbours.argtypes = [C.c_void_p, _f64p, C.c_int, _f64p]
+        L.orc_find_neighbours_ex.restype = _i64
+        L.orc_find_neighbours_ex.argtypes = [C.c_void_p, _f64p, C.c_void_p, C.c_int, _f64p]
         L.orc_neighbours_export.argtypes = [C.c_void_p, _i64p, _i32p]
         L.orc_influence.restype = C.c_double
         L.orc_influence.argtypes = [C.POINTER(_abi.Model), _f64p, _f64p]
         L.orc_influence_norm.restype = C.c_double
         L.orc_influence_norm.argtypes = [C.POINTER(_abi.Model)]
+        L.orc_influence_points.argtypes = [C.POINTER(_abi.Model), _f64p, _i64, _f64p, _f64p]
+        L.orc_distance_points.argtypes = [C.POINTER(_abi.Distance), _f64p, _f64p, _i64, _f64p, _f64p]
         L.orc_assemble.restype = C.c_void_p
         L.orc_assemble.argtypes = [C.c_void_p, C.c_int, C.POINTER(_abi.Model), C.c_void_p, _u8p, C.c_int, C.c_int,
                                    C.c_int]
@@ -229,11 +233,17 @@ class OracleMesh:
         lib().orc_quadrature(self.h, _ptr(xq, _f64p), _ptr(jac, _f64p), _ptr(D, _f64p))
         return xq, jac, D
 
-    def find_neighbours(self, radii, strategy=_abi.NLF_EXPAND_MAX):
-        """radii: (G, 2) search radii per group.  Returns (offsets, ids, expanded_radii)."""
+    def find_neighbours(self, radii, strategy=_abi.NLF_EXPAND_MAX, distances=None):
+        """radii: (G, 2) search radii per group; distances: one nlf_distance per group (None: Euclidean).
+        Returns (offsets, ids, expanded_radii)."""
         radii = np.ascontiguousarray(np.asarray(radii, dtype=np.float64).reshape(-1, 2))
         expanded = np.zeros_like(radii)
-        total = lib().orc_find_neighbours(self.h, _ptr(radii, _f64p), strategy, _ptr(expanded, _f64p))
+        if distances is None:
+            total = lib().orc_find_neighbours(self.h, _ptr(radii, _f64p), strategy, _ptr(expanded, _f64p))
+        else:
+            arr = (_abi.Distance * len(distances))(*distances)
+            total = lib().orc_find_neighbours_ex(self.h, _ptr(radii, _f64p), C.cast(arr, C.c_void_p), strategy,
+                                                 _ptr(expanded, _f64p))
         if total < 0:
             raise RuntimeError(lib().orc_last_error().decode())
         off = np.zeros(self.mesh.n_elems + 1, dtype=np.int64)
@@ -476,6 +486,28 @@ def influence(model, x, y) -> float:
     xa = np.ascontiguousarray(x, dtype=np.float64)
     ya = np.ascontiguousarray(y, dtype=np.float64)
     return lib().orc_influence(C.byref(model), _ptr(xa, _f64p), _ptr(ya, _f64p))
+
+
+def influence_points(model, x, ys) -> np.ndarray:
+    """phi(x, y) for every row y of ys (k, 2)."""
+    xa = np.ascontiguousarray(np.asarray(x, dtype=np.float64))
+    ya = np.ascontiguousarray(np.asarray(ys, dtype=np.float64).reshape(-1, 2))
+    out = np.empty(ya.shape[0])
+    if lib().orc_influence_points(C.byref(model), _ptr(xa, _f64p), ya.shape[0], _ptr(ya, _f64p), _ptr(out, _f64p)):
+        raise RuntimeError(lib().orc_last_error().decode())
+    return out
+
+
+def distance_points(dist, x, r, ys) -> np.ndarray:
+    """The search distance(x, y, r) of an nlf_distance for every row y of ys (k, 2)."""
+    xa = np.ascontiguousarray(np.asarray(x, dtype=np.float64))
+    ra = np.ascontiguousarray(np.asarray(r, dtype=np.float64))
+    ya = np.ascontiguousarray(np.asarray(ys, dtype=np.float64).reshape(-1, 2))
+    out = np.empty(ya.shape[0])
+    if lib().orc_distance_points(C.byref(dist), _ptr(xa, _f64p), _ptr(ra, _f64p), ya.shape[0], _ptr(ya, _f64p),
+                                 _ptr(out, _f64p)):
+        raise RuntimeError(lib().orc_last_error().decode())
+    return out
 
 
 # ---------------------------------------------------------------------------------------------

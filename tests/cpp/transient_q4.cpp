@@ -40,7 +40,7 @@ int main(int argc, char** argv) {
         const auto mesh = std::make_shared<mesh::mesh_2d<double, int64_t>>(d);
         using namespace nonlocal::solver_2d::thermal;
         const parameters_2d<double> parameters = {
-            {"material", {.model = {.influence = solver_2d::influence::polynomial<double>({r, r}, 2, 2, 2), .local_weight = p1},
+            {"material", {.model = {.influence = solver_2d::influence::polynomial<double>({r, r}, size_t{2}, size_t{2}, size_t{2}), .local_weight = p1},
                           .physical = {.conductivity = 1.0, .capacity = 2.0}}}};
         mesh->neighbours(mesh::find_neighbours(*mesh, mesh::influences<double>{{"material", {.radius = {r, r}}}}));
         thermal_boundaries_conditions_2d<double> conditions;

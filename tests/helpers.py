@@ -15,9 +15,13 @@ def radii(models):
             else [0.0, 0.0] for m in models]
 
 
+def distances(models):
+    return [A.distance_of(m) for m in models]
+
+
 def oracle_assembly(problem, part=A.NLF_PART_FULL, convection=True):
     om = O.OracleMesh(problem.mesh)
-    om.find_neighbours(radii(problem.models))
+    om.find_neighbours(radii(problem.models), distances=distances(problem.models))
     f, xD, inner = om.boundary(problem.bcs, dof=problem.dof)
     K = om.assemble(problem.physics, problem.models, problem.materials, inner, part=part)
     if convection and problem.physics == 0 and part == A.NLF_PART_FULL:
