@@ -35,15 +35,18 @@ def log(*a):
 
 
 TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "spmv_composite1000_hbm_counters.txt")
+TRAFFIC_PROFILE_SYM = os.path.join(ROOT, "profiles", "symv_composite1000_hbm_counters.txt")
 
 
-def counter_traffic(n, mode_fp64):
+def counter_traffic(n, mode_fp64, symmetric=False):
     """HBM bytes per SpMV launch from the committed FETCH_SIZE / WRITE_SIZE summary of the FP64
-    SpMV on the 1000x1000 composite (None for other workloads or value types)."""
+    SpMV on the 1000x1000 composite (the symmetric slice kernels or the full-storage kernel; None for
+    other workloads or value types)."""
     import re
-    if n != 1000 or not mode_fp64 or not os.path.exists(TRAFFIC_PROFILE):
+    path = TRAFFIC_PROFILE_SYM if symmetric else TRAFFIC_PROFILE
+    if n != 1000 or not mode_fp64 or not os.path.exists(path):
         return None
-    with open(TRAFFIC_PROFILE) as f:
+    with open(path) as f:
         m = re.search(r"SpMV traffic per launch:.*=\s*([0-9.]+e[+0-9]+)\s*B", f.read())
     return float(m.group(1)) if m else None
 
@@ -512,7 +515,8 @@ def main():
         step_s.append(t_step)
         if k == args.steps - 1:
             last = dict(x=sol.x.clone(), info=sol.matrix.info(), stats=sol.stats, timings=dict(sol.timings),
-                        iterations=int(sol.iterations), true_residual=sol.true_residual)
+                        iterations=int(sol.iterations), true_residual=sol.true_residual,
+                        symv=sol.matrix.symv_info() if mode == A.NLF_MODE_FP64 else None)
             if not args.no_cpu:
                 # the last step's matrix, for the oracle check after the timed region: K v for one
                 # random v and a few sampled row ranges of the upper CSR
@@ -535,6 +539,7 @@ def main():
     sx = 8
     algo_bytes = info["nnz_upper"] * (sv + 4) + rows * (8 + 2 * sx)     # upper-symmetric CSR formula
     achieved = algo_bytes / (st.spmv_ms * 1e-3) / 1e9 if st.spmv_ms > 0 else 0.0
+    symmetric = sv == 8 and last.get("symv") is not None
     moved_gbs = st.spmv_bytes / (st.spmv_ms * 1e-3) / 1e9 if st.spmv_ms > 0 else 0.0
     cpu = None
     if not args.no_cpu:
@@ -569,9 +574,11 @@ def main():
         "phase_seconds": {k: round(v, 4) for k, v in last["timings"].items()},
         "precond_apply_ms": round(st.precond_ms, 4),
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": counter_traffic(args.n, sv == 8),
-                     "traffic_source": os.path.relpath(TRAFFIC_PROFILE, ROOT),
-                     "kernel": f"k_spmv2 ({ {4: 'fp32', 6: 'fp32 + 16-bit remainder'}.get(sv, 'fp64')} values, full CSR)",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": counter_traffic(args.n, sv == 8, symmetric),
+                     "traffic_source": os.path.relpath(TRAFFIC_PROFILE_SYM if symmetric else TRAFFIC_PROFILE, ROOT),
+                     "kernel": ("k_symv_slices + k_symv_reduce (fp64 upper triangle read once, 64-row slices)"
+                                if symmetric else
+                                f"k_spmv2 ({ {4: 'fp32', 6: 'fp32 + 16-bit remainder'}.get(sv, 'fp64')} values, full CSR)"),
                      "avg_launch_ms": round(st.spmv_ms, 4), "samples": int(st.spmv_samples),
                      "algorithmic_bytes_per_launch": int(algo_bytes),
                      "moved_bytes_per_launch": int(st.spmv_bytes), "moved_GBps": round(moved_gbs, 1)},

@@ -158,6 +158,9 @@ int         nlf_ctx_synchronize(nlf_ctx ctx);
 #define NLF_OPT_PATTERN_THREADS  4   /* threads per node workgroup of the pattern build: 64, 128 (default), 256 */
 #define NLF_OPT_PAIR_BUDGET_MIB  5   /* MiB for the MFMA pair blocks of one assembly pass; 0 (default): 8 GiB;
                                         a mesh whose blocks do not fit is assembled in node chunks */
+#define NLF_OPT_SPMV_SYMMETRIC   6   /* 1 (default): the FP64 products of CG, flexible CG and nlf_spmv read the upper
+                                        triangle once (the reference's symmetric product), from a slice layout built
+                                        on the first solve; 0: the full (both-triangle) storage */
 int         nlf_ctx_set_option(nlf_ctx ctx, int option, int value);
 void*       nlf_ctx_stream(nlf_ctx ctx);
 
@@ -224,6 +227,11 @@ int nlf_matrix_export(nlf_matrix a, int part, int64_t* outer, int64_t* inner, do
  * SparseMatrix::middleRows would give them): outer[row1 - row0 + 1] from 0, inner/values may be NULL. */
 int nlf_matrix_export_rows(nlf_matrix a, int64_t row0, int64_t row1, int64_t* outer, int64_t* inner, double* values);
 int nlf_matrix_device(nlf_matrix a, const int64_t** outer, const int32_t** cols, double** values, const int64_t** diag);
+/* No reference counterpart.  Symmetric SpMV storage (NLF_OPT_SPMV_SYMMETRIC): builds it when it applies and
+ * reports its size: slots (64 values each: the upper triangle padded to each 64-row slice's union of column
+ * offsets) and strip doubles (the transposed sums each product writes and reads back).  NLF_ERR_STATE when it
+ * does not apply (the full storage serves the products). */
+int nlf_matrix_symv_info(nlf_matrix a, int64_t* slots, int64_t* strips);
 /* No reference counterpart.  Row-pattern dictionary of the SpMV (NLF_OPT_ROW_PATTERNS): number of
  * distinct offset rows and their total length (both 0 when the per-nonzero offsets are used). */
 int nlf_matrix_pattern_info(nlf_matrix a, int64_t* groups, int64_t* entries);
@@ -323,7 +331,9 @@ int nlf_cg_solve(nlf_matrix a, nlf_precond p, const double* b_dev, double* x_dev
  * work vectors, partial sums and solver state live in a fixed device array of the library, sized for
  * at most NLF_NOALLOC_MAX_ROWS rows (about 48 MiB, held per GPU until the process ends).  It calls no
  * hipMalloc or hipFree and records no HIP events (stats carry no kernel timings).  A larger matrix is
- * an argument error.  Every call on one GPU shares that array: calls that overlap on one GPU race. */
+ * an argument error.  Every call on one GPU shares that array: calls that overlap on one GPU race.
+ * Its products read the full storage (the symmetric slice storage, NLF_OPT_SPMV_SYMMETRIC, is memory it
+ * does not allocate), so it matches nlf_cg_solve bitwise with that option at 0. */
 #define NLF_NOALLOC_MAX_ROWS 2097152
 int nlf_cg_solve_noalloc(nlf_matrix a, nlf_precond p, const double* b_dev, double* x_dev, double tol, int64_t max_it,
                          nlf_solve_stats* stats);

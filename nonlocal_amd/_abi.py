@@ -24,6 +24,7 @@ NLF_OPT_ROW_PATTERNS = 2
 NLF_OPT_ASSEMBLY_THREADS = 3
 NLF_OPT_PATTERN_THREADS = 4
 NLF_OPT_PAIR_BUDGET_MIB = 5
+NLF_OPT_SPMV_SYMMETRIC = 6
 
 
 class Model(C.Structure):

@@ -17,7 +17,7 @@ import numpy as np
 import torch
 
 from . import _abi
-from ._abi import (NLF_EXPAND_MAX, NLF_MODE_FP64, NLF_PART_FULL, NLF_PART_LOCAL, NLF_PRECOND_IDENTITY,
+from ._abi import (NLF_ERR_STATE, NLF_EXPAND_MAX, NLF_MODE_FP64, NLF_PART_FULL, NLF_PART_LOCAL, NLF_PRECOND_IDENTITY,
                    NLF_PRECOND_ILLT, NLF_PRECOND_JACOBI)
 from ._lib import check, load, NLFError  # noqa: F401
 
@@ -182,6 +182,16 @@ class finite_element_matrix:
         rows, up, full, bound = C.c_int64(), C.c_int64(), C.c_int64(), C.c_int64()
         self.ctx.check(load().nlf_matrix_info(self.h, C.byref(rows), C.byref(up), C.byref(full), C.byref(bound)))
         return dict(rows=rows.value, nnz_upper=up.value, nnz_full=full.value, nnz_bound=bound.value)
+
+    def symv_info(self):
+        """(slots, strip doubles) of the symmetric SpMV storage, built here if it applies; None when it
+        does not (the products then read the full storage)."""
+        a, b = C.c_int64(), C.c_int64()
+        code = load().nlf_matrix_symv_info(self.h, C.byref(a), C.byref(b))
+        if code == NLF_ERR_STATE:
+            return None
+        self.ctx.check(code)
+        return a.value, b.value
 
     def pattern_info(self):
         """(distinct offset rows, their total length) of the SpMV's row-pattern dictionary; (0, 0)

@@ -1694,6 +1694,7 @@ void add_pairs_device(nlf_matrix_s* a, const std::vector<i64>& rc, const std::ve
     NLF_HIP(hipGetLastError());
     NLF_HIP(hipStreamSynchronize(s));
     a->split_ready = false;                           // the split remainders no longer match
+    ++a->val_version;
     a->extra_rc.insert(a->extra_rc.end(), rc.begin(), rc.end());
     a->extra_v.insert(a->extra_v.end(), v.begin(), v.end());
 }
@@ -1855,6 +1856,7 @@ int nlf_assemble_ex(nlf_mesh m, int physics, const nlf_model* models, const void
         assemble_values<double>(mat.get(), s, pairs_path ? &plan : nullptr);
         if (neumann) border_neumann(mat.get(), s);
         mat->nnz_upper = (mat->nnz_full + mat->rows) / 2;
+        ++mat->val_version;
         mat->seconds_values = std::chrono::duration<double>(std::chrono::steady_clock::now() - t1).count();
         *out = mat.release();
     });
@@ -1976,7 +1978,10 @@ int nlf_matrix_device(nlf_matrix a, const int64_t** outer, const int32_t** cols,
     if (!a) return NLF_ERR_ARGUMENT;
     if (outer) *outer = a->outer.p;
     if (cols) *cols = a->cols.p;
-    if (values) *values = a->val.p;
+    if (values) {
+        *values = a->val.p;
+        a->values_exposed = true;                     // the caller may change them: derived copies are rebuilt per solve
+    }
     if (diag) *diag = a->diag.p;
     return NLF_OK;
 }

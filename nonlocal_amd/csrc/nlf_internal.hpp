@@ -276,6 +276,7 @@ struct nlf_ctx_s {
     int assembly_threads = 64;           // NLF_OPT_ASSEMBLY_THREADS
     int pattern_threads = 128;           // NLF_OPT_PATTERN_THREADS
     int pair_budget_mib = 0;             // NLF_OPT_PAIR_BUDGET_MIB (0: automatic)
+    int spmv_symmetric = 1;              // NLF_OPT_SPMV_SYMMETRIC
 };
 
 struct nlf_mesh_s {
@@ -313,7 +314,10 @@ struct nlf_mesh_s {
     double last_neighbour_seconds = 0.0;
 };
 
+struct nlf_symv_s;                       // the symmetric SpMV storage (nlf_symv.inl)
+
 struct nlf_matrix_s {
+    ~nlf_matrix_s();
     nlf_ctx ctx = nullptr;
     nlf_mesh mesh = nullptr;
     int dof = 1;
@@ -348,6 +352,11 @@ struct nlf_matrix_s {
     bool combined = false;               // values changed by nlf_matrix_combine (no FP32 value pass)
     nlf::DevBuf<double> cq;              // coefficients per quadrature point, [(e * nq + q) * 6 + k], or empty
     bool neumann = false;                // bordered by the Lagrange row and column of a Neumann problem
+    // values version: bumped by every change of val; the symmetric SpMV storage is rebuilt from val
+    // when its version is stale, and on every solve once val was handed out (nlf_matrix_device)
+    nlf::i64 val_version = 0, symv_refused_version = -1;
+    bool values_exposed = false;
+    nlf_symv_s* symv = nullptr;
 };
 
 namespace nlf {

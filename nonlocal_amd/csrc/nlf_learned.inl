@@ -1508,9 +1508,13 @@ i64 fcg(hipStream_t s, nlf_matrix A, const V* val, nlf_precond M, const double* 
         download(h, sums.p, 2, s);
         return std::make_pair(std::sqrt(h[0]), std::sqrt(h[1]));
     };
+    // FP64 values: the products read the upper triangle once (the symmetric storage), when it applies
+    bool sym = false;
+    if constexpr (std::is_same_v<V, double>) sym = !lo && symv_prepare(A, s);
     auto spmv_pq = [&](bool sample) {             // q = A p, partial p.q into pb; partial p.r into pa
         if (sample) timer.start(0, s);
-        launch_spmv<V, double, double, true>(lpr, n, A->outer.p, spmv_cols(A), val, p.p, q.p, pb.p, nullptr, s, false, lo);
+        if (sym) launch_symv<true>(A, p.p, q.p, pb.p, nullptr, s, sb);
+        else launch_spmv<V, double, double, true>(lpr, n, A->outer.p, spmv_cols(A), val, p.p, q.p, pb.p, nullptr, s, false, lo);
         if (sample) timer.stop(0, s);
         hipLaunchKernelGGL(k_dot<double>, dim3(sb), dim3(SPMV_THREADS), 0, s, n, p.p, r.p, pa.p, (const CGState*)nullptr);
     };

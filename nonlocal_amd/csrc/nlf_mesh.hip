@@ -228,7 +228,8 @@ const char* nlf_last_error(nlf_ctx ctx) { return ctx ? ctx->error.c_str() : "nul
 // node workgroup of the node-centric value pass (64 default, 128, 256, 512); NLF_OPT_PATTERN_THREADS
 // = threads per node workgroup of the pattern build (64, 128 default, 256); NLF_OPT_PAIR_BUDGET_MIB =
 // device memory for the MFMA pair blocks of one assembly pass in MiB (0 = 8 GiB); a mesh whose blocks
-// do not fit is assembled in node chunks.
+// do not fit is assembled in node chunks; NLF_OPT_SPMV_SYMMETRIC 1 = FP64 products from the upper
+// triangle in the slice layout (default), 0 = from the full storage.
 int nlf_ctx_set_option(nlf_ctx ctx, int option, int value) {
     if (!ctx) return NLF_ERR_ARGUMENT;
     if (option == NLF_OPT_ASSEMBLY_PAIRS) ctx->assembly_pairs = value != 0;
@@ -242,6 +243,8 @@ int nlf_ctx_set_option(nlf_ctx ctx, int option, int value) {
     } else if (option == NLF_OPT_PAIR_BUDGET_MIB) {
         if (value < 0) return NLF_ERR_ARGUMENT;
         ctx->pair_budget_mib = value;
+    } else if (option == NLF_OPT_SPMV_SYMMETRIC) {
+        ctx->spmv_symmetric = value != 0;
     }
     else return NLF_ERR_ARGUMENT;
     return NLF_OK;

@@ -23,6 +23,9 @@
 #include <cstring>
 #include <map>
 #include <cmath>
+#include <memory>
+#include <numeric>
+#include <type_traits>
 
 namespace nlf {
 
@@ -1388,6 +1391,14 @@ struct CGWork {
     }
 };
 
+}  // namespace nlf
+
+#include "nlf_symv.inl"
+
+nlf_matrix_s::~nlf_matrix_s() { delete symv; }
+
+namespace nlf {
+
 // Preconditioned CG on device vectors (conjugate_gradient::solve).  Vectors of type X, matrix values V.
 // The work memory (CGWork) is allocated by the object, or given by the caller (no allocation then;
 // `timed` = false also records no HIP events).
@@ -1409,6 +1420,7 @@ struct CG {
     unsigned sblocks, vblocks;
     bool timed = true;
     bool indefinite = false;            // a bordered (Neumann) system: negative curvature allowed
+    const nlf_matrix_s* sym = nullptr;  // FP64 products from the symmetric storage (symv_prepare done)
     KernelTimer timer;
 
     CG(hipStream_t s_, i64 n_, i64 nnz, const i64* o, const i32* c, const V* v, const nlf_precond_s* P_,
@@ -1549,7 +1561,14 @@ struct CG {
     void iteration(X* x, bool sample = false) {
         // z = A p ; alpha = rz / p.z
         if (sample) timer.start(0, s);
-        launch_spmv<V, X, double, true>(lpr, n, outer, cs, val, p, z, part1, st, s, false, lo);
+        bool done_sym = false;
+        if constexpr (std::is_same_v<V, double> && std::is_same_v<X, double>) {
+            if (sym) {
+                launch_symv<true>(sym, p, z, part1, st, s, sblocks);
+                done_sym = true;
+            }
+        }
+        if (!done_sym) launch_spmv<V, X, double, true>(lpr, n, outer, cs, val, p, z, part1, st, s, false, lo);
         if (sample) timer.stop(0, s);
         hipLaunchKernelGGL(k_alpha, dim3(1), dim3(1024), 0, s, part1, int(sblocks), st);
         if (!P || P->kind == NLF_PRECOND_IDENTITY || jacobi()) {
@@ -1589,6 +1608,11 @@ namespace nlf {
 // HBM bytes one SpMV streams: values, column source, row pointers, x once and y.
 i64 spmv_stream_bytes(const nlf_matrix_s* a, size_t sv, size_t sx) {
     const i64 n = a->rows;
+    if (a->symv && sv == sizeof(double) && a->symv->version == a->val_version) {
+        // slice values, the strips written and read, the direct sums written and read, x once and y
+        const nlf_symv_s& v = *a->symv;
+        return v.slots * 64 * i64(sizeof(double)) + 2 * v.part_size * i64(sizeof(double)) + n * i64(4 * sizeof(double));
+    }
     i64 cols = a->nnz_full * i64(a->off16.p ? sizeof(int16_t) : sizeof(i32));
     if (a->pstart.p) cols = a->pat_size * i64(sizeof(int16_t)) + n * i64(sizeof(i64));
     return a->nnz_full * i64(sv) + cols + n * i64(sizeof(i64) + 2 * sx);
@@ -1610,11 +1634,25 @@ extern "C" {
 int nlf_spmv(nlf_matrix a, const double* x, double* y) {
     if (!a) return NLF_ERR_ARGUMENT;
     return guarded(a->ctx, [&] {
+        if (symv_prepare(a, a->ctx->stream)) {
+            launch_symv<false>(a, x, y, nullptr, nullptr, a->ctx->stream, SPMV_MAX_BLOCKS);
+            NLF_HIP(hipGetLastError());
+            return;
+        }
         const int lpr = pick_lpr(a);
         launch_spmv<double, double, double, false>(lpr, a->rows, a->outer.p, spmv_cols(a), a->val.p, x, y, nullptr,
                                                    nullptr,
                                                    a->ctx->stream);
         NLF_HIP(hipGetLastError());
+    });
+}
+
+int nlf_matrix_symv_info(nlf_matrix a, int64_t* slots, int64_t* strips) {
+    if (!a) return NLF_ERR_ARGUMENT;
+    return guarded(a->ctx, [&] {
+        NLF_CHECK(symv_prepare(a, a->ctx->stream), NLF_ERR_STATE, "the symmetric SpMV storage does not apply");
+        if (slots) *slots = a->symv->slots;
+        if (strips) *strips = a->symv->part_size;
     });
 }
 
@@ -1928,6 +1966,7 @@ int nlf_cg_solve(nlf_matrix a, nlf_precond p, const double* b, double* x, double
             CG<double, double> cg(s, n, a->nnz_full, a->outer.p, a->cols.p, a->val.p, p);
             cg.set_cols(a);
             cg.indefinite = a->neumann;
+            if (symv_prepare(a, s)) cg.sym = a;
             out.iterations = cg.solve(b, x, tol, max_it, &out.residual, &breakdown);
             timings(cg.timer, sizeof(double), sizeof(double));
         } else {

@@ -135,6 +135,7 @@ extern "C" int nlf_matrix_combine(nlf_matrix K, double alpha, nlf_matrix C, doub
         K->val32.release();
         K->val_lo.release();
         K->val32_assembled = false;
+        ++K->val_version;
         K->split_ready = false;
         K->combined = true;
     });

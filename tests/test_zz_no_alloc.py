@@ -39,7 +39,13 @@ def _solve(ctx, K, P, b, noalloc, tol=1e-13):
     if noalloc:
         rc = load().nlf_cg_solve_noalloc(K.h, P.h, api._dev(b), api._dev(x), tol, 100000, C.byref(st))
     else:
-        rc = load().nlf_cg_solve(K.h, P.h, api._dev(b), api._dev(x), tol, 100000, A.NLF_MODE_FP64, C.byref(st))
+        # the same products as the no-alloc solve, which reads the full storage (the symmetric slice
+        # storage of nlf_cg_solve is memory the no-alloc entry point does not allocate)
+        ctx.set_option(A.NLF_OPT_SPMV_SYMMETRIC, 0)
+        try:
+            rc = load().nlf_cg_solve(K.h, P.h, api._dev(b), api._dev(x), tol, 100000, A.NLF_MODE_FP64, C.byref(st))
+        finally:
+            ctx.set_option(A.NLF_OPT_SPMV_SYMMETRIC, 1)
     ctx.check(rc)
     ctx.synchronize()
     return x, st
