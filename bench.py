@@ -507,30 +507,31 @@ def main():
     # starts, so every step runs with the same free device memory; only the last step's results are
     # kept for the line.
     step_s = []
-    last = None
+    last_sol = None
     t0 = time.perf_counter()
     for k in range(args.steps):
-        last = None
+        last_sol = None
         sol, t_step = run_step(api, A, mesh, problem, pc, mode, args.tol, coarse_of.get(best), smoother_of.get(best))
         step_s.append(t_step)
         if k == args.steps - 1:
-            last = dict(x=sol.x.clone(), info=sol.matrix.info(), stats=sol.stats, timings=dict(sol.timings),
-                        iterations=int(sol.iterations), true_residual=sol.true_residual,
-                        symv=sol.matrix.symv_info() if mode == A.NLF_MODE_FP64 else None)
-            if not args.no_cpu:
-                # the last step's matrix, for the oracle check after the timed region: K v for one
-                # random v and a few sampled row ranges of the upper CSR
-                n_rows = last["info"]["rows"]
-                v = torch.from_numpy(np.random.default_rng(5).standard_normal(n_rows)).to(sol.x.device)
-                starts = np.linspace(0, n_rows - 64, 48).astype(np.int64)
-                last["check"] = dict(v=v.cpu().numpy(), Kv=sol.matrix.spmv(v).cpu().numpy(),
-                                     x=sol.x.cpu().numpy(),
-                                     rows=[(int(r), int(r) + 64, sol.matrix.export_rows(int(r), int(r) + 64))
-                                           for r in starts])
+            last_sol = sol
         del sol
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
     sec_per_step = elapsed / args.steps
+    # after the timed region: the last step's results, and for the oracle check K v for one random v
+    # and a few sampled row ranges of the upper CSR
+    sol = last_sol
+    last = dict(x=sol.x.clone(), info=sol.matrix.info(), stats=sol.stats, timings=dict(sol.timings),
+                iterations=int(sol.iterations), true_residual=sol.true_residual,
+                symv=sol.matrix.symv_info() if mode == A.NLF_MODE_FP64 else None)
+    if not args.no_cpu:
+        n_rows = last["info"]["rows"]
+        v = torch.from_numpy(np.random.default_rng(5).standard_normal(n_rows)).to(sol.x.device)
+        starts = np.linspace(0, n_rows - 64, 48).astype(np.int64)
+        last["check"] = dict(v=v.cpu().numpy(), Kv=sol.matrix.spmv(v).cpu().numpy(), x=sol.x.cpu().numpy(),
+                             rows=[(int(r), int(r) + 64, sol.matrix.export_rows(int(r), int(r) + 64)) for r in starts])
+    del sol, last_sol
     err = float(torch.linalg.norm(last["x"] - x_ref) / torch.linalg.norm(x_ref))
     info = last["info"]
     st = last["stats"]

@@ -31,8 +31,14 @@ struct SymBand {                         // a run of consecutive offsets of a sl
     int pad_;
 };
 
-constexpr int SYMV_WAVES = 4;            // slices per block of the product kernel
-constexpr int SYMV_UNROLL = 8;           // entries each lane has in flight
+#ifndef NLF_SYMV_WAVES
+#define NLF_SYMV_WAVES 4
+#endif
+#ifndef NLF_SYMV_UNROLL
+#define NLF_SYMV_UNROLL 8
+#endif
+constexpr int SYMV_WAVES = NLF_SYMV_WAVES;    // slices per block of the product kernel
+constexpr int SYMV_UNROLL = NLF_SYMV_UNROLL;  // entries each lane has in flight
 
 template<bool STOP>
 __global__ __launch_bounds__(64 * SYMV_WAVES) void k_symv_slices(i64 rows, i64 ns, int wmax, const i64* __restrict__ kptr,
