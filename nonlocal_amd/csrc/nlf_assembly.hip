@@ -406,6 +406,81 @@ __global__ __launch_bounds__(PATTERN_THREADS) void k_pattern(PatternArgs a) {
             if (!rowinner[d]) { a.cols[a.outer[rowv[d]]] = i32(rowv[d]); a.diag[rowv[d]] = a.outer[rowv[d]]; }
 }
 
+// Pattern of a matrix with no nonlocal group (assemble_part::LOCAL, or every group LOCAL): a row's
+// column nodes are the nodes of the node's own elements, at most PL_MAX of them, so one thread per
+// node gathers and sorts them (no bitmap).  Same classification and order as k_pattern: column nodes
+// ascending, then the DoF; inner columns of inner rows, Dirichlet rows keep their diagonal only,
+// bound columns of every row.  A node with more column nodes sets *overflow (k_pattern then builds).
+constexpr int PL_MAX = 64;
+constexpr int PL_THREADS = 64;
+
+template<int PASS>
+__global__ __launch_bounds__(PL_THREADS) void k_pattern_local(PatternArgs a) {
+    __shared__ i32 buf[PL_THREADS][PL_MAX + 1];
+    const i64 node = i64(blockIdx.x) * PL_THREADS + threadIdx.x;
+    if (node >= a.N) return;
+    i32* ids = buf[threadIdx.x];
+    int n = 0;
+    bool over = false;
+    for (i32 k = a.ne_off[node]; k < a.ne_off[node + 1] && !over; ++k) {
+        const i32 e = a.ne_ids[k];
+        for (int j = 0; j < a.nn; ++j) {
+            const i32 v = a.el[i64(e) * a.nn + j];
+            int pos = n;                                   // insertion into the sorted unique list
+            while (pos > 0 && ids[pos - 1] > v) --pos;
+            if (pos > 0 && ids[pos - 1] == v) continue;
+            if (n == PL_MAX) { over = true; break; }
+            for (int t = n; t > pos; --t) ids[t] = ids[t - 1];
+            ids[pos] = v;
+            ++n;
+        }
+    }
+    if (over) {
+        atomicOr(a.overflow, 1);
+        return;
+    }
+    const int D = a.dof;
+    i64 rowv[2];
+    bool rowinner[2];
+    for (int d = 0; d < D; ++d) { rowv[d] = i64(D) * node + d; rowinner[d] = a.inner[rowv[d]]; }
+    if (PASS == 0) {
+        i64 cnt[4] = {0, 0, 0, 0};
+        for (int t = 0; t < n; ++t)
+            for (int dc = 0; dc < D; ++dc) {
+                const i64 c = i64(D) * ids[t] + dc;
+                const bool ci = a.inner[c];
+                for (int d = 0; d < D; ++d) {
+                    if (ci) { if (rowinner[d]) cnt[d]++; }
+                    else if (c != rowv[d]) cnt[2 + d]++;
+                }
+            }
+        for (int d = 0; d < D; ++d) {
+            a.cnt[rowv[d]] = rowinner[d] ? cnt[d] : 1;
+            a.bcnt[rowv[d]] = cnt[2 + d];
+        }
+        return;
+    }
+    i64 pos[4];
+    for (int d = 0; d < D; ++d) { pos[d] = a.outer[rowv[d]]; pos[2 + d] = a.b_outer[rowv[d]]; }
+    for (int t = 0; t < n; ++t)
+        for (int dc = 0; dc < D; ++dc) {
+            const i64 c = i64(D) * ids[t] + dc;
+            const bool ci = a.inner[c];
+            for (int d = 0; d < D; ++d) {
+                if (ci) {
+                    if (rowinner[d]) {
+                        if (c == rowv[d]) a.diag[rowv[d]] = pos[d];
+                        a.cols[pos[d]++] = i32(c);
+                    }
+                } else if (c != rowv[d]) {
+                    a.b_cols[pos[2 + d]++] = i32(c);
+                }
+            }
+        }
+    for (int d = 0; d < D; ++d)
+        if (!rowinner[d]) { a.cols[a.outer[rowv[d]]] = i32(rowv[d]); a.diag[rowv[d]] = a.outer[rowv[d]]; }
+}
+
 // ---------------------------------------------------------------------------------------------
 // Values
 // ---------------------------------------------------------------------------------------------
@@ -463,22 +538,22 @@ __device__ __forceinline__ int lower_bound_i32(const i32* a, int n, i32 key) {
     return lo;
 }
 
-// Local integrand row i of element e: K^L_ij (thermal) or 2x2 blocks (elastic), scaled by p1.
-// conductivity_matrix_2d.hpp:106-130, stiffness_matrix_2d.hpp:44-80.  T = float is the FP32 value
-// pass: w_q / |det J| is formed in FP64, the products and sums in T.
+// Local integrand entry (i, j) of element e: K^L_ij (thermal) or its 2x2 block (elastic), scaled by
+// p1.  conductivity_matrix_2d.hpp:106-130, stiffness_matrix_2d.hpp:44-80.  T = float is the FP32
+// value pass: w_q / |det J| is formed in FP64, the products and sums in T.  The sum over the
+// quadrature points runs in q order, whichever caller asks for which j.
 template<int DOF, int NN, int NQ, class T>
-__device__ void local_row(const AssemblyArgs& a, const GroupParams& g, i64 e, int i, T (&v)[NN][DOF * DOF]) {
-    for (int j = 0; j < NN; ++j)
-        for (int k = 0; k < DOF * DOF; ++k) v[j][k] = T(0);
+__device__ void local_entry(const AssemblyArgs& a, const GroupParams& g, i64 e, int i, int j, T (&v)[DOF * DOF]) {
+    for (int k = 0; k < DOF * DOF; ++k) v[k] = T(0);
     if (DOF == 1 && a.mass) {
         // heat_capacity_matrix_2d::integrate_basic_pair (heat_capacity_matrix_2d.hpp:31-37), times
         // density * capacity (:55-56)
         for (int q = 0; q < NQ; ++q) {
             const T f = T(a.w[q] * a.qN[i * NQ + q]);
             const T jq = T(a.jac[e * NQ + q]);
-            for (int j = 0; j < NN; ++j) v[j][0] += f * T(a.qN[j * NQ + q]) * jq;
+            v[0] += f * T(a.qN[j * NQ + q]) * jq;
         }
-        for (int j = 0; j < NN; ++j) v[j][0] *= T(g.c[0]);
+        v[0] *= T(g.c[0]);
         return;
     }
     for (int q = 0; q < NQ; ++q) {
@@ -490,31 +565,34 @@ __device__ void local_row(const AssemblyArgs& a, const GroupParams& g, i64 e, in
         const T c0 = T(cp[0]), c1 = T(cp[1]), c2 = T(cp[2]);
         const T f = T(a.w[q] / a.jac[e * NQ + q]);
         const T ix = T(a.D[2 * ((e * NN + i) * NQ + q)]), iy = T(a.D[2 * ((e * NN + i) * NQ + q) + 1]);
-        for (int j = 0; j < NN; ++j) {
-            const T jx = T(a.D[2 * ((e * NN + j) * NQ + q)]), jy = T(a.D[2 * ((e * NN + j) * NQ + q) + 1]);
-            if (DOF == 1) {
-                T value;
-                if (g.ckind == NLF_CONDUCTIVITY_ISOTROPIC) value = c0 * (ix * jx + iy * jy);
-                else if (g.ckind == NLF_CONDUCTIVITY_ORTHOTROPIC) value = c0 * ix * jx + c1 * iy * jy;
-                else value = ix * (c0 * jx + c2 * jy) + iy * (c2 * jx + c1 * jy);
-                v[j][0] += f * value;
-            } else {
-                // B_i^T C B_j with C = [[h11, h12, h16], [h12, h22, h26], [h16, h26, h66]] (the
-                // anisotropic integrand, stiffness_matrix_2d.hpp:66-75; isotropic and orthotropic
-                // matrices have h16 = h26 = 0)
-                const T h11 = c0, h12 = c1, h66 = c2, h22 = T(cp[3]), h16 = T(cp[4]), h26 = T(cp[5]);
-                const T wx = f * ix, wy = f * iy;
-                const T s1 = h66 * wy + h16 * wx, s2 = h66 * wx + h26 * wy;
-                v[j][0] += (h11 * wx + h16 * wy) * jx + s1 * jy;
-                v[j][1] += (h12 * wx + h26 * wy) * jy + s1 * jx;
-                v[j][2] += (h12 * wy + h16 * wx) * jx + s2 * jy;
-                v[j][DOF * DOF - 1] += (h22 * wy + h26 * wx) * jy + s2 * jx;
-            }
+        const T jx = T(a.D[2 * ((e * NN + j) * NQ + q)]), jy = T(a.D[2 * ((e * NN + j) * NQ + q) + 1]);
+        if (DOF == 1) {
+            T value;
+            if (g.ckind == NLF_CONDUCTIVITY_ISOTROPIC) value = c0 * (ix * jx + iy * jy);
+            else if (g.ckind == NLF_CONDUCTIVITY_ORTHOTROPIC) value = c0 * ix * jx + c1 * iy * jy;
+            else value = ix * (c0 * jx + c2 * jy) + iy * (c2 * jx + c1 * jy);
+            v[0] += f * value;
+        } else {
+            // B_i^T C B_j with C = [[h11, h12, h16], [h12, h22, h26], [h16, h26, h66]] (the
+            // anisotropic integrand, stiffness_matrix_2d.hpp:66-75; isotropic and orthotropic
+            // matrices have h16 = h26 = 0)
+            const T h11 = c0, h12 = c1, h66 = c2, h22 = T(cp[3]), h16 = T(cp[4]), h26 = T(cp[5]);
+            const T wx = f * ix, wy = f * iy;
+            const T s1 = h66 * wy + h16 * wx, s2 = h66 * wx + h26 * wy;
+            v[0] += (h11 * wx + h16 * wy) * jx + s1 * jy;
+            v[1] += (h12 * wx + h26 * wy) * jy + s1 * jx;
+            v[2] += (h12 * wy + h16 * wx) * jx + s2 * jy;
+            v[DOF * DOF - 1] += (h22 * wy + h26 * wx) * jy + s2 * jx;
         }
     }
     const T p1 = T(g.p1);
-    for (int j = 0; j < NN; ++j)
-        for (int k = 0; k < DOF * DOF; ++k) v[j][k] *= p1;
+    for (int k = 0; k < DOF * DOF; ++k) v[k] *= p1;
+}
+
+// The local row i of element e: local_entry for every j.
+template<int DOF, int NN, int NQ, class T>
+__device__ void local_row(const AssemblyArgs& a, const GroupParams& g, i64 e, int i, T (&v)[NN][DOF * DOF]) {
+    for (int j = 0; j < NN; ++j) local_entry<DOF, NN, NQ, T>(a, g, e, i, j, v[j]);
 }
 
 // Nonlocal integrand row iL of pair (eL, eNL):
@@ -954,8 +1032,9 @@ __global__ __launch_bounds__(THREADS) void k_assemble(AssemblyArgs a) {
 
 // Local-only assembly (assemble_part::LOCAL, or no nonlocal group): one wave per node instead of
 // one workgroup.  A node's rows have at most 64 upper and 64 bound entries here (checked by the
-// host), so lane k owns entry k of each; every lane evaluates the element's local row and adds the
-// blocks whose column it owns, in the node kernel's order (elements ascending, then j), so no
+// host), so lane k owns entry k of each.  The lanes first compute one (element, local column)
+// entry each -- a node's elements times nn of them, 64 at a time -- and the owners then add them,
+// broadcast one after another in element, then column order (the node kernel's order), so no
 // atomics and no barriers are needed.
 template<int DOF, int NN, int NQ, class T>
 __global__ __launch_bounds__(256) void k_assemble_local(AssemblyArgs a) {
@@ -980,22 +1059,33 @@ __global__ __launch_bounds__(256) void k_assemble_local(AssemblyArgs a) {
         ab[d] = T(0);
         rinner[d] = a.inner[rows[d]];
     }
-    for (i32 k = a.ne_off[node]; k < a.ne_off[node + 1]; ++k) {
-        const i32 eL = a.ne_ids[k];
-        int iL = 0;
-        for (int i = 0; i < NN; ++i) if (a.el[i64(eL) * NN + i] == node) iL = i;
-        const GroupParams g = a.groups[a.group_of[eL]];
-        T v[NN][DOF * DOF];
-        local_row<DOF, NN, NQ, T>(a, g, eL, iL, v);
-        for (int j = 0; j < NN; ++j) {
-            const i64 m = a.el[i64(eL) * NN + j];
+    const i32 k0 = a.ne_off[node];
+    const int P = (a.ne_off[node + 1] - k0) * NN;
+    for (int p0 = 0; p0 < P; p0 += 64) {
+        T vb[DOF * DOF];
+        i32 mcol = -1;
+        if (p0 + lane < P) {
+            const i32 eL = a.ne_ids[k0 + (p0 + lane) / NN];
+            const int j = (p0 + lane) % NN;
+            int iL = 0;
+            for (int i = 0; i < NN; ++i) if (a.el[i64(eL) * NN + i] == node) iL = i;
+            local_entry<DOF, NN, NQ, T>(a, a.groups[a.group_of[eL]], eL, iL, j, vb);
+            mcol = a.el[i64(eL) * NN + j];
+        } else {
+            for (int k = 0; k < DOF * DOF; ++k) vb[k] = T(0);
+        }
+        const int cnt = P - p0 < 64 ? P - p0 : 64;
+        for (int q = 0; q < cnt; ++q) {
+            const i64 m = __shfl(mcol, q);
+            T v[DOF * DOF];
+            for (int k = 0; k < DOF * DOF; ++k) v[k] = __shfl(vb[k], q);
             for (int d = 0; d < DOF; ++d)
                 for (int dc = 0; dc < DOF; ++dc) {
                     const i64 c = i64(DOF) * m + dc;
                     if (a.inner[c]) {                  // INNER: inner row, c >= row (upper storage)
-                        if (rinner[d] && c >= rows[d] && cu[d] == i32(c)) au[d] += v[j][d * DOF + dc];
+                        if (rinner[d] && c >= rows[d] && cu[d] == i32(c)) au[d] += v[d * DOF + dc];
                     } else if (c != rows[d]) {         // BOUND
-                        if (cb[d] == i32(c)) ab[d] += v[j][d * DOF + dc];
+                        if (cb[d] == i32(c)) ab[d] += v[d * DOF + dc];
                     }
                 }
         }
@@ -1795,56 +1885,85 @@ int nlf_assemble_ex(nlf_mesh m, int physics, const nlf_model* models, const void
         DevBuf<int> overflow{1};
         NLF_HIP(hipMemsetAsync(overflow.p, 0, sizeof(int), s));
         pa.cnt = mat->outer.p; pa.bcnt = mat->b_outer.p; pa.overflow = overflow.p;
-        DevBuf<i32> el_min{size_t(m->E)}, lo_e{size_t(m->E)}, hi_e{size_t(m->E)};
-        hipLaunchKernelGGL(k_elem_minnode, dim3(blocks_for(m->E, 256)), dim3(256), 0, s, m->E, m->nn, m->el.p, el_min.p);
-        hipLaunchKernelGGL(k_elem_span, dim3(blocks_for(m->E, 4)), dim3(256), 0, s, pa, m->E, el_min.p,
-                           m->el_maxnode.p, lo_e.p, hi_e.p);
-        pa.lo_e = lo_e.p; pa.hi_e = hi_e.p;
-        DevBuf<unsigned int> d_words{1};
-        NLF_HIP(hipMemsetAsync(d_words.p, 0, sizeof(unsigned int), s));
-        hipLaunchKernelGGL(k_node_words, dim3(blocks_for(m->N, 256)), dim3(256), 0, s, m->N, m->ne_off.p, m->ne_ids.p,
-                           lo_e.p, hi_e.p, d_words.p);
-        NLF_HIP(hipGetLastError());
-        unsigned int words = 0;
-        download(&words, d_words.p, 1, s);
-        NLF_CHECK(int(words) <= PATTERN_MAX_WORDS, NLF_ERR_UNSUPPORTED,
-                  "column node ids of a row span more than 393k ids (renumber the mesh)");
-        const size_t lds_bitmap = std::max<size_t>(words, 1) * sizeof(unsigned int);
-        const int pth = m->ctx->pattern_threads;
-        // pass 0 keeps every node's bitmap (words per node at most the widest span) when that takes
-        // at most 8 GB, so that pass 1 emits the columns without rebuilding it
-        DevBuf<unsigned int> saved;
-        if (size_t(m->N) * std::max<size_t>(words, 1) * sizeof(unsigned int) <= (size_t(8) << 30)) {
-            saved.alloc(size_t(m->N) * std::max<size_t>(words, 1));
-            pa.saved = saved.p;
-            pa.stride = int(std::max<unsigned>(words, 1));
-        }
         // thermal Q4 nonlocal assembly takes the MFMA pair path: its chunk plan needs the neighbour
         // offsets on the host and is computed while pass 0 of the pattern runs
         const bool pairs_path = physics == 0 && m->nn == 4 && any_nonlocal && !only_local && m->ctx->assembly_pairs != 0;
-        std::vector<i64> nb_off_h;
-        if (pairs_path) {
-            nb_off_h.resize(size_t(m->E + 1));
-            download(nb_off_h.data(), m->nb_off.p, nb_off_h.size(), s);
-        }
-        hipLaunchKernelGGL(k_pattern<0>, dim3(unsigned(m->N)), dim3(pth), lds_bitmap, s, pa);
-        NLF_HIP(hipGetLastError());
         PairPlan plan;
-        if (pairs_path) plan = plan_pairs(m, nb_off_h, 16 * i64(sizeof(double)));
-        int ovf = 0;
-        download(&ovf, overflow.p, 1, s);
-        NLF_CHECK(!ovf, NLF_ERR_UNSUPPORTED, "column node ids of a row span more than 393k ids (renumber the mesh)");
-        exclusive_scan_i64(mat->outer.p, rows, s);
-        exclusive_scan_i64(mat->b_outer.p, rows, s);
-        download(&mat->nnz_full, mat->outer.p + rows, 1, s);
-        download(&mat->nnz_bound, mat->b_outer.p + rows, 1, s);
-        NLF_CHECK(mat->nnz_full < (i64(1) << 40), NLF_ERR_ARGUMENT, "pattern too large");
-        mat->cols.alloc(size_t(std::max<i64>(mat->nnz_full, 1)));
-        mat->b_cols.alloc(size_t(std::max<i64>(mat->nnz_bound, 1)));
-        pa.outer = mat->outer.p; pa.b_outer = mat->b_outer.p; pa.cols = mat->cols.p; pa.b_cols = mat->b_cols.p;
-        pa.diag = mat->diag.p;
-        hipLaunchKernelGGL(k_pattern<1>, dim3(unsigned(m->N)), dim3(pth), lds_bitmap, s, pa);
-        NLF_HIP(hipGetLastError());
+        // no nonlocal group: the columns of a node are the nodes of its own elements (k_pattern_local)
+        bool local_done = false;
+        if (!any_nonlocal) {
+            const unsigned lb = blocks_for(m->N, PL_THREADS);
+            hipLaunchKernelGGL(k_pattern_local<0>, dim3(lb), dim3(PL_THREADS), 0, s, pa);
+            NLF_HIP(hipGetLastError());
+            int lovf = 0;
+            download(&lovf, overflow.p, 1, s);
+            if (!lovf) {
+                exclusive_scan_i64(mat->outer.p, rows, s);
+                exclusive_scan_i64(mat->b_outer.p, rows, s);
+                download(&mat->nnz_full, mat->outer.p + rows, 1, s);
+                download(&mat->nnz_bound, mat->b_outer.p + rows, 1, s);
+                mat->cols.alloc(size_t(std::max<i64>(mat->nnz_full, 1)));
+                mat->b_cols.alloc(size_t(std::max<i64>(mat->nnz_bound, 1)));
+                pa.outer = mat->outer.p; pa.b_outer = mat->b_outer.p; pa.cols = mat->cols.p; pa.b_cols = mat->b_cols.p;
+                pa.diag = mat->diag.p;
+                hipLaunchKernelGGL(k_pattern_local<1>, dim3(lb), dim3(PL_THREADS), 0, s, pa);
+                NLF_HIP(hipGetLastError());
+                local_done = true;
+            } else {                                   // a node with too many column nodes: the bitmap kernels
+                NLF_HIP(hipMemsetAsync(mat->outer.p, 0, mat->outer.bytes(), s));
+                NLF_HIP(hipMemsetAsync(mat->b_outer.p, 0, mat->b_outer.bytes(), s));
+                NLF_HIP(hipMemsetAsync(overflow.p, 0, sizeof(int), s));
+            }
+        }
+        if (!local_done) {
+            DevBuf<i32> el_min{size_t(m->E)}, lo_e{size_t(m->E)}, hi_e{size_t(m->E)};
+            hipLaunchKernelGGL(k_elem_minnode, dim3(blocks_for(m->E, 256)), dim3(256), 0, s, m->E, m->nn, m->el.p, el_min.p);
+            hipLaunchKernelGGL(k_elem_span, dim3(blocks_for(m->E, 4)), dim3(256), 0, s, pa, m->E, el_min.p,
+                               m->el_maxnode.p, lo_e.p, hi_e.p);
+            pa.lo_e = lo_e.p; pa.hi_e = hi_e.p;
+            DevBuf<unsigned int> d_words{1};
+            NLF_HIP(hipMemsetAsync(d_words.p, 0, sizeof(unsigned int), s));
+            hipLaunchKernelGGL(k_node_words, dim3(blocks_for(m->N, 256)), dim3(256), 0, s, m->N, m->ne_off.p, m->ne_ids.p,
+                               lo_e.p, hi_e.p, d_words.p);
+            NLF_HIP(hipGetLastError());
+            unsigned int words = 0;
+            download(&words, d_words.p, 1, s);
+            NLF_CHECK(int(words) <= PATTERN_MAX_WORDS, NLF_ERR_UNSUPPORTED,
+                      "column node ids of a row span more than 393k ids (renumber the mesh)");
+            const size_t lds_bitmap = std::max<size_t>(words, 1) * sizeof(unsigned int);
+            const int pth = m->ctx->pattern_threads;
+            // pass 0 keeps every node's bitmap (words per node at most the widest span) when that takes
+            // at most 8 GB, so that pass 1 emits the columns without rebuilding it
+            DevBuf<unsigned int> saved;
+            if (size_t(m->N) * std::max<size_t>(words, 1) * sizeof(unsigned int) <= (size_t(8) << 30)) {
+                saved.alloc(size_t(m->N) * std::max<size_t>(words, 1));
+                pa.saved = saved.p;
+                pa.stride = int(std::max<unsigned>(words, 1));
+            }
+            std::vector<i64> nb_off_h;
+            if (pairs_path) {
+                nb_off_h.resize(size_t(m->E + 1));
+                download(nb_off_h.data(), m->nb_off.p, nb_off_h.size(), s);
+            }
+            hipLaunchKernelGGL(k_pattern<0>, dim3(unsigned(m->N)), dim3(pth), lds_bitmap, s, pa);
+            NLF_HIP(hipGetLastError());
+            if (pairs_path) plan = plan_pairs(m, nb_off_h, 16 * i64(sizeof(double)));
+            int ovf = 0;
+            download(&ovf, overflow.p, 1, s);
+            NLF_CHECK(!ovf, NLF_ERR_UNSUPPORTED, "column node ids of a row span more than 393k ids (renumber the mesh)");
+            exclusive_scan_i64(mat->outer.p, rows, s);
+            exclusive_scan_i64(mat->b_outer.p, rows, s);
+            download(&mat->nnz_full, mat->outer.p + rows, 1, s);
+            download(&mat->nnz_bound, mat->b_outer.p + rows, 1, s);
+            NLF_CHECK(mat->nnz_full < (i64(1) << 40), NLF_ERR_ARGUMENT, "pattern too large");
+            mat->cols.alloc(size_t(std::max<i64>(mat->nnz_full, 1)));
+            mat->b_cols.alloc(size_t(std::max<i64>(mat->nnz_bound, 1)));
+            pa.outer = mat->outer.p; pa.b_outer = mat->b_outer.p; pa.cols = mat->cols.p; pa.b_cols = mat->b_cols.p;
+            pa.diag = mat->diag.p;
+            hipLaunchKernelGGL(k_pattern<1>, dim3(unsigned(m->N)), dim3(pth), lds_bitmap, s, pa);
+            NLF_HIP(hipGetLastError());
+            NLF_HIP(hipStreamSynchronize(s));          // before the spans and saved bitmaps are released
+        }
         NLF_HIP(hipStreamSynchronize(s));
         if (any_nonlocal && !only_local && !m->nb_symmetric) symmetrize_pattern(mat.get(), s);
         const auto t1 = std::chrono::steady_clock::now();
