@@ -383,6 +383,9 @@ int nlf_batch_create(nlf_matrix a, int count, const int64_t* row_begin, nlf_batc
 int nlf_batch_destroy(nlf_batch b);
 /* picks each instance's SpMV tile (lanes per row, rows per block) by timing; cache_path may be NULL */
 int nlf_batch_autotune(nlf_batch b, const char* cache_path, double* chosen);
+/* No reference counterpart.  ILLT (IC(0) of the batch's local matrix, block diagonal like the batch) for the
+ * batch's solves, or NULL for Jacobi (the default); the handle must outlive the solves. */
+int nlf_batch_set_preconditioner(nlf_batch b, nlf_precond p);
 int nlf_batch_solve(nlf_batch b, const double* b_dev, double* x_dev, double tol, int64_t max_it, int mode, int use_graph,
                     int64_t* iterations, double* true_residual, nlf_solve_stats* stats);
 

@@ -40,6 +40,7 @@ SIGNATURES = {
     "nlf_find_neighbours": (_int, [_p, _f64p, _int, _f64p]),
     "nlf_find_neighbours_ex": (_int, [_p, _f64p, _p, _int, _f64p]),
     "nlf_matrix_symv_info": (_int, [_p, _i64p, _i64p]),
+    "nlf_batch_set_preconditioner": (_int, [_p, _p]),
     "nlf_neighbours_count": (_int, [_p, _i64p]),
     "nlf_neighbours_export": (_int, [_p, _i64p, _i32p]),
     "nlf_neighbours_device": (_int, [_p, _pp, _pp]),

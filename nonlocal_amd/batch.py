@@ -57,7 +57,9 @@ def concatenate(problems):
 class BatchSolver:
     """Assemble the batch once, then solve every instance together."""
 
-    def __init__(self, problems, ctx: api.Context | None = None):
+    def __init__(self, problems, ctx: api.Context | None = None, precond=_abi.NLF_PRECOND_JACOBI):
+        """precond NLF_PRECOND_JACOBI (of the matrix) or NLF_PRECOND_ILLT (IC(0) of the batch's local
+        matrix: block diagonal, one sweep applies every instance's factor)."""
         ctx = ctx or api.default_context()
         self.ctx = ctx
         self.problems = problems
@@ -82,6 +84,11 @@ class BatchSolver:
         rb = np.ascontiguousarray(node_begin, dtype=np.int64)
         ctx.check(load().nlf_batch_create(self.K.h, len(problems), rb.ctypes.data_as(_i64p), C.byref(h)))
         self.h = h
+        self.KL = self.P = None
+        if precond == _abi.NLF_PRECOND_ILLT:
+            self.KL = api._assemble(self.mesh, 0, models, mats, inner, part=_abi.NLF_PART_LOCAL)
+            self.P = api.preconditioner(self.KL, _abi.NLF_PRECOND_ILLT)
+            ctx.check(load().nlf_batch_set_preconditioner(self.h, self.P.h))
         ctx.synchronize()
         self.setup_seconds = time.perf_counter() - t0
         self.target = None

@@ -1,6 +1,7 @@
 // Batched solves: many independent instances stored as the diagonal blocks of one matrix (instance
-// i owns rows [row_begin[i], row_begin[i+1])), solved by lockstep Jacobi-preconditioned CG with
-// per-instance scalars.  Compiled as part of nlf_solver.hip (shares block_sum / shfl helpers).
+// i owns rows [row_begin[i], row_begin[i+1])), solved by lockstep preconditioned CG with
+// per-instance scalars: Jacobi, or an IC(0) (ILLT) preconditioner of the batch's local matrix, whose
+// factor is block diagonal like the matrix, so one level-scheduled sweep applies every instance's.  Compiled as part of nlf_solver.hip (shares block_sum / shfl helpers).
 //
 // Every launch covers all instances: a block never straddles two instances, partial dot products
 // are written per block and summed per instance in a fixed order (bitwise reproducible), and a
@@ -125,12 +126,33 @@ __global__ __launch_bounds__(BATCH_VROWS) void k_bupdate(const i32* __restrict__
         x[row] += a * p[row];
         const double ri = r[row] - a * q[row];
         r[row] = ri;
-        const double zi = invd[row] * ri;
-        z[row] = zi;
-        d = ri * zi;
+        if (invd) {                               // Jacobi; else z = M r and r.z follow (k_bdot)
+            const double zi = invd[row] * ri;
+            z[row] = zi;
+            d = ri * zi;
+        }
     }
     d = block_sum<double>(d, sh);
     if (threadIdx.x == 0) partial[blockIdx.x] = d;
+}
+
+// partial r.z per vector block of the live instances
+__global__ __launch_bounds__(BATCH_VROWS) void k_bdot(const i32* __restrict__ vinst, const i64* __restrict__ vrow0,
+                                                      const i64* __restrict__ row_begin, const double* __restrict__ r,
+                                                      const double* __restrict__ z, double* __restrict__ partial,
+                                                      const BatchState* __restrict__ st) {
+    __shared__ double sh[BATCH_VROWS / 64];
+    const int inst = vinst[blockIdx.x];
+    const i64 row = vrow0[blockIdx.x] + threadIdx.x;
+    double d = 0.0;
+    if (!st[inst].done && row < row_begin[inst + 1]) d = r[row] * z[row];
+    d = block_sum<double>(d, sh);
+    if (threadIdx.x == 0) partial[blockIdx.x] = d;
+}
+
+// rt = b - q over all rows (the true residual the preconditioner is applied to at a refresh)
+__global__ void k_bsub(i64 n, const double* __restrict__ b, const double* __restrict__ q, double* __restrict__ rt) {
+    for (i64 i = i64(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += i64(gridDim.x) * blockDim.x) rt[i] = b[i] - q[i];
 }
 
 // rz_new -> beta, iteration count and the reference criterion sqrt(r'z)/||b|| <= tol
@@ -161,24 +183,25 @@ __global__ __launch_bounds__(BATCH_VROWS) void k_bp(const i32* __restrict__ vins
     if (!st[inst].done && row < row_begin[inst + 1]) p[row] = z[row] + st[inst].beta * p[row];
 }
 
-// residual refresh: r_true = b - q (q = A x), z = invd r_true, partials r_true.z, b.b,
+// residual refresh: r_true = b - q (q = A x), z = invd r_true (or zt), partials r_true.z, b.b,
 // r_true.r_true, |r_true - r|^2 and |r|^2 (deviation of the recursive residual r).  r and z are
 // overwritten for instances being replaced (all of them at the start, where p = z as well).
 __global__ __launch_bounds__(BATCH_VROWS) void k_bresidual(const i32* __restrict__ vinst, const i64* __restrict__ vrow0,
                                                            const i64* __restrict__ row_begin, const double* __restrict__ b,
                                                            const double* __restrict__ q, double* __restrict__ r,
                                                            double* __restrict__ z, double* __restrict__ p,
-                                                           const double* __restrict__ invd, double* __restrict__ prz,
-                                                           double* __restrict__ pbb, double* __restrict__ prr,
-                                                           double* __restrict__ pdev, double* __restrict__ prec,
-                                                           const BatchState* __restrict__ st, int start) {
+                                                           const double* __restrict__ invd, const double* __restrict__ zt,
+                                                           double* __restrict__ prz, double* __restrict__ pbb,
+                                                           double* __restrict__ prr, double* __restrict__ pdev,
+                                                           double* __restrict__ prec, const BatchState* __restrict__ st,
+                                                           int start) {
     __shared__ double sh[BATCH_VROWS / 64];
     const int inst = vinst[blockIdx.x];
     const i64 row = vrow0[blockIdx.x] + threadIdx.x;
     double a = 0.0, c = 0.0, e = 0.0, dv = 0.0, rc = 0.0;
     if ((start || !st[inst].done) && row < row_begin[inst + 1]) {
         const double ri = b[row] - q[row];
-        const double zi = invd[row] * ri;
+        const double zi = zt ? zt[row] : invd[row] * ri;       // zt = M (b - q) when M is not Jacobi
         if (!start) {
             const double ro = r[row];
             dv = (ri - ro) * (ri - ro);
@@ -375,6 +398,7 @@ struct nlf_batch_s {
     nlf::DevBuf<double> invd;
     nlf::DevBuf<float> invd32;
     double tune_seconds = 0.0;
+    nlf_precond precond = nullptr;       // ILLT of the batch's local matrix (nlf_batch_set_preconditioner), or Jacobi
 };
 
 namespace nlf {
@@ -392,12 +416,20 @@ i64 batch_cg(nlf_batch B, const V* val, const double* b, double* x, double tol, 
     const size_t np = std::max<size_t>(P.sp_slots, P.v_blocks);
     DevBuf<double> pa{np}, pb{np}, pc{np}, pd{np}, pe{np};
     DevBuf<BatchState> st{size_t(count)};
+    const nlf_precond_s* M = B->precond && B->precond->kind == NLF_PRECOND_ILLT ? B->precond : nullptr;
+    DevBuf<double> rt, zt;
+    if (M) { rt.alloc(size_t(n)); zt.alloc(size_t(n)); }
     hipLaunchKernelGGL(k_bstate_init, dim3(blocks_for(count, 64)), dim3(64), 0, s, st.p, count);
     auto refresh = [&](int start) {
         // FP64 true residual of every live instance (and the start values): q = A x with FP64 values
         batch_spmv<double, false>(P, A, A->val.p, x, q.p, nullptr, start ? nullptr : st.p, s);
+        if (M) {
+            hipLaunchKernelGGL(k_bsub, dim3(grid_stride_blocks(n)), dim3(256), 0, s, n, b, q.p, rt.p);
+            precond_apply<double>(M, n, rt.p, zt.p, nullptr, s);
+        }
         hipLaunchKernelGGL(k_bresidual, dim3(P.v_blocks), dim3(BATCH_VROWS), 0, s, P.v_inst.p, P.v_row0.p, P.row_begin.p, b,
-                           q.p, r.p, z.p, p.p, B->invd.p, pa.p, pb.p, pc.p, pd.p, pe.p, (const BatchState*)st.p, start);
+                           q.p, r.p, z.p, p.p, B->invd.p, M ? (const double*)zt.p : nullptr, pa.p, pb.p, pc.p, pd.p, pe.p,
+                           (const BatchState*)st.p, start);
         hipLaunchKernelGGL(k_brefresh, dim3(count), dim3(256), 0, s, pa.p, pb.p, pc.p, pd.p, pe.p, P.v_first.p, P.v_count.p,
                            st.p, tol, max_it, start);
         if (!start)
@@ -408,7 +440,12 @@ i64 batch_cg(nlf_batch B, const V* val, const double* b, double* x, double tol, 
         batch_spmv<V, true>(P, A, val, p.p, q.p, pa.p, st.p, s);
         hipLaunchKernelGGL(k_balpha, dim3(count), dim3(256), 0, s, pa.p, P.sp_first.p, P.sp_count.p, st.p);
         hipLaunchKernelGGL(k_bupdate, dim3(P.v_blocks), dim3(BATCH_VROWS), 0, s, P.v_inst.p, P.v_row0.p, P.row_begin.p, x,
-                           r.p, p.p, q.p, z.p, B->invd.p, pb.p, st.p);
+                           r.p, p.p, q.p, z.p, M ? (const double*)nullptr : B->invd.p, pb.p, st.p);
+        if (M) {
+            precond_apply<double>(M, n, r.p, z.p, nullptr, s);
+            hipLaunchKernelGGL(k_bdot, dim3(P.v_blocks), dim3(BATCH_VROWS), 0, s, P.v_inst.p, P.v_row0.p, P.row_begin.p, r.p,
+                               z.p, pb.p, (const BatchState*)st.p);
+        }
         hipLaunchKernelGGL(k_bbeta, dim3(count), dim3(256), 0, s, pb.p, P.v_first.p, P.v_count.p, st.p, tol, max_it);
         hipLaunchKernelGGL(k_bp, dim3(P.v_blocks), dim3(BATCH_VROWS), 0, s, P.v_inst.p, P.v_row0.p, P.row_begin.p, p.p, z.p,
                            st.p);
@@ -555,7 +592,20 @@ int nlf_batch_autotune(nlf_batch b, const char* cache_path, double* chosen) {
     });
 }
 
-// Lockstep Jacobi-PCG of every instance.  x holds the initial guesses.  mode NLF_MODE_MIXED reads
+// No reference counterpart.  The preconditioner of the batch's solves: an ILLT built on the batch's local
+// matrix (block diagonal like the batch; same rows), or NULL for Jacobi of the matrix (the default).
+int nlf_batch_set_preconditioner(nlf_batch bt, nlf_precond p) {
+    if (!bt) return NLF_ERR_ARGUMENT;
+    return guarded(bt->ctx, [&] {
+        NLF_CHECK(!p || p->kind == NLF_PRECOND_ILLT || p->kind == NLF_PRECOND_JACOBI, NLF_ERR_UNSUPPORTED,
+                  "batched solves take Jacobi or ILLT");
+        NLF_CHECK(!p || p->rows == bt->A->rows, NLF_ERR_ARGUMENT, "the preconditioner has another size than the batch");
+        NLF_CHECK(!p || p->ok, NLF_ERR_NUMERIC, "the preconditioner's factorisation failed");
+        bt->precond = p && p->kind == NLF_PRECOND_ILLT ? p : nullptr;
+    });
+}
+
+// Lockstep preconditioned CG of every instance.  x holds the initial guesses.  mode NLF_MODE_MIXED reads
 // FP32 values in the Krylov SpMV (nlf_matrix_make_fp32 first) with FP64 reliable updates.
 // use_graph: capture the iterations into a hipGraph (1) or launch them eagerly (0).  iterations[i]
 // and true_residual[i] per instance (optional); stats->iterations = the largest count.
@@ -575,7 +625,7 @@ int nlf_batch_solve(nlf_batch bt, const double* b, double* x, double tol, int64_
             replays = batch_cg<double>(bt, bt->A->val.p, b, x, tol, max_it, use_graph != 0, h);
         }
         nlf_solve_stats out{};
-        out.preconditioner = NLF_PRECOND_JACOBI;
+        out.preconditioner = bt->precond && bt->precond->kind == NLF_PRECOND_ILLT ? NLF_PRECOND_ILLT : NLF_PRECOND_JACOBI;
         out.outer_iterations = replays;
         for (size_t i = 0; i < h.size(); ++i) {
             if (iterations) iterations[i] = h[i].it;

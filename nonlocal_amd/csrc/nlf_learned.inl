@@ -1501,7 +1501,7 @@ i64 fcg(hipStream_t s, nlf_matrix A, const V* val, nlf_precond M, const double* 
     DevBuf<double> r{size_t(n)}, z{size_t(n)}, p{size_t(n)}, q{size_t(n)}, pa{np}, pb{np}, sums{2};
     DevBuf<FCGState> st{1};
     auto true_residual = [&]() {
-        residual64(n, A->outer.p, spmv_cols(A), A->val.p, pick_lpr(A), x, b, r.p, pa.p, pb.p, rb, s);
+        residual64(A, x, b, r.p, pa.p, pb.p, rb, s);
         hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(1024), 0, s, pa.p, int(rb), sums.p);
         hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(1024), 0, s, pb.p, int(rb), sums.p + 1);
         double h[2];
@@ -1856,7 +1856,7 @@ int nlf_fcg_solve(nlf_matrix A, nlf_precond M, nlf_precond fallback, const doubl
                 const i64 more = cg.solve(b, x, pcg_tol, left, &out.residual, &breakdown);
                 total += more;
                 left -= more;
-                residual64(A->rows, A->outer.p, spmv_cols(A), A->val.p, pick_lpr(A), x, b, r.p, prr.p, pbb.p, rb, s);
+                residual64(A, x, b, r.p, prr.p, pbb.p, rb, s);
                 hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(1024), 0, s, prr.p, int(rb), sums.p);
                 hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(1024), 0, s, pbb.p, int(rb), sums.p + 1);
                 double h[2];

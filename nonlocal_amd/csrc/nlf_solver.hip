@@ -1379,6 +1379,7 @@ void residual64(i64 n, const i64* outer, SpmvCols cs, const double* val, int lpr
     hipLaunchKernelGGL(k_resid_finish, dim3(nb), dim3(SPMV_THREADS), 0, s, n, b, r, prr, pbb);
 }
 
+
 // Work memory of one CG solve: the vectors r, z, p (n entries of X each, 256-byte aligned), five
 // arrays of per-block partial sums (SPMV_MAX_BLOCKS doubles each: every launch of the solve writes at
 // most that many), two reduced sums and the solver state.
@@ -1398,6 +1399,18 @@ struct CGWork {
 nlf_matrix_s::~nlf_matrix_s() { delete symv; }
 
 namespace nlf {
+// residual64 from the matrix: the product from the symmetric storage when it is built (symv_prepare),
+// else from the full storage.
+void residual64(const nlf_matrix_s* a, const double* x, const double* b, double* r, double* prr, double* pbb, unsigned nb,
+                hipStream_t s) {
+    if (a->symv && a->symv->version == a->val_version && a->ctx->spmv_symmetric) {
+        launch_symv<false>(a, x, r, nullptr, nullptr, s, SPMV_MAX_BLOCKS);
+        hipLaunchKernelGGL(k_resid_finish, dim3(nb), dim3(SPMV_THREADS), 0, s, a->rows, b, r, prr, pbb);
+    } else {
+        residual64(a->rows, a->outer.p, spmv_cols(a), a->val.p, pick_lpr(a), x, b, r, prr, pbb, nb, s);
+    }
+}
+
 
 // Preconditioned CG on device vectors (conjugate_gradient::solve).  Vectors of type X, matrix values V.
 // The work memory (CGWork) is allocated by the object, or given by the caller (no allocation then;
@@ -1990,7 +2003,7 @@ int nlf_cg_solve(nlf_matrix a, nlf_precond p, const double* b, double* x, double
         {
             const unsigned rb = spmv_blocks(64, n);
             DevBuf<double> r{size_t(n)}, prr{rb}, pbb{rb}, sums{2};
-            residual64(n, a->outer.p, spmv_cols(a), a->val.p, pick_lpr(a), x, b, r.p, prr.p, pbb.p, rb, s);
+            residual64(a, x, b, r.p, prr.p, pbb.p, rb, s);
             hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(1024), 0, s, prr.p, int(rb), sums.p);
             hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(1024), 0, s, pbb.p, int(rb), sums.p + 1);
             double h[2];
