@@ -61,6 +61,8 @@ def parse():
     ap.add_argument("--cpu-cg-iterations", type=int, default=40,
                     help="ILLT-CG iterations the oracle runs on the full workload for its per-iteration time")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--batch-precond", choices=("jacobi", "illt"), default="illt",
+                    help="preconditioner of the batched solves (batch64)")
     ap.add_argument("--no-batch", action="store_true",
                     help="leave out the batch64 figures (64 uneven instances) of the default line")
     ap.add_argument("--methods", default="fp64_jacobi,mixed_jacobi,learned_fcg_onelevel,coarse_jacobi,learned_fcg")
@@ -174,10 +176,10 @@ def bench_batch(args):
     captured hipGraph, against 64 sequential plain FP64 ILLT-CG solves of the same instances.  (The
     batch's ILLT option cuts the iterations 2.7x here but its single-workgroup sweep runs the 64
     instances' levels one after another, 22 ms per iteration: slower than Jacobi on this batch.)"""
-    print(json.dumps(batch64_record(args.tol, args.steps, args.warmup)), flush=True)
+    print(json.dumps(batch64_record(args.tol, args.steps, args.warmup, args.batch_precond)), flush=True)
 
 
-def batch64_record(tol, steps, warmup):
+def batch64_record(tol, steps, warmup, precond="illt"):
     """The batch64 measurement as one JSON-ready dict (the --workload batch64 line, and the batch64
     key of the default line)."""
     import numpy as np
@@ -206,7 +208,7 @@ def batch64_record(tol, steps, warmup):
     for k in range(warmup + steps):
         torch.cuda.synchronize()
         t0 = time.perf_counter()
-        bs = batch.BatchSolver(problems, ctx)
+        bs = batch.BatchSolver(problems, ctx, precond=A.NLF_PRECOND_ILLT if precond == "illt" else A.NLF_PRECOND_JACOBI)
         t_asm = time.perf_counter() - t0
         target = bs.autotune(cache)
         t1 = time.perf_counter()
@@ -228,7 +230,7 @@ def batch64_record(tol, steps, warmup):
                                "lambda U(0.5,2), seed 3, solve to %g true residual" % tol,
                    "nodes": int(sum(p.mesh.n_nodes for p in problems)),
                    "nnz_full": int(bs.K.info()["nnz_full"]), "autotune_target_nnz_per_lane": target,
-                   "preconditioner": "Jacobi"},
+                   "preconditioner": "ILLT of the local matrix" if precond == "illt" else "Jacobi"},
         "sequential_fp64_illt_s": round(t_seq, 4), "speedup_vs_sequential": round(t_seq / value, 3),
         "max_rel_err_vs_fp64": max(errs), "max_true_residual": float(res.max()),
         "iterations": {"batch_max": int(its.max()), "batch_sum": int(its.sum()),
@@ -599,7 +601,7 @@ def main():
     if not args.no_batch and args.n == 1000:
         # the batching workload beside the headline (after its timed region; not part of `value`)
         log("[bench] batch64")
-        b = batch64_record(args.tol, 2, 1)
+        b = batch64_record(args.tol, 2, 1, args.batch_precond)
         line["batch64"] = {k: b[k] for k in ("value", "unit", "sequential_fp64_illt_s", "speedup_vs_sequential",
                                              "max_rel_err_vs_fp64", "max_true_residual", "iterations")}
         line["batch64"]["config"] = b["config"]["workload"]

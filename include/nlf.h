@@ -314,10 +314,15 @@ int nlf_precond_create(nlf_matrix a, int kind, nlf_precond* out);
 int nlf_precond_destroy(nlf_precond p);
 int nlf_precond_info(nlf_precond p, int* ok, double* shift, int* fw_levels, int* bw_levels, double* setup_seconds);
 /* No reference counterpart: the layout of the ILLT sweeps, for diagnostics.  Per sweep (forward,
- * then backward): out[0/4] levels as swept (wide levels split), out[1/5] compute width,
+ * then backward): out[0/4] levels as swept (wide levels split; for a block-diagonal matrix, whose
+ * independent blocks are factored and swept side by side, the most of any block), out[1/5] compute width,
  * out[2/6] levels held in the LDS dependency window (0: none), out[3/7] kernel (0 packed,
  * 1 ELL one level at a time, 2 ELL with prefetch). */
 int nlf_precond_sweep_info(nlf_precond p, int32_t out[8]);
+/* No reference counterpart: the independent diagonal blocks an ILLT found in its matrix (contiguous
+ * row ranges with no entry between them, such as a batch's instances, of at least 1024 rows each),
+ * each factored and swept by a workgroup of its own, side by side; 1 for a connected matrix. */
+int nlf_precond_blocks(nlf_precond p, int32_t* blocks);
 int nlf_precond_apply(nlf_precond p, const double* r_dev, double* z_dev);
 int nlf_precond_export_factor(nlf_precond p, double* values, double* scale);
 

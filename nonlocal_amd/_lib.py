@@ -77,6 +77,7 @@ SIGNATURES = {
     "nlf_precond_destroy": (_int, [_p]),
     "nlf_precond_info": (_int, [_p, C.POINTER(_int), _f64p, C.POINTER(_int), C.POINTER(_int), _f64p]),
     "nlf_precond_sweep_info": (_int, [_p, C.POINTER(C.c_int32)]),
+    "nlf_precond_blocks": (_int, [_p, C.POINTER(C.c_int32)]),
     "nlf_precond_apply": (_int, [_p, _p, _p]),
     "nlf_precond_export_factor": (_int, [_p, _f64p, _f64p]),
     "nlf_cg_solve": (_int, [_p, _p, _p, _p, _dbl, _i64, _int, C.POINTER(_abi.SolveStats)]),

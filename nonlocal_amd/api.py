@@ -407,6 +407,13 @@ class preconditioner:
         keys = ("levels", "width", "window_levels", "kernel")
         return {"forward": dict(zip(keys, out[0:4])), "backward": dict(zip(keys, out[4:8]))}
 
+    def blocks(self):
+        """Independent diagonal blocks of an ILLT's matrix, each factored and swept by a workgroup
+        of its own (nlf_precond_blocks)."""
+        out = C.c_int32()
+        self.ctx.check(load().nlf_precond_blocks(self.h, C.byref(out)))
+        return out.value
+
     def computation_info(self):
         return self.info()["ok"]
 

@@ -593,7 +593,8 @@ int nlf_batch_autotune(nlf_batch b, const char* cache_path, double* chosen) {
 }
 
 // No reference counterpart.  The preconditioner of the batch's solves: an ILLT built on the batch's local
-// matrix (block diagonal like the batch; same rows), or NULL for Jacobi of the matrix (the default).
+// matrix (block diagonal like the batch; same rows), or NULL for Jacobi of the matrix (the default).  An
+// ILLT of a block-diagonal matrix factors and sweeps every instance on a workgroup of its own.
 int nlf_batch_set_preconditioner(nlf_batch bt, nlf_precond p) {
     if (!bt) return NLF_ERR_ARGUMENT;
     return guarded(bt->ctx, [&] {

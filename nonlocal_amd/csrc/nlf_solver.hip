@@ -432,6 +432,12 @@ __global__ void k_inv_diag(i64 rows, const i64* __restrict__ diag, const double*
 }
 
 // ---- IC(0) ------------------------------------------------------------------------------------
+// One independent diagonal block of a block-diagonal factor, factored and swept by a workgroup of
+// its own: its level count, sweep window depth and first level offset (into the shared offsets).
+struct SweepBlock {
+    i32 levels, K, lptr_off, pad;
+};
+
 struct LevelArgs {
     i64 rows;
     int levels;
@@ -462,7 +468,15 @@ __global__ void k_scale_from_diag(i64 rows, const i64* __restrict__ diag, const 
 
 // Up-looking IC(0): for row k (all rows i < k it depends on are done and mirrored into row k's
 // lower positions), U_kk = sqrt(a_kk - sum_i U_ik^2), U_kj = (a_kj - sum_{i<k} U_ik U_ij) / U_kk.
-__global__ __launch_bounds__(LEVEL_THREADS) void k_ic_factor(LevelArgs a, double* __restrict__ F, int* __restrict__ fail) {
+// blk: one workgroup per independent diagonal block, over the block's own levels (a.lrows block
+// after block, a.lptr each block's level offsets from blk[b].lptr_off on).
+__global__ __launch_bounds__(LEVEL_THREADS) void k_ic_factor(LevelArgs a, const SweepBlock* __restrict__ blk,
+                                                             double* __restrict__ F, int* __restrict__ fail) {
+    if (blk) {
+        const SweepBlock b = blk[blockIdx.x];
+        a.levels = b.levels;
+        a.lptr += b.lptr_off;
+    }
     for (int L = 0; L < a.levels; ++L) {
         for (i32 t = a.lptr[L] + threadIdx.x; t < a.lptr[L + 1]; t += blockDim.x) {
             const i64 k = a.lrows[t];
@@ -566,6 +580,7 @@ struct EllSweep {
     const X* dinv;
 };
 
+
 // rhs_lvl[rec] = scale[rec] * r[row[rec]] (forward) or y[row[rec]] (backward, scale == nullptr);
 // also clears `clear` (the sweep's output, read through global dependencies).
 template<class X>
@@ -632,8 +647,18 @@ __device__ __forceinline__ void load16(const T* __restrict__ p, T (&v)[N]) {
 }
 
 template<class X, int WE, int CT, int SW>
-__global__ __launch_bounds__(CT + 64 * SW) void k_ic_sweep(EllSweep<X> s, const X* __restrict__ rhs,
-                                                           X* __restrict__ out_lvl, const CGState* __restrict__ st) {
+__global__ __launch_bounds__(CT + 64 * SW) void k_ic_sweep(EllSweep<X> s, const SweepBlock* __restrict__ blk,
+                                                           const X* __restrict__ rhs, X* __restrict__ out_lvl,
+                                                           const CGState* __restrict__ st) {
+    // blk: one sweep per workgroup over the records of independent diagonal blocks (a batch's
+    // instances).  Only the counts and the level offset come from memory: the record pointers stay
+    // kernel arguments, known global, so that their loads are not flat ones.
+    if (blk) {
+        const SweepBlock b = blk[blockIdx.x];
+        s.levels = b.levels;
+        s.K = b.K;
+        s.lptr += b.lptr_off;
+    }
     constexpr int D = sweep_depth<WE, CT>();
     extern __shared__ unsigned char smem_raw[];
     X* win = reinterpret_cast<X*>(smem_raw);
@@ -745,10 +770,16 @@ __global__ void k_sweep_scatter(i64 n, const i32* __restrict__ row, const X* __r
 // outside the window are read back from `out` (written by earlier levels of this workgroup), one
 // level at a time with no prefetch.
 template<class X, int WE>
-__global__ __launch_bounds__(LEVEL_THREADS) void k_ic_sweep_generic(EllSweep<X> s, const X* __restrict__ rhs,
-                                                                    X* __restrict__ out, X* __restrict__ out2,
-                                                                    const X* __restrict__ scale2,
+__global__ __launch_bounds__(LEVEL_THREADS) void k_ic_sweep_generic(EllSweep<X> s, const SweepBlock* __restrict__ blk,
+                                                                    const X* __restrict__ rhs, X* __restrict__ out,
+                                                                    X* __restrict__ out2, const X* __restrict__ scale2,
                                                                     const CGState* __restrict__ st) {
+    if (blk) {
+        const SweepBlock b = blk[blockIdx.x];
+        s.levels = b.levels;
+        s.K = b.K;
+        s.lptr += b.lptr_off;
+    }
     extern __shared__ unsigned char smem_raw[];
     X* win = reinterpret_cast<X*>(smem_raw);
     if (st && st->done) return;
@@ -844,6 +875,10 @@ struct nlf_precond_s {
     nlf::DevBuf<nlf::i64> outer, diag, tpos;
     nlf::DevBuf<nlf::i32> cols, fw_rows, fw_ptr, bw_rows, bw_ptr;
     int fw_levels = 0, bw_levels = 0;
+    // independent diagonal blocks (more than one: fw_rows / fw_ptr hold each block's levels in turn,
+    // and the factorisation runs a workgroup per block)
+    int blocks = 1;
+    nlf::DevBuf<nlf::SweepBlock> fw_blocks;
     nlf::DevBuf<double> F, scale, work;
     nlf::DevBuf<float> work32;
     struct Sweep {
@@ -865,6 +900,11 @@ struct nlf_precond_s {
         nlf::DevBuf<nlf::i32> lptr, row, dep;
         nlf::DevBuf<double> val, dinv, scale;
         nlf::DevBuf<float> val32, dinv32, scale32;
+        // blocks > 1: independent diagonal blocks, one sweep each (one workgroup per block): bdesc
+        // holds each block's levels, window depth and first level offset; levels and K above are
+        // the largest
+        int blocks = 1;
+        nlf::DevBuf<nlf::SweepBlock> bdesc;
         template<class X> nlf::EllSweep<X> get() const;
     } efw, ebw;
     bool ell = false;
@@ -1079,6 +1119,178 @@ static bool build_ell(const std::vector<i64>& outer, const std::vector<i32>& col
     return true;
 }
 
+// Contiguous row ranges with no entry between them: the diagonal blocks of a block-diagonal pattern
+// (a batch of independent instances), merged so that each holds at least min_rows rows; {0, n}
+// for one block.  The pattern is symmetric, so no row after a boundary reaches back across it
+// once no row before it reaches past it.
+static std::vector<i64> independent_blocks(const std::vector<i64>& outer, const std::vector<i32>& cols, i64 min_rows) {
+    const i64 n = i64(outer.size()) - 1;
+    std::vector<i64> b(1, 0);
+    i64 reach = -1;                 // the largest column of the rows so far
+    for (i64 r = 0; r + 1 < n; ++r) {
+        if (outer[r + 1] > outer[r]) reach = std::max<i64>(reach, cols[outer[r + 1] - 1]);
+        if (reach <= r && r + 1 - b.back() >= min_rows) b.push_back(r + 1);
+    }
+    if (n - b.back() < min_rows && b.size() > 1) b.pop_back();    // a short tail joins the last block
+    b.push_back(n);
+    return b;
+}
+
+// The global levels restricted to each block, in order: rows block after block (level order within
+// a block), the blocks' level offsets one after another (global record offsets; empty levels
+// dropped, a row's dependencies stay at lower levels), seg[b] the index of block b's first offset.
+struct BlockLevels {
+    std::vector<i32> rows, ptr, blk;
+    std::vector<i64> seg;
+    std::vector<i32> levels;
+};
+static BlockLevels block_levels(const std::vector<i32>& lrows, const std::vector<i32>& level_ptr,
+                                const std::vector<i64>& bounds) {
+    const i64 n = bounds.back();
+    const int B = int(bounds.size()) - 1;
+    BlockLevels bl;
+    bl.blk.resize(size_t(n));
+    for (int b = 0; b < B; ++b)
+        for (i64 r = bounds[b]; r < bounds[b + 1]; ++r) bl.blk[size_t(r)] = b;
+    bl.rows.resize(size_t(n));
+    std::vector<i64> pos(bounds.begin(), bounds.end() - 1);
+    std::vector<std::vector<i32>> bptr(static_cast<size_t>(B));
+    for (int b = 0; b < B; ++b) bptr[size_t(b)].push_back(i32(bounds[b]));
+    std::vector<int> touched;
+    std::vector<char> is_touched(static_cast<size_t>(B), 0);
+    for (size_t L = 0; L + 1 < level_ptr.size(); ++L) {
+        for (i32 t = level_ptr[L]; t < level_ptr[L + 1]; ++t) {
+            const int b = bl.blk[size_t(lrows[t])];
+            bl.rows[size_t(pos[size_t(b)]++)] = lrows[t];
+            if (!is_touched[size_t(b)]) { is_touched[size_t(b)] = 1; touched.push_back(b); }
+        }
+        for (int b : touched) { bptr[size_t(b)].push_back(i32(pos[size_t(b)])); is_touched[size_t(b)] = 0; }
+        touched.clear();
+    }
+    bl.seg.resize(size_t(B) + 1);
+    bl.levels.resize(size_t(B));
+    for (int b = 0; b < B; ++b) {
+        bl.seg[size_t(b)] = i64(bl.ptr.size());
+        bl.levels[size_t(b)] = i32(bptr[size_t(b)].size()) - 1;
+        bl.ptr.insert(bl.ptr.end(), bptr[size_t(b)].begin(), bptr[size_t(b)].end());
+    }
+    bl.seg[size_t(B)] = i64(bl.ptr.size());
+    return bl;
+}
+
+// The ELL layout for a factor made of independent diagonal blocks: every block gets its own levels
+// and its own sweep, one workgroup each, so that the blocks' levels run side by side instead of one
+// after another.  Records are block after block; all blocks share the compute width W (their widest
+// split level) and the record arrays, each has its level offsets and window depth.  Each record's
+// arithmetic is the single-sweep one, so the results are the same bits.
+static bool build_ell_blocks(const std::vector<i64>& outer, const std::vector<i32>& cols, const std::vector<i64>& diag,
+                             const BlockLevels& bl, bool forward, const double* F, const double* scale, const i64* d_diag,
+                             nlf_precond_s::Ell& el, hipStream_t s) {
+    const i64 n = i64(diag.size());
+    const int B = int(bl.levels.size());
+    for (int b = 0; b < B; ++b)
+        if (bl.levels[size_t(b)] == 0) return false;                // an empty block
+    // the split width, as in build_ell, over every block's levels
+    i32 cap;
+    {
+        i64 c_small = 0, c_large = 0;
+        for (int b = 0; b < B; ++b)
+            for (i64 L = bl.seg[size_t(b)]; L + 1 < bl.seg[size_t(b) + 1]; ++L) {
+                const i64 w = bl.ptr[size_t(L) + 1] - bl.ptr[size_t(L)];
+                c_small += (w + SWEEP_CT_SMALL - 1) / SWEEP_CT_SMALL;
+                c_large += (w + SWEEP_CT_LARGE - 1) / SWEEP_CT_LARGE;
+            }
+        cap = 4 * c_small <= 5 * c_large ? SWEEP_CT_SMALL : SWEEP_CT_LARGE;
+    }
+    // split levels; records keep the block-major order of bl.rows
+    std::vector<i32> lptr;
+    std::vector<i64> seg(static_cast<size_t>(B) + 1, 0);
+    std::vector<int> blevels(static_cast<size_t>(B), 0);
+    std::vector<i32> level_of(static_cast<size_t>(n)), pos_of(static_cast<size_t>(n));
+    int W = 1;
+    for (int b = 0; b < B; ++b) {
+        seg[size_t(b)] = i64(lptr.size());
+        lptr.push_back(bl.ptr[size_t(bl.seg[size_t(b)])]);
+        int lv = 0;
+        for (i64 L = bl.seg[size_t(b)]; L + 1 < bl.seg[size_t(b) + 1]; ++L) {
+            const i32 end = bl.ptr[size_t(L) + 1];
+            for (i32 x = bl.ptr[size_t(L)]; x < end; x += cap) {
+                const i32 y = std::min(x + cap, end);
+                for (i32 t = x; t < y; ++t) {
+                    level_of[size_t(bl.rows[size_t(t)])] = lv;
+                    pos_of[size_t(bl.rows[size_t(t)])] = t - x;
+                }
+                W = std::max(W, y - x);
+                lptr.push_back(y);
+                ++lv;
+            }
+        }
+        blevels[size_t(b)] = lv;
+    }
+    seg[size_t(B)] = i64(lptr.size());
+    int K = 1, WEmax = 0;
+    std::vector<int> bK(static_cast<size_t>(B), 1);
+    for (i64 t = 0; t < n; ++t) {
+        const i32 k = bl.rows[size_t(t)];
+        const int b = bl.blk[size_t(k)];
+        const i64 pb = forward ? outer[k] : diag[k] + 1, pe = forward ? diag[k] : outer[k + 1];
+        WEmax = std::max(WEmax, int(pe - pb));
+        for (i64 p = pb; p < pe; ++p) {
+            if (bl.blk[size_t(cols[p])] != b) return false;         // not block diagonal
+            bK[size_t(b)] = std::max(bK[size_t(b)], level_of[size_t(k)] - level_of[size_t(cols[p])] + 1);
+        }
+    }
+    for (int b = 0; b < B; ++b) K = std::max(K, bK[size_t(b)]);
+    int WE = 4;
+    while (WE < WEmax) WE *= 2;
+    if (W > LEVEL_THREADS || WE > 32) return false;
+    const int threads = std::min(LEVEL_THREADS, std::max(64, (W + 63) / 64 * 64));
+    W = threads;
+    const bool window = size_t(K) * size_t(W) * sizeof(double) <= IC_WINDOW_BYTES;
+    std::vector<i32> dep(size_t(WE) * size_t(n));
+    std::vector<i64> src(size_t(WE) * size_t(n), -1);
+    for (i64 t = 0; t < n; ++t) {
+        const i32 k = bl.rows[size_t(t)];
+        const int Kb = bK[size_t(bl.blk[size_t(k)])];
+        const i64 pb = forward ? outer[k] : diag[k] + 1, pe = forward ? diag[k] : outer[k + 1];
+        for (int j = 0; j < WE; ++j) {
+            const size_t e = size_t(WE) * size_t(t) + size_t(j);
+            if (pb + j < pe) {
+                const i32 i = cols[pb + j];
+                dep[e] = window ? (level_of[size_t(i)] % Kb) * W + pos_of[size_t(i)] : -(i + 1);
+                src[e] = pb + j;
+            } else {
+                dep[e] = window ? 0 : -1;           // value 0 times a finite slot
+            }
+        }
+    }
+    el.levels = *std::max_element(blevels.begin(), blevels.end());
+    el.K = window ? K : 0;
+    el.W = W;
+    el.WE = WE;
+    el.threads = threads;
+    el.rows = n;
+    el.blocks = B;
+    upload(el.lptr, lptr.data(), lptr.size(), s);
+    upload(el.row, bl.rows.data(), bl.rows.size(), s);
+    upload(el.dep, dep.data(), dep.size(), s);
+    DevBuf<i64> d_src;
+    upload(d_src, src.data(), src.size(), s);
+    el.val.alloc(src.size()); el.val32.alloc(src.size());
+    hipLaunchKernelGGL(k_pack_ell, dim3(grid_stride_blocks(i64(src.size()))), dim3(256), 0, s, i64(src.size()), d_src.p,
+                       F, el.val.p, el.val32.p);
+    el.dinv.alloc(size_t(n)); el.dinv32.alloc(size_t(n)); el.scale.alloc(size_t(n)); el.scale32.alloc(size_t(n));
+    hipLaunchKernelGGL(k_pack_rows, dim3(grid_stride_blocks(n)), dim3(256), 0, s, n, el.row.p, d_diag, F, scale, el.dinv.p,
+                       el.dinv32.p, el.scale.p, el.scale32.p);
+    NLF_HIP(hipGetLastError());
+    std::vector<SweepBlock> bd(static_cast<size_t>(B));
+    for (int b = 0; b < B; ++b)
+        bd[size_t(b)] = SweepBlock{blevels[size_t(b)], window ? bK[size_t(b)] : 0, i32(seg[size_t(b)]), 0};
+    upload(el.bdesc, bd.data(), bd.size(), s);
+    NLF_HIP(hipStreamSynchronize(s));
+    return true;
+}
+
 template<class X>
 const X* ell_scale(const nlf_precond_s::Ell& e);
 template<> const double* ell_scale<double>(const nlf_precond_s::Ell& e) { return e.scale.p; }
@@ -1101,16 +1313,17 @@ void ell_sweep(const nlf_precond_s::Ell& e, const X* rhs, X* lvl, X* out, X* out
     const size_t win = ell_window_bytes(e, sizeof(X));
     const size_t offs = size_t(e.levels + 1) * sizeof(i32);
     const EllSweep<X> a = e.template get<X>();
-    const dim3 g(1);
+    const SweepBlock* blk = e.blocks > 1 ? e.bdesc.p : nullptr;
+    const dim3 g(unsigned(e.blocks));
     if (e.levels == 0) return;
     if (!ell_prefetching(e, sizeof(X))) {
         const size_t lds = e.K ? win : 16;
         const dim3 b(unsigned(e.threads));
         switch (e.WE) {
-            case 4: hipLaunchKernelGGL((k_ic_sweep_generic<X, 4>), g, b, lds, s, a, rhs, out, out2, scale2, st); break;
-            case 8: hipLaunchKernelGGL((k_ic_sweep_generic<X, 8>), g, b, lds, s, a, rhs, out, out2, scale2, st); break;
-            case 16: hipLaunchKernelGGL((k_ic_sweep_generic<X, 16>), g, b, lds, s, a, rhs, out, out2, scale2, st); break;
-            default: hipLaunchKernelGGL((k_ic_sweep_generic<X, 32>), g, b, lds, s, a, rhs, out, out2, scale2, st); break;
+            case 4: hipLaunchKernelGGL((k_ic_sweep_generic<X, 4>), g, b, lds, s, a, blk, rhs, out, out2, scale2, st); break;
+            case 8: hipLaunchKernelGGL((k_ic_sweep_generic<X, 8>), g, b, lds, s, a, blk, rhs, out, out2, scale2, st); break;
+            case 16: hipLaunchKernelGGL((k_ic_sweep_generic<X, 16>), g, b, lds, s, a, blk, rhs, out, out2, scale2, st); break;
+            default: hipLaunchKernelGGL((k_ic_sweep_generic<X, 32>), g, b, lds, s, a, blk, rhs, out, out2, scale2, st); break;
         }
         return;
     }
@@ -1119,19 +1332,19 @@ void ell_sweep(const nlf_precond_s::Ell& e, const X* rhs, X* lvl, X* out, X* out
         constexpr int CT = SWEEP_CT_SMALL, SW = SWEEP_SW_SMALL;
         const dim3 b(unsigned(e.threads + 64 * SW));
         switch (e.WE) {
-            case 4: hipLaunchKernelGGL((k_ic_sweep<X, 4, CT, SW>), g, b, lds, s, a, rhs, lvl, st); break;
-            case 8: hipLaunchKernelGGL((k_ic_sweep<X, 8, CT, SW>), g, b, lds, s, a, rhs, lvl, st); break;
-            case 16: hipLaunchKernelGGL((k_ic_sweep<X, 16, CT, SW>), g, b, lds, s, a, rhs, lvl, st); break;
-            default: hipLaunchKernelGGL((k_ic_sweep<X, 32, CT, SW>), g, b, lds, s, a, rhs, lvl, st); break;
+            case 4: hipLaunchKernelGGL((k_ic_sweep<X, 4, CT, SW>), g, b, lds, s, a, blk, rhs, lvl, st); break;
+            case 8: hipLaunchKernelGGL((k_ic_sweep<X, 8, CT, SW>), g, b, lds, s, a, blk, rhs, lvl, st); break;
+            case 16: hipLaunchKernelGGL((k_ic_sweep<X, 16, CT, SW>), g, b, lds, s, a, blk, rhs, lvl, st); break;
+            default: hipLaunchKernelGGL((k_ic_sweep<X, 32, CT, SW>), g, b, lds, s, a, blk, rhs, lvl, st); break;
         }
     } else {
         constexpr int CT = SWEEP_CT_LARGE, SW = SWEEP_SW_LARGE;
         const dim3 b(unsigned(e.threads + 64 * SW));
         switch (e.WE) {
-            case 4: hipLaunchKernelGGL((k_ic_sweep<X, 4, CT, SW>), g, b, lds, s, a, rhs, lvl, st); break;
-            case 8: hipLaunchKernelGGL((k_ic_sweep<X, 8, CT, SW>), g, b, lds, s, a, rhs, lvl, st); break;
-            case 16: hipLaunchKernelGGL((k_ic_sweep<X, 16, CT, SW>), g, b, lds, s, a, rhs, lvl, st); break;
-            default: hipLaunchKernelGGL((k_ic_sweep<X, 32, CT, SW>), g, b, lds, s, a, rhs, lvl, st); break;
+            case 4: hipLaunchKernelGGL((k_ic_sweep<X, 4, CT, SW>), g, b, lds, s, a, blk, rhs, lvl, st); break;
+            case 8: hipLaunchKernelGGL((k_ic_sweep<X, 8, CT, SW>), g, b, lds, s, a, blk, rhs, lvl, st); break;
+            case 16: hipLaunchKernelGGL((k_ic_sweep<X, 16, CT, SW>), g, b, lds, s, a, blk, rhs, lvl, st); break;
+            default: hipLaunchKernelGGL((k_ic_sweep<X, 32, CT, SW>), g, b, lds, s, a, blk, rhs, lvl, st); break;
         }
     }
     const i64 n = i64(e.rows);
@@ -1773,11 +1986,27 @@ int nlf_precond_create(nlf_matrix a, int kind, nlf_precond* out) {
             level_sets(outer, cols, diag, fr, fp, br, bp);
             P->fw_levels = int(fp.size()) - 1;
             P->bw_levels = int(bp.size()) - 1;
+            // a block-diagonal matrix (a batch of independent instances) factors and sweeps block by
+            // block, the blocks side by side: one workgroup each instead of one for all
+            const std::vector<i64> bounds = independent_blocks(outer, cols, std::max<i64>(1024, n / 2048));
+            P->blocks = int(bounds.size()) - 1;
+            BlockLevels bfw, bbw;
+            if (P->blocks > 1) {
+                bfw = block_levels(fr, fp, bounds);
+                bbw = block_levels(br, bp, bounds);
+                std::vector<SweepBlock> bd(static_cast<size_t>(P->blocks));
+                for (int b = 0; b < P->blocks; ++b)
+                    bd[size_t(b)] = SweepBlock{bfw.levels[size_t(b)], 0, i32(bfw.seg[size_t(b)]), 0};
+                upload(P->fw_blocks, bd.data(), bd.size(), s);
+                upload(P->fw_rows, bfw.rows.data(), bfw.rows.size(), s);
+                upload(P->fw_ptr, bfw.ptr.data(), bfw.ptr.size(), s);
+            } else {
+                upload(P->fw_rows, fr.data(), fr.size(), s);
+                upload(P->fw_ptr, fp.data(), fp.size(), s);
+            }
             upload(P->outer, outer.data(), outer.size(), s);
             upload(P->diag, diag.data(), diag.size(), s);
             upload(P->cols, cols.data(), cols.size(), s);
-            upload(P->fw_rows, fr.data(), fr.size(), s);
-            upload(P->fw_ptr, fp.data(), fp.size(), s);
             upload(P->bw_rows, br.data(), br.size(), s);
             upload(P->bw_ptr, bp.data(), bp.size(), s);
             P->tpos.alloc(cols.size());
@@ -1796,19 +2025,27 @@ int nlf_precond_create(nlf_matrix a, int kind, nlf_precond* out) {
                 NLF_HIP(hipMemsetAsync(fail.p, 0, sizeof(int), s));
                 hipLaunchKernelGGL(k_ic_init, dim3(blocks_for(n, 256)), dim3(256), 0, s, n, P->outer.p, P->cols.p, P->diag.p,
                                    a->val.p, P->scale.p, shift, P->F.p);
-                hipLaunchKernelGGL(k_ic_factor, dim3(1), dim3(LEVEL_THREADS), 0, s, fw_args(P.get()), P->F.p, fail.p);
+                hipLaunchKernelGGL(k_ic_factor, dim3(unsigned(P->blocks)), dim3(LEVEL_THREADS), 0, s, fw_args(P.get()),
+                                   P->blocks > 1 ? P->fw_blocks.p : nullptr, P->F.p, fail.p);
                 NLF_HIP(hipGetLastError());
                 int f = 0;
                 download(&f, fail.p, 1, s);
                 if (!f) { P->ok = true; P->shift = shift; break; }
                 shift = shift == 0.0 ? 1e-3 : 2 * shift;
             }
-            build_sweep(outer, cols, diag, fr, fp, true, P->F.p, P->scale.p, P->diag.p, P->fw, s);
-            build_sweep(outer, cols, diag, br, bp, false, P->F.p, P->scale.p, P->diag.p, P->bw, s);
-            const int widest = std::max(P->fw.W, P->bw.W);
-            P->solve_threads = std::min(LEVEL_THREADS, std::max(64, (widest + 63) / 64 * 64));
-            P->ell = build_ell(outer, cols, diag, fr, fp, true, P->F.p, P->scale.p, P->diag.p, P->efw, s) &&
-                     build_ell(outer, cols, diag, br, bp, false, P->F.p, P->scale.p, P->diag.p, P->ebw, s);
+            P->ell = P->blocks > 1 &&
+                     build_ell_blocks(outer, cols, diag, bfw, true, P->F.p, P->scale.p, P->diag.p, P->efw, s) &&
+                     build_ell_blocks(outer, cols, diag, bbw, false, P->F.p, P->scale.p, P->diag.p, P->ebw, s);
+            if (!P->ell)
+                P->ell = build_ell(outer, cols, diag, fr, fp, true, P->F.p, P->scale.p, P->diag.p, P->efw, s) &&
+                         build_ell(outer, cols, diag, br, bp, false, P->F.p, P->scale.p, P->diag.p, P->ebw, s);
+            // the packed sweeps (one workgroup, any level width) where the ELL layout does not apply
+            if (!P->ell) {
+                build_sweep(outer, cols, diag, fr, fp, true, P->F.p, P->scale.p, P->diag.p, P->fw, s);
+                build_sweep(outer, cols, diag, br, bp, false, P->F.p, P->scale.p, P->diag.p, P->bw, s);
+                const int widest = std::max(P->fw.W, P->bw.W);
+                P->solve_threads = std::min(LEVEL_THREADS, std::max(64, (widest + 63) / 64 * 64));
+            }
             if (P->ell) {
                 P->rhs_lvl.alloc(size_t(n)); P->y_lvl.alloc(size_t(n)); P->y1.alloc(size_t(n)); P->y2.alloc(size_t(n));
                 P->rhs_lvl32.alloc(size_t(n)); P->y_lvl32.alloc(size_t(n)); P->y1_32.alloc(size_t(n));
@@ -1858,6 +2095,12 @@ int nlf_precond_sweep_info(nlf_precond p, int32_t out[8]) {
             out[4 * k + 3] = 0;
         }
     }
+    return NLF_OK;
+}
+
+int nlf_precond_blocks(nlf_precond p, int32_t* blocks) {
+    if (!p || !blocks || p->kind != NLF_PRECOND_ILLT) return NLF_ERR_ARGUMENT;
+    *blocks = p->blocks;
     return NLF_OK;
 }
 

@@ -66,3 +66,48 @@ def test_ic_sweep_matches_oracle(gpu_ctx, name, build, width, split):
     # a second apply gives the same bits (the sweep is deterministic)
     z2 = Pg.solve(torch.from_numpy(r).to(gpu_ctx.device)).cpu().numpy()
     assert np.array_equal(z, z2)
+
+
+def test_ic_of_independent_blocks(gpu_ctx):
+    """A block-diagonal matrix (four thermal instances of 1089 to 1681 nodes concatenated as in a
+    batch) is factored and swept block by block, one workgroup per block: its IC(0) is the blocks'
+    own, so z is every instance's oracle apply in turn."""
+    from nonlocal_amd import batch
+    problems = [configs.thermal_q4(n, 0.6, 2.0 / n, 1.0 + 0.1 * k, name=f"i{k}") for k, n in enumerate([35, 32, 40, 33])]
+    mesh, models, mats, bcs, node_begin = batch.concatenate(problems)
+    print(f"[ic sweep] independent blocks: {mesh.n_nodes} nodes in {len(problems)} instances", flush=True)
+    z_ref, r_all = [], []
+    rng = np.random.default_rng(11)
+    for p in problems:
+        om, KL, f, xD, inner = H.oracle_assembly(p, part=A.NLF_PART_LOCAL)
+        P = O.OraclePrecond(A.NLF_PRECOND_ILLT, KL)
+        assert P.ok()[0] and P.ok()[1] == 0.0
+        r = rng.standard_normal(f.size)
+        r_all.append(r)
+        z_ref.append(P.apply(r))
+    z_ref, r_all = np.concatenate(z_ref), np.concatenate(r_all)
+    g = api.mesh_2d(mesh, gpu_ctx)
+    inner = np.concatenate([H.oracle_assembly(p, part=A.NLF_PART_LOCAL)[4] for p in problems])
+    KLg = api._assemble(g, 0, models, mats, inner, part=A.NLF_PART_LOCAL)
+    Pg = api.preconditioner(KLg, A.NLF_PRECOND_ILLT)
+    assert Pg.info()["ok"] and Pg.info()["shift"] == 0.0
+    assert Pg.blocks() == len(problems)
+    info = Pg.sweep_info()
+    print(f"  blocks {Pg.blocks()} sweeps {info}", flush=True)
+    assert info["forward"]["kernel"] == 2 and info["backward"]["kernel"] == 2
+    z = Pg.solve(torch.from_numpy(r_all).to(gpu_ctx.device)).cpu().numpy()
+    err = H.rel_l2(z, z_ref)
+    print(f"  z rel err {err:.2e}", flush=True)
+    assert err <= 1e-12
+    for i in range(len(problems)):
+        a, b = node_begin[i], node_begin[i + 1]
+        assert H.rel_l2(z[a:b], z_ref[a:b]) <= 1e-12, f"instance {i}"
+    assert np.array_equal(z, Pg.solve(torch.from_numpy(r_all).to(gpu_ctx.device)).cpu().numpy())
+
+
+def test_connected_matrix_is_one_block(gpu_ctx):
+    problem = configs.thermal_q4(40, 0.6, 0.05)
+    om, KL, f, xD, inner = H.oracle_assembly(problem, part=A.NLF_PART_LOCAL)
+    g = api.mesh_2d(problem.mesh, gpu_ctx)
+    KLg = api._assemble(g, 0, problem.models, problem.materials, inner, part=A.NLF_PART_LOCAL)
+    assert api.preconditioner(KLg, A.NLF_PRECOND_ILLT).blocks() == 1
