@@ -1982,6 +1982,8 @@ int nlf_assemble_ex(nlf_mesh m, int physics, const nlf_model* models, const void
 }
 
 int nlf_matrix_destroy(nlf_matrix a) {
+    if (!a) return NLF_OK;
+    CtxScope scope(live_ctx(a->ctx));
     delete a;
     return NLF_OK;
 }

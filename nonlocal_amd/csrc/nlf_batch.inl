@@ -526,6 +526,8 @@ int nlf_batch_create(nlf_matrix a, int count, const int64_t* row_begin, nlf_batc
 }
 
 int nlf_batch_destroy(nlf_batch b) {
+    if (!b) return NLF_OK;
+    CtxScope scope(live_ctx(b->ctx));
     delete b;
     return NLF_OK;
 }

@@ -5,7 +5,9 @@
 
 #include <cstdint>
 #include <cstring>
+#include <map>
 #include <memory>
+#include <mutex>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -35,29 +37,59 @@ struct Error : std::runtime_error {
         if (!(cond)) throw ::nlf::Error((code), (msg));                                                 \
     } while (0)
 
-// Owning device buffer (hipMalloc / hipFree).
+// Device memory of the library's buffers.  While an entry point runs on a context (CtxScope, set by
+// guarded() and by the destroy functions), a block released by a buffer of that context goes back
+// to the context's free list instead of hipFree, and later allocations on that context take blocks
+// from there (best fit, at most 25 % larger).  All work of a context is queued on its one stream,
+// so a block taken again is only used by work queued after everything that used it before its
+// release.  hipMalloc of a large block costs up to a millisecond and hipFree waits for the whole
+// device; a workload that assembles and solves again and again reuses its blocks.  Outside a
+// context scope (or for another context's buffer) blocks are plain hipMalloc / hipFree.  The free
+// list holds at most an eighth of the device memory and is emptied when hipMalloc fails and when
+// the context is destroyed.
+struct CtxScope {
+    nlf_ctx prev;
+    explicit CtxScope(nlf_ctx c);
+    ~CtxScope();
+    CtxScope(const CtxScope&) = delete;
+    CtxScope& operator=(const CtxScope&) = delete;
+};
+nlf_ctx live_ctx(nlf_ctx c);                                // c, or null once it is destroyed
+void* device_alloc(size_t bytes, size_t* cap, nlf_ctx* owner);
+void device_free(void* p, size_t cap, nlf_ctx owner);
+
+// Owning device buffer.
 template<class T>
 struct DevBuf {
     T* p = nullptr;
     size_t n = 0;
+    size_t cap = 0;              // bytes of the block
+    nlf_ctx owner = nullptr;     // the context whose free list takes the block back
     DevBuf() = default;
     explicit DevBuf(size_t count) { alloc(count); }
     DevBuf(const DevBuf&) = delete;
     DevBuf& operator=(const DevBuf&) = delete;
-    DevBuf(DevBuf&& o) noexcept : p(o.p), n(o.n) { o.p = nullptr; o.n = 0; }
-    DevBuf& operator=(DevBuf&& o) noexcept { release(); p = o.p; n = o.n; o.p = nullptr; o.n = 0; return *this; }
+    DevBuf(DevBuf&& o) noexcept : p(o.p), n(o.n), cap(o.cap), owner(o.owner) { o.forget(); }
+    DevBuf& operator=(DevBuf&& o) noexcept {
+        release();
+        p = o.p; n = o.n; cap = o.cap; owner = o.owner;
+        o.forget();
+        return *this;
+    }
     ~DevBuf() { release(); }
     void alloc(size_t count) {
         release();
+        if (!count) return;
+        p = static_cast<T*>(device_alloc(count * sizeof(T), &cap, &owner));
         n = count;
-        if (count) NLF_HIP(hipMalloc(&p, count * sizeof(T)));
     }
     void release() {
-        if (p) (void)hipFree(p);
-        p = nullptr;
-        n = 0;
+        if (p) device_free(p, cap, owner);
+        forget();
     }
     size_t bytes() const { return n * sizeof(T); }
+private:
+    void forget() { p = nullptr; n = 0; cap = 0; owner = nullptr; }
 };
 
 template<class T>
@@ -277,6 +309,11 @@ struct nlf_ctx_s {
     int pattern_threads = 128;           // NLF_OPT_PATTERN_THREADS
     int pair_budget_mib = 0;             // NLF_OPT_PAIR_BUDGET_MIB (0: automatic)
     int spmv_symmetric = 1;              // NLF_OPT_SPMV_SYMMETRIC
+    // free device blocks of this context's buffers (device_alloc / device_free)
+    bool alive = true;
+    std::mutex blocks_mu;
+    std::multimap<size_t, void*> free_blocks;
+    size_t cached_bytes = 0, cache_limit = 0;
 };
 
 struct nlf_mesh_s {
@@ -404,6 +441,7 @@ void set_error(nlf_ctx ctx, const std::string& msg);
 // Runs f, translating exceptions into status codes and the context's last error.
 template<class F>
 int guarded(nlf_ctx ctx, F&& f) {
+    CtxScope scope(ctx);
     try {
         f();
         return NLF_OK;

@@ -1783,6 +1783,8 @@ int nlf_trainer_create(nlf_ctx ctx, const float* w, nlf_trainer* out) {
 }
 
 int nlf_trainer_destroy(nlf_trainer t) {
+    if (!t) return NLF_OK;
+    CtxScope scope(live_ctx(t->ctx));
     delete t;
     return NLF_OK;
 }

@@ -186,6 +186,79 @@ __global__ void k_elem_maxnode(i64 E, int nn, const i32* __restrict__ el, i32* _
     mx[e] = m;
 }
 
+// ---- device blocks of a context's buffers ------------------------------------------------------
+static thread_local nlf_ctx t_scope_ctx = nullptr;
+
+CtxScope::CtxScope(nlf_ctx c) : prev(t_scope_ctx) { t_scope_ctx = c; }
+CtxScope::~CtxScope() { t_scope_ctx = prev; }
+
+nlf_ctx live_ctx(nlf_ctx c) { return c && c->alive ? c : nullptr; }
+
+// 2 MiB granules from 1 MiB up (a workload's large buffers repeat their sizes), 512 B below
+static size_t block_bytes(size_t bytes) {
+    const size_t g = bytes >= (size_t(1) << 20) ? (size_t(2) << 20) : 512;
+    return (bytes + g - 1) / g * g;
+}
+
+// Waits for the context's stream, then returns every free block to the device; caller holds blocks_mu.
+static void release_free_blocks(nlf_ctx c) {
+    if (c->free_blocks.empty()) return;
+    (void)hipStreamSynchronize(c->stream);
+    for (auto& kv : c->free_blocks) (void)hipFree(kv.second);
+    c->free_blocks.clear();
+    c->cached_bytes = 0;
+}
+
+void* device_alloc(size_t bytes, size_t* cap, nlf_ctx* owner) {
+    nlf_ctx c = t_scope_ctx;
+    *owner = c;
+    void* p = nullptr;
+    if (!c) {
+        NLF_HIP(hipMalloc(&p, bytes));
+        *cap = bytes;
+        return p;
+    }
+    const size_t want = block_bytes(bytes);
+    {
+        std::lock_guard<std::mutex> lock(c->blocks_mu);
+        auto it = c->free_blocks.lower_bound(want);
+        if (it != c->free_blocks.end() && it->first <= want + want / 4) {
+            p = it->second;
+            *cap = it->first;
+            c->cached_bytes -= it->first;
+            c->free_blocks.erase(it);
+            return p;
+        }
+    }
+    hipError_t e = hipMalloc(&p, want);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();                // the failed allocation is not the next launch's error
+        {
+            std::lock_guard<std::mutex> lock(c->blocks_mu);
+            release_free_blocks(c);
+        }
+        e = hipMalloc(&p, want);
+        if (e != hipSuccess) {
+            (void)hipGetLastError();
+            throw Error(NLF_ERR_HIP, std::string("hipMalloc(") + std::to_string(want) + " bytes): " + hipGetErrorString(e));
+        }
+    }
+    *cap = want;
+    return p;
+}
+
+void device_free(void* p, size_t cap, nlf_ctx owner) {
+    if (owner && owner == t_scope_ctx && owner->alive) {
+        std::lock_guard<std::mutex> lock(owner->blocks_mu);
+        if (owner->cached_bytes + cap <= owner->cache_limit) {
+            owner->free_blocks.emplace(cap, p);
+            owner->cached_bytes += cap;
+            return;
+        }
+    }
+    (void)hipFree(p);
+}
+
 }  // namespace nlf
 
 using namespace nlf;
@@ -208,14 +281,23 @@ int nlf_ctx_create(int device, void* stream, nlf_ctx* out) {
         if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) { delete c; return NLF_ERR_HIP; }
         c->own_stream = true;
     }
+    size_t free_b = 0, total_b = 0;
+    if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) { (void)hipGetLastError(); total_b = 0; }
+    c->cache_limit = total_b / 8;
     *out = c;
     return NLF_OK;
 }
 
 int nlf_ctx_destroy(nlf_ctx ctx) {
-    if (!ctx) return NLF_OK;
+    if (!ctx || !ctx->alive) return NLF_OK;
+    {
+        std::lock_guard<std::mutex> lock(ctx->blocks_mu);
+        release_free_blocks(ctx);
+        ctx->alive = false;
+    }
     if (ctx->own_stream) (void)hipStreamDestroy(ctx->stream);
-    delete ctx;
+    // The structure itself stays allocated (a few hundred bytes): objects made on the context may be
+    // destroyed after it, and their destroy functions read its `alive` flag.
     return NLF_OK;
 }
 
@@ -382,6 +464,8 @@ int nlf_mesh_create(nlf_ctx ctx, int64_t n_nodes, const double* xy, int64_t n_el
 }
 
 int nlf_mesh_destroy(nlf_mesh m) {
+    if (!m) return NLF_OK;
+    CtxScope scope(live_ctx(m->ctx));
     delete m;
     return NLF_OK;
 }

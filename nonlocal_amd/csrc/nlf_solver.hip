@@ -2061,6 +2061,8 @@ int nlf_precond_create(nlf_matrix a, int kind, nlf_precond* out) {
 }
 
 int nlf_precond_destroy(nlf_precond p) {
+    if (!p) return NLF_OK;
+    CtxScope scope(live_ctx(p->ctx));
     delete p;
     return NLF_OK;
 }
