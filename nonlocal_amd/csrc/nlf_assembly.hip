@@ -822,6 +822,12 @@ __global__ __launch_bounds__(256) void k_pairs_mfma32(PairArgs<float> a) {
 
 // T = double: the FP64 value pass (inner and bound parts).  T = float: the FP32 value pass with
 // FP32 accumulation into val32 (inner part only).
+// Neighbours per thread and step where the rows come from the pair blocks (2: measured against 3
+// and 4, which were slower).
+#ifndef NLF_ASM_PAIR_ITEMS
+#define NLF_ASM_PAIR_ITEMS 2
+#endif
+
 template<int DOF, int NN, int NQ, bool INJECTIVE, class T, int THREADS = ASM_THREADS, bool PAIRS = false>
 __global__ __launch_bounds__(THREADS) void k_assemble(AssemblyArgs a) {
     constexpr bool F64 = sizeof(T) == sizeof(double);
@@ -903,7 +909,7 @@ __global__ __launch_bounds__(THREADS) void k_assemble(AssemblyArgs a) {
         // their loads are issued before either is used, and they are added item after item, so the
         // order of the additions is that of ITEMS consecutive single-item steps
         // (PAIRS: the nonlocal rows come from the pair blocks of k_pairs_mfma, thermal Q4 only)
-        constexpr int ITEMS = PAIRS ? 2 : 1;
+        constexpr int ITEMS = PAIRS ? NLF_ASM_PAIR_ITEMS : 1;
         const T* pairs = nullptr;
         if constexpr (F64) pairs = a.pairs; else pairs = a.pairs32;
         const bool from_pairs = PAIRS && nonlocal;

@@ -68,9 +68,14 @@ __global__ void k_unit_diag(i64 from, i64 ncp, double* __restrict__ a) {
     if (i < ncp) a[i * ncp + i] = 1.0;
 }
 
-// In-place Gauss-Jordan inverse of the 64 x 64 diagonal tile (k, k) into dinv (no pivoting); 1024
-// threads, four entries each per pivot step.
-__global__ __launch_bounds__(1024) void k_gj_diag(const double* __restrict__ a, i64 ncp, i64 k, double* __restrict__ dinv) {
+// In-place Gauss-Jordan inverse of the 64 x 64 diagonal tile (k, k) into dinv (no pivoting);
+// GJ_DIAG_THREADS threads, 4096 / GJ_DIAG_THREADS entries each per pivot step (1024: 256 and 64
+// threads were measured slower).
+#ifndef NLF_GJ_DIAG_THREADS
+#define NLF_GJ_DIAG_THREADS 1024
+#endif
+constexpr int GJ_DIAG_THREADS = NLF_GJ_DIAG_THREADS;
+__global__ __launch_bounds__(GJ_DIAG_THREADS) void k_gj_diag(const double* __restrict__ a, i64 ncp, i64 k, double* __restrict__ dinv) {
     __shared__ double d[TB][TB + 1];
     __shared__ double colk[TB];
     const int t = threadIdx.x, T = blockDim.x;
@@ -338,7 +343,7 @@ std::unique_ptr<CoarseSpace> build_coarse(const nlf_matrix_s* a, i64 target, hip
     const i64 nb = cs->ncp / TB;
     DevBuf<double> dinv{size_t(TB * TB)};
     for (i64 k = 0; k < nb; ++k) {
-        hipLaunchKernelGGL(k_gj_diag, dim3(1), dim3(1024), 0, s, cs->inv.p, cs->ncp, k, dinv.p);
+        hipLaunchKernelGGL(k_gj_diag, dim3(1), dim3(GJ_DIAG_THREADS), 0, s, cs->inv.p, cs->ncp, k, dinv.p);
         if (nb > 1) {
             hipLaunchKernelGGL(k_gj_rowpanel, dim3(unsigned(nb - 1)), dim3(256), 0, s, cs->inv.p, cs->ncp, k, dinv.p);
             hipLaunchKernelGGL(k_gj_update_mfma, dim3(unsigned(nb - 1), unsigned(nb - 1)), dim3(256), 0, s, cs->inv.p, cs->ncp, k);
