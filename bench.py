@@ -67,13 +67,14 @@ def parse():
                     help="leave out the batch64 figures (64 uneven instances) of the default line")
     ap.add_argument("--methods", default="fp64_jacobi,mixed_jacobi,learned_fcg_onelevel,coarse_jacobi,learned_fcg")
     ap.add_argument("--workload", default="composite1000",
-                    choices=["composite1000", "batch64", "precision", "spmv", "flux"],
+                    choices=["composite1000", "batch64", "precision", "spmv", "flux", "assembly"],
                     help="composite1000: the 1000x1000 composite (the metric's workload); batch64: 64 uneven "
                          "instances solved together vs one after another; precision: FP32 assembly and "
                          "mixed-precision errors against the FP64 path on every workload; spmv: the FP64 "
                          "SpMV of the composite alone, plus a device copy of known size (HBM counter "
                          "calibration); flux: the heat-flux post-processing (calc_flux) of the composite's "
-                         "solution")
+                         "solution; assembly: the composite's pattern and FP64 assembly of K alone, plus the "
+                         "same copy (counters of the value pass)")
     return ap.parse_args()
 
 
@@ -379,6 +380,50 @@ def bench_spmv(args):
 FP64_PEAK_TFLOPS = 78.6
 
 
+def bench_assembly(args):
+    """The composite's pattern and FP64 assembly of K alone (--steps assemblies after --warmup, each
+    matrix released before the next), then one device-to-device copy of 2^28 doubles: the kernel of
+    known traffic that scales the HBM counters of the value pass (k_pairs_mfma, k_assemble)."""
+    import torch
+    from nonlocal_amd import _abi as A
+    from nonlocal_amd import api, configs
+    ctx = api.default_context()
+    problem = configs.cfg4(args.n)
+    mesh = api.mesh_2d(problem.mesh, ctx)
+    api.find_neighbours(mesh, api._radii(problem.models))
+    inner = np.ones(problem.mesh.n_nodes, dtype=np.uint8)
+    f = np.zeros(problem.mesh.n_nodes)
+    xD = np.zeros(problem.mesh.n_nodes)
+    arr = A.array_of(A.Boundary, problem.bcs)
+    ctx.check(api.load().nlf_boundary_data(mesh.h, 1, arr, api._ptr(f, api._f64p), api._ptr(xD, api._f64p),
+                                           api._ptr(inner, api._u8p)))
+    times, info = [], None
+    for k in range(args.warmup + args.steps):
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        K = api._assemble(mesh, 0, problem.models, problem.materials, inner)
+        ctx.synchronize()
+        if k >= args.warmup:
+            times.append(time.perf_counter() - t0)
+        info = K.info()
+        del K
+        log(f"[assembly] step {k}: {times[-1] if k >= args.warmup else float('nan'):.4f} s")
+    a = torch.empty(1 << 28, dtype=torch.float64, device=ctx.device)
+    b = torch.empty_like(a)
+    a.fill_(1.0)
+    torch.cuda.synchronize()
+    b.copy_(a)
+    torch.cuda.synchronize()
+    value = sum(times) / len(times)
+    line = {"metric": "pattern + FP64 assembly of K, s per assembly", "value": round(value, 4), "unit": "s",
+            "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "higher_is_better": False, "dtype": "f64",
+            "data": "synthetic",
+            "config": {"workload": f"composite {args.n}x{args.n} Q4, pattern and values of K", "rows": info["rows"],
+                       "nnz_full": int(info["nnz_full"]), "nnz_upper": int(info["nnz_upper"])},
+            "calibration_copy_bytes": {"read": int(a.numel() * 8), "written": int(b.numel() * 8)}}
+    print(json.dumps(line), flush=True)
+
+
 def bench_flux(args):
     """Heat flux of the composite's temperature: gradient and -lambda grad T at the 4M quadrature points,
     the nonlocal average over every element's neighbours (the dominant kernel: one influence
@@ -459,6 +504,9 @@ def main():
         return
     if args.workload == "flux":
         bench_flux(args)
+        return
+    if args.workload == "assembly":
+        bench_assembly(args)
         return
     import torch
     from nonlocal_amd import _abi as A
