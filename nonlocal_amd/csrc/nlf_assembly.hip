@@ -1399,7 +1399,7 @@ GroupParams make_group(const nlf_model& md, int physics, const void* materials, 
 
 // 16-bit column offsets for the SpMV when every |col - row| < 2^15, and the row-pattern dictionary
 // over them (NLF_OPT_ROW_PATTERNS); any earlier ones are dropped first.
-void finish_spmv_storage(nlf_matrix_s* mat, hipStream_t s) {
+static void finish_spmv_storage(nlf_matrix_s* mat, hipStream_t s) {
     const i64 rows = mat->rows;
     mat->off16.release();
     mat->pat.release();
@@ -1415,8 +1415,20 @@ void finish_spmv_storage(nlf_matrix_s* mat, hipStream_t s) {
         hipLaunchKernelGGL(k_to_off16, dim3(blocks_for(rows, 4)), dim3(256), 0, s, rows, mat->outer.p, mat->cols.p,
                            mat->off16.p);
         NLF_HIP(hipGetLastError());
-        if (mat->ctx->row_patterns) build_row_patterns(mat, s);
+        if (mat->row_patterns) build_row_patterns(mat, s);
     }
+}
+
+// The full-storage SpMV's layout is built on the first product that reads it, not with the values:
+// the FP64 solvers read the symmetric slice storage instead, and a matrix used only by them never
+// needs it.  A new pattern or a rebuilt matrix drops it.
+static void drop_spmv_storage(nlf_matrix_s* mat) {
+    mat->off16.release();
+    mat->pat.release();
+    mat->pstart.release();
+    mat->pat_size = mat->pat_groups = 0;
+    mat->spmv_storage_ready = false;
+    mat->row_patterns = mat->ctx->row_patterns;      // the option in force at assembly applies
 }
 
 // ---- the Neumann problem's Lagrange row and column (conductivity_matrix_2d.hpp:39-72, 83-104) ----
@@ -1510,7 +1522,7 @@ void border_neumann(nlf_matrix_s* mat, hipStream_t s) {
     mat->rows = n + 1;
     mat->nnz_full = nnz2;
     mat->neumann = true;
-    finish_spmv_storage(mat, s);
+    drop_spmv_storage(mat);
     NLF_HIP(hipStreamSynchronize(s));
 }
 
@@ -1768,7 +1780,7 @@ void assemble_values(nlf_matrix_s* mat, hipStream_t s, const PairPlan* plan = nu
     hipLaunchKernelGGL(k_mirror<T>, dim3(blocks_for(rows, 4)), dim3(256), 0, s, rows, mat->outer.p, mat->cols.p,
                        mat->diag.p, values);
     NLF_HIP(hipGetLastError());
-    if constexpr (F64) finish_spmv_storage(mat, s);
+    if constexpr (F64) drop_spmv_storage(mat);
     NLF_HIP(hipStreamSynchronize(s));
 }
 
@@ -1796,6 +1808,17 @@ void add_pairs_device(nlf_matrix_s* a, const std::vector<i64>& rc, const std::ve
 }
 
 }  // namespace
+
+namespace nlf {
+void ensure_spmv_storage(const nlf_matrix_s* a) {
+    if (a->spmv_storage_ready) return;
+    nlf_matrix_s* m = const_cast<nlf_matrix_s*>(a);
+    hipStream_t s = m->ctx->stream;
+    finish_spmv_storage(m, s);
+    NLF_HIP(hipStreamSynchronize(s));
+    m->spmv_storage_ready = true;
+}
+}  // namespace nlf
 
 extern "C" {
 
@@ -2095,9 +2118,11 @@ int nlf_matrix_export(nlf_matrix a, int part, int64_t* outer, int64_t* inner, do
 
 int nlf_matrix_pattern_info(nlf_matrix a, int64_t* groups, int64_t* entries) {
     if (!a) return NLF_ERR_ARGUMENT;
-    if (groups) *groups = a->pstart.p ? a->pat_groups : 0;
-    if (entries) *entries = a->pstart.p ? a->pat_size : 0;
-    return NLF_OK;
+    return guarded(a->ctx, [&] {
+        ensure_spmv_storage(a);
+        if (groups) *groups = a->pstart.p ? a->pat_groups : 0;
+        if (entries) *entries = a->pstart.p ? a->pat_size : 0;
+    });
 }
 
 // Device views of the full inner storage: outer i64[rows+1], cols i32, values f64, diag i64[rows].

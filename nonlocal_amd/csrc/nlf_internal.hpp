@@ -373,6 +373,8 @@ struct nlf_matrix_s {
     nlf::DevBuf<int16_t> pat;            // dictionary of the distinct off16 rows (SpMV), when it pays
     nlf::DevBuf<nlf::i64> pstart;        // row r's offsets are pat[pstart[r] + p], p in [outer[r], outer[r+1])
     nlf::i64 pat_size = 0, pat_groups = 0;
+    bool spmv_storage_ready = false;     // off16 / pat / pstart built (ensure_spmv_storage)
+    int row_patterns = 1;                // NLF_OPT_ROW_PATTERNS when the values were assembled
     nlf::DevBuf<nlf::i64> b_outer;       // bound part: rows+1
     nlf::DevBuf<nlf::i32> b_cols;
     nlf::DevBuf<double> b_val;
@@ -427,7 +429,11 @@ struct CoarseSpace {
 };
 std::unique_ptr<CoarseSpace> build_coarse(const nlf_matrix_s* a, i64 target, hipStream_t s);
 void coarse_apply_add(const CoarseSpace& c, const double* r, double* z, hipStream_t s);   // z += P A_c^-1 P^T r
+// Builds the full-storage SpMV's 16-bit offsets and row-pattern dictionary on first use (and waits for
+// them on the matrix's stream).
+void ensure_spmv_storage(const nlf_matrix_s* a);
 inline SpmvCols spmv_cols(const nlf_matrix_s* a) {
+    ensure_spmv_storage(a);
     if (a->pstart.p) return {a->cols.p, a->pat.p, a->pstart.p};
     return {a->cols.p, a->off16.p, nullptr};
 }

@@ -1495,7 +1495,13 @@ i64 fcg(hipStream_t s, nlf_matrix A, const V* val, nlf_precond M, const double* 
         double* true_res, i64* updates, int window, KernelTimer& timer, const int16_t* lo = nullptr) {
     const i64 n = A->rows;
     LearnedGraph& g = *learned_of(M);
-    const int lpr = pick_lpr(A);
+    // FP64 values: the products read the upper triangle once (the symmetric storage), when it applies
+    bool sym = false;
+    if constexpr (std::is_same_v<V, double>) sym = !lo && symv_prepare(A, s);
+    // the lane count (and with it the partial-sum grid) the full-storage layout would take; the symmetric
+    // storage applies only where every offset is below 2^15, as the 16-bit one does, so it is known
+    // without building that layout
+    const int lpr = sym ? pick_lpr(double(A->nnz_full) / double(std::max<i64>(n, 1)), true) : pick_lpr(A);
     const unsigned sb = spmv_blocks(lpr, n), vb = grid_stride_blocks(n), rb = spmv_blocks(64, n);
     const size_t np = std::max<size_t>(std::max<size_t>(sb, vb), rb);
     DevBuf<double> r{size_t(n)}, z{size_t(n)}, p{size_t(n)}, q{size_t(n)}, pa{np}, pb{np}, sums{2};
@@ -1508,9 +1514,6 @@ i64 fcg(hipStream_t s, nlf_matrix A, const V* val, nlf_precond M, const double* 
         download(h, sums.p, 2, s);
         return std::make_pair(std::sqrt(h[0]), std::sqrt(h[1]));
     };
-    // FP64 values: the products read the upper triangle once (the symmetric storage), when it applies
-    bool sym = false;
-    if constexpr (std::is_same_v<V, double>) sym = !lo && symv_prepare(A, s);
     auto spmv_pq = [&](bool sample) {             // q = A p, partial p.q into pb; partial p.r into pa
         if (sample) timer.start(0, s);
         if (sym) launch_symv<true>(A, p.p, q.p, pb.p, nullptr, s, sb);

@@ -1408,6 +1408,7 @@ int pick_lpr(double avg, bool pairs = false) {
 }
 
 int pick_lpr(const nlf_matrix_s* a) {
+    ensure_spmv_storage(a);
     return pick_lpr(double(a->nnz_full) / double(std::max<i64>(a->rows, 1)), a->off16.p != nullptr);
 }
 
@@ -1839,6 +1840,7 @@ i64 spmv_stream_bytes(const nlf_matrix_s* a, size_t sv, size_t sx) {
         const nlf_symv_s& v = *a->symv;
         return v.slots * 64 * i64(sizeof(double)) + 2 * v.part_size * i64(sizeof(double)) + n * i64(4 * sizeof(double));
     }
+    ensure_spmv_storage(a);
     i64 cols = a->nnz_full * i64(a->off16.p ? sizeof(int16_t) : sizeof(i32));
     if (a->pstart.p) cols = a->pat_size * i64(sizeof(int16_t)) + n * i64(sizeof(i64));
     return a->nnz_full * i64(sv) + cols + n * i64(sizeof(i64) + 2 * sx);
@@ -1846,6 +1848,7 @@ i64 spmv_stream_bytes(const nlf_matrix_s* a, size_t sv, size_t sx) {
 
 template<int LPR, int UNROLL, bool NT>
 void launch_spmv_variant(nlf_matrix a, const double* x, double* y) {
+    ensure_spmv_storage(a);
     hipLaunchKernelGGL((k_spmv<LPR, double, double, double, false, false, UNROLL, NT>), dim3(spmv_blocks(LPR, a->rows)),
                        dim3(SPMV_THREADS), 0, a->ctx->stream, a->rows, a->outer.p, a->cols.p, a->off16.p,
                        static_cast<const i64*>(nullptr), a->val.p, static_cast<const int16_t*>(nullptr), x, y, nullptr,
