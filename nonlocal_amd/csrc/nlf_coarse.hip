@@ -69,33 +69,40 @@ __global__ void k_unit_diag(i64 from, i64 ncp, double* __restrict__ a) {
 }
 
 // In-place Gauss-Jordan inverse of the 64 x 64 diagonal tile (k, k) into dinv (no pivoting);
-// GJ_DIAG_THREADS threads, 4096 / GJ_DIAG_THREADS entries each per pivot step (1024: 256 and 64
-// threads were measured slower).
+// GJ_DIAG_THREADS threads (1024: 256 and 64 threads were measured slower), 4096 / GJ_DIAG_THREADS
+// entries each.  Per pivot p every thread reads what its entries need (the pivot, its rows' entries
+// in column p, its column's entry in row p), computes the new values -- row p scaled by the pivot,
+// the other rows eliminated with the scaled row, each thread scaling its own column's entry -- and
+// writes them after a barrier: two barriers per pivot.  The operations are those of the textbook
+// element-wise form, in the same order.
 #ifndef NLF_GJ_DIAG_THREADS
 #define NLF_GJ_DIAG_THREADS 1024
 #endif
 constexpr int GJ_DIAG_THREADS = NLF_GJ_DIAG_THREADS;
 __global__ __launch_bounds__(GJ_DIAG_THREADS) void k_gj_diag(const double* __restrict__ a, i64 ncp, i64 k, double* __restrict__ dinv) {
+    constexpr int PER = TB * TB / GJ_DIAG_THREADS;
+    static_assert(TB * TB % GJ_DIAG_THREADS == 0 && GJ_DIAG_THREADS % TB == 0, "whole rows per pass");
     __shared__ double d[TB][TB + 1];
-    __shared__ double colk[TB];
-    const int t = threadIdx.x, T = blockDim.x;
-    for (int e = t; e < TB * TB; e += T) d[e / TB][e % TB] = a[(k * TB + e / TB) * ncp + k * TB + e % TB];
+    const int t = threadIdx.x;
+    const int j = t % TB, i0 = t / TB;                        // column j, rows i0 + u * (threads / TB)
+    constexpr int RS = GJ_DIAG_THREADS / TB;
+    for (int e = t; e < TB * TB; e += GJ_DIAG_THREADS) d[e / TB][e % TB] = a[(k * TB + e / TB) * ncp + k * TB + e % TB];
     __syncthreads();
     for (int p = 0; p < TB; ++p) {
         const double piv = d[p][p];
-        __syncthreads();
-        if (t < TB) {
-            colk[t] = d[t][p];
-            d[p][t] = (t == p ? 1.0 : d[p][t]) / piv;
+        const double rp = (j == p ? 1.0 : d[p][j]) / piv;     // the scaled row p at column j
+        double nv[PER];
+#pragma unroll
+        for (int u = 0; u < PER; ++u) {
+            const int i = i0 + u * RS;
+            nv[u] = i == p ? rp : (j == p ? 0.0 : d[i][j]) - d[i][p] * rp;
         }
-        __syncthreads();
-        for (int e = t; e < TB * TB; e += T) {
-            const int i = e / TB, j = e % TB;
-            if (i != p) d[i][j] = (j == p ? 0.0 : d[i][j]) - colk[i] * d[p][j];
-        }
+        __syncthreads();                                      // every read of this pivot is done
+#pragma unroll
+        for (int u = 0; u < PER; ++u) d[i0 + u * RS][j] = nv[u];
         __syncthreads();
     }
-    for (int e = t; e < TB * TB; e += T) dinv[e] = d[e / TB][e % TB];
+    for (int e = t; e < TB * TB; e += GJ_DIAG_THREADS) dinv[e] = d[e / TB][e % TB];
 }
 
 // acc += X (64 x 64, leading dimension ldx) * Y (64 x 64, ldy); thread (tx, ty) owns rows 4 ty .. 4 ty + 3
