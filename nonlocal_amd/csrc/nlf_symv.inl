@@ -225,6 +225,15 @@ __global__ __launch_bounds__(SYMB_THREADS) void k_sym_count(i64 rows, const i64*
 // U_s and band(u) the number of run starts <= u less one, the window index of u is 63 band + rank:
 // the strips before band b take sum_{b' < b} (64 + w_b' - 1) = 63 b + rank(u_b) entries, and ranks
 // run on consecutively inside a band.
+// Slots per LDS tile of the value placement in k_sym_fill (64 rows x 8 B each: 32 KB), and the tile's
+// offset in the dynamic LDS, in words, after the three word arrays (bitmap, rank and band prefixes),
+// rounded up to 8 bytes.
+constexpr int SYMF_CHUNK = 64;
+__host__ __device__ constexpr size_t symf_tile_offset(int words) { return (3 * size_t(words) + 1) & ~size_t(1); }
+__host__ __device__ constexpr size_t symf_lds_bytes(int words) {
+    return symf_tile_offset(words) * sizeof(unsigned) + size_t(SYMF_CHUNK) * 64 * sizeof(double);
+}
+
 __global__ __launch_bounds__(SYMB_THREADS) void k_sym_fill(i64 rows, const i64* __restrict__ outer,
                                                            const i32* __restrict__ cols, const i64* __restrict__ diag,
                                                            const double* __restrict__ val, int words,
@@ -272,29 +281,47 @@ __global__ __launch_bounds__(SYMB_THREADS) void k_sym_fill(i64 rows, const i64* 
         const int next = b + 1 < nbands ? start_rank[b + 1] : total;
         band[bb + b] = SymBand{start_u[b], 63 * b + start_rank[b], next - start_rank[b], 0};
     }
-    // values: wave w places rows w, w + 4, ... (entries read 256 at a time, coalesced)
+    // values: the slice's slots SYMF_CHUNK at a time, each chunk assembled in an LDS tile (slot-major,
+    // 64 rows a slot, zero where a row lacks the offset) and then stored whole, so that every store
+    // is coalesced and the padding needs no clearing pass of its own.  A row's upper entries are in
+    // ascending offset, so in ascending slot rank: the entries of one chunk are the next run of the
+    // row, which wave w (rows w, w + 4, ...) reads 64 at a time from the row's cursor.
+    double (*tile)[64] = reinterpret_cast<double (*)[64]>(bm + symf_tile_offset(words));
+    __shared__ i64 cursor[64];
     const int lane = threadIdx.x & 63, wv = int(threadIdx.x >> 6);
-    for (int j = wv; j < 64; j += SYMB_THREADS / 64) {
-        const i64 r = s * 64 + j;
-        if (r >= rows) break;
-        const i64 b = diag[r], e = outer[r + 1];
-        for (i64 p0 = b; p0 < e; p0 += 256) {
-            i32 c[4];
-            double a[4];
-#pragma unroll
-            for (int t = 0; t < 4; ++t) {
-                const i64 p = p0 + t * 64 + lane;
-                c[t] = p < e ? cols[p] : -1;
-                a[t] = p < e ? val[p] : 0.0;
-            }
-#pragma unroll
-            for (int t = 0; t < 4; ++t)
-                if (c[t] >= 0) {
-                    const int u = int(c[t] - r);
-                    const unsigned rank = pre[u >> 5] + __popc(bm[u >> 5] & ((1u << (u & 31)) - 1u));
-                    sval[(kb + rank) * 64 + j] = a[t];
+    if (threadIdx.x < 64) {
+        const i64 r = s * 64 + threadIdx.x;
+        cursor[threadIdx.x] = r < rows ? diag[r] : 0;
+    }
+    for (int k0 = 0; k0 < total; k0 += SYMF_CHUNK) {
+        for (int e = threadIdx.x; e < SYMF_CHUNK * 64; e += SYMB_THREADS) tile[e >> 6][e & 63] = 0.0;
+        __syncthreads();
+        for (int j = wv; j < 64; j += SYMB_THREADS / 64) {
+            const i64 r = s * 64 + j;
+            if (r >= rows) break;
+            const i64 e = outer[r + 1];
+            i64 p0 = cursor[j];
+            for (;;) {
+                const i64 p = p0 + lane;
+                int rank = 0x7fffffff;                         // past the row's end: in no chunk
+                double v = 0.0;
+                if (p < e) {
+                    const int u = int(cols[p] - r);
+                    rank = int(pre[u >> 5] + __popc(bm[u >> 5] & ((1u << (u & 31)) - 1u)));
+                    v = val[p];
                 }
+                const bool in = rank < k0 + SYMF_CHUNK;
+                if (in) tile[rank - k0][j] = v;
+                const int n_in = __popcll(__ballot(in));
+                p0 += n_in;
+                if (n_in < 64) break;                          // the rest of the row lies past this chunk
+            }
+            if (lane == 0) cursor[j] = p0;
         }
+        __syncthreads();
+        const int nk = min(SYMF_CHUNK, total - k0);
+        for (int e = threadIdx.x; e < nk * 64; e += SYMB_THREADS) sval[(kb + k0) * 64 + e] = tile[e >> 6][e & 63];
+        __syncthreads();
     }
 }
 
@@ -422,8 +449,7 @@ bool symv_prepare(nlf_matrix_s* a, hipStream_t s) {
     sv->part.alloc(size_t(std::max<i64>(pbv[ns], 1)));
     sv->direct.alloc(size_t(rows));
     DevBuf<SymBand> band{size_t(std::max<i64>(bp[ns], 1))};
-    hipLaunchKernelGGL(k_zero<double>, dim3(2048), dim3(256), 0, s, slots * 64, sv->sval.p);
-    hipLaunchKernelGGL(k_sym_fill, dim3(unsigned(ns)), dim3(SYMB_THREADS), 3 * size_t(words) * sizeof(unsigned), s, rows,
+    hipLaunchKernelGGL(k_sym_fill, dim3(unsigned(ns)), dim3(SYMB_THREADS), symf_lds_bytes(words), s, rows,
                        a->outer.p, a->cols.p, a->diag.p, a->val.p, words, sv->kptr.p, bptr.p, sv->meta.p, band.p,
                        sv->sval.p);
     NLF_HIP(hipGetLastError());
