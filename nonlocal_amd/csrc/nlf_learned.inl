@@ -1316,6 +1316,13 @@ __global__ void k_lf_jacobi(i64 n, const float* __restrict__ s, const double* __
     }
 }
 
+// Lanes per row of the compiled operator's SpMV (0: the same rule as for the matrix, which gives 4
+// for its 37 nonzeros a row; measured per apply at 1000 x 1000: 1 lane 0.152 ms, 2 lanes 0.133,
+// 4 lanes 0.134, 8 lanes 0.171, 16 lanes 0.234, the coarse correction included).
+#ifndef NLF_M_LPR
+#define NLF_M_LPR 0
+#endif
+
 // z = M r, the symmetric part of N, on the lane-per-node kernels: forward layers (A, B channel-major
 // ping-pong), y; transposed layers (d in place, q through A then B) ending in the output.
 template<class Z>
@@ -1325,7 +1332,7 @@ void learned_apply_one(LearnedGraph& g, const float* w, const double* r, Z* z, h
         return;
     }
     if (g.m_ready && !g.network_apply) {
-        const int lpr = pick_lpr(double(g.m_nnz) / double(std::max<i64>(g.n, 1)), g.m_off16.p != nullptr);
+        const int lpr = NLF_M_LPR ? NLF_M_LPR : pick_lpr(double(g.m_nnz) / double(std::max<i64>(g.n, 1)), g.m_off16.p != nullptr);
         const SpmvCols cs = g.m_pstart.p ? SpmvCols{g.m_cols.p, g.m_pat.p, g.m_pstart.p}
                                          : SpmvCols{g.m_cols.p, g.m_off16.p, nullptr};
         launch_spmv<float, double, double, false>(lpr, g.n, g.m_outer.p, cs, g.m_val.p, r, z, nullptr,
