@@ -236,9 +236,18 @@ __global__ __launch_bounds__(256) void k_coarse_gemv(i64 ncp, const double* __re
     const i64 I = i64(blockIdx.x) * 4 + (threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
     if (I >= ncp) return;
-    const double* row = inv + I * ncp;
-    double s = 0.0;
-    for (i64 j = lane; j < ncp; j += 64) s += row[j] * rc[j];
+    // 16-byte loads, lane l takes the column pairs l, l + 64, ... (ncp is a multiple of 64), two sums
+    const double2* row = reinterpret_cast<const double2*>(inv + I * ncp);
+    const double2* r2 = reinterpret_cast<const double2*>(rc);
+    const i64 n2 = ncp / 2;
+    double s0 = 0.0, s1 = 0.0;
+#pragma unroll 4
+    for (i64 j = lane; j < n2; j += 64) {
+        const double2 a = row[j], b = r2[j];
+        s0 += a.x * b.x;
+        s1 += a.y * b.y;
+    }
+    double s = s0 + s1;
     for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
     if (lane == 0) yc[I] = s;
 }
