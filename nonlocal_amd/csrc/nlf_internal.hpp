@@ -45,8 +45,8 @@ struct Error : std::runtime_error {
 // release.  hipMalloc of a large block costs up to a millisecond and hipFree waits for the whole
 // device; a workload that assembles and solves again and again reuses its blocks.  Outside a
 // context scope (or for another context's buffer) blocks are plain hipMalloc / hipFree.  The free
-// list holds at most an eighth of the device memory and is emptied when hipMalloc fails and when
-// the context is destroyed.
+// lists of all contexts on a device hold at most an eighth of its memory together; a context's list
+// is emptied when hipMalloc fails and when the context is destroyed.
 struct CtxScope {
     nlf_ctx prev;
     explicit CtxScope(nlf_ctx c);
@@ -313,7 +313,7 @@ struct nlf_ctx_s {
     bool alive = true;
     std::mutex blocks_mu;
     std::multimap<size_t, void*> free_blocks;
-    size_t cached_bytes = 0, cache_limit = 0;
+    size_t cached_bytes = 0;
 };
 
 struct nlf_mesh_s {

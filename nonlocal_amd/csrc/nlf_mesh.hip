@@ -6,6 +6,7 @@
 #include "nlf_internal.hpp"
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <memory>
 #include <cmath>
@@ -200,12 +201,31 @@ static size_t block_bytes(size_t bytes) {
     return (bytes + g - 1) / g * g;
 }
 
+// Bytes held by the free lists of all contexts of a device, against one limit per device (an eighth of
+// its memory), so that several contexts together hold no more than one would.
+constexpr int BLOCK_DEVICES = 64;
+static std::atomic<size_t> g_free_bytes[BLOCK_DEVICES];
+static std::atomic<size_t> g_free_limit[BLOCK_DEVICES];
+
+static bool reserve_free_bytes(int device, size_t cap) {
+    if (device < 0 || device >= BLOCK_DEVICES) return false;
+    const size_t limit = g_free_limit[device].load();
+    size_t cur = g_free_bytes[device].load();
+    while (cur + cap <= limit)
+        if (g_free_bytes[device].compare_exchange_weak(cur, cur + cap)) return true;
+    return false;
+}
+static void unreserve_free_bytes(int device, size_t cap) {
+    if (device >= 0 && device < BLOCK_DEVICES) g_free_bytes[device].fetch_sub(cap);
+}
+
 // Waits for the context's stream, then returns every free block to the device; caller holds blocks_mu.
 static void release_free_blocks(nlf_ctx c) {
     if (c->free_blocks.empty()) return;
     (void)hipStreamSynchronize(c->stream);
     for (auto& kv : c->free_blocks) (void)hipFree(kv.second);
     c->free_blocks.clear();
+    unreserve_free_bytes(c->device, c->cached_bytes);
     c->cached_bytes = 0;
 }
 
@@ -226,6 +246,7 @@ void* device_alloc(size_t bytes, size_t* cap, nlf_ctx* owner) {
             p = it->second;
             *cap = it->first;
             c->cached_bytes -= it->first;
+            unreserve_free_bytes(c->device, it->first);
             c->free_blocks.erase(it);
             return p;
         }
@@ -250,7 +271,7 @@ void* device_alloc(size_t bytes, size_t* cap, nlf_ctx* owner) {
 void device_free(void* p, size_t cap, nlf_ctx owner) {
     if (owner && owner == t_scope_ctx && owner->alive) {
         std::lock_guard<std::mutex> lock(owner->blocks_mu);
-        if (owner->cached_bytes + cap <= owner->cache_limit) {
+        if (reserve_free_bytes(owner->device, cap)) {
             owner->free_blocks.emplace(cap, p);
             owner->cached_bytes += cap;
             return;
@@ -283,7 +304,7 @@ int nlf_ctx_create(int device, void* stream, nlf_ctx* out) {
     }
     size_t free_b = 0, total_b = 0;
     if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) { (void)hipGetLastError(); total_b = 0; }
-    c->cache_limit = total_b / 8;
+    if (device >= 0 && device < BLOCK_DEVICES) g_free_limit[device].store(total_b / 8);
     *out = c;
     return NLF_OK;
 }
