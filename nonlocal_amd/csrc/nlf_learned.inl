@@ -1062,7 +1062,9 @@ __device__ int union_rows(const i64* __restrict__ outer, const i32* __restrict__
     return total;
 }
 
-// PASS 0: |P3(u)| into cnt[u] (-1 on overflow).  PASS 1: columns and Y row values of u.
+// PASS 0: |P3(u)| into cnt[u] (-1 on overflow).  PASS 1: columns and Y row values of u at m_outer[u].
+// PASS 2: both in one pass, |P3(u)| into cnt[u] and the row at u * CP_P3 (a padded layout that
+// k_lf_compact then packs by the scanned counts).
 template<int PASS>
 __global__ __launch_bounds__(CP_THREADS) void k_lf_compile(i64 n, const i64* __restrict__ outer, const i32* __restrict__ cols,
                                                            const float4* __restrict__ feat, i64 band, int words,
@@ -1082,7 +1084,7 @@ __global__ __launch_bounds__(CP_THREADS) void k_lf_compile(i64 n, const i64* __r
     const i64 r0 = outer[u];
     const int n1 = int(outer[u + 1] - r0);
     if (n1 > CP_P1 || n1 > cdrows) {
-        if (threadIdx.x == 0) { if (PASS == 0) cnt[u] = 0; *overflow = 1; }
+        if (threadIdx.x == 0) { if (PASS != 1) cnt[u] = 0; *overflow = 1; }
         return;
     }
     for (int k = threadIdx.x; k < n1; k += blockDim.x) {
@@ -1097,14 +1099,15 @@ __global__ __launch_bounds__(CP_THREADS) void k_lf_compile(i64 n, const i64* __r
     __shared__ int pre[CP_P2 + 1];
     const int n2 = union_rows(outer, cols, p1, n1, base2, bm, words, p2, CP_P2, rs, pre);
     if (n2 < 0) {
-        if (threadIdx.x == 0) { if (PASS == 0) cnt[u] = 0; *overflow = 1; }
+        if (threadIdx.x == 0) { if (PASS != 1) cnt[u] = 0; *overflow = 1; }
         return;
     }
     const int n3 = union_rows(outer, cols, p2, n2, base3, bm, words, p3, CP_P3, rs, pre);
     if (n3 < 0) {
-        if (threadIdx.x == 0) { if (PASS == 0) cnt[u] = 0; *overflow = 1; }
+        if (threadIdx.x == 0) { if (PASS != 1) cnt[u] = 0; *overflow = 1; }
         return;
     }
+    if (PASS == 2 && threadIdx.x == 0) cnt[u] = n3;
     if (PASS == 0) {
         if (threadIdx.x == 0) cnt[u] = n3;
         return;
@@ -1180,7 +1183,7 @@ __global__ __launch_bounds__(CP_THREADS) void k_lf_compile(i64 n, const i64* __r
         }
     }
     __syncthreads();
-    const i64 o = m_outer[u];
+    const i64 o = PASS == 2 ? u * CP_P3 : m_outer[u];
     for (int k = threadIdx.x; k < n3; k += blockDim.x) {
         const i32 x = p3[k];
         const int ix = find_sorted(p2, n2, x);
@@ -1228,6 +1231,18 @@ __global__ void k_lf_off16(i64 n, const i64* __restrict__ outer, const i32* __re
     for (i64 p = outer[r] + (threadIdx.x & 63); p < outer[r + 1]; p += 64) off[p] = int16_t(i64(cols[p]) - r);
 }
 
+// Packs the padded rows of k_lf_compile<2> (row u at u * CP_P3) by the scanned counts.
+__global__ void k_lf_compact(i64 n, const i64* __restrict__ m_outer, const i32* __restrict__ pcols,
+                             const float* __restrict__ py, i32* __restrict__ m_cols, float* __restrict__ yrow) {
+    const i64 u = i64(blockIdx.x) * (blockDim.x / 64) + (threadIdx.x >> 6);
+    if (u >= n) return;
+    const i64 o = m_outer[u], len = m_outer[u + 1] - o;
+    for (i64 k = threadIdx.x & 63; k < len; k += 64) {
+        m_cols[o + k] = pcols[u * CP_P3 + k];
+        yrow[o + k] = py[u * CP_P3 + k];
+    }
+}
+
 // Builds M for the weights w; leaves m_ready false when a node's sets exceed the LDS capacities.
 void learned_compile(LearnedGraph& g, const float* w, hipStream_t s, bool ctx_row_patterns) {
     const auto t0 = std::chrono::steady_clock::now();
@@ -1246,9 +1261,13 @@ void learned_compile(LearnedGraph& g, const float* w, hipStream_t s, bool ctx_ro
     // dynamic LDS: the bitmap, or the contracted basis of up to ell_d 1-hop entries after it
     const int cdrows = std::min(g.ell_d, CP_P1);
     const size_t lds = size_t(std::max(words, 2 * cdrows * MP_H)) * sizeof(unsigned);
-    hipLaunchKernelGGL(k_lf_compile<0>, dim3(unsigned(n)), dim3(CP_THREADS), size_t(words) * sizeof(unsigned), s, n,
-                       g.outer.p, g.cols.p, feat, g.band, words, basis.p, w, cnt.p, (const i64*)nullptr, (i32*)nullptr,
-                       (float*)nullptr, overflow.p, cdrows);
+    // one pass into rows padded to CP_P3 entries (the capacity of a row's three-hop set), then the
+    // rows packed by the scanned lengths: the unions are formed once instead of once to count and once
+    // to fill
+    DevBuf<i32> pcols{size_t(n) * CP_P3};
+    DevBuf<float> py{size_t(n) * CP_P3};
+    hipLaunchKernelGGL(k_lf_compile<2>, dim3(unsigned(n)), dim3(CP_THREADS), lds, s, n, g.outer.p, g.cols.p, feat, g.band,
+                       words, basis.p, w, cnt.p, (const i64*)nullptr, pcols.p, py.p, overflow.p, cdrows);
     NLF_HIP(hipGetLastError());
     int ovf = 0;
     download(&ovf, overflow.p, 1, s);
@@ -1261,8 +1280,8 @@ void learned_compile(LearnedGraph& g, const float* w, hipStream_t s, bool ctx_ro
     g.m_cols.alloc(size_t(std::max<i64>(nnz, 1)));
     g.m_val.alloc(size_t(std::max<i64>(nnz, 1)));
     DevBuf<float> yrow{size_t(std::max<i64>(nnz, 1))};
-    hipLaunchKernelGGL(k_lf_compile<1>, dim3(unsigned(n)), dim3(CP_THREADS), lds, s, n, g.outer.p, g.cols.p, feat, g.band,
-                       words, basis.p, w, (i64*)nullptr, g.m_outer.p, g.m_cols.p, yrow.p, overflow.p, cdrows);
+    hipLaunchKernelGGL(k_lf_compact, dim3(blocks_for(n, 4)), dim3(256), 0, s, n, g.m_outer.p, pcols.p, py.p, g.m_cols.p,
+                       yrow.p);
     hipLaunchKernelGGL(k_lf_symmetrize, dim3(blocks_for(n, 4)), dim3(256), 0, s, n, g.m_outer.p, g.m_cols.p, yrow.p, g.s.p,
                        g.m_val.p);
     g.m_off16.release();
