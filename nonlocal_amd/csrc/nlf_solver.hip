@@ -1103,6 +1103,8 @@ static bool build_ell(const std::vector<i64>& outer, const std::vector<i32>& col
     el.WE = WE;
     el.threads = threads;
     el.rows = n;
+    el.blocks = 1;                                   // one sweep (a blocked layout tried before is replaced)
+    el.bdesc.release();
     upload(el.lptr, lptr.data(), lptr.size(), s);
     upload(el.row, lrows.data(), lrows.size(), s);
     upload(el.dep, dep.data(), dep.size(), s);
