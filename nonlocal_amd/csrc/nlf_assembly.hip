@@ -497,6 +497,9 @@ struct AssemblyArgs {
     const i32* nb_ids;
     const i32* el_maxnode;
     const uint8_t* el_dirichlet;
+    // per neighbour-list entry: the neighbour's largest node id, bit 31 set when it has a Dirichlet
+    // node (the pair path's activity test read with the list, not gathered through it), or null
+    const i32* nb_pack;
     const uint8_t* inner;
     const double* xq;
     const double* jac;
@@ -822,6 +825,12 @@ __global__ __launch_bounds__(256) void k_pairs_mfma32(PairArgs<float> a) {
 
 // T = double: the FP64 value pass (inner and bound parts).  T = float: the FP32 value pass with
 // FP32 accumulation into val32 (inner part only).
+// Activity test of the pair path from a per-entry pack (1) or gathered through the neighbour ids (0):
+// 71.1 against 71.8 ms per 1000 x 1000 assembly, the pack's build included.
+#ifndef NLF_ASM_NB_PACK
+#define NLF_ASM_NB_PACK 1
+#endif
+
 // Neighbours per thread and step where the rows come from the pair blocks (2: measured against 3
 // and 4, which were slower).
 #ifndef NLF_ASM_PAIR_ITEMS
@@ -920,17 +929,24 @@ __global__ __launch_bounds__(THREADS) void k_assemble(AssemblyArgs a) {
             bool active[ITEMS];
             i64 eNL[ITEMS];
             i32 mj[ITEMS][NN];
+            i32 pk[ITEMS];
+            const bool packed = nonlocal && a.nb_pack != nullptr;
 #pragma unroll
             for (int u = 0; u < ITEMS; ++u) {
                 const i64 t = base + u * i64(blockDim.x) + threadIdx.x;
                 eNL[u] = -1;
-                if (t < cnt) eNL[u] = nonlocal ? a.nb_ids[nb_b + t] : eL;
+                pk[u] = 0;
+                if (t < cnt) {
+                    eNL[u] = nonlocal ? a.nb_ids[nb_b + t] : eL;
+                    if (packed) pk[u] = a.nb_pack[nb_b + t];
+                }
             }
 #pragma unroll
             for (int u = 0; u < ITEMS; ++u) {
                 // pairs whose columns all lie left of the node's rows and have no Dirichlet column
                 // contribute nothing to the upper triangle nor to the bound part
-                active[u] = eNL[u] >= 0 && (a.el_maxnode[eNL[u]] >= node || a.el_dirichlet[eNL[u]]);
+                if (packed) active[u] = eNL[u] >= 0 && ((pk[u] & 0x7fffffff) >= node || pk[u] < 0);
+                else active[u] = eNL[u] >= 0 && (a.el_maxnode[eNL[u]] >= node || a.el_dirichlet[eNL[u]]);
             }
 #pragma unroll
             for (int u = 0; u < ITEMS; ++u) {
@@ -1225,6 +1241,15 @@ __global__ void k_elem_minnode(i64 E, int nn, const i32* __restrict__ el, i32* _
     i32 m = el[e * nn];
     for (int i = 1; i < nn; ++i) m = min(m, el[e * nn + i]);
     mn[e] = m;
+}
+
+// The activity test's data per neighbour-list entry (AssemblyArgs::nb_pack).
+__global__ void k_nb_pack(i64 total, const i32* __restrict__ nb_ids, const i32* __restrict__ el_maxnode,
+                          const uint8_t* __restrict__ el_dir, i32* __restrict__ out) {
+    for (i64 q = i64(blockIdx.x) * blockDim.x + threadIdx.x; q < total; q += i64(gridDim.x) * blockDim.x) {
+        const i32 e = nb_ids[q];
+        out[q] = el_maxnode[e] | (el_dir[e] ? i32(0x80000000u) : 0);
+    }
 }
 
 __global__ void k_el_dirichlet(i64 E, int nn, int dof, const i32* __restrict__ el, const uint8_t* __restrict__ inner,
@@ -1541,6 +1566,7 @@ struct PairChunk {
 };
 struct PairPlan {
     i64 per_pair = 0, all_pairs = 0, max_total = 0;
+    i64 entries = 0;                     // neighbour-list entries (all_pairs counts them padded to 4 per element)
     bool single = true;
     std::vector<PairChunk> chunks;
     std::vector<i32> list;
@@ -1562,6 +1588,7 @@ PairPlan plan_pairs(const nlf_mesh_s* m, const std::vector<i64>& nb_off, i64 per
     plan.per_pair = per_pair;
     auto slots = [&](i64 e) { return (nb_off[size_t(e + 1)] - nb_off[size_t(e)] + 3) & ~i64(3); };
     for (i64 e = 0; e < E; ++e) plan.all_pairs += slots(e);
+    plan.entries = nb_off[size_t(E)];
     plan.single = plan.all_pairs * per_pair <= budget;
     const std::vector<uint32_t>& key = m->el_key_h;
     if (plan.single) {
@@ -1696,6 +1723,13 @@ void assemble_values(nlf_matrix_s* mat, hipStream_t s, const PairPlan* plan = nu
             plan = &own;
         }
         const i64 all_pairs = plan->all_pairs;
+        DevBuf<i32> nb_pack;
+        if (NLF_ASM_NB_PACK && plan->entries > 0) {
+            nb_pack.alloc(size_t(plan->entries));
+            hipLaunchKernelGGL(k_nb_pack, dim3(4096), dim3(256), 0, s, plan->entries, m->nb_ids.p, m->el_maxnode.p,
+                               el_dir.p, nb_pack.p);
+            aa.nb_pack = nb_pack.p;
+        }
         // the pair kernel's element order: Morton order within each chunk
         DevBuf<i32> d_order;
         {
