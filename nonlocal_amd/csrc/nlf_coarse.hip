@@ -105,79 +105,30 @@ __global__ __launch_bounds__(GJ_DIAG_THREADS) void k_gj_diag(const double* __res
     for (int e = t; e < TB * TB; e += GJ_DIAG_THREADS) dinv[e] = d[e / TB][e % TB];
 }
 
-// acc += X (64 x 64, leading dimension ldx) * Y (64 x 64, ldy); thread (tx, ty) owns rows 4 ty .. 4 ty + 3
-// and columns 4 tx .. 4 tx + 3.  The inner dimension is staged through LDS in two halves.
-__device__ __forceinline__ void tile_mm(const double* __restrict__ X, i64 ldx, const double* __restrict__ Y, i64 ldy,
-                                        double (&acc)[4][4], double (*sx)[33], double (*sy)[65]) {
-    const int t = threadIdx.x, tx = t % 16, ty = t / 16;
-    for (int h = 0; h < 2; ++h) {
-        __syncthreads();
-        for (int e = t; e < 64 * 32; e += 256) {
-            const int i = e / 32, j = e % 32;
-            sx[i][j] = X[i64(i) * ldx + h * 32 + j];
-        }
-        for (int e = t; e < 32 * 64; e += 256) {
-            const int i = e / 64, j = e % 64;
-            sy[i][j] = Y[i64(h * 32 + i) * ldy + j];
-        }
-        __syncthreads();
-        for (int q = 0; q < 32; ++q) {
-            double xa[4], yb[4];
-            for (int u = 0; u < 4; ++u) xa[u] = sx[ty * 4 + u][q];
-            for (int v = 0; v < 4; ++v) yb[v] = sy[q][tx * 4 + v];
-            for (int u = 0; u < 4; ++u)
-                for (int v = 0; v < 4; ++v) acc[u][v] += xa[u] * yb[v];
-        }
-    }
-}
+typedef double gj_d4 __attribute__((ext_vector_type(4)));
 
-// Row panel: A(k, j) = D^-1 A(k, j) for every tile column j != k.
-__global__ __launch_bounds__(256) void k_gj_rowpanel(double* __restrict__ a, i64 ncp, i64 k, const double* __restrict__ dinv) {
-    __shared__ double sx[64][33], sy[32][65];
-    const i64 nb = ncp / TB;
-    const i64 j = blockIdx.x < k ? blockIdx.x : blockIdx.x + 1;
-    if (j >= nb) return;
-    double acc[4][4] = {};
-    double* blk = a + (k * TB) * ncp + j * TB;
-    tile_mm(dinv, TB, blk, ncp, acc, sx, sy);
-    __syncthreads();                                              // every read of the tile is done
-    const int t = threadIdx.x, tx = t % 16, ty = t / 16;
-    for (int u = 0; u < 4; ++u)
-        for (int v = 0; v < 4; ++v) blk[i64(ty * 4 + u) * ncp + tx * 4 + v] = acc[u][v];
-}
-
-// Trailing update: A(i, j) -= A(i, k) A(k, j) for tile rows i != k and tile
-// columns j != k with V_MFMA_F64_16X16X4_F64.  Lane l of a wave holds A[l % 16][l / 16] and
-// B[l / 16][l % 16] of a 16 x 4 by 4 x 16 product and D[l / 16 + 4 r][l % 16] in register r
-// (measured on gfx950).
-// Wave w computes the 32 x 32 quadrant (w / 2, w % 2) of the tile as 2 x 2 blocks of 16 x 16.  The
-// inner dimension is staged through LDS in two halves of 32, XOR-swizzled by row so that the
-// operand reads of one instruction hit distinct banks.
-__global__ __launch_bounds__(256) void k_gj_update_mfma(double* __restrict__ a, i64 ncp, i64 k) {
-    typedef double d4 __attribute__((ext_vector_type(4)));
-    __shared__ double sx[64][32];                                 // X(i, kk) at [i][kk ^ (2 i & 31)]
-    __shared__ double sy[32][64];                                 // Y(kk, j) at [kk][j ^ 16 (kk & 3)]
-    const i64 bi = blockIdx.y < k ? blockIdx.y : blockIdx.y + 1;
-    const i64 bj = blockIdx.x < k ? blockIdx.x : blockIdx.x + 1;
-    const i64 nb = ncp / TB;
-    if (bi >= nb || bj >= nb) return;
+// acc = X (64 x 64, leading dimension ldx) * Y (64 x 64, ldy) with V_MFMA_F64_16X16X4_F64.  Lane l
+// of a wave holds A[l % 16][l / 16] and B[l / 16][l % 16] of a 16 x 4 by 4 x 16 product and
+// D[l / 16 + 4 r][l % 16] in register r (measured on gfx950).  Wave w computes the 32 x 32 quadrant
+// (w / 2, w % 2) as 2 x 2 blocks of 16 x 16.  The inner dimension is staged through LDS in two halves
+// of 32, XOR-swizzled by row so that the operand reads of one instruction hit distinct banks.  Every
+// read of X and Y is done when it returns (the last staging is followed by a barrier).
+__device__ __forceinline__ void gj_tile_mfma(const double* __restrict__ X, i64 ldx, const double* __restrict__ Y, i64 ldy,
+                                             gj_d4 (&acc)[2][2], double (*sx)[32], double (*sy)[64]) {
     const int t = threadIdx.x, w = t >> 6, l = t & 63;
     const int li = l & 15, lk = l >> 4;
     const int r0 = 32 * (w >> 1), c0 = 32 * (w & 1);
-    const double* X = a + (bi * TB) * ncp + k * TB;
-    const double* Y = a + (k * TB) * ncp + bj * TB;
-    d4 acc[2][2];
     for (int u = 0; u < 2; ++u)
-        for (int v = 0; v < 2; ++v) acc[u][v] = d4{0.0, 0.0, 0.0, 0.0};
+        for (int v = 0; v < 2; ++v) acc[u][v] = gj_d4{0.0, 0.0, 0.0, 0.0};
     for (int half = 0; half < 2; ++half) {
         __syncthreads();
         for (int e = t; e < 64 * 32; e += 256) {
             const int i = e >> 5, j = e & 31;
-            sx[i][j ^ ((2 * i) & 31)] = X[i64(i) * ncp + 32 * half + j];
+            sx[i][j ^ ((2 * i) & 31)] = X[i64(i) * ldx + 32 * half + j];
         }
         for (int e = t; e < 32 * 64; e += 256) {
             const int kk = e >> 6, j = e & 63;
-            sy[kk][j ^ (16 * (kk & 3))] = Y[i64(32 * half + kk) * ncp + j];
+            sy[kk][j ^ (16 * (kk & 3))] = Y[i64(32 * half + kk) * ldy + j];
         }
         __syncthreads();
         for (int ks = 0; ks < 32; ks += 4) {
@@ -192,25 +143,60 @@ __global__ __launch_bounds__(256) void k_gj_update_mfma(double* __restrict__ a, 
                 for (int v = 0; v < 2; ++v) acc[u][v] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[u], bv[v], acc[u][v], 0, 0, 0);
         }
     }
-    double* C = a + (bi * TB) * ncp + bj * TB;
+}
+
+// C (64 x 64, ldc) op= acc in the layout of gj_tile_mfma: SIGN -1 subtracts, +1 stores, 0 stores -acc.
+template<int SIGN>
+__device__ __forceinline__ void gj_tile_store(double* __restrict__ C, i64 ldc, const gj_d4 (&acc)[2][2]) {
+    const int t = threadIdx.x, w = t >> 6, l = t & 63;
+    const int li = l & 15, lk = l >> 4;
+    const int r0 = 32 * (w >> 1), c0 = 32 * (w & 1);
     for (int u = 0; u < 2; ++u)
         for (int v = 0; v < 2; ++v)
-            for (int r = 0; r < 4; ++r) C[i64(r0 + 16 * u + lk + 4 * r) * ncp + c0 + 16 * v + li] -= acc[u][v][r];
+            for (int r = 0; r < 4; ++r) {
+                double& c = C[i64(r0 + 16 * u + lk + 4 * r) * ldc + c0 + 16 * v + li];
+                if constexpr (SIGN < 0) c -= acc[u][v][r];
+                else if constexpr (SIGN > 0) c = acc[u][v][r];
+                else c = -acc[u][v][r];
+            }
+}
+
+// Row panel: A(k, j) = D^-1 A(k, j) for every tile column j != k.
+__global__ __launch_bounds__(256) void k_gj_rowpanel(double* __restrict__ a, i64 ncp, i64 k, const double* __restrict__ dinv) {
+    __shared__ double sx[64][32], sy[32][64];
+    const i64 nb = ncp / TB;
+    const i64 j = blockIdx.x < k ? blockIdx.x : blockIdx.x + 1;
+    if (j >= nb) return;
+    double* blk = a + (k * TB) * ncp + j * TB;
+    gj_d4 acc[2][2];
+    gj_tile_mfma(dinv, TB, blk, ncp, acc, sx, sy);
+    __syncthreads();                                              // every read of the tile is done
+    gj_tile_store<1>(blk, ncp, acc);
+}
+
+// Trailing update: A(i, j) -= A(i, k) A(k, j) for tile rows i != k and tile columns j != k.
+__global__ __launch_bounds__(256) void k_gj_update_mfma(double* __restrict__ a, i64 ncp, i64 k) {
+    __shared__ double sx[64][32], sy[32][64];
+    const i64 bi = blockIdx.y < k ? blockIdx.y : blockIdx.y + 1;
+    const i64 bj = blockIdx.x < k ? blockIdx.x : blockIdx.x + 1;
+    const i64 nb = ncp / TB;
+    if (bi >= nb || bj >= nb) return;
+    gj_d4 acc[2][2];
+    gj_tile_mfma(a + (bi * TB) * ncp + k * TB, ncp, a + (k * TB) * ncp + bj * TB, ncp, acc, sx, sy);
+    gj_tile_store<-1>(a + (bi * TB) * ncp + bj * TB, ncp, acc);
 }
 
 // Column panel: A(i, k) = -A(i, k) D^-1 for tile rows i != k.
 __global__ __launch_bounds__(256) void k_gj_colpanel(double* __restrict__ a, i64 ncp, i64 k, const double* __restrict__ dinv) {
-    __shared__ double sx[64][33], sy[32][65];
+    __shared__ double sx[64][32], sy[32][64];
     const i64 nb = ncp / TB;
     const i64 i = blockIdx.x < k ? blockIdx.x : blockIdx.x + 1;
     if (i >= nb) return;
-    double acc[4][4] = {};
     double* blk = a + (i * TB) * ncp + k * TB;
-    tile_mm(blk, ncp, dinv, TB, acc, sx, sy);
+    gj_d4 acc[2][2];
+    gj_tile_mfma(blk, ncp, dinv, TB, acc, sx, sy);
     __syncthreads();
-    const int t = threadIdx.x, tx = t % 16, ty = t / 16;
-    for (int u = 0; u < 4; ++u)
-        for (int v = 0; v < 4; ++v) blk[i64(ty * 4 + u) * ncp + tx * 4 + v] = -acc[u][v];
+    gj_tile_store<0>(blk, ncp, acc);
 }
 
 __global__ void k_gj_setdiag(double* __restrict__ a, i64 ncp, i64 k, const double* __restrict__ dinv) {
