@@ -826,7 +826,7 @@ __global__ __launch_bounds__(256) void k_pairs_mfma32(PairArgs<float> a) {
 // T = double: the FP64 value pass (inner and bound parts).  T = float: the FP32 value pass with
 // FP32 accumulation into val32 (inner part only).
 // Activity test of the pair path from a per-entry pack (1) or gathered through the neighbour ids (0):
-// 71.1 against 71.8 ms per 1000 x 1000 assembly, the pack's build included.
+// 70.8 against 71.8 ms per 1000 x 1000 assembly, the pack's build (1.3 ms) included.
 #ifndef NLF_ASM_NB_PACK
 #define NLF_ASM_NB_PACK 1
 #endif
@@ -1244,12 +1244,19 @@ __global__ void k_elem_minnode(i64 E, int nn, const i32* __restrict__ el, i32* _
 }
 
 // The activity test's data per neighbour-list entry (AssemblyArgs::nb_pack).
+// Four entries per thread and step (16-byte loads and stores; the buffers start 256-byte aligned),
+// the last total % 4 one by one.
 __global__ void k_nb_pack(i64 total, const i32* __restrict__ nb_ids, const i32* __restrict__ el_maxnode,
                           const uint8_t* __restrict__ el_dir, i32* __restrict__ out) {
-    for (i64 q = i64(blockIdx.x) * blockDim.x + threadIdx.x; q < total; q += i64(gridDim.x) * blockDim.x) {
-        const i32 e = nb_ids[q];
-        out[q] = el_maxnode[e] | (el_dir[e] ? i32(0x80000000u) : 0);
+    auto pack = [&](i32 e) { return el_maxnode[e] | (el_dir[e] ? i32(0x80000000u) : 0); };
+    const i64 quads = total / 4, stride = i64(gridDim.x) * blockDim.x;
+    const int4* in4 = reinterpret_cast<const int4*>(nb_ids);
+    int4* out4 = reinterpret_cast<int4*>(out);
+    for (i64 q = i64(blockIdx.x) * blockDim.x + threadIdx.x; q < quads; q += stride) {
+        const int4 e = in4[q];
+        out4[q] = make_int4(pack(e.x), pack(e.y), pack(e.z), pack(e.w));
     }
+    for (i64 q = 4 * quads + i64(blockIdx.x) * blockDim.x + threadIdx.x; q < total; q += stride) out[q] = pack(nb_ids[q]);
 }
 
 __global__ void k_el_dirichlet(i64 E, int nn, int dof, const i32* __restrict__ el, const uint8_t* __restrict__ inner,
