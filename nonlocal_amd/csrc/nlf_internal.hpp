@@ -45,8 +45,9 @@ struct Error : std::runtime_error {
 // release.  hipMalloc of a large block costs up to a millisecond and hipFree waits for the whole
 // device; a workload that assembles and solves again and again reuses its blocks.  Outside a
 // context scope (or for another context's buffer) blocks are plain hipMalloc / hipFree.  The free
-// lists of all contexts on a device hold at most an eighth of its memory together; a context's list
-// is emptied when hipMalloc fails and when the context is destroyed.
+// lists of all contexts on a device hold at most an eighth of its memory together (past that, the
+// context's oldest free blocks are returned first); a context's list is emptied when hipMalloc fails
+// and when the context is destroyed.
 struct CtxScope {
     nlf_ctx prev;
     explicit CtxScope(nlf_ctx c);
@@ -312,7 +313,9 @@ struct nlf_ctx_s {
     // free device blocks of this context's buffers (device_alloc / device_free)
     bool alive = true;
     std::mutex blocks_mu;
-    std::multimap<size_t, void*> free_blocks;
+    std::multimap<size_t, std::pair<uint64_t, void*>> free_blocks;                    // bytes -> (age, block)
+    std::map<uint64_t, std::multimap<size_t, std::pair<uint64_t, void*>>::iterator> free_order;  // oldest first
+    uint64_t free_seq = 0;
     size_t cached_bytes = 0;
 };
 

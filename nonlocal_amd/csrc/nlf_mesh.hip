@@ -223,8 +223,9 @@ static void unreserve_free_bytes(int device, size_t cap) {
 static void release_free_blocks(nlf_ctx c) {
     if (c->free_blocks.empty()) return;
     (void)hipStreamSynchronize(c->stream);
-    for (auto& kv : c->free_blocks) (void)hipFree(kv.second);
+    for (auto& kv : c->free_blocks) (void)hipFree(kv.second.second);
     c->free_blocks.clear();
+    c->free_order.clear();
     unreserve_free_bytes(c->device, c->cached_bytes);
     c->cached_bytes = 0;
 }
@@ -243,10 +244,11 @@ void* device_alloc(size_t bytes, size_t* cap, nlf_ctx* owner) {
         std::lock_guard<std::mutex> lock(c->blocks_mu);
         auto it = c->free_blocks.lower_bound(want);
         if (it != c->free_blocks.end() && it->first <= want + want / 4) {
-            p = it->second;
+            p = it->second.second;
             *cap = it->first;
             c->cached_bytes -= it->first;
             unreserve_free_bytes(c->device, it->first);
+            c->free_order.erase(it->second.first);
             c->free_blocks.erase(it);
             return p;
         }
@@ -271,8 +273,23 @@ void* device_alloc(size_t bytes, size_t* cap, nlf_ctx* owner) {
 void device_free(void* p, size_t cap, nlf_ctx owner) {
     if (owner && owner == t_scope_ctx && owner->alive) {
         std::lock_guard<std::mutex> lock(owner->blocks_mu);
-        if (reserve_free_bytes(owner->device, cap)) {
-            owner->free_blocks.emplace(cap, p);
+        // over the limit: the context's oldest free blocks go back to the device first, since the
+        // blocks just released are the ones a repeating workload takes again
+        const bool fits = owner->device >= 0 && owner->device < BLOCK_DEVICES && cap <= g_free_limit[owner->device].load();
+        bool room = fits && reserve_free_bytes(owner->device, cap);
+        while (fits && !room && !owner->free_order.empty()) {
+            auto oldest = owner->free_order.begin()->second;
+            (void)hipFree(oldest->second.second);
+            owner->cached_bytes -= oldest->first;
+            unreserve_free_bytes(owner->device, oldest->first);
+            owner->free_order.erase(owner->free_order.begin());
+            owner->free_blocks.erase(oldest);
+            room = reserve_free_bytes(owner->device, cap);
+        }
+        if (room) {
+            const uint64_t age = owner->free_seq++;
+            auto it = owner->free_blocks.emplace(cap, std::make_pair(age, p));
+            owner->free_order.emplace(age, it);
             owner->cached_bytes += cap;
             return;
         }
